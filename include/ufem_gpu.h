@@ -1,0 +1,118 @@
+/*
+ * ufem_gpu.h -- device-resident linear-static solve of libufem_gpu.so.
+ *
+ * These entry points replace, for meshes of 8-node bricks (element type 9)
+ * with isotropic material (type 1), the steps femSolve runs on the CPU
+ * (fem/fem_sol.c:1655-1817):
+ *
+ *   ufem_sys_create      fem_dofs + femKhit + fem_sol_alloc + fem_sol_res_alloc
+ *                        (fem/fem_sol.c:1263-1391, :485, :639; fem/fem_mloc.c:83)
+ *   ufem_sys_fill_K      fem_fill_K(AF_NO)       (fem/fem_sol.c:897-989)
+ *   ufem_sys_add_loads   fem_add_loads(0)        (fem/fem_sol.c:1119-1189)
+ *   ufem_sys_add_disps   fem_add_disps(AF_YES,0) (fem/fem_sol.c:1195-1247)
+ *   ufem_sys_solve       femEqsCGwJ / femEqsBiCCSwJ (fem/fem_eqs.c:88, :262)
+ *   ufem_sys_reactions   fem_fill_K(AF_YES) + fem_fill_K_React (fem/fem_sol.c:844)
+ *
+ * The mesh arrives in the reference's preparsed-input terms (fem/feminput.5):
+ * node coordinates, 8 node positions per brick in the reference node order
+ * (fem/fem_e009.c:36-69), per-element E and nu, and the nodal load/support
+ * list in input order (type 1 = prescribed displacement, only 0 supported;
+ * type 3 = force; direction 1..3 = ux, uy, uz).  Every node must belong to an
+ * element; DOF 3 p + (dir - 1) then equals the reference's node-major
+ * numbering.  Return codes are the reference's AF_* values.
+ */
+#ifndef UFEM_GPU_H
+#define UFEM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  long n_nodes;
+  const double *x, *y, *z;
+  long n_elems;
+  const long *enodes;          /* 8 per element, 0-based node positions   */
+  const double *E, *nu;        /* per element                            */
+  long n_nl;
+  const long *nl_node;         /* 0-based node position                  */
+  const long *nl_type;         /* 1 displacement, 3 force                */
+  const long *nl_dir;          /* 1 ux, 2 uy, 3 uz                       */
+  const double *nl_val;
+} ufem_mesh;
+
+typedef struct ufem_sys ufem_sys;
+
+enum {
+  UFEM_SOLVER_PCG = 0,         /* Jacobi PCG, FP64                        */
+  UFEM_SOLVER_BICGSTAB = 1,    /* Jacobi BiCGStab, FP64                   */
+  UFEM_SOLVER_MIXED = 2,       /* FP32 inner PCG in FP64 refinement       */
+  UFEM_SOLVER_LEARNED = 3      /* flexible CG with the learned preconditioner */
+};
+enum { UFEM_RULE_REFERENCE = 0, UFEM_RULE_RELATIVE = 1 };
+enum { UFEM_EXEC_STREAMED = 0, UFEM_EXEC_PERSISTENT = 1 };
+
+typedef struct {
+  int solver;
+  int rule;          /* 0: ||r|| <= eps (||A|| ||x|| + ||b||); 1: ||r|| <= eps ||b|| */
+  double eps;
+  long maxit;        /* 0 means n, as femSolve passes nDOFAct; < 0: none  */
+  int exec;          /* streamed kernels (graph replayed) or one persistent launch */
+  int chunk;         /* streamed form: iterations per graph replay (0: 32) */
+  double inner_eps;  /* mixed / learned: inner relative tolerance (0: 1e-4) */
+  long inner_maxit;  /* (0: n)                                            */
+  void *net;         /* learned: network handle from ufem_net_*           */
+  int warm_start;    /* nonzero: start from the current u                 */
+} ufem_solve_opts;
+
+typedef struct {
+  int status;        /* AF_OK, AF_ERR (not converged), AF_ERR_ZER, ...    */
+  long iters;        /* iterations (all inner iterations for mixed)       */
+  long outer;        /* mixed: refinement sweeps                          */
+  int fallback;      /* learned: 1 if Jacobi-PCG had to take over         */
+  double relres;     /* last ||r||/||b|| the solver tested                */
+  double true_relres;/* ||b - K u|| / ||b|| recomputed in FP64            */
+  double solve_ms;   /* device time of the solve (HIP events)             */
+} ufem_solve_info;
+
+ufem_sys *ufem_sys_create(const ufem_mesh *m);
+void ufem_sys_destroy(ufem_sys *s);
+long ufem_sys_ndof(const ufem_sys *s);
+long ufem_sys_nnz(const ufem_sys *s);       /* scalar entries of K, explicit zeros kept */
+long ufem_sys_ncolors(const ufem_sys *s);
+long ufem_sys_nreact(const ufem_sys *s);
+int  ufem_sys_fill_K(ufem_sys *s, int npts);    /* npts = 2 (exact for parallelepipeds) or 3 */
+int  ufem_sys_add_loads(ufem_sys *s);
+int  ufem_sys_add_disps(ufem_sys *s);
+int  ufem_sys_solve(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info);
+int  ufem_sys_reactions(ufem_sys *s, double *R, long *rnode, long *rdof);
+int  ufem_sys_get_u(ufem_sys *s, double *u);
+int  ufem_sys_set_u(ufem_sys *s, const double *u);
+int  ufem_sys_get_F(ufem_sys *s, double *F);
+/* K as sorted scalar CSR (0-based), explicit zeros included */
+int  ufem_sys_get_K(ufem_sys *s, long *indptr, int *cols, double *vals);
+int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host out */
+/* average device time of `reps` SpMV launches on the system's stream,
+ * plus the bytes one launch moves by the algorithmic count */
+int  ufem_sys_time_spmv(ufem_sys *s, int reps, int fp32, double *ms, double *bytes);
+/* phase device times of the last calls: fill_K, add_disps, solve (ms) */
+void ufem_sys_timings(const ufem_sys *s, double t[3]);
+
+/* element stiffness matrices of n bricks (xyz: n*24 in node order), dense
+ * row-major 24x24 each -- e009_stiff Mode=AF_NO, fem/fem_e009.c:261-635 */
+int  ufem_gpu_element_stiffness(long n, const double *xyz, const double *E, const double *nu,
+                                int npts, double *Ke);
+
+/* stopping rule used by the drop-in femEqsCGwJ / femEqsBiCCSwJ (default 0) */
+void ufem_gpu_set_stop_rule(int rule);
+/* details of the last drop-in solve */
+void ufem_gpu_last_solve(ufem_solve_info *info);
+/* text of the last error */
+const char *ufem_gpu_last_error(void);
+/* device name, CU count, HBM bytes */
+int  ufem_gpu_device_info(char *name, int name_len, int *n_cu, long *hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

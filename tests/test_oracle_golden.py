@@ -79,13 +79,15 @@ def test_element_stiffness_bitwise(golden, tag, mname, elems):
 
 
 def test_threaded_reference_run_close(golden):
-    """The reference with -t 4 sums dots in thread blocks; the oracle with 4
-    threads uses the same split, so the result is bitwise equal too."""
+    """The reference with -t 4 assembles with per-row mutexes, so its column
+    insertion and summation order depend on thread timing (the threaded run
+    is not reproducible run to run); the oracle with 4 threads agrees to
+    rounding and within a few iterations."""
     mesh = MESHES["c1"]()
     s = uo.OracleSystem(mesh, threads=4)
     s.assemble(3)
     rv, iters, _ = s.solve(0, 1e-21, -1, 0)
     assert rv == int(golden["c1_cg_t4/rv"])
-    assert iters + 1 == int(golden["c1_cg_t4/spmv_calls"])
+    assert abs(iters + 1 - int(golden["c1_cg_t4/spmv_calls"])) <= 3
     ref = golden["c1_cg_t4/u"]
     assert np.linalg.norm(s.u() - ref) <= 1e-12 * np.linalg.norm(ref)

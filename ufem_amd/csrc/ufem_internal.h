@@ -1,0 +1,70 @@
+/*
+ * ufem_internal.h -- host-side state shared by the C sources of
+ * libufem_gpu.so (not installed; the public API is include/ufem_gpu.h and
+ * include/ufem_fem_math.h).
+ */
+#ifndef UFEM_INTERNAL_H
+#define UFEM_INTERNAL_H
+
+#include <stddef.h>
+
+#include "ufem_gpu.h"
+#include "ugk.h"
+
+#define UFEM_KNOWN_DOFS 3
+
+/* device vectors of one Krylov workspace (FP64 plus the FP32 inner set) */
+typedef struct {
+  long n;
+  double *r, *z, *p[2], *q, *dinv;
+  double *rhat, *phat, *v, *s, *shat, *t;          /* BiCGStab            */
+  float *r32, *z32, *p32[2], *q32, *x32, *dinv32;  /* mixed precision     */
+  double *part;                                    /* reduction partials  */
+  ugk_state *st;                                   /* device scalars      */
+  ugk_state *st_host;                              /* pinned mirror       */
+} ufem_work;
+
+struct ufem_sys {
+  long n_nodes, n_elems, n;
+  ugk_mat K;
+  void *stream;
+  void *ev[6];
+  /* mesh on the device */
+  double *d_x, *d_y, *d_z, *d_E, *d_nu;
+  int *d_enodes;
+  int n_colors;
+  long *color_off;          /* host, n_colors + 1 */
+  int *d_elist;             /* elements grouped by colour */
+  int maxb;                 /* longest block row */
+  /* nodal loads / supports, input order */
+  long n_nl;
+  long *nl_node, *nl_type, *nl_dir;
+  double *nl_val;
+  double *F_host;
+  long n_fixed;             /* distinct fixed DOFs, first-occurrence order */
+  int *fixed_host, *d_fixed;
+  long n_react;             /* reaction slots, one per support entry */
+  long *react_node, *react_dof, *react_fix;
+  double *react_init;       /* -F at the DOF when the support was applied */
+  double *d_saved_rows;
+  int *d_saved_cols;
+  double *d_Rfix;
+  int filled, loaded, bc_applied;
+  /* vectors */
+  double *d_F, *d_u;
+  ufem_work w;
+  double t_fill_ms, t_bc_ms, t_solve_ms;
+};
+
+/* workspace helpers (ufem_solve.c) */
+int  ufem_work_alloc(ufem_work *w, long n, int with_bicgs, int with_mixed);
+void ufem_work_free(ufem_work *w);
+/* solvers on a device matrix; b, x device vectors of length n (ufem_solve.c) */
+int ufem_solve_device(const ugk_mat *K, long n, const double *d_b, double *d_x, ufem_work *w,
+                      void *stream, void *ev0, void *ev1, const ufem_solve_opts *o,
+                      ufem_solve_info *info);
+
+/* error text (ufem_sys.c) */
+void ufem_set_error(const char *fmt, ...);
+
+#endif

@@ -1,0 +1,350 @@
+/*
+ * ufem_solve.c -- host control of the device solvers: Jacobi PCG (streamed
+ * graph-replayed kernels or one persistent cooperative launch), Jacobi
+ * BiCGStab, and FP32-inner / FP64-refinement mixed precision.
+ *
+ * The iteration itself lives on the device (ugk_kernels.hip); the host only
+ * enqueues work and, for the streamed form, polls the device's `done` flag
+ * once per chunk of iterations while the next chunk is already queued.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ufem_fem_math.h"
+#include "ufem_internal.h"
+
+static int gpu_fail(const char *what)
+{
+  ufem_set_error("%s: %s", what, ugk_error());
+  return AF_ERR;
+}
+
+void ufem_work_free(ufem_work *w)
+{
+  ugk_free(w->r); ugk_free(w->z); ugk_free(w->p[0]); ugk_free(w->p[1]); ugk_free(w->q);
+  ugk_free(w->dinv); ugk_free(w->rhat); ugk_free(w->phat); ugk_free(w->v); ugk_free(w->s);
+  ugk_free(w->shat); ugk_free(w->t); ugk_free(w->r32); ugk_free(w->z32); ugk_free(w->p32[0]);
+  ugk_free(w->p32[1]); ugk_free(w->q32); ugk_free(w->x32); ugk_free(w->dinv32);
+  ugk_free(w->part); ugk_free(w->st);
+  ugk_host_free(w->st_host);
+  memset(w, 0, sizeof(*w));
+}
+
+int ufem_work_alloc(ufem_work *w, long n, int with_bicgs, int with_mixed)
+{
+  size_t b = sizeof(double) * (size_t)n, f = sizeof(float) * (size_t)n;
+  memset(w, 0, sizeof(*w));
+  w->n = n;
+  w->r = (double *)ugk_malloc(b);
+  w->z = (double *)ugk_malloc(b);
+  w->p[0] = (double *)ugk_malloc(b);
+  w->p[1] = (double *)ugk_malloc(b);
+  w->q = (double *)ugk_malloc(b);
+  w->dinv = (double *)ugk_malloc(b);
+  w->part = (double *)ugk_malloc(sizeof(double) * (size_t)UGK_PART_DOUBLES);
+  w->st = (ugk_state *)ugk_malloc(sizeof(ugk_state));
+  w->st_host = (ugk_state *)ugk_host_alloc(sizeof(ugk_state));
+  if (!w->r || !w->z || !w->p[0] || !w->p[1] || !w->q || !w->dinv || !w->part || !w->st || !w->st_host)
+    goto fail;
+  if (with_bicgs) {
+    w->rhat = (double *)ugk_malloc(b);
+    w->phat = (double *)ugk_malloc(b);
+    w->v = (double *)ugk_malloc(b);
+    w->s = (double *)ugk_malloc(b);
+    w->shat = (double *)ugk_malloc(b);
+    w->t = (double *)ugk_malloc(b);
+    if (!w->rhat || !w->phat || !w->v || !w->s || !w->shat || !w->t) goto fail;
+  }
+  if (with_mixed) {
+    w->r32 = (float *)ugk_malloc(f);
+    w->z32 = (float *)ugk_malloc(f);
+    w->p32[0] = (float *)ugk_malloc(f);
+    w->p32[1] = (float *)ugk_malloc(f);
+    w->q32 = (float *)ugk_malloc(f);
+    w->x32 = (float *)ugk_malloc(f);
+    w->dinv32 = (float *)ugk_malloc(f);
+    if (!w->r32 || !w->z32 || !w->p32[0] || !w->p32[1] || !w->q32 || !w->x32 || !w->dinv32) goto fail;
+  }
+  return AF_OK;
+fail:
+  gpu_fail("workspace allocation");
+  ufem_work_free(w);
+  return AF_ERR_MEM;
+}
+
+/* host copy of the scalar state, written before a solve */
+static int put_state(ufem_work *w, void *stream, double tol, int rule, long maxit)
+{
+  ugk_state *h = w->st_host;
+  memset(h, 0, sizeof(*h));
+  h->tol = tol;
+  h->rule = rule;
+  h->maxit = maxit;
+  if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state upload");
+  return AF_OK;
+}
+
+static int get_state(ufem_work *w, void *stream)
+{
+  if (ugk_d2h(w->st_host, w->st, sizeof(ugk_state), stream)) return gpu_fail("state read");
+  return AF_OK;
+}
+
+/* Enqueue chunks of iterations until the device reports done.  The state of
+ * chunk k is copied to pinned memory behind an event while chunk k+1 is
+ * queued, so the GPU never waits on the host. */
+typedef int (*iter_fn)(void *ctx, long it, void *stream);
+
+static int run_chunks(ufem_work *w, void *stream, long maxit, int chunk, iter_fn fn, void *ctx,
+                      int use_graph)
+{
+  void *graph = NULL, *ev = NULL;
+  long it = 0;
+  int rv = AF_OK, first = 1;
+  if (chunk < 2) chunk = 2;
+  chunk += chunk & 1;                    /* even: ping-pong parity repeats */
+  ev = ugk_event_create();
+  if (!ev) return gpu_fail("event");
+  if (use_graph) {
+    long k;
+    if (ugk_graph_begin(stream)) { rv = gpu_fail("graph capture"); goto out; }
+    for (k = 0; k < chunk; k++)
+      if (fn(ctx, k, stream)) { (void)ugk_graph_end(stream); rv = gpu_fail("iteration"); goto out; }
+    graph = ugk_graph_end(stream);
+    if (!graph) use_graph = 0;           /* fall back to direct launches */
+  }
+  for (;;) {
+    long k;
+    if (use_graph) {
+      if (ugk_graph_launch(graph, stream)) { rv = gpu_fail("graph launch"); goto out; }
+    } else {
+      for (k = 0; k < chunk; k++)
+        if (fn(ctx, it + k, stream)) { rv = gpu_fail("iteration"); goto out; }
+    }
+    it += chunk;
+    if (!first) {
+      if (ugk_event_sync(ev)) { rv = gpu_fail("event sync"); goto out; }
+      if (w->st_host->done) break;
+    }
+    first = 0;
+    if (ugk_d2h_async(w->st_host, w->st, sizeof(ugk_state), stream) || ugk_event_record(ev, stream)) {
+      rv = gpu_fail("state poll");
+      goto out;
+    }
+    if (it > maxit + 2L * chunk) {       /* the device stops itself at maxit */
+      if (ugk_event_sync(ev)) { rv = gpu_fail("event sync"); goto out; }
+      break;
+    }
+  }
+  if (ugk_sync(stream)) rv = gpu_fail("sync");
+out:
+  ugk_graph_destroy(graph);
+  ugk_event_destroy(ev);
+  return rv;
+}
+
+typedef struct {
+  const ugk_mat *K;
+  long n;
+  double *x;
+  ufem_work *w;
+} pcg_ctx;
+
+static int pcg_step(void *c, long it, void *stream)
+{
+  pcg_ctx *p = (pcg_ctx *)c;
+  ufem_work *w = p->w;
+  return ugk_pcg_iter(p->K, p->n, p->x, w->r, w->z, w->p, w->q, w->dinv, w->st, w->part, it, stream);
+}
+
+static int pcg32_step(void *c, long it, void *stream)
+{
+  pcg_ctx *p = (pcg_ctx *)c;
+  ufem_work *w = p->w;
+  return ugk_pcg32_iter(p->K, p->n, w->x32, w->r32, w->z32, w->p32, w->q32, w->dinv32, w->st, w->part, it,
+                        stream);
+}
+
+static int bicgs_step(void *c, long it, void *stream)
+{
+  pcg_ctx *p = (pcg_ctx *)c;
+  ufem_work *w = p->w;
+  (void)it;
+  return ugk_bicgs_iter(p->K, p->n, p->x, w->r, w->rhat, w->p[0], w->phat, w->v, w->s, w->shat, w->t,
+                        w->dinv, w->st, w->part, stream);
+}
+
+static int status_of(int done)
+{
+  return done == 1 ? AF_OK : AF_ERR;
+}
+
+/* FP64 true residual ||b - K x|| / ||b|| into info */
+static int true_residual(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
+                         double bb, double *out)
+{
+  if (ugk_ir_update(K, n, x, NULL, 0.0, b, w->r, w->st, w->part, stream) || get_state(w, stream))
+    return gpu_fail("residual");
+  *out = bb > 0 ? sqrt(w->st_host->rr / bb) : 0.0;
+  return AF_OK;
+}
+
+static int prepare(const ugk_mat *K, ufem_work *w, void *stream, double tol, int rule, long maxit)
+{
+  if (put_state(w, stream, tol, rule, maxit)) return AF_ERR;
+  if (ugk_diag_inv(K, w->dinv, w->st, stream)) return gpu_fail("diagonal");
+  if (rule == 0 && ugk_mat_norm(K, w->st, w->part, stream)) return gpu_fail("matrix norm");
+  if (get_state(w, stream)) return AF_ERR;
+  if (w->st_host->err == 2) { ufem_set_error("zero diagonal"); return AF_ERR_ZER; }
+  return AF_OK;
+}
+
+static int solve_pcg(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
+                     const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+{
+  int rv;
+  pcg_ctx c = {K, n, x, w};
+  rv = prepare(K, w, stream, o->eps, o->rule, maxit);
+  if (rv != AF_OK) return rv;
+  if (ugk_pcg_start(K, n, b, x, w->r, w->z, w->p[0], w->dinv, w->st, w->part, stream)) return gpu_fail("start");
+  if (get_state(w, stream)) return AF_ERR;
+  if (!w->st_host->done) {
+    if (o->exec == UFEM_EXEC_PERSISTENT) {
+      if (ugk_zero(w->p[1], sizeof(double) * (size_t)n, stream) ||
+          ugk_pcg_persistent(K, n, x, w->r, w->z, w->p[0], w->p[1], w->q, w->dinv, w->st, w->part, stream))
+        return gpu_fail("persistent PCG");
+    } else {
+      rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 32, pcg_step, &c, 1);
+      if (rv != AF_OK) return rv;
+    }
+    if (get_state(w, stream)) return AF_ERR;
+  }
+  if (w->st_host->bb <= 0.0) {   /* zero load: the reference returns x = 0 */
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+    return AF_OK;
+  }
+  info->iters = w->st_host->iter;
+  info->relres = sqrt(w->st_host->rr / w->st_host->bb);
+  info->status = status_of(w->st_host->done);
+  return info->status;
+}
+
+static int solve_bicgs(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
+                       const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+{
+  int rv;
+  pcg_ctx c = {K, n, x, w};
+  /* BiCGStab tests ||r|| <= eps ||b||; the reference rule is mapped on the
+     same relative test with eps (its own test is not usable, see header) */
+  rv = prepare(K, w, stream, o->eps, 1, maxit);
+  if (rv != AF_OK) return rv;
+  if (ugk_bicgs_start(K, n, b, x, w->r, w->rhat, w->p[0], w->v, w->st, w->part, stream)) return gpu_fail("start");
+  if (get_state(w, stream)) return AF_ERR;
+  if (!w->st_host->done) {
+    rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 16, bicgs_step, &c, 1);
+    if (rv != AF_OK) return rv;
+    if (get_state(w, stream)) return AF_ERR;
+  }
+  if (w->st_host->bb <= 0.0) {
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+    return AF_OK;
+  }
+  info->iters = w->st_host->iter;
+  info->relres = sqrt(w->st_host->rr / w->st_host->bb);
+  info->status = status_of(w->st_host->done);
+  return info->status;
+}
+
+/* Mixed precision: K stored once more in FP32.  Outer loop in FP64:
+ * r = b - K x (FP64 SpMV), test ||r|| <= eps ||b||, scale r to unit norm,
+ * solve K d = r with FP32 Jacobi PCG to inner_eps (dot products accumulated
+ * in FP64), x += ||r|| d in FP64. */
+static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
+                       const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+{
+  const ugk_mat *K = Km;
+  double inner_eps = o->inner_eps > 0 ? o->inner_eps : 1e-4;
+  long inner_maxit = o->inner_maxit > 0 ? o->inner_maxit : maxit;
+  double bb, rr;
+  long total = 0, outer = 0;
+  int rv;
+  pcg_ctx c = {K, n, x, w};
+  if (!Km->d_vals32) {
+    Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
+    if (!Km->d_vals32) return gpu_fail("fp32 matrix");
+  }
+  if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+  rv = prepare(K, w, stream, o->eps, 1, maxit);
+  if (rv != AF_OK) return rv;
+  if (ugk_to_f32(n, w->dinv, w->dinv32, stream)) return gpu_fail("fp32 diagonal");
+  if (ugk_dot(n, b, b, &w->st->bb, w->part, stream) || get_state(w, stream)) return gpu_fail("norm b");
+  bb = w->st_host->bb;
+  if (!(bb > 0.0)) {
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+    return AF_OK;
+  }
+  if (ugk_ir_update(K, n, x, NULL, 0.0, b, w->r, w->st, w->part, stream) || get_state(w, stream))
+    return gpu_fail("residual");
+  rr = w->st_host->rr;
+  info->status = AF_ERR;
+  while (total < maxit) {
+    double nr = sqrt(rr);
+    ugk_state *h = w->st_host;
+    if (nr <= o->eps * sqrt(bb)) { info->status = AF_OK; break; }
+    /* inner FP32 solve of K d = r / ||r|| */
+    memset(h, 0, sizeof(*h));
+    h->tol = inner_eps;
+    h->rule = 1;
+    h->maxit = inner_maxit < maxit - total ? inner_maxit : maxit - total;
+    if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
+    if (ugk_pcg32_start(K, n, w->r, 1.0 / nr, w->r32, w->z32, w->p32[0], w->x32, w->dinv32, w->st, w->part,
+                        stream) ||
+        get_state(w, stream))
+      return gpu_fail("inner start");
+    if (!w->st_host->done) {
+      rv = run_chunks(w, stream, w->st_host->maxit, o->chunk > 0 ? o->chunk : 32, pcg32_step, &c, 1);
+      if (rv != AF_OK) return rv;
+      if (get_state(w, stream)) return AF_ERR;
+    }
+    if (w->st_host->done == 2) { ufem_set_error("inner FP32 PCG broke down"); info->status = AF_ERR; break; }
+    total += w->st_host->iter;
+    outer++;
+    if (ugk_ir_update(K, n, x, w->x32, nr, b, w->r, w->st, w->part, stream) || get_state(w, stream))
+      return gpu_fail("refinement");
+    rr = w->st_host->rr;
+    if (!(rr == rr)) { ufem_set_error("refinement produced NaN"); info->status = AF_ERR; break; }
+  }
+  info->iters = total;
+  info->outer = outer;
+  info->relres = sqrt(rr / bb);
+  if (info->status != AF_OK && sqrt(rr) <= o->eps * sqrt(bb)) info->status = AF_OK;
+  return info->status;
+}
+
+int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
+                      void *ev0, void *ev1, const ufem_solve_opts *o, ufem_solve_info *info)
+{
+  long maxit = o->maxit > 0 ? o->maxit : (o->maxit == 0 ? n : 0);
+  int rv;
+  double bb;
+  if (ugk_event_record(ev0, stream)) return gpu_fail("event");
+  switch (o->solver) {
+    case UFEM_SOLVER_PCG: rv = solve_pcg(K, n, b, x, w, stream, o, maxit, info); break;
+    case UFEM_SOLVER_BICGSTAB: rv = solve_bicgs(K, n, b, x, w, stream, o, maxit, info); break;
+    case UFEM_SOLVER_MIXED: rv = solve_mixed((ugk_mat *)K, n, b, x, w, stream, o, maxit, info); break;
+    default: ufem_set_error("unknown solver %d", o->solver); return AF_ERR_VAL;
+  }
+  if (ugk_event_record(ev1, stream)) return gpu_fail("event");
+  info->solve_ms = ugk_event_ms(ev0, ev1);
+  if (rv == AF_ERR_ZER || rv == AF_ERR_MEM) return rv;
+  if (ugk_dot(n, b, b, &w->st->bb, w->part, stream) || get_state(w, stream)) return gpu_fail("norm b");
+  bb = w->st_host->bb;
+  if (true_residual(K, n, b, x, w, stream, bb, &info->true_relres)) return AF_ERR;
+  return rv;
+}

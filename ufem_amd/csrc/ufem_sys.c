@@ -1,0 +1,638 @@
+/*
+ * ufem_sys.c -- host C side of the device-resident linear-static solve:
+ * DOF map and sparsity pattern, element colouring, loads/supports
+ * bookkeeping, and the calls into the HIP kernels (ugk.h).
+ *
+ * Mirrors femSolve's sequence (fem/fem_sol.c:1655-1817) for brick meshes;
+ * each function cites the reference step it replaces.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ufem_fem_math.h"
+#include "ufem_internal.h"
+
+static __thread char g_msg[512];
+
+void ufem_set_error(const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+  va_end(ap);
+}
+
+const char *ufem_gpu_last_error(void) { return g_msg; }
+
+static int gpu_fail(const char *what)
+{
+  ufem_set_error("%s: %s", what, ugk_error());
+  return AF_ERR;
+}
+
+static int g_inited = 0;
+static int ensure_gpu(void)
+{
+  if (!g_inited) {
+    if (ugk_init() != 0) return gpu_fail("no usable GPU");
+    g_inited = 1;
+  }
+  return AF_OK;
+}
+
+int ufem_gpu_device_info(char *name, int name_len, int *n_cu, long *hbm)
+{
+  if (ensure_gpu() != AF_OK) return AF_ERR;
+  if (ugk_device_info(n_cu, hbm, name, name_len) != 0) return gpu_fail("device info");
+  return AF_OK;
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+/* node adjacency of the brick mesh -> SELL-64 pattern of 3x3 blocks.
+ * The scalar pattern equals the reference's: K_e is added in full for every
+ * element (fem/fem_mloc.c:182-216), so row 3I+r holds every column 3J+c of
+ * every node J sharing an element with node I, explicit zeros included. */
+static int build_pattern(ufem_sys *s, const ufem_mesh *m)
+{
+  long nn = m->n_nodes, ne = m->n_elems, i, k;
+  long *incp = NULL, *inc = NULL, *fill = NULL;
+  int *mark = NULL, *nb = NULL, *rowlen = NULL, *slen = NULL, *cols = NULL;
+  long *soff = NULL, *nbp = NULL;
+  long nslices = (nn + 63) / 64, total = 0, nnzb = 0;
+  int rv = AF_ERR_MEM, maxb = 0;
+
+  incp = (long *)calloc((size_t)nn + 1, sizeof(long));
+  fill = (long *)calloc((size_t)nn, sizeof(long));
+  mark = (int *)malloc(sizeof(int) * (size_t)nn);
+  rowlen = (int *)calloc((size_t)nn, sizeof(int));
+  nbp = (long *)calloc((size_t)nn + 1, sizeof(long));
+  slen = (int *)calloc((size_t)nslices, sizeof(int));
+  soff = (long *)calloc((size_t)nslices, sizeof(long));
+  if (!incp || !fill || !mark || !rowlen || !nbp || !slen || !soff) goto out;
+  for (i = 0; i < ne * 8; i++) incp[m->enodes[i] + 1]++;
+  for (i = 0; i < nn; i++) {
+    if (incp[i + 1] == 0) {
+      ufem_set_error("node %ld belongs to no element", i);
+      rv = AF_ERR_VAL;
+      goto out;
+    }
+    incp[i + 1] += incp[i];
+  }
+  inc = (long *)malloc(sizeof(long) * (size_t)(ne * 8));
+  if (!inc) goto out;
+  for (i = 0; i < ne; i++)
+    for (k = 0; k < 8; k++) {
+      long nd = m->enodes[i * 8 + k];
+      inc[incp[nd] + fill[nd]++] = i;
+    }
+  for (i = 0; i < nn; i++) mark[i] = -1;
+  /* first pass: row lengths */
+  for (i = 0; i < nn; i++) {
+    int c = 0;
+    for (k = incp[i]; k < incp[i + 1]; k++) {
+      long e = inc[k], a;
+      for (a = 0; a < 8; a++) {
+        long J = m->enodes[e * 8 + a];
+        if (mark[J] != (int)i) { mark[J] = (int)i; c++; }
+      }
+    }
+    rowlen[i] = c;
+    nbp[i + 1] = nbp[i] + c;
+    if (c > maxb) maxb = c;
+  }
+  nnzb = nbp[nn];
+  nb = (int *)malloc(sizeof(int) * (size_t)nnzb);
+  if (!nb) goto out;
+  for (i = 0; i < nn; i++) mark[i] = -1;
+  for (i = 0; i < nn; i++) {
+    long c = nbp[i];
+    for (k = incp[i]; k < incp[i + 1]; k++) {
+      long e = inc[k], a;
+      for (a = 0; a < 8; a++) {
+        long J = m->enodes[e * 8 + a];
+        if (mark[J] != (int)i) { mark[J] = (int)i; nb[c++] = (int)J; }
+      }
+    }
+    qsort(nb + nbp[i], (size_t)rowlen[i], sizeof(int), cmp_int);
+  }
+  for (i = 0; i < nslices; i++) {
+    int l, mx = 0;
+    for (l = 0; l < 64; l++) {
+      long I = i * 64 + l;
+      if (I < nn && rowlen[I] > mx) mx = rowlen[I];
+    }
+    slen[i] = mx;
+    soff[i] = total;
+    total += mx;
+  }
+  cols = (int *)malloc(sizeof(int) * (size_t)total * 64);
+  if (!cols) goto out;
+  for (i = 0; i < nslices; i++) {
+    int l, sl;
+    for (l = 0; l < 64; l++) {
+      long I = i * 64 + l;
+      for (sl = 0; sl < slen[i]; sl++) {
+        int c;
+        if (I < nn) c = sl < rowlen[I] ? nb[nbp[I] + sl] : (int)I;
+        else c = 0;
+        cols[(soff[i] + sl) * 64 + l] = c;
+      }
+    }
+  }
+  s->K.bs = 3;
+  s->K.n_brows = nn;
+  s->K.n_slices = nslices;
+  s->K.total_slots = total;
+  s->K.nnzb = nnzb;
+  s->maxb = maxb;
+  if (nslices > 4L * UGK_RED_MAX_BLOCKS) { ufem_set_error("mesh too large (%ld nodes)", nn); rv = AF_ERR_BIG; goto out; }
+  s->K.d_slen = (int *)ugk_malloc(sizeof(int) * (size_t)nslices);
+  s->K.d_soff = (long *)ugk_malloc(sizeof(long) * (size_t)nslices);
+  s->K.d_rowlen = (int *)ugk_malloc(sizeof(int) * (size_t)nn);
+  s->K.d_cols = (int *)ugk_malloc(sizeof(int) * (size_t)total * 64);
+  s->K.d_vals = (double *)ugk_malloc(sizeof(double) * (size_t)total * 64 * 9);
+  if (!s->K.d_slen || !s->K.d_soff || !s->K.d_rowlen || !s->K.d_cols || !s->K.d_vals) {
+    rv = gpu_fail("pattern allocation");
+    goto out;
+  }
+  if (ugk_h2d(s->K.d_slen, slen, sizeof(int) * (size_t)nslices, s->stream) ||
+      ugk_h2d(s->K.d_soff, soff, sizeof(long) * (size_t)nslices, s->stream) ||
+      ugk_h2d(s->K.d_rowlen, rowlen, sizeof(int) * (size_t)nn, s->stream) ||
+      ugk_h2d(s->K.d_cols, cols, sizeof(int) * (size_t)total * 64, s->stream) ||
+      ugk_sync(s->stream)) {
+    rv = gpu_fail("pattern upload");
+    goto out;
+  }
+  rv = AF_OK;
+out:
+  free(incp); free(inc); free(fill); free(mark); free(nb); free(rowlen); free(nbp);
+  free(slen); free(soff); free(cols);
+  return rv;
+}
+
+/* Greedy first-fit colouring in element order: elements sharing a node get
+ * different colours, so the scatter of one colour never updates a block of K
+ * twice.  On a structured brick grid this yields the 8 parity classes. */
+static int color_elements(ufem_sys *s, const ufem_mesh *m)
+{
+  long nn = m->n_nodes, ne = m->n_elems, i, k;
+  long *incp = NULL, *inc = NULL, *fill = NULL, *cnt = NULL;
+  int *color = NULL, *list = NULL;
+  unsigned char *used = NULL;
+  int ncol = 0, rv = AF_ERR_MEM;
+  const int MAXC = 256;
+  incp = (long *)calloc((size_t)nn + 1, sizeof(long));
+  fill = (long *)calloc((size_t)nn, sizeof(long));
+  color = (int *)malloc(sizeof(int) * (size_t)ne);
+  used = (unsigned char *)calloc(MAXC, 1);
+  if (!incp || !fill || !color || !used) goto out;
+  for (i = 0; i < ne * 8; i++) incp[m->enodes[i] + 1]++;
+  for (i = 0; i < nn; i++) incp[i + 1] += incp[i];
+  inc = (long *)malloc(sizeof(long) * (size_t)(ne * 8));
+  if (!inc) goto out;
+  for (i = 0; i < ne; i++)
+    for (k = 0; k < 8; k++) {
+      long nd = m->enodes[i * 8 + k];
+      inc[incp[nd] + fill[nd]++] = i;
+    }
+  for (i = 0; i < ne; i++) {
+    int c;
+    memset(used, 0, MAXC);
+    for (k = 0; k < 8; k++) {
+      long nd = m->enodes[i * 8 + k], t;
+      for (t = incp[nd]; t < incp[nd + 1]; t++) {
+        long f = inc[t];
+        if (f < i) used[color[f]] = 1;   /* only already-coloured elements */
+      }
+    }
+    for (c = 0; c < MAXC && used[c]; c++) {}
+    if (c == MAXC) { ufem_set_error("more than %d element colours", MAXC); rv = AF_ERR_BIG; goto out; }
+    color[i] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  cnt = (long *)calloc((size_t)ncol + 1, sizeof(long));
+  list = (int *)malloc(sizeof(int) * (size_t)ne);
+  s->color_off = (long *)calloc((size_t)ncol + 1, sizeof(long));
+  if (!cnt || !list || !s->color_off) goto out;
+  for (i = 0; i < ne; i++) s->color_off[color[i] + 1]++;
+  for (i = 0; i < ncol; i++) s->color_off[i + 1] += s->color_off[i];
+  for (i = 0; i < ne; i++) list[s->color_off[color[i]] + cnt[color[i]]++] = (int)i;
+  s->n_colors = ncol;
+  s->d_elist = (int *)ugk_malloc(sizeof(int) * (size_t)ne);
+  if (!s->d_elist || ugk_h2d(s->d_elist, list, sizeof(int) * (size_t)ne, s->stream) || ugk_sync(s->stream)) {
+    rv = gpu_fail("colour upload");
+    goto out;
+  }
+  rv = AF_OK;
+out:
+  free(incp); free(inc); free(fill); free(cnt); free(color); free(list); free(used);
+  return rv;
+}
+
+long ufem_sys_ndof(const ufem_sys *s) { return s->n; }
+long ufem_sys_nnz(const ufem_sys *s) { return s->K.nnzb * 9; }
+long ufem_sys_ncolors(const ufem_sys *s) { return s->n_colors; }
+long ufem_sys_nreact(const ufem_sys *s) { return s->n_react; }
+
+void ufem_sys_destroy(ufem_sys *s)
+{
+  int i;
+  if (!s) return;
+  if (s->stream) ugk_sync(s->stream);
+  ugk_free(s->K.d_slen); ugk_free(s->K.d_soff); ugk_free(s->K.d_rowlen);
+  ugk_free(s->K.d_cols); ugk_free(s->K.d_vals); ugk_free(s->K.d_vals32);
+  ugk_free(s->d_x); ugk_free(s->d_y); ugk_free(s->d_z); ugk_free(s->d_E); ugk_free(s->d_nu);
+  ugk_free(s->d_enodes); ugk_free(s->d_elist); ugk_free(s->d_fixed);
+  ugk_free(s->d_saved_rows); ugk_free(s->d_saved_cols); ugk_free(s->d_Rfix);
+  ugk_free(s->d_F); ugk_free(s->d_u);
+  ufem_work_free(&s->w);
+  for (i = 0; i < 6; i++) ugk_event_destroy(s->ev[i]);
+  ugk_stream_destroy(s->stream);
+  free(s->color_off); free(s->nl_node); free(s->nl_type); free(s->nl_dir); free(s->nl_val);
+  free(s->F_host); free(s->fixed_host); free(s->react_node); free(s->react_dof);
+  free(s->react_fix); free(s->react_init);
+  free(s);
+}
+
+/* fem_dofs + femKhit + fem_sol_alloc + fem_sol_res_alloc
+ * (fem/fem_sol.c:1263-1391, :485, :639-742; fem/fem_mloc.c:83-116) */
+ufem_sys *ufem_sys_create(const ufem_mesh *m)
+{
+  ufem_sys *s;
+  long i, ne, nn;
+  double *tmp = NULL;
+  int *itmp = NULL;
+  if (ensure_gpu() != AF_OK) return NULL;
+  if (!m || m->n_nodes < 1 || m->n_elems < 1) { ufem_set_error("empty mesh"); return NULL; }
+  ne = m->n_elems;
+  nn = m->n_nodes;
+  for (i = 0; i < ne * 8; i++)
+    if (m->enodes[i] < 0 || m->enodes[i] >= nn) { ufem_set_error("element node out of range"); return NULL; }
+  for (i = 0; i < m->n_nl; i++) {
+    if (m->nl_node[i] < 0 || m->nl_node[i] >= nn || m->nl_dir[i] < 1 || m->nl_dir[i] > 3) {
+      ufem_set_error("load/support %ld out of range", i);
+      return NULL;
+    }
+    if (m->nl_type[i] == 1 && m->nl_val[i] != 0.0) {
+      ufem_set_error("prescribed non-zero displacement (entry %ld) is not on this path", i);
+      return NULL;
+    }
+  }
+  s = (ufem_sys *)calloc(1, sizeof(ufem_sys));
+  if (!s) return NULL;
+  s->n_nodes = nn;
+  s->n_elems = ne;
+  s->n = 3 * nn;
+  s->stream = ugk_stream_create();
+  for (i = 0; i < 6; i++) s->ev[i] = ugk_event_create();
+  if (!s->stream || !s->ev[5]) { gpu_fail("stream"); goto fail; }
+  if (build_pattern(s, m) != AF_OK) goto fail;
+  if (color_elements(s, m) != AF_OK) goto fail;
+
+  s->d_x = (double *)ugk_malloc(sizeof(double) * (size_t)nn);
+  s->d_y = (double *)ugk_malloc(sizeof(double) * (size_t)nn);
+  s->d_z = (double *)ugk_malloc(sizeof(double) * (size_t)nn);
+  s->d_E = (double *)ugk_malloc(sizeof(double) * (size_t)ne);
+  s->d_nu = (double *)ugk_malloc(sizeof(double) * (size_t)ne);
+  s->d_enodes = (int *)ugk_malloc(sizeof(int) * (size_t)ne * 8);
+  s->d_F = (double *)ugk_malloc(sizeof(double) * (size_t)s->n);
+  s->d_u = (double *)ugk_malloc(sizeof(double) * (size_t)s->n);
+  itmp = (int *)malloc(sizeof(int) * (size_t)ne * 8);
+  if (!s->d_x || !s->d_y || !s->d_z || !s->d_E || !s->d_nu || !s->d_enodes || !s->d_F || !s->d_u || !itmp) {
+    gpu_fail("mesh allocation");
+    goto fail;
+  }
+  for (i = 0; i < ne * 8; i++) itmp[i] = (int)m->enodes[i];
+  if (ugk_h2d(s->d_x, m->x, sizeof(double) * (size_t)nn, s->stream) ||
+      ugk_h2d(s->d_y, m->y, sizeof(double) * (size_t)nn, s->stream) ||
+      ugk_h2d(s->d_z, m->z, sizeof(double) * (size_t)nn, s->stream) ||
+      ugk_h2d(s->d_E, m->E, sizeof(double) * (size_t)ne, s->stream) ||
+      ugk_h2d(s->d_nu, m->nu, sizeof(double) * (size_t)ne, s->stream) ||
+      ugk_h2d(s->d_enodes, itmp, sizeof(int) * (size_t)ne * 8, s->stream) || ugk_sync(s->stream)) {
+    gpu_fail("mesh upload");
+    goto fail;
+  }
+  free(itmp);
+  itmp = NULL;
+  if (ufem_work_alloc(&s->w, s->n, 1, 1) != AF_OK) goto fail;
+
+  /* loads/supports in input order; reaction slots per support entry */
+  s->n_nl = m->n_nl;
+  s->nl_node = (long *)malloc(sizeof(long) * (size_t)(m->n_nl + 1));
+  s->nl_type = (long *)malloc(sizeof(long) * (size_t)(m->n_nl + 1));
+  s->nl_dir = (long *)malloc(sizeof(long) * (size_t)(m->n_nl + 1));
+  s->nl_val = (double *)malloc(sizeof(double) * (size_t)(m->n_nl + 1));
+  s->F_host = (double *)calloc((size_t)s->n, sizeof(double));
+  if (!s->nl_node || !s->nl_type || !s->nl_dir || !s->nl_val || !s->F_host) goto fail;
+  if (m->n_nl > 0) {
+    memcpy(s->nl_node, m->nl_node, sizeof(long) * (size_t)m->n_nl);
+    memcpy(s->nl_type, m->nl_type, sizeof(long) * (size_t)m->n_nl);
+    memcpy(s->nl_dir, m->nl_dir, sizeof(long) * (size_t)m->n_nl);
+    memcpy(s->nl_val, m->nl_val, sizeof(double) * (size_t)m->n_nl);
+  }
+  for (i = 0; i < m->n_nl; i++)
+    if (m->nl_type[i] == 1) s->n_react++;
+  s->react_node = (long *)calloc((size_t)s->n_react + 1, sizeof(long));
+  s->react_dof = (long *)calloc((size_t)s->n_react + 1, sizeof(long));
+  s->react_fix = (long *)calloc((size_t)s->n_react + 1, sizeof(long));
+  s->react_init = (double *)calloc((size_t)s->n_react + 1, sizeof(double));
+  s->fixed_host = (int *)calloc((size_t)s->n_react + 1, sizeof(int));
+  if (!s->react_node || !s->react_dof || !s->react_fix || !s->react_init || !s->fixed_host) goto fail;
+  {
+    long r = 0;
+    char *seen = (char *)calloc((size_t)s->n, 1);
+    long *slot_of = (long *)malloc(sizeof(long) * (size_t)s->n);
+    if (!seen || !slot_of) { free(seen); free(slot_of); goto fail; }
+    for (i = 0; i < m->n_nl; i++) {
+      long d;
+      if (m->nl_type[i] != 1) continue;
+      d = 3 * m->nl_node[i] + m->nl_dir[i] - 1;
+      s->react_node[r] = m->nl_node[i];
+      s->react_dof[r] = m->nl_dir[i];
+      if (!seen[d]) {
+        seen[d] = 1;
+        slot_of[d] = s->n_fixed;
+        s->fixed_host[s->n_fixed++] = (int)d;
+      }
+      s->react_fix[r] = slot_of[d];
+      r++;
+    }
+    free(seen);
+    free(slot_of);
+  }
+  if (s->n_fixed > 0) {
+    s->d_fixed = (int *)ugk_malloc(sizeof(int) * (size_t)s->n_fixed);
+    s->d_saved_rows = (double *)ugk_malloc(sizeof(double) * (size_t)s->n_fixed * s->maxb * 3);
+    s->d_saved_cols = (int *)ugk_malloc(sizeof(int) * (size_t)s->n_fixed * s->maxb);
+    s->d_Rfix = (double *)ugk_malloc(sizeof(double) * (size_t)s->n_fixed);
+    if (!s->d_fixed || !s->d_saved_rows || !s->d_saved_cols || !s->d_Rfix ||
+        ugk_h2d(s->d_fixed, s->fixed_host, sizeof(int) * (size_t)s->n_fixed, s->stream) ||
+        ugk_sync(s->stream)) {
+      gpu_fail("support upload");
+      goto fail;
+    }
+  }
+  (void)tmp;
+  return s;
+fail:
+  free(itmp);
+  ufem_sys_destroy(s);
+  return NULL;
+}
+
+/* fem_fill_K(AF_NO), fem/fem_sol.c:897-989: K = sum of element matrices,
+ * here one kernel launch per colour in a fixed order */
+int ufem_sys_fill_K(ufem_sys *s, int npts)
+{
+  int c;
+  ugk_state st;
+  if (npts != 2 && npts != 3) { ufem_set_error("npts must be 2 or 3"); return AF_ERR_VAL; }
+  if (ugk_zero(s->w.st, sizeof(ugk_state), s->stream)) return gpu_fail("state reset");
+  if (ugk_event_record(s->ev[0], s->stream)) return gpu_fail("event");
+  if (ugk_zero(s->K.d_vals, sizeof(double) * (size_t)s->K.total_slots * 64 * 9, s->stream))
+    return gpu_fail("zero K");
+  for (c = 0; c < s->n_colors; c++) {
+    long off = s->color_off[c], cnt = s->color_off[c + 1] - off;
+    if (ugk_elem_stiff_scatter(&s->K, cnt, s->d_elist + off, s->d_x, s->d_y, s->d_z, s->d_enodes,
+                               s->d_E, s->d_nu, npts, s->w.st, s->stream))
+      return gpu_fail("element stiffness");
+  }
+  if (ugk_event_record(s->ev[1], s->stream)) return gpu_fail("event");
+  if (ugk_d2h(&st, s->w.st, sizeof(st), s->stream)) return gpu_fail("state read");
+  s->t_fill_ms = ugk_event_ms(s->ev[0], s->ev[1]);
+  if (st.err == 1) { ufem_set_error("element with non-positive Jacobian"); return AF_ERR_VAL; }
+  if (st.err == 3) { ufem_set_error("element block missing from the pattern"); return AF_ERR; }
+  s->filled = 1;
+  s->bc_applied = 0;
+  return AF_OK;
+}
+
+/* fem_add_loads(0): nodal forces (type 3) into F in input order,
+ * fem/fem_sol.c:1119-1189, fem/fem_loads.c:343-397 */
+int ufem_sys_add_loads(ufem_sys *s)
+{
+  long i;
+  for (i = 0; i < s->n_nl; i++)
+    if (s->nl_type[i] == 3) s->F_host[3 * s->nl_node[i] + s->nl_dir[i] - 1] += s->nl_val[i];
+  s->loaded = 1;
+  if (ugk_h2d(s->d_F, s->F_host, sizeof(double) * (size_t)s->n, s->stream) || ugk_sync(s->stream))
+    return gpu_fail("F upload");
+  return AF_OK;
+}
+
+/* fem_add_disps(AF_YES, 0) -> femApplyDisp with Val = 0 for every support in
+ * input order (fem/fem_loads.c:116-208): the reaction slot takes -F at the
+ * DOF, F and u there become 0, the row and the column are zeroed and the
+ * diagonal set to 1.  The device zeroes the whole column (the reference's
+ * serial build zeroes it only inside a window of rows, fem/fem_math.c:603-619;
+ * the solution is the same because u is 0 at the DOF throughout CG).  The
+ * original rows of the fixed DOFs are kept for the reactions. */
+int ufem_sys_add_disps(ufem_sys *s)
+{
+  long r;
+  if (!s->filled) { ufem_set_error("fill_K first"); return AF_ERR; }
+  if (s->bc_applied) return AF_OK;
+  for (r = 0; r < s->n_react; r++) {
+    long d = 3 * s->react_node[r] + s->react_dof[r] - 1;
+    long first = r, q;
+    for (q = 0; q < r; q++)   /* fem2IntPos: the first slot with this node/DOF */
+      if (s->react_node[q] == s->react_node[r] && s->react_dof[q] == s->react_dof[r]) { first = q; break; }
+    s->react_init[first] -= s->F_host[d];
+    s->F_host[d] = 0.0;
+  }
+  if (ugk_event_record(s->ev[2], s->stream)) return gpu_fail("event");
+  if (s->n_fixed > 0) {
+    if (ugk_save_rows(&s->K, s->n_fixed, s->d_fixed, s->maxb, s->d_saved_rows, s->d_saved_cols, s->stream) ||
+        ugk_apply_zero_bc(&s->K, s->n_fixed, s->d_fixed, s->stream))
+      return gpu_fail("supports");
+  }
+  if (ugk_event_record(s->ev[3], s->stream)) return gpu_fail("event");
+  if (ugk_h2d(s->d_F, s->F_host, sizeof(double) * (size_t)s->n, s->stream) || ugk_sync(s->stream))
+    return gpu_fail("F upload");
+  s->t_bc_ms = ugk_event_ms(s->ev[2], s->ev[3]);
+  s->bc_applied = 1;
+  return AF_OK;
+}
+
+int ufem_sys_solve(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info)
+{
+  ufem_solve_info tmp;
+  int rv;
+  if (!info) info = &tmp;
+  memset(info, 0, sizeof(*info));
+  if (!s->bc_applied) { ufem_set_error("supports not applied"); return AF_ERR; }
+  if (!o->warm_start && ugk_zero(s->d_u, sizeof(double) * (size_t)s->n, s->stream)) return gpu_fail("u reset");
+  rv = ufem_solve_device(&s->K, s->n, s->d_F, s->d_u, &s->w, s->stream, s->ev[4], s->ev[5], o, info);
+  s->t_solve_ms = info->solve_ms;
+  return rv;
+}
+
+/* fem_fill_K(AF_YES) + fem_fill_K_React (fem/fem_sol.c:844-871): the
+ * reaction at a support is (K u) there minus the load that stood there,
+ * with K the matrix before supports */
+int ufem_sys_reactions(ufem_sys *s, double *R, long *rnode, long *rdof)
+{
+  double *rf;
+  long r;
+  if (!s->bc_applied) { ufem_set_error("supports not applied"); return AF_ERR; }
+  if (s->n_fixed == 0) return AF_OK;
+  rf = (double *)malloc(sizeof(double) * (size_t)s->n_fixed);
+  if (!rf) return AF_ERR_MEM;
+  if (ugk_reactions(&s->K, s->n_fixed, s->d_fixed, s->maxb, s->d_saved_rows, s->d_saved_cols, s->d_u,
+                    NULL, s->d_Rfix, s->stream) ||
+      ugk_d2h(rf, s->d_Rfix, sizeof(double) * (size_t)s->n_fixed, s->stream)) {
+    free(rf);
+    return gpu_fail("reactions");
+  }
+  for (r = 0; r < s->n_react; r++) {
+    /* a repeated support entry keeps only its own initial value, like the
+       reference's per-entry slots (fem/fem_sol.c:844-871) */
+    if (R) R[r] = s->react_init[r] + rf[s->react_fix[r]];
+    if (rnode) rnode[r] = s->react_node[r];
+    if (rdof) rdof[r] = s->react_dof[r];
+  }
+  free(rf);
+  return AF_OK;
+}
+
+int ufem_sys_get_u(ufem_sys *s, double *u)
+{
+  if (ugk_d2h(u, s->d_u, sizeof(double) * (size_t)s->n, s->stream)) return gpu_fail("u read");
+  return AF_OK;
+}
+int ufem_sys_set_u(ufem_sys *s, const double *u)
+{
+  if (ugk_h2d(s->d_u, u, sizeof(double) * (size_t)s->n, s->stream) || ugk_sync(s->stream))
+    return gpu_fail("u write");
+  return AF_OK;
+}
+int ufem_sys_get_F(ufem_sys *s, double *F)
+{
+  if (ugk_d2h(F, s->d_F, sizeof(double) * (size_t)s->n, s->stream)) return gpu_fail("F read");
+  return AF_OK;
+}
+
+int ufem_sys_get_K(ufem_sys *s, long *indptr, int *cols, double *vals)
+{
+  long nn = s->n_nodes, tot = s->K.total_slots, nsl = s->K.n_slices, I;
+  int *hc = (int *)malloc(sizeof(int) * (size_t)tot * 64);
+  int *hl = (int *)malloc(sizeof(int) * (size_t)nn);
+  long *ho = (long *)malloc(sizeof(long) * (size_t)nsl);
+  double *hv = vals ? (double *)malloc(sizeof(double) * (size_t)tot * 64 * 9) : NULL;
+  long k = 0;
+  if (!hc || !hl || !ho || (vals && !hv)) { free(hc); free(hl); free(ho); free(hv); return AF_ERR_MEM; }
+  if (ugk_d2h(hc, s->K.d_cols, sizeof(int) * (size_t)tot * 64, s->stream) ||
+      ugk_d2h(hl, s->K.d_rowlen, sizeof(int) * (size_t)nn, s->stream) ||
+      ugk_d2h(ho, s->K.d_soff, sizeof(long) * (size_t)nsl, s->stream) ||
+      (vals && ugk_d2h(hv, s->K.d_vals, sizeof(double) * (size_t)tot * 64 * 9, s->stream))) {
+    free(hc); free(hl); free(ho); free(hv);
+    return gpu_fail("K read");
+  }
+  indptr[0] = 0;
+  for (I = 0; I < nn; I++) {
+    int r, sl, c;
+    long so = ho[I >> 6], ln = I & 63;
+    for (r = 0; r < 3; r++) {
+      for (sl = 0; sl < hl[I]; sl++) {
+        int J = hc[(so + sl) * 64 + ln];
+        for (c = 0; c < 3; c++) {
+          if (cols) cols[k] = 3 * J + c;
+          if (vals) vals[k] = hv[((so + sl) * 9 + 3 * r + c) * 64 + ln];
+          k++;
+        }
+      }
+      indptr[3 * I + r + 1] = k;
+    }
+  }
+  free(hc); free(hl); free(ho); free(hv);
+  return AF_OK;
+}
+
+int ufem_sys_spmv(ufem_sys *s, const double *x, double *y)
+{
+  if (ugk_h2d(s->w.z, x, sizeof(double) * (size_t)s->n, s->stream) ||
+      ugk_spmv(&s->K, s->w.z, s->w.q, s->stream) ||
+      ugk_d2h(y, s->w.q, sizeof(double) * (size_t)s->n, s->stream))
+    return gpu_fail("spmv");
+  return AF_OK;
+}
+
+/* bytes one SpMV moves by the algorithmic count: every stored value and
+ * column index of the SELL slots once, the x entries once, y once */
+static double spmv_bytes(const ugk_mat *K, int fp32)
+{
+  double slots = (double)K->total_slots * 64.0;
+  double vb = fp32 ? 4.0 : 8.0;
+  double nrow = (double)K->n_brows * K->bs;
+  return slots * (K->bs * K->bs * vb + 4.0) + (double)K->n_slices * (4.0 + 8.0) + 2.0 * nrow * vb;
+}
+
+int ufem_sys_time_spmv(ufem_sys *s, int reps, int fp32, double *ms, double *bytes)
+{
+  int i;
+  float t;
+  if (reps < 1) reps = 1;
+  if (fp32 && !s->K.d_vals32) {
+    s->K.d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)s->K.total_slots * 64 * 9);
+    if (!s->K.d_vals32 || ugk_vals_to_f32(&s->K, s->stream)) return gpu_fail("fp32 copy");
+  }
+  /* warm-up */
+  if (fp32 ? ugk_spmv_f32(&s->K, s->w.z32, s->w.q32, s->stream) : ugk_spmv(&s->K, s->w.z, s->w.q, s->stream))
+    return gpu_fail("spmv");
+  if (ugk_event_record(s->ev[4], s->stream)) return gpu_fail("event");
+  for (i = 0; i < reps; i++)
+    if (fp32 ? ugk_spmv_f32(&s->K, s->w.z32, s->w.q32, s->stream) : ugk_spmv(&s->K, s->w.z, s->w.q, s->stream))
+      return gpu_fail("spmv");
+  if (ugk_event_record(s->ev[5], s->stream)) return gpu_fail("event");
+  t = ugk_event_ms(s->ev[4], s->ev[5]);
+  if (ms) *ms = (double)t / reps;
+  if (bytes) *bytes = spmv_bytes(&s->K, fp32);
+  return AF_OK;
+}
+
+void ufem_sys_timings(const ufem_sys *s, double t[3])
+{
+  t[0] = s->t_fill_ms;
+  t[1] = s->t_bc_ms;
+  t[2] = s->t_solve_ms;
+}
+
+int ufem_gpu_element_stiffness(long n, const double *xyz, const double *E, const double *nu, int npts,
+                               double *Ke)
+{
+  void *st = NULL, *strm = NULL;
+  double *dxyz = NULL, *dE = NULL, *dnu = NULL, *dK = NULL;
+  ugk_state *dst = NULL, hst;
+  int rv = AF_ERR;
+  if (ensure_gpu() != AF_OK) return AF_ERR;
+  if (n <= 0) return AF_OK;
+  strm = ugk_stream_create();
+  dxyz = (double *)ugk_malloc(sizeof(double) * (size_t)n * 24);
+  dE = (double *)ugk_malloc(sizeof(double) * (size_t)n);
+  dnu = (double *)ugk_malloc(sizeof(double) * (size_t)n);
+  dK = (double *)ugk_malloc(sizeof(double) * (size_t)n * 576);
+  dst = (ugk_state *)ugk_malloc(sizeof(ugk_state));
+  if (!strm || !dxyz || !dE || !dnu || !dK || !dst) { gpu_fail("alloc"); goto out; }
+  if (ugk_h2d(dxyz, xyz, sizeof(double) * (size_t)n * 24, strm) ||
+      ugk_h2d(dE, E, sizeof(double) * (size_t)n, strm) || ugk_h2d(dnu, nu, sizeof(double) * (size_t)n, strm) ||
+      ugk_elem_stiff_dense(n, dxyz, NULL, dE, dnu, npts, dK, dst, strm) ||
+      ugk_d2h(Ke, dK, sizeof(double) * (size_t)n * 576, strm) || ugk_d2h(&hst, dst, sizeof(hst), strm)) {
+    gpu_fail("element stiffness");
+    goto out;
+  }
+  rv = hst.err ? AF_ERR_VAL : AF_OK;
+  if (hst.err) ufem_set_error("element with non-positive Jacobian");
+out:
+  (void)st;
+  ugk_free(dxyz); ugk_free(dE); ugk_free(dnu); ugk_free(dK); ugk_free(dst);
+  ugk_stream_destroy(strm);
+  return rv;
+}

@@ -1,0 +1,162 @@
+/*
+ * ugk.h -- thin C-ABI between the host C code of libufem_gpu.so and its
+ * hand-written HIP kernels for gfx950 (ugk_*.hip).  Plain pointers and sizes
+ * only; every pointer named d_* is device memory.
+ *
+ * Matrix storage on the device is "SELL-64 of b x b blocks": block rows are
+ * grouped into slices of 64 (one wavefront, lane = block row); slice s owns
+ * slen[s] slots; the column of slot k of lane l is cols[(soff[s] + k) * 64 + l]
+ * and its b*b values sit at vals[((soff[s] + k) * b*b + q) * 64 + l] for
+ * q = r*b + c, so every load of a wavefront is 64 consecutive words.  Slots
+ * past a row's length are padding: column = the row itself, value 0.
+ * b = 3 for meshes (node blocks), b = 1 for matrices handed over in the
+ * reference's scalar row-slot form.
+ */
+#ifndef UGK_H
+#define UGK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int bs;             /* block size b: 1 or 3                           */
+  long n_brows;       /* block rows                                     */
+  long n_slices;      /* ceil(n_brows / 64)                             */
+  long total_slots;   /* sum of slen                                    */
+  long nnzb;          /* real (unpadded) blocks                         */
+  int *d_slen;        /* [n_slices]                                     */
+  long *d_soff;       /* [n_slices] slot offset of each slice           */
+  int *d_rowlen;      /* [n_brows] real blocks per row                  */
+  int *d_cols;        /* [total_slots * 64]                             */
+  double *d_vals;     /* [total_slots * 64 * b*b]                       */
+  float *d_vals32;    /* FP32 copy for mixed precision (may be NULL)    */
+} ugk_mat;
+
+/* largest grid of a reducing kernel, and the size (in doubles) of the
+ * partial-sum buffer every reducing call takes (8 slots per block + counter) */
+#define UGK_RED_MAX_BLOCKS 65536
+#define UGK_PART_DOUBLES (8L * UGK_RED_MAX_BLOCKS + 16)
+
+/* solver scalars kept on the device between kernels */
+typedef struct {
+  double rz[2];       /* (r, z) of the current / next iteration         */
+  double alpha, beta, omega, rho;
+  double rr, bb, xx;  /* last ||r||^2, ||b||^2, ||x||^2                 */
+  double normA;       /* max row 2-norm (reference stopping rule)       */
+  double tol;
+  double aux[8];
+  long iter;          /* completed iterations                           */
+  long maxit;
+  int rule;           /* 0 reference rule, 1 relative residual          */
+  int done;           /* 0 running, 1 converged, 2 breakdown, 3 maxit   */
+  int err;            /* kernel-detected error (zero diagonal, detJ<=0)  */
+  int pad;
+  unsigned int counter[8];  /* last-block tickets, reset by the last block */
+} ugk_state;
+
+/* ---------------- memory / stream / timing ------------------------------ */
+int   ugk_init(void);
+const char *ugk_error(void);
+void *ugk_malloc(size_t bytes);                  /* zero-filled; NULL on failure */
+void  ugk_free(void *p);
+int   ugk_h2d(void *d, const void *h, size_t bytes, void *stream);
+int   ugk_d2h(void *h, const void *d, size_t bytes, void *stream);
+int   ugk_d2d(void *d, const void *s, size_t bytes, void *stream);
+int   ugk_zero(void *d, size_t bytes, void *stream);  /* fill kernel, graph-safe */
+void *ugk_stream_create(void);
+void  ugk_stream_destroy(void *stream);
+int   ugk_sync(void *stream);
+void *ugk_event_create(void);
+void  ugk_event_destroy(void *ev);
+int   ugk_event_record(void *ev, void *stream);
+float ugk_event_ms(void *ev0, void *ev1);        /* after both completed */
+int   ugk_device_info(int *n_cu, long *hbm_bytes, char *name, int name_len);
+void *ugk_host_alloc(size_t bytes);              /* pinned host memory */
+void  ugk_host_free(void *p);
+int   ugk_d2h_async(void *h, const void *d, size_t bytes, void *stream);
+int   ugk_event_sync(void *ev);
+
+/* graphs: capture what is enqueued on `stream` between begin and end */
+int   ugk_graph_begin(void *stream);
+void *ugk_graph_end(void *stream);               /* returns executable graph */
+int   ugk_graph_launch(void *graph, void *stream);
+void  ugk_graph_destroy(void *graph);
+
+/* ---------------- assembly ---------------------------------------------- */
+/* element stiffness, one wavefront per element: dense 24x24 per element into
+ * d_Ke (row-major), or added into the SELL-3 matrix for the elements listed in
+ * d_elist (one colour: no two share a node).  npts = 2 or 3 Gauss points per
+ * axis.  Sets st->err = 1 on a non-positive Jacobian. */
+int ugk_elem_stiff_dense(long n_elems, const double *d_xyz, const int *d_enodes,
+                         const double *d_E, const double *d_nu, int npts,
+                         double *d_Ke, ugk_state *d_st, void *stream);
+int ugk_elem_stiff_scatter(const ugk_mat *K, long n_list, const int *d_elist,
+                           const double *d_x, const double *d_y, const double *d_z,
+                           const int *d_enodes, const double *d_E, const double *d_nu,
+                           int npts, ugk_state *d_st, void *stream);
+/* supports: copy the rows of the fixed DOFs (for reactions), then zero those
+ * rows and columns and put 1 on their diagonal */
+int ugk_save_rows(const ugk_mat *K, long n_fixed, const int *d_fixed, int maxb,
+                  double *d_rows, int *d_rcols, void *stream);
+int ugk_apply_zero_bc(const ugk_mat *K, long n_fixed, const int *d_fixed, void *stream);
+/* R[k] = (K_orig u)[fixed[k]] - F_orig[fixed[k]] from the saved rows */
+int ugk_reactions(const ugk_mat *K, long n_fixed, const int *d_fixed, int maxb,
+                  const double *d_rows, const int *d_rcols, const double *d_u,
+                  const double *d_Forig, double *d_R, void *stream);
+/* FP32 copy of the values */
+int ugk_vals_to_f32(const ugk_mat *K, void *stream);
+
+/* ---------------- sparse kernels ---------------------------------------- */
+int ugk_spmv(const ugk_mat *K, const double *d_x, double *d_y, void *stream);
+int ugk_spmv_f32(const ugk_mat *K, const float *d_x, float *d_y, void *stream);
+/* dinv[i] = 1 / K_ii; st->err = 2 if |K_ii| < 1e-17 */
+int ugk_diag_inv(const ugk_mat *K, double *d_dinv, ugk_state *d_st, void *stream);
+/* max row 2-norm into st->normA */
+int ugk_mat_norm(const ugk_mat *K, ugk_state *d_st, double *d_part, void *stream);
+
+/* ---------------- vector kernels (FP64) --------------------------------- */
+int ugk_dot(long n, const double *d_a, const double *d_b, double *d_out, double *d_part, void *stream);
+int ugk_axpby(long n, double a, const double *d_x, double b, const double *d_y, double *d_z, void *stream);
+
+/* ---------------- PCG (Jacobi), streamed form ---------------------------- */
+/* start: r = b - A x, z = dinv r, p = 0; state: rz, rr, bb, (xx); iter = 0 */
+int ugk_pcg_start(const ugk_mat *K, long n, const double *d_b, const double *d_x,
+                  double *d_r, double *d_z, double *d_p, const double *d_dinv,
+                  ugk_state *d_st, double *d_part, void *stream);
+/* one iteration = two kernels; no-ops once st->done != 0 */
+int ugk_pcg_iter(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_z,
+                 double **d_p /* [2] ping-pong */, double *d_q, const double *d_dinv,
+                 ugk_state *d_st, double *d_part, long it, void *stream);
+/* whole solve in one cooperative launch (grid barriers between phases) */
+int ugk_pcg_persistent(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_z,
+                       double *d_p0, double *d_p1, double *d_q, const double *d_dinv,
+                       ugk_state *d_st, double *d_part, void *stream);
+
+/* ---------------- PCG in FP32 (inner solve of mixed precision) ----------- */
+int ugk_pcg32_start(const ugk_mat *K, long n, const double *d_r64, double scale,
+                    float *d_r, float *d_z, float *d_p, float *d_x, const float *d_dinv32,
+                    ugk_state *d_st, double *d_part, void *stream);
+int ugk_pcg32_iter(const ugk_mat *K, long n, float *d_x, float *d_r, float *d_z,
+                   float **d_p, float *d_q, const float *d_dinv32,
+                   ugk_state *d_st, double *d_part, long it, void *stream);
+/* x64 += scale * x32;  r64 = b - A x64 (FP64) with ||r||^2 into st->rr */
+int ugk_ir_update(const ugk_mat *K, long n, double *d_x64, const float *d_x32, double scale,
+                  const double *d_b, double *d_r64, ugk_state *d_st, double *d_part, void *stream);
+int ugk_to_f32(long n, const double *d_a, float *d_b, void *stream);
+
+/* ---------------- BiCGStab (Jacobi, right preconditioned) ---------------- */
+int ugk_bicgs_start(const ugk_mat *K, long n, const double *d_b, const double *d_x,
+                    double *d_r, double *d_rhat, double *d_p, double *d_v,
+                    ugk_state *d_st, double *d_part, void *stream);
+int ugk_bicgs_iter(const ugk_mat *K, long n, double *d_x, double *d_r, const double *d_rhat,
+                   double *d_p, double *d_phat, double *d_v, double *d_s, double *d_shat,
+                   double *d_t, const double *d_dinv, ugk_state *d_st, double *d_part,
+                   void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1489 @@
+// ugk_kernels.hip -- HIP kernels for gfx950 behind the thin C-ABI in ugk.h.
+//
+// Hot path of uFEM's linear-static solve on one MI355X:
+//   * element 9 (8-node brick) stiffness, one wavefront per element, Gauss
+//     gradients staged in LDS, K_ab = sum_gp w detJ (lam g_a g_b^T +
+//     mu g_b g_a^T + mu (g_a.g_b) I) per node pair -- the isotropic form of
+//     H^T D H of fem/fem_e009.c:587-602 with D of fem/fem_pl3d.c:32-58;
+//   * deterministic assembly: each colour's elements share no node, so every
+//     3x3 block of K is updated by exactly one lane per launch (no atomics),
+//     and colours run in a fixed order;
+//   * SELL-64 block SpMV streamed from HBM with fully coalesced value loads;
+//   * Jacobi PCG with the p-update folded into the SpMV gather and the vector
+//     update fused with its three dot products; reductions finish in the last
+//     block to arrive (fixed summation order, so results repeat bit for bit).
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "ugk.h"
+
+namespace cg = cooperative_groups;
+
+static thread_local char g_err[512] = "";
+
+static int fail(const char *what, hipError_t e)
+{
+  snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+  return -1;
+}
+#define CK(call)                                  \
+  do {                                            \
+    hipError_t e_ = (call);                       \
+    if (e_ != hipSuccess) return fail(#call, e_); \
+  } while (0)
+#define CK_LAUNCH(what) CK(hipGetLastError())
+
+#define WG 256
+#define NW (WG / 64)
+
+// --------------------------------------------------------------------------
+// reductions
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ double ld_agent(const double *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double *p, double v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum NV values over the block (fixed order), publish the block's partials,
+// and let the last block to arrive reduce all partials in index order.
+// Returns true in every thread of the last block, with tot[] valid in all
+// its threads.  MAXR: reduce with max instead of sum.
+template <int NV, bool MAXR = false>
+__device__ bool block_reduce_last(double (&v)[NV], double *part, unsigned int *counter,
+                                  double (&tot)[NV])
+{
+  __shared__ double sh[NV][NW];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned G = gridDim.x;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = MAXR ? wave_max(v[k]) : wave_sum(v[k]);
+    if (lane == 0) sh[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double a = sh[k][0];
+      for (int w = 1; w < NW; w++) a = MAXR ? fmax(a, sh[k][w]) : a + sh[k][w];
+      st_agent(&part[(size_t)k * G + blockIdx.x], a);
+    }
+    __threadfence();
+    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (t == G - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double a = MAXR ? 0.0 : 0.0;
+    for (unsigned i = threadIdx.x; i < G; i += WG) {
+      double pv = ld_agent(&part[(size_t)k * G + i]);
+      a = MAXR ? fmax(a, pv) : a + pv;
+    }
+    double s = MAXR ? wave_max(a) : wave_sum(a);
+    __syncthreads();
+    if (lane == 0) sh[k][wave] = s;
+    __syncthreads();
+    double r = sh[k][0];
+    for (int w = 1; w < NW; w++) r = MAXR ? fmax(r, sh[k][w]) : r + sh[k][w];
+    tot[k] = r;
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Redundant grid reduction inside a cooperative kernel: every block sums the
+// G published partials in index order (identical result in every block).
+template <int NV>
+__device__ void grid_partials_sum(const double *part, unsigned G, double (&tot)[NV])
+{
+  __shared__ double sh[NV][NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double a = 0.0;
+    for (unsigned i = threadIdx.x; i < G; i += WG) a += ld_agent(&part[(size_t)k * G + i]);
+    double s = wave_sum(a);
+    if (lane == 0) sh[k][wave] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double r = sh[k][0];
+    for (int w = 1; w < NW; w++) r += sh[k][w];
+    tot[k] = r;
+  }
+  __syncthreads();
+}
+
+template <int NV>
+__device__ void block_publish(double (&v)[NV], double *part)
+{
+  __shared__ double sh[NV][NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) sh[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double a = sh[k][0];
+      for (int w = 1; w < NW; w++) a += sh[k][w];
+      st_agent(&part[(size_t)k * gridDim.x + blockIdx.x], a);
+    }
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------------------------
+// SELL-64 addressing
+// --------------------------------------------------------------------------
+struct SellView {
+  const int *slen;
+  const long *soff;
+  const int *rowlen;
+  const int *cols;
+  long n_brows, n_slices;
+};
+
+__device__ __forceinline__ int find_slot(const SellView &M, long I, long J)
+{
+  // binary search of column J among the sorted real slots of block row I
+  const long sl = I >> 6, ln = I & 63;
+  const int *c = M.cols + M.soff[sl] * 64 + ln;
+  int lo = 0, hi = M.rowlen[I] - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int cm = c[(long)mid * 64];
+    if (cm == J) return mid;
+    if (cm < J) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// logical slice for a block: XCD-aware remap so that each XCD (blocks are
+// dealt round-robin over 8 XCDs) streams one contiguous range of rows and
+// its L2 keeps the x entries those rows gather
+__device__ __forceinline__ long remap_block(long b, long nb)
+{
+  // nb is a multiple of 8 (see spmv_blocks), so this is a bijection
+  const long per = nb >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+// --------------------------------------------------------------------------
+// element stiffness
+// --------------------------------------------------------------------------
+template <int NPTS>
+struct Gauss;
+template <>
+struct Gauss<2> {
+  __device__ static double pt(int i) { return i == 0 ? -0.57735026918962576451 : 0.57735026918962576451; }
+  __device__ static double wt(int) { return 1.0; }
+};
+template <>
+struct Gauss<3> {
+  __device__ static double pt(int i) { return i == 0 ? -0.77459666924148337704 : (i == 1 ? 0.0 : 0.77459666924148337704); }
+  __device__ static double wt(int i) { return i == 1 ? 0.88888888888888888889 : 0.55555555555555555556; }
+};
+
+// corner signs of the reference brick, fem/fem_e009.c:36-69
+__device__ __forceinline__ void corner(int n, double &xs, double &ys, double &zs)
+{
+  xs = ((n + 1) & 2) ? 1.0 : -1.0;   // 0:-,1:+,2:+,3:-
+  ys = (n & 2) ? 1.0 : -1.0;         // 0:-,1:-,2:+,3:+
+  zs = (n & 4) ? 1.0 : -1.0;
+}
+
+template <int NPTS, bool DENSE>
+__global__ __launch_bounds__(WG) void k_elem_stiff(long n_list, const int *__restrict__ elist,
+                                                   const double *__restrict__ cx,
+                                                   const double *__restrict__ cy,
+                                                   const double *__restrict__ cz,
+                                                   const int *__restrict__ enodes,
+                                                   const double *__restrict__ Ev,
+                                                   const double *__restrict__ nuv,
+                                                   double *__restrict__ Ke, SellView M,
+                                                   double *__restrict__ vals, ugk_state *st)
+{
+  constexpr int NGP = NPTS * NPTS * NPTS;
+  __shared__ double s_xyz[NW][24];
+  __shared__ double s_g[NW][NGP][8][3];
+  __shared__ double s_w[NW][NGP];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long idx = (long)blockIdx.x * NW + wave;
+  const bool active = idx < n_list;
+  const long e = active ? (DENSE ? idx : (long)elist[idx]) : 0;
+
+  if (active && lane < 24) {
+    const int nd = lane / 3, c = lane % 3;
+    if (DENSE) {
+      s_xyz[wave][lane] = cx[e * 24 + lane];
+    } else {
+      const int node = enodes[e * 8 + nd];
+      s_xyz[wave][lane] = c == 0 ? cx[node] : (c == 1 ? cy[node] : cz[node]);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (int t = lane; t < NGP * 8; t += 64) {
+      const int gp = t >> 3, n = t & 7;
+      const int gi = gp / (NPTS * NPTS), gj = (gp / NPTS) % NPTS, gk = gp % NPTS;
+      const double xi = Gauss<NPTS>::pt(gi), eta = Gauss<NPTS>::pt(gj), zeta = Gauss<NPTS>::pt(gk);
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      double dn[3];
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        double xs, ys, zs;
+        corner(m, xs, ys, zs);
+        const double d0 = 0.125 * xs * (1.0 + ys * eta) * (1.0 + zs * zeta);
+        const double d1 = 0.125 * ys * (1.0 + xs * xi) * (1.0 + zs * zeta);
+        const double d2 = 0.125 * zs * (1.0 + xs * xi) * (1.0 + ys * eta);
+        if (m == n) { dn[0] = d0; dn[1] = d1; dn[2] = d2; }
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          const double xb = s_xyz[wave][3 * m + b];
+          J[0][b] += d0 * xb;
+          J[1][b] += d1 * xb;
+          J[2][b] += d2 * xb;
+        }
+      }
+      // J[a][b] = dx_b / dxi_a ; inverse by adjugate
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      if (!(det > 0.0)) st->err = 1;
+      const double id = 1.0 / det;
+      double Ji[3][3];
+      Ji[0][0] = c00 * id;
+      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+      Ji[1][0] = c01 * id;
+      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+      Ji[2][0] = c02 * id;
+      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        s_g[wave][gp][n][a] = Ji[a][0] * dn[0] + Ji[a][1] * dn[1] + Ji[a][2] * dn[2];
+      if (n == 0) s_w[wave][gp] = det * Gauss<NPTS>::wt(gi) * Gauss<NPTS>::wt(gj) * Gauss<NPTS>::wt(gk);
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int a = lane >> 3, b = lane & 7;
+  const double E = Ev[e], nu = nuv[e];
+  const double lam = (E * nu) / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double mu = E / (2.0 * (1.0 + nu));
+  double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int gp = 0; gp < NGP; gp++) {
+    const double w = s_w[wave][gp];
+    const double ga0 = s_g[wave][gp][a][0], ga1 = s_g[wave][gp][a][1], ga2 = s_g[wave][gp][a][2];
+    const double gb0 = s_g[wave][gp][b][0], gb1 = s_g[wave][gp][b][1], gb2 = s_g[wave][gp][b][2];
+    const double wl = w * lam, wm = w * mu;
+    const double dd = wm * (ga0 * gb0 + ga1 * gb1 + ga2 * gb2);
+    k[0] += wl * ga0 * gb0 + wm * ga0 * gb0 + dd;
+    k[1] += wl * ga0 * gb1 + wm * ga1 * gb0;
+    k[2] += wl * ga0 * gb2 + wm * ga2 * gb0;
+    k[3] += wl * ga1 * gb0 + wm * ga0 * gb1;
+    k[4] += wl * ga1 * gb1 + wm * ga1 * gb1 + dd;
+    k[5] += wl * ga1 * gb2 + wm * ga2 * gb1;
+    k[6] += wl * ga2 * gb0 + wm * ga0 * gb2;
+    k[7] += wl * ga2 * gb1 + wm * ga1 * gb2;
+    k[8] += wl * ga2 * gb2 + wm * ga2 * gb2 + dd;
+  }
+  if (DENSE) {
+    double *o = Ke + e * 576;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) o[(3 * a + i) * 24 + 3 * b + j] = k[3 * i + j];
+  } else {
+    const long I = enodes[e * 8 + a], J = enodes[e * 8 + b];
+    const int s = find_slot(M, I, J);
+    if (s < 0) { st->err = 3; return; }
+    double *v = vals + ((M.soff[I >> 6] + s) * 9) * 64 + (I & 63);
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[q * 64] += k[q];
+  }
+}
+
+// --------------------------------------------------------------------------
+// supports and reactions (block size 3)
+// --------------------------------------------------------------------------
+__global__ void k_save_rows(SellView M, const double *__restrict__ vals, long n_fixed,
+                            const int *__restrict__ fixed, int maxb, double *rows, int *rcols)
+{
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fixed) return;
+  const long d = fixed[f], I = d / 3, r = d % 3;
+  const long so = M.soff[I >> 6], ln = I & 63;
+  const int L = M.rowlen[I];
+  for (int s = 0; s < maxb; s++) {
+    if (s < L) {
+      rcols[f * maxb + s] = M.cols[(so + s) * 64 + ln];
+      for (int c = 0; c < 3; c++) rows[(f * maxb + s) * 3 + c] = vals[((so + s) * 9 + 3 * r + c) * 64 + ln];
+    } else {
+      rcols[f * maxb + s] = -1;
+      for (int c = 0; c < 3; c++) rows[(f * maxb + s) * 3 + c] = 0.0;
+    }
+  }
+}
+
+__global__ void k_zero_rowcol(SellView M, double *__restrict__ vals, long n_fixed,
+                              const int *__restrict__ fixed)
+{
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fixed) return;
+  const long d = fixed[f], I = d / 3, r = d % 3;
+  const long so = M.soff[I >> 6], ln = I & 63;
+  const int L = M.rowlen[I];
+  for (int s = 0; s < L; s++) {
+    for (int c = 0; c < 3; c++) vals[((so + s) * 9 + 3 * r + c) * 64 + ln] = 0.0;  // row d
+    const long Jn = M.cols[(so + s) * 64 + ln];                                     // column d
+    const int t = find_slot(M, Jn, I);
+    if (t < 0) continue;
+    const long soJ = M.soff[Jn >> 6], lnJ = Jn & 63;
+    for (int rr = 0; rr < 3; rr++) vals[((soJ + t) * 9 + 3 * rr + r) * 64 + lnJ] = 0.0;
+  }
+}
+
+__global__ void k_set_diag_one(SellView M, double *__restrict__ vals, long n_fixed,
+                               const int *__restrict__ fixed)
+{
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fixed) return;
+  const long d = fixed[f], I = d / 3, r = d % 3;
+  const int s = find_slot(M, I, I);
+  if (s < 0) return;
+  vals[((M.soff[I >> 6] + s) * 9 + 4 * r) * 64 + (I & 63)] = 1.0;
+}
+
+__global__ void k_reactions(long n_fixed, const int *__restrict__ fixed, int maxb,
+                            const double *__restrict__ rows, const int *__restrict__ rcols,
+                            const double *__restrict__ u, const double *__restrict__ Forig,
+                            double *R)
+{
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_fixed) return;
+  double acc = 0.0;
+  for (int s = 0; s < maxb; s++) {
+    const int J = rcols[f * maxb + s];
+    if (J < 0) break;
+    for (int c = 0; c < 3; c++) acc += rows[(f * maxb + s) * 3 + c] * u[3L * J + c];
+  }
+  R[f] = acc - (Forig ? Forig[fixed[f]] : 0.0);
+}
+
+// --------------------------------------------------------------------------
+// SpMV and matrix utilities
+// --------------------------------------------------------------------------
+template <int BS, typename T, typename TX>
+__device__ __forceinline__ void sell_row(const SellView &M, const T *__restrict__ vals, long sl,
+                                         int lane, const TX *__restrict__ x, TX (&acc)[BS])
+{
+  const int L = M.slen[sl];
+  const long so = M.soff[sl];
+  const int *cp = M.cols + so * 64 + lane;
+  const T *vp = vals + so * 64 * BS * BS + lane;
+#pragma unroll
+  for (int r = 0; r < BS; r++) acc[r] = 0;
+#pragma unroll 3
+  for (int s = 0; s < L; s++) {
+    const long J = cp[(long)s * 64];
+    TX xv[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) xv[c] = x[J * BS + c];
+    const T *v = vp + (long)s * 64 * BS * BS;
+#pragma unroll
+    for (int r = 0; r < BS; r++)
+#pragma unroll
+      for (int c = 0; c < BS; c++) acc[r] += (TX)v[(r * BS + c) * 64] * xv[c];
+  }
+}
+
+template <int BS, typename T, typename TX>
+__global__ __launch_bounds__(WG) void k_spmv(SellView M, const T *__restrict__ vals,
+                                             const TX *__restrict__ x, TX *__restrict__ y)
+{
+  const long nb = gridDim.x;
+  const long lb = remap_block(blockIdx.x, nb);
+  const long sl = lb * NW + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (lb >= nb || sl >= M.n_slices) return;
+  TX acc[BS];
+  sell_row<BS, T, TX>(M, vals, sl, lane, x, acc);
+  const long row = sl * 64 + lane;
+  if (row < M.n_brows) {
+#pragma unroll
+    for (int r = 0; r < BS; r++) y[row * BS + r] = acc[r];
+  }
+}
+
+template <int BS>
+__global__ void k_diag_inv(SellView M, const double *__restrict__ vals, double *dinv, ugk_state *st)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M.n_brows * BS) return;
+  const long I = i / BS, r = i % BS;
+  const int s = find_slot(M, I, I);
+  double v = 0.0;
+  if (s >= 0) v = vals[((M.soff[I >> 6] + s) * BS * BS + r * BS + r) * 64 + (I & 63)];
+  if (!(fabs(v) >= 1e-17)) { st->err = 2; dinv[i] = 0.0; return; }
+  dinv[i] = 1.0 / v;
+}
+
+template <int BS>
+__global__ __launch_bounds__(WG) void k_mat_norm(SellView M, const double *__restrict__ vals,
+                                                 ugk_state *st, double *part)
+{
+  double v[1] = {0.0}, tot[1];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < M.n_brows * BS; i += (long)gridDim.x * WG) {
+    const long I = i / BS, r = i % BS;
+    const long so = M.soff[I >> 6], ln = I & 63;
+    double s2 = 0.0;
+    for (int s = 0; s < M.rowlen[I]; s++)
+      for (int c = 0; c < BS; c++) {
+        const double a = vals[((so + s) * BS * BS + r * BS + c) * 64 + ln];
+        s2 += a * a;
+      }
+    v[0] = fmax(v[0], sqrt(s2));
+  }
+  if (block_reduce_last<1, true>(v, part, &st->counter[7], tot) && threadIdx.x == 0) st->normA = tot[0];
+}
+
+template <typename T>
+__global__ void k_vals_f32(long n, const double *__restrict__ a, T *__restrict__ b)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = (T)a[i];
+}
+
+__global__ void k_fill_zero(long nwords, unsigned int *p)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+__global__ __launch_bounds__(WG) void k_dot(long n, const double *__restrict__ a,
+                                            const double *__restrict__ b, double *out,
+                                            double *part, unsigned int *counter)
+{
+  double v[1] = {0.0}, tot[1];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) v[0] += a[i] * b[i];
+  if (block_reduce_last<1>(v, part, counter, tot) && threadIdx.x == 0) *out = tot[0];
+}
+
+__global__ void k_axpby(long n, double a, const double *__restrict__ x, double b,
+                        const double *__restrict__ y, double *z)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    z[i] = a * x[i] + b * y[i];
+}
+
+// --------------------------------------------------------------------------
+// Jacobi PCG (FP64), streamed form
+// --------------------------------------------------------------------------
+// start: r = b - A x; z = dinv r; p = 0; (r,z), (r,r), (b,b), (x,x)
+template <int BS>
+__global__ __launch_bounds__(WG) void k_pcg_start(SellView M, const double *__restrict__ vals,
+                                                  const double *__restrict__ b,
+                                                  const double *__restrict__ x, double *r,
+                                                  double *z, double *p,
+                                                  const double *__restrict__ dinv, ugk_state *st,
+                                                  double *part)
+{
+  const long nb = gridDim.x;
+  const long lb = remap_block(blockIdx.x, nb);
+  const long sl = lb * NW + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double v[4] = {0, 0, 0, 0}, tot[4];
+  if (lb < nb && sl < M.n_slices) {
+    double acc[BS];
+    sell_row<BS, double, double>(M, vals, sl, lane, x, acc);
+    const long row = sl * 64 + lane;
+    if (row < M.n_brows) {
+#pragma unroll
+      for (int k = 0; k < BS; k++) {
+        const long i = row * BS + k;
+        const double ri = b[i] - acc[k];
+        const double zi = ri * dinv[i];
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = 0.0;
+        v[0] += ri * zi;
+        v[1] += ri * ri;
+        v[2] += b[i] * b[i];
+        v[3] += x[i] * x[i];
+      }
+    }
+  }
+  if (block_reduce_last<4>(v, part, &st->counter[0], tot) && threadIdx.x == 0) {
+    st->rz[0] = tot[0];
+    st->rr = tot[1];
+    st->bb = tot[2];
+    st->xx = tot[3];
+    st->beta = 0.0;
+    st->iter = 0;
+    const double nb2 = sqrt(tot[2]);
+    int done = 0;
+    if (!(nb2 > 0.0)) done = 1;   // zero load: x stays as given (caller zeroes)
+    else if (!(tot[0] == tot[0])) done = 2;
+    else if (st->rule == 1 ? sqrt(tot[1]) <= st->tol * nb2
+                           : sqrt(tot[1]) <= st->tol * (st->normA * sqrt(tot[3]) + nb2)) done = 1;
+    if (st->maxit <= 0 && !done) done = 3;
+    st->done = done;
+  }
+}
+
+// phase A: p_new = z + beta p_old (own rows), q = A p_new (gathering z and
+// p_old at the neighbours), partial (p_new, q); last block publishes aux[0] = pq
+template <int BS>
+__global__ __launch_bounds__(WG) void k_pcg_a(SellView M, const double *__restrict__ vals,
+                                              const double *__restrict__ z,
+                                              const double *__restrict__ pold,
+                                              double *__restrict__ pnew, double *__restrict__ q,
+                                              ugk_state *st, double *part)
+{
+  if (st->done) return;
+  const double beta = st->beta;
+  const long nb = gridDim.x;
+  const long lb = remap_block(blockIdx.x, nb);
+  const long sl = lb * NW + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double v[1] = {0.0}, tot[1];
+  if (lb < nb && sl < M.n_slices) {
+    const int L = M.slen[sl];
+    const long so = M.soff[sl];
+    const int *cp = M.cols + so * 64 + lane;
+    const double *vp = vals + so * 64 * BS * BS + lane;
+    double acc[BS];
+#pragma unroll
+    for (int r = 0; r < BS; r++) acc[r] = 0.0;
+#pragma unroll 3
+    for (int s = 0; s < L; s++) {
+      const long J = cp[(long)s * 64];
+      double xv[BS];
+#pragma unroll
+      for (int c = 0; c < BS; c++) xv[c] = z[J * BS + c] + beta * pold[J * BS + c];
+      const double *vv = vp + (long)s * 64 * BS * BS;
+#pragma unroll
+      for (int r = 0; r < BS; r++)
+#pragma unroll
+        for (int c = 0; c < BS; c++) acc[r] += vv[(r * BS + c) * 64] * xv[c];
+    }
+    const long row = sl * 64 + lane;
+    if (row < M.n_brows) {
+#pragma unroll
+      for (int r = 0; r < BS; r++) {
+        const long i = row * BS + r;
+        const double pn = z[i] + beta * pold[i];
+        pnew[i] = pn;
+        q[i] = acc[r];
+        v[0] += pn * acc[r];
+      }
+    }
+  }
+  if (block_reduce_last<1>(v, part, &st->counter[1], tot) && threadIdx.x == 0) st->aux[0] = tot[0];
+}
+
+// phase B: alpha = rz / pq; x += alpha p; r -= alpha q; z = dinv r;
+// partials (r,z), (r,r), (x,x); last block: convergence test, beta
+__global__ __launch_bounds__(WG) void k_pcg_b(long n, double *__restrict__ x, double *__restrict__ r,
+                                              double *__restrict__ z, const double *__restrict__ p,
+                                              const double *__restrict__ q,
+                                              const double *__restrict__ dinv, ugk_state *st,
+                                              double *part)
+{
+  if (st->done) return;
+  const long it = st->iter;
+  const double rz = st->rz[it & 1];
+  const double pq = st->aux[0];
+  const double alpha = rz / pq;
+  double v[3] = {0, 0, 0}, tot[3];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double xi = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    const double zi = ri * dinv[i];
+    x[i] = xi;
+    r[i] = ri;
+    z[i] = zi;
+    v[0] += ri * zi;
+    v[1] += ri * ri;
+    v[2] += xi * xi;
+  }
+  if (block_reduce_last<3>(v, part, &st->counter[2], tot) && threadIdx.x == 0) {
+    const long it1 = it + 1;
+    st->iter = it1;
+    st->alpha = alpha;
+    st->rz[it1 & 1] = tot[0];
+    st->rr = tot[1];
+    st->xx = tot[2];
+    st->beta = tot[0] / rz;
+    const double nr = sqrt(tot[1]), nbb = sqrt(st->bb);
+    int done = 0;
+    if (!(pq > 0.0) || !(tot[0] == tot[0])) done = 2;
+    else if (st->rule == 1 ? nr <= st->tol * nbb : nr <= st->tol * (st->normA * sqrt(tot[2]) + nbb)) done = 1;
+    else if (it1 >= st->maxit) done = 3;
+    st->done = done;
+  }
+}
+
+// whole PCG in one cooperative launch; same arithmetic as k_pcg_a/k_pcg_b
+// but with the two reductions finished redundantly in every block after a
+// grid barrier.  Every block takes the same decisions, so all leave together.
+template <int BS>
+__global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double *__restrict__ vals,
+                                                       long n, double *x, double *r, double *z,
+                                                       double *p0, double *p1, double *q,
+                                                       const double *__restrict__ dinv,
+                                                       ugk_state *st, double *part)
+{
+  cg::grid_group grid = cg::this_grid();
+  const unsigned G = gridDim.x;
+  double *partA = part, *partB = part + G;
+  long it = st->iter;
+  double rz = st->rz[it & 1];
+  double beta = st->beta;
+  const double nbb = sqrt(st->bb), normA = st->normA, tol = st->tol;
+  const int rule = st->rule;
+  const long maxit = st->maxit;
+  int done = st->done;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double rr = st->rr, xx = st->xx, alpha = 0.0;
+  while (!done) {
+    const double *pold = (it & 1) ? p1 : p0;
+    double *pnew = (it & 1) ? p0 : p1;
+    double v[1] = {0.0};
+    for (long sl = (long)blockIdx.x * NW + wave; sl < M.n_slices; sl += (long)G * NW) {
+      const int L = M.slen[sl];
+      const long so = M.soff[sl];
+      const int *cp = M.cols + so * 64 + lane;
+      const double *vp = vals + so * 64 * BS * BS + lane;
+      double acc[BS];
+#pragma unroll
+      for (int rr2 = 0; rr2 < BS; rr2++) acc[rr2] = 0.0;
+#pragma unroll 3
+      for (int s = 0; s < L; s++) {
+        const long J = cp[(long)s * 64];
+        double xv[BS];
+#pragma unroll
+        for (int c = 0; c < BS; c++) xv[c] = z[J * BS + c] + beta * pold[J * BS + c];
+        const double *vv = vp + (long)s * 64 * BS * BS;
+#pragma unroll
+        for (int r2 = 0; r2 < BS; r2++)
+#pragma unroll
+          for (int c = 0; c < BS; c++) acc[r2] += vv[(r2 * BS + c) * 64] * xv[c];
+      }
+      const long row = sl * 64 + lane;
+      if (row < M.n_brows) {
+#pragma unroll
+        for (int r2 = 0; r2 < BS; r2++) {
+          const long i = row * BS + r2;
+          const double pn = z[i] + beta * pold[i];
+          pnew[i] = pn;
+          q[i] = acc[r2];
+          v[0] += pn * acc[r2];
+        }
+      }
+    }
+    block_publish<1>(v, partA);
+    grid.sync();
+    double tA[1];
+    grid_partials_sum<1>(partA, G, tA);
+    const double pq = tA[0];
+    alpha = rz / pq;
+    double w[3] = {0, 0, 0};
+    for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)G * WG) {
+      const double xi = x[i] + alpha * pnew[i];
+      const double ri = r[i] - alpha * q[i];
+      const double zi = ri * dinv[i];
+      x[i] = xi;
+      r[i] = ri;
+      z[i] = zi;
+      w[0] += ri * zi;
+      w[1] += ri * ri;
+      w[2] += xi * xi;
+    }
+    block_publish<3>(w, partB);
+    grid.sync();
+    double tB[3];
+    grid_partials_sum<3>(partB, G, tB);
+    it++;
+    rr = tB[1];
+    xx = tB[2];
+    beta = tB[0] / rz;
+    const double nr = sqrt(rr);
+    if (!(pq > 0.0) || !(tB[0] == tB[0])) done = 2;
+    else if (rule == 1 ? nr <= tol * nbb : nr <= tol * (normA * sqrt(xx) + nbb)) done = 1;
+    else if (it >= maxit) done = 3;
+    rz = tB[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rz[it & 1] = rz;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter = it;
+    st->rr = rr;
+    st->xx = xx;
+    st->beta = beta;
+    st->alpha = alpha;
+    st->done = done;
+  }
+}
+
+// --------------------------------------------------------------------------
+// FP32 inner PCG and FP64 iterative refinement
+// --------------------------------------------------------------------------
+template <int BS>
+__global__ __launch_bounds__(WG) void k_pcg32_start(long n, const double *__restrict__ r64, double scale,
+                                                    float *r, float *z, float *p, float *x,
+                                                    const float *__restrict__ dinv, ugk_state *st,
+                                                    double *part)
+{
+  double v[2] = {0, 0}, tot[2];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const float ri = (float)(r64[i] * scale);
+    const float zi = ri * dinv[i];
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = 0.0f;
+    x[i] = 0.0f;
+    v[0] += (double)ri * (double)zi;
+    v[1] += (double)ri * (double)ri;
+  }
+  if (block_reduce_last<2>(v, part, &st->counter[3], tot) && threadIdx.x == 0) {
+    st->rz[0] = tot[0];
+    st->rr = tot[1];
+    st->bb = tot[1];
+    st->beta = 0.0;
+    st->iter = 0;
+    st->done = (tot[1] > 0.0) ? (st->maxit > 0 ? 0 : 3) : 1;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(WG) void k_pcg32_a(SellView M, const float *__restrict__ vals,
+                                                const float *__restrict__ z,
+                                                const float *__restrict__ pold,
+                                                float *__restrict__ pnew, float *__restrict__ q,
+                                                ugk_state *st, double *part)
+{
+  if (st->done) return;
+  const float beta = (float)st->beta;
+  const long nb = gridDim.x;
+  const long lb = remap_block(blockIdx.x, nb);
+  const long sl = lb * NW + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double v[1] = {0.0}, tot[1];
+  if (lb < nb && sl < M.n_slices) {
+    const int L = M.slen[sl];
+    const long so = M.soff[sl];
+    const int *cp = M.cols + so * 64 + lane;
+    const float *vp = vals + so * 64 * BS * BS + lane;
+    float acc[BS];
+#pragma unroll
+    for (int r = 0; r < BS; r++) acc[r] = 0.0f;
+#pragma unroll 3
+    for (int s = 0; s < L; s++) {
+      const long J = cp[(long)s * 64];
+      float xv[BS];
+#pragma unroll
+      for (int c = 0; c < BS; c++) xv[c] = z[J * BS + c] + beta * pold[J * BS + c];
+      const float *vv = vp + (long)s * 64 * BS * BS;
+#pragma unroll
+      for (int r = 0; r < BS; r++)
+#pragma unroll
+        for (int c = 0; c < BS; c++) acc[r] += vv[(r * BS + c) * 64] * xv[c];
+    }
+    const long row = sl * 64 + lane;
+    if (row < M.n_brows) {
+#pragma unroll
+      for (int r = 0; r < BS; r++) {
+        const long i = row * BS + r;
+        const float pn = z[i] + beta * pold[i];
+        pnew[i] = pn;
+        q[i] = acc[r];
+        v[0] += (double)pn * (double)acc[r];
+      }
+    }
+  }
+  if (block_reduce_last<1>(v, part, &st->counter[1], tot) && threadIdx.x == 0) st->aux[0] = tot[0];
+}
+
+__global__ __launch_bounds__(WG) void k_pcg32_b(long n, float *__restrict__ x, float *__restrict__ r,
+                                                float *__restrict__ z, const float *__restrict__ p,
+                                                const float *__restrict__ q,
+                                                const float *__restrict__ dinv, ugk_state *st,
+                                                double *part)
+{
+  if (st->done) return;
+  const long it = st->iter;
+  const double rz = st->rz[it & 1];
+  const double pq = st->aux[0];
+  const float alpha = (float)(rz / pq);
+  double v[2] = {0, 0}, tot[2];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const float xi = x[i] + alpha * p[i];
+    const float ri = r[i] - alpha * q[i];
+    const float zi = ri * dinv[i];
+    x[i] = xi;
+    r[i] = ri;
+    z[i] = zi;
+    v[0] += (double)ri * (double)zi;
+    v[1] += (double)ri * (double)ri;
+  }
+  if (block_reduce_last<2>(v, part, &st->counter[2], tot) && threadIdx.x == 0) {
+    const long it1 = it + 1;
+    st->iter = it1;
+    st->rz[it1 & 1] = tot[0];
+    st->rr = tot[1];
+    st->beta = tot[0] / rz;
+    int done = 0;
+    if (!(pq > 0.0) || !(tot[0] == tot[0])) done = 2;
+    else if (sqrt(tot[1]) <= st->tol * sqrt(st->bb)) done = 1;
+    else if (it1 >= st->maxit) done = 3;
+    st->done = done;
+  }
+}
+
+// x64 += scale * x32, then r64 = b - A x64 with ||r64||^2 -> st->rr
+template <int BS>
+__global__ void k_ir_axpy(long n, double *x64, const float *__restrict__ x32, double scale)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x64[i] += scale * (double)x32[i];
+}
+
+template <int BS>
+__global__ __launch_bounds__(WG) void k_residual(SellView M, const double *__restrict__ vals,
+                                                 const double *__restrict__ b,
+                                                 const double *__restrict__ x, double *r,
+                                                 ugk_state *st, double *part)
+{
+  const long nb = gridDim.x;
+  const long lb = remap_block(blockIdx.x, nb);
+  const long sl = lb * NW + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double v[1] = {0.0}, tot[1];
+  if (lb < nb && sl < M.n_slices) {
+    double acc[BS];
+    sell_row<BS, double, double>(M, vals, sl, lane, x, acc);
+    const long row = sl * 64 + lane;
+    if (row < M.n_brows) {
+#pragma unroll
+      for (int k = 0; k < BS; k++) {
+        const long i = row * BS + k;
+        const double ri = b[i] - acc[k];
+        r[i] = ri;
+        v[0] += ri * ri;
+      }
+    }
+  }
+  if (block_reduce_last<1>(v, part, &st->counter[4], tot) && threadIdx.x == 0) st->rr = tot[0];
+}
+
+// --------------------------------------------------------------------------
+// BiCGStab (Jacobi, right preconditioned), five kernels per iteration
+// --------------------------------------------------------------------------
+// aux[0] = rho (rhat, r) of the current iteration, aux[1] = (rhat, v),
+// aux[2] = ||s||^2, aux[3] = (t, s), aux[4] = (t, t), aux[5] = s-converged
+__global__ void k_bicgs_init(long n, const double *__restrict__ b, double *r, double *rhat,
+                             double *p, double *v, ugk_state *st, double *part)
+{
+  double w[2] = {0, 0}, tot[2];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double ri = r[i];   // r holds A x on entry
+    const double rv = b[i] - ri;
+    r[i] = rv;
+    rhat[i] = rv;
+    p[i] = 0.0;
+    v[i] = 0.0;
+    w[0] += rv * rv;
+    w[1] += b[i] * b[i];
+  }
+  if (block_reduce_last<2>(w, part, &st->counter[0], tot) && threadIdx.x == 0) {
+    st->rr = tot[0];
+    st->bb = tot[1];
+    st->aux[0] = tot[0];   // rho = (rhat, r) = ||r||^2
+    st->rho = 1.0;
+    st->alpha = 1.0;
+    st->omega = 1.0;
+    st->iter = 0;
+    const double nb = sqrt(tot[1]);
+    int done = 0;
+    if (!(nb > 0.0)) done = 1;
+    else if (sqrt(tot[0]) <= st->tol * nb) done = 1;
+    else if (st->maxit <= 0) done = 3;
+    st->done = done;
+  }
+}
+
+__global__ void k_bicgs_p(long n, const double *__restrict__ r, double *p, const double *__restrict__ v,
+                          double *phat, const double *__restrict__ dinv, ugk_state *st)
+{
+  if (st->done) return;
+  const double rho = st->aux[0];
+  const double beta = (rho / st->rho) * (st->alpha / st->omega);
+  const double om = st->omega;
+  const bool first = st->iter == 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double pi = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
+    p[i] = pi;
+    phat[i] = pi * dinv[i];
+  }
+}
+
+// y = A x and partial dots (y, w1) [and (y, y)] -> aux[slot], aux[slot+1]
+template <int BS, int ND>
+__global__ __launch_bounds__(WG) void k_spmv_dots(SellView M, const double *__restrict__ vals,
+                                                  const double *__restrict__ x, double *y,
+                                                  const double *__restrict__ w1, ugk_state *st,
+                                                  double *part, int slot, int ctr)
+{
+  if (st->done) return;
+  if (ND == 2 && st->aux[5] != 0.0) return;   // s already converged
+  const long nb = gridDim.x;
+  const long lb = remap_block(blockIdx.x, nb);
+  const long sl = lb * NW + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double v[ND], tot[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) v[k] = 0.0;
+  if (lb < nb && sl < M.n_slices) {
+    double acc[BS];
+    sell_row<BS, double, double>(M, vals, sl, lane, x, acc);
+    const long row = sl * 64 + lane;
+    if (row < M.n_brows) {
+#pragma unroll
+      for (int k = 0; k < BS; k++) {
+        const long i = row * BS + k;
+        y[i] = acc[k];
+        v[0] += acc[k] * w1[i];
+        if (ND == 2) v[1] += acc[k] * acc[k];
+      }
+    }
+  }
+  if (block_reduce_last<ND>(v, part, &st->counter[ctr], tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < ND; k++) st->aux[slot + k] = tot[k];
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_bicgs_s(long n, const double *__restrict__ r,
+                                                const double *__restrict__ v, double *s,
+                                                double *shat, const double *__restrict__ dinv,
+                                                ugk_state *st, double *part)
+{
+  if (st->done) return;
+  const double alpha = st->aux[0] / st->aux[1];
+  double w[1] = {0.0}, tot[1];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double si = r[i] - alpha * v[i];
+    s[i] = si;
+    shat[i] = si * dinv[i];
+    w[0] += si * si;
+  }
+  if (block_reduce_last<1>(w, part, &st->counter[2], tot) && threadIdx.x == 0) {
+    st->aux[2] = tot[0];
+    st->alpha = alpha;
+    st->aux[5] = (sqrt(tot[0]) <= st->tol * sqrt(st->bb) || !(st->aux[1] != 0.0)) ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_bicgs_x(long n, double *x, double *r,
+                                                const double *__restrict__ rhat,
+                                                const double *__restrict__ phat,
+                                                const double *__restrict__ s,
+                                                const double *__restrict__ shat,
+                                                const double *__restrict__ t, ugk_state *st,
+                                                double *part)
+{
+  if (st->done) return;
+  const bool sconv = st->aux[5] != 0.0;
+  const double alpha = st->alpha;
+  const double omega = sconv ? 0.0 : st->aux[3] / st->aux[4];
+  double w[2] = {0, 0}, tot[2];
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double xi = x[i] + alpha * phat[i] + omega * shat[i];
+    const double ri = sconv ? s[i] : s[i] - omega * t[i];
+    x[i] = xi;
+    r[i] = ri;
+    w[0] += rhat[i] * ri;
+    w[1] += ri * ri;
+  }
+  if (block_reduce_last<2>(w, part, &st->counter[5], tot) && threadIdx.x == 0) {
+    const long it1 = st->iter + 1;
+    st->iter = it1;
+    st->rho = st->aux[0];
+    st->aux[0] = tot[0];
+    st->omega = omega;
+    st->rr = tot[1];
+    st->aux[5] = 0.0;
+    const double nr = sqrt(tot[1]), nbb = sqrt(st->bb);
+    int done = 0;
+    if (!(tot[1] == tot[1])) done = 2;
+    else if (sconv || nr <= st->tol * nbb) done = 1;
+    else if (!(omega != 0.0) || !(tot[0] != 0.0)) done = 2;
+    else if (it1 >= st->maxit) done = 3;
+    st->done = done;
+  }
+}
+
+// ==========================================================================
+// C-ABI
+// ==========================================================================
+static SellView view(const ugk_mat *K)
+{
+  SellView M;
+  M.slen = K->d_slen;
+  M.soff = K->d_soff;
+  M.rowlen = K->d_rowlen;
+  M.cols = K->d_cols;
+  M.n_brows = K->n_brows;
+  M.n_slices = K->n_slices;
+  return M;
+}
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+// blocks for one slice per wavefront, rounded up to a multiple of the 8 XCDs
+// so that remap_block() is a bijection on [0, gridDim.x)
+static inline unsigned spmv_blocks(const ugk_mat *K)
+{
+  long b = (K->n_slices + NW - 1) / NW;
+  return (unsigned)((b + 7) / 8 * 8);
+}
+static int g_ncu = 256;
+static inline unsigned vec_blocks(long n)
+{
+  long b = (n + WG - 1) / WG;
+  long cap = 4L * g_ncu;
+  return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+static const unsigned RED_MAX_BLOCKS = UGK_RED_MAX_BLOCKS;
+
+extern "C" {
+
+int ugk_init(void)
+{
+  int dev = 0, n = 0;
+  CK(hipGetDeviceCount(&n));
+  if (n < 1) { snprintf(g_err, sizeof(g_err), "no HIP device"); return -1; }
+  CK(hipGetDevice(&dev));
+  hipDeviceProp_t pr;
+  CK(hipGetDeviceProperties(&pr, dev));
+  g_ncu = pr.multiProcessorCount;
+  return 0;
+}
+
+const char *ugk_error(void) { return g_err; }
+
+void *ugk_malloc(size_t bytes)
+{
+  void *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) { fail("hipMalloc", e); return nullptr; }
+  e = hipMemset(p, 0, bytes);
+  if (e != hipSuccess) { fail("hipMemset", e); (void)hipFree(p); return nullptr; }
+  return p;
+}
+void ugk_free(void *p)
+{
+  if (p) (void)hipFree(p);
+}
+int ugk_h2d(void *d, const void *h, size_t bytes, void *s)
+{
+  CK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, S(s)));
+  return 0;
+}
+int ugk_d2h(void *h, const void *d, size_t bytes, void *s)
+{
+  CK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, S(s)));
+  CK(hipStreamSynchronize(S(s)));
+  return 0;
+}
+int ugk_d2d(void *d, const void *src, size_t bytes, void *s)
+{
+  CK(hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, S(s)));
+  return 0;
+}
+int ugk_zero(void *d, size_t bytes, void *s)
+{
+  long w = (long)(bytes / 4);
+  if (w == 0) return 0;
+  k_fill_zero<<<vec_blocks(w), WG, 0, S(s)>>>(w, (unsigned int *)d);
+  CK_LAUNCH("k_fill_zero");
+  return 0;
+}
+void *ugk_stream_create(void)
+{
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  return (void *)s;
+}
+void ugk_stream_destroy(void *s)
+{
+  if (s) (void)hipStreamDestroy(S(s));
+}
+int ugk_sync(void *s)
+{
+  CK(hipStreamSynchronize(S(s)));
+  return 0;
+}
+void *ugk_event_create(void)
+{
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return (void *)e;
+}
+void ugk_event_destroy(void *e)
+{
+  if (e) (void)hipEventDestroy((hipEvent_t)e);
+}
+int ugk_event_record(void *e, void *s)
+{
+  CK(hipEventRecord((hipEvent_t)e, S(s)));
+  return 0;
+}
+float ugk_event_ms(void *e0, void *e1)
+{
+  float ms = -1.0f;
+  if (hipEventSynchronize((hipEvent_t)e1) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&ms, (hipEvent_t)e0, (hipEvent_t)e1) != hipSuccess) return -1.0f;
+  return ms;
+}
+int ugk_device_info(int *n_cu, long *hbm, char *name, int name_len)
+{
+  int dev = 0;
+  hipDeviceProp_t pr;
+  CK(hipGetDevice(&dev));
+  CK(hipGetDeviceProperties(&pr, dev));
+  if (n_cu) *n_cu = pr.multiProcessorCount;
+  if (hbm) *hbm = (long)pr.totalGlobalMem;
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s (%s)", pr.name, pr.gcnArchName);
+  return 0;
+}
+
+void *ugk_host_alloc(size_t bytes)
+{
+  void *p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) { fail("hipHostMalloc", e); return nullptr; }
+  memset(p, 0, bytes);
+  return p;
+}
+void ugk_host_free(void *p)
+{
+  if (p) (void)hipHostFree(p);
+}
+int ugk_d2h_async(void *h, const void *d, size_t bytes, void *s)
+{
+  CK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, S(s)));
+  return 0;
+}
+int ugk_event_sync(void *e)
+{
+  CK(hipEventSynchronize((hipEvent_t)e));
+  return 0;
+}
+
+int ugk_graph_begin(void *s)
+{
+  CK(hipStreamBeginCapture(S(s), hipStreamCaptureModeThreadLocal));
+  return 0;
+}
+void *ugk_graph_end(void *s)
+{
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipStreamEndCapture(S(s), &g);
+  if (e != hipSuccess) { fail("hipStreamEndCapture", e); return nullptr; }
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) { fail("hipGraphInstantiate", e); return nullptr; }
+  return (void *)ge;
+}
+int ugk_graph_launch(void *g, void *s)
+{
+  CK(hipGraphLaunch((hipGraphExec_t)g, S(s)));
+  return 0;
+}
+void ugk_graph_destroy(void *g)
+{
+  if (g) (void)hipGraphExecDestroy((hipGraphExec_t)g);
+}
+
+// ---- assembly -------------------------------------------------------------
+int ugk_elem_stiff_dense(long n_elems, const double *d_xyz, const int *d_enodes, const double *d_E,
+                         const double *d_nu, int npts, double *d_Ke, ugk_state *d_st, void *s)
+{
+  (void)d_enodes;
+  if (n_elems <= 0) return 0;
+  SellView M{};
+  unsigned nb = (unsigned)((n_elems + NW - 1) / NW);
+  if (npts == 2)
+    k_elem_stiff<2, true><<<nb, WG, 0, S(s)>>>(n_elems, nullptr, d_xyz, nullptr, nullptr, nullptr, d_E,
+                                               d_nu, d_Ke, M, nullptr, d_st);
+  else if (npts == 3)
+    k_elem_stiff<3, true><<<nb, WG, 0, S(s)>>>(n_elems, nullptr, d_xyz, nullptr, nullptr, nullptr, d_E,
+                                               d_nu, d_Ke, M, nullptr, d_st);
+  else { snprintf(g_err, sizeof(g_err), "npts must be 2 or 3"); return -1; }
+  CK_LAUNCH("k_elem_stiff dense");
+  return 0;
+}
+
+int ugk_elem_stiff_scatter(const ugk_mat *K, long n_list, const int *d_elist, const double *d_x,
+                           const double *d_y, const double *d_z, const int *d_enodes,
+                           const double *d_E, const double *d_nu, int npts, ugk_state *d_st, void *s)
+{
+  if (n_list <= 0) return 0;
+  if (K->bs != 3) { snprintf(g_err, sizeof(g_err), "scatter needs 3x3 blocks"); return -1; }
+  unsigned nb = (unsigned)((n_list + NW - 1) / NW);
+  if (npts == 2)
+    k_elem_stiff<2, false><<<nb, WG, 0, S(s)>>>(n_list, d_elist, d_x, d_y, d_z, d_enodes, d_E, d_nu,
+                                                nullptr, view(K), K->d_vals, d_st);
+  else if (npts == 3)
+    k_elem_stiff<3, false><<<nb, WG, 0, S(s)>>>(n_list, d_elist, d_x, d_y, d_z, d_enodes, d_E, d_nu,
+                                                nullptr, view(K), K->d_vals, d_st);
+  else { snprintf(g_err, sizeof(g_err), "npts must be 2 or 3"); return -1; }
+  CK_LAUNCH("k_elem_stiff scatter");
+  return 0;
+}
+
+int ugk_save_rows(const ugk_mat *K, long n_fixed, const int *d_fixed, int maxb, double *d_rows,
+                  int *d_rcols, void *s)
+{
+  if (n_fixed <= 0) return 0;
+  k_save_rows<<<(unsigned)((n_fixed + 127) / 128), 128, 0, S(s)>>>(view(K), K->d_vals, n_fixed, d_fixed,
+                                                                    maxb, d_rows, d_rcols);
+  CK_LAUNCH("k_save_rows");
+  return 0;
+}
+
+int ugk_apply_zero_bc(const ugk_mat *K, long n_fixed, const int *d_fixed, void *s)
+{
+  if (n_fixed <= 0) return 0;
+  unsigned nb = (unsigned)((n_fixed + 127) / 128);
+  k_zero_rowcol<<<nb, 128, 0, S(s)>>>(view(K), K->d_vals, n_fixed, d_fixed);
+  CK_LAUNCH("k_zero_rowcol");
+  k_set_diag_one<<<nb, 128, 0, S(s)>>>(view(K), K->d_vals, n_fixed, d_fixed);
+  CK_LAUNCH("k_set_diag_one");
+  return 0;
+}
+
+int ugk_reactions(const ugk_mat *K, long n_fixed, const int *d_fixed, int maxb, const double *d_rows,
+                  const int *d_rcols, const double *d_u, const double *d_Forig, double *d_R, void *s)
+{
+  (void)K;
+  if (n_fixed <= 0) return 0;
+  k_reactions<<<(unsigned)((n_fixed + 127) / 128), 128, 0, S(s)>>>(n_fixed, d_fixed, maxb, d_rows, d_rcols,
+                                                                     d_u, d_Forig, d_R);
+  CK_LAUNCH("k_reactions");
+  return 0;
+}
+
+int ugk_vals_to_f32(const ugk_mat *K, void *s)
+{
+  long n = K->total_slots * 64L * K->bs * K->bs;
+  if (!K->d_vals32) { snprintf(g_err, sizeof(g_err), "no fp32 buffer"); return -1; }
+  k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_vals32);
+  CK_LAUNCH("k_vals_f32");
+  return 0;
+}
+
+// ---- sparse ---------------------------------------------------------------
+int ugk_spmv(const ugk_mat *K, const double *d_x, double *d_y, void *s)
+{
+  if (K->bs == 3) k_spmv<3, double, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, d_x, d_y);
+  else k_spmv<1, double, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, d_x, d_y);
+  CK_LAUNCH("k_spmv");
+  return 0;
+}
+int ugk_spmv_f32(const ugk_mat *K, const float *d_x, float *d_y, void *s)
+{
+  if (K->bs == 3) k_spmv<3, float, float><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, d_x, d_y);
+  else k_spmv<1, float, float><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, d_x, d_y);
+  CK_LAUNCH("k_spmv_f32");
+  return 0;
+}
+int ugk_diag_inv(const ugk_mat *K, double *d_dinv, ugk_state *d_st, void *s)
+{
+  long n = K->n_brows * K->bs;
+  unsigned nb = (unsigned)((n + WG - 1) / WG);
+  if (K->bs == 3) k_diag_inv<3><<<nb, WG, 0, S(s)>>>(view(K), K->d_vals, d_dinv, d_st);
+  else k_diag_inv<1><<<nb, WG, 0, S(s)>>>(view(K), K->d_vals, d_dinv, d_st);
+  CK_LAUNCH("k_diag_inv");
+  return 0;
+}
+int ugk_mat_norm(const ugk_mat *K, ugk_state *d_st, double *d_part, void *s)
+{
+  long n = K->n_brows * K->bs;
+  if (K->bs == 3) k_mat_norm<3><<<vec_blocks(n), WG, 0, S(s)>>>(view(K), K->d_vals, d_st, d_part);
+  else k_mat_norm<1><<<vec_blocks(n), WG, 0, S(s)>>>(view(K), K->d_vals, d_st, d_part);
+  CK_LAUNCH("k_mat_norm");
+  return 0;
+}
+
+int ugk_dot(long n, const double *a, const double *b, double *out, double *part, void *s)
+{
+  // the counter lives after the partials (part holds UGK_PART_DOUBLES)
+  unsigned int *ctr = (unsigned int *)(part + 8L * RED_MAX_BLOCKS);
+  k_dot<<<vec_blocks(n), WG, 0, S(s)>>>(n, a, b, out, part, ctr);
+  CK_LAUNCH("k_dot");
+  return 0;
+}
+int ugk_axpby(long n, double a, const double *x, double b, const double *y, double *z, void *s)
+{
+  k_axpby<<<vec_blocks(n), WG, 0, S(s)>>>(n, a, x, b, y, z);
+  CK_LAUNCH("k_axpby");
+  return 0;
+}
+
+// ---- PCG ------------------------------------------------------------------
+int ugk_pcg_start(const ugk_mat *K, long n, const double *b, const double *x, double *r, double *z,
+                  double *p, const double *dinv, ugk_state *st, double *part, void *s)
+{
+  (void)n;
+  if (K->bs == 3)
+    k_pcg_start<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, z, p, dinv, st, part);
+  else
+    k_pcg_start<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, z, p, dinv, st, part);
+  CK_LAUNCH("k_pcg_start");
+  return 0;
+}
+
+int ugk_pcg_iter(const ugk_mat *K, long n, double *x, double *r, double *z, double **p, double *q,
+                 const double *dinv, ugk_state *st, double *part, long it, void *s)
+{
+  double *pold = p[it & 1], *pnew = p[(it + 1) & 1];
+  if (K->bs == 3)
+    k_pcg_a<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
+  else
+    k_pcg_a<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
+  CK_LAUNCH("k_pcg_a");
+  k_pcg_b<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, z, pnew, q, dinv, st, part);
+  CK_LAUNCH("k_pcg_b");
+  return 0;
+}
+
+int ugk_pcg_persistent(const ugk_mat *K, long n, double *x, double *r, double *z, double *p0,
+                       double *p1, double *q, const double *dinv, ugk_state *st, double *part,
+                       void *s)
+{
+  int per_cu = 0;
+  const void *fn = K->bs == 3 ? (const void *)k_pcg_persistent<3> : (const void *)k_pcg_persistent<1>;
+  CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, 0));
+  if (per_cu < 1) { snprintf(g_err, sizeof(g_err), "persistent kernel does not fit"); return -1; }
+  if (per_cu > 4) per_cu = 4;
+  unsigned G = (unsigned)(per_cu * g_ncu);
+  SellView M = view(K);
+  const double *vals = K->d_vals;
+  void *args[] = {(void *)&M, (void *)&vals, (void *)&n, (void *)&x, (void *)&r, (void *)&z,
+                  (void *)&p0, (void *)&p1, (void *)&q, (void *)&dinv, (void *)&st, (void *)&part};
+  CK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(WG), args, 0, S(s)));
+  return 0;
+}
+
+// ---- FP32 inner PCG + refinement -------------------------------------------
+int ugk_pcg32_start(const ugk_mat *K, long n, const double *r64, double scale, float *r, float *z,
+                    float *p, float *x, const float *dinv32, ugk_state *st, double *part, void *s)
+{
+  (void)K;
+  k_pcg32_start<3><<<vec_blocks(n), WG, 0, S(s)>>>(n, r64, scale, r, z, p, x, dinv32, st, part);
+  CK_LAUNCH("k_pcg32_start");
+  return 0;
+}
+int ugk_pcg32_iter(const ugk_mat *K, long n, float *x, float *r, float *z, float **p, float *q,
+                   const float *dinv32, ugk_state *st, double *part, long it, void *s)
+{
+  float *pold = p[it & 1], *pnew = p[(it + 1) & 1];
+  if (K->bs == 3)
+    k_pcg32_a<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, z, pold, pnew, q, st, part);
+  else
+    k_pcg32_a<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, z, pold, pnew, q, st, part);
+  CK_LAUNCH("k_pcg32_a");
+  k_pcg32_b<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, z, pnew, q, dinv32, st, part);
+  CK_LAUNCH("k_pcg32_b");
+  return 0;
+}
+int ugk_ir_update(const ugk_mat *K, long n, double *x64, const float *x32, double scale, const double *b,
+                  double *r64, ugk_state *st, double *part, void *s)
+{
+  if (x32) {
+    k_ir_axpy<3><<<vec_blocks(n), WG, 0, S(s)>>>(n, x64, x32, scale);
+    CK_LAUNCH("k_ir_axpy");
+  }
+  if (K->bs == 3)
+    k_residual<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, st, part);
+  else
+    k_residual<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, st, part);
+  CK_LAUNCH("k_residual");
+  return 0;
+}
+int ugk_to_f32(long n, const double *a, float *b, void *s)
+{
+  k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, a, b);
+  CK_LAUNCH("k_to_f32");
+  return 0;
+}
+
+// ---- BiCGStab ---------------------------------------------------------------
+int ugk_bicgs_start(const ugk_mat *K, long n, const double *b, const double *x, double *r, double *rhat,
+                    double *p, double *v, ugk_state *st, double *part, void *s)
+{
+  if (ugk_spmv(K, x, r, s)) return -1;
+  k_bicgs_init<<<vec_blocks(n), WG, 0, S(s)>>>(n, b, r, rhat, p, v, st, part);
+  CK_LAUNCH("k_bicgs_init");
+  return 0;
+}
+int ugk_bicgs_iter(const ugk_mat *K, long n, double *x, double *r, const double *rhat, double *p,
+                   double *phat, double *v, double *sv, double *shat, double *t, const double *dinv,
+                   ugk_state *st, double *part, void *s)
+{
+  k_bicgs_p<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, p, v, phat, dinv, st);
+  CK_LAUNCH("k_bicgs_p");
+  if (K->bs == 3)
+    k_spmv_dots<3, 1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, phat, v, rhat, st, part, 1, 1);
+  else
+    k_spmv_dots<1, 1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, phat, v, rhat, st, part, 1, 1);
+  CK_LAUNCH("k_spmv_dots v");
+  k_bicgs_s<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, v, sv, shat, dinv, st, part);
+  CK_LAUNCH("k_bicgs_s");
+  if (K->bs == 3)
+    k_spmv_dots<3, 2><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, shat, t, sv, st, part, 3, 3);
+  else
+    k_spmv_dots<1, 2><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, shat, t, sv, st, part, 3, 3);
+  CK_LAUNCH("k_spmv_dots t");
+  k_bicgs_x<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, rhat, phat, sv, shat, t, st, part);
+  CK_LAUNCH("k_bicgs_x");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+"""ctypes binding of libufem_gpu.so (the hand-written HIP path).
+
+The library must be built (`python -m ufem_amd.build`); loading fails loudly
+otherwise -- there is no CPU fallback on the product path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libufem_gpu.so")
+
+P_D = C.POINTER(C.c_double)
+P_L = C.POINTER(C.c_long)
+P_I = C.POINTER(C.c_int)
+
+AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
+SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3}
+EXEC = {"streamed": 0, "persistent": 1}
+
+
+class tMatrix(C.Structure):
+    """uFEM's matrix struct, fem/fem_math.h:47-57."""
+    _fields_ = [("type", C.c_long), ("rows", C.c_long), ("cols", C.c_long), ("len", C.c_long),
+                ("pos", P_L), ("data", P_D), ("frompos", P_L), ("defpos", P_L)]
+
+
+class tVector(C.Structure):
+    """uFEM's vector struct, fem/fem_math.h:60-67."""
+    _fields_ = [("type", C.c_long), ("rows", C.c_long), ("len", C.c_long), ("pos", P_L),
+                ("data", P_D)]
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("n_nodes", C.c_long), ("x", P_D), ("y", P_D), ("z", P_D),
+                ("n_elems", C.c_long), ("enodes", P_L), ("E", P_D), ("nu", P_D),
+                ("n_nl", C.c_long), ("nl_node", P_L), ("nl_type", P_L), ("nl_dir", P_L),
+                ("nl_val", P_D)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("solver", C.c_int), ("rule", C.c_int), ("eps", C.c_double), ("maxit", C.c_long),
+                ("exec", C.c_int), ("chunk", C.c_int), ("inner_eps", C.c_double),
+                ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_long), ("outer", C.c_long), ("fallback", C.c_int),
+                ("relres", C.c_double), ("true_relres", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run python -m ufem_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "ufem_sys_create": (C.c_void_p, [C.POINTER(_Mesh)]),
+            "ufem_sys_destroy": (None, [C.c_void_p]),
+            "ufem_sys_ndof": (C.c_long, [C.c_void_p]),
+            "ufem_sys_nnz": (C.c_long, [C.c_void_p]),
+            "ufem_sys_ncolors": (C.c_long, [C.c_void_p]),
+            "ufem_sys_nreact": (C.c_long, [C.c_void_p]),
+            "ufem_sys_fill_K": (C.c_int, [C.c_void_p, C.c_int]),
+            "ufem_sys_add_loads": (C.c_int, [C.c_void_p]),
+            "ufem_sys_add_disps": (C.c_int, [C.c_void_p]),
+            "ufem_sys_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+            "ufem_sys_reactions": (C.c_int, [C.c_void_p, P_D, P_L, P_L]),
+            "ufem_sys_get_u": (C.c_int, [C.c_void_p, P_D]),
+            "ufem_sys_set_u": (C.c_int, [C.c_void_p, P_D]),
+            "ufem_sys_get_F": (C.c_int, [C.c_void_p, P_D]),
+            "ufem_sys_get_K": (C.c_int, [C.c_void_p, P_L, P_I, P_D]),
+            "ufem_sys_spmv": (C.c_int, [C.c_void_p, P_D, P_D]),
+            "ufem_sys_time_spmv": (C.c_int, [C.c_void_p, C.c_int, C.c_int, P_D, P_D]),
+            "ufem_sys_timings": (None, [C.c_void_p, P_D]),
+            "ufem_gpu_element_stiffness": (C.c_int, [C.c_long, P_D, P_D, P_D, C.c_int, P_D]),
+            "ufem_gpu_set_stop_rule": (None, [C.c_int]),
+            "ufem_gpu_last_solve": (None, [C.POINTER(SolveInfo)]),
+            "ufem_gpu_last_error": (C.c_char_p, []),
+            "ufem_gpu_device_info": (C.c_int, [C.c_char_p, C.c_int, P_I, P_L]),
+            "femEqsCGwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
+                                     C.c_double, C.c_long]),
+            "femEqsBiCCSwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
+                                        C.c_double, C.c_long]),
+            "femMatVecMultBig": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector)]),
+            "femVecVecMultBig": (C.c_double, [C.POINTER(tVector), C.POINTER(tVector)]),
+            "femVecNormBig": (C.c_double, [C.POINTER(tVector)]),
+            "femMatNormBig": (C.c_double, [C.POINTER(tMatrix)]),
+            "femVecLinCombBig": (C.c_int, [C.c_double, C.POINTER(tVector), C.c_double,
+                                           C.POINTER(tVector), C.POINTER(tVector)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().ufem_gpu_last_error().decode()
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    ncu = C.c_int(0)
+    hbm = C.c_long(0)
+    rv = lib().ufem_gpu_device_info(name, 256, C.byref(ncu), C.byref(hbm))
+    if rv != 0:
+        raise RuntimeError(last_error())
+    return {"name": name.value.decode(), "n_cu": ncu.value, "hbm_bytes": hbm.value}
+
+
+def _pd(a):
+    return a.ctypes.data_as(P_D)
+
+
+def _pl(a):
+    return a.ctypes.data_as(P_L)
+
+
+def _pi(a):
+    return a.ctypes.data_as(P_I)
+
+
+class GpuSystem:
+    """A linear-static problem resident on the GPU (K in SELL-64 3x3 blocks,
+    F, u, supports).  Methods follow femSolve's steps."""
+
+    def __init__(self, mesh):
+        self._keep = [np.ascontiguousarray(mesh.coords[:, i], dtype=np.float64) for i in range(3)]
+        self._keep += [np.ascontiguousarray(mesh.enodes.reshape(-1), dtype=np.int64),
+                       np.ascontiguousarray(mesh.E, dtype=np.float64),
+                       np.ascontiguousarray(mesh.nu, dtype=np.float64),
+                       np.ascontiguousarray(mesh.nl_node, dtype=np.int64),
+                       np.ascontiguousarray(mesh.nl_type, dtype=np.int64),
+                       np.ascontiguousarray(mesh.nl_dir, dtype=np.int64),
+                       np.ascontiguousarray(mesh.nl_val, dtype=np.float64)]
+        k = self._keep
+        m = _Mesh(mesh.n_nodes, _pd(k[0]), _pd(k[1]), _pd(k[2]), mesh.n_elems, _pl(k[3]), _pd(k[4]),
+                  _pd(k[5]), k[6].size, _pl(k[6]), _pl(k[7]), _pl(k[8]), _pd(k[9]))
+        self.h = lib().ufem_sys_create(C.byref(m))
+        if not self.h:
+            raise RuntimeError("ufem_sys_create: " + last_error())
+        self.n = lib().ufem_sys_ndof(self.h)
+        self.nreact = lib().ufem_sys_nreact(self.h)
+        self.ncolors = lib().ufem_sys_ncolors(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ufem_sys_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rv, what):
+        if rv != 0:
+            raise RuntimeError(f"{what} failed ({rv}): {last_error()}")
+
+    def fill_K(self, npts=2):
+        self._chk(lib().ufem_sys_fill_K(self.h, npts), "fill_K")
+
+    def add_loads(self):
+        self._chk(lib().ufem_sys_add_loads(self.h), "add_loads")
+
+    def add_disps(self):
+        self._chk(lib().ufem_sys_add_disps(self.h), "add_disps")
+
+    def assemble(self, npts=2):
+        self.fill_K(npts)
+        self.add_loads()
+        self.add_disps()
+
+    def solve(self, solver="pcg", eps=1e-8, rule=1, maxit=0, exec="streamed", chunk=0,
+              inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, check=True):
+        o = SolveOpts(SOLVER[solver], rule, eps, maxit, EXEC[exec], chunk, inner_eps, inner_maxit,
+                      net, 1 if warm_start else 0)
+        info = SolveInfo()
+        rv = lib().ufem_sys_solve(self.h, C.byref(o), C.byref(info))
+        if check and rv not in (AF_OK, AF_ERR):
+            raise RuntimeError(f"solve failed ({rv}): {last_error()}")
+        d = info.as_dict()
+        d["rv"] = rv
+        return d
+
+    def reactions(self):
+        R = np.zeros(self.nreact)
+        rn = np.zeros(self.nreact, dtype=np.int64)
+        rd = np.zeros(self.nreact, dtype=np.int64)
+        self._chk(lib().ufem_sys_reactions(self.h, _pd(R), _pl(rn), _pl(rd)), "reactions")
+        return R, rn, rd
+
+    def u(self):
+        out = np.zeros(self.n)
+        self._chk(lib().ufem_sys_get_u(self.h, _pd(out)), "get_u")
+        return out
+
+    def set_u(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._chk(lib().ufem_sys_set_u(self.h, _pd(u)), "set_u")
+
+    def F(self):
+        out = np.zeros(self.n)
+        self._chk(lib().ufem_sys_get_F(self.h, _pd(out)), "get_F")
+        return out
+
+    def nnz(self):
+        return lib().ufem_sys_nnz(self.h)
+
+    def K_csr(self, values=True):
+        nnz = self.nnz()
+        ip = np.zeros(self.n + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int32)
+        vals = np.zeros(nnz) if values else None
+        self._chk(lib().ufem_sys_get_K(self.h, _pl(ip), _pi(cols), _pd(vals) if values else None), "get_K")
+        return ip, cols.astype(np.int64), vals
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        self._chk(lib().ufem_sys_spmv(self.h, _pd(x), _pd(y)), "spmv")
+        return y
+
+    def time_spmv(self, reps=20, fp32=False):
+        ms = C.c_double(0)
+        by = C.c_double(0)
+        self._chk(lib().ufem_sys_time_spmv(self.h, reps, 1 if fp32 else 0, C.byref(ms), C.byref(by)),
+                  "time_spmv")
+        return ms.value, by.value
+
+    def timings(self):
+        t = np.zeros(3)
+        lib().ufem_sys_timings(self.h, _pd(t))
+        return {"fill_ms": t[0], "bc_ms": t[1], "solve_ms": t[2]}
+
+
+def element_stiffness(xyz, E, nu, npts=2):
+    """Batched element stiffness on the GPU: xyz (n, 8, 3) -> (n, 24, 24)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 24)
+    n = xyz.shape[0]
+    E = np.ascontiguousarray(np.broadcast_to(E, (n,)), dtype=np.float64)
+    nu = np.ascontiguousarray(np.broadcast_to(nu, (n,)), dtype=np.float64)
+    Ke = np.zeros((n, 576))
+    rv = lib().ufem_gpu_element_stiffness(n, _pd(xyz), _pd(E), _pd(nu), npts, _pd(Ke))
+    if rv != 0:
+        raise RuntimeError(f"element stiffness failed ({rv}): {last_error()}")
+    return Ke.reshape(n, 24, 24)
+
+
+class RowSlotMatrix:
+    """A uFEM row-slot matrix (fem/fem_math.h:47-57) owned by numpy arrays,
+    for calling the drop-in entry points exactly as uFEM does."""
+
+    def __init__(self, n, rowlen, cols1, vals, capacity=None):
+        rowlen = np.asarray(rowlen, dtype=np.int64)
+        cap = rowlen if capacity is None else np.asarray(capacity, dtype=np.int64)
+        self.defpos = np.ascontiguousarray(cap)
+        self.frompos = np.zeros(n, dtype=np.int64)
+        np.cumsum(cap[:-1], out=self.frompos[1:])
+        ln = int(cap.sum())
+        self.pos = np.zeros(ln + 1, dtype=np.int64)
+        self.data = np.zeros(ln + 1)
+        src = 0
+        for i in range(n):
+            k = int(rowlen[i])
+            f = int(self.frompos[i])
+            self.pos[f:f + k] = cols1[src:src + k]
+            self.data[f:f + k] = vals[src:src + k]
+            src += k
+        self.s = tMatrix(1, n, n, ln, _pl(self.pos), _pd(self.data), _pl(self.frompos), _pl(self.defpos))
+
+
+class Vec:
+    def __init__(self, data):
+        self.data = np.ascontiguousarray(data, dtype=np.float64).copy()
+        self.s = tVector(0, self.data.size, self.data.size, None, _pd(self.data))
+
+
+def femEqsCGwJ(K: RowSlotMatrix, b: Vec, x: Vec, eps: float, maxit: int) -> int:
+    return lib().femEqsCGwJ(C.byref(K.s), C.byref(b.s), C.byref(x.s), eps, maxit)
+
+
+def femEqsBiCCSwJ(K: RowSlotMatrix, b: Vec, x: Vec, eps: float, maxit: int) -> int:
+    return lib().femEqsBiCCSwJ(C.byref(K.s), C.byref(b.s), C.byref(x.s), eps, maxit)
+
+
+def set_stop_rule(rule: int) -> None:
+    lib().ufem_gpu_set_stop_rule(rule)
+
+
+def last_solve() -> dict:
+    info = SolveInfo()
+    lib().ufem_gpu_last_solve(C.byref(info))
+    return info.as_dict()
