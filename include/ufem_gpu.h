@@ -92,9 +92,11 @@ int  ufem_sys_get_F(ufem_sys *s, double *F);
 /* K as sorted scalar CSR (0-based), explicit zeros included */
 int  ufem_sys_get_K(ufem_sys *s, long *indptr, int *cols, double *vals);
 int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host out */
-/* average device time of `reps` SpMV launches on the system's stream,
- * plus the bytes one launch moves by the algorithmic count */
-int  ufem_sys_time_spmv(ufem_sys *s, int reps, int fp32, double *ms, double *bytes);
+/* average device time (HIP events on the system's stream) of `reps`
+ * launches of one kernel, and the bytes one launch moves by the algorithmic
+ * count.  mode 0: y = K x (FP64); 1: y = K x (FP32 copy of K); 2: the PCG
+ * phase-A kernel (p' = z + beta p, q = K p', (p', q)), which dominates a solve. */
+int  ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *bytes);
 /* phase device times of the last calls: fill_K, add_disps, solve (ms) */
 void ufem_sys_timings(const ufem_sys *s, double t[3]);
 

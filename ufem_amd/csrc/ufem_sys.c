@@ -565,36 +565,51 @@ int ufem_sys_spmv(ufem_sys *s, const double *x, double *y)
   return AF_OK;
 }
 
-/* bytes one SpMV moves by the algorithmic count: every stored value and
- * column index of the SELL slots once, the x entries once, y once */
-static double spmv_bytes(const ugk_mat *K, int fp32)
+/* Algorithmic bytes of one launch, counted on the real (unpadded) blocks:
+ *   mode 0  y = K x, FP64:      (72 + 4) nnzb + 8 n (x) + 8 n (y)
+ *   mode 1  y = K x, FP32:      (36 + 4) nnzb + 4 n + 4 n
+ *   mode 2  PCG phase A, FP64:  (72 + 4) nnzb + 8 n (z) + 8 n (p) + 8 n (p') + 8 n (q)
+ * (n = 3 n_nodes; each vector counted once although its entries are
+ * gathered by up to 27 rows -- those repeats are served by L2/MALL). */
+static double kernel_bytes(const ugk_mat *K, int mode)
 {
-  double slots = (double)K->total_slots * 64.0;
-  double vb = fp32 ? 4.0 : 8.0;
-  double nrow = (double)K->n_brows * K->bs;
-  return slots * (K->bs * K->bs * vb + 4.0) + (double)K->n_slices * (4.0 + 8.0) + 2.0 * nrow * vb;
+  double nnzb = (double)K->nnzb, n = (double)K->n_brows * K->bs, b2 = (double)K->bs * K->bs;
+  if (mode == 1) return nnzb * (b2 * 4.0 + 4.0) + 8.0 * n;
+  if (mode == 2) return nnzb * (b2 * 8.0 + 4.0) + 32.0 * n;
+  return nnzb * (b2 * 8.0 + 4.0) + 16.0 * n;
 }
 
-int ufem_sys_time_spmv(ufem_sys *s, int reps, int fp32, double *ms, double *bytes)
+static int launch_mode(ufem_sys *s, int mode)
+{
+  if (mode == 1) return ugk_spmv_f32(&s->K, s->w.z32, s->w.q32, s->stream);
+  if (mode == 2) return ugk_pcg_iter_a(&s->K, s->w.z, s->w.p[0], s->w.p[1], s->w.q, s->w.st, s->w.part, s->stream);
+  return ugk_spmv(&s->K, s->w.z, s->w.q, s->stream);
+}
+
+int ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *bytes)
 {
   int i;
   float t;
   if (reps < 1) reps = 1;
-  if (fp32 && !s->K.d_vals32) {
+  if (mode == 1 && !s->K.d_vals32) {
     s->K.d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)s->K.total_slots * 64 * 9);
     if (!s->K.d_vals32 || ugk_vals_to_f32(&s->K, s->stream)) return gpu_fail("fp32 copy");
   }
-  /* warm-up */
-  if (fp32 ? ugk_spmv_f32(&s->K, s->w.z32, s->w.q32, s->stream) : ugk_spmv(&s->K, s->w.z, s->w.q, s->stream))
-    return gpu_fail("spmv");
+  if (mode == 2) {   /* a running state so that phase A does its work */
+    ugk_state h;
+    memset(&h, 0, sizeof(h));
+    h.beta = 0.5;
+    h.maxit = 1;
+    if (ugk_h2d(s->w.st, &h, sizeof(h), s->stream)) return gpu_fail("state");
+  }
+  if (launch_mode(s, mode)) return gpu_fail("kernel");   /* warm-up */
   if (ugk_event_record(s->ev[4], s->stream)) return gpu_fail("event");
   for (i = 0; i < reps; i++)
-    if (fp32 ? ugk_spmv_f32(&s->K, s->w.z32, s->w.q32, s->stream) : ugk_spmv(&s->K, s->w.z, s->w.q, s->stream))
-      return gpu_fail("spmv");
+    if (launch_mode(s, mode)) return gpu_fail("kernel");
   if (ugk_event_record(s->ev[5], s->stream)) return gpu_fail("event");
   t = ugk_event_ms(s->ev[4], s->ev[5]);
   if (ms) *ms = (double)t / reps;
-  if (bytes) *bytes = spmv_bytes(&s->K, fp32);
+  if (bytes) *bytes = kernel_bytes(&s->K, mode);
   return AF_OK;
 }
 

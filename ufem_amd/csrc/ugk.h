@@ -130,6 +130,9 @@ int ugk_pcg_start(const ugk_mat *K, long n, const double *d_b, const double *d_x
 int ugk_pcg_iter(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_z,
                  double **d_p /* [2] ping-pong */, double *d_q, const double *d_dinv,
                  ugk_state *d_st, double *d_part, long it, void *stream);
+/* phase A alone (p' = z + beta p, q = A p', (p', q)) -- for timing */
+int ugk_pcg_iter_a(const ugk_mat *K, const double *d_z, const double *d_pold, double *d_pnew,
+                   double *d_q, ugk_state *d_st, double *d_part, void *stream);
 /* whole solve in one cooperative launch (grid barriers between phases) */
 int ugk_pcg_persistent(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_z,
                        double *d_p0, double *d_p1, double *d_q, const double *d_dinv,

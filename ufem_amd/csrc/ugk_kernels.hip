@@ -1393,6 +1393,17 @@ int ugk_pcg_iter(const ugk_mat *K, long n, double *x, double *r, double *z, doub
   return 0;
 }
 
+int ugk_pcg_iter_a(const ugk_mat *K, const double *z, const double *pold, double *pnew, double *q,
+                   ugk_state *st, double *part, void *s)
+{
+  if (K->bs == 3)
+    k_pcg_a<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
+  else
+    k_pcg_a<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
+  CK_LAUNCH("k_pcg_a");
+  return 0;
+}
+
 int ugk_pcg_persistent(const ugk_mat *K, long n, double *x, double *r, double *z, double *p0,
                        double *p1, double *q, const double *dinv, ugk_state *st, double *part,
                        void *s)

@@ -236,11 +236,12 @@ class GpuSystem:
         self._chk(lib().ufem_sys_spmv(self.h, _pd(x), _pd(y)), "spmv")
         return y
 
-    def time_spmv(self, reps=20, fp32=False):
+    def time_kernel(self, mode="spmv", reps=20):
+        """(avg ms, algorithmic bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a'."""
+        m = {"spmv": 0, "spmv32": 1, "pcg_a": 2}[mode]
         ms = C.c_double(0)
         by = C.c_double(0)
-        self._chk(lib().ufem_sys_time_spmv(self.h, reps, 1 if fp32 else 0, C.byref(ms), C.byref(by)),
-                  "time_spmv")
+        self._chk(lib().ufem_sys_time_spmv(self.h, reps, m, C.byref(ms), C.byref(by)), "time_kernel")
         return ms.value, by.value
 
     def timings(self):
