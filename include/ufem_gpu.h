@@ -50,7 +50,13 @@ enum {
   UFEM_SOLVER_LEARNED = 3      /* flexible CG with the learned preconditioner */
 };
 enum { UFEM_RULE_REFERENCE = 0, UFEM_RULE_RELATIVE = 1 };
-enum { UFEM_EXEC_STREAMED = 0, UFEM_EXEC_PERSISTENT = 1 };
+/* streamed: single-reduction PCG, two kernels per iteration (w = K u with
+ * (w,u); one vector kernel with the reduction, test and all updates),
+ * replayed from a HIP graph;  persistent: one cooperative launch with grid
+ * barriers;  fused: classic PCG, two kernels, the p update folded into the
+ * SpMV gather;  classic: classic PCG, three kernels (p update, SpMV + (p,q),
+ * vector update) */
+enum { UFEM_EXEC_STREAMED = 0, UFEM_EXEC_PERSISTENT = 1, UFEM_EXEC_FUSED = 2, UFEM_EXEC_CLASSIC = 3 };
 
 typedef struct {
   int solver;
@@ -59,8 +65,9 @@ typedef struct {
   long maxit;        /* 0 means n, as femSolve passes nDOFAct; < 0: none  */
   int exec;          /* streamed kernels (graph replayed) or one persistent launch */
   int chunk;         /* streamed form: iterations per graph replay (0: 32) */
-  double inner_eps;  /* mixed / learned: inner relative tolerance (0: 1e-4) */
-  long inner_maxit;  /* (0: n)                                            */
+  double inner_eps;  /* mixed: the FP64 residual is refreshed each time the FP32
+                        residual has dropped by this factor (0: 0.1) */
+  long inner_maxit;  /* mixed: ... and at least every this many iterations (0: 500) */
   void *net;         /* learned: network handle from ufem_net_*           */
   int warm_start;    /* nonzero: start from the current u                 */
 } ufem_solve_opts;
@@ -68,7 +75,7 @@ typedef struct {
 typedef struct {
   int status;        /* AF_OK, AF_ERR (not converged), AF_ERR_ZER, ...    */
   long iters;        /* iterations (all inner iterations for mixed)       */
-  long outer;        /* mixed: refinement sweeps                          */
+  long outer;        /* mixed: FP64 residual refreshes                    */
   int fallback;      /* learned: 1 if Jacobi-PCG had to take over         */
   double relres;     /* last ||r||/||b|| the solver tested                */
   double true_relres;/* ||b - K u|| / ||b|| recomputed in FP64            */

@@ -151,21 +151,35 @@ typedef struct {
   long n;
   double *x;
   ufem_work *w;
+  int variant;
 } pcg_ctx;
 
 static int pcg_step(void *c, long it, void *stream)
 {
   pcg_ctx *p = (pcg_ctx *)c;
   ufem_work *w = p->w;
-  return ugk_pcg_iter(p->K, p->n, p->x, w->r, w->z, w->p, w->q, w->dinv, w->st, w->part, it, stream);
+  (void)it;
+  if (p->variant == 2)   /* single reduction: u = z, p = p[0], s = p[1], w = q */
+    return ugk_cg1_iter(p->K, p->n, p->x, w->r, w->z, w->p[0], w->p[1], w->q, w->dinv, w->st, w->part, stream);
+  return ugk_pcg_iter(p->K, p->n, p->x, w->r, w->z, w->p, w->q, w->dinv, w->st, w->part, p->variant,
+                      stream);
 }
 
-static int pcg32_step(void *c, long it, void *stream)
+typedef struct {
+  const ugk_mat *K;
+  long n;
+  const double *b;
+  double *x;
+  ufem_work *w;
+} mx_ctx;
+
+static int mx_step(void *c, long it, void *stream)
 {
-  pcg_ctx *p = (pcg_ctx *)c;
-  ufem_work *w = p->w;
-  return ugk_pcg32_iter(p->K, p->n, w->x32, w->r32, w->z32, w->p32, w->q32, w->dinv32, w->st, w->part, it,
-                        stream);
+  mx_ctx *m = (mx_ctx *)c;
+  ufem_work *w = m->w;
+  (void)it;
+  return ugk_mx_iter(m->K, m->n, m->b, m->x, w->r, w->r32, w->z32, w->p32[0], w->q32, w->x32, w->dinv32, w->st,
+                     w->part, stream);
 }
 
 static int bicgs_step(void *c, long it, void *stream)
@@ -186,7 +200,7 @@ static int status_of(int done)
 static int true_residual(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
                          double bb, double *out)
 {
-  if (ugk_ir_update(K, n, x, NULL, 0.0, b, w->r, w->st, w->part, stream) || get_state(w, stream))
+  if (ugk_residual(K, n, b, x, w->r, w->st, w->part, stream) || get_state(w, stream))
     return gpu_fail("residual");
   *out = bb > 0 ? sqrt(w->st_host->rr / bb) : 0.0;
   return AF_OK;
@@ -206,22 +220,23 @@ static int solve_pcg(const ugk_mat *K, long n, const double *b, double *x, ufem_
                      const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
 {
   int rv;
-  pcg_ctx c = {K, n, x, w};
+  pcg_ctx c = {K, n, x, w,
+               o->exec == UFEM_EXEC_FUSED ? 0 : (o->exec == UFEM_EXEC_STREAMED ? 2 : 1)};
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
-  if (ugk_pcg_start(K, n, b, x, w->r, w->z, w->p[0], w->dinv, w->st, w->part, stream)) return gpu_fail("start");
-  if (get_state(w, stream)) return AF_ERR;
-  if (!w->st_host->done) {
-    if (o->exec == UFEM_EXEC_PERSISTENT) {
-      if (ugk_zero(w->p[1], sizeof(double) * (size_t)n, stream) ||
-          ugk_pcg_persistent(K, n, x, w->r, w->z, w->p[0], w->p[1], w->q, w->dinv, w->st, w->part, stream))
-        return gpu_fail("persistent PCG");
-    } else {
-      rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 32, pcg_step, &c, 1);
-      if (rv != AF_OK) return rv;
-    }
-    if (get_state(w, stream)) return AF_ERR;
+  if (c.variant == 2) {
+    if (ugk_cg1_start(K, n, b, x, w->r, w->z, w->p[0], w->p[1], w->dinv, w->part, stream))
+      return gpu_fail("start");
+  } else if (ugk_pcg_start(K, n, b, x, w->r, w->z, w->p[0], w->p[1], w->dinv, w->st, w->part, stream))
+    return gpu_fail("start");
+  if (o->exec == UFEM_EXEC_PERSISTENT) {
+    if (ugk_pcg_persistent(K, n, x, w->r, w->z, w->p[0], w->p[1], w->q, w->dinv, w->st, w->part, stream))
+      return gpu_fail("persistent PCG");
+  } else {
+    rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 32, pcg_step, &c, 1);
+    if (rv != AF_OK) return rv;
   }
+  if (get_state(w, stream)) return AF_ERR;
   if (w->st_host->bb <= 0.0) {   /* zero load: the reference returns x = 0 */
     if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
     info->status = AF_OK;
@@ -237,7 +252,7 @@ static int solve_bicgs(const ugk_mat *K, long n, const double *b, double *x, ufe
                        const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
 {
   int rv;
-  pcg_ctx c = {K, n, x, w};
+  pcg_ctx c = {K, n, x, w, 0};
   /* BiCGStab tests ||r|| <= eps ||b||; the reference rule is mapped on the
      same relative test with eps (its own test is not usable, see header) */
   rv = prepare(K, w, stream, o->eps, 1, maxit);
@@ -260,20 +275,21 @@ static int solve_bicgs(const ugk_mat *K, long n, const double *b, double *x, ufe
   return info->status;
 }
 
-/* Mixed precision: K stored once more in FP32.  Outer loop in FP64:
- * r = b - K x (FP64 SpMV), test ||r|| <= eps ||b||, scale r to unit norm,
- * solve K d = r with FP32 Jacobi PCG to inner_eps (dot products accumulated
- * in FP64), x += ||r|| d in FP64. */
+/* Mixed precision: K stored once more in FP32; CG runs on the FP32 matrix
+ * and FP32 vectors (dot products summed in FP64) without restarts, and
+ * whenever its recursive residual has dropped by inner_eps (default 0.1)
+ * below the last true residual -- or inner_maxit (default 500) iterations
+ * have passed -- the FP32 solution increment is added to the FP64 solution
+ * and r = b - K u is recomputed with the FP64 matrix.  The solve stops only
+ * on such an FP64 residual, ||b - K u|| <= eps ||b||, or after five refreshes
+ * in a row without progress (AF_ERR). */
 static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
                        const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
 {
   const ugk_mat *K = Km;
-  double inner_eps = o->inner_eps > 0 ? o->inner_eps : 1e-4;
-  long inner_maxit = o->inner_maxit > 0 ? o->inner_maxit : maxit;
-  double bb, rr;
-  long total = 0, outer = 0;
+  ugk_state *h = w->st_host;
+  mx_ctx c = {K, n, b, x, w};
   int rv;
-  pcg_ctx c = {K, n, x, w};
   if (!Km->d_vals32) {
     Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
     if (!Km->d_vals32) return gpu_fail("fp32 matrix");
@@ -282,48 +298,29 @@ static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_wor
   rv = prepare(K, w, stream, o->eps, 1, maxit);
   if (rv != AF_OK) return rv;
   if (ugk_to_f32(n, w->dinv, w->dinv32, stream)) return gpu_fail("fp32 diagonal");
-  if (ugk_dot(n, b, b, &w->st->bb, w->part, stream) || get_state(w, stream)) return gpu_fail("norm b");
-  bb = w->st_host->bb;
-  if (!(bb > 0.0)) {
+  memset(h, 0, sizeof(*h));
+  h->tol = o->eps;
+  h->rule = 1;
+  h->maxit = maxit;
+  h->delta = o->inner_eps > 0 ? o->inner_eps : 0.1;
+  h->ru_req = -1;
+  h->ru_at = 0;
+  h->max_gap = o->inner_maxit > 0 ? o->inner_maxit : 500;
+  if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
+  if (ugk_mx_start(K, n, b, x, w->r, w->r32, w->z32, w->p32[0], w->x32, w->dinv32, w->part, stream))
+    return gpu_fail("mixed start");
+  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 32, mx_step, &c, 1);
+  if (rv != AF_OK) return rv;
+  if (get_state(w, stream)) return AF_ERR;
+  if (h->bb <= 0.0) {
     if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
     info->status = AF_OK;
     return AF_OK;
   }
-  if (ugk_ir_update(K, n, x, NULL, 0.0, b, w->r, w->st, w->part, stream) || get_state(w, stream))
-    return gpu_fail("residual");
-  rr = w->st_host->rr;
-  info->status = AF_ERR;
-  while (total < maxit) {
-    double nr = sqrt(rr);
-    ugk_state *h = w->st_host;
-    if (nr <= o->eps * sqrt(bb)) { info->status = AF_OK; break; }
-    /* inner FP32 solve of K d = r / ||r|| */
-    memset(h, 0, sizeof(*h));
-    h->tol = inner_eps;
-    h->rule = 1;
-    h->maxit = inner_maxit < maxit - total ? inner_maxit : maxit - total;
-    if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
-    if (ugk_pcg32_start(K, n, w->r, 1.0 / nr, w->r32, w->z32, w->p32[0], w->x32, w->dinv32, w->st, w->part,
-                        stream) ||
-        get_state(w, stream))
-      return gpu_fail("inner start");
-    if (!w->st_host->done) {
-      rv = run_chunks(w, stream, w->st_host->maxit, o->chunk > 0 ? o->chunk : 32, pcg32_step, &c, 1);
-      if (rv != AF_OK) return rv;
-      if (get_state(w, stream)) return AF_ERR;
-    }
-    if (w->st_host->done == 2) { ufem_set_error("inner FP32 PCG broke down"); info->status = AF_ERR; break; }
-    total += w->st_host->iter;
-    outer++;
-    if (ugk_ir_update(K, n, x, w->x32, nr, b, w->r, w->st, w->part, stream) || get_state(w, stream))
-      return gpu_fail("refinement");
-    rr = w->st_host->rr;
-    if (!(rr == rr)) { ufem_set_error("refinement produced NaN"); info->status = AF_ERR; break; }
-  }
-  info->iters = total;
-  info->outer = outer;
-  info->relres = sqrt(rr / bb);
-  if (info->status != AF_OK && sqrt(rr) <= o->eps * sqrt(bb)) info->status = AF_OK;
+  info->iters = h->iter;
+  info->outer = h->n_ru;
+  info->relres = sqrt(h->rr_true / h->bb);
+  info->status = status_of(h->done);
   return info->status;
 }
 

@@ -568,7 +568,9 @@ int ufem_sys_spmv(ufem_sys *s, const double *x, double *y)
 /* Algorithmic bytes of one launch, counted on the real (unpadded) blocks:
  *   mode 0  y = K x, FP64:      (72 + 4) nnzb + 8 n (x) + 8 n (y)
  *   mode 1  y = K x, FP32:      (36 + 4) nnzb + 4 n + 4 n
- *   mode 2  PCG phase A, FP64:  (72 + 4) nnzb + 8 n (z) + 8 n (p) + 8 n (p') + 8 n (q)
+ *   mode 2  fused PCG kernel:   (72 + 4) nnzb + 8 n (z) + 8 n (p) + 8 n (p') + 8 n (q)
+ *   mode 3  q = K p, (p, q):    (72 + 4) nnzb + 8 n (p) + 8 n (q)
+ *   mode 4  w = K u, (w, u):    (72 + 4) nnzb + 8 n (u) + 8 n (w)
  * (n = 3 n_nodes; each vector counted once although its entries are
  * gathered by up to 27 rows -- those repeats are served by L2/MALL). */
 static double kernel_bytes(const ugk_mat *K, int mode)
@@ -576,13 +578,15 @@ static double kernel_bytes(const ugk_mat *K, int mode)
   double nnzb = (double)K->nnzb, n = (double)K->n_brows * K->bs, b2 = (double)K->bs * K->bs;
   if (mode == 1) return nnzb * (b2 * 4.0 + 4.0) + 8.0 * n;
   if (mode == 2) return nnzb * (b2 * 8.0 + 4.0) + 32.0 * n;
-  return nnzb * (b2 * 8.0 + 4.0) + 16.0 * n;
+  return nnzb * (b2 * 8.0 + 4.0) + 16.0 * n;   /* modes 0, 3, 4: x read, y written */
 }
 
 static int launch_mode(ufem_sys *s, int mode)
 {
   if (mode == 1) return ugk_spmv_f32(&s->K, s->w.z32, s->w.q32, s->stream);
   if (mode == 2) return ugk_pcg_iter_a(&s->K, s->w.z, s->w.p[0], s->w.p[1], s->w.q, s->w.st, s->w.part, s->stream);
+  if (mode == 3) return ugk_pcg_iter_q(&s->K, s->w.p[0], s->w.q, s->w.st, s->w.part, s->stream);
+  if (mode == 4) return ugk_cg1_iter_w(&s->K, s->w.z, s->w.q, s->w.st, s->w.part, s->stream);
   return ugk_spmv(&s->K, s->w.z, s->w.q, s->stream);
 }
 
@@ -595,11 +599,15 @@ int ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *byte
     s->K.d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)s->K.total_slots * 64 * 9);
     if (!s->K.d_vals32 || ugk_vals_to_f32(&s->K, s->stream)) return gpu_fail("fp32 copy");
   }
-  if (mode == 2) {   /* a running state so that phase A does its work */
+  if (mode >= 2) {   /* a running state (after a solve) so the kernel does its work */
     ugk_state h;
     memset(&h, 0, sizeof(h));
-    h.beta = 0.5;
-    h.maxit = 1;
+    h.iter = 1;
+    h.rule = 1;
+    h.tol = 0.0;
+    h.maxit = 1L << 40;
+    h.bb = 1.0;
+    h.rz[0] = h.rz[1] = 1.0;
     if (ugk_h2d(s->w.st, &h, sizeof(h), s->stream)) return gpu_fail("state");
   }
   if (launch_mode(s, mode)) return gpu_fail("kernel");   /* warm-up */
@@ -608,6 +616,11 @@ int ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *byte
     if (launch_mode(s, mode)) return gpu_fail("kernel");
   if (ugk_event_record(s->ev[5], s->stream)) return gpu_fail("event");
   t = ugk_event_ms(s->ev[4], s->ev[5]);
+  if (mode >= 2) {
+    ugk_state h;
+    if (ugk_d2h(&h, s->w.st, sizeof(h), s->stream)) return gpu_fail("state");
+    if (h.done) { ufem_set_error("timed kernel exited early (state %d)", h.done); return AF_ERR; }
+  }
   if (ms) *ms = (double)t / reps;
   if (bytes) *bytes = kernel_bytes(&s->K, mode);
   return AF_OK;

@@ -49,6 +49,18 @@ typedef struct {
   double tol;
   double aux[8];
   long iter;          /* completed iterations                           */
+  double rzB;         /* (r, z) of the running iteration, for its update */
+  long pidx;          /* which ping-pong buffer holds the current p      */
+  double maxrr[2];    /* mixed: largest sloppy ||r|| since the last true residual */
+  double rr_true;     /* mixed: ||b - K x||^2 of the last FP64 residual   */
+  double delta;       /* mixed: refresh when the sloppy ||r|| drops by this */
+  long ru_req, ru_at; /* mixed: iteration of the last refresh request / done */
+  long n_ru;          /* mixed: FP64 residual refreshes                   */
+  long cur;           /* single-reduction PCG: iteration latched by its SpMV */
+  double gam[2], alf[2];  /* single-reduction PCG: gamma, alpha by parity  */
+  double best_true;   /* mixed: smallest true ||r||^2 seen                */
+  long n_stall;       /* mixed: refreshes in a row without progress       */
+  long max_gap;       /* mixed: refresh at least every max_gap iterations */
   long maxit;
   int rule;           /* 0 reference rule, 1 relative residual          */
   int done;           /* 0 running, 1 converged, 2 breakdown, 3 maxit   */
@@ -122,33 +134,54 @@ int ugk_dot(long n, const double *d_a, const double *d_b, double *d_out, double 
 int ugk_axpby(long n, double a, const double *d_x, double b, const double *d_y, double *d_z, void *stream);
 
 /* ---------------- PCG (Jacobi), streamed form ---------------------------- */
-/* start: r = b - A x, z = dinv r, p = 0; state: rz, rr, bb, (xx); iter = 0 */
+/* start: r = b - A x, z = dinv r, p0 = p1 = 0; block partials of (r,z),
+ * (r,r), (x,x), (b,b) for the first iteration kernel; st->iter must be 0 */
 int ugk_pcg_start(const ugk_mat *K, long n, const double *d_b, const double *d_x,
-                  double *d_r, double *d_z, double *d_p, const double *d_dinv,
+                  double *d_r, double *d_z, double *d_p0, double *d_p1, const double *d_dinv,
                   ugk_state *d_st, double *d_part, void *stream);
-/* one iteration = two kernels; no-ops once st->done != 0 */
+/* one iteration; no-ops once st->done != 0.  variant 0: two kernels (the
+ * p update folded into the SpMV gather); variant 1: three kernels (p update,
+ * SpMV, vector update).  Iteration state lives on the device only. */
 int ugk_pcg_iter(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_z,
                  double **d_p /* [2] ping-pong */, double *d_q, const double *d_dinv,
-                 ugk_state *d_st, double *d_part, long it, void *stream);
-/* phase A alone (p' = z + beta p, q = A p', (p', q)) -- for timing */
+                 ugk_state *d_st, double *d_part, int variant, void *stream);
+/* one kernel of the iteration alone, for timing: the fused p-update + SpMV
+ * (ugk_pcg_iter_a) or the streamed form's q = A p + (p, q) (ugk_pcg_iter_q) */
 int ugk_pcg_iter_a(const ugk_mat *K, const double *d_z, const double *d_pold, double *d_pnew,
                    double *d_q, ugk_state *d_st, double *d_part, void *stream);
+int ugk_pcg_iter_q(const ugk_mat *K, const double *d_p, double *d_q, ugk_state *d_st, double *d_part,
+                   void *stream);
+int ugk_cg1_iter_w(const ugk_mat *K, const double *d_u, double *d_w, ugk_state *d_st, double *d_part,
+                   void *stream);
+/* single-reduction (Chronopoulos-Gear) Jacobi PCG: start, then per iteration
+ * w = A u with (w,u) partials, and one vector kernel doing the reduction,
+ * the convergence test and p, s, x, r, u updates.  d_s, d_w: two more vectors */
+int ugk_cg1_start(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r,
+                  double *d_u, double *d_p, double *d_s, const double *d_dinv, double *d_part,
+                  void *stream);
+int ugk_cg1_iter(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_u, double *d_p,
+                 double *d_s, double *d_w, const double *d_dinv, ugk_state *d_st, double *d_part,
+                 void *stream);
 /* whole solve in one cooperative launch (grid barriers between phases) */
 int ugk_pcg_persistent(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_z,
                        double *d_p0, double *d_p1, double *d_q, const double *d_dinv,
                        ugk_state *d_st, double *d_part, void *stream);
 
-/* ---------------- PCG in FP32 (inner solve of mixed precision) ----------- */
-int ugk_pcg32_start(const ugk_mat *K, long n, const double *d_r64, double scale,
-                    float *d_r, float *d_z, float *d_p, float *d_x, const float *d_dinv32,
-                    ugk_state *d_st, double *d_part, void *stream);
-int ugk_pcg32_iter(const ugk_mat *K, long n, float *d_x, float *d_r, float *d_z,
-                   float **d_p, float *d_q, const float *d_dinv32,
-                   ugk_state *d_st, double *d_part, long it, void *stream);
-/* x64 += scale * x32;  r64 = b - A x64 (FP64) with ||r||^2 into st->rr */
-int ugk_ir_update(const ugk_mat *K, long n, double *d_x64, const float *d_x32, double scale,
-                  const double *d_b, double *d_r64, ugk_state *d_st, double *d_part, void *stream);
+/* ---------------- mixed precision: FP32 CG with FP64 refreshes ----------- */
+/* start: r64 = b - K x64 (FP64), r32 = r64, z32 = dinv32 r32, p32 = x32 = 0 */
+int ugk_mx_start(const ugk_mat *K, long n, const double *d_b, const double *d_x64, double *d_r64,
+                 float *d_r32, float *d_z32, float *d_p32, float *d_x32, const float *d_dinv32,
+                 double *d_part, void *stream);
+/* one iteration: p update, q = K32 p (FP32), x32/r32/z32 update, refresh test,
+ * and -- when the FP32 residual has dropped by delta -- x64 += x32 and
+ * r64 = b - K x64 in FP64 */
+int ugk_mx_iter(const ugk_mat *K, long n, const double *d_b, double *d_x64, double *d_r64,
+                float *d_r32, float *d_z32, float *d_p32, float *d_q32, float *d_x32,
+                const float *d_dinv32, ugk_state *d_st, double *d_part, void *stream);
 int ugk_to_f32(long n, const double *d_a, float *d_b, void *stream);
+/* FP64 residual r = b - K x with ||r||^2 into st->rr (one-off, not in loops) */
+int ugk_residual(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r,
+                 ugk_state *d_st, double *d_part, void *stream);
 
 /* ---------------- BiCGStab (Jacobi, right preconditioned) ---------------- */
 int ugk_bicgs_start(const ugk_mat *K, long n, const double *d_b, const double *d_x,

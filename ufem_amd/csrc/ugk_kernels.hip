@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "ugk.h"
@@ -126,7 +127,7 @@ __device__ void grid_partials_sum(const double *part, unsigned G, double (&tot)[
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double a = 0.0;
-    for (unsigned i = threadIdx.x; i < G; i += WG) a += ld_agent(&part[(size_t)k * G + i]);
+    for (unsigned i = threadIdx.x; i < G; i += WG) a += part[(size_t)k * G + i];  // after grid.sync
     double s = wave_sum(a);
     if (lane == 0) sh[k][wave] = s;
   }
@@ -156,10 +157,66 @@ __device__ void block_publish(double (&v)[NV], double *part)
     for (int k = 0; k < NV; k++) {
       double a = sh[k][0];
       for (int w = 1; w < NW; w++) a += sh[k][w];
-      st_agent(&part[(size_t)k * gridDim.x + blockIdx.x], a);
+      part[(size_t)k * gridDim.x + blockIdx.x] = a;   // published by the next grid.sync
     }
   }
   __syncthreads();
+}
+
+// Block partials with plain stores.  The consumer is always a later kernel
+// on the same stream, whose start acquires what this kernel released at its
+// end, so no fences or atomics are needed on the iteration's hot path.
+template <int NV>
+__device__ void block_partials(double (&v)[NV], double *part)
+{
+  __shared__ double sh[NV][NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) sh[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double a = sh[threadIdx.x][0];
+    for (int w = 1; w < NW; w++) a += sh[threadIdx.x][w];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
+  }
+}
+
+// Every block sums the partials a previous kernel (grid G) published, in
+// index order, so all blocks hold bit-identical totals.
+template <int NV>
+__device__ void reduce_prev(const double *__restrict__ part, unsigned G, double (&tot)[NV])
+{
+  __shared__ double sh[NV][NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double a = 0.0;
+    for (unsigned i = threadIdx.x; i < G; i += WG) a += part[(size_t)k * G + i];
+    double s = wave_sum(a);
+    if (lane == 0) sh[k][wave] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double r = sh[k][0];
+    for (int w = 1; w < NW; w++) r += sh[k][w];
+    tot[k] = r;
+  }
+}
+
+// convergence test shared by the PCG kernels
+__device__ __forceinline__ int pcg_test(const ugk_state *st, long k, double rr, double xx, double bb,
+                                        double rz)
+{
+  const double nr = sqrt(rr), nb = sqrt(bb);
+  if (!(nb > 0.0)) return 1;
+  if (!(rz == rz) || !(rr == rr)) return 2;
+  if (st->rule == 1 ? nr <= st->tol * nb : nr <= st->tol * (st->normA * sqrt(xx) + nb)) return 1;
+  if (k >= st->maxit) return 3;
+  return 0;
 }
 
 // --------------------------------------------------------------------------
@@ -431,19 +488,66 @@ __device__ __forceinline__ void sell_row(const SellView &M, const T *__restrict_
   }
 }
 
+// Slot-split SELL row: a workgroup of 4 waves serves 2 slices; the two waves
+// of a slice stream the first and second half of its slots (more loads in
+// flight per row), the second half's sums meet the first's in LDS.  Matrix
+// values and columns are read once, so they are loaded non-temporally to
+// leave L2 to the gathered vector.  Returns true in the lanes that hold a
+// finished row sum (first-half wave, real row); `row` is the block row.
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p)
+{
+  return __builtin_nontemporal_load(p);
+}
+
+template <int BS, typename T, typename TX>
+__device__ __forceinline__ bool sell_row_split(const SellView &M, const T *__restrict__ vals,
+                                               const TX *__restrict__ x, TX (&acc)[BS], long &row)
+{
+  __shared__ TX sh[2][BS][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, half = w & 1, pair = w >> 1;
+  const long lb = remap_block(blockIdx.x, gridDim.x);
+  const long sl = lb * 2 + pair;
+#pragma unroll
+  for (int r = 0; r < BS; r++) acc[r] = 0;
+  if (sl < M.n_slices) {
+    const int L = M.slen[sl];
+    const int h = (L + 1) >> 1;
+    const int s0 = half ? h : 0, s1 = half ? L : h;
+    const long so = M.soff[sl];
+    const int *cp = M.cols + so * 64 + lane;
+    const T *vp = vals + so * 64 * BS * BS + lane;
+    for (int s = s0; s < s1; s++) {
+      const long J = ldnt(cp + (long)s * 64);
+      TX xv[BS];
+#pragma unroll
+      for (int c = 0; c < BS; c++) xv[c] = x[J * BS + c];
+      const T *v = vp + (long)s * 64 * BS * BS;
+#pragma unroll
+      for (int r = 0; r < BS; r++)
+#pragma unroll
+        for (int c = 0; c < BS; c++) acc[r] += (TX)ldnt(v + (r * BS + c) * 64) * xv[c];
+    }
+  }
+  if (half) {
+#pragma unroll
+    for (int r = 0; r < BS; r++) sh[pair][r][lane] = acc[r];
+  }
+  __syncthreads();
+  row = sl * 64 + lane;
+  if (half || sl >= M.n_slices) return false;
+#pragma unroll
+  for (int r = 0; r < BS; r++) acc[r] += sh[pair][r][lane];
+  return row < M.n_brows;
+}
+
 template <int BS, typename T, typename TX>
 __global__ __launch_bounds__(WG) void k_spmv(SellView M, const T *__restrict__ vals,
                                              const TX *__restrict__ x, TX *__restrict__ y)
 {
-  const long nb = gridDim.x;
-  const long lb = remap_block(blockIdx.x, nb);
-  const long sl = lb * NW + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (lb >= nb || sl >= M.n_slices) return;
   TX acc[BS];
-  sell_row<BS, T, TX>(M, vals, sl, lane, x, acc);
-  const long row = sl * 64 + lane;
-  if (row < M.n_brows) {
+  long row;
+  if (sell_row_split<BS, T, TX>(M, vals, x, acc, row)) {
 #pragma unroll
     for (int r = 0; r < BS; r++) y[row * BS + r] = acc[r];
   }
@@ -513,75 +617,127 @@ __global__ void k_axpby(long n, double a, const double *__restrict__ x, double b
 // --------------------------------------------------------------------------
 // Jacobi PCG (FP64), streamed form
 // --------------------------------------------------------------------------
-// start: r = b - A x; z = dinv r; p = 0; (r,z), (r,r), (b,b), (x,x)
+// Partial-sum slots (part = base of the solver's buffer):
+//   S  = part                 start kernel, 4 x GS:  (r,z) (r,r) (x,x) (b,b)
+//   B  = part + 4 * 65536     update kernel, 4 x GB: (r,z) (r,r) (x,x) 0
+//   A  = part + 8 * 65536/2.. SpMV kernel,   1 x GA: (p,q)
+#define PART_S(p) (p)                    /* 4 x up to 65536 blocks */
+#define PART_B(p) ((p) + 4L * 65536)      /* 4 x up to 16384 blocks */
+#define PART_A(p) ((p) + 5L * 65536)      /* 1 x up to 65536 blocks */
+
+// start: r = b - A x; z = dinv r; p = 0; partials (r,z), (r,r), (x,x), (b,b)
 template <int BS>
 __global__ __launch_bounds__(WG) void k_pcg_start(SellView M, const double *__restrict__ vals,
                                                   const double *__restrict__ b,
                                                   const double *__restrict__ x, double *r,
-                                                  double *z, double *p,
-                                                  const double *__restrict__ dinv, ugk_state *st,
-                                                  double *part)
+                                                  double *z, double *p0, double *p1,
+                                                  const double *__restrict__ dinv, double *part)
 {
-  const long nb = gridDim.x;
-  const long lb = remap_block(blockIdx.x, nb);
-  const long sl = lb * NW + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  double v[4] = {0, 0, 0, 0}, tot[4];
-  if (lb < nb && sl < M.n_slices) {
-    double acc[BS];
-    sell_row<BS, double, double>(M, vals, sl, lane, x, acc);
-    const long row = sl * 64 + lane;
-    if (row < M.n_brows) {
+  double v[4] = {0, 0, 0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
 #pragma unroll
-      for (int k = 0; k < BS; k++) {
-        const long i = row * BS + k;
-        const double ri = b[i] - acc[k];
-        const double zi = ri * dinv[i];
-        r[i] = ri;
-        z[i] = zi;
-        p[i] = 0.0;
-        v[0] += ri * zi;
-        v[1] += ri * ri;
-        v[2] += b[i] * b[i];
-        v[3] += x[i] * x[i];
-      }
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      const double ri = b[i] - acc[k];
+      const double zi = ri * dinv[i];
+      r[i] = ri;
+      z[i] = zi;
+      p0[i] = 0.0;
+      p1[i] = 0.0;
+      v[0] += ri * zi;
+      v[1] += ri * ri;
+      v[2] += x[i] * x[i];
+      v[3] += b[i] * b[i];
     }
   }
-  if (block_reduce_last<4>(v, part, &st->counter[0], tot) && threadIdx.x == 0) {
-    st->rz[0] = tot[0];
-    st->rr = tot[1];
-    st->bb = tot[2];
-    st->xx = tot[3];
-    st->beta = 0.0;
-    st->iter = 0;
-    const double nb2 = sqrt(tot[2]);
-    int done = 0;
-    if (!(nb2 > 0.0)) done = 1;   // zero load: x stays as given (caller zeroes)
-    else if (!(tot[0] == tot[0])) done = 2;
-    else if (st->rule == 1 ? sqrt(tot[1]) <= st->tol * nb2
-                           : sqrt(tot[1]) <= st->tol * (st->normA * sqrt(tot[3]) + nb2)) done = 1;
-    if (st->maxit <= 0 && !done) done = 3;
-    st->done = done;
-  }
+  block_partials<4>(v, PART_S(part));
 }
 
-// phase A: p_new = z + beta p_old (own rows), q = A p_new (gathering z and
-// p_old at the neighbours), partial (p_new, q); last block publishes aux[0] = pq
+// iteration head shared by the p-update kernels: totals of iteration k's
+// residual, convergence test, beta; block 0 records the state
+__device__ __forceinline__ bool pcg_head(ugk_state *st, const double *part, unsigned GS, unsigned GB,
+                                         long &k, double &beta)
+{
+  k = st->iter;
+  double t[4];
+  if (k == 0) reduce_prev<4>(PART_S(part), GS, t);
+  else reduce_prev<4>(PART_B(part), GB, t);
+  const double bb = (k == 0) ? t[3] : st->bb;
+  const int done = pcg_test(st, k, t[1], t[2], bb, t[0]);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  beta = (k == 0) ? 0.0 : t[0] / st->rz[(k + 1) & 1];
+  if (lead) {
+    st->rr = t[1];
+    st->xx = t[2];
+    if (k == 0) st->bb = bb;
+    if (done) st->done = done;
+    else {
+      st->rz[k & 1] = t[0];
+      st->rzB = t[0];
+      st->beta = beta;
+      st->pidx = k & 1;
+    }
+  }
+  return done != 0;
+}
+
+// variant 1, kernel 1: p = z + beta p (in place)
+__global__ __launch_bounds__(WG) void k_pcg_p(long n, const double *__restrict__ z, double *p,
+                                              ugk_state *st, const double *part, unsigned GS,
+                                              unsigned GB)
+{
+  if (st->done) return;
+  long k;
+  double beta;
+  if (pcg_head(st, part, GS, GB, k, beta)) return;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+    p[i] = z[i] + beta * p[i];
+}
+
+// variant 1, kernel 2: q = A p, partial (p, q)
 template <int BS>
-__global__ __launch_bounds__(WG) void k_pcg_a(SellView M, const double *__restrict__ vals,
-                                              const double *__restrict__ z,
-                                              const double *__restrict__ pold,
-                                              double *__restrict__ pnew, double *__restrict__ q,
+__global__ __launch_bounds__(WG) void k_pcg_q(SellView M, const double *__restrict__ vals,
+                                              const double *__restrict__ p, double *__restrict__ q,
                                               ugk_state *st, double *part)
 {
   if (st->done) return;
-  const double beta = st->beta;
+  double v[1] = {0.0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, p, acc, row)) {
+#pragma unroll
+    for (int r = 0; r < BS; r++) {
+      const long i = row * BS + r;
+      q[i] = acc[r];
+      v[0] += p[i] * acc[r];
+    }
+  }
+  block_partials<1>(v, PART_A(part));
+}
+
+// variant 0, the single kernel before the update: head + p' = z + beta p
+// (own rows written to the other ping-pong buffer) + q = A p' with the
+// neighbours' p' formed on the fly from gathered z and p
+template <int BS>
+__global__ __launch_bounds__(WG) void k_pcg_a(SellView M, const double *__restrict__ vals,
+                                              const double *__restrict__ z, double *p0, double *p1,
+                                              double *__restrict__ q, ugk_state *st, double *part,
+                                              unsigned GS, unsigned GB)
+{
+  if (st->done) return;
+  long k;
+  double beta;
+  if (pcg_head(st, part, GS, GB, k, beta)) return;
+  const double *pold = (k & 1) ? p0 : p1;
+  double *pnew = (k & 1) ? p1 : p0;
   const long nb = gridDim.x;
   const long lb = remap_block(blockIdx.x, nb);
   const long sl = lb * NW + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  double v[1] = {0.0}, tot[1];
-  if (lb < nb && sl < M.n_slices) {
+  double v[1] = {0.0};
+  if (sl < M.n_slices) {
     const int L = M.slen[sl];
     const long so = M.soff[sl];
     const int *cp = M.cols + so * 64 + lane;
@@ -613,23 +769,141 @@ __global__ __launch_bounds__(WG) void k_pcg_a(SellView M, const double *__restri
       }
     }
   }
-  if (block_reduce_last<1>(v, part, &st->counter[1], tot) && threadIdx.x == 0) st->aux[0] = tot[0];
+  block_partials<1>(v, PART_A(part));
 }
 
-// phase B: alpha = rz / pq; x += alpha p; r -= alpha q; z = dinv r;
-// partials (r,z), (r,r), (x,x); last block: convergence test, beta
-__global__ __launch_bounds__(WG) void k_pcg_b(long n, double *__restrict__ x, double *__restrict__ r,
-                                              double *__restrict__ z, const double *__restrict__ p,
-                                              const double *__restrict__ q,
-                                              const double *__restrict__ dinv, ugk_state *st,
-                                              double *part)
+// ---- single-reduction Jacobi PCG (Chronopoulos-Gear) ----------------------
+// u = dinv r, w = A u, gamma = (r,u), delta = (w,u);
+// beta_i = gamma_i / gamma_{i-1}, alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1});
+// p = u + beta p, s = w + beta s (s = A p), x += alpha p, r -= alpha s.
+// PART_B holds two parity slots of (gamma, rr, xx, bb) partials.
+#define PART_B2(p, par) ((par) ? (p) + 7L * 65536 : PART_B(p))   /* 4 x up to 16384 blocks each */
+
+template <int BS>
+__global__ __launch_bounds__(WG) void k_cg1_start(SellView M, const double *__restrict__ vals,
+                                                  const double *__restrict__ b,
+                                                  const double *__restrict__ x, double *r, double *u,
+                                                  double *p, double *s,
+                                                  const double *__restrict__ dinv, double *part)
+{
+  double v[4] = {0, 0, 0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      const double ri = b[i] - acc[k];
+      const double ui = ri * dinv[i];
+      r[i] = ri;
+      u[i] = ui;
+      p[i] = 0.0;
+      s[i] = 0.0;
+      v[0] += ri * ui;
+      v[1] += ri * ri;
+      v[2] += x[i] * x[i];
+      v[3] += b[i] * b[i];
+    }
+  }
+  // grid of this kernel may exceed the parity slot's 16384 blocks: the slot
+  // is sized for vec grids; the start is launched on a vec-sized grid below
+  block_partials<4>(v, PART_B2(part, 0));
+}
+
+// w = A u, partial (w, u); latches the iteration for the vector kernel
+template <int BS>
+__global__ __launch_bounds__(WG) void k_cg1_w(SellView M, const double *__restrict__ vals,
+                                              const double *__restrict__ u, double *__restrict__ w,
+                                              ugk_state *st, double *part)
 {
   if (st->done) return;
-  const long it = st->iter;
-  const double rz = st->rz[it & 1];
-  const double pq = st->aux[0];
-  const double alpha = rz / pq;
-  double v[3] = {0, 0, 0}, tot[3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->cur = st->iter;
+  double v[1] = {0.0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, u, acc, row)) {
+#pragma unroll
+    for (int r = 0; r < BS; r++) {
+      const long i = row * BS + r;
+      w[i] = acc[r];
+      v[0] += u[i] * acc[r];
+    }
+  }
+  block_partials<1>(v, PART_A(part));
+}
+
+__global__ __launch_bounds__(WG) void k_cg1_upd(long n, double *__restrict__ x, double *__restrict__ r,
+                                                double *__restrict__ u, double *__restrict__ p,
+                                                double *__restrict__ s, const double *__restrict__ w,
+                                                const double *__restrict__ dinv, ugk_state *st,
+                                                double *part, unsigned G0, unsigned GB, unsigned GA)
+{
+  if (st->done) return;
+  const long i0 = st->cur;
+  double t[4], d1[1];
+  reduce_prev<4>(PART_B2(part, i0 & 1), i0 == 0 ? G0 : GB, t);
+  reduce_prev<1>(PART_A(part), GA, d1);
+  const double gam = t[0], dlt = d1[0];
+  const double bb = (i0 == 0) ? t[3] : st->bb;
+  int done = pcg_test(st, i0, t[1], t[2], bb, gam);
+  const double beta = (i0 == 0) ? 0.0 : gam / st->gam[(i0 + 1) & 1];
+  const double den = (i0 == 0) ? dlt : dlt - beta * gam / st->alf[(i0 + 1) & 1];
+  const double alpha = gam / den;
+  if (!done && (!(den > 0.0) || !(alpha == alpha))) done = 2;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) {
+    st->rr = t[1];
+    st->xx = t[2];
+    if (i0 == 0) st->bb = bb;
+    if (done) st->done = done;
+    else {
+      st->gam[i0 & 1] = gam;
+      st->alf[i0 & 1] = alpha;
+      st->alpha = alpha;
+      st->beta = beta;
+    }
+  }
+  if (done) return;
+  double v[4] = {0, 0, 0, 0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double pi = u[i] + beta * p[i];
+    const double si = w[i] + beta * s[i];
+    const double xi = x[i] + alpha * pi;
+    const double ri = r[i] - alpha * si;
+    const double ui = ri * dinv[i];
+    p[i] = pi;
+    s[i] = si;
+    x[i] = xi;
+    r[i] = ri;
+    u[i] = ui;
+    v[0] += ri * ui;
+    v[1] += ri * ri;
+    v[2] += xi * xi;
+  }
+  block_partials<4>(v, PART_B2(part, (i0 + 1) & 1));
+  if (lead) st->iter = i0 + 1;
+}
+
+// update: alpha = rz / (p, q); x += alpha p; r -= alpha q; z = dinv r;
+// partials (r,z), (r,r), (x,x); block 0 advances the iteration count
+__global__ __launch_bounds__(WG) void k_pcg_b(long n, double *__restrict__ x, double *__restrict__ r,
+                                              double *__restrict__ z, const double *p0,
+                                              const double *p1, const double *__restrict__ q,
+                                              const double *__restrict__ dinv, ugk_state *st,
+                                              double *part, unsigned GA, int pingpong)
+{
+  if (st->done) return;
+  double t[1];
+  reduce_prev<1>(PART_A(part), GA, t);
+  const double pq = t[0];
+  const double alpha = st->rzB / pq;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(pq > 0.0) || !(alpha == alpha)) {
+    if (lead) st->done = 2;
+    return;
+  }
+  const double *p = (pingpong && st->pidx) ? p1 : p0;
+  double v[4] = {0, 0, 0, 0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
     const double xi = x[i] + alpha * p[i];
     const double ri = r[i] - alpha * q[i];
@@ -641,45 +915,59 @@ __global__ __launch_bounds__(WG) void k_pcg_b(long n, double *__restrict__ x, do
     v[1] += ri * ri;
     v[2] += xi * xi;
   }
-  if (block_reduce_last<3>(v, part, &st->counter[2], tot) && threadIdx.x == 0) {
-    const long it1 = it + 1;
-    st->iter = it1;
+  block_partials<4>(v, PART_B(part));
+  if (lead) {
     st->alpha = alpha;
-    st->rz[it1 & 1] = tot[0];
-    st->rr = tot[1];
-    st->xx = tot[2];
-    st->beta = tot[0] / rz;
-    const double nr = sqrt(tot[1]), nbb = sqrt(st->bb);
-    int done = 0;
-    if (!(pq > 0.0) || !(tot[0] == tot[0])) done = 2;
-    else if (st->rule == 1 ? nr <= st->tol * nbb : nr <= st->tol * (st->normA * sqrt(tot[2]) + nbb)) done = 1;
-    else if (it1 >= st->maxit) done = 3;
-    st->done = done;
+    st->iter = st->iter + 1;
   }
+}
+
+// Sense-counting grid barrier for a co-resident (cooperative) grid: one
+// ticket per block on `bar[0]`; the last arrival resets it and bumps the
+// generation `bar[1]` that the others wait on.  Every wait ends because the
+// launch is cooperative (all blocks resident) and every block arrives.
+__device__ __forceinline__ void grid_barrier(unsigned int *bar, unsigned G)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned t = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == G - 1) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen)
+        __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
 }
 
 // whole PCG in one cooperative launch; same arithmetic as k_pcg_a/k_pcg_b
 // but with the two reductions finished redundantly in every block after a
 // grid barrier.  Every block takes the same decisions, so all leave together.
-template <int BS>
+template <int BS, bool CGSYNC>
 __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double *__restrict__ vals,
                                                        long n, double *x, double *r, double *z,
                                                        double *p0, double *p1, double *q,
                                                        const double *__restrict__ dinv,
-                                                       ugk_state *st, double *part)
+                                                       ugk_state *st, double *part, unsigned GS)
 {
   cg::grid_group grid = cg::this_grid();
   const unsigned G = gridDim.x;
-  double *partA = part, *partB = part + G;
-  long it = st->iter;
-  double rz = st->rz[it & 1];
-  double beta = st->beta;
-  const double nbb = sqrt(st->bb), normA = st->normA, tol = st->tol;
+  double *partA = PART_B(part), *partB = PART_B(part) + G;   // after the start partials
+  double t0[4];
+  reduce_prev<4>(PART_S(part), GS, t0);      // written by the start kernel
+  long it = 0;
+  double rz = t0[0];
+  double beta = 0.0;
+  const double nbb = sqrt(t0[3]), normA = st->normA, tol = st->tol;
   const int rule = st->rule;
   const long maxit = st->maxit;
-  int done = st->done;
+  int done = pcg_test(st, 0, t0[1], t0[2], t0[3], t0[0]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double rr = st->rr, xx = st->xx, alpha = 0.0;
+  double rr = t0[1], xx = t0[2], alpha = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->bb = t0[3];
   while (!done) {
     const double *pold = (it & 1) ? p1 : p0;
     double *pnew = (it & 1) ? p0 : p1;
@@ -717,7 +1005,7 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
       }
     }
     block_publish<1>(v, partA);
-    grid.sync();
+    if (CGSYNC) grid.sync(); else grid_barrier(&st->counter[6], G);
     double tA[1];
     grid_partials_sum<1>(partA, G, tA);
     const double pq = tA[0];
@@ -735,7 +1023,7 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
       w[2] += xi * xi;
     }
     block_publish<3>(w, partB);
-    grid.sync();
+    if (CGSYNC) grid.sync(); else grid_barrier(&st->counter[6], G);
     double tB[3];
     grid_partials_sum<3>(partB, G, tB);
     it++;
@@ -757,102 +1045,133 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
     st->alpha = alpha;
     st->done = done;
   }
+  (void)partB;
 }
 
 // --------------------------------------------------------------------------
-// FP32 inner PCG and FP64 iterative refinement
+// Mixed precision: FP32 CG (K in FP32, vectors in FP32, dot products summed
+// in FP64) that is never restarted; whenever its recursive residual has
+// dropped by `delta` since the last true residual, the FP32 solution
+// increment is folded into the FP64 solution and r = b - K x is recomputed
+// with the FP64 matrix (a "reliable update").  Convergence is only declared
+// on such an FP64 residual.
 // --------------------------------------------------------------------------
+#define PART_R(p) ((p) + 6L * 65536)      /* 2 x up to 65536 blocks */
+
 template <int BS>
-__global__ __launch_bounds__(WG) void k_pcg32_start(long n, const double *__restrict__ r64, double scale,
-                                                    float *r, float *z, float *p, float *x,
-                                                    const float *__restrict__ dinv, ugk_state *st,
-                                                    double *part)
+__global__ __launch_bounds__(WG) void k_mx_start(SellView M, const double *__restrict__ vals,
+                                                 const double *__restrict__ b,
+                                                 const double *__restrict__ x64, double *r64,
+                                                 float *r32, float *z32, float *p32, float *x32,
+                                                 const float *__restrict__ dinv, double *part)
 {
-  double v[2] = {0, 0}, tot[2];
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
-    const float ri = (float)(r64[i] * scale);
-    const float zi = ri * dinv[i];
-    r[i] = ri;
-    z[i] = zi;
-    p[i] = 0.0f;
-    x[i] = 0.0f;
-    v[0] += (double)ri * (double)zi;
-    v[1] += (double)ri * (double)ri;
+  double v[4] = {0, 0, 0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x64, acc, row)) {
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      const double ri = b[i] - acc[k];
+      const float rf = (float)ri;
+      const float zf = rf * dinv[i];
+      r64[i] = ri;
+      r32[i] = rf;
+      z32[i] = zf;
+      p32[i] = 0.0f;
+      x32[i] = 0.0f;
+      v[0] += (double)rf * (double)zf;
+      v[1] += ri * ri;
+      v[3] += b[i] * b[i];
+    }
   }
-  if (block_reduce_last<2>(v, part, &st->counter[3], tot) && threadIdx.x == 0) {
-    st->rz[0] = tot[0];
-    st->rr = tot[1];
-    st->bb = tot[1];
-    st->beta = 0.0;
-    st->iter = 0;
-    st->done = (tot[1] > 0.0) ? (st->maxit > 0 ? 0 : 3) : 1;
+  block_partials<4>(v, PART_S(part));
+}
+
+// head: totals of iteration k's residual -- the start, a refresh (true FP64
+// residual) or the FP32 update -- convergence on true residuals only, beta,
+// p = z + beta p
+__global__ __launch_bounds__(WG) void k_mx_p(long n, const float *__restrict__ z, float *p,
+                                             ugk_state *st, const double *part, unsigned GS,
+                                             unsigned GB, unsigned GR)
+{
+  if (st->done) return;
+  const long k = st->iter;
+  double t[4];
+  const bool fresh = (k == 0) || (st->ru_at == k);
+  if (k == 0) reduce_prev<4>(PART_S(part), GS, t);
+  else if (fresh) { double u[2]; reduce_prev<2>(PART_R(part), GR, u); t[0] = u[0]; t[1] = u[1]; t[2] = t[3] = 0.0; }
+  else { double u[2]; reduce_prev<2>(PART_B(part), GB, u); t[0] = u[0]; t[1] = u[1]; t[2] = t[3] = 0.0; }
+  const double bb = (k == 0) ? t[3] : st->bb;
+  const double nb = sqrt(bb), nr = sqrt(t[1]);
+  int done = 0;
+  // a refresh that does not improve the best true residual by 10% is a
+  // stall; five in a row end the solve (stagnation at the FP32 floor)
+  const bool progress = (k == 0) || t[1] < 0.81 * st->best_true;
+  const long stall = fresh ? (progress ? 0 : st->n_stall + 1) : st->n_stall;
+  if (!(nb > 0.0)) done = 1;
+  else if (!(t[0] == t[0]) || !(t[1] == t[1])) done = 2;
+  else if (fresh && nr <= st->tol * nb) done = 1;
+  else if (stall >= 5) done = 2;
+  else if (k >= st->maxit) done = 3;
+  const double beta = (k == 0) ? 0.0 : t[0] / st->rz[(k + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (k == 0) st->bb = bb;
+    if (fresh) {
+      st->rr_true = t[1];
+      st->maxrr[k & 1] = nr;
+      st->n_stall = stall;
+      if (progress) st->best_true = t[1];
+    }
+    st->rr = t[1];
+    if (done) st->done = done;
+    else { st->rz[k & 1] = t[0]; st->rzB = t[0]; st->beta = beta; }
   }
+  if (done) return;
+  const float bf = (float)beta;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) p[i] = z[i] + bf * p[i];
 }
 
 template <int BS>
-__global__ __launch_bounds__(WG) void k_pcg32_a(SellView M, const float *__restrict__ vals,
-                                                const float *__restrict__ z,
-                                                const float *__restrict__ pold,
-                                                float *__restrict__ pnew, float *__restrict__ q,
-                                                ugk_state *st, double *part)
+__global__ __launch_bounds__(WG) void k_mx_q(SellView M, const float *__restrict__ vals,
+                                             const float *__restrict__ p, float *__restrict__ q,
+                                             ugk_state *st, double *part)
 {
   if (st->done) return;
-  const float beta = (float)st->beta;
-  const long nb = gridDim.x;
-  const long lb = remap_block(blockIdx.x, nb);
-  const long sl = lb * NW + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  double v[1] = {0.0}, tot[1];
-  if (lb < nb && sl < M.n_slices) {
-    const int L = M.slen[sl];
-    const long so = M.soff[sl];
-    const int *cp = M.cols + so * 64 + lane;
-    const float *vp = vals + so * 64 * BS * BS + lane;
-    float acc[BS];
+  double v[1] = {0.0};
+  float acc[BS];
+  long row;
+  if (sell_row_split<BS, float, float>(M, vals, p, acc, row)) {
 #pragma unroll
-    for (int r = 0; r < BS; r++) acc[r] = 0.0f;
-#pragma unroll 3
-    for (int s = 0; s < L; s++) {
-      const long J = cp[(long)s * 64];
-      float xv[BS];
-#pragma unroll
-      for (int c = 0; c < BS; c++) xv[c] = z[J * BS + c] + beta * pold[J * BS + c];
-      const float *vv = vp + (long)s * 64 * BS * BS;
-#pragma unroll
-      for (int r = 0; r < BS; r++)
-#pragma unroll
-        for (int c = 0; c < BS; c++) acc[r] += vv[(r * BS + c) * 64] * xv[c];
-    }
-    const long row = sl * 64 + lane;
-    if (row < M.n_brows) {
-#pragma unroll
-      for (int r = 0; r < BS; r++) {
-        const long i = row * BS + r;
-        const float pn = z[i] + beta * pold[i];
-        pnew[i] = pn;
-        q[i] = acc[r];
-        v[0] += (double)pn * (double)acc[r];
-      }
+    for (int r = 0; r < BS; r++) {
+      const long i = row * BS + r;
+      q[i] = acc[r];
+      v[0] += (double)p[i] * (double)acc[r];
     }
   }
-  if (block_reduce_last<1>(v, part, &st->counter[1], tot) && threadIdx.x == 0) st->aux[0] = tot[0];
+  block_partials<1>(v, PART_A(part));
 }
 
-__global__ __launch_bounds__(WG) void k_pcg32_b(long n, float *__restrict__ x, float *__restrict__ r,
-                                                float *__restrict__ z, const float *__restrict__ p,
-                                                const float *__restrict__ q,
-                                                const float *__restrict__ dinv, ugk_state *st,
-                                                double *part)
+__global__ __launch_bounds__(WG) void k_mx_b(long n, float *__restrict__ x, float *__restrict__ r,
+                                             float *__restrict__ z, const float *__restrict__ p,
+                                             const float *__restrict__ q,
+                                             const float *__restrict__ dinv, ugk_state *st,
+                                             double *part, unsigned GA)
 {
   if (st->done) return;
-  const long it = st->iter;
-  const double rz = st->rz[it & 1];
-  const double pq = st->aux[0];
-  const float alpha = (float)(rz / pq);
-  double v[2] = {0, 0}, tot[2];
+  double t[1];
+  reduce_prev<1>(PART_A(part), GA, t);
+  const double pq = t[0];
+  const double alpha = st->rzB / pq;
+  if (!(pq > 0.0) || !(alpha == alpha)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 2;
+    return;
+  }
+  const float af = (float)alpha;
+  double v[2] = {0, 0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
-    const float xi = x[i] + alpha * p[i];
-    const float ri = r[i] - alpha * q[i];
+    const float xi = x[i] + af * p[i];
+    const float ri = r[i] - af * q[i];
     const float zi = ri * dinv[i];
     x[i] = xi;
     r[i] = ri;
@@ -860,26 +1179,67 @@ __global__ __launch_bounds__(WG) void k_pcg32_b(long n, float *__restrict__ x, f
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
   }
-  if (block_reduce_last<2>(v, part, &st->counter[2], tot) && threadIdx.x == 0) {
-    const long it1 = it + 1;
-    st->iter = it1;
-    st->rz[it1 & 1] = tot[0];
-    st->rr = tot[1];
-    st->beta = tot[0] / rz;
-    int done = 0;
-    if (!(pq > 0.0) || !(tot[0] == tot[0])) done = 2;
-    else if (sqrt(tot[1]) <= st->tol * sqrt(st->bb)) done = 1;
-    else if (it1 >= st->maxit) done = 3;
-    st->done = done;
+  block_partials<2>(v, PART_B(part));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alpha;
+    st->iter = st->iter + 1;
   }
 }
 
-// x64 += scale * x32, then r64 = b - A x64 with ||r64||^2 -> st->rr
-template <int BS>
-__global__ void k_ir_axpy(long n, double *x64, const float *__restrict__ x32, double scale)
+// refresh test on the FP32 residual just produced; on refresh fold x32 into
+// x64 (the FP64 residual follows in k_mx_r)
+__global__ __launch_bounds__(WG) void k_mx_ru(long n, double *x64, float *x32, ugk_state *st,
+                                              const double *part, unsigned GB)
 {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    x64[i] += scale * (double)x32[i];
+  if (st->done) return;
+  const long k1 = st->iter;
+  double t[2];
+  reduce_prev<2>(PART_B(part), GB, t);
+  const double rn = sqrt(t[1]);
+  const double mx = st->maxrr[(k1 + 1) & 1];
+  const bool refresh = !(rn >= st->delta * mx) || rn <= st->tol * sqrt(st->bb) || k1 >= st->maxit ||
+                       k1 - st->ru_at >= st->max_gap;
+  if (!refresh) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->maxrr[k1 & 1] = fmax(mx, rn);
+    return;
+  }
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    x64[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->ru_req = k1;
+}
+
+template <int BS>
+__global__ __launch_bounds__(WG) void k_mx_r(SellView M, const double *__restrict__ vals,
+                                             const double *__restrict__ b,
+                                             const double *__restrict__ x64, double *r64,
+                                             float *r32, float *z32, const float *__restrict__ dinv,
+                                             ugk_state *st, double *part)
+{
+  if (st->done || st->ru_req != st->iter) return;
+  double v[2] = {0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x64, acc, row)) {
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      const double ri = b[i] - acc[k];
+      const float rf = (float)ri;
+      const float zf = rf * dinv[i];
+      r64[i] = ri;
+      r32[i] = rf;
+      z32[i] = zf;
+      v[0] += (double)rf * (double)zf;
+      v[1] += ri * ri;
+    }
+  }
+  block_partials<2>(v, PART_R(part));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->ru_at = st->iter;
+    st->n_ru = st->n_ru + 1;
+  }
 }
 
 template <int BS>
@@ -888,23 +1248,16 @@ __global__ __launch_bounds__(WG) void k_residual(SellView M, const double *__res
                                                  const double *__restrict__ x, double *r,
                                                  ugk_state *st, double *part)
 {
-  const long nb = gridDim.x;
-  const long lb = remap_block(blockIdx.x, nb);
-  const long sl = lb * NW + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
   double v[1] = {0.0}, tot[1];
-  if (lb < nb && sl < M.n_slices) {
-    double acc[BS];
-    sell_row<BS, double, double>(M, vals, sl, lane, x, acc);
-    const long row = sl * 64 + lane;
-    if (row < M.n_brows) {
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
 #pragma unroll
-      for (int k = 0; k < BS; k++) {
-        const long i = row * BS + k;
-        const double ri = b[i] - acc[k];
-        r[i] = ri;
-        v[0] += ri * ri;
-      }
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      const double ri = b[i] - acc[k];
+      r[i] = ri;
+      v[0] += ri * ri;
     }
   }
   if (block_reduce_last<1>(v, part, &st->counter[4], tot) && threadIdx.x == 0) st->rr = tot[0];
@@ -970,25 +1323,18 @@ __global__ __launch_bounds__(WG) void k_spmv_dots(SellView M, const double *__re
 {
   if (st->done) return;
   if (ND == 2 && st->aux[5] != 0.0) return;   // s already converged
-  const long nb = gridDim.x;
-  const long lb = remap_block(blockIdx.x, nb);
-  const long sl = lb * NW + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
   double v[ND], tot[ND];
 #pragma unroll
   for (int k = 0; k < ND; k++) v[k] = 0.0;
-  if (lb < nb && sl < M.n_slices) {
-    double acc[BS];
-    sell_row<BS, double, double>(M, vals, sl, lane, x, acc);
-    const long row = sl * 64 + lane;
-    if (row < M.n_brows) {
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
 #pragma unroll
-      for (int k = 0; k < BS; k++) {
-        const long i = row * BS + k;
-        y[i] = acc[k];
-        v[0] += acc[k] * w1[i];
-        if (ND == 2) v[1] += acc[k] * acc[k];
-      }
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      y[i] = acc[k];
+      v[0] += acc[k] * w1[i];
+      if (ND == 2) v[1] += acc[k] * acc[k];
     }
   }
   if (block_reduce_last<ND>(v, part, &st->counter[ctr], tot) && threadIdx.x == 0) {
@@ -1076,14 +1422,24 @@ static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 // so that remap_block() is a bijection on [0, gridDim.x)
 static inline unsigned spmv_blocks(const ugk_mat *K)
 {
+  long b = (K->n_slices + 1) / 2;       // two slices per 4-wave block (slot-split)
+  return (unsigned)((b + 7) / 8 * 8);
+}
+// blocks for kernels that still map one slice per wavefront (k_pcg_a, FP32 inner)
+static inline unsigned spmv_blocks1(const ugk_mat *K)
+{
   long b = (K->n_slices + NW - 1) / NW;
   return (unsigned)((b + 7) / 8 * 8);
 }
 static int g_ncu = 256;
+static int g_vec_cap = 512;   // blocks of the vector kernels (UFEM_VEC_BLOCKS)
+// Vector kernels also sum the previous kernel's block partials redundantly in
+// every block, so their grid is kept small: 2 blocks per CU stream at ~95% of
+// the bandwidth of larger grids while reading 4x fewer partials.
 static inline unsigned vec_blocks(long n)
 {
   long b = (n + WG - 1) / WG;
-  long cap = 4L * g_ncu;
+  long cap = g_vec_cap;
   return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 static const unsigned RED_MAX_BLOCKS = UGK_RED_MAX_BLOCKS;
@@ -1099,6 +1455,11 @@ int ugk_init(void)
   hipDeviceProp_t pr;
   CK(hipGetDeviceProperties(&pr, dev));
   g_ncu = pr.multiProcessorCount;
+  g_vec_cap = 2 * g_ncu;
+  if (const char *e = getenv("UFEM_VEC_BLOCKS")) {
+    int v = atoi(e);
+    if (v >= 8 && v <= 16384) g_vec_cap = v;
+  }
   return 0;
 }
 
@@ -1368,39 +1729,94 @@ int ugk_axpby(long n, double a, const double *x, double b, const double *y, doub
 
 // ---- PCG ------------------------------------------------------------------
 int ugk_pcg_start(const ugk_mat *K, long n, const double *b, const double *x, double *r, double *z,
-                  double *p, const double *dinv, ugk_state *st, double *part, void *s)
+                  double *p0, double *p1, const double *dinv, ugk_state *st, double *part, void *s)
 {
   (void)n;
+  (void)st;
   if (K->bs == 3)
-    k_pcg_start<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, z, p, dinv, st, part);
+    k_pcg_start<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, z, p0, p1, dinv, part);
   else
-    k_pcg_start<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, z, p, dinv, st, part);
+    k_pcg_start<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, z, p0, p1, dinv, part);
   CK_LAUNCH("k_pcg_start");
   return 0;
 }
 
 int ugk_pcg_iter(const ugk_mat *K, long n, double *x, double *r, double *z, double **p, double *q,
-                 const double *dinv, ugk_state *st, double *part, long it, void *s)
+                 const double *dinv, ugk_state *st, double *part, int variant, void *s)
 {
-  double *pold = p[it & 1], *pnew = p[(it + 1) & 1];
-  if (K->bs == 3)
-    k_pcg_a<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
-  else
-    k_pcg_a<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
-  CK_LAUNCH("k_pcg_a");
-  k_pcg_b<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, z, pnew, q, dinv, st, part);
-  CK_LAUNCH("k_pcg_b");
+  const unsigned GS = spmv_blocks(K), GB = vec_blocks(n);
+  const unsigned GA = variant == 1 ? spmv_blocks(K) : spmv_blocks1(K);
+  if (variant == 1) {
+    k_pcg_p<<<GB, WG, 0, S(s)>>>(n, z, p[0], st, part, GS, GB);
+    CK_LAUNCH("k_pcg_p");
+    if (K->bs == 3) k_pcg_q<3><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, p[0], q, st, part);
+    else k_pcg_q<1><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, p[0], q, st, part);
+    CK_LAUNCH("k_pcg_q");
+    k_pcg_b<<<GB, WG, 0, S(s)>>>(n, x, r, z, p[0], p[0], q, dinv, st, part, GA, 0);
+    CK_LAUNCH("k_pcg_b");
+  } else {
+    if (K->bs == 3) k_pcg_a<3><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, z, p[0], p[1], q, st, part, GS, GB);
+    else k_pcg_a<1><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, z, p[0], p[1], q, st, part, GS, GB);
+    CK_LAUNCH("k_pcg_a");
+    k_pcg_b<<<GB, WG, 0, S(s)>>>(n, x, r, z, p[0], p[1], q, dinv, st, part, GA, 1);
+    CK_LAUNCH("k_pcg_b");
+  }
   return 0;
 }
 
 int ugk_pcg_iter_a(const ugk_mat *K, const double *z, const double *pold, double *pnew, double *q,
                    ugk_state *st, double *part, void *s)
 {
+  // the fused kernel as a solve runs it, from a state with iter = 1
+  (void)pold;
+  const unsigned GA = spmv_blocks1(K);
   if (K->bs == 3)
-    k_pcg_a<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
+    k_pcg_a<3><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, z, pnew, (double *)pold, q, st, part, GA, GA);
   else
-    k_pcg_a<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, z, pold, pnew, q, st, part);
+    k_pcg_a<1><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, z, pnew, (double *)pold, q, st, part, GA, GA);
   CK_LAUNCH("k_pcg_a");
+  return 0;
+}
+
+int ugk_cg1_start(const ugk_mat *K, long n, const double *b, const double *x, double *r, double *u, double *p,
+                  double *sv, const double *dinv, double *part, void *s)
+{
+  (void)n;
+  // the start's partials share the parity slot sized for vector grids
+  if (spmv_blocks(K) > 16384) { snprintf(g_err, sizeof(g_err), "mesh too large for the single-reduction form"); return -1; }
+  if (K->bs == 3)
+    k_cg1_start<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, u, p, sv, dinv, part);
+  else
+    k_cg1_start<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, u, p, sv, dinv, part);
+  CK_LAUNCH("k_cg1_start");
+  return 0;
+}
+
+int ugk_cg1_iter(const ugk_mat *K, long n, double *x, double *r, double *u, double *p, double *sv, double *w,
+                 const double *dinv, ugk_state *st, double *part, void *s)
+{
+  const unsigned G0 = spmv_blocks(K), GA = spmv_blocks(K), GB = vec_blocks(n);
+  if (K->bs == 3) k_cg1_w<3><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, u, w, st, part);
+  else k_cg1_w<1><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals, u, w, st, part);
+  CK_LAUNCH("k_cg1_w");
+  k_cg1_upd<<<GB, WG, 0, S(s)>>>(n, x, r, u, p, sv, w, dinv, st, part, G0, GB, GA);
+  CK_LAUNCH("k_cg1_upd");
+  return 0;
+}
+
+int ugk_cg1_iter_w(const ugk_mat *K, const double *u, double *w, ugk_state *st, double *part, void *s)
+{
+  if (K->bs == 3) k_cg1_w<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, u, w, st, part);
+  else k_cg1_w<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, u, w, st, part);
+  CK_LAUNCH("k_cg1_w");
+  return 0;
+}
+
+int ugk_pcg_iter_q(const ugk_mat *K, const double *p, double *q, ugk_state *st, double *part, void *s)
+{
+  if (K->bs == 3) k_pcg_q<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, p, q, st, part);
+  else k_pcg_q<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, p, q, st, part);
+  CK_LAUNCH("k_pcg_q");
   return 0;
 }
 
@@ -1409,55 +1825,72 @@ int ugk_pcg_persistent(const ugk_mat *K, long n, double *x, double *r, double *z
                        void *s)
 {
   int per_cu = 0;
-  const void *fn = K->bs == 3 ? (const void *)k_pcg_persistent<3> : (const void *)k_pcg_persistent<1>;
+  const char *env = getenv("UFEM_GRID_SYNC");
+  const bool cgs = env && env[0] == 'c';
+  const void *fn = K->bs == 3 ? (cgs ? (const void *)k_pcg_persistent<3, true> : (const void *)k_pcg_persistent<3, false>)
+                              : (cgs ? (const void *)k_pcg_persistent<1, true> : (const void *)k_pcg_persistent<1, false>);
   CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, 0));
   if (per_cu < 1) { snprintf(g_err, sizeof(g_err), "persistent kernel does not fit"); return -1; }
   if (per_cu > 4) per_cu = 4;
   unsigned G = (unsigned)(per_cu * g_ncu);
   SellView M = view(K);
   const double *vals = K->d_vals;
+  unsigned GS = spmv_blocks(K);
+  if (G > 16384) G = 16384;
   void *args[] = {(void *)&M, (void *)&vals, (void *)&n, (void *)&x, (void *)&r, (void *)&z,
-                  (void *)&p0, (void *)&p1, (void *)&q, (void *)&dinv, (void *)&st, (void *)&part};
+                  (void *)&p0, (void *)&p1, (void *)&q, (void *)&dinv, (void *)&st, (void *)&part,
+                  (void *)&GS};
   CK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(WG), args, 0, S(s)));
   return 0;
 }
 
-// ---- FP32 inner PCG + refinement -------------------------------------------
-int ugk_pcg32_start(const ugk_mat *K, long n, const double *r64, double scale, float *r, float *z,
-                    float *p, float *x, const float *dinv32, ugk_state *st, double *part, void *s)
+// ---- mixed precision -------------------------------------------------------
+int ugk_mx_start(const ugk_mat *K, long n, const double *b, const double *x64, double *r64, float *r32,
+                 float *z32, float *p32, float *x32, const float *dinv32, double *part, void *s)
 {
-  (void)K;
-  k_pcg32_start<3><<<vec_blocks(n), WG, 0, S(s)>>>(n, r64, scale, r, z, p, x, dinv32, st, part);
-  CK_LAUNCH("k_pcg32_start");
+  (void)n;
+  if (K->bs == 3)
+    k_mx_start<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, r32, z32, p32, x32, dinv32,
+                                                   part);
+  else
+    k_mx_start<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, r32, z32, p32, x32, dinv32,
+                                                   part);
+  CK_LAUNCH("k_mx_start");
   return 0;
 }
-int ugk_pcg32_iter(const ugk_mat *K, long n, float *x, float *r, float *z, float **p, float *q,
-                   const float *dinv32, ugk_state *st, double *part, long it, void *s)
+
+int ugk_mx_iter(const ugk_mat *K, long n, const double *b, double *x64, double *r64, float *r32, float *z32,
+                float *p32, float *q32, float *x32, const float *dinv32, ugk_state *st, double *part,
+                void *s)
 {
-  float *pold = p[it & 1], *pnew = p[(it + 1) & 1];
-  if (K->bs == 3)
-    k_pcg32_a<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, z, pold, pnew, q, st, part);
-  else
-    k_pcg32_a<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, z, pold, pnew, q, st, part);
-  CK_LAUNCH("k_pcg32_a");
-  k_pcg32_b<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, z, pnew, q, dinv32, st, part);
-  CK_LAUNCH("k_pcg32_b");
+  const unsigned GS = spmv_blocks(K), GA = spmv_blocks(K), GR = spmv_blocks(K), GB = vec_blocks(n);
+  k_mx_p<<<GB, WG, 0, S(s)>>>(n, z32, p32, st, part, GS, GB, GR);
+  CK_LAUNCH("k_mx_p");
+  if (K->bs == 3) k_mx_q<3><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals32, p32, q32, st, part);
+  else k_mx_q<1><<<GA, WG, 0, S(s)>>>(view(K), K->d_vals32, p32, q32, st, part);
+  CK_LAUNCH("k_mx_q");
+  k_mx_b<<<GB, WG, 0, S(s)>>>(n, x32, r32, z32, p32, q32, dinv32, st, part, GA);
+  CK_LAUNCH("k_mx_b");
+  k_mx_ru<<<GB, WG, 0, S(s)>>>(n, x64, x32, st, part, GB);
+  CK_LAUNCH("k_mx_ru");
+  if (K->bs == 3) k_mx_r<3><<<GR, WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, r32, z32, dinv32, st, part);
+  else k_mx_r<1><<<GR, WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, r32, z32, dinv32, st, part);
+  CK_LAUNCH("k_mx_r");
   return 0;
 }
-int ugk_ir_update(const ugk_mat *K, long n, double *x64, const float *x32, double scale, const double *b,
-                  double *r64, ugk_state *st, double *part, void *s)
+
+int ugk_residual(const ugk_mat *K, long n, const double *b, const double *x, double *r, ugk_state *st,
+                 double *part, void *s)
 {
-  if (x32) {
-    k_ir_axpy<3><<<vec_blocks(n), WG, 0, S(s)>>>(n, x64, x32, scale);
-    CK_LAUNCH("k_ir_axpy");
-  }
+  (void)n;
   if (K->bs == 3)
-    k_residual<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, st, part);
+    k_residual<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, st, part);
   else
-    k_residual<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x64, r64, st, part);
+    k_residual<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, st, part);
   CK_LAUNCH("k_residual");
   return 0;
 }
+
 int ugk_to_f32(long n, const double *a, float *b, void *s)
 {
   k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, a, b);

@@ -20,7 +20,7 @@ P_I = C.POINTER(C.c_int)
 
 AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
 SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3}
-EXEC = {"streamed": 0, "persistent": 1}
+EXEC = {"streamed": 0, "persistent": 1, "fused": 2, "classic": 3}
 
 
 class tMatrix(C.Structure):
@@ -237,8 +237,8 @@ class GpuSystem:
         return y
 
     def time_kernel(self, mode="spmv", reps=20):
-        """(avg ms, algorithmic bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a'."""
-        m = {"spmv": 0, "spmv32": 1, "pcg_a": 2}[mode]
+        """(avg ms, algorithmic bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a', 'pcg_q', 'cg1_w'."""
+        m = {"spmv": 0, "spmv32": 1, "pcg_a": 2, "pcg_q": 3, "cg1_w": 4}[mode]
         ms = C.c_double(0)
         by = C.c_double(0)
         self._chk(lib().ufem_sys_time_spmv(self.h, reps, m, C.byref(ms), C.byref(by)), "time_kernel")
