@@ -107,6 +107,35 @@ int  ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *byt
 /* phase device times of the last calls: fill_K, add_disps, solve (ms) */
 void ufem_sys_timings(const ufem_sys *s, double t[3]);
 
+/* ---- many samples in one persistent launch ------------------------------
+ * The reference solves Monte-Carlo samples one at a time through its plugin
+ * (monte_solution -> monte_linear_solver, fem/fem_mont.c:480-511, :694-855).
+ * A batch takes n meshes; those that share connectivity, coordinates, loads
+ * and supports share one device pattern, every sample keeps its own E, nu,
+ * K values and Krylov vectors.  One launch solves all samples with Jacobi
+ * PCG (x0 = 0, ||r|| <= eps ||f|| each), a workgroup per sample at a time,
+ * samples dealt largest first. */
+typedef struct ufem_batch ufem_batch;
+ufem_batch *ufem_batch_create(int n, const ufem_mesh *meshes);
+void ufem_batch_destroy(ufem_batch *b);
+int  ufem_batch_nclasses(const ufem_batch *b);
+long ufem_batch_ndof(const ufem_batch *b, int i);
+/* element matrices, loads and supports of every sample; device ms out */
+int  ufem_batch_assemble(ufem_batch *b, int npts, double *ms);
+/* pick the launch shape (workgroup size, workgroups, team size of the
+ * largest meshes) by timing each candidate on a budget of iterations per
+ * sample; cached in `cache` (a text file; NULL: no cache) */
+int  ufem_batch_autotune(ufem_batch *b, const char *cache, int budget_iters, int *wg, int *grid);
+void ufem_batch_shape(const ufem_batch *b, int *wg, int *grid, int *team_big);
+void ufem_batch_set_shape(ufem_batch *b, int wg, int grid, int team_big);
+/* workgroups that solved each sample in the last solve */
+int  ufem_batch_teams(const ufem_batch *b, int *team);
+/* solve all samples; per-sample outputs may be NULL; maxit <= 0: largest n.
+ * sample_ms: device wall-clock time from a sample's start to its end */
+int  ufem_batch_solve(ufem_batch *b, double eps, long maxit, double *total_ms, long *iters,
+                      double *relres, int *status, double *sample_ms);
+int  ufem_batch_get_u(ufem_batch *b, int i, double *u);
+
 /* element stiffness matrices of n bricks (xyz: n*24 in node order), dense
  * row-major 24x24 each -- e009_stiff Mode=AF_NO, fem/fem_e009.c:261-635 */
 int  ufem_gpu_element_stiffness(long n, const double *xyz, const double *E, const double *nu,

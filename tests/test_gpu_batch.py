@@ -1,0 +1,80 @@
+"""Batched solve: every sample of one persistent launch matches its own
+single-system solve (and the oracle), whatever the mesh mix."""
+
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import ufem_oracle as uo
+from ufem_amd import gpu as G
+from ufem_amd.mesh import batch_meshes, cantilever, lognormal_E
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_matches_single_solves():
+    meshes = batch_meshes(12, seed=3, sizes=((8, 2, 2), (12, 3, 3), (16, 4, 4)))
+    b = G.GpuBatch(meshes)
+    assert b.nclasses <= 3
+    b.assemble(2)
+    res = b.solve(eps=1e-12)
+    print("batch", res["rv"], res["iters"], res["wg"], res["grid"])
+    assert res["rv"] == 0 and np.all(res["status"] == 1)
+    for i, m in enumerate(meshes):
+        g = G.GpuSystem(m)
+        g.assemble(2)
+        info = g.solve("pcg", eps=1e-12, rule=1)
+        ub, us = b.u(i), g.u()
+        assert np.linalg.norm(ub - us) <= 1e-9 * np.linalg.norm(us), i
+        assert abs(int(res["iters"][i]) - info["iters"]) <= 10
+    # one sample against the oracle
+    o = uo.OracleSystem(meshes[0])
+    o.assemble(2)
+    o.solve(0, 1e-14, 100000, 1)
+    assert np.linalg.norm(b.u(0) - o.u()) <= 1e-9 * np.linalg.norm(o.u())
+
+
+def test_batch_autotune_and_cache():
+    meshes = batch_meshes(10, seed=5, sizes=((8, 2, 2), (16, 4, 4)))
+    b = G.GpuBatch(meshes)
+    b.assemble(2)
+    with tempfile.TemporaryDirectory() as td:
+        cache = os.path.join(td, "tune.txt")
+        wg, grid = b.autotune(cache, budget_iters=20)
+        assert wg in (256, 512, 1024) and grid > 0
+        fields = open(cache).read().strip().split("\t")[-1].split()
+        assert int(fields[0]) == wg and int(fields[1]) == grid and int(fields[2]) in (1, 2, 4, 8)
+        b2 = G.GpuBatch(meshes)
+        b2.assemble(2)
+        assert b2.autotune(cache, budget_iters=20) == (wg, grid)   # from the cache
+        res = b2.solve(eps=1e-10)
+        assert res["rv"] == 0 and res["wg"] == wg
+
+
+def test_batch_reports_unconverged():
+    meshes = [cantilever(10, 2, 2, E=lognormal_E(1, 40)), cantilever(6, 2, 2)]
+    b = G.GpuBatch(meshes)
+    b.assemble(2)
+    res = b.solve(eps=1e-12, maxit=3)
+    assert res["rv"] == -1 and np.all(res["status"] == 3) and np.all(res["iters"] == 3)
+
+
+@pytest.mark.parametrize("team", [2, 4, 8])
+def test_batch_teams_agree(team):
+    """Samples solved by teams of workgroups (team barrier, team partial sums)
+    give the same solution as one workgroup per sample."""
+    meshes = batch_meshes(9, seed=11, sizes=((8, 2, 2), (24, 6, 6)))
+    b = G.GpuBatch(meshes)
+    b.assemble(2)
+    b.set_shape(512, 256, 1)
+    r1 = b.solve(eps=1e-11)
+    u1 = [b.u(i) for i in range(len(meshes))]
+    b.set_shape(512, 256, team)
+    r2 = b.solve(eps=1e-11)
+    assert r1["rv"] == 0 and r2["rv"] == 0
+    assert r2["team"].max() == team
+    for i in range(len(meshes)):
+        assert np.linalg.norm(b.u(i) - u1[i]) <= 1e-9 * np.linalg.norm(u1[i])
+        assert abs(int(r1["iters"][i]) - int(r2["iters"][i])) <= 3

@@ -192,6 +192,46 @@ int ugk_bicgs_iter(const ugk_mat *K, long n, double *d_x, double *d_r, const dou
                    double *d_t, const double *d_dinv, ugk_state *d_st, double *d_part,
                    void *stream);
 
+/* ---------------- batched solve: one persistent launch ------------------- */
+/* one sample: its K (pattern shared within a mesh class, values own), Jacobi
+ * diagonal inverse, right-hand side and Krylov vectors (device pointers) */
+typedef struct {
+  const int *slen;
+  const long *soff;
+  const int *cols;
+  long n_brows, n_slices;
+  const double *vals;
+  const double *dinv;
+  const double *f;
+  double *x, *r, *u, *p, *s, *w;
+  long maxit;         /* per-sample iteration cap (0: the launch's) */
+} ugk_bsample;
+
+typedef struct {
+  long iters;
+  double relres;      /* recursive ||r|| / ||f|| at exit */
+  int status;         /* 1 converged, 2 breakdown, 3 maxit */
+  int team;           /* workgroups that solved it */
+  long t_start, t_end;   /* device wall clock ticks */
+} ugk_bresult;
+
+/* Every workgroup repeatedly takes the next slot (a ticket counter over
+ * d_slots) and works on that slot's sample as one rank of a team: slot =
+ * {sample, (team size << 8) | rank}, a team's slots adjacent.  Jacobi PCG
+ * (single reduction) with workgroup barriers, and a team barrier on
+ * d_tctr[sample] when the team has more than one workgroup (d_tpart:
+ * 64 doubles per sample).  d_tctr must be zero at launch.  The grid must not
+ * exceed the residency (ugk_batch_occupancy), so that every team forms.
+ * wg: 256, 512 or 1024 threads. */
+int ugk_batch_pcg(const ugk_bsample *d_samples, const int *d_slots, int n_slots, unsigned *d_ticket,
+                  unsigned *d_tctr, double *d_tpart, ugk_bresult *d_res, double tol, long maxit,
+                  int wg, int grid, void *stream);
+/* blocks of `wg` threads the batch kernel can keep resident per CU */
+int ugk_batch_occupancy(int wg);
+/* dinv = 1 / diag(K) for one block-3 matrix (no state) */
+int ugk_diag_inv3(const ugk_mat *K, double *d_dinv, ugk_state *d_st, void *stream);
+long ugk_wall_clock_khz(void);
+
 #ifdef __cplusplus
 }
 #endif

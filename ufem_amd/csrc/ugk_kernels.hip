@@ -1403,6 +1403,188 @@ __global__ __launch_bounds__(WG) void k_bicgs_x(long n, double *x, double *r,
   }
 }
 
+// --------------------------------------------------------------------------
+// Batched solve: samples of different sizes in one persistent launch.  A
+// workgroup owns a sample from start to finish, so all synchronisation is
+// workgroup barriers (no grid-wide barriers, no fences); the ticket counter
+// deals samples out largest first.  Per iteration: w = K u with (w,u), then
+// the single-reduction PCG update of p, s, x, r, u with (r,u), (r,r).
+// --------------------------------------------------------------------------
+template <int WGS, int NV>
+__device__ __forceinline__ void wg_sum(double (&v)[NV], double (*red)[NV])
+{
+  constexpr int W = WGS / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) red[wave][k] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double a = red[0][k];
+    for (int w = 1; w < W; w++) a += red[w][k];
+    v[k] = a;
+  }
+  __syncthreads();
+}
+
+// Team of T workgroups on one sample: the slices and vector entries are
+// dealt round robin over the team; sums go through per-rank partial slots
+// (two buffers, by reduction parity) and a monotonic team counter: barrier k
+// waits until the counter reaches T k.  A team forms only from resident
+// workgroups (the grid never exceeds residency), so every wait ends.
+__device__ __forceinline__ void team_barrier(unsigned int *ctr, unsigned target)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(2);
+  }
+  __syncthreads();
+}
+
+template <int WGS, int NV>
+__device__ __forceinline__ void team_sum(double (&v)[NV], double (*red)[NV], int T, int rank,
+                                         double *tpart, unsigned int *ctr, unsigned &nbar, int &par)
+{
+  wg_sum<WGS, NV>(v, red);
+  if (T == 1) return;
+  double *slot = tpart + (size_t)par * 8 * 4;   // [2][8 ranks][4]
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) slot[rank * 4 + k] = v[k];
+  }
+  nbar++;
+  team_barrier(ctr, nbar * (unsigned)T);
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double a = 0.0;
+    for (int r = 0; r < T; r++) a += __hip_atomic_load(&slot[r * 4 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v[k] = a;
+  }
+  par ^= 1;
+}
+
+template <int WGS>
+__global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict__ samples,
+                                                   const int2 *__restrict__ slots, int nslots,
+                                                   unsigned *ticket, unsigned int *tctr,
+                                                   double *tpart, ugk_bresult *res, double tol,
+                                                   long maxit)
+{
+  constexpr int W = WGS / 64;
+  __shared__ int s_k;
+  __shared__ double red3[W][3];
+  __shared__ double red1[W][1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (;;) {
+    if (threadIdx.x == 0) s_k = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int k = s_k;
+    __syncthreads();
+    if (k >= nslots) return;
+    const int id = slots[k].x;            // sample
+    const int T = slots[k].y >> 8;        // team size
+    const int rank = slots[k].y & 255;    // rank in the team
+    const ugk_bsample S = samples[id];
+    const long mxit = S.maxit > 0 ? S.maxit : maxit;
+    unsigned int *ctr = tctr + id;
+    double *tp = tpart + (size_t)id * 2 * 8 * 4;
+    unsigned nbar = 0;
+    int par = 0;
+    const long n = S.n_brows * 3;
+    const long e0 = (long)rank * WGS + threadIdx.x, estep = (long)T * WGS;
+    const long w0 = (long)rank * W + wave, wstep = (long)T * W;
+    long t0 = 0;
+    if (threadIdx.x == 0 && rank == 0) t0 = wall_clock64();
+    double v3[3] = {0, 0, 0};
+    for (long i = e0; i < n; i += estep) {
+      const double ri = S.f[i];
+      const double ui = ri * S.dinv[i];
+      S.x[i] = 0.0;
+      S.r[i] = ri;
+      S.u[i] = ui;
+      S.p[i] = 0.0;
+      S.s[i] = 0.0;
+      v3[0] += ri * ui;
+      v3[1] += ri * ri;
+    }
+    team_sum<WGS, 3>(v3, red3, T, rank, tp, ctr, nbar, par);   // also publishes u
+    double gam = v3[0], rr = v3[1], gam_old = 1.0, alf_old = 1.0;
+    const double nb = sqrt(rr);
+    long it = 0;
+    int status = 0;
+    for (;;) {
+      if (!(nb > 0.0) || sqrt(rr) <= tol * nb) { status = 1; break; }
+      if (!(gam == gam) || !(rr == rr)) { status = 2; break; }
+      if (it >= mxit) { status = 3; break; }
+      double v1[1] = {0.0};
+      for (long sl = w0; sl < S.n_slices; sl += wstep) {
+        const int L = S.slen[sl];
+        const long so = S.soff[sl];
+        const int *cp = S.cols + so * 64 + lane;
+        const double *vp = S.vals + so * 64 * 9 + lane;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int s = 0; s < L; s++) {
+          const long J = ldnt(cp + (long)s * 64);
+          const double x0 = S.u[3 * J], x1 = S.u[3 * J + 1], x2 = S.u[3 * J + 2];
+          const double *vv = vp + (long)s * 576;
+          a0 += ldnt(vv) * x0 + ldnt(vv + 64) * x1 + ldnt(vv + 128) * x2;
+          a1 += ldnt(vv + 192) * x0 + ldnt(vv + 256) * x1 + ldnt(vv + 320) * x2;
+          a2 += ldnt(vv + 384) * x0 + ldnt(vv + 448) * x1 + ldnt(vv + 512) * x2;
+        }
+        const long row = sl * 64 + lane;
+        if (row < S.n_brows) {
+          S.w[3 * row] = a0;
+          S.w[3 * row + 1] = a1;
+          S.w[3 * row + 2] = a2;
+          v1[0] += a0 * S.u[3 * row] + a1 * S.u[3 * row + 1] + a2 * S.u[3 * row + 2];
+        }
+      }
+      team_sum<WGS, 1>(v1, red1, T, rank, tp, ctr, nbar, par);   // publishes w
+      const double dlt = v1[0];
+      const double beta = (it == 0) ? 0.0 : gam / gam_old;
+      const double den = (it == 0) ? dlt : dlt - beta * gam / alf_old;
+      const double alpha = gam / den;
+      if (!(den > 0.0) || !(alpha == alpha)) { status = 2; break; }
+      double w3[3] = {0, 0, 0};
+      for (long i = e0; i < n; i += estep) {
+        const double pi = S.u[i] + beta * S.p[i];
+        const double si = S.w[i] + beta * S.s[i];
+        const double ri = S.r[i] - alpha * si;
+        const double ui = ri * S.dinv[i];
+        S.p[i] = pi;
+        S.s[i] = si;
+        S.x[i] += alpha * pi;
+        S.r[i] = ri;
+        S.u[i] = ui;
+        w3[0] += ri * ui;
+        w3[1] += ri * ri;
+      }
+      team_sum<WGS, 3>(w3, red3, T, rank, tp, ctr, nbar, par);   // publishes u
+      gam_old = gam;
+      alf_old = alpha;
+      gam = w3[0];
+      rr = w3[1];
+      it++;
+    }
+    if (threadIdx.x == 0 && rank == 0) {
+      ugk_bresult R;
+      R.iters = it;
+      R.relres = nb > 0.0 ? sqrt(rr) / nb : 0.0;
+      R.status = status;
+      R.team = T;
+      R.t_start = t0;
+      R.t_end = wall_clock64();
+      res[id] = R;
+    }
+    __syncthreads();
+  }
+}
+
 // ==========================================================================
 // C-ABI
 // ==========================================================================
@@ -1928,6 +2110,42 @@ int ugk_bicgs_iter(const ugk_mat *K, long n, double *x, double *r, const double 
   k_bicgs_x<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, rhat, phat, sv, shat, t, st, part);
   CK_LAUNCH("k_bicgs_x");
   return 0;
+}
+
+// ---- batched solve -----------------------------------------------------------
+int ugk_batch_occupancy(int wg)
+{
+  int per = 0;
+  const void *fn = wg == 256 ? (const void *)k_batch_pcg<256>
+                 : (wg == 512 ? (const void *)k_batch_pcg<512> : (const void *)k_batch_pcg<1024>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, wg, 0) != hipSuccess) return 0;
+  return per;
+}
+
+int ugk_batch_pcg(const ugk_bsample *samples, const int *slots, int nslots, unsigned *ticket, unsigned *tctr,
+                  double *tpart, ugk_bresult *res, double tol, long maxit, int wg, int grid, void *s)
+{
+  if (grid < 1) grid = g_ncu;
+  const int2 *sl = (const int2 *)slots;
+  if (wg == 256) k_batch_pcg<256><<<grid, 256, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit);
+  else if (wg == 512) k_batch_pcg<512><<<grid, 512, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit);
+  else if (wg == 1024) k_batch_pcg<1024><<<grid, 1024, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit);
+  else { snprintf(g_err, sizeof(g_err), "workgroup size %d not built", wg); return -1; }
+  CK_LAUNCH("k_batch_pcg");
+  return 0;
+}
+
+int ugk_diag_inv3(const ugk_mat *K, double *d_dinv, ugk_state *d_st, void *s)
+{
+  return ugk_diag_inv(K, d_dinv, d_st, s);
+}
+
+long ugk_wall_clock_khz(void)
+{
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0;
+  return khz;
 }
 
 }  // extern "C"

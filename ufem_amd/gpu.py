@@ -89,6 +89,17 @@ def lib():
             "ufem_gpu_last_solve": (None, [C.POINTER(SolveInfo)]),
             "ufem_gpu_last_error": (C.c_char_p, []),
             "ufem_gpu_device_info": (C.c_int, [C.c_char_p, C.c_int, P_I, P_L]),
+            "ufem_batch_create": (C.c_void_p, [C.c_int, C.POINTER(_Mesh)]),
+            "ufem_batch_destroy": (None, [C.c_void_p]),
+            "ufem_batch_nclasses": (C.c_int, [C.c_void_p]),
+            "ufem_batch_ndof": (C.c_long, [C.c_void_p, C.c_int]),
+            "ufem_batch_assemble": (C.c_int, [C.c_void_p, C.c_int, P_D]),
+            "ufem_batch_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, P_I, P_I]),
+            "ufem_batch_shape": (None, [C.c_void_p, P_I, P_I, P_I]),
+            "ufem_batch_set_shape": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+            "ufem_batch_teams": (C.c_int, [C.c_void_p, P_I]),
+            "ufem_batch_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_long, P_D, P_L, P_D, P_I, P_D]),
+            "ufem_batch_get_u": (C.c_int, [C.c_void_p, C.c_int, P_D]),
             "femEqsCGwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
                                      C.c_double, C.c_long]),
             "femEqsBiCCSwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
@@ -248,6 +259,85 @@ class GpuSystem:
         t = np.zeros(3)
         lib().ufem_sys_timings(self.h, _pd(t))
         return {"fill_ms": t[0], "bc_ms": t[1], "solve_ms": t[2]}
+
+
+def _mesh_struct(mesh, keep):
+    arrs = [np.ascontiguousarray(mesh.coords[:, i], dtype=np.float64) for i in range(3)]
+    arrs += [np.ascontiguousarray(mesh.enodes.reshape(-1), dtype=np.int64),
+             np.ascontiguousarray(mesh.E, dtype=np.float64),
+             np.ascontiguousarray(mesh.nu, dtype=np.float64),
+             np.ascontiguousarray(mesh.nl_node, dtype=np.int64),
+             np.ascontiguousarray(mesh.nl_type, dtype=np.int64),
+             np.ascontiguousarray(mesh.nl_dir, dtype=np.int64),
+             np.ascontiguousarray(mesh.nl_val, dtype=np.float64)]
+    keep.append(arrs)
+    k = arrs
+    return _Mesh(mesh.n_nodes, _pd(k[0]), _pd(k[1]), _pd(k[2]), mesh.n_elems, _pl(k[3]), _pd(k[4]),
+                 _pd(k[5]), k[6].size, _pl(k[6]), _pl(k[7]), _pl(k[8]), _pd(k[9]))
+
+
+class GpuBatch:
+    """Many samples solved in one persistent launch (ufem_batch_*)."""
+
+    def __init__(self, meshes):
+        keep = []
+        arr = (_Mesh * len(meshes))(*[_mesh_struct(m, keep) for m in meshes])
+        self.h = lib().ufem_batch_create(len(meshes), arr)
+        if not self.h:
+            raise RuntimeError("ufem_batch_create: " + last_error())
+        self.n = len(meshes)
+        self.nclasses = lib().ufem_batch_nclasses(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ufem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assemble(self, npts=2):
+        ms = C.c_double(0)
+        rv = lib().ufem_batch_assemble(self.h, npts, C.byref(ms))
+        if rv != 0:
+            raise RuntimeError(f"batch assemble failed ({rv}): {last_error()}")
+        return ms.value
+
+    def autotune(self, cache=None, budget_iters=100):
+        wg, grid = C.c_int(0), C.c_int(0)
+        rv = lib().ufem_batch_autotune(self.h, cache.encode() if cache else None, budget_iters,
+                                       C.byref(wg), C.byref(grid))
+        if rv != 0:
+            raise RuntimeError(f"autotune failed ({rv}): {last_error()}")
+        return wg.value, grid.value
+
+    def solve(self, eps=1e-8, maxit=0):
+        n = self.n
+        ms = C.c_double(0)
+        it = np.zeros(n, dtype=np.int64)
+        rr = np.zeros(n)
+        st = np.zeros(n, dtype=np.int32)
+        sm = np.zeros(n)
+        rv = lib().ufem_batch_solve(self.h, eps, maxit, C.byref(ms), _pl(it), _pd(rr), _pi(st), _pd(sm))
+        wg, grid, tb = C.c_int(0), C.c_int(0), C.c_int(0)
+        lib().ufem_batch_shape(self.h, C.byref(wg), C.byref(grid), C.byref(tb))
+        team = np.zeros(n, dtype=np.int32)
+        lib().ufem_batch_teams(self.h, _pi(team))
+        return {"rv": rv, "total_ms": ms.value, "iters": it, "relres": rr, "status": st, "sample_ms": sm,
+                "wg": wg.value, "grid": grid.value, "team_big": tb.value, "team": team}
+
+    def set_shape(self, wg, grid, team_big):
+        lib().ufem_batch_set_shape(self.h, wg, grid, team_big)
+
+    def u(self, i):
+        out = np.zeros(lib().ufem_batch_ndof(self.h, i))
+        rv = lib().ufem_batch_get_u(self.h, i, _pd(out))
+        if rv != 0:
+            raise RuntimeError(last_error())
+        return out
 
 
 def element_stiffness(xyz, E, nu, npts=2):

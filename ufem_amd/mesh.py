@@ -184,6 +184,20 @@ def config_mesh(name: str, seed: int = 2024) -> Mesh:
     raise KeyError(name)
 
 
+BATCH_SIZES = ((32, 8, 8), (48, 12, 12), (64, 16, 16), (96, 24, 24))
+
+
+def batch_meshes(n: int = 256, seed: int = 7, sizes=BATCH_SIZES, cov: float = 0.3):
+    """Monte-Carlo batch: each sample's mesh drawn uniformly from `sizes`
+    (same 2.0 x 0.4 x 0.4 m cantilever), its own i.i.d. lognormal E field."""
+    pick = (counter_normal(seed, np.arange(n), stream=21) * 1e6).astype(np.int64) % len(sizes)
+    out = []
+    for i in range(n):
+        nx, ny, nz = sizes[int(pick[i])]
+        out.append(cantilever(nx, ny, nz, E=lognormal_E(seed * 1000003 + i, nx * ny * nz, cov=cov)))
+    return out
+
+
 def write_fem_input(mesh: Mesh, path: str, material_per_element: bool = True) -> None:
     """Write `mesh` in the reference's preparsed input format (fem/feminput.5,
     read by femReadInput fem/fem_io.c:380-640 with -p): material type 1 with
