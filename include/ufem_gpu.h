@@ -70,6 +70,7 @@ typedef struct {
   long inner_maxit;  /* mixed: ... and at least every this many iterations (0: 500) */
   void *net;         /* learned: network handle from ufem_net_*           */
   int warm_start;    /* nonzero: start from the current u                 */
+  int fcg_m;         /* learned: directions kept by flexible CG (1..8; 0: 4) */
 } ufem_solve_opts;
 
 typedef struct {
@@ -88,6 +89,9 @@ long ufem_sys_ndof(const ufem_sys *s);
 long ufem_sys_nnz(const ufem_sys *s);       /* scalar entries of K, explicit zeros kept */
 long ufem_sys_ncolors(const ufem_sys *s);
 long ufem_sys_nreact(const ufem_sys *s);
+/* new Young's moduli, one per element, on the same mesh (Monte-Carlo and
+ * training fields); fill_K and add_disps must follow */
+int  ufem_sys_set_E(ufem_sys *s, const double *E);
 int  ufem_sys_fill_K(ufem_sys *s, int npts);    /* npts = 2 (exact for parallelepipeds) or 3 */
 int  ufem_sys_add_loads(ufem_sys *s);
 int  ufem_sys_add_disps(ufem_sys *s);
@@ -135,6 +139,65 @@ int  ufem_batch_teams(const ufem_batch *b, int *team);
 int  ufem_batch_solve(ufem_batch *b, double eps, long maxit, double *total_ms, long *iters,
                       double *relres, int *status, double *sample_ms);
 int  ufem_batch_get_u(ufem_batch *b, int i, double *u);
+
+/* ---- learned preconditioner -----------------------------------------------
+ * No counterpart in the reference: an accelerator for the Jacobi-PCG solve
+ * of a box mesh of W x H x D nodes numbered x fastest (node = x + W (y + H z),
+ * DOF 3 node + dir - 1, as ufem_sys numbers them).  Flexible CG with
+ *   z = dinv r + mask dsq N(mask dsq r),  dsq = sqrt(dinv),
+ * where N is a symmetric 3-level 3-D convolution U-Net (3x3x3 taps, no
+ * bias): C0 3->16 channels, C1 16->32 and C2 32->64 with stride 2 (levels of
+ * (n + 1) / 2 nodes per axis), N(v) = C0^T (g0 h0 + C1^T (g1 h1 +
+ * C2^T (g2 h2))), h0 = C0 v, h1 = C1 h0, h2 = C2 h1, and per-node, per-channel
+ * gates g_l = exp(b_l - (1 + a_l) ebar_l) of the log-modulus log(E / 30 GPa)
+ * (node average of the bricks, then 3x3x3 means per coarser level).  N is
+ * linear, symmetric and positive semi-definite, so M is SPD.  Jacobi PCG
+ * takes over if a direction fails to descend.  Parameters (f32, packed):
+ * W0 [27][16][3], W1 [27][32][16], W2 [27][64][32], then a0, b0 (16), a1, b1
+ * (32), a2, b2 (64); taps t = 9 (dz+1) + 3 (dy+1) + (dx+1).  Weights enter
+ * the matrix cores as bf16, activations as f32 (exact bf16 splits). */
+typedef struct ufem_net ufem_net;
+ufem_net *ufem_net_create(int W, int H, int D, int batch);  /* batch: training samples per step */
+void ufem_net_destroy(ufem_net *N);
+long ufem_net_nparams(void);
+int  ufem_net_init(ufem_net *N, unsigned long seed);          /* W ~ N(0, 1/(27 cin)), gates 0 */
+int  ufem_net_set_weights(ufem_net *N, const float *w);
+int  ufem_net_get_weights(ufem_net *N, float *w);
+/* y = N(x) for nb samples: x [nb][V][3], ebar0 [nb][V] (node log-moduli),
+ * y [nb][V][3] (V = W H D) */
+int  ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, float *y);
+/* gradient of sum <dy, N(x)> with respect to the packed parameters */
+int  ufem_net_backward(ufem_net *N, int nb, const float *x, const float *eb0, const float *dy, float *grad);
+/* attach to a system and its current E per element (mask of fixed DOFs,
+ * gate inputs); required by collect, train and UFEM_SOLVER_LEARNED */
+int  ufem_net_bind(ufem_net *N, ufem_sys *s, const double *E);
+/* room for `cap` training pairs on the device */
+int  ufem_net_reserve(ufem_net *N, long cap);
+long ufem_net_npairs(const ufem_net *N);
+/* solve the bound system with Jacobi PCG to eps (relative), snapshot (x, r)
+ * every `every` iterations, keep the field (FP32 K, dinv, gate inputs) and
+ * add up to `pairs` evenly spread pairs (r, e = u - x); returns the pairs
+ * added (< 0: AF_* error) */
+long ufem_net_collect(ufem_net *N, int pairs, double eps, int every, ufem_solve_info *info);
+/* pair i as stored (scaled by 1 / rms(dsq r)): network input v [V][4],
+ * r and e [3 V], e.r, and the index of its material field */
+int  ufem_net_get_pair(ufem_net *N, long i, float *v, float *rr, float *ee, double *er, int *field);
+/* energy loss (e - z)^T (r - K z) / (e^T r) of pairs [i0, i0 + cnt) */
+int  ufem_net_eval(ufem_net *N, long i0, long cnt, double *loss);
+/* Adam (0.9, 0.999) on the mean energy loss of `batch` random pairs per
+ * step, learning rate decaying from lr to lr / 10 (cosine); loss[steps] and
+ * device ms may be NULL */
+int  ufem_net_train(ufem_net *N, long steps, int batch, double lr, unsigned long seed, double *loss,
+                    double *ms);
+/* one convolution kernel on host data, for tests: y = conv(x) with
+ * w [27][cout][cin] (mode 0 stride 1, 1 stride 2, 2 stride-2 transpose; flip:
+ * taps reversed; x_f32: f32 activations, else bf16), dw = its weight
+ * gradient for dy (modes 0, 1) */
+int  ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin,
+                         int cout, const float *x, const float *w, float *y, const float *dy, float *dw);
+/* average device time (us) of `reps` launches of one such convolution */
+int  ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin, int cout,
+                        int reps, double *us);
 
 /* element stiffness matrices of n bricks (xyz: n*24 in node order), dense
  * row-major 24x24 each -- e009_stiff Mode=AF_NO, fem/fem_e009.c:261-635 */

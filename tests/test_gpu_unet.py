@@ -1,0 +1,275 @@
+"""The learned preconditioner's kernels against the FP64 oracle
+(oracle/unet_oracle.py), and flexible CG with the network.
+
+* single convolution kernels on integer-valued data (exact in bf16, sums
+  exact in f32): every mode, the tap flip and the weight gradient must match
+  the oracle bit for bit; f32 activations (three-way bf16 split) must match
+  the bf16 path on bf16-exact data and the FP64 oracle to 2e-6 otherwise;
+* the whole network on bf16-rounded weights: forward within 1e-5 (f32
+  arithmetic); parameter gradient within 3e-2 for the weights (the weight
+  gradient kernel rounds its operands to bf16) and 1e-4 for the gates;
+* the energy loss of stored training pairs within 1e-4 of the oracle's;
+* flexible CG: with N = 0 it is Jacobi PCG (iterations +-2); a non-finite
+  network output switches it to Jacobi PCG (fallback flag, same solution);
+  a trained network needs fewer iterations than Jacobi PCG on a field it
+  was not trained on, to the same solution (1e-6 relative)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import unet_oracle as U
+from ufem_amd import gpu as G
+from ufem_amd.mesh import cantilever, grf_E
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def ints(rng, shape, lo=-3, hi=3):
+    return rng.integers(lo, hi + 1, size=shape).astype(np.float32)
+
+
+SHAPES = [((5, 4, 7), 1, 16, 16), ((6, 3, 9), 2, 4, 32), ((3, 5, 5), 1, 48, 16), ((5, 5, 9), 1, 96, 32),
+          ((4, 4, 4), 3, 64, 64), ((5, 5, 9), 1, 16, 3), ((5, 5, 9), 2, 32, 64)]
+
+
+@pytest.mark.parametrize("x_f32", [0, 1])
+@pytest.mark.parametrize("shape,nb,cin,cout", SHAPES)
+def test_conv_stride1_and_flip_exact(shape, nb, cin, cout, x_f32):
+    rng = np.random.default_rng(cin * 7 + cout)
+    x = ints(rng, (nb,) + shape + (cin,))
+    w = ints(rng, (27, cout, cin), -2, 2) / 4
+    for flip in (0, 1):
+        y = G.probe_conv(0, flip, x.reshape(nb, -1, cin), w, shape, shape, x_f32=x_f32)
+        wr = w[::-1] if flip else w
+        ref = np.stack([U.conv(x[b].astype(np.float64), wr.astype(np.float64), 1) for b in range(nb)])
+        assert np.array_equal(y.reshape(ref.shape), ref.astype(np.float32)), f"flip={flip}"
+
+
+@pytest.mark.parametrize("x_f32", [0, 1])
+@pytest.mark.parametrize("shape,nb,cin,cout", SHAPES[:4] + SHAPES[6:])
+def test_conv_stride2_and_transpose_exact(shape, nb, cin, cout, x_f32):
+    rng = np.random.default_rng(cin + 3 * cout)
+    cs = U.coarse_shape(shape)
+    x = ints(rng, (nb,) + shape + (cin,))
+    w = ints(rng, (27, cout, cin), -2, 2) / 4
+    y = G.probe_conv(1, 0, x.reshape(nb, -1, cin), w, shape, cs, x_f32=x_f32)
+    ref = np.stack([U.conv(x[b].astype(np.float64), w.astype(np.float64), 2) for b in range(nb)])
+    assert np.array_equal(y.reshape(ref.shape), ref.astype(np.float32))
+    # T2: coarse -> fine, y[f] = sum_t W[t] x[(f - off_t) / 2]: the adjoint of
+    # the stride-2 convolution with weights W^T
+    xc = ints(rng, (nb,) + cs + (cin,))
+    yt = G.probe_conv(2, 0, xc.reshape(nb, -1, cin), w, cs, shape, x_f32=x_f32)
+    wT = np.transpose(w, (0, 2, 1)).astype(np.float64)
+    reft = np.stack([U.conv_adjoint(xc[b].astype(np.float64), wT, 2, shape) for b in range(nb)])
+    assert np.array_equal(yt.reshape(reft.shape), reft.astype(np.float32))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_conv_f32_split_accuracy(mode):
+    """f32 activations through the three-way bf16 split: FP64-oracle accuracy
+    to 2e-6 relative (bf16-rounded weights are exact in the oracle)."""
+    rng = np.random.default_rng(mode)
+    fine, cin, cout = (5, 6, 9), 32, 16
+    cs = U.coarse_shape(fine)
+    gi, go = {0: (fine, fine), 1: (fine, cs), 2: (cs, fine)}[mode]
+    x = rng.standard_normal((2,) + gi + (cin,)).astype(np.float32)
+    w = bf16(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
+    y = G.probe_conv(mode, 0, x.reshape(2, -1, cin), w, gi, go, x_f32=True)
+    w64 = w.astype(np.float64)
+    if mode == 2:
+        ref = np.stack([U.conv_adjoint(x[b].astype(np.float64), np.transpose(w64, (0, 2, 1)), 2, go)
+                        for b in range(2)])
+    else:
+        ref = np.stack([U.conv(x[b].astype(np.float64), w64, mode + 1) for b in range(2)])
+    assert rel(y, ref) <= 2e-6
+    # a bf16-rounded input (the bf16 path) is far less accurate
+    yb = G.probe_conv(mode, 0, x.reshape(2, -1, cin), w, gi, go, x_f32=False)
+    assert rel(yb, ref) > 1e-4
+
+
+@pytest.mark.parametrize("stride", [1, 2])
+@pytest.mark.parametrize("shape,nb,cin,cout", SHAPES)
+def test_weight_gradient_exact(stride, shape, nb, cin, cout):
+    rng = np.random.default_rng(5 + cin + cout + stride)
+    os_ = shape if stride == 1 else U.coarse_shape(shape)
+    x = ints(rng, (nb,) + shape + (cin,), -2, 2)
+    w = np.zeros((27, cout, cin), dtype=np.float32)
+    dy = ints(rng, (nb,) + os_ + (cout,), -2, 2)
+    for x_f32 in (0, 1):
+        _, dw = G.probe_conv(stride - 1, 0, x.reshape(nb, -1, cin), w, shape, os_, dy=dy.reshape(nb, -1, cout),
+                             x_f32=x_f32)
+        ref = sum(U.conv_wgrad(x[b].astype(np.float64), dy[b].astype(np.float64), stride) for b in range(nb))
+        assert np.array_equal(dw, ref.astype(np.float32))
+
+
+def pack(P):
+    return np.concatenate([P["W0"].ravel(), P["W1"].ravel(), P["W2"].ravel()] +
+                          [P[f"{k}{l}"] for l in range(3) for k in ("a", "b")]).astype(np.float32)
+
+
+def unpack(v):
+    P, o = {}, 0
+    for name, ci, co, _ in U.LAYERS:
+        P[name] = v[o:o + 27 * co * ci].reshape(27, co, ci).astype(np.float64)
+        o += 27 * co * ci
+    for l, c in enumerate(U.CH):
+        for k in ("a", "b"):
+            P[f"{k}{l}"] = v[o:o + c].astype(np.float64)
+            o += c
+    return P
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    P = U.init_params(seed)
+    for l, c in enumerate(U.CH):
+        P[f"a{l}"] = 0.3 * rng.standard_normal(c)
+        P[f"b{l}"] = 0.3 * rng.standard_normal(c)
+    return unpack(bf16(pack(P)))      # weights exactly as the kernels see them
+
+
+@pytest.fixture(scope="module")
+def small_net():
+    D, H, W = 5, 5, 9
+    return G.GpuNet(W, H, D, batch=2), (D, H, W)
+
+
+def test_network_forward_matches_oracle(small_net):
+    net, shape = small_net
+    rng = np.random.default_rng(0)
+    P = random_params(4)
+    net.set_weights(pack(P))
+    x = rng.standard_normal((2,) + shape + (3,))
+    e0 = 0.5 * rng.standard_normal((2,) + shape)
+    y = net.forward(x.reshape(2, -1, 3), e0.reshape(2, -1))
+    for b in range(2):
+        ref, _ = U.forward(P, x[b], U.ebar_levels(e0[b].astype(np.float32).astype(np.float64)))
+        assert rel(y[b], ref) <= 1e-5
+
+
+def test_network_gradient_matches_oracle(small_net):
+    net, shape = small_net
+    rng = np.random.default_rng(1)
+    P = random_params(5)
+    net.set_weights(pack(P))
+    x = rng.standard_normal((2,) + shape + (3,))
+    e0 = 0.5 * rng.standard_normal((2,) + shape)
+    dy = rng.standard_normal((2,) + shape + (3,))
+    g = unpack(net.backward(x.reshape(2, -1, 3), e0.reshape(2, -1), dy.reshape(2, -1, 3)))
+    ref = {k: 0.0 for k in P}
+    for b in range(2):
+        _, c = U.forward(P, x[b], U.ebar_levels(e0[b].astype(np.float32).astype(np.float64)))
+        gb = U.backward(P, c, dy[b])
+        for k in gb:
+            ref[k] = ref[k] + gb[k]
+    for k in P:
+        a, r = g[k].ravel(), ref[k].ravel()
+        cos = float(a @ r / (np.linalg.norm(a) * np.linalg.norm(r)))
+        tol = 3e-2 if k.startswith("W") else 1e-4
+        assert rel(a, r) <= tol and cos >= 0.999, (k, rel(a, r), cos)
+
+
+def beam_mesh(seed, nx=24, ny=6, nz=6):
+    return cantilever(nx, ny, nz, L=2.0, B=0.5, H=0.5, E=grf_E(seed, nx, ny, nz, (2.0 / nx, 0.5 / ny, 0.5 / nz)))
+
+
+@pytest.fixture(scope="module")
+def beam():
+    m = beam_mesh(3)
+    s = G.GpuSystem(m)
+    s.assemble(2)
+    ref = s.solve("pcg", eps=1e-8)
+    return m, s, ref, s.u(), (m.nx + 1, m.ny + 1, m.nz + 1)
+
+
+def test_energy_loss_matches_oracle(beam):
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D)
+    net.reserve(8)
+    net.bind(s, m.E)
+    P = random_params(7)
+    net.set_weights(pack(P))
+    n_added, _ = net.collect(4, eps=1e-11, every=8)
+    assert n_added >= 3
+    ip, cols, vals = s.K_csr()
+    K = sp.csr_matrix((vals, cols, ip), shape=(s.n, s.n))
+    dinv = 1.0 / K.diagonal()
+    fixed = np.unique(3 * m.nl_node[m.nl_type == 1] + m.nl_dir[m.nl_type == 1] - 1)
+    mask = np.ones(s.n)
+    mask[fixed] = 0.0
+    eb = U.ebar_levels(U.node_log_modulus(m.E, m.nx, m.ny, m.nz).astype(np.float32).astype(np.float64))
+    losses = net.eval(0, n_added)
+    for i in range(n_added):
+        v, rr, ee, er, f = net.pair(i)
+        assert f == 0
+        assert abs(er - float(ee.astype(np.float64) @ rr.astype(np.float64))) <= 1e-5 * er
+        L, _, _ = U.energy_loss(P, K, rr.astype(np.float64), ee.astype(np.float64), dinv, mask, eb, (D, H, W))
+        assert abs(losses[i] - L) <= 1e-4 * max(abs(L), 1e-3), (i, losses[i], L)
+
+
+def test_zero_network_is_jacobi_pcg(beam):
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D)
+    net.bind(s, m.E)
+    net.set_weights(np.zeros(net.P, dtype=np.float32))
+    r = s.solve("learned", eps=1e-8, net=net)
+    assert r["rv"] == 0 and not r["fallback"]
+    assert abs(r["iters"] - ref["iters"]) <= 2
+    assert rel(s.u(), u_ref) <= 1e-6
+
+
+def test_non_finite_network_falls_back_to_jacobi(beam):
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D)
+    net.bind(s, m.E)
+    P = random_params(8)
+    P["b0"][:] = 200.0                    # exp(200) overflows f32: the output is not finite
+    net.set_weights(pack(P))
+    r = s.solve("learned", eps=1e-8, net=net)
+    assert r["rv"] == 0 and r["fallback"] == 1 and r["outer"] == 0
+    assert abs(r["iters"] - ref["iters"]) <= 2
+    assert rel(s.u(), u_ref) <= 1e-6
+
+
+def test_trained_network_beats_jacobi(beam):
+    """Pairs from plain-PCG solves on 8 fresh random fields, 600 Adam steps;
+    on a field not seen in training, flexible CG with the network needs at
+    most 70 % of Jacobi PCG's iterations and reaches the same solution."""
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D, batch=4)
+    net.init(1)
+    net.reserve(256)
+    for f in range(8):
+        mf = beam_mesh(100 + f)
+        s.set_E(mf.E)
+        s.fill_K(2)
+        s.add_disps()
+        net.bind(s, mf.E)
+        net.collect(32, eps=1e-11, every=4)
+    assert net.npairs() >= 150
+    loss, ms = net.train(600, batch=4, lr=3e-3, seed=0)
+    assert np.mean(loss[-50:]) < np.mean(loss[:20])
+    s.set_E(m.E)
+    s.fill_K(2)
+    s.add_disps()
+    net.bind(s, m.E)
+    plain = s.solve("pcg", eps=1e-8)
+    u_plain = s.u()
+    r = s.solve("learned", eps=1e-8, net=net)
+    print(f"plain {plain['iters']} it, learned {r['iters']} it (fallback {r['fallback']}), "
+          f"loss {loss[:20].mean():.3g} -> {loss[-50:].mean():.3g}, train {ms:.0f} ms")
+    assert r["rv"] == 0 and not r["fallback"] and r["true_relres"] <= 2e-8
+    assert rel(s.u(), u_plain) <= 1e-6
+    assert r["iters"] <= 0.7 * plain["iters"]

@@ -19,9 +19,15 @@ typedef struct {
   double *r, *z, *p[2], *q, *dinv;
   double *rhat, *phat, *v, *s, *shat, *t;          /* BiCGStab            */
   float *r32, *z32, *p32[2], *q32, *x32, *dinv32;  /* mixed precision     */
+  double *fP, *fQ;                                 /* flexible CG rings   */
+  int fm;                                          /* ring slots          */
   double *part;                                    /* reduction partials  */
   ugk_state *st;                                   /* device scalars      */
   ugk_state *st_host;                              /* pinned mirror       */
+  /* optional: called by the chunk loop with the iterations enqueued so far
+   * (0 before the first chunk), on the solver's stream */
+  int (*hook)(void *ctx, long it, void *stream);
+  void *hook_ctx;
 } ufem_work;
 
 struct ufem_sys {
@@ -63,6 +69,14 @@ void ufem_work_free(ufem_work *w);
 int ufem_solve_device(const ugk_mat *K, long n, const double *d_b, double *d_x, ufem_work *w,
                       void *stream, void *ev0, void *ev1, const ufem_solve_opts *o,
                       ufem_solve_info *info);
+
+/* learned preconditioner inside flexible CG (ufem_net.c) */
+long ufem_net_bound_n(const ufem_net *N);           /* DOFs of the bound system, 0 if none */
+float *ufem_net_input(ufem_net *N);                /* f32 [V][4], channel 3 stays 0       */
+const float *ufem_net_mask(const ufem_net *N);     /* 0 at fixed DOFs, else 1             */
+const float *ufem_net_output(const ufem_net *N);   /* f32 [V][4]                          */
+int ufem_net_prepare(ufem_net *N, void *stream);   /* gates of the bound field            */
+int ufem_net_infer(ufem_net *N, const int *d_skip, void *stream);
 
 /* error text (ufem_sys.c) */
 void ufem_set_error(const char *fmt, ...);

@@ -1,0 +1,960 @@
+/*
+ * ufem_net.c -- the learned preconditioner: a symmetric 3-level 3-D
+ * convolution U-Net on the structured node grid of a box mesh, trained and
+ * applied on the GPU by the kernels of ugk_unet.hip.
+ *
+ * The network has no counterpart in the reference; it accelerates the
+ * Jacobi-PCG solve (femEqsCGwJ, fem/fem_eqs.c:88-237) as the preconditioner
+ *
+ *   z = dinv r + mask dsq N(mask dsq r),   dsq = sqrt(dinv),
+ *   N(v) = C0^T (g0 h0 + C1^T (g1 h1 + C2^T (g2 h2))),
+ *   h0 = C0 v, h1 = C1 h0, h2 = C2 h1,
+ *   g_l = exp(b_l - (1 + a_l) ebar_l)   (per node and channel),
+ *
+ * with C0 a 3x3x3 convolution 3 -> 16 channels, C1 16 -> 32 and C2 32 -> 64
+ * stride-2 convolutions (levels of (n + 1) / 2 nodes per axis), C^T their
+ * adjoints (the same weights, transposed: the up path is tied to the down
+ * path), and ebar_l the log-modulus log(E / 30 GPa) at the nodes, averaged
+ * over 3x3x3 windows per coarser level.  N = sum_l H_l^T diag(g_l) H_l is
+ * linear in r, symmetric and positive semi-definite, so M is SPD and the
+ * short recurrences of CG stay valid.  The input is scaled by 1 / rms(dsq r)
+ * and the output by rms(dsq r) only to keep the numbers near 1.
+ *
+ * Weights are bf16 in the matrix cores (f32 masters for Adam); activations
+ * are f32 and enter the bf16 matrix cores as exact three-way splits, so the
+ * preconditioner is linear to f32 rounding (bf16 activations would add
+ * noise that costs CG many iterations).
+ *
+ * Training pairs come from plain Jacobi-PCG solves on random material
+ * fields: snapshots (x_k, r_k) every few iterations, and once the solve has
+ * converged to u the error e_k = u - x_k, so that K e_k = r_k.  The loss of
+ * a pair is the energy-norm error of z relative to that of e,
+ * L = (e - z)^T (r - K z) / (e^T r), whose gradient in z is -2 (r - K z) /
+ * (e^T r): one FP32 SpMV with that field's matrix per pair and step.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ufem_fem_math.h"
+#include "ufem_internal.h"
+
+#define E_REF 30e9
+
+static const int CH[3] = {16, 32, 64};    /* output channels of C0, C1, C2 */
+static const int CI[3] = {3, 16, 32};     /* input channels                */
+
+struct ufem_net {
+  ugk_grid3 g[3];
+  long V[3];
+  int nbmax;
+  void *stream;
+  void *ev[2];
+  long P, offW[3], offA[3], offB[3];
+  float *w, *gr, *m, *v;          /* packed f32 parameters, gradient, Adam moments */
+  unsigned short *wf[3], *wt[3];  /* bf16 [27][cop][cip] and [27][cip][cop]        */
+  float *dwpart;
+  double *gpart;
+  /* activations (f32, nbmax samples) */
+  float *x0, *h[3], *t[3], *gt[3], *eb[3], *y;
+  float *dy, *dt[3], *dh[3];
+  float *zz, *kz;
+  double *dpart, *hpart;
+  long step;
+  /* training pairs */
+  long cap, count;
+  float *pv, *prr, *pee;
+  double *per;
+  int *pfield;
+  /* material fields of the pairs: FP32 K values, dinv, ebar levels */
+  int nfield, capf;
+  float **fvals, **fdinv, **feb;
+  /* binding to a system */
+  ufem_sys *sys;
+  float *mask;
+  float *x0i, *yi, *gi[3], *ebi;
+  float *ebh;                     /* host ebar levels of the bound field */
+  double *snap;
+  long snap_cap, snap_n;
+  int snap_every;
+};
+
+static int r16(int c) { return (c + 15) / 16 * 16; }
+
+static int net_fail(const char *what)
+{
+  ufem_set_error("%s: %s", what, ugk_error());
+  return AF_ERR;
+}
+
+static ugk_grid3 coarse(ugk_grid3 g)
+{
+  ugk_grid3 c = {(g.D + 1) / 2, (g.H + 1) / 2, (g.W + 1) / 2};
+  return c;
+}
+
+long ufem_net_nparams(void)
+{
+  long P = 0;
+  int l;
+  for (l = 0; l < 3; l++) P += 27L * CI[l] * CH[l] + 2L * CH[l];
+  return P;
+}
+
+static void *dalloc(size_t bytes, int *bad)
+{
+  void *p = ugk_malloc(bytes);
+  if (!p) *bad = 1;
+  return p;
+}
+
+void ufem_net_destroy(ufem_net *N)
+{
+  int l, f;
+  if (!N) return;
+  ugk_free(N->w); ugk_free(N->gr); ugk_free(N->m); ugk_free(N->v);
+  for (l = 0; l < 3; l++) {
+    ugk_free(N->wf[l]); ugk_free(N->wt[l]);
+    ugk_free(N->h[l]); ugk_free(N->t[l]); ugk_free(N->gt[l]); ugk_free(N->eb[l]);
+    ugk_free(N->dt[l]); ugk_free(N->dh[l]); ugk_free(N->gi[l]);
+  }
+  ugk_free(N->dwpart); ugk_free(N->gpart);
+  ugk_free(N->x0); ugk_free(N->y); ugk_free(N->dy); ugk_free(N->zz); ugk_free(N->kz); ugk_free(N->dpart);
+  ugk_free(N->pv); ugk_free(N->prr); ugk_free(N->pee);
+  for (f = 0; f < N->nfield; f++) { ugk_free(N->fvals[f]); ugk_free(N->fdinv[f]); ugk_free(N->feb[f]); }
+  free(N->fvals); free(N->fdinv); free(N->feb);
+  ugk_free(N->mask); ugk_free(N->x0i); ugk_free(N->yi); ugk_free(N->ebi); ugk_free(N->snap);
+  free(N->hpart); free(N->per); free(N->pfield); free(N->ebh);
+  if (N->ev[0]) ugk_event_destroy(N->ev[0]);
+  if (N->ev[1]) ugk_event_destroy(N->ev[1]);
+  if (N->stream) ugk_stream_destroy(N->stream);
+  free(N);
+}
+
+static ugk_dw_desc dw_desc(const ufem_net *N, int l, const float *x, int ldx, int cin, const float *dy, int ldd,
+                           int cout, int stride, int lev_in, int nb)
+{
+  ugk_dw_desc d;
+  memset(&d, 0, sizeof(d));
+  d.x = x; d.ldx = ldx; d.cin = cin; d.x_f32 = 1;
+  d.dy = dy; d.ldd = ldd; d.cout = cout; d.dy_f32 = 1;
+  d.cip = r16(cin); d.cop = r16(cout);
+  d.mode = stride == 2 ? UGK_CONV_S2 : UGK_CONV_S1;
+  d.nb = nb;
+  d.gi = N->g[lev_in];
+  d.go = N->g[stride == 2 ? lev_in + 1 : lev_in];
+  (void)l;
+  return d;
+}
+
+ufem_net *ufem_net_create(int W, int H, int D, int batch)
+{
+  ufem_net *N;
+  int l, bad = 0;
+  long nb, maxpart = 0, n;
+  if (W < 2 || H < 2 || D < 2 || batch < 1) { ufem_set_error("net grid or batch out of range"); return NULL; }
+  if (ugk_init()) { net_fail("device"); return NULL; }
+  N = (ufem_net *)calloc(1, sizeof(*N));
+  if (!N) return NULL;
+  N->g[0].D = D; N->g[0].H = H; N->g[0].W = W;
+  N->g[1] = coarse(N->g[0]);
+  N->g[2] = coarse(N->g[1]);
+  for (l = 0; l < 3; l++) N->V[l] = (long)N->g[l].D * N->g[l].H * N->g[l].W;
+  N->nbmax = batch;
+  nb = batch;
+  n = 3 * N->V[0];
+  N->stream = ugk_stream_create();
+  N->ev[0] = ugk_event_create();
+  N->ev[1] = ugk_event_create();
+  if (!N->stream || !N->ev[0] || !N->ev[1]) { net_fail("stream"); ufem_net_destroy(N); return NULL; }
+  N->P = 0;
+  for (l = 0; l < 3; l++) { N->offW[l] = N->P; N->P += 27L * CI[l] * CH[l]; }
+  for (l = 0; l < 3; l++) {
+    N->offA[l] = N->P; N->P += CH[l];
+    N->offB[l] = N->P; N->P += CH[l];
+  }
+  N->w = (float *)dalloc(sizeof(float) * N->P, &bad);
+  N->gr = (float *)dalloc(sizeof(float) * N->P, &bad);
+  N->m = (float *)dalloc(sizeof(float) * N->P, &bad);
+  N->v = (float *)dalloc(sizeof(float) * N->P, &bad);
+  for (l = 0; l < 3; l++) {
+    const size_t b = sizeof(unsigned short) * 27 * r16(CI[l]) * r16(CH[l]);
+    const long Vl = N->V[l];
+    N->wf[l] = (unsigned short *)dalloc(b, &bad);
+    N->wt[l] = (unsigned short *)dalloc(b, &bad);
+    N->h[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
+    N->t[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
+    N->gt[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
+    N->dt[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
+    N->dh[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
+    N->eb[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl), &bad);
+    N->gi[l] = (float *)dalloc(sizeof(float) * (size_t)(Vl * CH[l]), &bad);
+    {
+      /* weight-gradient partials: the largest of the calls the backward makes */
+      ugk_dw_desc d = dw_desc(N, l, NULL, 4, CI[l], NULL, 4, CH[l], l ? 2 : 1, l ? l - 1 : 0, batch);
+      long need = (long)ugk_conv_dw_slices(&d) * 27L * d.cop * d.cip;
+      if (need > maxpart) maxpart = need;
+    }
+  }
+  N->dwpart = (float *)dalloc(sizeof(float) * maxpart, &bad);
+  N->gpart = (double *)dalloc(sizeof(double) * 512 * 2 * 64, &bad);
+  N->x0 = (float *)dalloc(sizeof(float) * (size_t)(nb * N->V[0] * 4), &bad);
+  N->y = (float *)dalloc(sizeof(float) * (size_t)(nb * N->V[0] * 4), &bad);
+  N->dy = (float *)dalloc(sizeof(float) * (size_t)(nb * N->V[0] * 4), &bad);
+  N->zz = (float *)dalloc(sizeof(float) * (size_t)n, &bad);
+  N->kz = (float *)dalloc(sizeof(float) * (size_t)n, &bad);
+  N->dpart = (double *)dalloc(sizeof(double) * 1024 * (size_t)(nb > 4 ? nb : 4), &bad);
+  N->hpart = (double *)malloc(sizeof(double) * 1024 * (size_t)(nb > 4 ? nb : 4));
+  N->x0i = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] * 4), &bad);
+  N->yi = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] * 4), &bad);
+  N->ebi = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] + N->V[1] + N->V[2]), &bad);
+  if (bad || !N->hpart) { net_fail("network buffers"); ufem_net_destroy(N); return NULL; }
+  return N;
+}
+
+/* ---------------- weights ---------------------------------------------- */
+static int adam_all(ufem_net *N, float lr)
+{
+  int l;
+  if (lr > 0) N->step++;
+  for (l = 0; l < 3; l++) {
+    if (ugk_adam(27L * CI[l] * CH[l], CH[l], CI[l], r16(CH[l]), r16(CI[l]), N->w + N->offW[l], N->gr + N->offW[l],
+                 N->m + N->offW[l], N->v + N->offW[l], N->wf[l], N->wt[l], lr, 0.9f, 0.999f, 1e-8f, N->step,
+                 N->stream))
+      return net_fail("weights");
+    if (lr > 0 &&
+        (ugk_adam(CH[l], 1, CH[l], 0, 0, N->w + N->offA[l], N->gr + N->offA[l], N->m + N->offA[l],
+                  N->v + N->offA[l], NULL, NULL, lr, 0.9f, 0.999f, 1e-8f, N->step, N->stream) ||
+         ugk_adam(CH[l], 1, CH[l], 0, 0, N->w + N->offB[l], N->gr + N->offB[l], N->m + N->offB[l],
+                  N->v + N->offB[l], NULL, NULL, lr, 0.9f, 0.999f, 1e-8f, N->step, N->stream)))
+      return net_fail("gate parameters");
+  }
+  return AF_OK;
+}
+
+int ufem_net_set_weights(ufem_net *N, const float *w)
+{
+  if (ugk_h2d(N->w, w, sizeof(float) * N->P, N->stream)) return net_fail("weights upload");
+  if (ugk_zero(N->m, sizeof(float) * N->P, N->stream) || ugk_zero(N->v, sizeof(float) * N->P, N->stream))
+    return net_fail("moments");
+  N->step = 0;
+  if (adam_all(N, 0.f)) return AF_ERR;
+  return ugk_sync(N->stream) ? net_fail("sync") : AF_OK;
+}
+
+int ufem_net_get_weights(ufem_net *N, float *w)
+{
+  return ugk_d2h(w, N->w, sizeof(float) * N->P, N->stream) ? net_fail("weights download") : AF_OK;
+}
+
+static unsigned long long mix64(unsigned long long z)
+{
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+/* weights N(0, 1 / (27 cin)) from a counter-based generator; gates a = b = 0 */
+int ufem_net_init(ufem_net *N, unsigned long seed)
+{
+  float *w = (float *)calloc((size_t)N->P, sizeof(float));
+  long i = 0;
+  int l, rv;
+  if (!w) return AF_ERR_MEM;
+  for (l = 0; l < 3; l++) {
+    const long Pl = 27L * CI[l] * CH[l];
+    const double sd = 1.0 / sqrt(27.0 * CI[l]);
+    long k;
+    for (k = 0; k < Pl; k++, i++) {
+      unsigned long long a = mix64((unsigned long long)seed * 0x632BE59BD9B4E019ULL + 2ULL * (unsigned long long)i);
+      unsigned long long b = mix64(a ^ 0xD1B54A32D192ED03ULL);
+      double u1 = ((double)(a >> 11) + 0.5) / 9007199254740992.0;
+      double u2 = (double)(b >> 11) / 9007199254740992.0;
+      w[N->offW[l] + k] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+  }
+  rv = ufem_net_set_weights(N, w);
+  free(w);
+  return rv;
+}
+
+/* ---------------- forward / backward ------------------------------------- */
+static int conv(ufem_net *N, const float *x, int ldx, int cin, int l, int adjoint, float *y, int ldy, int cout,
+                int lev_in, int lev_out, const float *ga, const float *gb, int ldg, int nb, const int *skip,
+                void *stream)
+{
+  ugk_conv_desc d;
+  memset(&d, 0, sizeof(d));
+  d.x = x; d.ldx = ldx; d.cin = cin; d.x_f32 = 1;
+  d.w = adjoint ? N->wt[l] : N->wf[l];
+  d.cip = adjoint ? r16(CH[l]) : r16(CI[l]);
+  d.cop = adjoint ? r16(CI[l]) : r16(CH[l]);
+  d.y = y; d.ldy = ldy; d.cout = cout; d.y_f32 = 1;
+  d.ga = ga; d.ldga = ldg; d.gb = gb; d.ldgb = ldg;
+  d.mode = l == 0 ? UGK_CONV_S1 : (adjoint ? UGK_CONV_T2 : UGK_CONV_S2);
+  d.flip = (l == 0 && adjoint) ? 1 : 0;
+  d.nb = nb;
+  d.gi = N->g[lev_in];
+  d.go = N->g[lev_out];
+  d.skip = skip;
+  return ugk_conv(&d, stream);
+}
+
+/* y = N(v) for nb samples with gates g[l] (nb x V_l x C_l) */
+static int net_forward(ufem_net *N, const float *v, float *const *g, float *y, int nb, const int *skip,
+                       void *st)
+{
+  if (conv(N, v, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, 0, nb, skip, st) ||
+      conv(N, N->h[0], 16, 16, 1, 0, N->h[1], 32, 32, 0, 1, NULL, NULL, 0, nb, skip, st) ||
+      conv(N, N->h[1], 32, 32, 2, 0, N->h[2], 64, 64, 1, 2, NULL, NULL, 0, nb, skip, st) ||
+      ugk_mul((long)nb * N->V[2] * 64, g[2], N->h[2], N->t[2], skip, st) ||
+      conv(N, N->t[2], 64, 64, 2, 1, N->t[1], 32, 32, 2, 1, g[1], N->h[1], 32, nb, skip, st) ||
+      conv(N, N->t[1], 32, 32, 1, 1, N->t[0], 16, 16, 1, 0, g[0], N->h[0], 16, nb, skip, st) ||
+      conv(N, N->t[0], 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, 0, nb, skip, st))
+    return net_fail("network forward");
+  return AF_OK;
+}
+
+static int wgrad(ufem_net *N, int l, const float *x, int ldx, int cin, const float *dy, int ldd, int lev_in,
+                 int nb, int acc)
+{
+  ugk_dw_desc d = dw_desc(N, l, x, ldx, cin, dy, ldd, CH[l], l ? 2 : 1, lev_in, nb);
+  return ugk_conv_dw(&d, N->dwpart, ugk_conv_dw_slices(&d), N->gr + N->offW[l], acc, N->stream);
+}
+
+/* parameter gradient of <dy, N(v)> into N->gr (forward fields in place) */
+static int net_backward(ufem_net *N, const float *v, const float *dy, int nb)
+{
+  float *const *g = N->gt;
+  const long n2 = (long)nb * N->V[2] * 64;
+  int l;
+  if (conv(N, dy, 4, 4, 0, 0, N->dt[0], 16, 16, 0, 0, NULL, NULL, 0, nb, NULL, N->stream) ||
+      wgrad(N, 0, dy, 4, 3, N->t[0], 16, 0, nb, 0) ||
+      conv(N, N->dt[0], 16, 16, 1, 0, N->dt[1], 32, 32, 0, 1, NULL, NULL, 0, nb, NULL, N->stream) ||
+      wgrad(N, 1, N->dt[0], 16, 16, N->t[1], 32, 0, nb, 0) ||
+      conv(N, N->dt[1], 32, 32, 2, 0, N->dt[2], 64, 64, 1, 2, NULL, NULL, 0, nb, NULL, N->stream) ||
+      wgrad(N, 2, N->dt[1], 32, 32, N->t[2], 64, 1, nb, 0) ||
+      ugk_mul(n2, g[2], N->dt[2], N->dh[2], NULL, N->stream) ||
+      conv(N, N->dh[2], 64, 64, 2, 1, N->dh[1], 32, 32, 2, 1, g[1], N->dt[1], 32, nb, NULL, N->stream) ||
+      wgrad(N, 2, N->h[1], 32, 32, N->dh[2], 64, 1, nb, 1) ||
+      conv(N, N->dh[1], 32, 32, 1, 1, N->dh[0], 16, 16, 1, 0, g[0], N->dt[0], 16, nb, NULL, N->stream) ||
+      wgrad(N, 1, N->h[0], 16, 16, N->dh[1], 32, 0, nb, 1) ||
+      wgrad(N, 0, v, 4, 3, N->dh[0], 16, 0, nb, 1))
+    return net_fail("network backward");
+  for (l = 0; l < 3; l++)
+    if (ugk_gate_grad((long)nb * N->V[l], CH[l], g[l], N->h[l], N->dt[l], N->eb[l], N->gpart,
+                      N->gr + N->offA[l], N->gr + N->offB[l], N->stream))
+      return net_fail("gate gradient");
+  return AF_OK;
+}
+
+static int make_gates(ufem_net *N, float *const *eb, float *const *g, int nb, void *st)
+{
+  int l;
+  for (l = 0; l < 3; l++)
+    if (ugk_gates((long)nb * N->V[l], CH[l], eb[l], N->w + N->offA[l], N->w + N->offB[l], g[l], st))
+      return net_fail("gates");
+  return AF_OK;
+}
+
+/* ebar levels on the host: node level, then 3x3x3 means around 2c */
+static void pool_host(const float *f, ugk_grid3 gf, float *c, ugk_grid3 gc)
+{
+  int z, y, x, dz, dy, dx;
+  for (z = 0; z < gc.D; z++)
+    for (y = 0; y < gc.H; y++)
+      for (x = 0; x < gc.W; x++) {
+        double s = 0.0;
+        int k = 0;
+        for (dz = -1; dz <= 1; dz++)
+          for (dy = -1; dy <= 1; dy++)
+            for (dx = -1; dx <= 1; dx++) {
+              const int a = 2 * z + dz, b = 2 * y + dy, e = 2 * x + dx;
+              if (a < 0 || a >= gf.D || b < 0 || b >= gf.H || e < 0 || e >= gf.W) continue;
+              s += f[((long)a * gf.H + b) * gf.W + e];
+              k++;
+            }
+        c[((long)z * gc.H + y) * gc.W + x] = (float)(s / k);
+      }
+}
+
+static void levels_host(const ufem_net *N, const float *e0, float *out)
+{
+  memcpy(out, e0, sizeof(float) * (size_t)N->V[0]);
+  pool_host(out, N->g[0], out + N->V[0], N->g[1]);
+  pool_host(out + N->V[0], N->g[1], out + N->V[0] + N->V[1], N->g[2]);
+}
+
+/* upload nb samples' ebar0 (host [nb][V0]) into the batch level buffers */
+static int upload_levels(ufem_net *N, const float *eb0, int nb)
+{
+  const long Vt = N->V[0] + N->V[1] + N->V[2];
+  float *lv = (float *)malloc(sizeof(float) * (size_t)Vt);
+  int b, l, rv = AF_OK;
+  if (!lv) return AF_ERR_MEM;
+  for (b = 0; b < nb && rv == AF_OK; b++) {
+    long off = 0;
+    levels_host(N, eb0 + b * N->V[0], lv);
+    for (l = 0; l < 3; l++) {
+      if (ugk_h2d(N->eb[l] + b * N->V[l], lv + off, sizeof(float) * (size_t)N->V[l], N->stream) ||
+          ugk_sync(N->stream))
+        rv = net_fail("levels upload");
+      off += N->V[l];
+    }
+  }
+  free(lv);
+  return rv;
+}
+
+static int upload_input(ufem_net *N, const float *x, int nb)
+{
+  const long V = N->V[0];
+  float *h = (float *)calloc((size_t)(nb * V * 4), sizeof(float));
+  long i;
+  int c, rv;
+  if (!h) return AF_ERR_MEM;
+  for (i = 0; i < nb * V; i++)
+    for (c = 0; c < 3; c++) h[i * 4 + c] = x[i * 3 + c];
+  rv = ugk_h2d(N->x0, h, sizeof(float) * (size_t)(nb * V * 4), N->stream) || ugk_sync(N->stream);
+  free(h);
+  return rv ? net_fail("upload") : AF_OK;
+}
+
+int ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, float *y)
+{
+  const long V = N->V[0];
+  float *h;
+  long i;
+  int c;
+  if (nb < 1 || nb > N->nbmax) { ufem_set_error("batch %d out of range", nb); return AF_ERR_VAL; }
+  if (upload_input(N, x, nb) || upload_levels(N, eb0, nb) || make_gates(N, N->eb, N->gt, nb, N->stream) ||
+      net_forward(N, N->x0, N->gt, N->y, nb, NULL, N->stream))
+    return AF_ERR;
+  h = (float *)malloc(sizeof(float) * (size_t)(nb * V * 4));
+  if (!h) return AF_ERR_MEM;
+  if (ugk_d2h(h, N->y, sizeof(float) * (size_t)(nb * V * 4), N->stream)) { free(h); return net_fail("download"); }
+  for (i = 0; i < nb * V; i++)
+    for (c = 0; c < 3; c++) y[i * 3 + c] = h[i * 4 + c];
+  free(h);
+  return AF_OK;
+}
+
+/* gradient of <dy, N(x)> with respect to the parameters (packed like the
+ * weights), for nb samples */
+int ufem_net_backward(ufem_net *N, int nb, const float *x, const float *eb0, const float *dy, float *grad)
+{
+  const long V = N->V[0];
+  float *h, *y;
+  long i;
+  int c, rv;
+  if (nb < 1 || nb > N->nbmax) { ufem_set_error("batch %d out of range", nb); return AF_ERR_VAL; }
+  y = (float *)malloc(sizeof(float) * (size_t)(nb * V * 3));
+  h = (float *)calloc((size_t)(nb * V * 4), sizeof(float));
+  if (!y || !h) { free(y); free(h); return AF_ERR_MEM; }
+  rv = ufem_net_forward(N, nb, x, eb0, y);
+  if (rv == AF_OK) {
+    for (i = 0; i < nb * V; i++)
+      for (c = 0; c < 3; c++) h[i * 4 + c] = dy[i * 3 + c];
+    if (ugk_h2d(N->dy, h, sizeof(float) * (size_t)(nb * V * 4), N->stream) || ugk_sync(N->stream))
+      rv = net_fail("upload");
+  }
+  if (rv == AF_OK) rv = net_backward(N, N->x0, N->dy, nb);
+  if (rv == AF_OK && ugk_d2h(grad, N->gr, sizeof(float) * N->P, N->stream)) rv = net_fail("download");
+  free(y);
+  free(h);
+  return rv;
+}
+
+/* ---------------- binding, training pairs, training ---------------------- */
+int ufem_net_bind(ufem_net *N, ufem_sys *s, const double *E)
+{
+  const ugk_grid3 g = N->g[0];
+  const long V = N->V[0], nx = g.W - 1, ny = g.H - 1, nz = g.D - 1;
+  const long Vt = N->V[0] + N->V[1] + N->V[2];
+  double *acc;
+  int *cnt;
+  float *e0, *mk;
+  long e, i;
+  int l, rv = AF_OK;
+  if (s->n != 3 * V || s->n_elems != nx * ny * nz) {
+    ufem_set_error("system (%ld DOF, %ld bricks) does not match the %dx%dx%d node grid", s->n, s->n_elems,
+                   g.W, g.H, g.D);
+    return AF_ERR_VAL;
+  }
+  acc = (double *)calloc((size_t)V, sizeof(double));
+  cnt = (int *)calloc((size_t)V, sizeof(int));
+  e0 = (float *)malloc(sizeof(float) * (size_t)V);
+  mk = (float *)malloc(sizeof(float) * (size_t)s->n);
+  if (!N->ebh) N->ebh = (float *)malloc(sizeof(float) * (size_t)Vt);
+  if (!acc || !cnt || !e0 || !mk || !N->ebh) { rv = AF_ERR_MEM; goto out; }
+  /* brick e = i + nx (j + ny k) touches nodes (i+a) + W ((j+b) + H (k+c)) */
+  for (e = 0; e < nx * ny * nz; e++) {
+    const long ii = e % nx, jj = (e / nx) % ny, kk = e / (nx * ny);
+    const double le = log(E[e] / E_REF);
+    int a, b, c;
+    for (c = 0; c < 2; c++)
+      for (b = 0; b < 2; b++)
+        for (a = 0; a < 2; a++) {
+          const long p = (ii + a) + g.W * ((jj + b) + g.H * (kk + c));
+          acc[p] += le;
+          cnt[p]++;
+        }
+  }
+  for (i = 0; i < V; i++) e0[i] = (float)(cnt[i] ? acc[i] / cnt[i] : 0.0);
+  levels_host(N, e0, N->ebh);
+  if (!N->mask) N->mask = (float *)ugk_malloc(sizeof(float) * (size_t)s->n);
+  if (!N->mask) { rv = net_fail("mask"); goto out; }
+  for (i = 0; i < s->n; i++) mk[i] = 1.f;
+  for (i = 0; i < s->n_fixed; i++) mk[s->fixed_host[i]] = 0.f;
+  if (ugk_h2d(N->mask, mk, sizeof(float) * (size_t)s->n, N->stream) ||
+      ugk_h2d(N->ebi, N->ebh, sizeof(float) * (size_t)Vt, N->stream) || ugk_sync(N->stream)) {
+    rv = net_fail("upload");
+    goto out;
+  }
+  N->sys = s;
+  (void)l;
+out:
+  free(acc); free(cnt); free(e0); free(mk);
+  return rv;
+}
+
+int ufem_net_reserve(ufem_net *N, long cap)
+{
+  const long V = N->V[0], n = 3 * V;
+  float *pv, *prr, *pee;
+  double *per;
+  int *pf;
+  if (cap <= N->cap) return AF_OK;
+  pv = (float *)ugk_malloc(sizeof(float) * (size_t)(V * 4) * (size_t)cap);
+  prr = (float *)ugk_malloc(sizeof(float) * (size_t)n * (size_t)cap);
+  pee = (float *)ugk_malloc(sizeof(float) * (size_t)n * (size_t)cap);
+  per = (double *)calloc((size_t)cap, sizeof(double));
+  pf = (int *)calloc((size_t)cap, sizeof(int));
+  if (!pv || !prr || !pee || !per || !pf) {
+    ugk_free(pv); ugk_free(prr); ugk_free(pee); free(per); free(pf);
+    return net_fail("pair store");
+  }
+  if (N->count && (ugk_d2d(pv, N->pv, sizeof(float) * (size_t)(V * 4 * N->count), N->stream) ||
+                   ugk_d2d(prr, N->prr, sizeof(float) * (size_t)(n * N->count), N->stream) ||
+                   ugk_d2d(pee, N->pee, sizeof(float) * (size_t)(n * N->count), N->stream) ||
+                   ugk_sync(N->stream))) {
+    ugk_free(pv); ugk_free(prr); ugk_free(pee); free(per); free(pf);
+    return net_fail("pair store copy");
+  }
+  if (N->count) {
+    memcpy(per, N->per, sizeof(double) * (size_t)N->count);
+    memcpy(pf, N->pfield, sizeof(int) * (size_t)N->count);
+  }
+  ugk_free(N->pv); ugk_free(N->prr); ugk_free(N->pee); free(N->per); free(N->pfield);
+  N->pv = pv; N->prr = prr; N->pee = pee; N->per = per; N->pfield = pf;
+  N->cap = cap;
+  return AF_OK;
+}
+
+long ufem_net_npairs(const ufem_net *N) { return N->count; }
+
+/* chunk hook of the PCG driver: snapshot (x, r) every snap_every iterations */
+static int snap_hook(void *ctx, long it, void *stream)
+{
+  ufem_net *N = (ufem_net *)ctx;
+  ufem_sys *s = N->sys;
+  const long j = it / N->snap_every;
+  const size_t b = sizeof(double) * (size_t)s->n;
+  if (it % N->snap_every || j >= N->snap_cap) return 0;
+  if (ugk_d2d(N->snap + 2 * j * s->n, s->d_u, b, stream) || ugk_d2d(N->snap + (2 * j + 1) * s->n, s->w.r, b, stream))
+    return -1;
+  if (j + 1 > N->snap_n) N->snap_n = j + 1;
+  return 0;
+}
+
+/* keep the bound system's current material field (FP32 K, dinv, ebar) */
+static int add_field(ufem_net *N)
+{
+  ufem_sys *s = N->sys;
+  const long Vt = N->V[0] + N->V[1] + N->V[2];
+  const size_t nv = (size_t)s->K.total_slots * 64 * 9;
+  ugk_mat tmp;
+  int f = N->nfield;
+  if (f == N->capf) {
+    int cap = N->capf ? 2 * N->capf : 16;
+    float **a = (float **)realloc(N->fvals, sizeof(float *) * (size_t)cap);
+    float **b = a ? (float **)realloc(N->fdinv, sizeof(float *) * (size_t)cap) : NULL;
+    float **c = b ? (float **)realloc(N->feb, sizeof(float *) * (size_t)cap) : NULL;
+    if (a) N->fvals = a;
+    if (b) N->fdinv = b;
+    if (c) N->feb = c;
+    if (!a || !b || !c) return AF_ERR_MEM;
+    N->capf = cap;
+  }
+  N->fvals[f] = (float *)ugk_malloc(sizeof(float) * nv);
+  N->fdinv[f] = (float *)ugk_malloc(sizeof(float) * (size_t)s->n);
+  N->feb[f] = (float *)ugk_malloc(sizeof(float) * (size_t)Vt);
+  if (!N->fvals[f] || !N->fdinv[f] || !N->feb[f]) {
+    ugk_free(N->fvals[f]); ugk_free(N->fdinv[f]); ugk_free(N->feb[f]);
+    return net_fail("field store");
+  }
+  tmp = s->K;
+  tmp.d_vals32 = N->fvals[f];
+  if (ugk_vals_to_f32(&tmp, N->stream) || ugk_to_f32(s->n, s->w.dinv, N->fdinv[f], N->stream) ||
+      ugk_h2d(N->feb[f], N->ebh, sizeof(float) * (size_t)Vt, N->stream) || ugk_sync(N->stream))
+    return net_fail("field store");
+  N->nfield++;
+  return AF_OK;
+}
+
+long ufem_net_collect(ufem_net *N, int pairs, double eps, int every, ufem_solve_info *info)
+{
+  ufem_sys *s = N->sys;
+  ufem_solve_opts o;
+  ufem_solve_info tmp;
+  long added = 0, jmax, last = -1, k;
+  double er0 = 0.0;
+  int rv, f;
+  if (!s) { ufem_set_error("bind the network to a system first"); return AF_ERR; }
+  if (pairs < 1 || every < 2 || every % 2) { ufem_set_error("pairs >= 1, even every >= 2"); return AF_ERR_VAL; }
+  if (!info) info = &tmp;
+  if (!N->snap) {
+    long cap = (long)(4e9 / (16.0 * (double)s->n));
+    N->snap_cap = cap < 16 ? 16 : (cap > 512 ? 512 : cap);
+    N->snap = (double *)ugk_malloc(sizeof(double) * 2 * (size_t)s->n * (size_t)N->snap_cap);
+    if (!N->snap) return net_fail("snapshot ring");
+  }
+  N->snap_every = every;
+  N->snap_n = 0;
+  memset(&o, 0, sizeof(o));
+  o.solver = UFEM_SOLVER_PCG;
+  o.rule = UFEM_RULE_RELATIVE;
+  o.eps = eps;
+  o.exec = UFEM_EXEC_STREAMED;
+  o.chunk = every;
+  s->w.hook = snap_hook;
+  s->w.hook_ctx = N;
+  rv = ufem_sys_solve(s, &o, info);
+  s->w.hook = NULL;
+  s->w.hook_ctx = NULL;
+  if (rv != AF_OK) return rv;
+  if (add_field(N)) return AF_ERR;
+  f = N->nfield - 1;
+  jmax = info->iters / every;
+  if (jmax > N->snap_n - 1) jmax = N->snap_n - 1;
+  for (k = 0; k < pairs && N->count < N->cap; k++) {
+    const long j = pairs == 1 ? 0 : (long)floor((double)k * (double)jmax / (double)(pairs - 1) + 0.5);
+    const double *x = N->snap + 2 * j * s->n, *r = N->snap + (2 * j + 1) * s->n;
+    double a2 = 0.0, er = 0.0, sc;
+    int G, i;
+    if (j == last) continue;
+    last = j;
+    G = ugk_pair_stats(s->n, x, r, s->d_u, s->w.dinv, N->dpart, N->stream);
+    if (G < 0 || ugk_d2h(N->hpart, N->dpart, sizeof(double) * 2 * (size_t)G, N->stream))
+      return net_fail("pair statistics");
+    for (i = 0; i < G; i++) {
+      a2 += N->hpart[2 * i];
+      er += N->hpart[2 * i + 1];
+    }
+    if (k == 0) er0 = er;
+    /* skip converged snapshots, whose error is at the solve's own accuracy */
+    if (!(a2 > 0.0) || !(er > 1e-14 * er0)) continue;
+    sc = sqrt(a2 / (double)s->n);
+    if (ugk_pair_write(s->n, x, r, s->d_u, s->w.dinv, N->mask, sc, N->pv + N->count * N->V[0] * 4,
+                       N->prr + N->count * s->n, N->pee + N->count * s->n, N->stream))
+      return net_fail("pair write");
+    N->per[N->count] = er / (sc * sc);
+    N->pfield[N->count] = f;
+    N->count++;
+    added++;
+  }
+  if (ugk_sync(N->stream)) return net_fail("sync");
+  return added;
+}
+
+/* energy loss and its output gradient for batch slot b holding pair q */
+static int pair_loss(ufem_net *N, int b, long q, int nb, double *part_out)
+{
+  ufem_sys *s = N->sys;
+  const long n = s->n;
+  const int f = N->pfield[q];
+  ugk_mat Kf = s->K;
+  int G;
+  Kf.d_vals32 = N->fvals[f];
+  if (ugk_el_z(n, N->prr + q * n, N->fdinv[f], N->mask, N->y + b * N->V[0] * 4, N->zz, N->stream) ||
+      ugk_spmv_f32(&Kf, N->zz, N->kz, N->stream))
+    return net_fail("loss");
+  G = ugk_el_grad(n, N->prr + q * n, N->pee + q * n, N->zz, N->kz, N->fdinv[f], N->mask,
+                  (float)(-2.0 / (N->per[q] * nb)), N->dy + b * N->V[0] * 4, part_out, N->stream);
+  if (G < 0) return net_fail("loss");
+  return G;
+}
+
+/* load pairs idx[0..nb) into the batch: inputs, ebar levels, gates */
+static int load_batch(ufem_net *N, const long *idx, int nb)
+{
+  int b, l;
+  for (b = 0; b < nb; b++) {
+    const long q = idx[b];
+    long off = 0;
+    if (ugk_d2d(N->x0 + b * N->V[0] * 4, N->pv + q * N->V[0] * 4, sizeof(float) * (size_t)(N->V[0] * 4),
+                N->stream))
+      return net_fail("batch copy");
+    for (l = 0; l < 3; l++) {
+      if (ugk_d2d(N->eb[l] + b * N->V[l], N->feb[N->pfield[q]] + off, sizeof(float) * (size_t)N->V[l], N->stream))
+        return net_fail("batch copy");
+      off += N->V[l];
+    }
+  }
+  return make_gates(N, N->eb, N->gt, nb, N->stream);
+}
+
+/* losses of pairs [i0, i0 + cnt) under the current weights (no update) */
+int ufem_net_eval(ufem_net *N, long i0, long cnt, double *loss)
+{
+  long k;
+  int G, i;
+  if (!N->sys || i0 < 0 || i0 + cnt > N->count) { ufem_set_error("pair range"); return AF_ERR_VAL; }
+  for (k = 0; k < cnt; k++) {
+    long q = i0 + k;
+    double sum = 0.0;
+    if (load_batch(N, &q, 1) || net_forward(N, N->x0, N->gt, N->y, 1, NULL, N->stream)) return AF_ERR;
+    G = pair_loss(N, 0, q, 1, N->dpart);
+    if (G < 0 || ugk_d2h(N->hpart, N->dpart, sizeof(double) * (size_t)G, N->stream)) return AF_ERR;
+    for (i = 0; i < G; i++) sum += N->hpart[i];
+    loss[k] = sum / N->per[q];
+  }
+  return AF_OK;
+}
+
+int ufem_net_train(ufem_net *N, long steps, int batch, double lr, unsigned long seed, double *loss, double *ms)
+{
+  long *perm = NULL, pos, st, idx[64];
+  unsigned long long rng = (unsigned long long)seed * 0x9E3779B97F4A7C15ULL + 1;
+  int rv = AF_OK;
+  if (!N->sys) { ufem_set_error("bind the network to a system first"); return AF_ERR; }
+  if (batch < 1 || batch > N->nbmax || batch > 64) { ufem_set_error("batch %d out of range", batch); return AF_ERR_VAL; }
+  if (N->count < batch) { ufem_set_error("%ld training pairs, batch %d", N->count, batch); return AF_ERR_VAL; }
+  perm = (long *)malloc(sizeof(long) * (size_t)N->count);
+  if (!perm) return AF_ERR_MEM;
+  pos = N->count;
+  if (ugk_event_record(N->ev[0], N->stream)) { free(perm); return net_fail("event"); }
+  for (st = 0; st < steps; st++) {
+    /* cosine decay from lr to lr / 10 */
+    const double f = 0.55 + 0.45 * cos(3.141592653589793 * (double)st / (double)(steps > 1 ? steps - 1 : 1));
+    int k, G[64];
+    for (k = 0; k < batch; k++) {
+      if (pos >= N->count) {   /* new epoch: Fisher-Yates shuffle */
+        long i;
+        for (i = 0; i < N->count; i++) perm[i] = i;
+        for (i = N->count - 1; i > 0; i--) {
+          long jj, tq;
+          rng = mix64(rng);
+          jj = (long)(rng % (unsigned long long)(i + 1));
+          tq = perm[i]; perm[i] = perm[jj]; perm[jj] = tq;
+        }
+        pos = 0;
+      }
+      idx[k] = perm[pos++];
+    }
+    if (load_batch(N, idx, batch) || net_forward(N, N->x0, N->gt, N->y, batch, NULL, N->stream)) {
+      rv = AF_ERR;
+      goto out;
+    }
+    for (k = 0; k < batch; k++) {
+      G[k] = pair_loss(N, k, idx[k], batch, N->dpart + 1024L * k);
+      if (G[k] < 0) { rv = AF_ERR; goto out; }
+    }
+    if (net_backward(N, N->x0, N->dy, batch) || adam_all(N, (float)(lr * f))) { rv = AF_ERR; goto out; }
+    if (loss) {
+      double sum = 0.0;
+      if (ugk_d2h(N->hpart, N->dpart, sizeof(double) * 1024 * (size_t)batch, N->stream)) {
+        rv = net_fail("loss");
+        goto out;
+      }
+      for (k = 0; k < batch; k++) {
+        double sk = 0.0;
+        int i;
+        for (i = 0; i < G[k]; i++) sk += N->hpart[1024L * k + i];
+        sum += sk / N->per[idx[k]];
+      }
+      loss[st] = sum / batch;
+    }
+  }
+  if (ugk_event_record(N->ev[1], N->stream) || ugk_sync(N->stream)) { rv = net_fail("sync"); goto out; }
+  if (ms) *ms = ugk_event_ms(N->ev[0], N->ev[1]);
+out:
+  free(perm);
+  return rv;
+}
+
+/* ---------------- inference inside flexible CG (ufem_solve.c) ------------- */
+long ufem_net_bound_n(const ufem_net *N) { return N->sys ? N->sys->n : 0; }
+float *ufem_net_input(ufem_net *N) { return N->x0i; }
+const float *ufem_net_mask(const ufem_net *N) { return N->mask; }
+const float *ufem_net_output(const ufem_net *N) { return N->yi; }
+
+/* gates of the bound field, before a solve (the weights may have changed) */
+int ufem_net_prepare(ufem_net *N, void *stream)
+{
+  float *eb[3] = {N->ebi, N->ebi + N->V[0], N->ebi + N->V[0] + N->V[1]};
+  return make_gates(N, eb, N->gi, 1, stream);
+}
+
+int ufem_net_infer(ufem_net *N, const int *skip, void *stream)
+{
+  return net_forward(N, N->x0i, N->gi, N->yi, 1, skip, stream);
+}
+
+/* pair i as stored: v [V][4], rr [n], ee [n], e.r and its field index */
+int ufem_net_get_pair(ufem_net *N, long i, float *v, float *rr, float *ee, double *er, int *field)
+{
+  const long n = 3 * N->V[0];
+  if (i < 0 || i >= N->count) { ufem_set_error("pair %ld out of range", i); return AF_ERR_VAL; }
+  if (ugk_d2h(v, N->pv + i * N->V[0] * 4, sizeof(float) * (size_t)(N->V[0] * 4), N->stream) ||
+      ugk_d2h(rr, N->prr + i * n, sizeof(float) * (size_t)n, N->stream) ||
+      ugk_d2h(ee, N->pee + i * n, sizeof(float) * (size_t)n, N->stream))
+    return net_fail("pair download");
+  *er = N->per[i];
+  *field = N->pfield[i];
+  return AF_OK;
+}
+
+/* ---------------- single kernels, for tests ------------------------------- */
+static unsigned short h_f2bf(float f)
+{
+  unsigned u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+/* y = conv(x) in mode (0 stride 1, 1 stride 2, 2 stride-2 transpose) with
+ * weights w [27][cout][cin] (f32 out), x bf16-rounded or f32 (x_f32); when
+ * dy is given, also dw [27][cout][cin] = the weight gradient (modes 0, 1) */
+int ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin, int cout,
+                        const float *x, const float *w, float *y, const float *dy, float *dw)
+{
+  ugk_grid3 Gi, Go;
+  long Vi, Vo, i;
+  const int cip = r16(cin), cop = r16(cout), ldx = (cin + 3) / 4 * 4, ldy = (cout + 3) / 4 * 4;
+  const size_t xb = (x_f32 ? sizeof(float) : sizeof(unsigned short)) * (size_t)ldx;
+  void *hx = NULL, *dx_ = NULL;
+  unsigned short *hw = NULL, *dw_ = NULL;
+  float *dy_ = NULL, *part = NULL, *dgr = NULL, *hy = NULL, *hd = NULL, *ddy = NULL;
+  void *st = NULL;
+  int rv = AF_ERR, c, t, o;
+  Gi.D = gi[0]; Gi.H = gi[1]; Gi.W = gi[2];
+  Go.D = go[0]; Go.H = go[1]; Go.W = go[2];
+  if (Gi.D < 1 || Gi.H < 1 || Gi.W < 1 || Go.D < 1 || Go.H < 1 || Go.W < 1 || nb < 1 || cin < 1 || cout < 1 ||
+      cip > 96 || cop > 96) {
+    ufem_set_error("probe shape");
+    return AF_ERR_VAL;
+  }
+  if (ugk_init()) return net_fail("device");
+  Vi = (long)Gi.D * Gi.H * Gi.W;
+  Vo = (long)Go.D * Go.H * Go.W;
+  st = ugk_stream_create();
+  hx = calloc((size_t)(nb * Vi), xb);
+  hw = (unsigned short *)calloc((size_t)27 * cop * cip, sizeof(unsigned short));
+  hy = (float *)malloc(sizeof(float) * (size_t)(nb * Vo * ldy));
+  if (!st || !hx || !hw || !hy) goto out;
+  for (i = 0; i < nb * Vi; i++)
+    for (c = 0; c < cin; c++) {
+      if (x_f32) ((float *)hx)[i * ldx + c] = x[i * cin + c];
+      else ((unsigned short *)hx)[i * ldx + c] = h_f2bf(x[i * cin + c]);
+    }
+  for (t = 0; t < 27; t++)
+    for (o = 0; o < cout; o++)
+      for (c = 0; c < cin; c++) hw[((long)t * cop + o) * cip + c] = h_f2bf(w[((long)t * cout + o) * cin + c]);
+  dx_ = ugk_malloc(xb * (size_t)(nb * Vi));
+  dw_ = (unsigned short *)ugk_malloc(sizeof(unsigned short) * (size_t)27 * cop * cip);
+  dy_ = (float *)ugk_malloc(sizeof(float) * (size_t)(nb * Vo * ldy));
+  if (!dx_ || !dw_ || !dy_) goto out;
+  if (ugk_h2d(dx_, hx, xb * (size_t)(nb * Vi), st) || ugk_h2d(dw_, hw, sizeof(unsigned short) * 27 * cop * cip, st))
+    goto out;
+  {
+    ugk_conv_desc d;
+    memset(&d, 0, sizeof(d));
+    d.x = dx_; d.ldx = ldx; d.cin = ldx; d.x_f32 = x_f32;
+    d.w = dw_; d.cip = cip; d.cop = cop;
+    d.y = dy_; d.ldy = ldy; d.cout = cout; d.y_f32 = 1;
+    d.mode = mode; d.flip = flip; d.nb = nb; d.gi = Gi; d.go = Go;
+    if (ugk_conv(&d, st) || ugk_d2h(hy, dy_, sizeof(float) * (size_t)(nb * Vo * ldy), st)) goto out;
+  }
+  for (i = 0; i < nb * Vo; i++)
+    for (c = 0; c < cout; c++) y[i * cout + c] = hy[i * ldy + c];
+  if (dy && dw) {
+    ugk_dw_desc d;
+    int ks;
+    hd = (float *)calloc((size_t)(nb * Vo * ldy), sizeof(float));
+    ddy = (float *)ugk_malloc(sizeof(float) * (size_t)(nb * Vo * ldy));
+    if (!hd || !ddy) goto out;
+    for (i = 0; i < nb * Vo; i++)
+      for (c = 0; c < cout; c++) hd[i * ldy + c] = dy[i * cout + c];
+    if (ugk_h2d(ddy, hd, sizeof(float) * (size_t)(nb * Vo * ldy), st)) goto out;
+    memset(&d, 0, sizeof(d));
+    d.x = dx_; d.ldx = ldx; d.cin = cin; d.x_f32 = x_f32;
+    d.dy = ddy; d.ldd = ldy; d.cout = cout; d.dy_f32 = 1;
+    d.cip = cip; d.cop = cop; d.mode = mode; d.nb = nb; d.gi = Gi; d.go = Go;
+    ks = ugk_conv_dw_slices(&d);
+    part = (float *)ugk_malloc(sizeof(float) * (size_t)ks * 27 * cop * cip);
+    dgr = (float *)ugk_malloc(sizeof(float) * (size_t)27 * cout * cin);
+    if (!part || !dgr) goto out;
+    if (ugk_conv_dw(&d, part, ks, dgr, 0, st) ||
+        ugk_d2h(dw, dgr, sizeof(float) * (size_t)27 * cout * cin, st))
+      goto out;
+  }
+  rv = AF_OK;
+out:
+  if (rv != AF_OK) net_fail("probe");
+  free(hx); free(hw); free(hy); free(hd);
+  ugk_free(dx_); ugk_free(dw_); ugk_free(dy_); ugk_free(ddy); ugk_free(part); ugk_free(dgr);
+  if (st) ugk_stream_destroy(st);
+  return rv;
+}
+
+/* average device time of `reps` launches of one convolution on random data
+ * (for tuning and for the roofline of the network kernels) */
+int ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin, int cout,
+                       int reps, double *us)
+{
+  ugk_grid3 Gi, Go;
+  long Vi, Vo;
+  const int cip = r16(cin), cop = r16(cout), ldx = (cin + 3) / 4 * 4, ldy = (cout + 3) / 4 * 4;
+  void *x = NULL, *st = NULL, *e0 = NULL, *e1 = NULL;
+  unsigned short *w = NULL;
+  float *y = NULL;
+  ugk_conv_desc d;
+  int r, rv = AF_ERR;
+  Gi.D = gi[0]; Gi.H = gi[1]; Gi.W = gi[2];
+  Go.D = go[0]; Go.H = go[1]; Go.W = go[2];
+  if (ugk_init()) return net_fail("device");
+  Vi = (long)Gi.D * Gi.H * Gi.W;
+  Vo = (long)Go.D * Go.H * Go.W;
+  st = ugk_stream_create();
+  e0 = ugk_event_create();
+  e1 = ugk_event_create();
+  x = ugk_malloc((x_f32 ? 4 : 2) * (size_t)(nb * Vi * ldx));
+  w = (unsigned short *)ugk_malloc(2 * (size_t)27 * cop * cip);
+  y = (float *)ugk_malloc(4 * (size_t)(nb * Vo * ldy));
+  if (!st || !e0 || !e1 || !x || !w || !y) goto out;
+  memset(&d, 0, sizeof(d));
+  d.x = x; d.ldx = ldx; d.cin = ldx; d.x_f32 = x_f32;
+  d.w = w; d.cip = cip; d.cop = cop;
+  d.y = y; d.ldy = ldy; d.cout = cout; d.y_f32 = 1;
+  d.mode = mode; d.flip = flip; d.nb = nb; d.gi = Gi; d.go = Go;
+  if (ugk_conv(&d, st) || ugk_sync(st)) goto out;
+  if (ugk_event_record(e0, st)) goto out;
+  for (r = 0; r < reps; r++)
+    if (ugk_conv(&d, st)) goto out;
+  if (ugk_event_record(e1, st) || ugk_sync(st)) goto out;
+  *us = 1000.0 * ugk_event_ms(e0, e1) / reps;
+  rv = AF_OK;
+out:
+  if (rv != AF_OK) net_fail("conv timing");
+  ugk_free(x); ugk_free(w); ugk_free(y);
+  if (e0) ugk_event_destroy(e0);
+  if (e1) ugk_event_destroy(e1);
+  if (st) ugk_stream_destroy(st);
+  return rv;
+}

@@ -28,6 +28,7 @@ void ufem_work_free(ufem_work *w)
   ugk_free(w->dinv); ugk_free(w->rhat); ugk_free(w->phat); ugk_free(w->v); ugk_free(w->s);
   ugk_free(w->shat); ugk_free(w->t); ugk_free(w->r32); ugk_free(w->z32); ugk_free(w->p32[0]);
   ugk_free(w->p32[1]); ugk_free(w->q32); ugk_free(w->x32); ugk_free(w->dinv32);
+  ugk_free(w->fP); ugk_free(w->fQ);
   ugk_free(w->part); ugk_free(w->st);
   ugk_host_free(w->st_host);
   memset(w, 0, sizeof(*w));
@@ -108,6 +109,7 @@ static int run_chunks(ufem_work *w, void *stream, long maxit, int chunk, iter_fn
   chunk += chunk & 1;                    /* even: ping-pong parity repeats */
   ev = ugk_event_create();
   if (!ev) return gpu_fail("event");
+  if (w->hook && w->hook(w->hook_ctx, 0, stream)) { rv = gpu_fail("chunk hook"); goto out; }
   if (use_graph) {
     long k;
     if (ugk_graph_begin(stream)) { rv = gpu_fail("graph capture"); goto out; }
@@ -125,6 +127,7 @@ static int run_chunks(ufem_work *w, void *stream, long maxit, int chunk, iter_fn
         if (fn(ctx, it + k, stream)) { rv = gpu_fail("iteration"); goto out; }
     }
     it += chunk;
+    if (w->hook && w->hook(w->hook_ctx, it, stream)) { rv = gpu_fail("chunk hook"); goto out; }
     if (!first) {
       if (ugk_event_sync(ev)) { rv = gpu_fail("event sync"); goto out; }
       if (w->st_host->done) break;
@@ -324,6 +327,77 @@ static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_wor
   return info->status;
 }
 
+/* Flexible CG with the learned preconditioner (o->net bound to this system):
+ * per iteration the head kernel tests convergence and writes the network
+ * input, the network runs on the same stream (skipped through st->halt once
+ * done or after the Jacobi fallback), then z = dinv r + s mask N, the
+ * Polak-Ribiere direction update, q = K p and the x, r update.  The whole
+ * iteration is captured in the chunk graph like the PCG kernels. */
+typedef struct {
+  const ugk_mat *K;
+  long n;
+  double *x;
+  ufem_work *w;
+  ufem_net *net;
+  int m;
+} fcg_ctx;
+
+static int fcg_step(void *c, long it, void *stream)
+{
+  fcg_ctx *f = (fcg_ctx *)c;
+  ufem_work *w = f->w;
+  (void)it;
+  if (ugk_fcg_head(f->K, f->n, w->r, w->dinv, ufem_net_input(f->net), w->st, w->part, stream) ||
+      ufem_net_infer(f->net, &w->st->halt, stream) ||
+      ugk_fcg_combine(f->n, w->r, w->dinv, ufem_net_mask(f->net), ufem_net_output(f->net), w->fP, w->fQ, f->m,
+                      w->z, w->st, w->part, stream) ||
+      ugk_fcg_p(f->n, w->r, w->dinv, w->z, w->fP, f->m, w->st, w->part, stream) ||
+      ugk_fcg_q(f->K, f->n, w->fP, w->fQ, f->m, w->st, w->part, stream))
+    return -1;
+  return ugk_fcg_update(f->K, f->n, f->x, w->r, w->fP, w->fQ, f->m, w->dinv, w->st, w->part, stream);
+}
+
+static int solve_learned(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
+                         const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+{
+  ufem_net *N = (ufem_net *)o->net;
+  const int m = o->fcg_m > 0 ? o->fcg_m : 4;
+  fcg_ctx c = {K, n, x, w, N, m};
+  ugk_state *h = w->st_host;
+  int rv;
+  if (!N || ufem_net_bound_n(N) != n) {
+    ufem_set_error("learned solver: no network bound to this system");
+    return AF_ERR_VAL;
+  }
+  if (m > 8) { ufem_set_error("learned solver: at most 8 kept directions"); return AF_ERR_VAL; }
+  if (w->fm < m) {
+    ugk_free(w->fP);
+    ugk_free(w->fQ);
+    w->fP = (double *)ugk_malloc(sizeof(double) * (size_t)n * (size_t)m);
+    w->fQ = (double *)ugk_malloc(sizeof(double) * (size_t)n * (size_t)m);
+    w->fm = (w->fP && w->fQ) ? m : 0;
+    if (!w->fm) return gpu_fail("direction rings");
+  }
+  rv = prepare(K, w, stream, o->eps, o->rule, maxit);
+  if (rv != AF_OK) return rv;
+  if (ufem_net_prepare(N, stream)) return AF_ERR;
+  if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
+  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, fcg_step, &c, 1);
+  if (rv != AF_OK) return rv;
+  if (get_state(w, stream)) return AF_ERR;
+  if (h->bb <= 0.0) {
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+    return AF_OK;
+  }
+  info->iters = h->iter;
+  info->fallback = h->fb;
+  info->outer = h->fb ? h->fb_iter : 0;
+  info->relres = sqrt(h->rr / h->bb);
+  info->status = status_of(h->done);
+  return info->status;
+}
+
 int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
                       void *ev0, void *ev1, const ufem_solve_opts *o, ufem_solve_info *info)
 {
@@ -335,6 +409,7 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem
     case UFEM_SOLVER_PCG: rv = solve_pcg(K, n, b, x, w, stream, o, maxit, info); break;
     case UFEM_SOLVER_BICGSTAB: rv = solve_bicgs(K, n, b, x, w, stream, o, maxit, info); break;
     case UFEM_SOLVER_MIXED: rv = solve_mixed((ugk_mat *)K, n, b, x, w, stream, o, maxit, info); break;
+    case UFEM_SOLVER_LEARNED: rv = solve_learned(K, n, b, x, w, stream, o, maxit, info); break;
     default: ufem_set_error("unknown solver %d", o->solver); return AF_ERR_VAL;
   }
   if (ugk_event_record(ev1, stream)) return gpu_fail("event");

@@ -390,6 +390,15 @@ fail:
 
 /* fem_fill_K(AF_NO), fem/fem_sol.c:897-989: K = sum of element matrices,
  * here one kernel launch per colour in a fixed order */
+int ufem_sys_set_E(ufem_sys *s, const double *E)
+{
+  if (ugk_h2d(s->d_E, E, sizeof(double) * (size_t)s->n_elems, s->stream) || ugk_sync(s->stream))
+    return gpu_fail("E upload");
+  s->filled = 0;
+  s->bc_applied = 0;
+  return AF_OK;
+}
+
 int ufem_sys_fill_K(ufem_sys *s, int npts)
 {
   int c;

@@ -66,6 +66,11 @@ typedef struct {
   int done;           /* 0 running, 1 converged, 2 breakdown, 3 maxit   */
   int err;            /* kernel-detected error (zero diagonal, detJ<=0)  */
   int pad;
+  double net_s;       /* flexible CG: rms of dinv r fed to the network   */
+  double pqs[8];      /* flexible CG: (p, A p) of the kept directions     */
+  long fb_iter;       /* flexible CG: iteration of the Jacobi fallback    */
+  int fb;             /* flexible CG: 1 once the network is switched off  */
+  int halt;           /* flexible CG: done or fallback (network skipped)  */
   unsigned int counter[8];  /* last-block tickets, reset by the last block */
 } ugk_state;
 
@@ -231,6 +236,106 @@ int ugk_batch_occupancy(int wg);
 /* dinv = 1 / diag(K) for one block-3 matrix (no state) */
 int ugk_diag_inv3(const ugk_mat *K, double *d_dinv, ugk_state *d_st, void *stream);
 long ugk_wall_clock_khz(void);
+
+/* ---------------- flexible CG with a nonlinear preconditioner ------------- */
+/* FCG(m): directions are A-orthogonalised against the last m (1..8), kept in
+ * ring slots (iteration mod m) of d_P and d_Q (m vectors of n each).  Per
+ * iteration: ugk_fcg_head (convergence test on the previous update's
+ * partials; network input X0[v][c] = sqrt(dinv) r / s, s = rms(sqrt(dinv) r),
+ * for c < 3, channel 3 left alone), the network (ugk_conv and friends,
+ * skipped through st->halt), ugk_fcg_combine (z = dinv r + s mask sqrt(dinv)
+ * out and the dots with the kept directions), ugk_fcg_p (coefficients, descent check and
+ * Jacobi fallback, new direction), ugk_fcg_q (q = A p), ugk_fcg_update.
+ * out: f32 [n/3][4]. */
+int ugk_fcg_start(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r,
+                  const double *d_dinv, double *d_part, void *stream);
+int ugk_fcg_head(const ugk_mat *K, long n, const double *d_r, const double *d_dinv, float *d_x0,
+                 ugk_state *d_st, double *d_part, void *stream);
+int ugk_fcg_combine(long n, const double *d_r, const double *d_dinv, const float *d_mask,
+                    const float *d_out, const double *d_P, const double *d_Q, int m, double *d_z,
+                    ugk_state *d_st, double *d_part, void *stream);
+int ugk_fcg_p(long n, const double *d_r, const double *d_dinv, const double *d_z, double *d_P, int m,
+              ugk_state *d_st, double *d_part, void *stream);
+int ugk_fcg_q(const ugk_mat *K, long n, const double *d_P, double *d_Q, int m, ugk_state *d_st,
+              double *d_part, void *stream);
+int ugk_fcg_update(const ugk_mat *K, long n, double *d_x, double *d_r, const double *d_P, const double *d_Q,
+                   int m, const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
+
+/* ---------------- 3-D convolution network kernels (ugk_unet.hip) ---------- */
+/* Activations are bf16 (stored as unsigned short), channels-last on a node
+ * grid [nb][D][H][W][ld]: channel c of voxel v of sample b sits at
+ * ((b * D + z) * H + y) * W + x) * ld + c.  Every ld and every channel
+ * offset folded into a pointer is a multiple of 4. */
+typedef struct { int D, H, W; } ugk_grid3;
+
+enum { UGK_CONV_S1 = 0,   /* y[v] = sum_t W[t] x[v + off_t]                 */
+       UGK_CONV_S2 = 1,   /* y[c] = sum_t W[t] x[2 c + off_t]  (coarse out)  */
+       UGK_CONV_T2 = 2 }; /* y[f] = sum_t W[t] x[(f - off_t) / 2], even only */
+
+/* 3x3x3 taps t = 9 (dz+1) + 3 (dy+1) + (dx+1), zero padding, no bias.
+ * Activations x are bf16, or f32 (x_f32) fed to the bf16 matrix cores as an
+ * exact three-way bf16 split.  Epilogue per output element:
+ * v = acc (* emul) (+ ga * gb) (f32 fields, optional), stored as bf16, or f32
+ * if y_f32. */
+typedef struct {
+  const void *x; int ldx, cin, x_f32;      /* cin % 4 == 0, cin <= cip        */
+  const unsigned short *w; int cip, cop;   /* [27][cop][cip], zero padded, %16 */
+  void *y; int ldy, cout, y_f32;           /* cout <= cop                      */
+  const float *emul; int ldm;              /* optional (NULL)                  */
+  const float *ga; int ldga;               /* optional (NULL)                  */
+  const float *gb; int ldgb;
+  int mode, flip, nb;                      /* flip: use tap 26 - t             */
+  ugk_grid3 gi, go;
+  const int *skip;                         /* optional: no-op while *skip != 0 */
+} ugk_conv_desc;
+int ugk_conv(const ugk_conv_desc *d, void *stream);
+
+/* weight gradient of y = conv(x) (mode S1 or S2), x and dy bf16 or f32
+ * (rounded to bf16 for the matrix cores): block partials over ks voxel
+ * slices into d_part [ks][27][cop][cip] (f32), then the fixed-order sum over
+ * slices into d_grad [27][cout][cin] (added to it when accumulate) */
+typedef struct {
+  const void *x; int ldx, cin, x_f32;
+  const void *dy; int ldd, cout, dy_f32;
+  int cip, cop, mode, nb;
+  ugk_grid3 gi, go;
+} ugk_dw_desc;
+int ugk_conv_dw(const ugk_dw_desc *d, float *d_part, int ks, float *d_grad, int accumulate, void *stream);
+int ugk_conv_dw_slices(const ugk_dw_desc *d);   /* ks the call above will use */
+
+/* out = a * b elementwise (f32) */
+int ugk_mul(long n, const float *d_a, const float *d_b, float *d_out, const int *d_skip, void *stream);
+/* gates g[v][c] = exp(b[c] - (1 + a[c]) eb[v]) for nvox voxels, C channels */
+int ugk_gates(long nvox, int C, const float *d_eb, const float *d_a, const float *d_b, float *d_g,
+              void *stream);
+/* gate gradient: q = g h dt; d_grad_b[c] = sum q, d_grad_a[c] = -sum eb q
+ * (d_part: 512 x 2C doubles; C divides 256) */
+int ugk_gate_grad(long nvox, int C, const float *d_g, const float *d_h, const float *d_dt, const float *d_eb,
+                  double *d_part, float *d_grad_a, float *d_grad_b, void *stream);
+/* energy loss of one training pair (r, e) scaled by s, network output y
+ * [n/3][4]: z = dinv rr + mask sqrt(dinv) y; then, after kz = K z,
+ * loss partials sum (ee - z)(rr - kz) and dy = coef mask sqrt(dinv) (rr - kz);
+ * returns the number of partials */
+int ugk_el_z(long n, const float *d_rr, const float *d_dinv, const float *d_mask, const float *d_y,
+             float *d_z, void *stream);
+int ugk_el_grad(long n, const float *d_rr, const float *d_ee, const float *d_z, const float *d_kz,
+                const float *d_dinv, const float *d_mask, float coef, float *d_dy, double *d_lpart,
+                void *stream);
+/* Adam on one f32 weight tensor [27][cout][cin] (or a vector: 1 x P x 1);
+ * refreshes the bf16 forward [27][cop][cip] and transposed [27][cip][cop]
+ * copies when d_wf is not NULL.  lr <= 0: copies only */
+int ugk_adam(long P, int cout, int cin, int cop, int cip, float *d_w, const float *d_g, float *d_m,
+             float *d_v, unsigned short *d_wf, unsigned short *d_wt, float lr, float b1, float b2,
+             float eps, long step, void *stream);
+/* training pair from a PCG snapshot (x, r) and the converged u: per-block
+ * partials of (sum dinv r^2, sum (u - x) r) (returns the blocks); then, with
+ * s = rms(sqrt(dinv) r): V[v] = mask sqrt(dinv) r / s, rr = r / s,
+ * ee = (u - x) / s (f32) */
+int ugk_pair_stats(long n, const double *d_x, const double *d_r, const double *d_u,
+                   const double *d_dinv, double *d_part, void *stream);
+int ugk_pair_write(long n, const double *d_x, const double *d_r, const double *d_u,
+                   const double *d_dinv, const float *d_mask, double s, float *d_V, float *d_rr,
+                   float *d_ee, void *stream);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@ static int fail(const char *what, hipError_t e)
   snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
   return -1;
 }
+// error reporting for the other kernel sources of the library
+int ugk_hip_fail(const char *what, hipError_t e) { return fail(what, e); }
 #define CK(call)                                  \
   do {                                            \
     hipError_t e_ = (call);                       \
@@ -1588,6 +1590,235 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
 // ==========================================================================
 // C-ABI
 // ==========================================================================
+// --------------------------------------------------------------------------
+// flexible CG with the learned preconditioner z = M(r), truncated to the
+// last m directions (Notay's FCG(m)):
+//   z    = dinv r + s mask sqrt(dinv) N(sqrt(dinv) r / s),  s = rms(sqrt(dinv) r)
+//   p_k  = z_k - sum_j (z_k, q_j) / (p_j, q_j) p_j    over the m previous
+//          directions p_j, q_j = A p_j (for m = 1 the Polak-Ribiere update
+//          (z_k, r_k - r_{k-1}) / (z_{k-1}, r_{k-1}), since r_k - r_{k-1} =
+//          -alpha_{k-1} q_{k-1})
+//   alpha = (p_k, r_k) / (p_k, q_k)
+// Direction k lives in ring slot k mod m of P and Q (m vectors each).  When
+// (p_k, r_k) <= 0 (not a descent direction) or is not finite, the iteration
+// restarts along dinv r and stays Jacobi PCG from there on (st->fb).
+// Partials: B2(parity) update -> head: (r, dinv r), (r,r), (x,x), (b,b);
+// S combine -> p: (z,r), (dinv r, r), (z, q_j) x 8, (p_j, r) x 8;
+// A SpMV -> update: (p,q).
+// --------------------------------------------------------------------------
+#define FCG_MAXM 8
+#define FCG_NV (2 + 2 * FCG_MAXM)
+
+template <int BS>
+__global__ __launch_bounds__(WG) void k_fcg_start(SellView M, const double *__restrict__ vals,
+                                                  const double *__restrict__ b,
+                                                  const double *__restrict__ x, double *r,
+                                                  const double *__restrict__ dinv, double *part)
+{
+  double v[4] = {0, 0, 0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      const double ri = b[i] - acc[k];
+      r[i] = ri;
+      v[0] += dinv[i] * ri * ri;
+      v[1] += ri * ri;
+      v[2] += x[i] * x[i];
+      v[3] += b[i] * b[i];
+    }
+  }
+  block_partials<4>(v, PART_B2(part, 0));
+}
+
+__global__ __launch_bounds__(WG) void k_fcg_head(long n, const double *__restrict__ r,
+                                                 const double *__restrict__ dinv, float *x0,
+                                                 ugk_state *st, const double *part, unsigned G0, unsigned GB)
+{
+  if (st->done) return;
+  const long k = st->iter;
+  double t[4];
+  reduce_prev<4>(PART_B2(part, k & 1), k == 0 ? G0 : GB, t);
+  const double bb = (k == 0) ? t[3] : st->bb;
+  const int done = pcg_test(st, k, t[1], t[2], bb, t[0]);
+  const int fb = st->fb;
+  const double sc = sqrt(t[0] / (double)n);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) {
+    st->cur = k;
+    st->rr = t[1];
+    st->xx = t[2];
+    if (k == 0) st->bb = bb;
+    if (done) st->done = done;
+    st->halt = (done || fb) ? 1 : 0;
+    st->net_s = sc;
+  }
+  if (done || fb) return;
+  const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const long v = i / 3;
+    x0[v * 4 + (i - 3 * v)] = (float)(sqrt(dinv[i]) * r[i] * inv);
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_fcg_combine(long n, const double *__restrict__ r,
+                                                    const double *__restrict__ dinv,
+                                                    const float *__restrict__ mask,
+                                                    const float *__restrict__ out,
+                                                    const double *__restrict__ P,
+                                                    const double *__restrict__ Q, int m,
+                                                    double *__restrict__ z, const ugk_state *st,
+                                                    double *part)
+{
+  if (st->done) return;
+  const long k = st->cur;
+  const int nv = k < m ? (int)k : m;          // slots 0 .. nv-1 hold directions
+  const double sc = st->net_s;
+  const int fb = st->fb;
+  double v[FCG_NV];
+#pragma unroll
+  for (int j = 0; j < FCG_NV; j++) v[j] = 0.0;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double ri = r[i];
+    const double a = dinv[i] * ri;
+    double zi = a;
+    if (!fb) {
+      const long vx = i / 3;
+      zi += sc * (double)mask[i] * sqrt(dinv[i]) * (double)out[vx * 4 + (i - 3 * vx)];
+    }
+    z[i] = zi;
+    v[0] += zi * ri;
+    v[1] += a * ri;
+#pragma unroll
+    for (int j = 0; j < FCG_MAXM; j++) {
+      if (j < nv) {
+        v[2 + j] += zi * Q[(long)j * n + i];
+        v[2 + FCG_MAXM + j] += P[(long)j * n + i] * ri;
+      }
+    }
+  }
+  block_partials<FCG_NV>(v, PART_S(part));
+}
+
+__global__ __launch_bounds__(WG) void k_fcg_p(long n, const double *__restrict__ r,
+                                              const double *__restrict__ dinv,
+                                              const double *__restrict__ z, double *P, int m,
+                                              ugk_state *st, const double *part, unsigned GV)
+{
+  if (st->done) return;
+  const long k = st->cur;
+  const int nv = k < m ? (int)k : m;
+  double t[FCG_NV];
+  reduce_prev<FCG_NV>(PART_S(part), GV, t);
+  double beta[FCG_MAXM];
+  double pr = t[0];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < FCG_MAXM; j++) {
+    beta[j] = (j < nv) ? -t[2 + j] / st->pqs[j] : 0.0;
+    ok = ok && isfinite(beta[j]);
+    pr += beta[j] * t[2 + FCG_MAXM + j];
+  }
+  // st->fb is set by block 0 below, so every block reads the copy the head
+  // kernel latched (halt == fb here, since done returned above)
+  const bool restart = !st->halt && !(ok && pr > 0.0 && isfinite(pr));
+  if (restart) {
+#pragma unroll
+    for (int j = 0; j < FCG_MAXM; j++) beta[j] = 0.0;
+    pr = t[1];
+  }
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(pr > 0.0) || !isfinite(pr)) {
+    if (lead) {
+      st->done = 2;
+      st->halt = 1;
+    }
+    return;
+  }
+  if (lead) {
+    if (restart) {
+      st->fb = 1;
+      st->fb_iter = k;
+    }
+    st->rzB = pr;
+    st->beta = beta[0];
+  }
+  const long slot = k % m;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    double acc = restart ? dinv[i] * r[i] : z[i];
+#pragma unroll
+    for (int j = 0; j < FCG_MAXM; j++)
+      if (j < nv) acc += beta[j] * P[(long)j * n + i];   // slot k mod m is read before it is overwritten
+    P[slot * n + i] = acc;
+  }
+}
+
+// q = A p for the direction of this iteration (ring slot), partial (p, q)
+template <int BS>
+__global__ __launch_bounds__(WG) void k_fcg_q(SellView M, const double *__restrict__ vals,
+                                              const double *__restrict__ P, double *__restrict__ Q, long n,
+                                              int m, const ugk_state *st, double *part)
+{
+  if (st->done) return;
+  const long slot = st->cur % m;
+  const double *p = P + slot * n;
+  double *q = Q + slot * n;
+  double v[1] = {0.0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, p, acc, row)) {
+#pragma unroll
+    for (int r = 0; r < BS; r++) {
+      const long i = row * BS + r;
+      q[i] = acc[r];
+      v[0] += p[i] * acc[r];
+    }
+  }
+  block_partials<1>(v, PART_A(part));
+}
+
+__global__ __launch_bounds__(WG) void k_fcg_update(long n, double *__restrict__ x, double *__restrict__ r,
+                                                   const double *__restrict__ P,
+                                                   const double *__restrict__ Q, int m,
+                                                   const double *__restrict__ dinv, ugk_state *st,
+                                                   double *part, unsigned GA)
+{
+  if (st->done) return;
+  const long k = st->cur;
+  const long slot = k % m;
+  const double *p = P + slot * n, *q = Q + slot * n;
+  double t[1];
+  reduce_prev<1>(PART_A(part), GA, t);
+  const double pq = t[0];
+  const double alpha = st->rzB / pq;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(pq > 0.0) || !isfinite(alpha)) {
+    if (lead) {
+      st->done = 2;
+      st->halt = 1;
+    }
+    return;
+  }
+  double v[4] = {0, 0, 0, 0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double xi = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    x[i] = xi;
+    r[i] = ri;
+    v[0] += dinv[i] * ri * ri;
+    v[1] += ri * ri;
+    v[2] += xi * xi;
+  }
+  block_partials<4>(v, PART_B2(part, (k + 1) & 1));
+  if (lead) {
+    st->pqs[slot] = pq;
+    st->alpha = alpha;
+    st->iter = k + 1;
+  }
+}
+
 static SellView view(const ugk_mat *K)
 {
   SellView M;
@@ -2146,6 +2377,57 @@ long ugk_wall_clock_khz(void)
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0;
   return khz;
+}
+
+int ugk_fcg_start(const ugk_mat *K, long n, const double *b, const double *x, double *r, const double *dinv,
+                  double *part, void *s)
+{
+  (void)n;
+  if (K->bs == 3) k_fcg_start<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, part);
+  else k_fcg_start<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, part);
+  CK_LAUNCH("k_fcg_start");
+  return 0;
+}
+
+int ugk_fcg_head(const ugk_mat *K, long n, const double *r, const double *dinv, float *x0,
+                 ugk_state *st, double *part, void *s)
+{
+  k_fcg_head<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, x0, st, part, spmv_blocks(K), vec_blocks(n));
+  CK_LAUNCH("k_fcg_head");
+  return 0;
+}
+
+int ugk_fcg_combine(long n, const double *r, const double *dinv, const float *mask, const float *out,
+                    const double *P, const double *Q, int m, double *z, ugk_state *st, double *part, void *s)
+{
+  if (m < 1 || m > FCG_MAXM) { snprintf(g_err, sizeof(g_err), "FCG: 1 <= m <= %d", FCG_MAXM); return -1; }
+  k_fcg_combine<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, mask, out, P, Q, m, z, st, part);
+  CK_LAUNCH("k_fcg_combine");
+  return 0;
+}
+
+int ugk_fcg_p(long n, const double *r, const double *dinv, const double *z, double *P, int m, ugk_state *st,
+              double *part, void *s)
+{
+  k_fcg_p<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, z, P, m, st, part, vec_blocks(n));
+  CK_LAUNCH("k_fcg_p");
+  return 0;
+}
+
+int ugk_fcg_q(const ugk_mat *K, long n, const double *P, double *Q, int m, ugk_state *st, double *part, void *s)
+{
+  if (K->bs == 3) k_fcg_q<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, P, Q, n, m, st, part);
+  else k_fcg_q<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, P, Q, n, m, st, part);
+  CK_LAUNCH("k_fcg_q");
+  return 0;
+}
+
+int ugk_fcg_update(const ugk_mat *K, long n, double *x, double *r, const double *P, const double *Q, int m,
+                   const double *dinv, ugk_state *st, double *part, void *s)
+{
+  k_fcg_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, P, Q, m, dinv, st, part, spmv_blocks(K));
+  CK_LAUNCH("k_fcg_update");
+  return 0;
 }
 
 }  // extern "C"

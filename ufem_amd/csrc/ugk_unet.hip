@@ -1,0 +1,640 @@
+// ugk_unet.hip -- BF16 matrix-core kernels of the learned preconditioner's
+// 3-D convolution network (gfx950).
+//
+// All activations are bf16, channels-last on the structured node grid, so a
+// (voxel, 4 channels) group is one 8-byte load.  A 3x3x3 convolution is an
+// implicit GEMM on the 16x16x16 bf16 MFMA: out-channels x voxels, summed over
+// 27 taps x in-channels.  The same kernel computes data gradients: a
+// stride-1 layer's gradient is the convolution with the flipped, transposed
+// taps; a stride-2 layer's is the "T2" transpose mode, where output voxel f
+// takes input (f - off) / 2 when that is integral.  Weight gradients stage a
+// 64-voxel chunk of the output gradient and of the gathered input, both
+// transposed, in LDS, and each block sums one tap over a slice of voxels;
+// slices are summed in a fixed order afterwards (no atomics, so training
+// steps repeat bit for bit).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "ugk.h"
+
+int ugk_hip_fail(const char *what, hipError_t e);   // ugk_kernels.hip
+
+#define CK(call)                                            \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return ugk_hip_fail(#call, e_);   \
+  } while (0)
+#define CK_LAUNCH(what) CK(hipGetLastError())
+
+typedef unsigned short bf16;
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define SLOPE 0.1f
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ float bf2f(short h) { return __uint_as_float((unsigned)(unsigned short)h << 16); }
+// round to nearest even
+__device__ __forceinline__ bf16 f2bf(float f)
+{
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+__host__ __device__ static inline long vox(ugk_grid3 g) { return (long)g.D * g.H * g.W; }
+
+// input voxel of tap (dz, dy, dx) for output voxel (z, y, x); -1 if none
+template <int MODE>
+__device__ __forceinline__ long tap_src(int z, int y, int x, int dz, int dy, int dx, const ugk_grid3 &gi)
+{
+  int zi, yi, xi;
+  if (MODE == UGK_CONV_S1) {
+    zi = z + dz; yi = y + dy; xi = x + dx;
+  } else if (MODE == UGK_CONV_S2) {
+    zi = 2 * z + dz; yi = 2 * y + dy; xi = 2 * x + dx;
+  } else {
+    const int a = z - dz, b = y - dy, c = x - dx;
+    if ((a | b | c) & 1) return -1;        // odd (also catches -1)
+    zi = a >> 1; yi = b >> 1; xi = c >> 1;
+  }
+  if (zi < 0 || zi >= gi.D || yi < 0 || yi >= gi.H || xi < 0 || xi >= gi.W) return -1;
+  return ((long)zi * gi.H + yi) * gi.W + xi;
+}
+
+// --------------------------------------------------------------------------
+// convolution: each wave computes NCO x 16 output channels for NV x 16
+// voxels (A = weights [co][ci], B = activations [ci][voxel]; a lane of the
+// accumulator holds 4 consecutive channels of one voxel -> 8/16-byte stores).
+// XF: f32 activations, each split exactly into three bf16 parts
+// (x = hi + mid + lo to 24 bits) that are multiplied by the same bf16
+// weights, so the product is the f32 activation times the bf16 weight.
+// --------------------------------------------------------------------------
+// B fragment of 4 channels: f32 loaded raw (split after the load arrives),
+// or bf16 as is
+template <int XF> struct Frag;
+template <> struct Frag<1> { typedef f32x4 T; };
+template <> struct Frag<0> { typedef bf16x4 T; };
+
+template <int XF>
+__device__ __forceinline__ typename Frag<XF>::T load_raw(const void *x, long off)
+{
+  if constexpr (XF) return *(const f32x4 *)((const float *)x + off);
+  else return *(const bf16x4 *)((const bf16 *)x + off);
+}
+
+// three-way bf16 split of 4 f32 values (x = hi + mid + lo to 24 bits);
+// invalid lanes give zeros
+__device__ __forceinline__ void split3(const f32x4 &x, bool ok, bf16x4 (&b)[3])
+{
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const float v = ok ? x[j] : 0.f;
+    const bf16 h = f2bf(v);
+    const float r1 = v - bf2f(h);
+    const bf16 m = f2bf(r1);
+    b[0][j] = (short)h;
+    b[1][j] = (short)m;
+    b[2][j] = (short)f2bf(r1 - bf2f(m));
+  }
+}
+
+// Each wave computes one 16-channel output group (blockIdx.y) for NV x 16
+// voxels; the 27 taps are unrolled without branches (clamped addresses,
+// zeroed fragments), so the gathers of many taps are in flight together.
+template <int NV, int MODE, int XF>
+__global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
+{
+  if (d.skip && *d.skip) return;
+  constexpr int NS = XF ? 3 : 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int mg = blockIdx.y;
+  const long Vo = vox(d.go), Vi = vox(d.gi);
+  const long total = Vo * d.nb;
+  const long base = ((long)blockIdx.x * 4 + wave) * (16 * NV);
+  if (base >= total) return;               // no block-wide barriers below
+  const int HWo = d.go.H * d.go.W;
+  int vz[NV], vy[NV], vx[NV];
+  long vb[NV];
+  bool vok[NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) {
+    const long g = base + n * 16 + col;
+    vok[n] = g < total;
+    const long gg = vok[n] ? g : 0;
+    const long b = gg / Vo;
+    const int v = (int)(gg - b * Vo);
+    vz[n] = v / HWo;
+    const int r = v - vz[n] * HWo;
+    vy[n] = r / d.go.W;
+    vx[n] = r - vy[n] * d.go.W;
+    vb[n] = b * Vi;
+  }
+  f32x4 acc[NV];
+#pragma unroll
+  for (int n = 0; n < NV; n++) acc[n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const bf16 *wrow = d.w + (long)(mg * 16 + col) * d.cip;
+  const long wstep = (long)d.cop * d.cip;
+
+  for (int c0 = 0; c0 < d.cin; c0 += 16) {
+    const int c = c0 + 4 * kq;
+    const bool cok = c < d.cin;
+    const int cs = cok ? c : 0;              // in-bounds channel for masked lanes
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      typename Frag<XF>::T raw[NV];
+      bool ok[NV];
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        const long sidx = tap_src<MODE>(vz[n], vy[n], vx[n], dz, dy, dx, d.gi);
+        ok[n] = vok[n] && cok && sidx >= 0;
+        raw[n] = load_raw<XF>(d.x, (ok[n] ? vb[n] + sidx : 0) * d.ldx + cs);
+      }
+      const bf16x4 a = *(const bf16x4 *)(wrow + (d.flip ? 26 - t : t) * wstep + c);
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        bf16x4 bx[3];
+        if constexpr (XF) split3(raw[n], ok[n], bx);
+        else bx[0] = ok[n] ? raw[n] : (bf16x4){0, 0, 0, 0};
+#pragma unroll
+        for (int k = NS - 1; k >= 0; k--)      // smallest part first
+          acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[k], acc[n], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: lane holds channels mg*16 + 4 kq + j of voxel base + n*16 + col
+  const int co = mg * 16 + 4 * kq;
+  if (co >= d.cout) return;
+  const int nj = d.cout - co < 4 ? d.cout - co : 4;
+#pragma unroll
+  for (int n = 0; n < NV; n++) {
+    if (!vok[n]) continue;
+    const long g = base + n * 16 + col;
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = acc[n][j];
+    if (d.emul)
+      for (int j = 0; j < nj; j++) v[j] *= d.emul[g * d.ldm + co + j];
+    if (d.ga)
+      for (int j = 0; j < nj; j++) v[j] += d.ga[g * d.ldga + co + j] * d.gb[g * d.ldgb + co + j];
+    if (d.y_f32) {
+      float *y = (float *)d.y + g * d.ldy + co;
+      if (nj == 4) *(f32x4 *)y = (f32x4){v[0], v[1], v[2], v[3]};
+      else for (int j = 0; j < nj; j++) y[j] = v[j];
+    } else {
+      bf16 *y = (bf16 *)d.y + g * d.ldy + co;
+      if (nj == 4) *(bf16x4 *)y = (bf16x4){(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
+      else for (int j = 0; j < nj; j++) y[j] = f2bf(v[j]);
+    }
+  }
+}
+
+template <int NV, int XF>
+static void launch_conv_nv(const ugk_conv_desc &d, dim3 grid, hipStream_t s)
+{
+  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF><<<grid, 256, 0, s>>>(d);
+  else if (d.mode == UGK_CONV_S2) k_conv<NV, UGK_CONV_S2, XF><<<grid, 256, 0, s>>>(d);
+  else k_conv<NV, UGK_CONV_T2, XF><<<grid, 256, 0, s>>>(d);
+}
+
+template <int XF>
+static void launch_conv(const ugk_conv_desc &d, long total, hipStream_t s)
+{
+  // two voxel tiles per wave only when there are waves to spare
+  const unsigned groups = (unsigned)(d.cop / 16);
+  static int nv_env = -1;
+  if (nv_env < 0) {
+    const char *e = getenv("UFEM_CONV_NV");
+    nv_env = e ? atoi(e) : 0;
+  }
+  const bool two = nv_env ? nv_env == 2 : total * groups >= 64L * 4 * 2048;
+  if (two) launch_conv_nv<2, XF>(d, dim3((unsigned)((total + 127) / 128), groups), s);
+  else launch_conv_nv<1, XF>(d, dim3((unsigned)((total + 63) / 64), groups), s);
+}
+
+// --------------------------------------------------------------------------
+// weight gradient: block (slice, tap) accumulates dW[tap] tiles over its
+// voxel chunks; TPW output tiles of 16 x 16 per wave
+// --------------------------------------------------------------------------
+#define DW_PITCH 68      // bf16 per LDS row: 64 voxels + 4 (8-byte aligned rows)
+#define DW_MAX_COP 64
+#define DW_MAX_CIP 96
+
+// 4 channels of a bf16 or f32 field as bf16 (f32 rounded to nearest even)
+template <int F32>
+__device__ __forceinline__ bf16x4 load4_bf(const void *p, long off)
+{
+  if (F32) {
+    const f32x4 x = *(const f32x4 *)((const float *)p + off);
+    return (bf16x4){(short)f2bf(x[0]), (short)f2bf(x[1]), (short)f2bf(x[2]), (short)f2bf(x[3])};
+  }
+  return *(const bf16x4 *)((const bf16 *)p + off);
+}
+
+template <int TPW, int MODE, int XF, int DF>
+__global__ __launch_bounds__(256) void k_conv_dw(ugk_dw_desc d, float *part, long cps)
+{
+  __shared__ __attribute__((aligned(16))) bf16 sdy[DW_MAX_COP * DW_PITCH];
+  __shared__ __attribute__((aligned(16))) bf16 sx[DW_MAX_CIP * DW_PITCH];
+  __shared__ long src[64];
+  const int t = blockIdx.y;
+  const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+  const long Vo = vox(d.go), Vi = vox(d.gi);
+  const long total = Vo * d.nb;
+  const long nch = (total + 63) / 64;
+  const long ch0 = (long)blockIdx.x * cps;
+  const long ch1 = ch0 + cps < nch ? ch0 + cps : nch;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int nci = d.cip / 16, ntiles = (d.cop / 16) * nci;
+  const int gco = d.cop / 4, gci = d.cip / 4;       // staged 4-channel groups
+  const int rco = (d.cout + 3) / 4, rci = (d.cin + 3) / 4;   // groups read from memory
+  const int HWo = d.go.H * d.go.W;
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; q++) acc[q] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  for (long ch = ch0; ch < ch1; ch++) {
+    const long g0 = ch * 64;
+    __syncthreads();                         // previous chunk fully consumed
+    if (threadIdx.x < 64) {
+      const long g = g0 + threadIdx.x;
+      long s = -1;
+      if (g < total) {
+        const long b = g / Vo;
+        const int v = (int)(g - b * Vo);
+        const int z = v / HWo, r = v - z * HWo, y = r / d.go.W, x = r - y * d.go.W;
+        s = tap_src<MODE>(z, y, x, dz, dy, dx, d.gi);
+        if (s >= 0) s += b * Vi;
+      }
+      src[threadIdx.x] = s;
+    }
+    for (int i = threadIdx.x; i < 64 * gco; i += 256) {
+      const int vl = i / gco, grp = i - vl * gco;
+      const long g = g0 + vl;
+      bf16x4 v = (bf16x4){0, 0, 0, 0};
+      if (g < total && grp < rco) v = load4_bf<DF>(d.dy, g * d.ldd + 4 * grp);
+#pragma unroll
+      for (int j = 0; j < 4; j++) sdy[(4 * grp + j) * DW_PITCH + vl] = (bf16)v[j];
+    }
+    __syncthreads();                         // src[] ready
+    for (int i = threadIdx.x; i < 64 * gci; i += 256) {
+      const int vl = i / gci, grp = i - vl * gci;
+      const long s = src[vl];
+      bf16x4 v = (bf16x4){0, 0, 0, 0};
+      if (s >= 0 && grp < rci) v = load4_bf<XF>(d.x, s * d.ldx + 4 * grp);
+#pragma unroll
+      for (int j = 0; j < 4; j++) sx[(4 * grp + j) * DW_PITCH + vl] = (bf16)v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int tile = wave + 4 * q;
+      if (tile < ntiles) {                   // wave-uniform
+        const int mt = tile / nci, nt = tile - mt * nci;
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+          const bf16x4 a = *(const bf16x4 *)(sdy + (mt * 16 + col) * DW_PITCH + ks * 16 + 4 * kq);
+          const bf16x4 b = *(const bf16x4 *)(sx + (nt * 16 + col) * DW_PITCH + ks * 16 + 4 * kq);
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int tile = wave + 4 * q;
+    if (tile < ntiles) {
+      const int mt = tile / nci, nt = tile - mt * nci;
+      float *pp = part + ((long)blockIdx.x * 27 + t) * d.cop * d.cip;
+#pragma unroll
+      for (int j = 0; j < 4; j++) pp[(long)(mt * 16 + 4 * kq + j) * d.cip + nt * 16 + col] = acc[q][j];
+    }
+  }
+}
+
+__global__ void k_dw_reduce(const float *__restrict__ part, int ks, int cop, int cip, int cout, int cin,
+                            float *__restrict__ grad, int accumulate)
+{
+  const long P = 27L * cout * cin;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < P; i += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(i / ((long)cout * cin));
+    const int r = (int)(i - (long)t * cout * cin);
+    const int co = r / cin, ci = r - co * cin;
+    float s = 0.f;
+    for (int k = 0; k < ks; k++) s += part[(((long)k * 27 + t) * cop + co) * cip + ci];
+    grad[i] = accumulate ? grad[i] + s : s;
+  }
+}
+
+// --------------------------------------------------------------------------
+// gates, loss, optimiser, training pairs
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v)
+{
+  __shared__ double sh[16];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w];
+  return s;                                   // valid in thread 0
+}
+
+#define GRID_STRIDE(i, n) \
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
+__global__ void k_mul(long n, const float *__restrict__ a, const float *__restrict__ b, float *__restrict__ out,
+                      const int *skip)
+{
+  if (skip && *skip) return;
+  GRID_STRIDE(i, n) out[i] = a[i] * b[i];
+}
+
+// g[v][c] = exp(bp[c] - (1 + ap[c]) eb[v]) over nvox voxels (all samples)
+__global__ void k_gates(long nvox, int C, const float *__restrict__ eb, const float *__restrict__ ap,
+                        const float *__restrict__ bp, float *__restrict__ g)
+{
+  GRID_STRIDE(i, nvox * C) {
+    const long v = i / C;
+    const int c = (int)(i - v * C);
+    g[i] = expf(bp[c] - (1.f + ap[c]) * eb[v]);
+  }
+}
+
+// per-block, per-channel sums of q = g h dt and of eb q (C divides 256, so a
+// thread keeps one channel over the grid-stride loop); part [blocks][2C]
+__global__ __launch_bounds__(256) void k_gate_grad(long nvox, int C, const float *__restrict__ g,
+                                                   const float *__restrict__ h, const float *__restrict__ dt,
+                                                   const float *__restrict__ eb, double *part)
+{
+  __shared__ double s0[256], s1[256];
+  double a = 0.0, b = 0.0;
+  GRID_STRIDE(i, nvox * C) {
+    const double q = (double)g[i] * h[i] * dt[i];
+    a += q;
+    b += q * eb[i / C];
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  if ((int)threadIdx.x < C) {
+    double x = 0.0, y = 0.0;
+    for (int k = threadIdx.x; k < 256; k += C) {
+      x += s0[k];
+      y += s1[k];
+    }
+    part[(long)blockIdx.x * 2 * C + threadIdx.x] = x;
+    part[(long)blockIdx.x * 2 * C + C + threadIdx.x] = y;
+  }
+}
+
+// fixed-order sum over blocks: d/db = sum q, d/da = -sum eb q
+__global__ void k_gate_reduce(int G, int C, const double *__restrict__ part, float *ga, float *gb)
+{
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  double x = 0.0, y = 0.0;
+  for (int k = 0; k < G; k++) {
+    x += part[(long)k * 2 * C + c];
+    y += part[(long)k * 2 * C + C + c];
+  }
+  gb[c] = (float)x;
+  ga[c] = (float)(-y);
+}
+
+// energy loss of one pair: z = dinv rr + mask sqrt(dinv) y
+__global__ void k_el_z(long n, const float *__restrict__ rr, const float *__restrict__ dinv,
+                       const float *__restrict__ mask, const float *__restrict__ y, float *__restrict__ zz)
+{
+  GRID_STRIDE(i, n) {
+    const long v = i / 3;
+    const float d = dinv[i];
+    zz[i] = d * rr[i] + mask[i] * sqrtf(d) * y[v * 4 + (i - 3 * v)];
+  }
+}
+
+// res = rr - K z; loss partial sum (ee - zz) res; dy = coef mask sqrt(dinv) res
+__global__ __launch_bounds__(256) void k_el_grad(long n, const float *__restrict__ rr,
+                                                 const float *__restrict__ ee, const float *__restrict__ zz,
+                                                 const float *__restrict__ kz, const float *__restrict__ dinv,
+                                                 const float *__restrict__ mask, float coef,
+                                                 float *__restrict__ dy, double *lpart)
+{
+  double l = 0.0;
+  GRID_STRIDE(i, n) {
+    const float res = rr[i] - kz[i];
+    l += (double)(ee[i] - zz[i]) * (double)res;
+    const long v = i / 3;
+    dy[v * 4 + (i - 3 * v)] = coef * mask[i] * sqrtf(dinv[i]) * res;
+  }
+  const double s = block_sum(l);
+  if (threadIdx.x == 0) lpart[blockIdx.x] = s;
+}
+
+__global__ void k_adam(long P, int cout, int cin, int cop, int cip, float *w, const float *__restrict__ g,
+                       float *m, float *v, bf16 *wf, bf16 *wt, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2)
+{
+  GRID_STRIDE(i, P) {
+    float wi = w[i];
+    if (lr > 0.f) {
+      const float gi = g[i];
+      const float mi = b1 * m[i] + (1.f - b1) * gi;
+      const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+      m[i] = mi;
+      v[i] = vi;
+      wi -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+      w[i] = wi;
+    }
+    if (wf) {
+      const int t = (int)(i / ((long)cout * cin));
+      const int r = (int)(i - (long)t * cout * cin);
+      const int co = r / cin, ci = r - co * cin;
+      const bf16 h = f2bf(wi);
+      wf[((long)t * cop + co) * cip + ci] = h;
+      wt[((long)t * cip + ci) * cop + co] = h;
+    }
+  }
+}
+
+// training pair from a PCG snapshot (x, r) and the converged u:
+// partials (sum dinv r^2, sum (u - x) r) per block
+__global__ __launch_bounds__(256) void k_pair_stats(long n, const double *__restrict__ x,
+                                                    const double *__restrict__ r, const double *__restrict__ u,
+                                                    const double *__restrict__ dinv, double *part)
+{
+  double a2 = 0.0, er = 0.0;
+  GRID_STRIDE(i, n) {
+    a2 += dinv[i] * r[i] * r[i];
+    er += (u[i] - x[i]) * r[i];
+  }
+  const double s0 = block_sum(a2);
+  const double s1 = block_sum(er);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// v[vox][c] = mask sqrt(dinv) r / s, rr = r / s, ee = (u - x) / s
+__global__ void k_pair_write(long n, const double *__restrict__ x, const double *__restrict__ r,
+                             const double *__restrict__ u, const double *__restrict__ dinv,
+                             const float *__restrict__ mask, double inv_s, float *V, float *rr, float *ee)
+{
+  GRID_STRIDE(i, n) {
+    const long v = i / 3;
+    V[v * 4 + (i - 3 * v)] = (float)(mask[i] * sqrt(dinv[i]) * r[i] * inv_s);
+    rr[i] = (float)(r[i] * inv_s);
+    ee[i] = (float)((u[i] - x[i]) * inv_s);
+  }
+}
+
+template <int TPW, int MODE>
+static void dw_launch(const ugk_dw_desc &d, dim3 grid, float *part, long cps, hipStream_t s)
+{
+  if (d.x_f32 && d.dy_f32) k_conv_dw<TPW, MODE, 1, 1><<<grid, 256, 0, s>>>(d, part, cps);
+  else if (d.x_f32) k_conv_dw<TPW, MODE, 1, 0><<<grid, 256, 0, s>>>(d, part, cps);
+  else if (d.dy_f32) k_conv_dw<TPW, MODE, 0, 1><<<grid, 256, 0, s>>>(d, part, cps);
+  else k_conv_dw<TPW, MODE, 0, 0><<<grid, 256, 0, s>>>(d, part, cps);
+}
+
+template <int TPW>
+static void dw_mode(const ugk_dw_desc &d, dim3 grid, float *part, long cps, hipStream_t s)
+{
+  if (d.mode == UGK_CONV_S1) dw_launch<TPW, UGK_CONV_S1>(d, grid, part, cps, s);
+  else dw_launch<TPW, UGK_CONV_S2>(d, grid, part, cps, s);
+}
+
+static unsigned grid_for(long work, int per = 256, long cap = 4096)
+{
+  long b = (work + per - 1) / per;
+  return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+extern "C" {
+
+int ugk_conv(const ugk_conv_desc *d, void *s)
+{
+  const long total = vox(d->go) * d->nb;
+  if (total <= 0) return 0;
+  if (d->cin % 4 || d->cip % 16 || d->cop % 16 || d->cin > d->cip || d->cout > d->cop || d->ldx % 4 ||
+      d->ldy % 4 || d->mode < 0 || d->mode > 2 || (d->ga && !d->gb))
+    return ugk_hip_fail("ugk_conv: channel layout", hipErrorInvalidValue);
+  if (d->x_f32) launch_conv<1>(*d, total, S(s));
+  else launch_conv<0>(*d, total, S(s));
+  CK_LAUNCH("k_conv");
+  return 0;
+}
+
+int ugk_conv_dw_slices(const ugk_dw_desc *d)
+{
+  const long nch = (vox(d->go) * d->nb + 63) / 64;
+  long ks = nch / 8;
+  return (int)(ks < 1 ? 1 : (ks > 96 ? 96 : ks));
+}
+
+int ugk_conv_dw(const ugk_dw_desc *d, float *part, int ks, float *grad, int accumulate, void *s)
+{
+  const long nch = (vox(d->go) * d->nb + 63) / 64;
+  if (nch <= 0) return 0;
+  if (d->cop > DW_MAX_COP || d->cip > DW_MAX_CIP || d->cop % 16 || d->cip % 16 || d->ldx % 4 ||
+      d->ldd % 4 || (d->mode != UGK_CONV_S1 && d->mode != UGK_CONV_S2) || ks < 1)
+    return ugk_hip_fail("ugk_conv_dw: layout", hipErrorInvalidValue);
+  const long cps = (nch + ks - 1) / ks;
+  const int tiles = (d->cop / 16) * (d->cip / 16);
+  dim3 grid((unsigned)((nch + cps - 1) / cps), 27);
+  const int ksr = (int)grid.x;
+  if (tiles <= 4) dw_mode<1>(*d, grid, part, cps, S(s));
+  else if (tiles <= 8) dw_mode<2>(*d, grid, part, cps, S(s));
+  else if (tiles <= 16) dw_mode<4>(*d, grid, part, cps, S(s));
+  else if (tiles <= 24) dw_mode<6>(*d, grid, part, cps, S(s));
+  else return ugk_hip_fail("ugk_conv_dw: too many tiles", hipErrorInvalidValue);
+  CK_LAUNCH("k_conv_dw");
+  k_dw_reduce<<<grid_for(27L * d->cout * d->cin), 256, 0, S(s)>>>(part, ksr, d->cop, d->cip, d->cout, d->cin,
+                                                                   grad, accumulate);
+  CK_LAUNCH("k_dw_reduce");
+  return 0;
+}
+
+int ugk_mul(long n, const float *a, const float *b, float *out, const int *skip, void *s)
+{
+  k_mul<<<grid_for(n), 256, 0, S(s)>>>(n, a, b, out, skip);
+  CK_LAUNCH("k_mul");
+  return 0;
+}
+
+int ugk_gates(long nvox, int C, const float *eb, const float *ap, const float *bp, float *g, void *s)
+{
+  k_gates<<<grid_for(nvox * C), 256, 0, S(s)>>>(nvox, C, eb, ap, bp, g);
+  CK_LAUNCH("k_gates");
+  return 0;
+}
+
+int ugk_gate_grad(long nvox, int C, const float *g, const float *h, const float *dt, const float *eb,
+                  double *part, float *grad_a, float *grad_b, void *s)
+{
+  if (C < 1 || C > 256 || 256 % C) return ugk_hip_fail("ugk_gate_grad: channels", hipErrorInvalidValue);
+  const unsigned G = grid_for(nvox * C, 256, 512);
+  k_gate_grad<<<G, 256, 0, S(s)>>>(nvox, C, g, h, dt, eb, part);
+  CK_LAUNCH("k_gate_grad");
+  k_gate_reduce<<<1, 256, 0, S(s)>>>((int)G, C, part, grad_a, grad_b);
+  CK_LAUNCH("k_gate_reduce");
+  return 0;
+}
+
+int ugk_el_z(long n, const float *rr, const float *dinv, const float *mask, const float *y, float *zz, void *s)
+{
+  k_el_z<<<grid_for(n), 256, 0, S(s)>>>(n, rr, dinv, mask, y, zz);
+  CK_LAUNCH("k_el_z");
+  return 0;
+}
+
+int ugk_el_grad(long n, const float *rr, const float *ee, const float *zz, const float *kz, const float *dinv,
+                const float *mask, float coef, float *dy, double *lpart, void *s)
+{
+  const unsigned G = grid_for(n, 256, 1024);
+  k_el_grad<<<G, 256, 0, S(s)>>>(n, rr, ee, zz, kz, dinv, mask, coef, dy, lpart);
+  CK_LAUNCH("k_el_grad");
+  return (int)G;
+}
+
+int ugk_adam(long P, int cout, int cin, int cop, int cip, float *w, const float *g, float *m, float *v,
+             unsigned short *wf, unsigned short *wt, float lr, float b1, float b2, float eps, long step,
+             void *s)
+{
+  const float bc1 = step > 0 ? 1.f - powf(b1, (float)step) : 1.f;
+  const float bc2 = step > 0 ? 1.f - powf(b2, (float)step) : 1.f;
+  k_adam<<<grid_for(P), 256, 0, S(s)>>>(P, cout, cin, cop, cip, w, g, m, v, wf, wt, lr, b1, b2, eps, bc1, bc2);
+  CK_LAUNCH("k_adam");
+  return 0;
+}
+
+int ugk_pair_stats(long n, const double *x, const double *r, const double *u, const double *dinv, double *part,
+                   void *s)
+{
+  const unsigned G = grid_for(n, 256, 512);
+  k_pair_stats<<<G, 256, 0, S(s)>>>(n, x, r, u, dinv, part);
+  CK_LAUNCH("k_pair_stats");
+  return (int)G;
+}
+
+int ugk_pair_write(long n, const double *x, const double *r, const double *u, const double *dinv,
+                   const float *mask, double sc, float *V, float *rr, float *ee, void *s)
+{
+  k_pair_write<<<grid_for(n), 256, 0, S(s)>>>(n, x, r, u, dinv, mask, sc > 0 ? 1.0 / sc : 0.0, V, rr, ee);
+  CK_LAUNCH("k_pair_write");
+  return 0;
+}
+
+}  // extern "C"

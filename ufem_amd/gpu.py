@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libufem_gpu.so")
 P_D = C.POINTER(C.c_double)
 P_L = C.POINTER(C.c_long)
 P_I = C.POINTER(C.c_int)
+P_F = C.POINTER(C.c_float)
 
 AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
 SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3}
@@ -45,7 +46,7 @@ class _Mesh(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("solver", C.c_int), ("rule", C.c_int), ("eps", C.c_double), ("maxit", C.c_long),
                 ("exec", C.c_int), ("chunk", C.c_int), ("inner_eps", C.c_double),
-                ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int)]
+                ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int), ("fcg_m", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -73,6 +74,26 @@ def lib():
             "ufem_sys_ncolors": (C.c_long, [C.c_void_p]),
             "ufem_sys_nreact": (C.c_long, [C.c_void_p]),
             "ufem_sys_fill_K": (C.c_int, [C.c_void_p, C.c_int]),
+            "ufem_sys_set_E": (C.c_int, [C.c_void_p, P_D]),
+            "ufem_net_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ufem_net_destroy": (None, [C.c_void_p]),
+            "ufem_net_nparams": (C.c_long, []),
+            "ufem_net_init": (C.c_int, [C.c_void_p, C.c_ulong]),
+            "ufem_net_set_weights": (C.c_int, [C.c_void_p, P_F]),
+            "ufem_net_get_weights": (C.c_int, [C.c_void_p, P_F]),
+            "ufem_net_forward": (C.c_int, [C.c_void_p, C.c_int, P_F, P_F, P_F]),
+            "ufem_net_backward": (C.c_int, [C.c_void_p, C.c_int, P_F, P_F, P_F, P_F]),
+            "ufem_net_eval": (C.c_int, [C.c_void_p, C.c_long, C.c_long, P_D]),
+            "ufem_net_get_pair": (C.c_int, [C.c_void_p, C.c_long, P_F, P_F, P_F, P_D, P_I]),
+            "ufem_net_bind": (C.c_int, [C.c_void_p, C.c_void_p, P_D]),
+            "ufem_net_reserve": (C.c_int, [C.c_void_p, C.c_long]),
+            "ufem_net_npairs": (C.c_long, [C.c_void_p]),
+            "ufem_net_collect": (C.c_long, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p]),
+            "ufem_net_train": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_double, C.c_ulong, P_D, P_D]),
+            "ufem_net_time_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P_I, P_I, C.c_int, C.c_int,
+                                             C.c_int, P_D]),
+            "ufem_net_probe_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P_I, P_I, C.c_int, C.c_int,
+                                              P_F, P_F, P_F, P_F, P_F]),
             "ufem_sys_add_loads": (C.c_int, [C.c_void_p]),
             "ufem_sys_add_disps": (C.c_int, [C.c_void_p]),
             "ufem_sys_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
@@ -186,6 +207,11 @@ class GpuSystem:
     def fill_K(self, npts=2):
         self._chk(lib().ufem_sys_fill_K(self.h, npts), "fill_K")
 
+    def set_E(self, E):
+        """New per-element Young's moduli (then fill_K and add_disps again)."""
+        self._keep_E = np.ascontiguousarray(E, dtype=np.float64)
+        self._chk(lib().ufem_sys_set_E(self.h, _pd(self._keep_E)), "set_E")
+
     def add_loads(self):
         self._chk(lib().ufem_sys_add_loads(self.h), "add_loads")
 
@@ -198,9 +224,9 @@ class GpuSystem:
         self.add_disps()
 
     def solve(self, solver="pcg", eps=1e-8, rule=1, maxit=0, exec="streamed", chunk=0,
-              inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, check=True):
+              inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, fcg_m=0, check=True):
         o = SolveOpts(SOLVER[solver], rule, eps, maxit, EXEC[exec], chunk, inner_eps, inner_maxit,
-                      net, 1 if warm_start else 0)
+                      net.h if net is not None else None, 1 if warm_start else 0, fcg_m)
         info = SolveInfo()
         rv = lib().ufem_sys_solve(self.h, C.byref(o), C.byref(info))
         if check and rv not in (AF_OK, AF_ERR):
@@ -259,6 +285,138 @@ class GpuSystem:
         t = np.zeros(3)
         lib().ufem_sys_timings(self.h, _pd(t))
         return {"fill_ms": t[0], "bc_ms": t[1], "solve_ms": t[2]}
+
+
+def _pf(a):
+    return a.ctypes.data_as(P_F)
+
+
+class GpuNet:
+    """The learned preconditioner's U-Net (ufem_net_*) on a W x H x D node grid."""
+
+    def __init__(self, W, H, D, batch=1):
+        self.h = lib().ufem_net_create(W, H, D, batch)
+        if not self.h:
+            raise RuntimeError("ufem_net_create: " + last_error())
+        self.shape = (D, H, W)
+        self.V = W * H * D
+        self.P = lib().ufem_net_nparams()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ufem_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rv, what):
+        if rv != 0:
+            raise RuntimeError(f"{what} failed ({rv}): {last_error()}")
+
+    def init(self, seed=0):
+        self._chk(lib().ufem_net_init(self.h, seed), "net init")
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.P
+        self._chk(lib().ufem_net_set_weights(self.h, _pf(w)), "set_weights")
+
+    def weights(self):
+        w = np.zeros(self.P, dtype=np.float32)
+        self._chk(lib().ufem_net_get_weights(self.h, _pf(w)), "get_weights")
+        return w
+
+    def forward(self, x, eb0):
+        """y = N(x): x (nb, V, 3), eb0 (nb, V) node log-moduli -> (nb, V, 3)"""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        eb0 = np.ascontiguousarray(eb0, dtype=np.float32)
+        nb = x.shape[0]
+        y = np.zeros((nb, self.V, 3), dtype=np.float32)
+        self._chk(lib().ufem_net_forward(self.h, nb, _pf(x), _pf(eb0), _pf(y)), "forward")
+        return y
+
+    def backward(self, x, eb0, dy):
+        """gradient of sum <dy, N(x)> with respect to the packed parameters"""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        eb0 = np.ascontiguousarray(eb0, dtype=np.float32)
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        g = np.zeros(self.P, dtype=np.float32)
+        self._chk(lib().ufem_net_backward(self.h, x.shape[0], _pf(x), _pf(eb0), _pf(dy), _pf(g)), "backward")
+        return g
+
+    def pair(self, i):
+        v = np.zeros((self.V, 4), dtype=np.float32)
+        rr = np.zeros(3 * self.V, dtype=np.float32)
+        ee = np.zeros(3 * self.V, dtype=np.float32)
+        er = C.c_double(0)
+        f = C.c_int(0)
+        self._chk(lib().ufem_net_get_pair(self.h, i, _pf(v), _pf(rr), _pf(ee), C.byref(er), C.byref(f)), "pair")
+        return v, rr, ee, er.value, f.value
+
+    def eval(self, i0, cnt):
+        out = np.zeros(cnt)
+        self._chk(lib().ufem_net_eval(self.h, i0, cnt, _pd(out)), "eval")
+        return out
+
+    def bind(self, system, E):
+        self._E = np.ascontiguousarray(E, dtype=np.float64)
+        self._sys = system
+        self._chk(lib().ufem_net_bind(self.h, system.h, _pd(self._E)), "bind")
+
+    def reserve(self, cap):
+        self._chk(lib().ufem_net_reserve(self.h, cap), "reserve")
+
+    def npairs(self):
+        return lib().ufem_net_npairs(self.h)
+
+    def collect(self, pairs, eps=1e-11, every=8):
+        info = SolveInfo()
+        n = lib().ufem_net_collect(self.h, pairs, eps, every, C.byref(info))
+        if n < 0:
+            raise RuntimeError(f"collect failed ({n}): {last_error()}")
+        return n, info.as_dict()
+
+    def train(self, steps, batch=4, lr=1e-3, seed=0):
+        loss = np.zeros(steps)
+        ms = C.c_double(0)
+        self._chk(lib().ufem_net_train(self.h, steps, batch, lr, seed, _pd(loss), C.byref(ms)), "train")
+        return loss, ms.value
+
+
+def probe_conv(mode, flip, x, w, gi, go, dy=None, x_f32=False):
+    """One convolution kernel on host data: x (nb, Vi, cin), w (27, cout, cin)
+    -> y (nb, Vo, cout) [and dw (27, cout, cin) for dy (nb, Vo, cout)]."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    nb, _, cin = x.shape
+    cout = w.shape[1]
+    Vo = int(np.prod(go))
+    gi_ = np.ascontiguousarray(gi, dtype=np.int32)
+    go_ = np.ascontiguousarray(go, dtype=np.int32)
+    y = np.zeros((nb, Vo, cout), dtype=np.float32)
+    dw = None
+    if dy is not None:
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        dw = np.zeros((27, cout, cin), dtype=np.float32)
+    rv = lib().ufem_net_probe_conv(mode, flip, 1 if x_f32 else 0, nb, _pi(gi_), _pi(go_), cin, cout, _pf(x), _pf(w),
+                                   _pf(y), _pf(dy) if dy is not None else None, _pf(dw) if dw is not None else None)
+    if rv != 0:
+        raise RuntimeError(f"probe_conv failed ({rv}): {last_error()}")
+    return (y, dw) if dy is not None else y
+
+
+def time_conv(mode, flip, x_f32, nb, gi, go, cin, cout, reps=50):
+    gi_ = np.ascontiguousarray(gi, dtype=np.int32)
+    go_ = np.ascontiguousarray(go, dtype=np.int32)
+    us = C.c_double(0)
+    rv = lib().ufem_net_time_conv(mode, flip, x_f32, nb, _pi(gi_), _pi(go_), cin, cout, reps, C.byref(us))
+    if rv != 0:
+        raise RuntimeError(f"time_conv failed ({rv}): {last_error()}")
+    return us.value
 
 
 def _mesh_struct(mesh, keep):
