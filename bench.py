@@ -9,11 +9,15 @@ one step = one FP64 Jacobi-PCG solve from u = 0 to ||r||/||f|| <= 1e-8 with
 K, f already resident in HBM.
 
 Printed: one JSON line with the solve time (value, seconds), the roofline of
-the solve's dominant kernel (PCG phase A: p' = z + beta p, q = K p', (p',q)),
-SpMV bandwidth, the mixed-precision solve on the same system, and the CPU
-baseline (the oracle's pthread restatement of uFEM's solver, bounded sample).
+the solve's dominant kernel (w = K u with (w, u) of the single-reduction PCG),
+SpMV bandwidth, the mixed-precision solve on the same system, the learned
+preconditioner on the 128 x 32 x 32 random-field cantilever (training on the
+GPU, then plain vs learned flexible CG on an unseen field), the batched
+Monte-Carlo solve of 256 uneven meshes in one launch, and the CPU baseline
+(the oracle's pthread restatement of uFEM's solver, bounded sample).
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-mixed]
+                    [--no-learned] [--no-batch] [--kernels-only]
 """
 
 from __future__ import annotations
@@ -70,6 +74,98 @@ def cpu_baseline(mesh, gpu_iters, sample_iters=60):
     }
 
 
+def bench_learned(fields, steps, seed):
+    """Learned preconditioner on the 128 x 32 x 32 GRF cantilever (421,443 DOF):
+    pairs from plain-PCG solves on `fields` fresh fields (32 each), `steps`
+    Adam steps, then plain PCG and flexible CG with the network on a field
+    not seen in training (seed `seed`), both to ||r||/||f|| <= 1e-8."""
+    from ufem_amd import gpu as G
+    from ufem_amd import learned as Lr
+    from ufem_amd.mesh import cantilever
+    nx, ny, nz = 128, 32, 32
+    mesh = cantilever(nx, ny, nz)
+    E_test = Lr.field_E(mesh, seed)
+    mesh.E[:] = E_test
+    s = G.GpuSystem(mesh)
+    s.assemble(2)
+    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=4)
+    net.init(1)
+    net.reserve(32 * fields)
+    pairs, collect_s = Lr.collect(s, net, mesh, range(1000, 1000 + fields), 32, every=8, log=log)
+    loss, train_ms = Lr.train(net, steps, batch=4, lr=3e-3)
+    log(f"learned: {pairs} pairs in {collect_s:.1f} s, {steps} steps in {train_ms / 1e3:.1f} s, "
+        f"loss {loss[:20].mean():.3f} -> {loss[-50:].mean():.3f}")
+    Lr.rebind(s, net, mesh, E_test)
+
+    def timed(**kw):
+        s.solve(**kw)
+        t0 = time.perf_counter()
+        r = s.solve(**kw)
+        return r, time.perf_counter() - t0, s.u()
+
+    rp, tp, up = timed(solver="pcg", eps=1e-8, rule=1)
+    rl, tl, ul = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=4)
+    stages = net.time_stages()
+    out = {
+        "workload": (f"cantilever 2.0x0.4x0.4 m, {nx}x{ny}x{nz} bricks, {mesh.n_dof} DOF, E = exp(GRF, "
+                     f"corr 0.2 m, log-std 0.5, mean 30 GPa); test field seed {seed}, training fields "
+                     f"1000..{1000 + fields - 1}"),
+        "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
+        "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
+        "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
+        "learned": {"iters": rl["iters"], "solve_s": round(tl, 4), "fcg_m": 4, "fallback": rl["fallback"],
+                    "true_relres": rl["true_relres"],
+                    "u_rel_err_vs_plain": float(np.linalg.norm(ul - up) / np.linalg.norm(up))},
+        "iteration_ratio": round(rp["iters"] / max(rl["iters"], 1), 3),
+        "speedup_vs_plain": round(tp / tl, 3),
+        "us_per_iteration": {"plain": round(tp / rp["iters"] * 1e6, 1), "learned": round(tl / rl["iters"] * 1e6, 1)},
+        "network_stage_us": {k: round(v, 1) for k, v in stages.items()},
+    }
+    net.close()
+    s.close()
+    return out
+
+
+def bench_batch(n_samples=256, seed=7, sequential_sample=16):
+    """Batched Monte-Carlo: n_samples cantilevers drawn uniformly from four mesh
+    sizes (32x8x8 .. 96x24x24), each with its own lognormal E field, solved in
+    one persistent launch (autotuned shape) to 1e-8; compared with solving a
+    sample of them one at a time with the streamed PCG."""
+    from ufem_amd import gpu as G
+    from ufem_amd.mesh import batch_meshes
+    meshes = batch_meshes(n_samples, seed=seed)
+    b = G.GpuBatch(meshes)
+    asm_ms = b.assemble(2)
+    t0 = time.perf_counter()
+    wg, grid = b.autotune(None, 100)
+    tune_s = time.perf_counter() - t0
+    b.solve(eps=1e-8)
+    r = b.solve(eps=1e-8)
+    sm = r["sample_ms"]
+    dofs = float(sum(m.n_dof for m in meshes))
+    idx = np.linspace(0, n_samples - 1, sequential_sample).astype(int)
+    seq_ms = []
+    for i in idx:
+        g = G.GpuSystem(meshes[i])
+        g.assemble(2)
+        g.solve(solver="pcg", eps=1e-8, rule=1)
+        seq_ms.append(g.solve(solver="pcg", eps=1e-8, rule=1)["solve_ms"])
+        g.close()
+    seq_total = float(np.mean(seq_ms)) * n_samples
+    out = {
+        "samples": n_samples, "classes": b.nclasses, "assembly_ms": round(asm_ms, 2),
+        "autotune": {"wg": r["wg"], "grid": r["grid"], "team_big": r["team_big"], "seconds": round(tune_s, 2)},
+        "total_ms": round(r["total_ms"], 2), "samples_per_s": round(n_samples / (r["total_ms"] / 1e3), 2),
+        "dof_per_s": round(dofs / (r["total_ms"] / 1e3), 1),
+        "sample_ms_p50": round(float(np.percentile(sm, 50)), 2), "sample_ms_p99": round(float(np.percentile(sm, 99)), 2),
+        "all_converged": bool(np.all(r["status"] == 1)), "iters_min_max": [int(r["iters"].min()), int(r["iters"].max())],
+        "one_at_a_time_ms_estimate": round(seq_total, 1),
+        "speedup_vs_one_at_a_time": round(seq_total / r["total_ms"], 3),
+    }
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +176,13 @@ def main():
     ap.add_argument("--exec", default="streamed", choices=["streamed", "persistent", "fused", "classic"])
     ap.add_argument("--size", default="200,40,40", help="bricks nx,ny,nz (default: the metric's mesh)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--learn-fields", type=int, default=63)
+    ap.add_argument("--learn-steps", type=int, default=3000)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="launch only the dominant solve kernel and a device copy of known size "
+                         "(for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -107,6 +210,11 @@ def main():
     log(f"setup {t_setup:.2f} s, assembly {fill_ms:.2f} ms ({mesh.n_elems / fill_ms * 1e3:.3e} elem/s), "
         f"supports {bc_ms:.3f} ms, {g.ncolors} colours")
 
+    if args.kernels_only:
+        g.time_kernel("copy", reps=5)
+        g.time_kernel("cg1_w", reps=5)
+        g.close()
+        return
     opts = dict(solver="pcg", eps=1e-8, rule=1, exec=args.exec)
     for _ in range(args.warmup):
         g.solve(**opts)
@@ -137,7 +245,7 @@ def main():
     csr_bytes = 12.0 * nnz + 4.0 * (g.n + 1) + 16.0 * g.n
     achieved = a_bytes / (a_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{dom[0]}_plain_gpu_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"k_{dom[0]}_cantilever_{nx}x{ny}x{nz}_hbm_traffic.json")
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
@@ -166,6 +274,18 @@ def main():
                  "u_rel_err_vs_fp64": float(np.linalg.norm(um - u64) / np.linalg.norm(u64)),
                  "speedup_vs_fp64": round(solve_s / m_s, 3)}
         log("mixed", mixed)
+
+    learned = None
+    if not args.no_learned:
+        log("learned preconditioner ...")
+        learned = bench_learned(args.learn_fields, args.learn_steps, 9999)
+        log("learned", learned)
+
+    batch = None
+    if not args.no_batch:
+        log("batched Monte-Carlo ...")
+        batch = bench_batch()
+        log("batch", batch)
 
     cpu = None
     if not args.no_cpu:
@@ -199,6 +319,8 @@ def main():
         "roofline": roofline,
         "spmv": spmv,
         "mixed": mixed,
+        "learned": learned,
+        "batch": batch,
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),
         "device": dev["name"],
     }

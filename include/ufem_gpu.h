@@ -195,6 +195,9 @@ int  ufem_net_train(ufem_net *N, long steps, int batch, double lr, unsigned long
  * gradient for dy (modes 0, 1) */
 int  ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin,
                          int cout, const float *x, const float *w, float *y, const float *dy, float *dw);
+/* average device time (us) of each of the 7 inference stages on the bound
+ * system's grid: C0, C1, C2, gate, C2^T, C1^T, C0^T */
+int  ufem_net_time_stages(ufem_net *N, int reps, double *us);
 /* average device time (us) of `reps` launches of one such convolution */
 int  ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin, int cout,
                         int reps, double *us);

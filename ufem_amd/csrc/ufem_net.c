@@ -303,18 +303,30 @@ static int conv(ufem_net *N, const float *x, int ldx, int cin, int l, int adjoin
   return ugk_conv(&d, stream);
 }
 
-/* y = N(v) for nb samples with gates g[l] (nb x V_l x C_l) */
+/* stage k of y = N(v) for nb samples with gates g[l] (nb x V_l x C_l):
+ * 0 h0 = C0 v, 1 h1 = C1 h0, 2 h2 = C2 h1, 3 t2 = g2 h2, 4 t1 = g1 h1 +
+ * C2^T t2, 5 t0 = g0 h0 + C1^T t1, 6 y = C0^T t0 */
+#define NET_STAGES 7
+static int net_stage(ufem_net *N, int k, const float *v, float *const *g, float *y, int nb, const int *skip,
+                     void *st)
+{
+  switch (k) {
+    case 0: return conv(N, v, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, 0, nb, skip, st);
+    case 1: return conv(N, N->h[0], 16, 16, 1, 0, N->h[1], 32, 32, 0, 1, NULL, NULL, 0, nb, skip, st);
+    case 2: return conv(N, N->h[1], 32, 32, 2, 0, N->h[2], 64, 64, 1, 2, NULL, NULL, 0, nb, skip, st);
+    case 3: return ugk_mul((long)nb * N->V[2] * 64, g[2], N->h[2], N->t[2], skip, st);
+    case 4: return conv(N, N->t[2], 64, 64, 2, 1, N->t[1], 32, 32, 2, 1, g[1], N->h[1], 32, nb, skip, st);
+    case 5: return conv(N, N->t[1], 32, 32, 1, 1, N->t[0], 16, 16, 1, 0, g[0], N->h[0], 16, nb, skip, st);
+    default: return conv(N, N->t[0], 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, 0, nb, skip, st);
+  }
+}
+
 static int net_forward(ufem_net *N, const float *v, float *const *g, float *y, int nb, const int *skip,
                        void *st)
 {
-  if (conv(N, v, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, 0, nb, skip, st) ||
-      conv(N, N->h[0], 16, 16, 1, 0, N->h[1], 32, 32, 0, 1, NULL, NULL, 0, nb, skip, st) ||
-      conv(N, N->h[1], 32, 32, 2, 0, N->h[2], 64, 64, 1, 2, NULL, NULL, 0, nb, skip, st) ||
-      ugk_mul((long)nb * N->V[2] * 64, g[2], N->h[2], N->t[2], skip, st) ||
-      conv(N, N->t[2], 64, 64, 2, 1, N->t[1], 32, 32, 2, 1, g[1], N->h[1], 32, nb, skip, st) ||
-      conv(N, N->t[1], 32, 32, 1, 1, N->t[0], 16, 16, 1, 0, g[0], N->h[0], 16, nb, skip, st) ||
-      conv(N, N->t[0], 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, 0, nb, skip, st))
-    return net_fail("network forward");
+  int k;
+  for (k = 0; k < NET_STAGES; k++)
+    if (net_stage(N, k, v, g, y, nb, skip, st)) return net_fail("network forward");
   return AF_OK;
 }
 
@@ -957,4 +969,22 @@ out:
   if (e1) ugk_event_destroy(e1);
   if (st) ugk_stream_destroy(st);
   return rv;
+}
+
+/* average device time (us) of each inference stage on the bound system's
+ * grid (us[0..6], stages as in net_stage), `reps` launches each */
+int ufem_net_time_stages(ufem_net *N, int reps, double *us)
+{
+  int k, r;
+  if (!N->sys) { ufem_set_error("bind the network to a system first"); return AF_ERR; }
+  if (ufem_net_prepare(N, N->stream)) return AF_ERR;
+  for (k = 0; k < NET_STAGES; k++) {
+    if (net_stage(N, k, N->x0i, N->gi, N->yi, 1, NULL, N->stream)) return net_fail("stage");
+    if (ugk_event_record(N->ev[0], N->stream)) return net_fail("event");
+    for (r = 0; r < reps; r++)
+      if (net_stage(N, k, N->x0i, N->gi, N->yi, 1, NULL, N->stream)) return net_fail("stage");
+    if (ugk_event_record(N->ev[1], N->stream)) return net_fail("event");
+    us[k] = 1000.0 * ugk_event_ms(N->ev[0], N->ev[1]) / reps;
+  }
+  return AF_OK;
 }

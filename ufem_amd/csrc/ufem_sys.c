@@ -587,6 +587,7 @@ static double kernel_bytes(const ugk_mat *K, int mode)
   double nnzb = (double)K->nnzb, n = (double)K->n_brows * K->bs, b2 = (double)K->bs * K->bs;
   if (mode == 1) return nnzb * (b2 * 4.0 + 4.0) + 8.0 * n;
   if (mode == 2) return nnzb * (b2 * 8.0 + 4.0) + 32.0 * n;
+  if (mode == 5) return 16.0 * n;                      /* device copy of one vector */
   return nnzb * (b2 * 8.0 + 4.0) + 16.0 * n;   /* modes 0, 3, 4: x read, y written */
 }
 
@@ -596,6 +597,7 @@ static int launch_mode(ufem_sys *s, int mode)
   if (mode == 2) return ugk_pcg_iter_a(&s->K, s->w.z, s->w.p[0], s->w.p[1], s->w.q, s->w.st, s->w.part, s->stream);
   if (mode == 3) return ugk_pcg_iter_q(&s->K, s->w.p[0], s->w.q, s->w.st, s->w.part, s->stream);
   if (mode == 4) return ugk_cg1_iter_w(&s->K, s->w.z, s->w.q, s->w.st, s->w.part, s->stream);
+  if (mode == 5) return ugk_d2d(s->w.q, s->w.z, sizeof(double) * (size_t)s->n, s->stream);
   return ugk_spmv(&s->K, s->w.z, s->w.q, s->stream);
 }
 

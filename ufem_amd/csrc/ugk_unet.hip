@@ -88,30 +88,71 @@ __device__ __forceinline__ typename Frag<XF>::T load_raw(const void *x, long off
   else return *(const bf16x4 *)((const bf16 *)x + off);
 }
 
-// three-way bf16 split of 4 f32 values (x = hi + mid + lo to 24 bits);
-// invalid lanes give zeros
+// exact three-way bf16 split of 4 f32 values by truncation: hi keeps the
+// top 8 significant bits, mid the next 8 of the remainder, lo the last 8
+// (x = hi + mid + lo exactly, the subtractions are exact); invalid lanes
+// give zeros
 __device__ __forceinline__ void split3(const f32x4 &x, bool ok, bf16x4 (&b)[3])
 {
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const float v = ok ? x[j] : 0.f;
-    const bf16 h = f2bf(v);
-    const float r1 = v - bf2f(h);
-    const bf16 m = f2bf(r1);
-    b[0][j] = (short)h;
-    b[1][j] = (short)m;
-    b[2][j] = (short)f2bf(r1 - bf2f(m));
+    const unsigned u = ok ? __float_as_uint(x[j]) : 0u;
+    const unsigned h = u & 0xffff0000u;
+    const float r1 = __uint_as_float(u) - __uint_as_float(h);
+    const unsigned m = __float_as_uint(r1) & 0xffff0000u;
+    const float r2 = r1 - __uint_as_float(m);
+    b[0][j] = (short)(h >> 16);
+    b[1][j] = (short)(m >> 16);
+    b[2][j] = (short)(__float_as_uint(r2) >> 16);
   }
 }
 
-// Each wave computes one 16-channel output group (blockIdx.y) for NV x 16
-// voxels; the 27 taps are unrolled without branches (clamped addresses,
-// zeroed fragments), so the gathers of many taps are in flight together.
-template <int NV, int MODE, int XF>
+// epilogue for 4 consecutive channels co.. of output voxel g
+__device__ __forceinline__ void store_out(const ugk_conv_desc &d, long g, int co, const f32x4 &acc)
+{
+  const int nj = d.cout - co < 4 ? d.cout - co : 4;
+  float v[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) v[j] = acc[j];
+  if (d.emul)
+    for (int j = 0; j < nj; j++) v[j] *= d.emul[g * d.ldm + co + j];
+  if (d.ga)
+    for (int j = 0; j < nj; j++) v[j] += d.ga[g * d.ldga + co + j] * d.gb[g * d.ldgb + co + j];
+  if (d.y_f32) {
+    float *y = (float *)d.y + g * d.ldy + co;
+    if (nj == 4) *(f32x4 *)y = (f32x4){v[0], v[1], v[2], v[3]};
+    else for (int j = 0; j < nj; j++) y[j] = v[j];
+  } else {
+    bf16 *y = (bf16 *)d.y + g * d.ldy + co;
+    if (nj == 4) *(bf16x4 *)y = (bf16x4){(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
+    else for (int j = 0; j < nj; j++) y[j] = f2bf(v[j]);
+  }
+}
+
+template <int XF>
+__device__ __forceinline__ void mfma_frag(const bf16x4 &a, const typename Frag<XF>::T &raw, bool ok, f32x4 &acc)
+{
+  if constexpr (XF) {
+    bf16x4 bx[3];
+    split3(raw, ok, bx);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[2], acc, 0, 0, 0);   // smallest part first
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[0], acc, 0, 0, 0);
+  } else {
+    const bf16x4 bx = ok ? raw : (bf16x4){0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx, acc, 0, 0, 0);
+  }
+}
+
+// Stride-1 and stride-2 convolution.  Each wave computes one 16-channel
+// output group (blockIdx.y) for NV x 16 voxels; the 27 taps are unrolled
+// without branches (clamped addresses, zeroed fragments), so the gathers of
+// many taps are in flight together.  PK: the input has 4 channels, and the
+// four lane groups of the MFMA's K dimension take four taps at once.
+template <int NV, int MODE, int XF, int PK>
 __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
 {
   if (d.skip && *d.skip) return;
-  constexpr int NS = XF ? 3 : 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
   const int mg = blockIdx.y;
@@ -142,81 +183,189 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
   const bf16 *wrow = d.w + (long)(mg * 16 + col) * d.cip;
   const long wstep = (long)d.cop * d.cip;
 
-  for (int c0 = 0; c0 < d.cin; c0 += 16) {
-    const int c = c0 + 4 * kq;
-    const bool cok = c < d.cin;
-    const int cs = cok ? c : 0;              // in-bounds channel for masked lanes
+  if constexpr (PK) {
+    // tap groups of 4: lane group kq takes tap 4 q + kq, channels 0..3
 #pragma unroll
-    for (int t = 0; t < 27; t++) {
-      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    for (int q = 0; q < 7; q++) {
+      const int t = 4 * q + kq;
+      const bool tok = t < 27;
+      const int tt = tok ? t : 0;
+      const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
       typename Frag<XF>::T raw[NV];
       bool ok[NV];
 #pragma unroll
       for (int n = 0; n < NV; n++) {
         const long sidx = tap_src<MODE>(vz[n], vy[n], vx[n], dz, dy, dx, d.gi);
-        ok[n] = vok[n] && cok && sidx >= 0;
-        raw[n] = load_raw<XF>(d.x, (ok[n] ? vb[n] + sidx : 0) * d.ldx + cs);
+        ok[n] = vok[n] && tok && sidx >= 0;
+        raw[n] = load_raw<XF>(d.x, (ok[n] ? vb[n] + sidx : 0) * d.ldx);
       }
-      const bf16x4 a = *(const bf16x4 *)(wrow + (d.flip ? 26 - t : t) * wstep + c);
+      bf16x4 a = *(const bf16x4 *)(wrow + (d.flip ? 26 - tt : tt) * wstep);
+      if (!tok) a = (bf16x4){0, 0, 0, 0};
 #pragma unroll
-      for (int n = 0; n < NV; n++) {
-        bf16x4 bx[3];
-        if constexpr (XF) split3(raw[n], ok[n], bx);
-        else bx[0] = ok[n] ? raw[n] : (bf16x4){0, 0, 0, 0};
+      for (int n = 0; n < NV; n++) mfma_frag<XF>(a, raw[n], ok[n], acc[n]);
+    }
+  } else {
+    for (int c0 = 0; c0 < d.cin; c0 += 16) {
+      const int c = c0 + 4 * kq;
+      const bool cok = c < d.cin;
+      const int cs = cok ? c : 0;              // in-bounds channel for masked lanes
 #pragma unroll
-        for (int k = NS - 1; k >= 0; k--)      // smallest part first
-          acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[k], acc[n], 0, 0, 0);
+      for (int t = 0; t < 27; t++) {
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        typename Frag<XF>::T raw[NV];
+        bool ok[NV];
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+          const long sidx = tap_src<MODE>(vz[n], vy[n], vx[n], dz, dy, dx, d.gi);
+          ok[n] = vok[n] && cok && sidx >= 0;
+          raw[n] = load_raw<XF>(d.x, (ok[n] ? vb[n] + sidx : 0) * d.ldx + cs);
+        }
+        const bf16x4 a = *(const bf16x4 *)(wrow + (d.flip ? 26 - t : t) * wstep + c);
+#pragma unroll
+        for (int n = 0; n < NV; n++) mfma_frag<XF>(a, raw[n], ok[n], acc[n]);
       }
     }
   }
-  // epilogue: lane holds channels mg*16 + 4 kq + j of voxel base + n*16 + col
+  // lane holds channels mg*16 + 4 kq + j of voxel base + n*16 + col
   const int co = mg * 16 + 4 * kq;
   if (co >= d.cout) return;
-  const int nj = d.cout - co < 4 ? d.cout - co : 4;
 #pragma unroll
-  for (int n = 0; n < NV; n++) {
-    if (!vok[n]) continue;
-    const long g = base + n * 16 + col;
-    float v[4];
+  for (int n = 0; n < NV; n++)
+    if (vok[n]) store_out(d, base + n * 16 + col, co, acc[n]);
+}
+
+// Stride-2 transpose (coarse input gi -> fine output go): y[f] = sum_t W[t]
+// x[(f - off_t) / 2] over the taps with f - off_t even.  Output voxels are
+// taken parity class by parity class (f = 2c + p), so every lane of a wave
+// has the same 1, 2, 4 or 8 live taps: per axis, p = 0 takes off 0 at c, and
+// p = 1 takes off +1 at c and off -1 at c + 1.  Work items are
+// [nb][8][Vc rounded up to 16] (c over the coarse grid, rows of 16 never
+// straddle two classes).
+template <int XF>
+__global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
+{
+  if (d.skip && *d.skip) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int mg = blockIdx.y;
+  const long Vc = vox(d.gi), Vf = vox(d.go);
+  const long Vp = (Vc + 15) / 16 * 16;
+  const long total = Vp * 8 * d.nb;
+  const long base = ((long)blockIdx.x * 4 + wave) * 16;
+  if (base >= total) return;
+  const long cls = base / Vp;               // (b, p), uniform over the wave
+  const int p = (int)(cls & 7);
+  const long b = cls >> 3;
+  const long ci = base - cls * Vp + col;     // coarse voxel of this lane
+  const bool cin_ok = ci < Vc;
+  const int HWc = d.gi.H * d.gi.W;
+  const int cz = cin_ok ? (int)(ci / HWc) : 0;
+  const int rr = cin_ok ? (int)(ci - (long)cz * HWc) : 0;
+  const int cy = rr / d.gi.W, cx = rr - cy * d.gi.W;
+  const int pz = (p >> 2) & 1, py = (p >> 1) & 1, px = p & 1;
+  const int fz = 2 * cz + pz, fy = 2 * cy + py, fx = 2 * cx + px;
+  const bool fok = cin_ok && fz < d.go.D && fy < d.go.H && fx < d.go.W;
+  f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const bf16 *wrow = d.w + (long)(mg * 16 + col) * d.cip;
+  const long wstep = (long)d.cop * d.cip;
+  for (int c0 = 0; c0 < d.cin; c0 += 16) {
+    const int c = c0 + 4 * kq;
+    const bool cok = c < d.cin;
+    const int cs = cok ? c : 0;
+    for (int iz = 0; iz <= pz; iz++)
+      for (int iy = 0; iy <= py; iy++)
+        for (int ix = 0; ix <= px; ix++) {
+          // offset +1 reads c, offset -1 reads c + 1; parity 0 has offset 0
+          const int dz = pz ? 1 - 2 * iz : 0, dy = py ? 1 - 2 * iy : 0, dx = px ? 1 - 2 * ix : 0;
+          const int sz = cz + iz, sy = cy + iy, sx = cx + ix;
+          const bool ok = fok && cok && sz < d.gi.D && sy < d.gi.H && sx < d.gi.W;
+          const long src = ok ? b * Vc + ((long)sz * d.gi.H + sy) * d.gi.W + sx : 0;
+          const typename Frag<XF>::T raw = load_raw<XF>(d.x, src * d.ldx + cs);
+          const int t = 9 * (dz + 1) + 3 * (dy + 1) + (dx + 1);
+          const bf16x4 a = *(const bf16x4 *)(wrow + t * wstep + c);
+          mfma_frag<XF>(a, raw, ok, acc);
+        }
+  }
+  const int co = mg * 16 + 4 * kq;
+  if (co >= d.cout || !fok) return;
+  store_out(d, b * Vf + ((long)fz * d.go.H + fy) * d.go.W + fx, co, acc);
+}
+
+// Stride-1 convolution to at most 4 output channels (the network's last
+// layer) on the vector ALUs in f32: one thread per output voxel, the
+// bf16 weights of its channels widened to f32 in LDS.  A 16-row matrix-core
+// tile would leave 12 of its rows idle here.
+#define SO_MAX_CIN 64
+__global__ __launch_bounds__(256) void k_conv_small_out(ugk_conv_desc d)
+{
+  __shared__ float sw[27][4][SO_MAX_CIN];
+  if (d.skip && *d.skip) return;
+  for (int i = threadIdx.x; i < 27 * 4 * SO_MAX_CIN; i += 256) {
+    const int t = i / (4 * SO_MAX_CIN), r = i - t * 4 * SO_MAX_CIN, co = r / SO_MAX_CIN, ci = r - co * SO_MAX_CIN;
+    const int tw = d.flip ? 26 - t : t;
+    sw[t][co][ci] = (co < d.cout && ci < d.cin) ? bf2f(d.w[((long)tw * d.cop + co) * d.cip + ci]) : 0.f;
+  }
+  __syncthreads();
+  const long Vo = vox(d.go), Vi = vox(d.gi);
+  const long total = Vo * d.nb;
+  const int HWo = d.go.H * d.go.W;
+  for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long)gridDim.x * 256) {
+    const long b = g / Vo;
+    const int v = (int)(g - b * Vo);
+    const int z = v / HWo, r = v - z * HWo, y = r / d.go.W, x = r - y * d.go.W;
+    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = acc[n][j];
-    if (d.emul)
-      for (int j = 0; j < nj; j++) v[j] *= d.emul[g * d.ldm + co + j];
-    if (d.ga)
-      for (int j = 0; j < nj; j++) v[j] += d.ga[g * d.ldga + co + j] * d.gb[g * d.ldgb + co + j];
-    if (d.y_f32) {
-      float *y = (float *)d.y + g * d.ldy + co;
-      if (nj == 4) *(f32x4 *)y = (f32x4){v[0], v[1], v[2], v[3]};
-      else for (int j = 0; j < nj; j++) y[j] = v[j];
-    } else {
-      bf16 *y = (bf16 *)d.y + g * d.ldy + co;
-      if (nj == 4) *(bf16x4 *)y = (bf16x4){(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
-      else for (int j = 0; j < nj; j++) y[j] = f2bf(v[j]);
+    for (int t = 0; t < 27; t++) {
+      const long src = tap_src<UGK_CONV_S1>(z, y, x, t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1, d.gi);
+      if (src < 0) continue;
+      const float *xp = (const float *)d.x + (b * Vi + src) * d.ldx;
+      for (int c = 0; c < d.cin; c += 4) {
+        const f32x4 xv = *(const f32x4 *)(xp + c);
+#pragma unroll
+        for (int co = 0; co < 4; co++)
+          acc[co] += sw[t][co][c] * xv[0] + sw[t][co][c + 1] * xv[1] + sw[t][co][c + 2] * xv[2] +
+                     sw[t][co][c + 3] * xv[3];
+      }
     }
+    store_out(d, g, 0, acc);
   }
 }
 
-template <int NV, int XF>
+template <int NV, int XF, int PK>
 static void launch_conv_nv(const ugk_conv_desc &d, dim3 grid, hipStream_t s)
 {
-  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF><<<grid, 256, 0, s>>>(d);
-  else if (d.mode == UGK_CONV_S2) k_conv<NV, UGK_CONV_S2, XF><<<grid, 256, 0, s>>>(d);
-  else k_conv<NV, UGK_CONV_T2, XF><<<grid, 256, 0, s>>>(d);
+  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF, PK><<<grid, 256, 0, s>>>(d);
+  else k_conv<NV, UGK_CONV_S2, XF, PK><<<grid, 256, 0, s>>>(d);
 }
 
 template <int XF>
 static void launch_conv(const ugk_conv_desc &d, long total, hipStream_t s)
 {
-  // two voxel tiles per wave only when there are waves to spare
   const unsigned groups = (unsigned)(d.cop / 16);
+  if (d.mode == UGK_CONV_T2) {
+    const long items = (vox(d.gi) + 15) / 16 * 16 * 8 * d.nb;
+    k_conv_t2<XF><<<dim3((unsigned)((items + 63) / 64), groups), 256, 0, s>>>(d);
+    return;
+  }
+  if (d.mode == UGK_CONV_S1 && d.cout <= 4 && XF && d.cin <= SO_MAX_CIN) {
+    k_conv_small_out<<<(unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096), 256, 0, s>>>(d);
+    return;
+  }
   static int nv_env = -1;
   if (nv_env < 0) {
     const char *e = getenv("UFEM_CONV_NV");
     nv_env = e ? atoi(e) : 0;
   }
+  // two voxel tiles per wave only when there are waves to spare
   const bool two = nv_env ? nv_env == 2 : total * groups >= 64L * 4 * 2048;
-  if (two) launch_conv_nv<2, XF>(d, dim3((unsigned)((total + 127) / 128), groups), s);
-  else launch_conv_nv<1, XF>(d, dim3((unsigned)((total + 63) / 64), groups), s);
+  const dim3 g1((unsigned)((total + 63) / 64), groups), g2((unsigned)((total + 127) / 128), groups);
+  if (d.cin == 4) {
+    if (two) launch_conv_nv<2, XF, 1>(d, g2, s);
+    else launch_conv_nv<1, XF, 1>(d, g1, s);
+  } else {
+    if (two) launch_conv_nv<2, XF, 0>(d, g2, s);
+    else launch_conv_nv<1, XF, 0>(d, g1, s);
+  }
 }
 
 // --------------------------------------------------------------------------

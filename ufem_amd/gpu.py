@@ -90,6 +90,7 @@ def lib():
             "ufem_net_npairs": (C.c_long, [C.c_void_p]),
             "ufem_net_collect": (C.c_long, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p]),
             "ufem_net_train": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_double, C.c_ulong, P_D, P_D]),
+            "ufem_net_time_stages": (C.c_int, [C.c_void_p, C.c_int, P_D]),
             "ufem_net_time_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P_I, P_I, C.c_int, C.c_int,
                                              C.c_int, P_D]),
             "ufem_net_probe_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P_I, P_I, C.c_int, C.c_int,
@@ -274,8 +275,8 @@ class GpuSystem:
         return y
 
     def time_kernel(self, mode="spmv", reps=20):
-        """(avg ms, algorithmic bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a', 'pcg_q', 'cg1_w'."""
-        m = {"spmv": 0, "spmv32": 1, "pcg_a": 2, "pcg_q": 3, "cg1_w": 4}[mode]
+        """(avg ms, algorithmic bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a', 'pcg_q', 'cg1_w', 'copy'."""
+        m = {"spmv": 0, "spmv32": 1, "pcg_a": 2, "pcg_q": 3, "cg1_w": 4, "copy": 5}[mode]
         ms = C.c_double(0)
         by = C.c_double(0)
         self._chk(lib().ufem_sys_time_spmv(self.h, reps, m, C.byref(ms), C.byref(by)), "time_kernel")
@@ -356,6 +357,13 @@ class GpuNet:
         f = C.c_int(0)
         self._chk(lib().ufem_net_get_pair(self.h, i, _pf(v), _pf(rr), _pf(ee), C.byref(er), C.byref(f)), "pair")
         return v, rr, ee, er.value, f.value
+
+    STAGES = ("C0", "C1", "C2", "gate", "C2T", "C1T", "C0T")
+
+    def time_stages(self, reps=50):
+        us = np.zeros(7)
+        self._chk(lib().ufem_net_time_stages(self.h, reps, _pd(us)), "time_stages")
+        return dict(zip(self.STAGES, us))
 
     def eval(self, i0, cnt):
         out = np.zeros(cnt)
