@@ -160,7 +160,7 @@ typedef struct ufem_net ufem_net;
 ufem_net *ufem_net_create(int W, int H, int D, int batch);  /* batch: training samples per step */
 void ufem_net_destroy(ufem_net *N);
 long ufem_net_nparams(void);
-int  ufem_net_init(ufem_net *N, unsigned long seed);          /* W ~ N(0, 1/(27 cin)), gates 0 */
+int  ufem_net_init(ufem_net *N, unsigned long seed);   /* W ~ N(0, 1/(27 cin)), a = 0, b = -3 */
 int  ufem_net_set_weights(ufem_net *N, const float *w);
 int  ufem_net_get_weights(ufem_net *N, float *w);
 /* y = N(x) for nb samples: x [nb][V][3], ebar0 [nb][V] (node log-moduli),

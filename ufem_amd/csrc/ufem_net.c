@@ -114,6 +114,7 @@ void ufem_net_destroy(ufem_net *N)
 {
   int l, f;
   if (!N) return;
+  if (N->stream) (void)ugk_sync(N->stream);
   ugk_free(N->w); ugk_free(N->gr); ugk_free(N->m); ugk_free(N->v);
   for (l = 0; l < 3; l++) {
     ugk_free(N->wf[l]); ugk_free(N->wt[l]);
@@ -257,13 +258,19 @@ static unsigned long long mix64(unsigned long long z)
   return z ^ (z >> 31);
 }
 
-/* weights N(0, 1 / (27 cin)) from a counter-based generator; gates a = b = 0 */
+/* weights N(0, 1 / (27 cin)) from a counter-based generator; gates a = 0,
+ * b = -3, so that the untrained correction is small (g ~ 0.05 / E-ratio) and
+ * training starts close to Jacobi PCG */
 int ufem_net_init(ufem_net *N, unsigned long seed)
 {
   float *w = (float *)calloc((size_t)N->P, sizeof(float));
   long i = 0;
   int l, rv;
   if (!w) return AF_ERR_MEM;
+  for (l = 0; l < 3; l++) {
+    long k;
+    for (k = 0; k < CH[l]; k++) w[N->offB[l] + k] = -3.f;
+  }
   for (l = 0; l < 3; l++) {
     const long Pl = 27L * CI[l] * CH[l];
     const double sd = 1.0 / sqrt(27.0 * CI[l]);

@@ -1884,7 +1884,10 @@ void *ugk_malloc(size_t bytes)
   if (bytes == 0) bytes = 8;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) { fail("hipMalloc", e); return nullptr; }
+  // hipMemset runs on the null stream, which does not order itself with the
+  // library's non-blocking streams: finish it before anyone uploads into p
   e = hipMemset(p, 0, bytes);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) { fail("hipMemset", e); (void)hipFree(p); return nullptr; }
   return p;
 }

@@ -21,6 +21,7 @@ import math
 from dataclasses import dataclass, field
 
 import numpy as np
+import scipy.fft as sfft
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -78,7 +79,8 @@ def grf_E(seed: int, nx: int, ny: int, nz: int, h: tuple[float, float, float],
     r2 = dx * dx + dy * dy + dz * dz
     ker = np.exp(-2.0 * r2 / (corr_len * corr_len))
     ker /= math.sqrt(float(np.sum(ker * ker)))
-    g = np.fft.ifftn(np.fft.fftn(w) * np.fft.fftn(ker)).real
+    # real-to-complex transforms on all host cores (scipy.fft)
+    g = sfft.irfftn(sfft.rfftn(w, workers=-1) * sfft.rfftn(ker, workers=-1), s=w.shape, workers=-1)
     g = g[pad[2]:pad[2] + nz, pad[1]:pad[1] + ny, pad[0]:pad[0] + nx]
     mu = math.log(mean) - 0.5 * log_std * log_std
     return np.exp(mu + log_std * g).reshape(-1)
