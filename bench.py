@@ -74,6 +74,34 @@ def cpu_baseline(mesh, gpu_iters, sample_iters=60):
     }
 
 
+def bench_mixed_high_contrast(nx, ny, nz, seed):
+    """FP64 Jacobi PCG vs FP32-inner / FP64-refinement PCG on the high-contrast
+    field (lognormal E, COV 0.6), both to ||r||/||f|| <= 1e-8, and the
+    displacement error of the mixed solve against the FP64 one."""
+    from ufem_amd import gpu as G
+    from ufem_amd.mesh import cantilever, lognormal_E
+    mesh = cantilever(nx, ny, nz, E=lognormal_E(seed, nx * ny * nz, cov=0.6))
+    g = G.GpuSystem(mesh)
+    g.assemble(2)
+    g.solve(solver="pcg", eps=1e-8, rule=1)
+    t0 = time.perf_counter()
+    r64 = g.solve(solver="pcg", eps=1e-8, rule=1)
+    t64 = time.perf_counter() - t0
+    u64 = g.u()
+    g.solve(solver="mixed", eps=1e-8, rule=1, maxit=40000)
+    t0 = time.perf_counter()
+    rm = g.solve(solver="mixed", eps=1e-8, rule=1, maxit=40000)
+    tm = time.perf_counter() - t0
+    um = g.u()
+    g.close()
+    return {"workload": f"cantilever {nx}x{ny}x{nz}, E lognormal COV 0.6 seed {seed}",
+            "fp64": {"iters": r64["iters"], "solve_s": round(t64, 4), "true_relres": r64["true_relres"]},
+            "mixed": {"iters": rm["iters"], "outer": rm["outer"], "solve_s": round(tm, 4), "rv": rm["rv"],
+                      "true_relres": rm["true_relres"]},
+            "u_rel_err_vs_fp64": float(np.linalg.norm(um - u64) / np.linalg.norm(u64)),
+            "speedup_vs_fp64": round(t64 / tm, 3)}
+
+
 def bench_learned(fields, steps, seed):
     """Learned preconditioner on the 128 x 32 x 32 GRF cantilever (421,443 DOF):
     pairs from plain-PCG solves on `fields` fresh fields (32 each), `steps`
@@ -275,6 +303,11 @@ def main():
                  "speedup_vs_fp64": round(solve_s / m_s, 3)}
         log("mixed", mixed)
 
+    mixed_cov06 = None
+    if not args.no_mixed:
+        mixed_cov06 = bench_mixed_high_contrast(nx, ny, nz, args.seed + 1)
+        log("mixed, COV 0.6", mixed_cov06)
+
     learned = None
     if not args.no_learned:
         log("learned preconditioner ...")
@@ -319,6 +352,7 @@ def main():
         "roofline": roofline,
         "spmv": spmv,
         "mixed": mixed,
+        "mixed_cov06": mixed_cov06,
         "learned": learned,
         "batch": batch,
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),

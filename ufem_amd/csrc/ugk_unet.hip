@@ -184,45 +184,59 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
   const long wstep = (long)d.cop * d.cip;
 
   if constexpr (PK) {
-    // tap groups of 4: lane group kq takes tap 4 q + kq, channels 0..3
+    // tap groups of 4: lane group kq takes tap 4 q + kq, channels 0..3; all
+    // 7 groups' fragments are loaded before the first product
+    typename Frag<XF>::T raw[7][NV];
+    bool ok[7][NV];
+    bf16x4 a[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) {
       const int t = 4 * q + kq;
       const bool tok = t < 27;
       const int tt = tok ? t : 0;
       const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
-      typename Frag<XF>::T raw[NV];
-      bool ok[NV];
 #pragma unroll
       for (int n = 0; n < NV; n++) {
         const long sidx = tap_src<MODE>(vz[n], vy[n], vx[n], dz, dy, dx, d.gi);
-        ok[n] = vok[n] && tok && sidx >= 0;
-        raw[n] = load_raw<XF>(d.x, (ok[n] ? vb[n] + sidx : 0) * d.ldx);
+        ok[q][n] = vok[n] && tok && sidx >= 0;
+        raw[q][n] = load_raw<XF>(d.x, (ok[q][n] ? vb[n] + sidx : 0) * d.ldx);
       }
-      bf16x4 a = *(const bf16x4 *)(wrow + (d.flip ? 26 - tt : tt) * wstep);
-      if (!tok) a = (bf16x4){0, 0, 0, 0};
-#pragma unroll
-      for (int n = 0; n < NV; n++) mfma_frag<XF>(a, raw[n], ok[n], acc[n]);
+      a[q] = *(const bf16x4 *)(wrow + (d.flip ? 26 - tt : tt) * wstep);
+      if (!tok) a[q] = (bf16x4){0, 0, 0, 0};
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < 7; q++)
+#pragma unroll
+      for (int n = 0; n < NV; n++) mfma_frag<XF>(a[q], raw[q][n], ok[q][n], acc[n]);
   } else {
     for (int c0 = 0; c0 < d.cin; c0 += 16) {
       const int c = c0 + 4 * kq;
       const bool cok = c < d.cin;
       const int cs = cok ? c : 0;              // in-bounds channel for masked lanes
+      // one dz plane (9 taps) of gathers in flight before its products
 #pragma unroll
-      for (int t = 0; t < 27; t++) {
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        typename Frag<XF>::T raw[NV];
-        bool ok[NV];
+      for (int pz = 0; pz < 3; pz++) {
+        typename Frag<XF>::T raw[9][NV];
+        bool ok[9][NV];
+        bf16x4 a[9];
 #pragma unroll
-        for (int n = 0; n < NV; n++) {
-          const long sidx = tap_src<MODE>(vz[n], vy[n], vx[n], dz, dy, dx, d.gi);
-          ok[n] = vok[n] && cok && sidx >= 0;
-          raw[n] = load_raw<XF>(d.x, (ok[n] ? vb[n] + sidx : 0) * d.ldx + cs);
+        for (int k = 0; k < 9; k++) {
+          const int t = 9 * pz + k;
+          const int dz = pz - 1, dy = k / 3 - 1, dx = k % 3 - 1;
+#pragma unroll
+          for (int n = 0; n < NV; n++) {
+            const long sidx = tap_src<MODE>(vz[n], vy[n], vx[n], dz, dy, dx, d.gi);
+            ok[k][n] = vok[n] && cok && sidx >= 0;
+            raw[k][n] = load_raw<XF>(d.x, (ok[k][n] ? vb[n] + sidx : 0) * d.ldx + cs);
+          }
+          a[k] = *(const bf16x4 *)(wrow + (d.flip ? 26 - t : t) * wstep + c);
         }
-        const bf16x4 a = *(const bf16x4 *)(wrow + (d.flip ? 26 - t : t) * wstep + c);
+        __builtin_amdgcn_sched_barrier(0);   // keep the plane's gathers ahead of its products
 #pragma unroll
-        for (int n = 0; n < NV; n++) mfma_frag<XF>(a, raw[n], ok[n], acc[n]);
+        for (int k = 0; k < 9; k++)
+#pragma unroll
+          for (int n = 0; n < NV; n++) mfma_frag<XF>(a[k], raw[k][n], ok[k][n], acc[n]);
       }
     }
   }
@@ -292,42 +306,62 @@ __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
 }
 
 // Stride-1 convolution to at most 4 output channels (the network's last
-// layer) on the vector ALUs in f32: one thread per output voxel, the
-// bf16 weights of its channels widened to f32 in LDS.  A 16-row matrix-core
-// tile would leave 12 of its rows idle here.
+// layer) on the vector ALUs in f32.  Four lanes share one output voxel, each
+// taking one 4-channel quarter of every 16 input channels, so a lane has
+// the 9 tap gathers of a dz plane in flight at once; the four partial sums are
+// added by lane shuffles in a fixed order.  The bf16 weights of the output
+// channels are widened to f32 in LDS.  A 16-row matrix-core tile would
+// leave 12 of its rows idle here.
 #define SO_MAX_CIN 64
+template <int C16>
 __global__ __launch_bounds__(256) void k_conv_small_out(ugk_conv_desc d)
 {
-  __shared__ float sw[27][4][SO_MAX_CIN];
+  __shared__ f32x4 sw[27][4][4 * C16];          // [tap][co][ci / 4]
   if (d.skip && *d.skip) return;
-  for (int i = threadIdx.x; i < 27 * 4 * SO_MAX_CIN; i += 256) {
-    const int t = i / (4 * SO_MAX_CIN), r = i - t * 4 * SO_MAX_CIN, co = r / SO_MAX_CIN, ci = r - co * SO_MAX_CIN;
+  for (int i = threadIdx.x; i < 27 * 4 * 16 * C16; i += 256) {
+    const int t = i / (64 * C16), r = i - t * 64 * C16, co = r / (16 * C16), ci = r - co * 16 * C16;
     const int tw = d.flip ? 26 - t : t;
-    sw[t][co][ci] = (co < d.cout && ci < d.cin) ? bf2f(d.w[((long)tw * d.cop + co) * d.cip + ci]) : 0.f;
+    const bool in = co < d.cout && ci < d.cin;
+    ((float *)&sw[0][0][0])[i] = in ? bf2f(d.w[((long)tw * d.cop + co) * d.cip + ci]) : 0.f;
   }
   __syncthreads();
   const long Vo = vox(d.go), Vi = vox(d.gi);
   const long total = Vo * d.nb;
   const int HWo = d.go.H * d.go.W;
-  for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long)gridDim.x * 256) {
+  const int cq = threadIdx.x & 3;
+  // the loop condition depends on the voxel only, so the four lanes of a
+  // voxel stay together for the shuffles
+  for (long g = ((long)blockIdx.x * 256 + threadIdx.x) >> 2; g < total; g += (long)gridDim.x * 64) {
     const long b = g / Vo;
     const int v = (int)(g - b * Vo);
     const int z = v / HWo, r = v - z * HWo, y = r / d.go.W, x = r - y * d.go.W;
     f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const float *xb = (const float *)d.x + b * Vi * d.ldx;
+#pragma unroll 1
+    for (int jz = 0; jz < 3 * C16; jz++) {       // (16-channel group, dz plane)
+      const int j = jz / 3, dz = jz - 3 * j - 1;
+      const int c = 16 * j + 4 * cq;
+      f32x4 xv[9];
 #pragma unroll
-    for (int t = 0; t < 27; t++) {
-      const long src = tap_src<UGK_CONV_S1>(z, y, x, t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1, d.gi);
-      if (src < 0) continue;
-      const float *xp = (const float *)d.x + (b * Vi + src) * d.ldx;
-      for (int c = 0; c < d.cin; c += 4) {
-        const f32x4 xv = *(const f32x4 *)(xp + c);
-#pragma unroll
-        for (int co = 0; co < 4; co++)
-          acc[co] += sw[t][co][c] * xv[0] + sw[t][co][c + 1] * xv[1] + sw[t][co][c + 2] * xv[2] +
-                     sw[t][co][c + 3] * xv[3];
+      for (int t = 0; t < 9; t++) {
+        const long src = tap_src<UGK_CONV_S1>(z, y, x, dz, t / 3 - 1, t % 3 - 1, d.gi);
+        const f32x4 v4 = *(const f32x4 *)(xb + (src < 0 ? 0 : src) * d.ldx + c);
+        xv[t] = src < 0 ? (f32x4){0.f, 0.f, 0.f, 0.f} : v4;
       }
+#pragma unroll
+      for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int co = 0; co < 4; co++) {
+          const f32x4 w4 = sw[9 * (dz + 1) + t][co][4 * j + cq];
+          acc[co] += w4[0] * xv[t][0] + w4[1] * xv[t][1] + w4[2] * xv[t][2] + w4[3] * xv[t][3];
+        }
     }
-    store_out(d, g, 0, acc);
+#pragma unroll
+    for (int co = 0; co < 4; co++) {
+      acc[co] += __shfl_xor(acc[co], 1);
+      acc[co] += __shfl_xor(acc[co], 2);
+    }
+    if (cq == 0) store_out(d, g, 0, acc);
   }
 }
 
@@ -347,8 +381,14 @@ static void launch_conv(const ugk_conv_desc &d, long total, hipStream_t s)
     k_conv_t2<XF><<<dim3((unsigned)((items + 63) / 64), groups), 256, 0, s>>>(d);
     return;
   }
-  if (d.mode == UGK_CONV_S1 && d.cout <= 4 && XF && d.cin <= SO_MAX_CIN) {
-    k_conv_small_out<<<(unsigned)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096), 256, 0, s>>>(d);
+  if (d.mode == UGK_CONV_S1 && d.cout <= 4 && XF && d.cin <= SO_MAX_CIN && d.cin % 16 == 0) {
+    const unsigned nblk = (unsigned)((total + 63) / 64 < 8192 ? (total + 63) / 64 : 8192);
+    switch (d.cin / 16) {
+      case 1: k_conv_small_out<1><<<nblk, 256, 0, s>>>(d); break;
+      case 2: k_conv_small_out<2><<<nblk, 256, 0, s>>>(d); break;
+      case 3: k_conv_small_out<3><<<nblk, 256, 0, s>>>(d); break;
+      default: k_conv_small_out<4><<<nblk, 256, 0, s>>>(d); break;
+    }
     return;
   }
   static int nv_env = -1;
