@@ -10,9 +10,12 @@ K, f already resident in HBM.
 
 Printed: one JSON line with the solve time (value, seconds), the roofline of
 the solve's dominant kernel (w = K u with (w, u) of the single-reduction PCG),
-SpMV bandwidth, the mixed-precision solve on the same system, the learned
-preconditioner on the 128 x 32 x 32 random-field cantilever (training on the
-GPU, then plain vs learned flexible CG on an unseen field), the batched
+SpMV bandwidth, the mixed-precision solve on the same system (and on a COV
+0.6 field), learned + mixed precision on the same system (a network trained
+on the GPU from fresh lognormal fields; plain FP64 PCG vs flexible CG with
+the network, FP64 and with the FP32 matrix), the learned preconditioner on
+the 128 x 32 x 32 random-field cantilever (training on the GPU, then plain
+vs learned flexible CG on an unseen field), the batched
 Monte-Carlo solve of 256 uneven meshes in one launch, and the CPU baseline
 (the oracle's pthread restatement of uFEM's solver, bounded sample).
 
@@ -132,7 +135,7 @@ def bench_learned(fields, steps, seed):
         return r, time.perf_counter() - t0, s.u()
 
     rp, tp, up = timed(solver="pcg", eps=1e-8, rule=1)
-    rl, tl, ul = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=4)
+    rl, tl, ul = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=1)
     stages = net.time_stages()
     out = {
         "workload": (f"cantilever 2.0x0.4x0.4 m, {nx}x{ny}x{nz} bricks, {mesh.n_dof} DOF, E = exp(GRF, "
@@ -141,7 +144,7 @@ def bench_learned(fields, steps, seed):
         "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
         "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
         "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
-        "learned": {"iters": rl["iters"], "solve_s": round(tl, 4), "fcg_m": 4, "fallback": rl["fallback"],
+        "learned": {"iters": rl["iters"], "solve_s": round(tl, 4), "fcg_m": 1, "fallback": rl["fallback"],
                     "true_relres": rl["true_relres"],
                     "u_rel_err_vs_plain": float(np.linalg.norm(ul - up) / np.linalg.norm(up))},
         "iteration_ratio": round(rp["iters"] / max(rl["iters"], 1), 3),
@@ -151,6 +154,62 @@ def bench_learned(fields, steps, seed):
     }
     net.close()
     s.close()
+    return out
+
+
+def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3):
+    """Learned + mixed precision on the metric's own system `g` (the 1M-DOF
+    lognormal cantilever, E = mesh.E): pairs from plain-PCG solves on `fields`
+    fresh i.i.d. lognormal fields of the same distribution (seeds seed_base..),
+    `steps` Adam steps, then plain FP64 PCG, flexible CG with the network
+    (FP64) and flexible CG with the network and q = K p from the FP32 copy of
+    K (FP64 residual refreshes), all to ||r||/||f|| <= 1e-8 from u = 0."""
+    from ufem_amd import gpu as G
+    from ufem_amd import learned as Lr
+    from ufem_amd.mesh import lognormal_E
+    E_test = mesh.E.copy()
+    net = G.GpuNet(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, batch=4)
+    net.init(1)
+    net.reserve(32 * fields)
+    pairs, collect_s = Lr.collect(g, net, mesh, range(seed_base, seed_base + fields), 32, every=8, log=log,
+                                  field=lambda m, sd: lognormal_E(sd, m.n_elems, cov=cov))
+    loss, train_ms = Lr.train(net, steps, batch=4, lr=3e-3)
+    log(f"learned (1M DOF): {pairs} pairs in {collect_s:.1f} s, {steps} steps in {train_ms / 1e3:.1f} s, "
+        f"loss {loss[:20].mean():.3f} -> {loss[-50:].mean():.3f}")
+    Lr.rebind(g, net, mesh, E_test)
+
+    def timed(**kw):
+        g.solve(**kw)
+        t0 = time.perf_counter()
+        r = g.solve(**kw)
+        return r, time.perf_counter() - t0, g.u()
+
+    rp, tp, up = timed(solver="pcg", eps=1e-8, rule=1)
+    rl, tl, ul = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=1)
+    rm, tm, um = timed(solver="learned_mixed", eps=1e-8, rule=1, net=net, fcg_m=1)
+    stages = net.time_stages()
+
+    def err(u):
+        return float(np.linalg.norm(u - up) / np.linalg.norm(up))
+
+    out = {
+        "workload": (f"the metric's system: {mesh.nx}x{mesh.ny}x{mesh.nz} bricks, {mesh.n_dof} DOF, E lognormal "
+                     f"COV {cov}; training fields: {fields} fresh lognormal fields, seeds "
+                     f"{seed_base}..{seed_base + fields - 1}"),
+        "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
+        "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
+        "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
+        "learned": {"iters": rl["iters"], "solve_s": round(tl, 4), "fallback": rl["fallback"],
+                    "true_relres": rl["true_relres"], "u_rel_err_vs_fp64": err(ul)},
+        "learned_mixed": {"iters": rm["iters"], "solve_s": round(tm, 4), "fp64_refreshes": rm["outer"],
+                          "fallback": rm["fallback"], "true_relres": rm["true_relres"], "u_rel_err_vs_fp64": err(um)},
+        "iteration_ratio": round(rp["iters"] / max(rm["iters"], 1), 3),
+        "speedup_vs_plain": {"learned": round(tp / tl, 3), "learned_mixed": round(tp / tm, 3)},
+        "us_per_iteration": {"plain": round(tp / rp["iters"] * 1e6, 1), "learned": round(tl / rl["iters"] * 1e6, 1),
+                             "learned_mixed": round(tm / rm["iters"] * 1e6, 1)},
+        "network_stage_us": {k: round(v, 1) for k, v in stages.items()},
+    }
+    net.close()
     return out
 
 
@@ -308,6 +367,12 @@ def main():
         mixed_cov06 = bench_mixed_high_contrast(nx, ny, nz, args.seed + 1)
         log("mixed, COV 0.6", mixed_cov06)
 
+    learned_mixed = None
+    if not args.no_learned:
+        log("learned + mixed precision at the metric's size ...")
+        learned_mixed = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps)
+        log("learned_mixed", learned_mixed)
+
     learned = None
     if not args.no_learned:
         log("learned preconditioner ...")
@@ -353,6 +418,7 @@ def main():
         "spmv": spmv,
         "mixed": mixed,
         "mixed_cov06": mixed_cov06,
+        "learned_mixed_1m": learned_mixed,
         "learned": learned,
         "batch": batch,
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),

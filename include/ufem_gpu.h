@@ -47,7 +47,9 @@ enum {
   UFEM_SOLVER_PCG = 0,         /* Jacobi PCG, FP64                        */
   UFEM_SOLVER_BICGSTAB = 1,    /* Jacobi BiCGStab, FP64                   */
   UFEM_SOLVER_MIXED = 2,       /* FP32 inner PCG in FP64 refinement       */
-  UFEM_SOLVER_LEARNED = 3      /* flexible CG with the learned preconditioner */
+  UFEM_SOLVER_LEARNED = 3,     /* flexible CG with the learned preconditioner */
+  UFEM_SOLVER_LEARNED_MIXED = 4 /* the same with q = A p from the FP32 copy of A and
+                                  FP64 residual refreshes (inner_eps, inner_maxit) */
 };
 enum { UFEM_RULE_REFERENCE = 0, UFEM_RULE_RELATIVE = 1 };
 /* streamed: single-reduction PCG, two kernels per iteration (w = K u with
@@ -70,13 +72,14 @@ typedef struct {
   long inner_maxit;  /* mixed: ... and at least every this many iterations (0: 500) */
   void *net;         /* learned: network handle from ufem_net_*           */
   int warm_start;    /* nonzero: start from the current u                 */
-  int fcg_m;         /* learned: directions kept by flexible CG (1..8; 0: 4) */
+  int fcg_m;         /* learned: directions kept by flexible CG (1..8; 0: 1) */
 } ufem_solve_opts;
 
 typedef struct {
   int status;        /* AF_OK, AF_ERR (not converged), AF_ERR_ZER, ...    */
   long iters;        /* iterations (all inner iterations for mixed)       */
-  long outer;        /* mixed: FP64 residual refreshes                    */
+  long outer;        /* mixed, learned-mixed: FP64 residual refreshes;
+                        learned: iteration of the Jacobi fallback         */
   int fallback;      /* learned: 1 if Jacobi-PCG had to take over         */
   double relres;     /* last ||r||/||b|| the solver tested                */
   double true_relres;/* ||b - K u|| / ||b|| recomputed in FP64            */
@@ -191,12 +194,13 @@ int  ufem_net_train(ufem_net *N, long steps, int batch, double lr, unsigned long
                     double *ms);
 /* one convolution kernel on host data, for tests: y = conv(x) with
  * w [27][cout][cin] (mode 0 stride 1, 1 stride 2, 2 stride-2 transpose; flip:
- * taps reversed; x_f32: f32 activations, else bf16), dw = its weight
- * gradient for dy (modes 0, 1) */
+ * taps reversed; x_f32: 1 f32 activations, 2 f32 activations stored in
+ * the split format, 0 bf16), dw = its weight gradient for dy (modes 0, 1;
+ * x_f32 0 or 1) */
 int  ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin,
                          int cout, const float *x, const float *w, float *y, const float *dy, float *dw);
-/* average device time (us) of each of the 7 inference stages on the bound
- * system's grid: C0, C1, C2, gate, C2^T, C1^T, C0^T */
+/* average device time (us) of each of the 6 inference stages on the bound
+ * system's grid: C0, C1, C2 (with the gate product), C2^T, C1^T, C0^T */
 int  ufem_net_time_stages(ufem_net *N, int reps, double *us);
 /* average device time (us) of `reps` launches of one such convolution */
 int  ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin, int cout,

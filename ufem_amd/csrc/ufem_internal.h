@@ -72,7 +72,7 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *d_b, double *d_x, 
 
 /* learned preconditioner inside flexible CG (ufem_net.c) */
 long ufem_net_bound_n(const ufem_net *N);           /* DOFs of the bound system, 0 if none */
-float *ufem_net_input(ufem_net *N);                /* f32 [V][4], channel 3 stays 0       */
+unsigned short *ufem_net_input(ufem_net *N);       /* split format [V][3][4], channel 3 stays 0 */
 const float *ufem_net_mask(const ufem_net *N);     /* 0 at fixed DOFs, else 1             */
 const float *ufem_net_output(const ufem_net *N);   /* f32 [V][4]                          */
 int ufem_net_prepare(ufem_net *N, void *stream);   /* gates of the bound field            */

@@ -74,7 +74,8 @@ struct ufem_net {
   /* binding to a system */
   ufem_sys *sys;
   float *mask;
-  float *x0i, *yi, *gi[3], *ebi;
+  float *yi, *gi[3], *ebi;
+  unsigned short *x0s;            /* inference input, split format [V][3][4] */
   float *ebh;                     /* host ebar levels of the bound field */
   double *snap;
   long snap_cap, snap_n;
@@ -126,7 +127,8 @@ void ufem_net_destroy(ufem_net *N)
   ugk_free(N->pv); ugk_free(N->prr); ugk_free(N->pee);
   for (f = 0; f < N->nfield; f++) { ugk_free(N->fvals[f]); ugk_free(N->fdinv[f]); ugk_free(N->feb[f]); }
   free(N->fvals); free(N->fdinv); free(N->feb);
-  ugk_free(N->mask); ugk_free(N->x0i); ugk_free(N->yi); ugk_free(N->ebi); ugk_free(N->snap);
+  ugk_free(N->mask); ugk_free(N->yi); ugk_free(N->ebi); ugk_free(N->snap);
+  ugk_free(N->x0s);
   free(N->hpart); free(N->per); free(N->pfield); free(N->ebh);
   if (N->ev[0]) ugk_event_destroy(N->ev[0]);
   if (N->ev[1]) ugk_event_destroy(N->ev[1]);
@@ -208,8 +210,8 @@ ufem_net *ufem_net_create(int W, int H, int D, int batch)
   N->kz = (float *)dalloc(sizeof(float) * (size_t)n, &bad);
   N->dpart = (double *)dalloc(sizeof(double) * 1024 * (size_t)(nb > 4 ? nb : 4), &bad);
   N->hpart = (double *)malloc(sizeof(double) * 1024 * (size_t)(nb > 4 ? nb : 4));
-  N->x0i = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] * 4), &bad);
   N->yi = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] * 4), &bad);
+  N->x0s = (unsigned short *)dalloc(3 * sizeof(unsigned short) * (size_t)(N->V[0] * 4), &bad);
   N->ebi = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] + N->V[1] + N->V[2]), &bad);
   if (bad || !N->hpart) { net_fail("network buffers"); ufem_net_destroy(N); return NULL; }
   return N;
@@ -289,17 +291,20 @@ int ufem_net_init(ufem_net *N, unsigned long seed)
 }
 
 /* ---------------- forward / backward ------------------------------------- */
-static int conv(ufem_net *N, const float *x, int ldx, int cin, int l, int adjoint, float *y, int ldy, int cout,
-                int lev_in, int lev_out, const float *ga, const float *gb, int ldg, int nb, const int *skip,
-                void *stream)
+/* layer l (or its adjoint) from level lev_in to lev_out, input format xf
+ * (UGK_ACT_*), f32 output; epilogue (* emul) (+ ga gb) */
+static int conv_f(ufem_net *N, const void *x, int xf, int ldx, int cin, int l, int adjoint, float *y, int ldy,
+                  int cout, int lev_in, int lev_out, const float *emul, const float *ga, const float *gb, int ldg,
+                  int nb, const int *skip, void *stream)
 {
   ugk_conv_desc d;
   memset(&d, 0, sizeof(d));
-  d.x = x; d.ldx = ldx; d.cin = cin; d.x_f32 = 1;
+  d.x = x; d.ldx = ldx; d.cin = cin; d.x_f32 = xf;
   d.w = adjoint ? N->wt[l] : N->wf[l];
   d.cip = adjoint ? r16(CH[l]) : r16(CI[l]);
   d.cop = adjoint ? r16(CI[l]) : r16(CH[l]);
   d.y = y; d.ldy = ldy; d.cout = cout; d.y_f32 = 1;
+  d.emul = emul; d.ldm = ldg;
   d.ga = ga; d.ldga = ldg; d.gb = gb; d.ldgb = ldg;
   d.mode = l == 0 ? UGK_CONV_S1 : (adjoint ? UGK_CONV_T2 : UGK_CONV_S2);
   d.flip = (l == 0 && adjoint) ? 1 : 0;
@@ -308,6 +313,15 @@ static int conv(ufem_net *N, const float *x, int ldx, int cin, int l, int adjoin
   d.go = N->g[lev_out];
   d.skip = skip;
   return ugk_conv(&d, stream);
+}
+
+/* the same on f32 activations (training) */
+static int conv(ufem_net *N, const float *x, int ldx, int cin, int l, int adjoint, float *y, int ldy, int cout,
+                int lev_in, int lev_out, const float *ga, const float *gb, int ldg, int nb, const int *skip,
+                void *stream)
+{
+  return conv_f(N, x, UGK_ACT_F32, ldx, cin, l, adjoint, y, ldy, cout, lev_in, lev_out, NULL, ga, gb, ldg, nb, skip,
+                stream);
 }
 
 /* stage k of y = N(v) for nb samples with gates g[l] (nb x V_l x C_l):
@@ -334,6 +348,34 @@ static int net_forward(ufem_net *N, const float *v, float *const *g, float *y, i
   int k;
   for (k = 0; k < NET_STAGES; k++)
     if (net_stage(N, k, v, g, y, nb, skip, st)) return net_fail("network forward");
+  return AF_OK;
+}
+
+/* inference, one sample, input x0s in the split format (made once by the
+ * producer, so the first layer does no per-load split work), f32
+ * activations in the first sample of the training buffers:
+ * 0 h0 = C0 v, 1 h1 = C1 h0, 2 t2 = g2 C2 h1 (the gate product in C2's
+ * epilogue), 3 t1 = g1 h1 + C2^T t2, 4 t0 = g0 h0 + C1^T t1, 5 y = C0^T t0
+ * (f32 [V][4]); the same products as net_forward, bit for bit */
+#define INFER_STAGES 6
+static int infer_stage(ufem_net *N, int k, float *const *g, float *y, const int *skip, void *st)
+{
+  const int F = UGK_ACT_F32;
+  switch (k) {
+    case 0: return conv_f(N, N->x0s, UGK_ACT_SPLIT, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
+    case 1: return conv_f(N, N->h[0], F, 16, 16, 1, 0, N->h[1], 32, 32, 0, 1, NULL, NULL, NULL, 0, 1, skip, st);
+    case 2: return conv_f(N, N->h[1], F, 32, 32, 2, 0, N->t[2], 64, 64, 1, 2, g[2], NULL, NULL, 64, 1, skip, st);
+    case 3: return conv_f(N, N->t[2], F, 64, 64, 2, 1, N->t[1], 32, 32, 2, 1, NULL, g[1], N->h[1], 32, 1, skip, st);
+    case 4: return conv_f(N, N->t[1], F, 32, 32, 1, 1, N->t[0], 16, 16, 1, 0, NULL, g[0], N->h[0], 16, 1, skip, st);
+    default: return conv_f(N, N->t[0], F, 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
+  }
+}
+
+static int infer(ufem_net *N, float *const *g, float *y, const int *skip, void *st)
+{
+  int k;
+  for (k = 0; k < INFER_STAGES; k++)
+    if (infer_stage(N, k, g, y, skip, st)) return net_fail("network inference");
   return AF_OK;
 }
 
@@ -448,10 +490,18 @@ int ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, floa
   float *h;
   long i;
   int c;
+  int b;
   if (nb < 1 || nb > N->nbmax) { ufem_set_error("batch %d out of range", nb); return AF_ERR_VAL; }
-  if (upload_input(N, x, nb) || upload_levels(N, eb0, nb) || make_gates(N, N->eb, N->gt, nb, N->stream) ||
-      net_forward(N, N->x0, N->gt, N->y, nb, NULL, N->stream))
+  if (upload_input(N, x, nb) || upload_levels(N, eb0, nb) || make_gates(N, N->eb, N->gt, nb, N->stream))
     return AF_ERR;
+  /* the inference path (split activations), one sample at a time */
+  for (b = 0; b < nb; b++) {
+    float *g[3];
+    int l;
+    for (l = 0; l < 3; l++) g[l] = N->gt[l] + (long)b * N->V[l] * CH[l];
+    if (ugk_to_split(V, N->x0 + (long)b * V * 4, N->x0s, N->stream)) return net_fail("split input");
+    if (infer(N, g, N->y + (long)b * V * 4, NULL, N->stream)) return AF_ERR;
+  }
   h = (float *)malloc(sizeof(float) * (size_t)(nb * V * 4));
   if (!h) return AF_ERR_MEM;
   if (ugk_d2h(h, N->y, sizeof(float) * (size_t)(nb * V * 4), N->stream)) { free(h); return net_fail("download"); }
@@ -466,14 +516,15 @@ int ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, floa
 int ufem_net_backward(ufem_net *N, int nb, const float *x, const float *eb0, const float *dy, float *grad)
 {
   const long V = N->V[0];
-  float *h, *y;
+  float *h;
   long i;
   int c, rv;
   if (nb < 1 || nb > N->nbmax) { ufem_set_error("batch %d out of range", nb); return AF_ERR_VAL; }
-  y = (float *)malloc(sizeof(float) * (size_t)(nb * V * 3));
   h = (float *)calloc((size_t)(nb * V * 4), sizeof(float));
-  if (!y || !h) { free(y); free(h); return AF_ERR_MEM; }
-  rv = ufem_net_forward(N, nb, x, eb0, y);
+  if (!h) return AF_ERR_MEM;
+  /* the training forward (f32 activations kept for the backward) */
+  rv = (upload_input(N, x, nb) || upload_levels(N, eb0, nb) || make_gates(N, N->eb, N->gt, nb, N->stream) ||
+        net_forward(N, N->x0, N->gt, N->y, nb, NULL, N->stream)) ? AF_ERR : AF_OK;
   if (rv == AF_OK) {
     for (i = 0; i < nb * V; i++)
       for (c = 0; c < 3; c++) h[i * 4 + c] = dy[i * 3 + c];
@@ -482,7 +533,6 @@ int ufem_net_backward(ufem_net *N, int nb, const float *x, const float *eb0, con
   }
   if (rv == AF_OK) rv = net_backward(N, N->x0, N->dy, nb);
   if (rv == AF_OK && ugk_d2h(grad, N->gr, sizeof(float) * N->P, N->stream)) rv = net_fail("download");
-  free(y);
   free(h);
   return rv;
 }
@@ -807,7 +857,7 @@ out:
 
 /* ---------------- inference inside flexible CG (ufem_solve.c) ------------- */
 long ufem_net_bound_n(const ufem_net *N) { return N->sys ? N->sys->n : 0; }
-float *ufem_net_input(ufem_net *N) { return N->x0i; }
+unsigned short *ufem_net_input(ufem_net *N) { return N->x0s; }
 const float *ufem_net_mask(const ufem_net *N) { return N->mask; }
 const float *ufem_net_output(const ufem_net *N) { return N->yi; }
 
@@ -820,7 +870,7 @@ int ufem_net_prepare(ufem_net *N, void *stream)
 
 int ufem_net_infer(ufem_net *N, const int *skip, void *stream)
 {
-  return net_forward(N, N->x0i, N->gi, N->yi, 1, skip, stream);
+  return infer(N, N->gi, N->yi, skip, stream);
 }
 
 /* pair i as stored: v [V][4], rr [n], ee [n], e.r and its field index */
@@ -857,7 +907,7 @@ int ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, co
   long Vi, Vo, i;
   const int cip = r16(cin), cop = r16(cout), ldx = (cin + 3) / 4 * 4, ldy = (cout + 3) / 4 * 4;
   const size_t xb = (x_f32 ? sizeof(float) : sizeof(unsigned short)) * (size_t)ldx;
-  void *hx = NULL, *dx_ = NULL;
+  void *hx = NULL, *dx_ = NULL, *dxs = NULL;
   unsigned short *hw = NULL, *dw_ = NULL;
   float *dy_ = NULL, *part = NULL, *dgr = NULL, *hy = NULL, *hd = NULL, *ddy = NULL;
   void *st = NULL;
@@ -891,10 +941,14 @@ int ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, co
   if (!dx_ || !dw_ || !dy_) goto out;
   if (ugk_h2d(dx_, hx, xb * (size_t)(nb * Vi), st) || ugk_h2d(dw_, hw, sizeof(unsigned short) * 27 * cop * cip, st))
     goto out;
+  if (x_f32 == UGK_ACT_SPLIT) {     /* the f32 input, stored split */
+    dxs = ugk_malloc(3 * sizeof(unsigned short) * (size_t)(nb * Vi * ldx));
+    if (!dxs || ugk_to_split(nb * Vi * ldx / 4, (const float *)dx_, (unsigned short *)dxs, st)) goto out;
+  }
   {
     ugk_conv_desc d;
     memset(&d, 0, sizeof(d));
-    d.x = dx_; d.ldx = ldx; d.cin = ldx; d.x_f32 = x_f32;
+    d.x = dxs ? dxs : dx_; d.ldx = ldx; d.cin = ldx; d.x_f32 = x_f32;
     d.w = dw_; d.cip = cip; d.cop = cop;
     d.y = dy_; d.ldy = ldy; d.cout = cout; d.y_f32 = 1;
     d.mode = mode; d.flip = flip; d.nb = nb; d.gi = Gi; d.go = Go;
@@ -912,7 +966,7 @@ int ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, co
       for (c = 0; c < cout; c++) hd[i * ldy + c] = dy[i * cout + c];
     if (ugk_h2d(ddy, hd, sizeof(float) * (size_t)(nb * Vo * ldy), st)) goto out;
     memset(&d, 0, sizeof(d));
-    d.x = dx_; d.ldx = ldx; d.cin = cin; d.x_f32 = x_f32;
+    d.x = dx_; d.ldx = ldx; d.cin = cin; d.x_f32 = x_f32 ? 1 : 0;
     d.dy = ddy; d.ldd = ldy; d.cout = cout; d.dy_f32 = 1;
     d.cip = cip; d.cop = cop; d.mode = mode; d.nb = nb; d.gi = Gi; d.go = Go;
     ks = ugk_conv_dw_slices(&d);
@@ -927,7 +981,7 @@ int ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, co
 out:
   if (rv != AF_OK) net_fail("probe");
   free(hx); free(hw); free(hy); free(hd);
-  ugk_free(dx_); ugk_free(dw_); ugk_free(dy_); ugk_free(ddy); ugk_free(part); ugk_free(dgr);
+  ugk_free(dx_); ugk_free(dxs); ugk_free(dw_); ugk_free(dy_); ugk_free(ddy); ugk_free(part); ugk_free(dgr);
   if (st) ugk_stream_destroy(st);
   return rv;
 }
@@ -953,7 +1007,7 @@ int ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, con
   st = ugk_stream_create();
   e0 = ugk_event_create();
   e1 = ugk_event_create();
-  x = ugk_malloc((x_f32 ? 4 : 2) * (size_t)(nb * Vi * ldx));
+  x = ugk_malloc((x_f32 == UGK_ACT_SPLIT ? 6 : (x_f32 ? 4 : 2)) * (size_t)(nb * Vi * ldx));
   w = (unsigned short *)ugk_malloc(2 * (size_t)27 * cop * cip);
   y = (float *)ugk_malloc(4 * (size_t)(nb * Vo * ldy));
   if (!st || !e0 || !e1 || !x || !w || !y) goto out;
@@ -979,17 +1033,17 @@ out:
 }
 
 /* average device time (us) of each inference stage on the bound system's
- * grid (us[0..6], stages as in net_stage), `reps` launches each */
+ * grid (us[0..5], stages as in infer_stage), `reps` launches each */
 int ufem_net_time_stages(ufem_net *N, int reps, double *us)
 {
   int k, r;
   if (!N->sys) { ufem_set_error("bind the network to a system first"); return AF_ERR; }
   if (ufem_net_prepare(N, N->stream)) return AF_ERR;
-  for (k = 0; k < NET_STAGES; k++) {
-    if (net_stage(N, k, N->x0i, N->gi, N->yi, 1, NULL, N->stream)) return net_fail("stage");
+  for (k = 0; k < INFER_STAGES; k++) {
+    if (infer_stage(N, k, N->gi, N->yi, NULL, N->stream)) return net_fail("stage");
     if (ugk_event_record(N->ev[0], N->stream)) return net_fail("event");
     for (r = 0; r < reps; r++)
-      if (net_stage(N, k, N->x0i, N->gi, N->yi, 1, NULL, N->stream)) return net_fail("stage");
+      if (infer_stage(N, k, N->gi, N->yi, NULL, N->stream)) return net_fail("stage");
     if (ugk_event_record(N->ev[1], N->stream)) return net_fail("event");
     us[k] = 1000.0 * ugk_event_ms(N->ev[0], N->ev[1]) / reps;
   }

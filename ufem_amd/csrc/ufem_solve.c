@@ -336,10 +336,12 @@ static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_wor
 typedef struct {
   const ugk_mat *K;
   long n;
+  const double *b;
   double *x;
   ufem_work *w;
   ufem_net *net;
   int m;
+  int mixed;
 } fcg_ctx;
 
 static int fcg_step(void *c, long it, void *stream)
@@ -347,22 +349,24 @@ static int fcg_step(void *c, long it, void *stream)
   fcg_ctx *f = (fcg_ctx *)c;
   ufem_work *w = f->w;
   (void)it;
+  if (f->mixed && ugk_fcg_refresh(f->K, f->b, f->x, w->r, w->dinv, w->st, w->part, stream)) return -1;
   if (ugk_fcg_head(f->K, f->n, w->r, w->dinv, ufem_net_input(f->net), w->st, w->part, stream) ||
       ufem_net_infer(f->net, &w->st->halt, stream) ||
       ugk_fcg_combine(f->n, w->r, w->dinv, ufem_net_mask(f->net), ufem_net_output(f->net), w->fP, w->fQ, f->m,
                       w->z, w->st, w->part, stream) ||
       ugk_fcg_p(f->n, w->r, w->dinv, w->z, w->fP, f->m, w->st, w->part, stream) ||
-      ugk_fcg_q(f->K, f->n, w->fP, w->fQ, f->m, w->st, w->part, stream))
+      ugk_fcg_q(f->K, f->n, w->fP, w->fQ, f->m, f->mixed, w->st, w->part, stream))
     return -1;
   return ugk_fcg_update(f->K, f->n, f->x, w->r, w->fP, w->fQ, f->m, w->dinv, w->st, w->part, stream);
 }
 
-static int solve_learned(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
-                         const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
+                         const ufem_solve_opts *o, long maxit, int mixed, ufem_solve_info *info)
 {
+  const ugk_mat *K = Km;
   ufem_net *N = (ufem_net *)o->net;
-  const int m = o->fcg_m > 0 ? o->fcg_m : 4;
-  fcg_ctx c = {K, n, x, w, N, m};
+  const int m = o->fcg_m > 0 ? o->fcg_m : 1;   /* the network is a fixed SPD map */
+  fcg_ctx c = {K, n, b, x, w, N, m, mixed};
   ugk_state *h = w->st_host;
   int rv;
   if (!N || ufem_net_bound_n(N) != n) {
@@ -378,8 +382,22 @@ static int solve_learned(const ugk_mat *K, long n, const double *b, double *x, u
     w->fm = (w->fP && w->fQ) ? m : 0;
     if (!w->fm) return gpu_fail("direction rings");
   }
+  if (mixed) {
+    if (!Km->d_vals32) {
+      Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
+      if (!Km->d_vals32) return gpu_fail("fp32 matrix");
+    }
+    if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+  }
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
+  if (mixed) {
+    h->delta = o->inner_eps > 0 ? o->inner_eps : 0.1;
+    h->max_gap = o->inner_maxit > 0 ? o->inner_maxit : 500;
+    h->ru_req = -1;
+    h->ru_at = 0;
+    if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
+  }
   if (ufem_net_prepare(N, stream)) return AF_ERR;
   if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
   rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, fcg_step, &c, 1);
@@ -392,7 +410,7 @@ static int solve_learned(const ugk_mat *K, long n, const double *b, double *x, u
   }
   info->iters = h->iter;
   info->fallback = h->fb;
-  info->outer = h->fb ? h->fb_iter : 0;
+  info->outer = mixed ? h->n_ru : (h->fb ? h->fb_iter : 0);
   info->relres = sqrt(h->rr / h->bb);
   info->status = status_of(h->done);
   return info->status;
@@ -409,7 +427,10 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem
     case UFEM_SOLVER_PCG: rv = solve_pcg(K, n, b, x, w, stream, o, maxit, info); break;
     case UFEM_SOLVER_BICGSTAB: rv = solve_bicgs(K, n, b, x, w, stream, o, maxit, info); break;
     case UFEM_SOLVER_MIXED: rv = solve_mixed((ugk_mat *)K, n, b, x, w, stream, o, maxit, info); break;
-    case UFEM_SOLVER_LEARNED: rv = solve_learned(K, n, b, x, w, stream, o, maxit, info); break;
+    case UFEM_SOLVER_LEARNED: rv = solve_learned((ugk_mat *)K, n, b, x, w, stream, o, maxit, 0, info); break;
+    case UFEM_SOLVER_LEARNED_MIXED:
+      rv = solve_learned((ugk_mat *)K, n, b, x, w, stream, o, maxit, 1, info);
+      break;
     default: ufem_set_error("unknown solver %d", o->solver); return AF_ERR_VAL;
   }
   if (ugk_event_record(ev1, stream)) return gpu_fail("event");

@@ -66,7 +66,7 @@ typedef struct {
   int done;           /* 0 running, 1 converged, 2 breakdown, 3 maxit   */
   int err;            /* kernel-detected error (zero diagonal, detJ<=0)  */
   int pad;
-  double net_s;       /* flexible CG: rms of dinv r fed to the network   */
+  double net_s;       /* flexible CG: rms of sqrt(dinv) r (network input scale) */
   double pqs[8];      /* flexible CG: (p, A p) of the kept directions     */
   long fb_iter;       /* flexible CG: iteration of the Jacobi fallback    */
   int fb;             /* flexible CG: 1 once the network is switched off  */
@@ -242,22 +242,30 @@ long ugk_wall_clock_khz(void);
  * ring slots (iteration mod m) of d_P and d_Q (m vectors of n each).  Per
  * iteration: ugk_fcg_head (convergence test on the previous update's
  * partials; network input X0[v][c] = sqrt(dinv) r / s, s = rms(sqrt(dinv) r),
- * for c < 3, channel 3 left alone), the network (ugk_conv and friends,
+ * for c < 3 in the split format [V][3 parts][4], channel 3 left alone), the network (ugk_conv and friends,
  * skipped through st->halt), ugk_fcg_combine (z = dinv r + s mask sqrt(dinv)
  * out and the dots with the kept directions), ugk_fcg_p (coefficients, descent check and
  * Jacobi fallback, new direction), ugk_fcg_q (q = A p), ugk_fcg_update.
- * out: f32 [n/3][4]. */
+ * out: f32 [n/3][4].
+ * Mixed form (state max_gap > 0, delta, ru_req = -1, ru_at = 0): ugk_fcg_q
+ * with f32 = 1 multiplies by the FP32 copy of A; ugk_fcg_refresh, launched
+ * before each head, recomputes r = b - A x in FP64 for the iterations the
+ * head asked for (the FP32-based residual has dropped by delta since the
+ * last refresh, max_gap iterations have passed, or it passed the test: a
+ * convergence is only accepted on a recomputed residual). */
 int ugk_fcg_start(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r,
                   const double *d_dinv, double *d_part, void *stream);
-int ugk_fcg_head(const ugk_mat *K, long n, const double *d_r, const double *d_dinv, float *d_x0,
+int ugk_fcg_head(const ugk_mat *K, long n, const double *d_r, const double *d_dinv, unsigned short *d_x0,
                  ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_combine(long n, const double *d_r, const double *d_dinv, const float *d_mask,
                     const float *d_out, const double *d_P, const double *d_Q, int m, double *d_z,
                     ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_p(long n, const double *d_r, const double *d_dinv, const double *d_z, double *d_P, int m,
               ugk_state *d_st, double *d_part, void *stream);
-int ugk_fcg_q(const ugk_mat *K, long n, const double *d_P, double *d_Q, int m, ugk_state *d_st,
+int ugk_fcg_q(const ugk_mat *K, long n, const double *d_P, double *d_Q, int m, int f32, ugk_state *d_st,
               double *d_part, void *stream);
+int ugk_fcg_refresh(const ugk_mat *K, const double *d_b, const double *d_x, double *d_r, const double *d_dinv,
+                    ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_update(const ugk_mat *K, long n, double *d_x, double *d_r, const double *d_P, const double *d_Q,
                    int m, const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
 
@@ -272,11 +280,19 @@ enum { UGK_CONV_S1 = 0,   /* y[v] = sum_t W[t] x[v + off_t]                 */
        UGK_CONV_S2 = 1,   /* y[c] = sum_t W[t] x[2 c + off_t]  (coarse out)  */
        UGK_CONV_T2 = 2 }; /* y[f] = sum_t W[t] x[(f - off_t) / 2], even only */
 
+/* Activation formats (x_f32 below): 0 bf16, 1 f32, 2 "split" -- an f32
+ * tensor kept as its exact three-way bf16 split, each group of 4 channels
+ * stored as hi[4], mid[4], lo[4] (24 bytes; f32 = (hi + mid) + lo exactly),
+ * so the element (v, c) of a tensor of stride ld starts at bf16 offset
+ * 3 (v ld + c) for c % 4 == 0. */
+enum { UGK_ACT_BF16 = 0, UGK_ACT_F32 = 1, UGK_ACT_SPLIT = 2 };
+
 /* 3x3x3 taps t = 9 (dz+1) + 3 (dy+1) + (dx+1), zero padding, no bias.
- * Activations x are bf16, or f32 (x_f32) fed to the bf16 matrix cores as an
- * exact three-way bf16 split.  Epilogue per output element:
- * v = acc (* emul) (+ ga * gb) (f32 fields, optional), stored as bf16, or f32
- * if y_f32. */
+ * Activations x are bf16, f32 (fed to the bf16 matrix cores as an exact
+ * three-way bf16 split made at each load) or split (the same products with
+ * the split made once by the producer).  Epilogue per output element:
+ * v = acc (* emul) (+ ga * gb) (f32 fields, optional), stored as bf16, or
+ * f32 if y_f32. */
 typedef struct {
   const void *x; int ldx, cin, x_f32;      /* cin % 4 == 0, cin <= cip        */
   const unsigned short *w; int cip, cop;   /* [27][cop][cip], zero padded, %16 */
@@ -303,6 +319,8 @@ typedef struct {
 int ugk_conv_dw(const ugk_dw_desc *d, float *d_part, int ks, float *d_grad, int accumulate, void *stream);
 int ugk_conv_dw_slices(const ugk_dw_desc *d);   /* ks the call above will use */
 
+/* f32 [n4][4] -> split format (n4 groups of 4 channels) */
+int ugk_to_split(long n4, const float *d_x, unsigned short *d_y, void *stream);
 /* out = a * b elementwise (f32) */
 int ugk_mul(long n, const float *d_a, const float *d_b, float *d_out, const int *d_skip, void *stream);
 /* gates g[v][c] = exp(b[c] - (1 + a[c]) eb[v]) for nvox voxels, C channels */

@@ -1634,15 +1634,28 @@ __global__ __launch_bounds__(WG) void k_fcg_start(SellView M, const double *__re
 }
 
 __global__ __launch_bounds__(WG) void k_fcg_head(long n, const double *__restrict__ r,
-                                                 const double *__restrict__ dinv, float *x0,
+                                                 const double *__restrict__ dinv, unsigned short *x0,
                                                  ugk_state *st, const double *part, unsigned G0, unsigned GB)
 {
   if (st->done) return;
   const long k = st->iter;
+  // mixed form (max_gap > 0): the residual of this iteration is the FP64
+  // b - K x when the refresh kernel ran for it (its partials, G0 blocks)
+  const bool mixed = st->max_gap > 0;
+  const bool fresh = k == 0 || (mixed && st->ru_at == k);
   double t[4];
-  reduce_prev<4>(PART_B2(part, k & 1), k == 0 ? G0 : GB, t);
+  reduce_prev<4>(PART_B2(part, k & 1), fresh ? G0 : GB, t);
   const double bb = (k == 0) ? t[3] : st->bb;
-  const int done = pcg_test(st, k, t[1], t[2], bb, t[0]);
+  int done = pcg_test(st, k, t[1], t[2], bb, t[0]);
+  long req = -1;
+  if (mixed && !fresh) {
+    // convergence is only accepted on a recomputed residual
+    if (done == 1) {
+      done = 0;
+      req = k + 1;
+    } else if (done == 0 && (t[1] < st->delta * st->delta * st->rr_true || k - st->ru_at >= st->max_gap))
+      req = k + 1;
+  }
   const int fb = st->fb;
   const double sc = sqrt(t[0] / (double)n);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -1651,15 +1664,35 @@ __global__ __launch_bounds__(WG) void k_fcg_head(long n, const double *__restric
     st->rr = t[1];
     st->xx = t[2];
     if (k == 0) st->bb = bb;
+    if (mixed && fresh) st->rr_true = t[1];
+    if (req >= 0) st->ru_req = req;
     if (done) st->done = done;
     st->halt = (done || fb) ? 1 : 0;
     st->net_s = sc;
   }
   if (done || fb) return;
   const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
-    const long v = i / 3;
-    x0[v * 4 + (i - 3 * v)] = (float)(sqrt(dinv[i]) * r[i] * inv);
+  // split format, one voxel (3 DOFs) per thread: hi[4], mid[4], lo[4] of its
+  // exact truncation split, channel 3 zero
+  typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+  for (long v = (long)blockIdx.x * WG + threadIdx.x; v < n / 3; v += (long)gridDim.x * WG) {
+    us4 part[3] = {(us4){0, 0, 0, 0}, (us4){0, 0, 0, 0}, (us4){0, 0, 0, 0}};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const long i = 3 * v + c;
+      const float f = (float)(sqrt(dinv[i]) * r[i] * inv);
+      const unsigned u = __float_as_uint(f), h = u & 0xffff0000u;
+      const float r1 = f - __uint_as_float(h);
+      const unsigned m = __float_as_uint(r1) & 0xffff0000u;
+      const float r2 = r1 - __uint_as_float(m);
+      part[0][c] = (unsigned short)(h >> 16);
+      part[1][c] = (unsigned short)(m >> 16);
+      part[2][c] = (unsigned short)(__float_as_uint(r2) >> 16);
+    }
+    us4 *o = (us4 *)(x0 + v * 12);
+    o[0] = part[0];
+    o[1] = part[1];
+    o[2] = part[2];
   }
 }
 
@@ -1755,9 +1788,10 @@ __global__ __launch_bounds__(WG) void k_fcg_p(long n, const double *__restrict__
   }
 }
 
-// q = A p for the direction of this iteration (ring slot), partial (p, q)
-template <int BS>
-__global__ __launch_bounds__(WG) void k_fcg_q(SellView M, const double *__restrict__ vals,
+// q = A p for the direction of this iteration (ring slot), partial (p, q);
+// T = float: the FP32 copy of A (mixed form), vectors and sums stay FP64
+template <int BS, typename T>
+__global__ __launch_bounds__(WG) void k_fcg_q(SellView M, const T *__restrict__ vals,
                                               const double *__restrict__ P, double *__restrict__ Q, long n,
                                               int m, const ugk_state *st, double *part)
 {
@@ -1768,7 +1802,7 @@ __global__ __launch_bounds__(WG) void k_fcg_q(SellView M, const double *__restri
   double v[1] = {0.0};
   double acc[BS];
   long row;
-  if (sell_row_split<BS, double, double>(M, vals, p, acc, row)) {
+  if (sell_row_split<BS, T, double>(M, vals, p, acc, row)) {
 #pragma unroll
     for (int r = 0; r < BS; r++) {
       const long i = row * BS + r;
@@ -1777,6 +1811,39 @@ __global__ __launch_bounds__(WG) void k_fcg_q(SellView M, const double *__restri
     }
   }
   block_partials<1>(v, PART_A(part));
+}
+
+// mixed form: when the head kernel of the previous iteration asked for it,
+// r = b - A x in FP64 and the head's partials of this iteration taken from it
+template <int BS>
+__global__ __launch_bounds__(WG) void k_fcg_refresh(SellView M, const double *__restrict__ vals,
+                                                    const double *__restrict__ b,
+                                                    const double *__restrict__ x, double *r,
+                                                    const double *__restrict__ dinv, ugk_state *st,
+                                                    double *part)
+{
+  if (st->done || st->ru_req != st->iter) return;   // the same for every block
+  const long k = st->iter;
+  double v[4] = {0, 0, 0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      const long i = row * BS + c;
+      const double ri = b[i] - acc[c];
+      r[i] = ri;
+      v[0] += dinv[i] * ri * ri;
+      v[1] += ri * ri;
+      v[2] += x[i] * x[i];
+      v[3] += b[i] * b[i];
+    }
+  }
+  block_partials<4>(v, PART_B2(part, k & 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->ru_at = k;
+    st->n_ru += 1;
+  }
 }
 
 __global__ __launch_bounds__(WG) void k_fcg_update(long n, double *__restrict__ x, double *__restrict__ r,
@@ -2392,7 +2459,7 @@ int ugk_fcg_start(const ugk_mat *K, long n, const double *b, const double *x, do
   return 0;
 }
 
-int ugk_fcg_head(const ugk_mat *K, long n, const double *r, const double *dinv, float *x0,
+int ugk_fcg_head(const ugk_mat *K, long n, const double *r, const double *dinv, unsigned short *x0,
                  ugk_state *st, double *part, void *s)
 {
   k_fcg_head<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, x0, st, part, spmv_blocks(K), vec_blocks(n));
@@ -2417,11 +2484,27 @@ int ugk_fcg_p(long n, const double *r, const double *dinv, const double *z, doub
   return 0;
 }
 
-int ugk_fcg_q(const ugk_mat *K, long n, const double *P, double *Q, int m, ugk_state *st, double *part, void *s)
+int ugk_fcg_q(const ugk_mat *K, long n, const double *P, double *Q, int m, int f32, ugk_state *st,
+              double *part, void *s)
 {
-  if (K->bs == 3) k_fcg_q<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, P, Q, n, m, st, part);
-  else k_fcg_q<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, P, Q, n, m, st, part);
+  if (f32 && !K->d_vals32) { snprintf(g_err, sizeof(g_err), "FCG: no FP32 copy of the matrix"); return -1; }
+  if (f32) {
+    if (K->bs == 3) k_fcg_q<3, float><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, P, Q, n, m, st, part);
+    else k_fcg_q<1, float><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, P, Q, n, m, st, part);
+  } else {
+    if (K->bs == 3) k_fcg_q<3, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, P, Q, n, m, st, part);
+    else k_fcg_q<1, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, P, Q, n, m, st, part);
+  }
   CK_LAUNCH("k_fcg_q");
+  return 0;
+}
+
+int ugk_fcg_refresh(const ugk_mat *K, const double *b, const double *x, double *r, const double *dinv,
+                    ugk_state *st, double *part, void *s)
+{
+  if (K->bs == 3) k_fcg_refresh<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, st, part);
+  else k_fcg_refresh<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, st, part);
+  CK_LAUNCH("k_fcg_refresh");
   return 0;
 }
 

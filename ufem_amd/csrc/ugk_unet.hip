@@ -71,21 +71,36 @@ __device__ __forceinline__ long tap_src(int z, int y, int x, int dz, int dy, int
 // convolution: each wave computes NCO x 16 output channels for NV x 16
 // voxels (A = weights [co][ci], B = activations [ci][voxel]; a lane of the
 // accumulator holds 4 consecutive channels of one voxel -> 8/16-byte stores).
-// XF: f32 activations, each split exactly into three bf16 parts
+// XF = 1: f32 activations, each split exactly into three bf16 parts
 // (x = hi + mid + lo to 24 bits) that are multiplied by the same bf16
 // weights, so the product is the f32 activation times the bf16 weight.
+// XF = 2: the same three parts, stored split by the producer (no per-load
+// split work; the products are bit for bit those of XF = 1).
 // --------------------------------------------------------------------------
 // B fragment of 4 channels: f32 loaded raw (split after the load arrives),
-// or bf16 as is
+// the three stored parts, or bf16 as is
+struct Split4 { bf16x4 p[3]; };
 template <int XF> struct Frag;
+template <> struct Frag<2> { typedef Split4 T; };
 template <> struct Frag<1> { typedef f32x4 T; };
 template <> struct Frag<0> { typedef bf16x4 T; };
 
+// off: element offset v * ld + c of channel c (c % 4 == 0) of voxel v
 template <int XF>
 __device__ __forceinline__ typename Frag<XF>::T load_raw(const void *x, long off)
 {
-  if constexpr (XF) return *(const f32x4 *)((const float *)x + off);
-  else return *(const bf16x4 *)((const bf16 *)x + off);
+  if constexpr (XF == 2) {
+    const bf16 *q = (const bf16 *)x + off * 3;
+    Split4 r;
+    r.p[0] = *(const bf16x4 *)q;
+    r.p[1] = *(const bf16x4 *)(q + 4);
+    r.p[2] = *(const bf16x4 *)(q + 8);
+    return r;
+  } else if constexpr (XF == 1) {
+    return *(const f32x4 *)((const float *)x + off);
+  } else {
+    return *(const bf16x4 *)((const bf16 *)x + off);
+  }
 }
 
 // exact three-way bf16 split of 4 f32 values by truncation: hi keeps the
@@ -107,17 +122,50 @@ __device__ __forceinline__ void split3(const f32x4 &x, bool ok, bf16x4 (&b)[3])
   }
 }
 
-// epilogue for 4 consecutive channels co.. of output voxel g
-__device__ __forceinline__ void store_out(const ugk_conv_desc &d, long g, int co, const f32x4 &acc)
+// epilogue operands (emul, ga, gb) of 4 consecutive channels co.. of output
+// voxel g: 16-byte loads where the layout allows, issued by the callers
+// ahead of their products so that the latency overlaps
+struct Epi { f32x4 em, ga, gb; };
+
+__device__ __forceinline__ bool al16(const float *p, int ld) { return !(((unsigned long)p & 15) | (ld & 3)); }
+
+__device__ __forceinline__ Epi epi_load(const ugk_conv_desc &d, long g, int co, bool valid)
+{
+  Epi e;
+  e.em = (f32x4){1.f, 1.f, 1.f, 1.f};
+  e.ga = (f32x4){0.f, 0.f, 0.f, 0.f};
+  e.gb = e.ga;
+  const int nj = d.cout - co < 4 ? d.cout - co : 4;
+  if (!valid || nj <= 0) return e;
+  if (d.emul) {
+    if (nj == 4 && al16(d.emul, d.ldm)) e.em = *(const f32x4 *)(d.emul + g * d.ldm + co);
+    else for (int j = 0; j < nj; j++) e.em[j] = d.emul[g * d.ldm + co + j];
+  }
+  if (d.ga) {
+    if (nj == 4 && al16(d.ga, d.ldga) && al16(d.gb, d.ldgb)) {
+      e.ga = *(const f32x4 *)(d.ga + g * d.ldga + co);
+      e.gb = *(const f32x4 *)(d.gb + g * d.ldgb + co);
+    } else {
+      for (int j = 0; j < nj; j++) {
+        e.ga[j] = d.ga[g * d.ldga + co + j];
+        e.gb[j] = d.gb[g * d.ldgb + co + j];
+      }
+    }
+  }
+  return e;
+}
+
+// epilogue with operands loaded by epi_load: v = acc (* emul) (+ ga gb)
+__device__ __forceinline__ void store_pre(const ugk_conv_desc &d, long g, int co, const f32x4 &acc, const Epi &e)
 {
   const int nj = d.cout - co < 4 ? d.cout - co : 4;
   float v[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) v[j] = acc[j];
   if (d.emul)
-    for (int j = 0; j < nj; j++) v[j] *= d.emul[g * d.ldm + co + j];
+    for (int j = 0; j < nj; j++) v[j] *= e.em[j];
   if (d.ga)
-    for (int j = 0; j < nj; j++) v[j] += d.ga[g * d.ldga + co + j] * d.gb[g * d.ldgb + co + j];
+    for (int j = 0; j < nj; j++) v[j] += e.ga[j] * e.gb[j];
   if (d.y_f32) {
     float *y = (float *)d.y + g * d.ldy + co;
     if (nj == 4) *(f32x4 *)y = (f32x4){v[0], v[1], v[2], v[3]};
@@ -129,10 +177,20 @@ __device__ __forceinline__ void store_out(const ugk_conv_desc &d, long g, int co
   }
 }
 
+__device__ __forceinline__ void store_out(const ugk_conv_desc &d, long g, int co, const f32x4 &acc)
+{
+  store_pre(d, g, co, acc, epi_load(d, g, co, true));
+}
+
 template <int XF>
 __device__ __forceinline__ void mfma_frag(const bf16x4 &a, const typename Frag<XF>::T &raw, bool ok, f32x4 &acc)
 {
-  if constexpr (XF) {
+  if constexpr (XF == 2) {
+    const bf16x4 z = (bf16x4){0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ok ? raw.p[2] : z, acc, 0, 0, 0);   // smallest part first
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ok ? raw.p[1] : z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ok ? raw.p[0] : z, acc, 0, 0, 0);
+  } else if constexpr (XF == 1) {
     bf16x4 bx[3];
     split3(raw, ok, bx);
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx[2], acc, 0, 0, 0);   // smallest part first
@@ -149,7 +207,12 @@ __device__ __forceinline__ void mfma_frag(const bf16x4 &a, const typename Frag<X
 // without branches (clamped addresses, zeroed fragments), so the gathers of
 // many taps are in flight together.  PK: the input has 4 channels, and the
 // four lane groups of the MFMA's K dimension take four taps at once.
-template <int NV, int MODE, int XF, int PK>
+// KS = 1: the (16-channel chunk, dz plane) units of the sum in a rolled
+// loop (fewer registers, more waves resident); KS = 4 (small grids): the
+// four waves of a block share one tile and take every fourth unit, which
+// the block adds in wave order in LDS -- four times the waves, a quarter of
+// the chain.  KS = 0: all three planes of a chunk unrolled.
+template <int NV, int MODE, int XF, int PK, int KS>
 __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
 {
   if (d.skip && *d.skip) return;
@@ -158,8 +221,8 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
   const int mg = blockIdx.y;
   const long Vo = vox(d.go), Vi = vox(d.gi);
   const long total = Vo * d.nb;
-  const long base = ((long)blockIdx.x * 4 + wave) * (16 * NV);
-  if (base >= total) return;               // no block-wide barriers below
+  const long base = (KS == 4 ? (long)blockIdx.x : ((long)blockIdx.x * 4 + wave)) * (16 * NV);
+  if (base >= total) return;               // uniform over the block when KS = 4
   const int HWo = d.go.H * d.go.W;
   int vz[NV], vy[NV], vx[NV];
   long vb[NV];
@@ -183,7 +246,50 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
   const bf16 *wrow = d.w + (long)(mg * 16 + col) * d.cip;
   const long wstep = (long)d.cop * d.cip;
 
-  if constexpr (PK) {
+  Epi ep[NV];
+  if constexpr (KS == 4) {
+    const int co = mg * 16 + 4 * kq;
+#pragma unroll
+    for (int n = 0; n < NV; n++) ep[n] = epi_load(d, base + n * 16 + col, co, wave == 0 && vok[n] && co < d.cout);
+  }
+  if constexpr (KS >= 1) {
+    const int U = (d.cin + 15) / 16 * 3;
+    for (int u = KS == 4 ? wave : 0; u < U; u += KS) {
+      const int c = (u / 3) * 16 + 4 * kq, pz = u % 3;
+      const bool cok = c < d.cin;
+      const int cs = cok ? c : 0;
+      typename Frag<XF>::T raw[9][NV];
+      bool ok[9][NV];
+      bf16x4 a[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int t = 9 * pz + k;
+        const int dz = pz - 1, dy = k / 3 - 1, dx = k % 3 - 1;
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+          const long sidx = tap_src<MODE>(vz[n], vy[n], vx[n], dz, dy, dx, d.gi);
+          ok[k][n] = vok[n] && cok && sidx >= 0;
+          raw[k][n] = load_raw<XF>(d.x, (ok[k][n] ? vb[n] + sidx : 0) * d.ldx + cs);
+        }
+        a[k] = *(const bf16x4 *)(wrow + (d.flip ? 26 - t : t) * wstep + c);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < 9; k++)
+#pragma unroll
+        for (int n = 0; n < NV; n++) mfma_frag<XF>(a[k], raw[k][n], ok[k][n], acc[n]);
+    }
+    if constexpr (KS == 4) {
+      __shared__ f32x4 red[4][NV][64];
+#pragma unroll
+      for (int n = 0; n < NV; n++) red[wave][n][lane] = acc[n];
+      __syncthreads();
+      if (wave) return;
+#pragma unroll
+      for (int n = 0; n < NV; n++)
+        acc[n] = ((red[0][n][lane] + red[1][n][lane]) + red[2][n][lane]) + red[3][n][lane];
+    }
+  } else if constexpr (PK) {
     // tap groups of 4: lane group kq takes tap 4 q + kq, channels 0..3; all
     // 7 groups' fragments are loaded before the first product
     typename Frag<XF>::T raw[7][NV];
@@ -244,8 +350,11 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
   const int co = mg * 16 + 4 * kq;
   if (co >= d.cout) return;
 #pragma unroll
-  for (int n = 0; n < NV; n++)
-    if (vok[n]) store_out(d, base + n * 16 + col, co, acc[n]);
+  for (int n = 0; n < NV; n++) {
+    if (!vok[n]) continue;
+    if constexpr (KS == 4) store_pre(d, base + n * 16 + col, co, acc[n], ep[n]);
+    else store_out(d, base + n * 16 + col, co, acc[n]);
+  }
 }
 
 // Stride-2 transpose (coarse input gi -> fine output go): y[f] = sum_t W[t]
@@ -254,7 +363,38 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
 // has the same 1, 2, 4 or 8 live taps: per axis, p = 0 takes off 0 at c, and
 // p = 1 takes off +1 at c and off -1 at c + 1.  Work items are
 // [nb][8][Vc rounded up to 16] (c over the coarse grid, rows of 16 never
-// straddle two classes).
+// straddle two classes).  The class is a compile-time case, so the gathers
+// of all live taps of a 16-channel chunk are in flight before its products.
+template <int PZ, int PY, int PX, int XF>
+__device__ __forceinline__ void t2_class(const ugk_conv_desc &d, const bf16 *wrow, long wstep, long xb, int cz,
+                                         int cy, int cx, bool fok, int kq, f32x4 &acc)
+{
+  constexpr int NY = PY + 1, NX = PX + 1, NT = (PZ + 1) * NY * NX;
+  for (int c0 = 0; c0 < d.cin; c0 += 16) {
+    const int c = c0 + 4 * kq;
+    const bool cok = c < d.cin;
+    const int cs = cok ? c : 0;
+    typename Frag<XF>::T raw[NT];
+    bool ok[NT];
+    bf16x4 a[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+      const int iz = j / (NY * NX), iy = (j / NX) % NY, ix = j % NX;
+      // offset +1 reads c, offset -1 reads c + 1; parity 0 has offset 0
+      const int dz = PZ ? 1 - 2 * iz : 0, dy = PY ? 1 - 2 * iy : 0, dx = PX ? 1 - 2 * ix : 0;
+      const int sz = cz + iz, sy = cy + iy, sx = cx + ix;
+      ok[j] = fok && cok && sz < d.gi.D && sy < d.gi.H && sx < d.gi.W;
+      const long src = ok[j] ? xb + ((long)sz * d.gi.H + sy) * d.gi.W + sx : 0;
+      raw[j] = load_raw<XF>(d.x, src * d.ldx + cs);
+      const int t = 9 * (dz + 1) + 3 * (dy + 1) + (dx + 1);
+      a[j] = *(const bf16x4 *)(wrow + t * wstep + c);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // the chunk's gathers ahead of its products
+#pragma unroll
+    for (int j = 0; j < NT; j++) mfma_frag<XF>(a[j], raw[j], ok[j], acc);
+  }
+}
+
 template <int XF>
 __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
 {
@@ -282,27 +422,22 @@ __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
   f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
   const bf16 *wrow = d.w + (long)(mg * 16 + col) * d.cip;
   const long wstep = (long)d.cop * d.cip;
-  for (int c0 = 0; c0 < d.cin; c0 += 16) {
-    const int c = c0 + 4 * kq;
-    const bool cok = c < d.cin;
-    const int cs = cok ? c : 0;
-    for (int iz = 0; iz <= pz; iz++)
-      for (int iy = 0; iy <= py; iy++)
-        for (int ix = 0; ix <= px; ix++) {
-          // offset +1 reads c, offset -1 reads c + 1; parity 0 has offset 0
-          const int dz = pz ? 1 - 2 * iz : 0, dy = py ? 1 - 2 * iy : 0, dx = px ? 1 - 2 * ix : 0;
-          const int sz = cz + iz, sy = cy + iy, sx = cx + ix;
-          const bool ok = fok && cok && sz < d.gi.D && sy < d.gi.H && sx < d.gi.W;
-          const long src = ok ? b * Vc + ((long)sz * d.gi.H + sy) * d.gi.W + sx : 0;
-          const typename Frag<XF>::T raw = load_raw<XF>(d.x, src * d.ldx + cs);
-          const int t = 9 * (dz + 1) + 3 * (dy + 1) + (dx + 1);
-          const bf16x4 a = *(const bf16x4 *)(wrow + t * wstep + c);
-          mfma_frag<XF>(a, raw, ok, acc);
-        }
-  }
+  const long xb = b * Vc;
   const int co = mg * 16 + 4 * kq;
+  const long gout = b * Vf + ((long)fz * d.go.H + fy) * d.go.W + fx;
+  const Epi ep = epi_load(d, gout, co, fok && co < d.cout);
+  switch (p) {                              // wave-uniform
+    case 0: t2_class<0, 0, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    case 1: t2_class<0, 0, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    case 2: t2_class<0, 1, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    case 3: t2_class<0, 1, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    case 4: t2_class<1, 0, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    case 5: t2_class<1, 0, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    case 6: t2_class<1, 1, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    default: t2_class<1, 1, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+  }
   if (co >= d.cout || !fok) return;
-  store_out(d, b * Vf + ((long)fz * d.go.H + fy) * d.go.W + fx, co, acc);
+  store_pre(d, gout, co, acc, ep);
 }
 
 // Stride-1 convolution to at most 4 output channels (the network's last
@@ -313,7 +448,7 @@ __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
 // channels are widened to f32 in LDS.  A 16-row matrix-core tile would
 // leave 12 of its rows idle here.
 #define SO_MAX_CIN 64
-template <int C16>
+template <int C16, int NCO>
 __global__ __launch_bounds__(256) void k_conv_small_out(ugk_conv_desc d)
 {
   __shared__ f32x4 sw[27][4][4 * C16];          // [tap][co][ci / 4]
@@ -351,13 +486,13 @@ __global__ __launch_bounds__(256) void k_conv_small_out(ugk_conv_desc d)
 #pragma unroll
       for (int t = 0; t < 9; t++)
 #pragma unroll
-        for (int co = 0; co < 4; co++) {
+        for (int co = 0; co < NCO; co++) {
           const f32x4 w4 = sw[9 * (dz + 1) + t][co][4 * j + cq];
           acc[co] += w4[0] * xv[t][0] + w4[1] * xv[t][1] + w4[2] * xv[t][2] + w4[3] * xv[t][3];
         }
     }
 #pragma unroll
-    for (int co = 0; co < 4; co++) {
+    for (int co = 0; co < NCO; co++) {
       acc[co] += __shfl_xor(acc[co], 1);
       acc[co] += __shfl_xor(acc[co], 2);
     }
@@ -365,11 +500,11 @@ __global__ __launch_bounds__(256) void k_conv_small_out(ugk_conv_desc d)
   }
 }
 
-template <int NV, int XF, int PK>
+template <int NV, int XF, int PK, int KS = 0>
 static void launch_conv_nv(const ugk_conv_desc &d, dim3 grid, hipStream_t s)
 {
-  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF, PK><<<grid, 256, 0, s>>>(d);
-  else k_conv<NV, UGK_CONV_S2, XF, PK><<<grid, 256, 0, s>>>(d);
+  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF, PK, KS><<<grid, 256, 0, s>>>(d);
+  else k_conv<NV, UGK_CONV_S2, XF, PK, KS><<<grid, 256, 0, s>>>(d);
 }
 
 template <int XF>
@@ -381,20 +516,35 @@ static void launch_conv(const ugk_conv_desc &d, long total, hipStream_t s)
     k_conv_t2<XF><<<dim3((unsigned)((items + 63) / 64), groups), 256, 0, s>>>(d);
     return;
   }
-  if (d.mode == UGK_CONV_S1 && d.cout <= 4 && XF && d.cin <= SO_MAX_CIN && d.cin % 16 == 0) {
+  if (d.mode == UGK_CONV_S1 && d.cout <= 4 && XF == 1 && d.cin <= SO_MAX_CIN && d.cin % 16 == 0) {
     const unsigned nblk = (unsigned)((total + 63) / 64 < 8192 ? (total + 63) / 64 : 8192);
-    switch (d.cin / 16) {
-      case 1: k_conv_small_out<1><<<nblk, 256, 0, s>>>(d); break;
-      case 2: k_conv_small_out<2><<<nblk, 256, 0, s>>>(d); break;
-      case 3: k_conv_small_out<3><<<nblk, 256, 0, s>>>(d); break;
-      default: k_conv_small_out<4><<<nblk, 256, 0, s>>>(d); break;
+    if (d.cout == 3 && d.cin == 16) k_conv_small_out<1, 3><<<nblk, 256, 0, s>>>(d);   // the network's last layer
+    else switch (d.cin / 16) {
+      case 1: k_conv_small_out<1, 4><<<nblk, 256, 0, s>>>(d); break;
+      case 2: k_conv_small_out<2, 4><<<nblk, 256, 0, s>>>(d); break;
+      case 3: k_conv_small_out<3, 4><<<nblk, 256, 0, s>>>(d); break;
+      default: k_conv_small_out<4, 4><<<nblk, 256, 0, s>>>(d); break;
     }
     return;
   }
-  static int nv_env = -1;
+  static int nv_env = -1, ks_env = -1;
   if (nv_env < 0) {
     const char *e = getenv("UFEM_CONV_NV");
     nv_env = e ? atoi(e) : 0;
+    e = getenv("UFEM_CONV_KS");     /* 4, 1, or 9 for the unrolled form; unset: by size */
+    ks_env = e ? atoi(e) : 0;
+  }
+  // unpacked inputs: the sum split over the 4 waves of a block (measured
+  // fastest at every layer size of the network)
+  int ks = 0;
+  if (d.cin != 4) ks = (ks_env == 1 || ks_env == 9) ? (ks_env == 1 ? 1 : 0) : 4;
+  if (ks == 4) {
+    launch_conv_nv<1, XF, 0, 4>(d, dim3((unsigned)((total + 15) / 16), groups), s);
+    return;
+  }
+  if (ks == 1) {
+    launch_conv_nv<1, XF, 0, 1>(d, dim3((unsigned)((total + 63) / 64), groups), s);
+    return;
   }
   // two voxel tiles per wave only when there are waves to spare
   const bool two = nv_env ? nv_env == 2 : total * groups >= 64L * 4 * 2048;
@@ -698,6 +848,17 @@ static void dw_launch(const ugk_dw_desc &d, dim3 grid, float *part, long cps, hi
   else k_conv_dw<TPW, MODE, 0, 0><<<grid, 256, 0, s>>>(d, part, cps);
 }
 
+__global__ __launch_bounds__(256) void k_to_split(long n4, const float *__restrict__ x, bf16 *__restrict__ y)
+{
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long)gridDim.x * 256) {
+    bf16x4 b[3];
+    split3(*(const f32x4 *)(x + 4 * i), true, b);
+    *(bf16x4 *)(y + 12 * i) = b[0];
+    *(bf16x4 *)(y + 12 * i + 4) = b[1];
+    *(bf16x4 *)(y + 12 * i + 8) = b[2];
+  }
+}
+
 template <int TPW>
 static void dw_mode(const ugk_dw_desc &d, dim3 grid, float *part, long cps, hipStream_t s)
 {
@@ -720,7 +881,10 @@ int ugk_conv(const ugk_conv_desc *d, void *s)
   if (d->cin % 4 || d->cip % 16 || d->cop % 16 || d->cin > d->cip || d->cout > d->cop || d->ldx % 4 ||
       d->ldy % 4 || d->mode < 0 || d->mode > 2 || (d->ga && !d->gb))
     return ugk_hip_fail("ugk_conv: channel layout", hipErrorInvalidValue);
-  if (d->x_f32) launch_conv<1>(*d, total, S(s));
+  if (d->x_f32 < 0 || d->x_f32 > 2 || d->y_f32 < 0 || d->y_f32 > 1)
+    return ugk_hip_fail("ugk_conv: activation format", hipErrorInvalidValue);
+  if (d->x_f32 == UGK_ACT_SPLIT) launch_conv<2>(*d, total, S(s));
+  else if (d->x_f32 == UGK_ACT_F32) launch_conv<1>(*d, total, S(s));
   else launch_conv<0>(*d, total, S(s));
   CK_LAUNCH("k_conv");
   return 0;
@@ -738,7 +902,8 @@ int ugk_conv_dw(const ugk_dw_desc *d, float *part, int ks, float *grad, int accu
   const long nch = (vox(d->go) * d->nb + 63) / 64;
   if (nch <= 0) return 0;
   if (d->cop > DW_MAX_COP || d->cip > DW_MAX_CIP || d->cop % 16 || d->cip % 16 || d->ldx % 4 ||
-      d->ldd % 4 || (d->mode != UGK_CONV_S1 && d->mode != UGK_CONV_S2) || ks < 1)
+      d->ldd % 4 || (d->mode != UGK_CONV_S1 && d->mode != UGK_CONV_S2) || ks < 1 ||
+      d->x_f32 < 0 || d->x_f32 > 1 || d->dy_f32 < 0 || d->dy_f32 > 1)
     return ugk_hip_fail("ugk_conv_dw: layout", hipErrorInvalidValue);
   const long cps = (nch + ks - 1) / ks;
   const int tiles = (d->cop / 16) * (d->cip / 16);
@@ -753,6 +918,14 @@ int ugk_conv_dw(const ugk_dw_desc *d, float *part, int ks, float *grad, int accu
   k_dw_reduce<<<grid_for(27L * d->cout * d->cin), 256, 0, S(s)>>>(part, ksr, d->cop, d->cip, d->cout, d->cin,
                                                                    grad, accumulate);
   CK_LAUNCH("k_dw_reduce");
+  return 0;
+}
+
+int ugk_to_split(long n4, const float *x, unsigned short *y, void *s)
+{
+  if (n4 <= 0) return 0;
+  k_to_split<<<grid_for(n4), 256, 0, S(s)>>>(n4, x, y);
+  CK_LAUNCH("k_to_split");
   return 0;
 }
 

@@ -20,7 +20,7 @@ P_I = C.POINTER(C.c_int)
 P_F = C.POINTER(C.c_float)
 
 AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
-SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3}
+SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4}
 EXEC = {"streamed": 0, "persistent": 1, "fused": 2, "classic": 3}
 
 
@@ -358,10 +358,10 @@ class GpuNet:
         self._chk(lib().ufem_net_get_pair(self.h, i, _pf(v), _pf(rr), _pf(ee), C.byref(er), C.byref(f)), "pair")
         return v, rr, ee, er.value, f.value
 
-    STAGES = ("C0", "C1", "C2", "gate", "C2T", "C1T", "C0T")
+    STAGES = ("C0", "C1", "C2g", "C2T", "C1T", "C0T")
 
     def time_stages(self, reps=50):
-        us = np.zeros(7)
+        us = np.zeros(len(self.STAGES))
         self._chk(lib().ufem_net_time_stages(self.h, reps, _pd(us)), "time_stages")
         return dict(zip(self.STAGES, us))
 
@@ -397,7 +397,8 @@ class GpuNet:
 
 def probe_conv(mode, flip, x, w, gi, go, dy=None, x_f32=False):
     """One convolution kernel on host data: x (nb, Vi, cin), w (27, cout, cin)
-    -> y (nb, Vo, cout) [and dw (27, cout, cin) for dy (nb, Vo, cout)]."""
+    -> y (nb, Vo, cout) [and dw (27, cout, cin) for dy (nb, Vo, cout)].
+    x_f32: False/0 bf16 activations, True/1 f32, 2 f32 stored split."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     w = np.ascontiguousarray(w, dtype=np.float32)
     nb, _, cin = x.shape
@@ -410,7 +411,8 @@ def probe_conv(mode, flip, x, w, gi, go, dy=None, x_f32=False):
     if dy is not None:
         dy = np.ascontiguousarray(dy, dtype=np.float32)
         dw = np.zeros((27, cout, cin), dtype=np.float32)
-    rv = lib().ufem_net_probe_conv(mode, flip, 1 if x_f32 else 0, nb, _pi(gi_), _pi(go_), cin, cout, _pf(x), _pf(w),
+    xf = 2 if (x_f32 is not True and x_f32 == 2) else (1 if x_f32 else 0)
+    rv = lib().ufem_net_probe_conv(mode, flip, xf, nb, _pi(gi_), _pi(go_), cin, cout, _pf(x), _pf(w),
                                    _pf(y), _pf(dy) if dy is not None else None, _pf(dw) if dw is not None else None)
     if rv != 0:
         raise RuntimeError(f"probe_conv failed ({rv}): {last_error()}")

@@ -24,13 +24,15 @@ def field_E(mesh, seed):
     return grf_E(seed, mesh.nx, mesh.ny, mesh.nz, (L / mesh.nx, B / mesh.ny, H / mesh.nz))
 
 
-def collect(system, net, mesh, seeds, pairs_per_field=32, every=8, eps=1e-11, npts=2, log=None):
-    """Assemble and solve `system` for each E field of `seeds`, adding pairs to
-    `net`; leaves the system holding the last field.  Returns (pairs, seconds)."""
+def collect(system, net, mesh, seeds, pairs_per_field=32, every=8, eps=1e-11, npts=2, log=None, field=None):
+    """Assemble and solve `system` for each E field of `seeds` (field(mesh,
+    seed), by default `field_E`), adding pairs to `net`; leaves the system
+    holding the last field.  Returns (pairs, seconds)."""
+    field = field or field_E
     t0 = time.perf_counter()
     added = 0
     for k, sd in enumerate(seeds):
-        E = field_E(mesh, sd)
+        E = field(mesh, sd)
         system.set_E(E)
         system.fill_K(npts)
         system.add_disps()
