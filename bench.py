@@ -20,7 +20,7 @@ Monte-Carlo solve of 256 uneven meshes in one launch, and the CPU baseline
 (the oracle's pthread restatement of uFEM's solver, bounded sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-mixed]
-                    [--no-learned] [--no-batch] [--kernels-only]
+                    [--no-learned] [--no-batch] [--kernels-only] [--net-kernels-only]
 """
 
 from __future__ import annotations
@@ -270,6 +270,9 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the dominant solve kernel and a device copy of known size "
                          "(for hardware-counter runs)")
+    ap.add_argument("--net-kernels-only", action="store_true",
+                    help="launch only the network's inference stages on the 128x32x32 random-field "
+                         "cantilever (initial weights; for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -278,6 +281,20 @@ def main():
 
     from ufem_amd import gpu as G
     from ufem_amd.mesh import cantilever, lognormal_E
+
+    if args.net_kernels_only:
+        from ufem_amd import learned as Lr
+        m5 = cantilever(128, 32, 32)
+        m5.E[:] = Lr.field_E(m5, 9999)
+        s5 = G.GpuSystem(m5)
+        s5.assemble(2)
+        net = G.GpuNet(129, 33, 33, batch=4)
+        net.init(1)
+        net.bind(s5, m5.E)
+        log("network stages (us)", {k: round(v, 1) for k, v in net.time_stages(reps=20).items()})
+        net.close()
+        s5.close()
+        return
 
     nx, ny, nz = (int(v) for v in args.size.split(","))
     t0 = time.perf_counter()
