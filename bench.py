@@ -11,7 +11,7 @@ K, f already resident in HBM.
 Printed: one JSON line with the solve time (value, seconds), the roofline of
 the solve's dominant kernel (w = K u with (w, u) of the single-reduction PCG),
 SpMV bandwidth, the mixed-precision solve on the same system (and on a COV
-0.6 field), learned + mixed precision on the same system (a network trained
+0.6 field), CG with the SSOR preconditioner on the same system, learned + mixed precision on the same system (a network trained
 on the GPU from fresh lognormal fields; plain FP64 PCG vs flexible CG with
 the network, FP64 and with the FP32 matrix), the learned preconditioner on
 the 128 x 32 x 32 random-field cantilever (training on the GPU, then plain
@@ -379,6 +379,18 @@ def main():
                  "speedup_vs_fp64": round(solve_s / m_s, 3)}
         log("mixed", mixed)
 
+    # CG with the SSOR preconditioner (femEqsCGwSSOR) on the same system; the
+    # first call also builds the colour order
+    g.solve(solver="ssor", eps=1e-8, rule=1)
+    t0 = time.perf_counter()
+    si = g.solve(solver="ssor", eps=1e-8, rule=1)
+    s_s = time.perf_counter() - t0
+    ssor = {"iters": si["iters"], "colours": si["outer"], "solve_s": round(s_s, 4), "rv": si["rv"],
+            "true_relres": si["true_relres"],
+            "u_rel_err_vs_jacobi": float(np.linalg.norm(g.u() - u64) / np.linalg.norm(u64)),
+            "speedup_vs_jacobi": round(solve_s / s_s, 3)}
+    log("ssor", ssor)
+
     mixed_cov06 = None
     if not args.no_mixed:
         mixed_cov06 = bench_mixed_high_contrast(nx, ny, nz, args.seed + 1)
@@ -435,6 +447,7 @@ def main():
         "spmv": spmv,
         "mixed": mixed,
         "mixed_cov06": mixed_cov06,
+        "ssor": ssor,
         "learned_mixed_1m": learned_mixed,
         "learned": learned,
         "batch": batch,

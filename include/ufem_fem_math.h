@@ -75,6 +75,14 @@ int femEqsCGwJ(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt);
  * Replaces femEqsBiCCSwJ, fem/fem_eqs.c:262-468 (fem/fem_math.h:155). */
 int femEqsBiCCSwJ(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt);
 
+/* CG with the SSOR (omega = 1, symmetric Gauss-Seidel) preconditioner: the
+ * same arguments, stopping rule and return codes as femEqsCGwJ.  The sweeps
+ * run in a greedy colour order of the rows instead of the reference's
+ * natural order, so the preconditioner (and the iteration count) differ
+ * while x solves the same system.  Replaces femEqsCGwSSOR,
+ * fem/fem_eqs.c:999-1188 (fem/fem_math.h:154). */
+int femEqsCGwSSOR(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt);
+
 /* c = a b.  Replaces femMatVecMultBig, fem/fem_math.c:1350-1451. */
 int femMatVecMultBig(tMatrix *a, tVector *b, tVector *c);
 

@@ -48,8 +48,10 @@ enum {
   UFEM_SOLVER_BICGSTAB = 1,    /* Jacobi BiCGStab, FP64                   */
   UFEM_SOLVER_MIXED = 2,       /* FP32 inner PCG in FP64 refinement       */
   UFEM_SOLVER_LEARNED = 3,     /* flexible CG with the learned preconditioner */
-  UFEM_SOLVER_LEARNED_MIXED = 4 /* the same with q = A p from the FP32 copy of A and
-                                  FP64 residual refreshes (inner_eps, inner_maxit) */
+  UFEM_SOLVER_LEARNED_MIXED = 4, /* the same with q = A p from the FP32 copy of A and
+                                   FP64 residual refreshes (inner_eps, inner_maxit) */
+  UFEM_SOLVER_SSOR = 5          /* CG with the SSOR (symmetric Gauss-Seidel) preconditioner,
+                                   rows in greedy colour order */
 };
 enum { UFEM_RULE_REFERENCE = 0, UFEM_RULE_RELATIVE = 1 };
 /* streamed: single-reduction PCG, two kernels per iteration (w = K u with
@@ -79,7 +81,8 @@ typedef struct {
   int status;        /* AF_OK, AF_ERR (not converged), AF_ERR_ZER, ...    */
   long iters;        /* iterations (all inner iterations for mixed)       */
   long outer;        /* mixed, learned-mixed: FP64 residual refreshes;
-                        learned: iteration of the Jacobi fallback         */
+                        learned: iteration of the Jacobi fallback;
+                        SSOR: colours of the sweep order                  */
   int fallback;      /* learned: 1 if Jacobi-PCG had to take over         */
   double relres;     /* last ||r||/||b|| the solver tested                */
   double true_relres;/* ||b - K u|| / ||b|| recomputed in FP64            */

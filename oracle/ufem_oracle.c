@@ -848,12 +848,94 @@ done:
   return converged ? UO_OK : UO_ERR;
 }
 
+/* femEqsCGwSSOR, fem/fem_eqs.c:999-1188, the MAT_SPAR branch (:1086-1112):
+ * symmetric Gauss-Seidel in natural order, zz = M (r - L zz) forward over
+ * the slots with 1 <= column < row, then z = zz - M U z backward over the
+ * slots with column > row (M = 1 / diag). */
+static int cg_ssor(uo_sys *s, double eps, long maxIt, int rule, long *iters, double *relres)
+{
+  const rs_mat *a = &s->K;
+  long n = s->ndof, i, ii, j, ipos;
+  double *M, *r, *z, *zz, *p, *q, *x = s->u, *b = s->F;
+  double ro, alpha, beta, roro = 0.0, normRes = 0.0, normX, normA, normB, val;
+  int rv = UO_OK, converged = 0;
+  *iters = 0;
+  normA = uo_mat_norm(s);
+  normB = vnorm(s, b, n);
+  if (normB <= 0.0) {
+    for (i = 0; i < n; i++) x[i] = 0.0;
+    *relres = 0.0;
+    return UO_OK;
+  }
+  M = (double *)calloc((size_t)n, sizeof(double));
+  r = (double *)calloc((size_t)n, sizeof(double));
+  z = (double *)calloc((size_t)n, sizeof(double));
+  zz = (double *)calloc((size_t)n, sizeof(double));
+  p = (double *)calloc((size_t)n, sizeof(double));
+  q = (double *)calloc((size_t)n, sizeof(double));
+  if (!M || !r || !z || !zz || !p || !q) { rv = UO_ERR_MEM; goto done; }
+  for (i = 1; i <= n; i++) {
+    val = rs_get(a, i, i);
+    if (fabs(val) < FEM_ZERO) { rv = UO_ERR_ZER; goto done; }
+    M[i - 1] = 1.0 / val;
+  }
+  spmv(s, x, r);
+  for (i = 0; i < n; i++) r[i] = b[i] - r[i];
+  for (i = 1; i <= maxIt; i++) {
+    for (ii = 1; ii <= n; ii++) {
+      val = 0.0;
+      for (j = a->frompos[ii - 1]; j < a->frompos[ii - 1] + a->defpos[ii - 1]; j++) {
+        ipos = a->pos[j];
+        if (ipos >= ii || ipos < 1) continue;
+        val += a->data[j] * zz[ipos - 1];
+      }
+      zz[ii - 1] = M[ii - 1] * (r[ii - 1] - val);
+    }
+    for (ii = n; ii >= 1; ii--) {
+      val = 0.0;
+      for (j = a->frompos[ii - 1]; j < a->frompos[ii - 1] + a->defpos[ii - 1]; j++) {
+        ipos = a->pos[j];
+        if (ipos > ii) val += a->data[j] * z[ipos - 1];
+      }
+      z[ii - 1] = zz[ii - 1] - M[ii - 1] * val;
+    }
+    ro = dot(s, r, z, n);
+    if (i == 1) {
+      for (j = 0; j < n; j++) p[j] = z[j];
+    } else {
+      beta = ro / roro;
+      for (j = 0; j < n; j++) p[j] = (z[j] + (beta * p[j]));
+    }
+    spmv(s, p, q);
+    alpha = ro / dot(s, p, q, n);
+    for (j = 0; j < n; j++) {
+      x[j] = x[j] + (alpha * p[j]);
+      r[j] = r[j] - (alpha * q[j]);
+    }
+    normRes = vnorm(s, r, n);
+    normX = vnorm(s, x, n);
+    *iters = i;
+    if (rule == 0) {
+      if (normRes <= (eps * ((normA * normX) + normB))) { converged = 1; break; }
+    } else {
+      if (normRes <= eps * normB) { converged = 1; break; }
+    }
+    roro = ro;
+  }
+  if (!converged) rv = UO_ERR;
+done:
+  *relres = normRes / normB;
+  free(M); free(r); free(z); free(zz); free(p); free(q);
+  return rv;
+}
+
 int uo_solve(uo_sys *s, int solver, double eps, long maxit, int rule, long *iters, double *relres)
 {
   int rv;
   double t0 = now_s();
   if (maxit < 0) maxit = s->ndof;
   if (solver == 0) rv = cg_jacobi(s, eps, maxit, rule, iters, relres);
+  else if (solver == 3) rv = cg_ssor(s, eps, maxit, rule, iters, relres);
   else rv = bicgs_jacobi(s, solver, eps, maxit, rule, iters, relres);
   s->t[3] = now_s() - t0;
   return rv;

@@ -49,7 +49,7 @@ int  uo_fill_K(uo_sys *s, int npts);            /* fem_fill_K(AF_NO)            
 int  uo_add_loads(uo_sys *s);                   /* fem_add_loads(0)             */
 int  uo_add_disps(uo_sys *s);                   /* fem_add_disps(AF_YES, 0)     */
 /* solver: 0 = femEqsCGwJ, 1 = femEqsBiCCSwJ (as written), 2 = BiCGStab with
- * the true residual b - A x.  rule: 0 = reference criterion
+ * the true residual b - A x, 3 = femEqsCGwSSOR.  rule: 0 = reference criterion
  * ||r|| <= eps (||A|| ||x|| + ||b||); 1 = ||r|| <= eps ||b||. */
 int  uo_solve(uo_sys *s, int solver, double eps, long maxit, int rule,
               long *iters, double *relres);

@@ -25,7 +25,7 @@ def declared():
 
 def test_headers_declare_the_dropin_symbols():
     names = declared()
-    for n in ("femEqsCGwJ", "femEqsBiCCSwJ", "femMatVecMultBig", "femVecVecMultBig",
+    for n in ("femEqsCGwJ", "femEqsBiCCSwJ", "femEqsCGwSSOR", "femMatVecMultBig", "femVecVecMultBig",
               "femVecNormBig", "femMatNormBig", "femVecLinCombBig", "ufem_sys_create",
               "ufem_sys_solve", "ufem_gpu_element_stiffness"):
         assert n in names

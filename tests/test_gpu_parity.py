@@ -185,6 +185,74 @@ def test_native_bicgstab(golden):
     assert rel(g.u(), golden["het8_cg/u"]) <= 1e-8
 
 
+def oracle_ssor(mesh, eps, npts=3):
+    s = uo.OracleSystem(mesh)
+    s.assemble(npts)
+    rv, it, rr = s.solve(3, eps, 20 * s.n, 1)
+    return s, rv, it
+
+
+@pytest.mark.parametrize("mname", ["c1", "het8", "het4", "dist"])
+def test_native_ssor_vs_oracle(golden, mname):
+    """CG with the SSOR preconditioner on the device (colour-ordered sweeps)
+    against the oracle's natural-order femEqsCGwSSOR: the same displacements
+    and reactions to the parity bars; fewer iterations than Jacobi PCG."""
+    mesh = MESHES[mname]()
+    o, rv_o, it_o = oracle_ssor(mesh, 1e-13)
+    assert rv_o == 0
+    g = G.GpuSystem(mesh)
+    g.assemble(3)
+    info = g.solve("ssor", eps=1e-13, rule=1)
+    jac = g.solve("pcg", eps=1e-13, rule=1)
+    g.solve("ssor", eps=1e-13, rule=1)
+    print(mname, "ssor", info["iters"], "colours", info["outer"], "oracle ssor", it_o, "jacobi", jac["iters"])
+    assert info["rv"] == 0 and info["true_relres"] <= 1e-10     # the recurrence drifts below ~1e-12
+    assert rel(g.u(), o.u()) <= 1e-8
+    if mname != "het4":                  # the reference stops unconverged on het4
+        assert rel(g.u(), golden[f"{mname}_cg/u"]) <= 1e-8
+    o.reactions(3)
+    assert rel(g.reactions()[0], o.R()[0]) <= 1e-6
+    assert info["iters"] < jac["iters"]
+    assert info["outer"] >= 2
+
+
+def test_medium_ssor_vs_oracle():
+    mesh = cantilever(40, 8, 8, E=lognormal_E(99, 40 * 64, cov=0.3))
+    o, rv_o, it_o = oracle_ssor(mesh, 1e-12, npts=2)
+    g = G.GpuSystem(mesh)
+    g.assemble(2)
+    info = g.solve("ssor", eps=1e-12, rule=1)
+    print("medium ssor", info["iters"], "colours", info["outer"], "oracle", it_o)
+    assert info["rv"] == 0
+    assert rel(g.u(), o.u()) <= 1e-8
+    assert info["iters"] <= 1.5 * it_o
+
+
+def test_dropin_ssor_reference_arguments(golden):
+    """femEqsCGwSSOR through the C-ABI on uFEM's own row-slot K (scalar rows,
+    so the colouring is of DOFs), the reference rule, eps 1e-21, maxIt = n;
+    a second call on the same matrix reuses the colour order."""
+    mesh = MESHES["c1"]()
+    s = uo.OracleSystem(mesh)
+    s.assemble(3)
+    rl, c1, v = s.K_rowslot()
+    K = G.RowSlotMatrix(s.n, rl, c1, v, capacity=rl + 3)
+    b = G.Vec(s.F())
+    G.set_stop_rule(0)
+    for _ in range(2):
+        x = G.Vec(np.zeros(s.n))
+        rv = G.femEqsCGwSSOR(K, b, x, 1e-21, s.n)
+        info = G.last_solve()
+        print("dropin ssor", rv, info)
+        assert rel(x.data, golden["c1_cg/u"]) <= 1e-8
+    G.set_stop_rule(1)
+    x = G.Vec(np.zeros(s.n))
+    rv = G.femEqsCGwSSOR(K, b, x, 1e-12, s.n)
+    G.set_stop_rule(0)
+    assert rv == 0 and G.last_solve()["true_relres"] <= 1e-10
+    assert rel(x.data, golden["c1_cg/u"]) <= 1e-8
+
+
 def test_dropin_spmv_and_reductions():
     mesh = MESHES["het8"]()
     s = uo.OracleSystem(mesh)

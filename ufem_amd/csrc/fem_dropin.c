@@ -151,6 +151,7 @@ static int upload_matrix(const tMatrix *a)
     g_ctx.cap_rows = n;
   }
   g_ctx.K.bs = 1;
+  g_ctx.K.gen++;
   g_ctx.K.n_brows = n;
   g_ctx.K.n_slices = nsl;
   g_ctx.K.total_slots = total;
@@ -211,6 +212,12 @@ static int dropin_solve(tMatrix *a, tVector *b, tVector *x, double eps, long max
 int femEqsCGwJ(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt)
 {
   return dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_PCG);
+}
+
+/* femEqsCGwSSOR, fem/fem_eqs.c:999-1188 (colour-ordered sweeps, see header) */
+int femEqsCGwSSOR(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt)
+{
+  return dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_SSOR);
 }
 
 /* femEqsBiCCSwJ, fem/fem_eqs.c:262-468 (true-residual test, see header) */

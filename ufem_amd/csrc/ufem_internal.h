@@ -13,6 +13,21 @@
 
 #define UFEM_KNOWN_DOFS 3
 
+/* colour-permuted copy of a matrix for CG with the SSOR preconditioner
+ * (ufem_ssor.c): built for one matrix and pattern generation, refilled with
+ * its values at every solve */
+typedef struct ufem_ssor {
+  const ugk_mat *src;
+  unsigned long gen;
+  long src_slots;
+  ugk_mat Kc;
+  int *d_perm;              /* new block row -> old block row */
+  int ncol;
+  long *cb;                 /* host, ncol + 1: first row of each colour */
+} ufem_ssor;
+int ufem_ssor_prepare(const ugk_mat *K, ufem_ssor **S, void *stream);
+void ufem_ssor_free(ufem_ssor *S);
+
 /* device vectors of one Krylov workspace (FP64 plus the FP32 inner set) */
 typedef struct {
   long n;
@@ -21,6 +36,7 @@ typedef struct {
   float *r32, *z32, *p32[2], *q32, *x32, *dinv32;  /* mixed precision     */
   double *fP, *fQ;                                 /* flexible CG rings   */
   int fm;                                          /* ring slots          */
+  ufem_ssor *ssor;                                 /* SSOR colour order   */
   double *part;                                    /* reduction partials  */
   ugk_state *st;                                   /* device scalars      */
   ugk_state *st_host;                              /* pinned mirror       */

@@ -29,6 +29,7 @@ void ufem_work_free(ufem_work *w)
   ugk_free(w->shat); ugk_free(w->t); ugk_free(w->r32); ugk_free(w->z32); ugk_free(w->p32[0]);
   ugk_free(w->p32[1]); ugk_free(w->q32); ugk_free(w->x32); ugk_free(w->dinv32);
   ugk_free(w->fP); ugk_free(w->fQ);
+  ufem_ssor_free(w->ssor);
   ugk_free(w->part); ugk_free(w->st);
   ugk_host_free(w->st_host);
   memset(w, 0, sizeof(*w));
@@ -416,6 +417,65 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   return info->status;
 }
 
+/* CG with the SSOR preconditioner (femEqsCGwSSOR, fem/fem_eqs.c:999-1188)
+ * in colour order: b, x and 1/diag moved into colour order, the iteration of
+ * ugk_ssor_iter captured in the chunk graph, x moved back.  Uses the
+ * BiCGStab buffers of the workspace: v = zz, s = b, t = x, shat = 1/diag. */
+typedef struct {
+  ufem_ssor *S;
+  long n;
+  ufem_work *w;
+} ssor_ctx;
+
+static int ssor_step(void *c, long it, void *stream)
+{
+  ssor_ctx *q = (ssor_ctx *)c;
+  ufem_work *w = q->w;
+  (void)it;
+  return ugk_ssor_iter(&q->S->Kc, q->S->ncol, q->S->cb, q->n, w->t, w->r, w->v, w->z, w->p[0], w->q, w->shat,
+                       w->st, w->part, stream);
+}
+
+static int solve_ssor(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
+                      const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+{
+  ssor_ctx c;
+  ufem_ssor *S;
+  ugk_state *h = w->st_host;
+  int rv;
+  if (!w->v || !w->s || !w->t || !w->shat) {
+    ufem_set_error("SSOR: workspace without the BiCGStab vectors");
+    return AF_ERR_VAL;
+  }
+  rv = ufem_ssor_prepare(K, &w->ssor, stream);
+  if (rv != AF_OK) return rv;
+  S = w->ssor;
+  rv = prepare(K, w, stream, o->eps, o->rule, maxit);
+  if (rv != AF_OK) return rv;
+  if (ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, b, w->s, 0, stream) ||
+      ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, x, w->t, 0, stream) ||
+      ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, w->dinv, w->shat, 0, stream))
+    return gpu_fail("SSOR: colour order");
+  if (ugk_fcg_start(&S->Kc, n, w->s, w->t, w->r, w->shat, w->part, stream)) return gpu_fail("SSOR: start");
+  c.S = S;
+  c.n = n;
+  c.w = w;
+  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, ssor_step, &c, 1);
+  if (rv != AF_OK) return rv;
+  if (ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, w->t, x, 1, stream)) return gpu_fail("SSOR: order back");
+  if (get_state(w, stream)) return AF_ERR;
+  if (h->bb <= 0.0) {
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+    return AF_OK;
+  }
+  info->iters = h->iter;
+  info->outer = S->ncol;
+  info->relres = sqrt(h->rr / h->bb);
+  info->status = status_of(h->done);
+  return info->status;
+}
+
 int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
                       void *ev0, void *ev1, const ufem_solve_opts *o, ufem_solve_info *info)
 {
@@ -431,6 +491,7 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem
     case UFEM_SOLVER_LEARNED_MIXED:
       rv = solve_learned((ugk_mat *)K, n, b, x, w, stream, o, maxit, 1, info);
       break;
+    case UFEM_SOLVER_SSOR: rv = solve_ssor(K, n, b, x, w, stream, o, maxit, info); break;
     default: ufem_set_error("unknown solver %d", o->solver); return AF_ERR_VAL;
   }
   if (ugk_event_record(ev1, stream)) return gpu_fail("event");

@@ -148,6 +148,7 @@ static int build_pattern(ufem_sys *s, const ufem_mesh *m)
     }
   }
   s->K.bs = 3;
+  s->K.gen = 1;
   s->K.n_brows = nn;
   s->K.n_slices = nslices;
   s->K.total_slots = total;

@@ -33,6 +33,7 @@ typedef struct {
   int *d_cols;        /* [total_slots * 64]                             */
   double *d_vals;     /* [total_slots * 64 * b*b]                       */
   float *d_vals32;    /* FP32 copy for mixed precision (may be NULL)    */
+  unsigned long gen;  /* bumped whenever the pattern is (re)uploaded     */
 } ugk_mat;
 
 /* largest grid of a reducing kernel, and the size (in doubles) of the
@@ -268,6 +269,22 @@ int ugk_fcg_refresh(const ugk_mat *K, const double *d_b, const double *d_x, doub
                     ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_update(const ugk_mat *K, long n, double *d_x, double *d_r, const double *d_P, const double *d_Q,
                    int m, const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
+
+/* ---------------- CG with the SSOR preconditioner ------------------------ */
+/* Kc: A with block rows (and columns) in colour order, new row I' = old row
+ * perm[I']; colour c owns rows [cb[c], cb[c+1]) and no two of its rows
+ * couple.  ugk_ssor_permute_vals fills Kc's values from A (pattern already
+ * on the device); ugk_ssor_permute_vec moves vectors into (dir 0) and out
+ * of (dir 1) colour order.  One ugk_ssor_iter = convergence test, forward
+ * and backward symmetric Gauss-Seidel sweeps (one launch per colour),
+ * (r, z), p = z + beta p, q = Kc p, x and r update; all vectors in colour
+ * order, r and the state's partials made by ugk_fcg_start on Kc. */
+int ugk_ssor_permute_vals(const ugk_mat *K, const ugk_mat *Kc, const int *d_perm, void *stream);
+int ugk_ssor_permute_vec(long n_brows, int bs, const int *d_perm, const double *d_src, double *d_dst, int dir,
+                         void *stream);
+int ugk_ssor_iter(const ugk_mat *Kc, int ncol, const long *cb, long n, double *d_x, double *d_r, double *d_zz,
+                  double *d_z, double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st, double *d_part,
+                  void *stream);
 
 /* ---------------- 3-D convolution network kernels (ugk_unet.hip) ---------- */
 /* Activations are bf16 (stored as unsigned short), channels-last on a node

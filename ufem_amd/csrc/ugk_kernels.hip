@@ -1886,6 +1886,194 @@ __global__ __launch_bounds__(WG) void k_fcg_update(long n, double *__restrict__ 
   }
 }
 
+
+// --------------------------------------------------------------------------
+// CG with the SSOR (omega = 1, symmetric Gauss-Seidel) preconditioner,
+// femEqsCGwSSOR (fem/fem_eqs.c:999-1188), on a colour-permuted copy of A:
+// block rows are ordered colour by colour (no two rows of a colour couple),
+// so a sweep is one launch per colour over a contiguous range [r0, r1) of
+// rows.  Forward: zz_I = D_I^-1 (r_I - sum_{J < r0} A_IJ zz_J), the 3x3
+// diagonal block solved lower-triangular in place; backward, colours in
+// reverse: z_I = zz_I - D_I^-1 (sum_{J >= r1} A_IJ z_J + upper part of the
+// diagonal block).  Rows of later (forward) or earlier (backward) colours
+// are never read, so nothing is cleared between sweeps.
+// Partials: start -> test: B2(0); upd(k) -> test(k+1): B2((k+1) & 1), with
+// (., r.r, x.x, b.b); dot -> p: S (r.z); q -> upd: A (p.q).
+// --------------------------------------------------------------------------
+template <int BS, int FWD>
+__global__ __launch_bounds__(WG) void k_sgs(SellView M, const double *__restrict__ vals, long r0, long r1,
+                                            const double *__restrict__ src, const double *__restrict__ dinv,
+                                            double *y, const ugk_state *st)
+{
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  const long sl = r0 / 64 + (long)blockIdx.x * NW + (threadIdx.x >> 6);
+  const long I = sl * 64 + lane;
+  if (I < r0 || I >= r1) return;           // no barriers below
+  const int L = M.rowlen[I];
+  const long so = M.soff[sl];
+  const int *cp = M.cols + so * 64 + lane;
+  const double *vp = vals + so * 64 * BS * BS + lane;
+  double acc[BS], D[BS][BS];
+#pragma unroll
+  for (int a = 0; a < BS; a++) {
+    acc[a] = 0.0;
+#pragma unroll
+    for (int b = 0; b < BS; b++) D[a][b] = 0.0;
+  }
+  for (int k = 0; k < L; k++) {
+    const long J = cp[(long)k * 64];
+    const double *v = vp + (long)k * 64 * BS * BS;
+    if (J == I) {
+#pragma unroll
+      for (int q = 0; q < BS * BS; q++) D[q / BS][q % BS] = v[q * 64];
+    } else if (FWD ? J < r0 : J >= r1) {
+      double yv[BS];
+#pragma unroll
+      for (int c = 0; c < BS; c++) yv[c] = y[J * BS + c];
+#pragma unroll
+      for (int a = 0; a < BS; a++)
+#pragma unroll
+        for (int c = 0; c < BS; c++) acc[a] += v[(a * BS + c) * 64] * yv[c];
+    }
+  }
+  double yl[BS];
+  if (FWD) {
+#pragma unroll
+    for (int d = 0; d < BS; d++) {
+      double t = acc[d];
+#pragma unroll
+      for (int e = 0; e < d; e++) t += D[d][e] * yl[e];
+      yl[d] = dinv[I * BS + d] * (src[I * BS + d] - t);
+      y[I * BS + d] = yl[d];
+    }
+  } else {
+#pragma unroll
+    for (int d = BS - 1; d >= 0; d--) {
+      double t = acc[d];
+#pragma unroll
+      for (int e = d + 1; e < BS; e++) t += D[d][e] * yl[e];
+      yl[d] = src[I * BS + d] - dinv[I * BS + d] * t;
+      y[I * BS + d] = yl[d];
+    }
+  }
+}
+
+// convergence test on the previous update's partials (iteration st->iter)
+__global__ __launch_bounds__(WG) void k_ssor_test(ugk_state *st, const double *part, unsigned G0, unsigned GB)
+{
+  if (st->done) return;
+  const long k = st->iter;
+  double t[4];
+  reduce_prev<4>(PART_B2(part, k & 1), k == 0 ? G0 : GB, t);
+  const double bb = (k == 0) ? t[3] : st->bb;
+  const int done = pcg_test(st, k, t[1], t[2], bb, t[1]);
+  if (threadIdx.x == 0) {
+    st->cur = k;
+    st->rr = t[1];
+    st->xx = t[2];
+    if (k == 0) st->bb = bb;
+    if (done) st->done = done;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_ssor_dot(long n, const double *__restrict__ r, const double *__restrict__ z,
+                                                 const ugk_state *st, double *part)
+{
+  if (st->done) return;
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) v[0] += r[i] * z[i];
+  block_partials<1>(v, PART_S(part));
+}
+
+// beta = (r,z) / (r,z)_prev, p = z + beta p (p = z at the first iteration)
+__global__ __launch_bounds__(WG) void k_ssor_p(long n, const double *__restrict__ z, double *p, ugk_state *st,
+                                               const double *part, unsigned GV)
+{
+  if (st->done) return;
+  const long k = st->cur;
+  double t[1];
+  reduce_prev<1>(PART_S(part), GV, t);
+  const double ro = t[0];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!isfinite(ro)) {
+    if (lead) st->done = 2;
+    return;
+  }
+  const double beta = k == 0 ? 0.0 : ro / st->rz[(k + 1) & 1];
+  if (lead) {
+    st->rz[k & 1] = ro;
+    st->rzB = ro;
+    st->beta = beta;
+  }
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+    p[i] = k == 0 ? z[i] : z[i] + beta * p[i];
+}
+
+__global__ __launch_bounds__(WG) void k_ssor_upd(long n, double *__restrict__ x, double *__restrict__ r,
+                                                 const double *__restrict__ p, const double *__restrict__ q,
+                                                 ugk_state *st, double *part, unsigned GA)
+{
+  if (st->done) return;
+  const long k = st->cur;
+  double t[1];
+  reduce_prev<1>(PART_A(part), GA, t);
+  const double alpha = st->rzB / t[0];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!isfinite(alpha)) {
+    if (lead) st->done = 2;
+    return;
+  }
+  double v[4] = {0, 0, 0, 0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+    const double xi = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    x[i] = xi;
+    r[i] = ri;
+    v[1] += ri * ri;
+    v[2] += xi * xi;
+  }
+  block_partials<4>(v, PART_B2(part, (k + 1) & 1));
+  if (lead) {
+    st->alpha = alpha;
+    st->iter = k + 1;
+  }
+}
+
+// values of the colour-permuted matrix Kc from K: new block row I' is old
+// row perm[I'], its column J' old column perm[J'] (found by binary search)
+template <int BS>
+__global__ __launch_bounds__(WG) void k_ssor_permute_vals(SellView M, const double *__restrict__ vals, SellView Mc,
+                                                          double *valc, const int *__restrict__ perm)
+{
+  const long I = (long)blockIdx.x * WG + threadIdx.x;
+  if (I >= Mc.n_brows) return;
+  const long sl = I >> 6, ln = I & 63;
+  const long Io = perm[I];
+  const long slo = Io >> 6, lno = Io & 63;
+  const int L = Mc.rowlen[I];
+  for (int k = 0; k < L; k++) {
+    const long at = (Mc.soff[sl] + k) * 64 + ln;
+    const int Jo = perm[Mc.cols[at]];
+    const int ko = find_slot(M, Io, Jo);
+    const double *v = vals + (M.soff[slo] + ko) * 64 * BS * BS + lno;
+#pragma unroll
+    for (int q = 0; q < BS * BS; q++) valc[(Mc.soff[sl] + k) * 64 * BS * BS + q * 64 + ln] = ko >= 0 ? v[q * 64] : 0.0;
+  }
+}
+
+// dir 0: dst[I'] = src[perm[I']] (into colour order); dir 1: dst[perm[I']] = src[I']
+__global__ __launch_bounds__(WG) void k_ssor_permute_vec(long nb, int bs, const int *__restrict__ perm,
+                                                         const double *__restrict__ src, double *__restrict__ dst,
+                                                         int dir)
+{
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < nb * bs; i += (long)gridDim.x * WG) {
+    const long I = i / bs, c = i - I * bs, Io = perm[I];
+    if (dir == 0) dst[i] = src[Io * bs + c];
+    else dst[Io * bs + c] = src[i];
+  }
+}
+
 static SellView view(const ugk_mat *K)
 {
   SellView M;
@@ -2447,6 +2635,54 @@ long ugk_wall_clock_khz(void)
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0;
   return khz;
+}
+
+int ugk_ssor_permute_vals(const ugk_mat *K, const ugk_mat *Kc, const int *perm, void *s)
+{
+  const unsigned g = (unsigned)((Kc->n_brows + WG - 1) / WG);
+  if (K->bs == 3) k_ssor_permute_vals<3><<<g, WG, 0, S(s)>>>(view(K), K->d_vals, view(Kc), Kc->d_vals, perm);
+  else k_ssor_permute_vals<1><<<g, WG, 0, S(s)>>>(view(K), K->d_vals, view(Kc), Kc->d_vals, perm);
+  CK_LAUNCH("k_ssor_permute_vals");
+  return 0;
+}
+
+int ugk_ssor_permute_vec(long n_brows, int bs, const int *perm, const double *src, double *dst, int dir, void *s)
+{
+  k_ssor_permute_vec<<<vec_blocks(n_brows * bs), WG, 0, S(s)>>>(n_brows, bs, perm, src, dst, dir);
+  CK_LAUNCH("k_ssor_permute_vec");
+  return 0;
+}
+
+int ugk_ssor_iter(const ugk_mat *Kc, int ncol, const long *cb, long n, double *x, double *r, double *zz,
+                  double *z, double *p, double *q, const double *dinv, ugk_state *st, double *part, void *s)
+{
+  const unsigned GV = vec_blocks(n), GA = spmv_blocks(Kc);
+  const SellView M = view(Kc);
+  int c;
+  k_ssor_test<<<1, WG, 0, S(s)>>>(st, part, spmv_blocks(Kc), GV);
+  CK_LAUNCH("k_ssor_test");
+  for (c = 0; c < ncol; c++) {
+    const long r0 = cb[c], r1 = cb[c + 1];
+    const unsigned g = (unsigned)(((r1 - 1) / 64 - r0 / 64 + 1 + NW - 1) / NW);
+    if (r1 <= r0) continue;
+    if (Kc->bs == 3) k_sgs<3, 1><<<g, WG, 0, S(s)>>>(M, Kc->d_vals, r0, r1, r, dinv, zz, st);
+    else k_sgs<1, 1><<<g, WG, 0, S(s)>>>(M, Kc->d_vals, r0, r1, r, dinv, zz, st);
+  }
+  for (c = ncol - 1; c >= 0; c--) {
+    const long r0 = cb[c], r1 = cb[c + 1];
+    const unsigned g = (unsigned)(((r1 - 1) / 64 - r0 / 64 + 1 + NW - 1) / NW);
+    if (r1 <= r0) continue;
+    if (Kc->bs == 3) k_sgs<3, 0><<<g, WG, 0, S(s)>>>(M, Kc->d_vals, r0, r1, zz, dinv, z, st);
+    else k_sgs<1, 0><<<g, WG, 0, S(s)>>>(M, Kc->d_vals, r0, r1, zz, dinv, z, st);
+  }
+  CK_LAUNCH("k_sgs");
+  k_ssor_dot<<<GV, WG, 0, S(s)>>>(n, r, z, st, part);
+  k_ssor_p<<<GV, WG, 0, S(s)>>>(n, z, p, st, part, GV);
+  if (Kc->bs == 3) k_pcg_q<3><<<GA, WG, 0, S(s)>>>(M, Kc->d_vals, p, q, st, part);
+  else k_pcg_q<1><<<GA, WG, 0, S(s)>>>(M, Kc->d_vals, p, q, st, part);
+  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA);
+  CK_LAUNCH("k_ssor_upd");
+  return 0;
 }
 
 int ugk_fcg_start(const ugk_mat *K, long n, const double *b, const double *x, double *r, const double *dinv,

@@ -20,7 +20,7 @@ P_I = C.POINTER(C.c_int)
 P_F = C.POINTER(C.c_float)
 
 AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
-SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4}
+SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4, "ssor": 5}
 EXEC = {"streamed": 0, "persistent": 1, "fused": 2, "classic": 3}
 
 
@@ -125,6 +125,8 @@ def lib():
             "femEqsCGwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
                                      C.c_double, C.c_long]),
             "femEqsBiCCSwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
+                                        C.c_double, C.c_long]),
+            "femEqsCGwSSOR": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
                                         C.c_double, C.c_long]),
             "femMatVecMultBig": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector)]),
             "femVecVecMultBig": (C.c_double, [C.POINTER(tVector), C.POINTER(tVector)]),
@@ -556,6 +558,10 @@ def femEqsCGwJ(K: RowSlotMatrix, b: Vec, x: Vec, eps: float, maxit: int) -> int:
 
 def femEqsBiCCSwJ(K: RowSlotMatrix, b: Vec, x: Vec, eps: float, maxit: int) -> int:
     return lib().femEqsBiCCSwJ(C.byref(K.s), C.byref(b.s), C.byref(x.s), eps, maxit)
+
+
+def femEqsCGwSSOR(K: RowSlotMatrix, b: Vec, x: Vec, eps: float, maxit: int) -> int:
+    return lib().femEqsCGwSSOR(C.byref(K.s), C.byref(b.s), C.byref(x.s), eps, maxit)
 
 
 def set_stop_rule(rule: int) -> None:
