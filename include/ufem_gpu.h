@@ -103,6 +103,12 @@ int  ufem_sys_add_loads(ufem_sys *s);
 int  ufem_sys_add_disps(ufem_sys *s);
 int  ufem_sys_solve(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info);
 int  ufem_sys_reactions(ufem_sys *s, double *R, long *rnode, long *rdof);
+/* strains and stresses at the npts^3 Gauss points (npts 2 or 3) of every
+ * element for the current u, the results pass of fem_fill_K(AF_YES)
+ * (fem/fem_e009.c:417-535, material type 1): res [n_elems][npts^3][12] =
+ * sigma x, y, z, yz, zx, xy then epsilon x, y, z, yz, zx, xy (engineering
+ * shears), point (i npts + j) npts + k along the local axes */
+int  ufem_sys_element_results(ufem_sys *s, int npts, double *res);
 int  ufem_sys_get_u(ufem_sys *s, double *u);
 int  ufem_sys_set_u(ufem_sys *s, const double *u);
 int  ufem_sys_get_F(ufem_sys *s, double *F);

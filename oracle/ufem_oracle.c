@@ -275,9 +275,11 @@ int uo_element_stiffness(const double xyz[24], double E, double nu, int npts, do
 
 /* F_e = sum_gp detJ w H^T (D H u_e): e009_stiff Mode=AF_YES with the default
  * femTangentMatrix = AF_YES (fem/fem_sol.c:35) and a material-1 element, the
- * "else mtype" branch fem/fem_e009.c:503-535 and :571-577. */
+ * "else mtype" branch fem/fem_e009.c:503-535 and :571-577.  gp (optional):
+ * the results that branch adds at each point (femAddEResVal, :521-535),
+ * [npts^3][12] = sigma (RES_SX .. RES_SXY) then epsilon (RES_EX .. RES_EXY). */
 static int element_forces(const double xyz[24], double E, double nu, int npts,
-                          const double ue[24], double Fe[24])
+                          const double ue[24], double Fe[24], double *gp)
 {
   double pt[3], wt[3], H[6][24], D[6][6], eps[6], dsig[6], sig[6], F0[24], mult, val;
   int i, j, k, ii, jj, rv;
@@ -305,6 +307,10 @@ static int element_forces(const double xyz[24], double E, double nu, int npts,
           sig[ii] = val;
         }
         (void)dsig;
+        if (gp) {
+          double *o = gp + 12L * ((i * npts + j) * npts + k);
+          for (ii = 0; ii < 6; ii++) { o[ii] = sig[ii]; o[6 + ii] = eps[ii]; }
+        }
         for (ii = 0; ii < 24; ii++) {         /* F_0 = H^T sigma */
           val = 0.0;
           for (jj = 0; jj < 6; jj++) val += sig[jj] * H[jj][ii];
@@ -955,7 +961,7 @@ int uo_reactions(uo_sys *s, int npts)
     elem_xyz(&s->m, e, xyz);
     elem_locvec(s, e, lf);
     for (k = 0; k < 24; k++) ue[k] = lf[k] ? 0.0 + s->u[lf[k] - 1] : 0.0 + 0.0;
-    rv = element_forces(xyz, s->m.E[e], s->m.nu[e], npts, ue, Fe);
+    rv = element_forces(xyz, s->m.E[e], s->m.nu[e], npts, ue, Fe, NULL);
     if (rv != UO_OK) return rv;
     for (k = 0; k < 24; k++)
       if (lf[k]) s->F[lf[k] - 1] += Fe[k];
@@ -966,6 +972,23 @@ int uo_reactions(uo_sys *s, int npts)
     s->rval[i] += s->F[p - 1];
   }
   s->t[4] = now_s() - t0;
+  return UO_OK;
+}
+
+/* the element results of fem_fill_K(AF_YES) for the current u: res
+ * [n_elems][npts^3][12] (see element_forces) */
+int uo_element_results(const uo_sys *s, int npts, double *res)
+{
+  double xyz[24], ue[24], Fe[24];
+  long lf[24], e;
+  int k, rv;
+  for (e = 0; e < s->m.n_elems; e++) {
+    elem_xyz(&s->m, e, xyz);
+    elem_locvec(s, e, lf);
+    for (k = 0; k < 24; k++) ue[k] = lf[k] ? 0.0 + s->u[lf[k] - 1] : 0.0 + 0.0;
+    rv = element_forces(xyz, s->m.E[e], s->m.nu[e], npts, ue, Fe, res + 12L * npts * npts * npts * e);
+    if (rv != UO_OK) return rv;
+  }
   return UO_OK;
 }
 

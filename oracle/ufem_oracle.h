@@ -54,6 +54,9 @@ int  uo_add_disps(uo_sys *s);                   /* fem_add_disps(AF_YES, 0)     
 int  uo_solve(uo_sys *s, int solver, double eps, long maxit, int rule,
               long *iters, double *relres);
 int  uo_reactions(uo_sys *s, int npts);         /* fem_fill_K(AF_YES) + reactions */
+/* element results of fem_fill_K(AF_YES) for the current u:
+ * res [n_elems][npts^3][12] = sigma x..xy, epsilon x..xy at each point */
+int  uo_element_results(const uo_sys *s, int npts, double *res);
 
 /* accessors: copy out state (arrays sized by the caller) */
 long uo_K_nnz(const uo_sys *s);

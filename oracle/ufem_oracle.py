@@ -48,6 +48,7 @@ def lib():
         L.uo_add_disps.argtypes = [C.c_void_p]
         L.uo_solve.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_long, C.c_int, P_L, P_D]
         L.uo_reactions.argtypes = [C.c_void_p, C.c_int]
+        L.uo_element_results.argtypes = [C.c_void_p, C.c_int, P_D]
         L.uo_K_nnz.restype = C.c_long
         L.uo_K_nnz.argtypes = [C.c_void_p]
         L.uo_get_K.argtypes = [C.c_void_p, P_L, P_L, P_D]
@@ -140,6 +141,12 @@ class OracleSystem:
 
     def reactions(self, npts=3):
         self._chk(lib().uo_reactions(self.h, npts), "reactions")
+
+    def element_results(self, npts=3):
+        """[n_elems, npts^3, 12]: sigma x..xy, epsilon x..xy at each Gauss point"""
+        out = np.zeros((self._m.n_elems, npts ** 3, 12))
+        self._chk(lib().uo_element_results(self.h, npts, _pd(out)), "element results")
+        return out
 
     def K_rowslot(self):
         nnz = lib().uo_K_nnz(self.h)

@@ -185,6 +185,28 @@ def test_native_bicgstab(golden):
     assert rel(g.u(), golden["het8_cg/u"]) <= 1e-8
 
 
+@pytest.mark.parametrize("mname,npts", [("c1", 3), ("c1", 2), ("dist", 3), ("het8", 3)])
+def test_element_results_vs_oracle(mname, npts):
+    """Strains and stresses at the Gauss points (results pass of
+    fem_fill_K(AF_YES)): on the oracle's displacements they agree with the
+    oracle to 1e-12 of the largest value; from the GPU's own solve, to the
+    displacement parity."""
+    mesh = MESHES[mname]()
+    o = oracle_tight(mesh, npts=npts)
+    ref = o.element_results(npts)
+    g = G.GpuSystem(mesh)
+    g.assemble(npts)
+    g.set_u(o.u())
+    res = g.element_results(npts)
+    for sl in (slice(0, 6), slice(6, 12)):
+        scale = np.abs(ref[:, :, sl]).max()
+        assert np.abs(res[:, :, sl] - ref[:, :, sl]).max() <= 1e-12 * scale
+    info = g.solve("pcg", eps=1e-14, rule=1)
+    assert info["rv"] == 0
+    own = g.element_results(npts)
+    assert rel(own, ref) <= 1e-8
+
+
 def oracle_ssor(mesh, eps, npts=3):
     s = uo.OracleSystem(mesh)
     s.assemble(npts)

@@ -513,6 +513,25 @@ int ufem_sys_reactions(ufem_sys *s, double *R, long *rnode, long *rdof)
   return AF_OK;
 }
 
+/* strains and stresses at the Gauss points (results pass of fem_fill_K(AF_YES),
+ * fem/fem_e009.c:417-535) for the current u: res [n_elems][npts^3][12] */
+int ufem_sys_element_results(ufem_sys *s, int npts, double *res)
+{
+  const long ng = (long)npts * npts * npts;
+  const size_t bytes = sizeof(double) * (size_t)(s->n_elems * ng * 12);
+  double *d = NULL;
+  int rv = AF_OK;
+  if (npts != 2 && npts != 3) { ufem_set_error("npts must be 2 or 3"); return AF_ERR_VAL; }
+  d = (double *)ugk_malloc(bytes);
+  if (!d) return gpu_fail("results buffer");
+  if (ugk_elem_results(s->n_elems, npts, s->d_x, s->d_y, s->d_z, s->d_enodes, s->d_E, s->d_nu, s->d_u, d,
+                       s->w.st, s->stream) ||
+      ugk_d2h(res, d, bytes, s->stream))
+    rv = gpu_fail("element results");
+  ugk_free(d);
+  return rv;
+}
+
 int ufem_sys_get_u(ufem_sys *s, double *u)
 {
   if (ugk_d2h(u, s->d_u, sizeof(double) * (size_t)s->n, s->stream)) return gpu_fail("u read");

@@ -108,6 +108,13 @@ void  ugk_graph_destroy(void *graph);
  * d_Ke (row-major), or added into the SELL-3 matrix for the elements listed in
  * d_elist (one colour: no two share a node).  npts = 2 or 3 Gauss points per
  * axis.  Sets st->err = 1 on a non-positive Jacobian. */
+/* strains and stresses at the npts^3 Gauss points of every element of a
+ * brick mesh for the displacement vector d_u (3 per node): d_out
+ * [ne][npts^3][12], sigma x, y, z, yz, zx, xy then epsilon (engineering
+ * shears), point (i npts + j) npts + k along xi, eta, zeta */
+int ugk_elem_results(long ne, int npts, const double *d_x, const double *d_y, const double *d_z,
+                     const int *d_enodes, const double *d_E, const double *d_nu, const double *d_u, double *d_out,
+                     ugk_state *d_st, void *stream);
 int ugk_elem_stiff_dense(long n_elems, const double *d_xyz, const int *d_enodes,
                          const double *d_E, const double *d_nu, int npts,
                          double *d_Ke, ugk_state *d_st, void *stream);

@@ -281,6 +281,58 @@ __device__ __forceinline__ void corner(int n, double &xs, double &ys, double &zs
   zs = (n & 4) ? 1.0 : -1.0;
 }
 
+// physical shape-function gradients g[gp][n][a] = dN_n/dx_a and weights
+// w[gp] = detJ w_i w_j w_k of one element (xyz: 8 corners x 3, all in LDS),
+// spread over the 64 lanes of its wave; detJ <= 0 flags st->err = 1
+template <int NPTS>
+__device__ __forceinline__ void elem_gradients(const double *xyz, double *g, double *w, int lane, ugk_state *st)
+{
+  constexpr int NGP = NPTS * NPTS * NPTS;
+  for (int t = lane; t < NGP * 8; t += 64) {
+    const int gp = t >> 3, n = t & 7;
+    const int gi = gp / (NPTS * NPTS), gj = (gp / NPTS) % NPTS, gk = gp % NPTS;
+    const double xi = Gauss<NPTS>::pt(gi), eta = Gauss<NPTS>::pt(gj), zeta = Gauss<NPTS>::pt(gk);
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double dn[3];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      double xs, ys, zs;
+      corner(m, xs, ys, zs);
+      const double d0 = 0.125 * xs * (1.0 + ys * eta) * (1.0 + zs * zeta);
+      const double d1 = 0.125 * ys * (1.0 + xs * xi) * (1.0 + zs * zeta);
+      const double d2 = 0.125 * zs * (1.0 + xs * xi) * (1.0 + ys * eta);
+      if (m == n) { dn[0] = d0; dn[1] = d1; dn[2] = d2; }
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        const double xb = xyz[3 * m + b];
+        J[0][b] += d0 * xb;
+        J[1][b] += d1 * xb;
+        J[2][b] += d2 * xb;
+      }
+    }
+    // J[a][b] = dx_b / dxi_a ; inverse by adjugate
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    if (!(det > 0.0)) st->err = 1;
+    const double id = 1.0 / det;
+    double Ji[3][3];
+    Ji[0][0] = c00 * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][0] = c01 * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][0] = c02 * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+#pragma unroll
+    for (int a = 0; a < 3; a++) g[(gp * 8 + n) * 3 + a] = Ji[a][0] * dn[0] + Ji[a][1] * dn[1] + Ji[a][2] * dn[2];
+    if (n == 0) w[gp] = det * Gauss<NPTS>::wt(gi) * Gauss<NPTS>::wt(gj) * Gauss<NPTS>::wt(gk);
+  }
+}
+
 template <int NPTS, bool DENSE>
 __global__ __launch_bounds__(WG) void k_elem_stiff(long n_list, const int *__restrict__ elist,
                                                    const double *__restrict__ cx,
@@ -311,52 +363,7 @@ __global__ __launch_bounds__(WG) void k_elem_stiff(long n_list, const int *__res
     }
   }
   __syncthreads();
-  if (active) {
-    for (int t = lane; t < NGP * 8; t += 64) {
-      const int gp = t >> 3, n = t & 7;
-      const int gi = gp / (NPTS * NPTS), gj = (gp / NPTS) % NPTS, gk = gp % NPTS;
-      const double xi = Gauss<NPTS>::pt(gi), eta = Gauss<NPTS>::pt(gj), zeta = Gauss<NPTS>::pt(gk);
-      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      double dn[3];
-#pragma unroll
-      for (int m = 0; m < 8; m++) {
-        double xs, ys, zs;
-        corner(m, xs, ys, zs);
-        const double d0 = 0.125 * xs * (1.0 + ys * eta) * (1.0 + zs * zeta);
-        const double d1 = 0.125 * ys * (1.0 + xs * xi) * (1.0 + zs * zeta);
-        const double d2 = 0.125 * zs * (1.0 + xs * xi) * (1.0 + ys * eta);
-        if (m == n) { dn[0] = d0; dn[1] = d1; dn[2] = d2; }
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-          const double xb = s_xyz[wave][3 * m + b];
-          J[0][b] += d0 * xb;
-          J[1][b] += d1 * xb;
-          J[2][b] += d2 * xb;
-        }
-      }
-      // J[a][b] = dx_b / dxi_a ; inverse by adjugate
-      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-      if (!(det > 0.0)) st->err = 1;
-      const double id = 1.0 / det;
-      double Ji[3][3];
-      Ji[0][0] = c00 * id;
-      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-      Ji[1][0] = c01 * id;
-      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-      Ji[2][0] = c02 * id;
-      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-        s_g[wave][gp][n][a] = Ji[a][0] * dn[0] + Ji[a][1] * dn[1] + Ji[a][2] * dn[2];
-      if (n == 0) s_w[wave][gp] = det * Gauss<NPTS>::wt(gi) * Gauss<NPTS>::wt(gj) * Gauss<NPTS>::wt(gk);
-    }
-  }
+  if (active) elem_gradients<NPTS>(s_xyz[wave], &s_g[wave][0][0][0], s_w[wave], lane, st);
   __syncthreads();
   if (!active) return;
   const int a = lane >> 3, b = lane & 7;
@@ -393,6 +400,72 @@ __global__ __launch_bounds__(WG) void k_elem_stiff(long n_list, const int *__res
     double *v = vals + ((M.soff[I >> 6] + s) * 9) * 64 + (I & 63);
 #pragma unroll
     for (int q = 0; q < 9; q++) v[q * 64] += k[q];
+  }
+}
+
+// strains and stresses at the Gauss points of every element (the results
+// pass of e009_stiff Mode=AF_YES for a linear material, fem/fem_e009.c:417-535):
+// epsilon = H u_e in the order x, y, z, yz, zx, xy (engineering shears),
+// sigma = D epsilon with the isotropic D of fem/fem_pl3d.c:32-58; out
+// [e][gp][12] = sigma (6) then epsilon (6), the order of the element's
+// repeating results (fem/fem_e009.c:881-883); gp = (i NPTS + j) NPTS + k
+template <int NPTS>
+__global__ __launch_bounds__(WG) void k_elem_results(long ne, const double *__restrict__ cx,
+                                                     const double *__restrict__ cy,
+                                                     const double *__restrict__ cz,
+                                                     const int *__restrict__ enodes,
+                                                     const double *__restrict__ Ev,
+                                                     const double *__restrict__ nuv,
+                                                     const double *__restrict__ u, double *__restrict__ out,
+                                                     ugk_state *st)
+{
+  constexpr int NGP = NPTS * NPTS * NPTS;
+  __shared__ double s_xyz[NW][24];
+  __shared__ double s_u[NW][24];
+  __shared__ double s_g[NW][NGP][8][3];
+  __shared__ double s_w[NW][NGP];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long e = (long)blockIdx.x * NW + wave;
+  const bool active = e < ne;
+  if (active && lane < 24) {
+    const int node = enodes[e * 8 + lane / 3], c = lane % 3;
+    s_xyz[wave][lane] = c == 0 ? cx[node] : (c == 1 ? cy[node] : cz[node]);
+    s_u[wave][lane] = u[(long)node * 3 + c];
+  }
+  __syncthreads();
+  if (active) elem_gradients<NPTS>(s_xyz[wave], &s_g[wave][0][0][0], s_w[wave], lane, st);
+  __syncthreads();
+  if (!active) return;
+  const double E = Ev[e], nu = nuv[e];
+  const double d1 = E * (1.0 - nu) / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double d2 = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double d3 = E / (2.0 * (1.0 + nu));
+  for (int gp = lane; gp < NGP; gp += 64) {
+    double ex = 0, ey = 0, ez = 0, gyz = 0, gzx = 0, gxy = 0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const double g0 = s_g[wave][gp][n][0], g1 = s_g[wave][gp][n][1], g2 = s_g[wave][gp][n][2];
+      const double u0 = s_u[wave][3 * n], u1 = s_u[wave][3 * n + 1], u2 = s_u[wave][3 * n + 2];
+      ex += g0 * u0;
+      ey += g1 * u1;
+      ez += g2 * u2;
+      gyz += g2 * u1 + g1 * u2;
+      gzx += g0 * u2 + g2 * u0;
+      gxy += g1 * u0 + g0 * u1;
+    }
+    double *o = out + (e * NGP + gp) * 12;
+    o[0] = d1 * ex + d2 * (ey + ez);
+    o[1] = d1 * ey + d2 * (ex + ez);
+    o[2] = d1 * ez + d2 * (ex + ey);
+    o[3] = d3 * gyz;
+    o[4] = d3 * gzx;
+    o[5] = d3 * gxy;
+    o[6] = ex;
+    o[7] = ey;
+    o[8] = ez;
+    o[9] = gyz;
+    o[10] = gzx;
+    o[11] = gxy;
   }
 }
 
@@ -2273,6 +2346,19 @@ void ugk_graph_destroy(void *g)
 }
 
 // ---- assembly -------------------------------------------------------------
+int ugk_elem_results(long ne, int npts, const double *d_x, const double *d_y, const double *d_z,
+                     const int *d_enodes, const double *d_E, const double *d_nu, const double *d_u, double *d_out,
+                     ugk_state *d_st, void *s)
+{
+  const unsigned nb = (unsigned)((ne + NW - 1) / NW);
+  if (ne <= 0) return 0;
+  if (npts == 2) k_elem_results<2><<<nb, WG, 0, S(s)>>>(ne, d_x, d_y, d_z, d_enodes, d_E, d_nu, d_u, d_out, d_st);
+  else if (npts == 3) k_elem_results<3><<<nb, WG, 0, S(s)>>>(ne, d_x, d_y, d_z, d_enodes, d_E, d_nu, d_u, d_out, d_st);
+  else { snprintf(g_err, sizeof(g_err), "npts must be 2 or 3"); return -1; }
+  CK_LAUNCH("k_elem_results");
+  return 0;
+}
+
 int ugk_elem_stiff_dense(long n_elems, const double *d_xyz, const int *d_enodes, const double *d_E,
                          const double *d_nu, int npts, double *d_Ke, ugk_state *d_st, void *s)
 {

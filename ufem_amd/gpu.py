@@ -99,6 +99,7 @@ def lib():
             "ufem_sys_add_disps": (C.c_int, [C.c_void_p]),
             "ufem_sys_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
             "ufem_sys_reactions": (C.c_int, [C.c_void_p, P_D, P_L, P_L]),
+            "ufem_sys_element_results": (C.c_int, [C.c_void_p, C.c_int, P_D]),
             "ufem_sys_get_u": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_set_u": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_get_F": (C.c_int, [C.c_void_p, P_D]),
@@ -189,6 +190,7 @@ class GpuSystem:
         if not self.h:
             raise RuntimeError("ufem_sys_create: " + last_error())
         self.n = lib().ufem_sys_ndof(self.h)
+        self.n_elems = mesh.n_elems
         self.nreact = lib().ufem_sys_nreact(self.h)
         self.ncolors = lib().ufem_sys_ncolors(self.h)
 
@@ -248,6 +250,12 @@ class GpuSystem:
     def u(self):
         out = np.zeros(self.n)
         self._chk(lib().ufem_sys_get_u(self.h, _pd(out)), "get_u")
+        return out
+
+    def element_results(self, npts=3):
+        """[n_elems, npts^3, 12]: sigma x..xy, epsilon x..xy at each Gauss point"""
+        out = np.zeros((self.n_elems, npts ** 3, 12))
+        self._chk(lib().ufem_sys_element_results(self.h, npts, _pd(out)), "element_results")
         return out
 
     def set_u(self, u):
