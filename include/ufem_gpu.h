@@ -109,6 +109,12 @@ int  ufem_sys_reactions(ufem_sys *s, double *R, long *rnode, long *rdof);
  * sigma x, y, z, yz, zx, xy then epsilon x, y, z, yz, zx, xy (engineering
  * shears), point (i npts + j) npts + k along the local axes */
 int  ufem_sys_element_results(ufem_sys *s, int npts, double *res);
+/* the reference's result file (femWriteRes, fem/fem_io.c:648-734) for the
+ * current u, with the 27-point element results in the reference's element
+ * result layout; sol_id, dep_on and sim_num fill the header line (the input
+ * file's solution ID and dependency, the simulation number); node and
+ * element IDs are written as 1-based positions */
+int  ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep_on, double sim_num);
 int  ufem_sys_get_u(ufem_sys *s, double *u);
 int  ufem_sys_set_u(ufem_sys *s, const double *u);
 int  ufem_sys_get_F(ufem_sys *s, double *F);

@@ -23,6 +23,7 @@
  *   femEqsCGwJ / femEqsBiCCSwJ  fem/fem_eqs.c:88 / :262
  *                                               -> dump u, SpMV call count
  *   fem_fill_K(AF_YES)    fem/fem_sol.c:897     -> dump reactions resRval
+ *   femWriteRes           fem/fem_io.c:648      -> outdir/result.res
  *
  * SpMV calls made by the solvers are counted through the linker's
  * --wrap=femMatVecMultBig (fem_eqs.o references the symbol defined in
@@ -215,6 +216,12 @@ int main(int argc, char **argv)
     dump_raw("R", "f64", resRval, sizeof(double), (size_t)resRLen);
     dump_raw("Rnode", "i64", resRnode, sizeof(long), (size_t)resRLen);
     dump_raw("Rdof", "i64", resRdof, sizeof(long), (size_t)resRLen);
+  }
+  {
+    /* the reference's own result file (fem/fem_io.c:648-734) */
+    char path[4096];
+    snprintf(path, sizeof(path), "%s/result.res", g_out);
+    if ((rv = femWriteRes(path)) != AF_OK) { fprintf(stderr, "writeres %d\n", rv); return 23; }
   }
 
   printf("{\"n\": %ld, \"solver\": \"%s\", \"rv\": %d, \"spmv_calls\": %ld, "

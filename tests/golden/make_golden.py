@@ -138,7 +138,25 @@ def cases():
     ]
 
 
+def write_res_golden(tag="het4_cg"):
+    """The reference's own result file (femWriteRes, fem/fem_io.c:648-734) for
+    one golden case, kept as text next to the vectors."""
+    mesh, kw = {t: (m, k) for t, m, k in cases()}[tag]
+    with tempfile.TemporaryDirectory() as td:
+        from ufem_amd.mesh import write_fem_input
+        fin = os.path.join(td, "in.fem")
+        write_fem_input(mesh, fin)
+        cmd = [HARNESS, "-i", fin, "-o", td, "-solver", kw.get("solver", "cg"), "-eps", repr(kw.get("eps", 1e-21))]
+        subprocess.run(cmd, check=True, capture_output=True, text=True)
+        with open(os.path.join(td, "result.res")) as f, open(os.path.join(HERE, f"{tag}.res"), "w") as g:
+            g.write(f.read())
+    print("wrote", os.path.join(HERE, f"{tag}.res"))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--res":
+        write_res_golden()
+        return
     if not os.path.exists(HARNESS):
         raise SystemExit(f"missing {HARNESS}: run `make -C oracle ref` first")
     out = {}

@@ -5,6 +5,8 @@ Bars: element matrices and assembled K within 1e-12 of max|K| (bit-exact
 sparsity pattern), displacements within 1e-8 relative, reactions within 1e-6
 relative (FP64 plain path)."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -205,6 +207,27 @@ def test_element_results_vs_oracle(mname, npts):
     assert info["rv"] == 0
     own = g.element_results(npts)
     assert rel(own, ref) <= 1e-8
+
+
+def test_result_file_matches_reference(golden, tmp_path):
+    """ufem_sys_write_results on the reference's u for the heterogeneous 4x2x2
+    cantilever against the reference's own femWriteRes file: the same layout
+    and integers, and every number equal to the 7 digits printed."""
+    from tests.resfile import close, read_res
+    mesh = MESHES["het4"]()
+    g = G.GpuSystem(mesh)
+    g.assemble(3)
+    g.set_u(golden["het4_cg/u"])
+    path = str(tmp_path / "het4.res")
+    g.write_results(path)
+    mine = read_res(path)
+    ref = read_res(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "het4_cg.res"))
+    for k in ("header", "n_act", "n_dof_len", "n_nodes", "n_elems", "n_res"):
+        assert mine[k] == ref[k], k
+    for k in ("node_id", "dof_map", "r_node", "r_id", "r_dof", "elem_id", "res_from"):
+        assert np.array_equal(mine[k], ref[k]), k
+    for k in ("u", "R", "res"):
+        assert close(mine[k], ref[k]), (k, np.abs(mine[k] - ref[k]).max())
 
 
 def oracle_ssor(mesh, eps, npts=3):

@@ -532,6 +532,65 @@ int ufem_sys_element_results(ufem_sys *s, int npts, double *res)
   return rv;
 }
 
+/* femWriteRes (fem/fem_io.c:648-734): the reference's result file for the
+ * current u -- solution header, node IDs, the DOF map (KNOWN_DOFS = 7 per
+ * node, ux, uy, uz numbered), u, reactions, and the element results of the
+ * 27-point results pass in the element's result layout: 20 repeating values
+ * per point (sigma, epsilon, 6 plastic strains, 2 more; zero past epsilon
+ * for a linear material) in a field of 14 x 27 values per element
+ * (e009_rvals, fem/fem_e009.c:825-828), so that only the first 378 values,
+ * points 1-18 and most of 19, are kept (fem/fem_accs.c:396-483).  Node and
+ * element IDs are 1-based positions. */
+#define WR_KNOWN_DOFS 7
+#define WR_RES_RP 20
+#define WR_ELEM_VALS (14 * 27)
+int ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep_on, double sim_num)
+{
+  const long nn = s->n_nodes, ne = s->n_elems, n = s->n, nr = s->n_react;
+  double *u = NULL, *R = NULL, *res = NULL;
+  long *rn = NULL, *rd = NULL, i, e;
+  FILE *f = NULL;
+  int rv = AF_ERR_MEM, d;
+  u = (double *)malloc(sizeof(double) * (size_t)n);
+  R = (double *)malloc(sizeof(double) * (size_t)(nr > 0 ? nr : 1));
+  rn = (long *)malloc(sizeof(long) * (size_t)(nr > 0 ? nr : 1));
+  rd = (long *)malloc(sizeof(long) * (size_t)(nr > 0 ? nr : 1));
+  res = (double *)malloc(sizeof(double) * (size_t)(ne * 27 * 12));
+  if (!u || !R || !rn || !rd || !res) goto out;
+  if ((rv = ufem_sys_get_u(s, u)) != AF_OK) goto out;
+  if ((rv = ufem_sys_reactions(s, R, rn, rd)) != AF_OK) goto out;
+  if ((rv = ufem_sys_element_results(s, 3, res)) != AF_OK) goto out;
+  if (!(f = fopen(fname, "w"))) {
+    ufem_set_error("cannot write %s", fname);
+    rv = AF_ERR_IO;
+    goto out;
+  }
+  fprintf(f, "S\n %li %li %e \n", sol_id, dep_on, sim_num);
+  fprintf(f, " %li \n %li \n %li\n", n, (long)WR_KNOWN_DOFS * nn, nn);
+  for (i = 0; i < nn; i++) fprintf(f, " %li\n", i + 1);
+  for (i = 0; i < nn; i++)
+    for (d = 0; d < WR_KNOWN_DOFS; d++) fprintf(f, " %li\n", d < 3 ? 3 * i + d + 1 : 0L);
+  fprintf(f, " %li\n", n);
+  for (i = 0; i < n; i++) fprintf(f, " %e \n", u[i]);
+  fprintf(f, " %li \n", nr);
+  for (i = 0; i < nr; i++) fprintf(f, " %li %li %li %e\n", rn[i], rn[i] + 1, rd[i], R[i]);
+  fprintf(f, " %li %li\n", ne, (long)WR_ELEM_VALS * ne);
+  for (e = 0; e < ne; e++) fprintf(f, " %li \n", e + 1);
+  for (e = 0; e < ne; e++) fprintf(f, " %li \n", (long)WR_ELEM_VALS * e);
+  for (e = 0; e < ne; e++)
+    for (i = 0; i < WR_ELEM_VALS; i++) {
+      const long gp = i / WR_RES_RP, t = i % WR_RES_RP;
+      fprintf(f, " %e \n", t < 12 ? res[(e * 27 + gp) * 12 + t] : 0.0);
+    }
+  rv = fclose(f) == 0 ? AF_OK : AF_ERR_IO;
+  f = NULL;
+  if (rv != AF_OK) ufem_set_error("cannot close %s", fname);
+out:
+  if (f) fclose(f);
+  free(u); free(R); free(rn); free(rd); free(res);
+  return rv;
+}
+
 int ufem_sys_get_u(ufem_sys *s, double *u)
 {
   if (ugk_d2h(u, s->d_u, sizeof(double) * (size_t)s->n, s->stream)) return gpu_fail("u read");

@@ -100,6 +100,7 @@ def lib():
             "ufem_sys_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
             "ufem_sys_reactions": (C.c_int, [C.c_void_p, P_D, P_L, P_L]),
             "ufem_sys_element_results": (C.c_int, [C.c_void_p, C.c_int, P_D]),
+            "ufem_sys_write_results": (C.c_int, [C.c_void_p, C.c_char_p, C.c_long, C.c_long, C.c_double]),
             "ufem_sys_get_u": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_set_u": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_get_F": (C.c_int, [C.c_void_p, P_D]),
@@ -257,6 +258,10 @@ class GpuSystem:
         out = np.zeros((self.n_elems, npts ** 3, 12))
         self._chk(lib().ufem_sys_element_results(self.h, npts, _pd(out)), "element_results")
         return out
+
+    def write_results(self, path, sol_id=1, dep_on=-1, sim_num=0.0):
+        """the reference's result file (femWriteRes format) for the current u"""
+        self._chk(lib().ufem_sys_write_results(self.h, path.encode(), sol_id, dep_on, sim_num), "write_results")
 
     def set_u(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
