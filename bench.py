@@ -366,6 +366,14 @@ def main():
     log("roofline", roofline)
     log("spmv", spmv)
 
+    # results pass (strains and stresses at the 27 Gauss points of every
+    # element, fem_fill_K(AF_YES)) on the solved u, written to HBM
+    r_ms = g.time_results(3, reps=10)
+    r_bytes = mesh.n_elems * 27 * 12 * 8.0
+    results = {"ms": round(r_ms, 3), "elements_per_s": round(mesh.n_elems / r_ms * 1e3, 1),
+               "write_GBps": round(r_bytes / (r_ms * 1e-3) / 1e9, 1)}
+    log("results pass", results)
+
     mixed = None
     if not args.no_mixed:
         g.solve(solver="mixed", eps=1e-8, rule=1, maxit=20000)      # warm-up
@@ -448,6 +456,7 @@ def main():
         "mixed": mixed,
         "mixed_cov06": mixed_cov06,
         "ssor": ssor,
+        "results_pass": results,
         "learned_mixed_1m": learned_mixed,
         "learned": learned,
         "batch": batch,

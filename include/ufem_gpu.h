@@ -109,6 +109,9 @@ int  ufem_sys_reactions(ufem_sys *s, double *R, long *rnode, long *rdof);
  * sigma x, y, z, yz, zx, xy then epsilon x, y, z, yz, zx, xy (engineering
  * shears), point (i npts + j) npts + k along the local axes */
 int  ufem_sys_element_results(ufem_sys *s, int npts, double *res);
+/* average device time (ms) of the results pass, `reps` launches into a
+ * device buffer; negative on error */
+double ufem_sys_time_results(ufem_sys *s, int npts, int reps);
 /* the reference's result file (femWriteRes, fem/fem_io.c:648-734) for the
  * current u, with the 27-point element results in the reference's element
  * result layout; sol_id, dep_on and sim_num fill the header line (the input

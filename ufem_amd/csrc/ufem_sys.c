@@ -532,6 +532,30 @@ int ufem_sys_element_results(ufem_sys *s, int npts, double *res)
   return rv;
 }
 
+/* average device time (ms) of the results pass over `reps` launches, into a
+ * device buffer (for measurement); negative on error */
+double ufem_sys_time_results(ufem_sys *s, int npts, int reps)
+{
+  const long ng = (long)npts * npts * npts;
+  double *d, ms = -1.0;
+  int r;
+  if ((npts != 2 && npts != 3) || reps < 1) return -1.0;
+  d = (double *)ugk_malloc(sizeof(double) * (size_t)(s->n_elems * ng * 12));
+  if (!d) return -1.0;
+  if (!ugk_elem_results(s->n_elems, npts, s->d_x, s->d_y, s->d_z, s->d_enodes, s->d_E, s->d_nu, s->d_u, d,
+                        s->w.st, s->stream) &&
+      !ugk_event_record(s->ev[0], s->stream)) {
+    for (r = 0; r < reps; r++)
+      if (ugk_elem_results(s->n_elems, npts, s->d_x, s->d_y, s->d_z, s->d_enodes, s->d_E, s->d_nu, s->d_u, d,
+                           s->w.st, s->stream))
+        break;
+    if (r == reps && !ugk_event_record(s->ev[1], s->stream) && !ugk_sync(s->stream))
+      ms = ugk_event_ms(s->ev[0], s->ev[1]) / reps;
+  }
+  ugk_free(d);
+  return ms;
+}
+
 /* femWriteRes (fem/fem_io.c:648-734): the reference's result file for the
  * current u -- solution header, node IDs, the DOF map (KNOWN_DOFS = 7 per
  * node, ux, uy, uz numbered), u, reactions, and the element results of the

@@ -101,6 +101,7 @@ def lib():
             "ufem_sys_reactions": (C.c_int, [C.c_void_p, P_D, P_L, P_L]),
             "ufem_sys_element_results": (C.c_int, [C.c_void_p, C.c_int, P_D]),
             "ufem_sys_write_results": (C.c_int, [C.c_void_p, C.c_char_p, C.c_long, C.c_long, C.c_double]),
+            "ufem_sys_time_results": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
             "ufem_sys_get_u": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_set_u": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_get_F": (C.c_int, [C.c_void_p, P_D]),
@@ -258,6 +259,13 @@ class GpuSystem:
         out = np.zeros((self.n_elems, npts ** 3, 12))
         self._chk(lib().ufem_sys_element_results(self.h, npts, _pd(out)), "element_results")
         return out
+
+    def time_results(self, npts=3, reps=10):
+        """average device ms of the results pass"""
+        ms = lib().ufem_sys_time_results(self.h, npts, reps)
+        if ms < 0:
+            raise RuntimeError("time_results: " + last_error())
+        return ms
 
     def write_results(self, path, sol_id=1, dep_on=-1, sim_num=0.0):
         """the reference's result file (femWriteRes format) for the current u"""
