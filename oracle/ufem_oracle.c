@@ -674,11 +674,27 @@ int uo_add_disps(uo_sys *s)
   double t0 = now_s();
   for (i = 0; i < s->m.n_nl; i++) {
     long p;
-    double Fval;
+    double Fval, Val = s->m.nl_val[i];
     if (s->m.nl_type[i] != 1) continue;
-    if (s->m.nl_val[i] != 0.0) return UO_ERR_VAL;   /* prescribed non-zero: not on this path */
     p = s->dofmap[s->m.nl_node[i] * KNOWN_DOFS + s->m.nl_dir[i] - 1];
     if (p < 1) return UO_ERR_VAL;
+    if (Val != 0.0) {
+      /* femApplyDisp, Val != 0 (fem/fem_loads.c:210-264): F -= K(:, p) Val
+       * with the current K, row and column zeroed, u_p = Val and the
+       * diagonal F_p / Val (or 1 with F_p = Val); no reaction bookkeeping */
+      long k;
+      for (k = 1; k <= s->K.rows; k++) s->F[k - 1] += -1.0 * rs_get(&s->K, k, p) * Val;
+      rs_zero_row(&s->K, p);
+      rs_zero_col(&s->K, p);
+      s->u[p - 1] = Val;
+      if (fabs(s->F[p - 1] / Val) > FEM_ZERO) {
+        if (rs_putadd(&s->K, p, p, s->F[p - 1] / Val, 0) != UO_OK) return UO_ERR_BIG;
+      } else {
+        if (rs_putadd(&s->K, p, p, 1.0, 0) != UO_OK) return UO_ERR_BIG;
+        s->F[p - 1] = Val;
+      }
+      continue;
+    }
     Fval = s->F[p - 1];
     rs_zero_row(&s->K, p);
     rs_zero_col(&s->K, p);

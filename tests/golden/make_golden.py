@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from ufem_amd.mesh import cantilever, lognormal_E  # noqa: E402
+from ufem_amd.mesh import cantilever, lognormal_E, prescribed_cantilever  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ufem_ref_harness")
 
@@ -125,6 +125,7 @@ def cases():
     het4 = cantilever(4, 2, 2, E=lognormal_E(11, 16, cov=0.6))
     dist = cantilever(6, 3, 3, distort=0.25, seed=5, E=lognormal_E(3, 54, cov=0.3))
     cube = cantilever(1, 1, 1, L=1.0, B=1.0, H=1.0)
+    pd = prescribed_cantilever(12, 3, 3, E=lognormal_E(13, 108, cov=0.3))   # windows of the column zeroing stay inside K
     return [
         ("c1_cg", c1, dict(solver="cg", ke=(0, 319))),
         ("c1_cg_eps12", c1, dict(solver="cg", eps=1e-12)),
@@ -135,6 +136,7 @@ def cases():
         ("het4_cg", het4, dict(solver="cg", full_k=True)),
         ("dist_cg", dist, dict(solver="cg", full_k=True, ke=(0, 20, 53))),
         ("cube_cg", cube, dict(solver="cg", ke=(0,), full_k=True)),
+        ("pd_cg", pd, dict(solver="cg", eps=1e-12, full_k=True)),
     ]
 
 

@@ -230,6 +230,43 @@ def test_result_file_matches_reference(golden, tmp_path):
         assert close(mine[k], ref[k]), (k, np.abs(mine[k] - ref[k]).max())
 
 
+def test_prescribed_displacements_vs_eliminated_system():
+    """Non-zero prescribed displacements (some listed before the supports): the
+    loads of the free rows lose K(:, P) v in one SpMV and the supports zero
+    full rows and columns, so u solves the eliminated system (a dense solve
+    here) to 1e-8; the reactions are the support forces (K u - F) at the
+    prescribed DOFs, within 1e-6, and balance the applied load."""
+    import scipy.sparse as sp
+    from ufem_amd.mesh import prescribed_cantilever
+    mesh = prescribed_cantilever(12, 3, 3, E=lognormal_E(13, 108, cov=0.3))
+    o = uo.OracleSystem(mesh)
+    o.fill_K(3)
+    o.add_loads()
+    ip, c, v = o.K_csr()
+    K0 = sp.csr_matrix((v, c, ip), shape=(o.n, o.n)).toarray()
+    F0 = o.F()
+    pres = {3 * n + d - 1: val for n, t, d, val in zip(mesh.nl_node, mesh.nl_type, mesh.nl_dir, mesh.nl_val) if t == 1}
+    P = np.array(sorted(pres))
+    vals = np.array([pres[p] for p in P])
+    free = np.setdiff1d(np.arange(o.n), P)
+    u = np.zeros(o.n)
+    u[P] = vals
+    u[free] = np.linalg.solve(K0[np.ix_(free, free)], F0[free] - K0[np.ix_(free, P)] @ vals)
+    g = G.GpuSystem(mesh)
+    g.assemble(3)
+    for solver in ("pcg", "ssor"):
+        info = g.solve(solver, eps=1e-13, rule=1)
+        assert info["rv"] == 0, (solver, info)
+        ug = g.u()
+        assert rel(ug, u) <= 1e-8, solver
+        assert np.abs(ug[P] - vals).max() <= 1e-12 * np.abs(vals).max()
+    R, rn, rd = g.reactions()
+    Rd = (K0 @ u - F0)[3 * rn + rd - 1]
+    assert rel(R, Rd) <= 1e-6
+    load = mesh.nl_val[mesh.nl_type == 3].sum()
+    assert abs(R[rd == 3].sum() + load) <= 1e-6 * np.abs(R[rd == 3]).sum()
+
+
 def oracle_ssor(mesh, eps, npts=3):
     s = uo.OracleSystem(mesh)
     s.assemble(npts)

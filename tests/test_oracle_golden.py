@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import ufem_oracle as uo
-from ufem_amd.mesh import cantilever, lognormal_E
+from ufem_amd.mesh import cantilever, lognormal_E, prescribed_cantilever
 from tests.golden.make_golden import pattern_digest, probe_vector, csr_matvec, _diag_pos
 
 MESHES = {
@@ -16,6 +16,7 @@ MESHES = {
     "het4": lambda: cantilever(4, 2, 2, E=lognormal_E(11, 16, cov=0.6)),
     "dist": lambda: cantilever(6, 3, 3, distort=0.25, seed=5, E=lognormal_E(3, 54, cov=0.3)),
     "cube": lambda: cantilever(1, 1, 1, L=1.0, B=1.0, H=1.0),
+    "pd": lambda: prescribed_cantilever(12, 3, 3, E=lognormal_E(13, 108, cov=0.3)),
 }
 
 # (tag, mesh, solver id, eps)
@@ -28,6 +29,7 @@ CASES = [
     ("het4_cg", "het4", 0, 1e-21),
     ("dist_cg", "dist", 0, 1e-21),
     ("cube_cg", "cube", 0, 1e-21),
+    ("pd_cg", "pd", 0, 1e-12),       # prescribed non-zero displacements, some before the supports
 ]
 
 
@@ -91,3 +93,30 @@ def test_threaded_reference_run_close(golden):
     assert abs(iters + 1 - int(golden["c1_cg_t4/spmv_calls"])) <= 3
     ref = golden["c1_cg_t4/u"]
     assert np.linalg.norm(s.u() - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_reference_double_counts_prescribed_couplings_outside_the_window(golden):
+    """With non-zero prescribed displacements the reference subtracts K(:, p) v
+    from F for every row (fem/fem_loads.c:210-224) but zeroes column p only in
+    its row window (fem/fem_math.c:592-620), so rows outside the window count
+    the coupling twice: the reference's u (reproduced bitwise by the oracle,
+    test above) is far from the solution of the eliminated system, which the
+    GPU path solves."""
+    import scipy.sparse as sp
+    mesh = MESHES["pd"]()
+    s = uo.OracleSystem(mesh)
+    s.fill_K(3)
+    s.add_loads()
+    ip, c, v = s.K_csr()
+    K0 = sp.csr_matrix((v, c, ip), shape=(s.n, s.n)).toarray()
+    F0 = s.F()
+    pres = {3 * n + d - 1: val for n, t, d, val in zip(mesh.nl_node, mesh.nl_type, mesh.nl_dir, mesh.nl_val) if t == 1}
+    P = np.array(sorted(pres))
+    vals = np.array([pres[p] for p in P])
+    free = np.setdiff1d(np.arange(s.n), P)
+    u = np.zeros(s.n)
+    u[P] = vals
+    u[free] = np.linalg.solve(K0[np.ix_(free, free)], F0[free] - K0[np.ix_(free, P)] @ vals)
+    ref = golden["pd_cg/u"]
+    assert np.abs(ref[P] - vals).max() <= 1e-12 * np.abs(vals).max()
+    assert np.linalg.norm(ref - u) > 1.0 * np.linalg.norm(u)

@@ -65,6 +65,8 @@ struct ufem_sys {
   double *F_host;
   long n_fixed;             /* distinct fixed DOFs, first-occurrence order */
   int *fixed_host, *d_fixed;
+  double *fix_val;          /* prescribed value of each fixed DOF          */
+  int any_prescribed;       /* some fixed DOF has a non-zero value         */
   long n_react;             /* reaction slots, one per support entry */
   long *react_node, *react_dof, *react_fix;
   double *react_init;       /* -F at the DOF when the support was applied */

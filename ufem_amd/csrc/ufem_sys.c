@@ -259,7 +259,7 @@ void ufem_sys_destroy(ufem_sys *s)
   ugk_stream_destroy(s->stream);
   free(s->color_off); free(s->nl_node); free(s->nl_type); free(s->nl_dir); free(s->nl_val);
   free(s->F_host); free(s->fixed_host); free(s->react_node); free(s->react_dof);
-  free(s->react_fix); free(s->react_init);
+  free(s->react_fix); free(s->react_init); free(s->fix_val);
   free(s);
 }
 
@@ -280,10 +280,6 @@ ufem_sys *ufem_sys_create(const ufem_mesh *m)
   for (i = 0; i < m->n_nl; i++) {
     if (m->nl_node[i] < 0 || m->nl_node[i] >= nn || m->nl_dir[i] < 1 || m->nl_dir[i] > 3) {
       ufem_set_error("load/support %ld out of range", i);
-      return NULL;
-    }
-    if (m->nl_type[i] == 1 && m->nl_val[i] != 0.0) {
-      ufem_set_error("prescribed non-zero displacement (entry %ld) is not on this path", i);
       return NULL;
     }
   }
@@ -346,7 +342,9 @@ ufem_sys *ufem_sys_create(const ufem_mesh *m)
   s->react_fix = (long *)calloc((size_t)s->n_react + 1, sizeof(long));
   s->react_init = (double *)calloc((size_t)s->n_react + 1, sizeof(double));
   s->fixed_host = (int *)calloc((size_t)s->n_react + 1, sizeof(int));
-  if (!s->react_node || !s->react_dof || !s->react_fix || !s->react_init || !s->fixed_host) goto fail;
+  s->fix_val = (double *)calloc((size_t)s->n_react + 1, sizeof(double));
+  if (!s->react_node || !s->react_dof || !s->react_fix || !s->react_init || !s->fixed_host || !s->fix_val)
+    goto fail;
   {
     long r = 0;
     char *seen = (char *)calloc((size_t)s->n, 1);
@@ -361,7 +359,14 @@ ufem_sys *ufem_sys_create(const ufem_mesh *m)
       if (!seen[d]) {
         seen[d] = 1;
         slot_of[d] = s->n_fixed;
+        s->fix_val[s->n_fixed] = m->nl_val[i];
+        if (m->nl_val[i] != 0.0) s->any_prescribed = 1;
         s->fixed_host[s->n_fixed++] = (int)d;
+      } else if (m->nl_val[i] != s->fix_val[slot_of[d]]) {
+        ufem_set_error("DOF prescribed twice with different values (entry %ld)", i);
+        free(seen);
+        free(slot_of);
+        goto fail;
       }
       s->react_fix[r] = slot_of[d];
       r++;
@@ -457,6 +462,30 @@ int ufem_sys_add_disps(ufem_sys *s)
       if (s->react_node[q] == s->react_node[r] && s->react_dof[q] == s->react_dof[r]) { first = q; break; }
     s->react_init[first] -= s->F_host[d];
     s->F_host[d] = 0.0;
+  }
+  if (s->any_prescribed) {
+    /* prescribed non-zero values: the free rows' loads lose K(:, p) v_p, as
+     * femApplyDisp's row loop (fem/fem_loads.c:210-224) does one column at a
+     * time -- here one SpMV with the vector of values, before the supports
+     * zero K (the full rows and columns, so that no coupling is counted twice) */
+    double *v = (double *)calloc((size_t)s->n, sizeof(double));
+    double *y = (double *)malloc(sizeof(double) * (size_t)s->n);
+    long i, f;
+    int bad;
+    if (!v || !y) { free(v); free(y); return AF_ERR_MEM; }
+    for (f = 0; f < s->n_fixed; f++) v[s->fixed_host[f]] = s->fix_val[f];
+    bad = ugk_h2d(s->w.z, v, sizeof(double) * (size_t)s->n, s->stream) ||
+          ugk_spmv(&s->K, s->w.z, s->w.q, s->stream) ||
+          ugk_d2h(y, s->w.q, sizeof(double) * (size_t)s->n, s->stream);
+    if (!bad)
+      for (i = 0; i < s->n; i++) s->F_host[i] -= y[i];
+    free(v);
+    free(y);
+    if (bad) return gpu_fail("prescribed displacements");
+  }
+  {
+    long f;   /* the support rows solve 1 u_p = v_p after the zeroing */
+    for (f = 0; f < s->n_fixed; f++) s->F_host[s->fixed_host[f]] = s->fix_val[f];
   }
   if (ugk_event_record(s->ev[2], s->stream)) return gpu_fail("event");
   if (s->n_fixed > 0) {

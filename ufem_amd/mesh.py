@@ -171,6 +171,31 @@ def cantilever(nx: int, ny: int, nz: int, L: float = 2.0, B: float = 0.4, H: flo
                 meta={"L": L, "B": B, "H": H, "total_load": total_load, "distort": distort})
 
 
+def prescribed_cantilever(nx: int, ny: int, nz: int, E: float | np.ndarray = 30e9, tip_uz: float = -1e-4,
+                          pull_ux: float = 2e-5, load: float = -2000.0) -> Mesh:
+    """Cantilever driven by prescribed displacements: ux = pull_ux on the
+    nodes of the plane x = L/nx, listed before the supports (so that it meets
+    them in the reference's sequential order), the x = 0 supports, uz = tip_uz
+    on the x = L face, and a force `load` (z) spread over the mid-length
+    plane."""
+    m = cantilever(nx, ny, nz, E=E)
+
+    def plane(i):
+        jj, kk = np.meshgrid(np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+        return (i + (nx + 1) * (jj.T.reshape(-1) + (ny + 1) * kk.T.reshape(-1))).astype(np.int64)
+
+    p1, p0, pl, pm = plane(1), plane(0), plane(nx), plane(nx // 2)
+    nl_node = np.concatenate([p1, np.repeat(p0, 3), pl, pm])
+    nl_type = np.concatenate([np.full(p1.size, 1), np.full(3 * p0.size, 1), np.full(pl.size, 1),
+                              np.full(pm.size, 3)]).astype(np.int64)
+    nl_dir = np.concatenate([np.full(p1.size, 1), np.tile([1, 2, 3], p0.size), np.full(pl.size, 3),
+                             np.full(pm.size, 3)]).astype(np.int64)
+    nl_val = np.concatenate([np.full(p1.size, pull_ux), np.zeros(3 * p0.size), np.full(pl.size, tip_uz),
+                             np.full(pm.size, load / pm.size)])
+    return Mesh(nx, ny, nz, m.coords, m.enodes, m.E, m.nu, nl_node, nl_type, nl_dir, nl_val.astype(np.float64),
+                meta=dict(m.meta, prescribed=True))
+
+
 def config_mesh(name: str, seed: int = 2024) -> Mesh:
     """The benchmark's named workloads (2.0 x 0.4 x 0.4 m cantilever, nu 0.2)."""
     if name == "plain_cpu":          # 20x4x4, homogeneous
