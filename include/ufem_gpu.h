@@ -98,7 +98,13 @@ long ufem_sys_nreact(const ufem_sys *s);
 /* new Young's moduli, one per element, on the same mesh (Monte-Carlo and
  * training fields); fill_K and add_disps must follow */
 int  ufem_sys_set_E(ufem_sys *s, const double *E);
-int  ufem_sys_fill_K(ufem_sys *s, int npts);    /* npts = 2 (exact for parallelepipeds) or 3 */
+/* new node coordinates, E, nu and load/support values on the same mesh
+ * (same sizes, connectivity and load/support list), and F, K and the
+ * reaction slots back to the state a solve starts from -- what
+ * monte_linear_solver does between samples (fem/fem_mont.c:466-487);
+ * fill_K, add_loads and add_disps must follow */
+int  ufem_sys_update(ufem_sys *s, const ufem_mesh *m);
+int  ufem_sys_fill_K(ufem_sys *s, int npts);   /* npts = 2 (exact for parallelepipeds) or 3 */
 int  ufem_sys_add_loads(ufem_sys *s);
 int  ufem_sys_add_disps(ufem_sys *s);
 int  ufem_sys_solve(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info);

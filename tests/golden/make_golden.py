@@ -2,7 +2,10 @@
 uFEM reference (oracle/_ref/ufem_ref_harness, built by `make -C oracle ref`
 from the reference sources in this container).
 
-Run:  python tests/golden/make_golden.py
+Run:  python tests/golden/make_golden.py           (vectors)
+      python tests/golden/make_golden.py --res     (result file, het4_cg.res)
+      python tests/golden/make_golden.py --monte   (Monte plugin, monte_golden.json;
+                                                    needs `make -C oracle ref-monte`)
 Each case generates a synthetic cantilever with ufem_amd.mesh, writes it in the
 reference's preparsed input format, runs the reference harness (serial, like
 `fem -p`), and keeps small arrays: K pattern digests and products, full K for
@@ -23,6 +26,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
 
 from ufem_amd.mesh import cantilever, lognormal_E, prescribed_cantilever  # noqa: E402
 
@@ -155,9 +159,46 @@ def write_res_golden(tag="het4_cg"):
     print("wrote", os.path.join(HERE, f"{tag}.res"))
 
 
+MONTE = os.path.join(ROOT, "oracle", "_ref", "ufem_ref_monte")
+
+
+def write_monte_golden():
+    """The reference's Monte plugin functions (fem/fem_mont.c with USE_MONTE,
+    driven by oracle/ref_monte.c) on the cases of monte_cases.py: variable
+    counts and names, and per sample the return code and the output field."""
+    import monte_cases as mc
+    if not os.path.exists(MONTE):
+        raise SystemExit(f"missing {MONTE}: run `make -C oracle ref-monte` first")
+    out = {}
+    for tag in ("mc_a", "mc_b"):
+        smp = mc.samples(tag)
+        with tempfile.TemporaryDirectory() as td:
+            fin = os.path.join(td, "in.fem")
+            fs = os.path.join(td, "samples.txt")
+            mc.write_case(tag, fin)
+            with open(fs, "w") as f:
+                f.write("\n".join(" ".join(repr(v) for v in row) for row in smp) + "\n")
+            lines = subprocess.run([MONTE, fin, fs], check=True, capture_output=True,
+                                   text=True).stdout.splitlines()
+        itype, il, ol, ff = (int(v) for v in lines[0].split())
+        rows = [ln.split() for ln in lines[1 + il + ol:]]
+        out[tag] = {"interface": itype, "ilen": il, "olen": ol, "ffunc": ff,
+                    "inames": lines[1:1 + il], "onames": lines[1 + il:1 + il + ol],
+                    "samples": smp, "rv": [int(r[0]) for r in rows],
+                    "ofld": [[float(v) for v in r[1:]] for r in rows]}
+        print(tag, "samples", len(rows), "rv", out[tag]["rv"])
+    path = os.path.join(HERE, "monte_golden.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", path)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--res":
         write_res_golden()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--monte":
+        write_monte_golden()
         return
     if not os.path.exists(HARNESS):
         raise SystemExit(f"missing {HARNESS}: run `make -C oracle ref` first")

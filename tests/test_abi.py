@@ -9,7 +9,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "ufem_amd", "libufem_gpu.so")
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("ufem_fem_math.h", "ufem_gpu.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("ufem_fem_math.h", "ufem_gpu.h", "ufem_monte.h")]
 
 DECL = re.compile(r"^\s*(?:const\s+)?(?:int|long|double|void|char|ufem_sys)\s*\*?\s*(\w+)\s*\(", re.M)
 
@@ -27,7 +27,9 @@ def test_headers_declare_the_dropin_symbols():
     names = declared()
     for n in ("femEqsCGwJ", "femEqsBiCCSwJ", "femEqsCGwSSOR", "femMatVecMultBig", "femVecVecMultBig",
               "femVecNormBig", "femMatNormBig", "femVecLinCombBig", "ufem_sys_create",
-              "ufem_sys_solve", "ufem_gpu_element_stiffness"):
+              "ufem_sys_solve", "ufem_gpu_element_stiffness", "ufem_sys_update",
+              "monte_dlib_interface_type", "monte_nums_of_vars", "monte_ivar_name", "monte_ovar_name",
+              "monte_init_lib_stuff", "monte_clean_lib_stuff", "monte_solution"):
         assert n in names
 
 

@@ -99,4 +99,34 @@ int ufem_net_infer(ufem_net *N, const int *d_skip, void *stream);
 /* error text (ufem_sys.c) */
 void ufem_set_error(const char *fmt, ...);
 
+/* uFEM's preparsed input file as femReadInput holds it (fem/fem_io.c:380-640),
+ * every index 0-based (ufem_input.c).  Sets of values (real sets, materials)
+ * keep the reference's flat list + start offsets. */
+typedef struct {
+  long sol_id, sol_dep_on, sol_lin, sol_nonlin, sol_nonlin_steps;
+  long rs_len, rs_len_l, *rs_id, *rs_type, *rs_from;
+  double *rs_val;
+  long mp_len, mp_len_l, *mp_id, *mp_type, *mp_from;
+  double *mp_val;
+  long n_len, *n_id;
+  double *n_x, *n_y, *n_z;
+  long e_len, e_len_l, *e_id, *e_type, *e_rs, *e_mp, *e_from, *e_nodes;
+  long gr_dir;
+  double gr_val;
+  long nl_len, *nl_node, *nl_type, *nl_dir;
+  double *nl_val;
+  long el_len;
+  /* the Monte section (fem_monte_read_data, fem/fem_mont.c:866-909):
+     inputs first, then outputs */
+  long mc_ilen, mc_olen, mc_fail_id;
+  long *mc_pos, *mc_type, *mc_item, *mc_sub, *mc_iter;
+} ufem_input;
+int  ufem_input_read(const char *fname, ufem_input *in);
+void ufem_input_free(ufem_input *in);
+/* value `type` (repetition `rep`, 0: the non-repeating part) of material set
+ * `mp` of type 1: its index in mp_val, -1 when absent (femGetRepValIndex) */
+long ufem_input_mat_index(const ufem_input *in, long mp, long type, long rep);
+/* E and nu of every element from its material set (femGetMPValPos) */
+void ufem_input_elastic(const ufem_input *in, double *E, double *nu);
+
 #endif

@@ -405,6 +405,60 @@ int ufem_sys_set_E(ufem_sys *s, const double *E)
   return AF_OK;
 }
 
+/* new values on an unchanged mesh (node coordinates, E, nu, load and support
+ * values; the connectivity and the load/support list stay), then the state
+ * monte_linear_solver starts from (fem/fem_mont.c:470-475: K, F and u set to
+ * zero before fem_fill_K): F and the reaction slots cleared, K to be refilled */
+int ufem_sys_update(ufem_sys *s, const ufem_mesh *m)
+{
+  long i, r = 0;
+  if (!m || m->n_nodes != s->n_nodes || m->n_elems != s->n_elems || m->n_nl != s->n_nl) {
+    ufem_set_error("update: mesh sizes differ from the system's");
+    return AF_ERR_SIZ;
+  }
+  for (i = 0; i < m->n_nl; i++)
+    if (m->nl_node[i] != s->nl_node[i] || m->nl_type[i] != s->nl_type[i] || m->nl_dir[i] != s->nl_dir[i]) {
+      ufem_set_error("update: load/support %ld differs from the system's", i);
+      return AF_ERR_VAL;
+    }
+  /* prescribed value of each fixed DOF from its first support entry; a later
+     entry of the same DOF must agree, as at creation */
+  {
+    char *seen = (char *)calloc((size_t)s->n_fixed + 1, 1);
+    if (!seen) return AF_ERR_MEM;
+    for (i = 0; i < m->n_nl; i++) {
+      long f;
+      if (m->nl_type[i] != 1) continue;
+      f = s->react_fix[r++];
+      if (!seen[f]) {
+        seen[f] = 1;
+        s->fix_val[f] = m->nl_val[i];
+      } else if (m->nl_val[i] != s->fix_val[f]) {
+        ufem_set_error("DOF prescribed twice with different values (entry %ld)", i);
+        free(seen);
+        return AF_ERR_VAL;
+      }
+    }
+    free(seen);
+  }
+  s->any_prescribed = 0;
+  for (i = 0; i < s->n_fixed; i++)
+    if (s->fix_val[i] != 0.0) s->any_prescribed = 1;
+  if (m->n_nl > 0) memcpy(s->nl_val, m->nl_val, sizeof(double) * (size_t)m->n_nl);
+  if (ugk_h2d(s->d_x, m->x, sizeof(double) * (size_t)s->n_nodes, s->stream) ||
+      ugk_h2d(s->d_y, m->y, sizeof(double) * (size_t)s->n_nodes, s->stream) ||
+      ugk_h2d(s->d_z, m->z, sizeof(double) * (size_t)s->n_nodes, s->stream) ||
+      ugk_h2d(s->d_E, m->E, sizeof(double) * (size_t)s->n_elems, s->stream) ||
+      ugk_h2d(s->d_nu, m->nu, sizeof(double) * (size_t)s->n_elems, s->stream) || ugk_sync(s->stream))
+    return gpu_fail("mesh upload");
+  memset(s->F_host, 0, sizeof(double) * (size_t)s->n);
+  memset(s->react_init, 0, sizeof(double) * (size_t)(s->n_react + 1));
+  s->filled = 0;
+  s->loaded = 0;
+  s->bc_applied = 0;
+  return AF_OK;
+}
+
 int ufem_sys_fill_K(ufem_sys *s, int npts)
 {
   int c;

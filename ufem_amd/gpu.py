@@ -132,6 +132,14 @@ def lib():
             "femEqsCGwSSOR": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
                                         C.c_double, C.c_long]),
             "femMatVecMultBig": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector)]),
+            "ufem_sys_update": (C.c_int, [C.c_void_p, C.POINTER(_Mesh)]),
+            "monte_dlib_interface_type": (C.c_long, []),
+            "monte_nums_of_vars": (None, [C.c_char_p, P_L, P_L, P_L]),
+            "monte_ivar_name": (C.c_char_p, [C.c_char_p, C.c_long]),
+            "monte_ovar_name": (C.c_char_p, [C.c_char_p, C.c_long]),
+            "monte_init_lib_stuff": (C.c_int, [C.c_char_p]),
+            "monte_clean_lib_stuff": (C.c_int, [C.c_char_p]),
+            "monte_solution": (C.c_int, [C.c_char_p, P_D, P_D, C.c_long]),
             "femVecVecMultBig": (C.c_double, [C.POINTER(tVector), C.POINTER(tVector)]),
             "femVecNormBig": (C.c_double, [C.POINTER(tVector)]),
             "femMatNormBig": (C.c_double, [C.POINTER(tMatrix)]),
@@ -195,6 +203,15 @@ class GpuSystem:
         self.n_elems = mesh.n_elems
         self.nreact = lib().ufem_sys_nreact(self.h)
         self.ncolors = lib().ufem_sys_ncolors(self.h)
+
+    def update(self, mesh):
+        """new coordinates, moduli and load/support values on the same mesh;
+        F, K and the reactions start over (fill_K, add_loads, add_disps next)"""
+        keep = []
+        m = _mesh_struct(mesh, keep)
+        rv = lib().ufem_sys_update(self.h, C.byref(m))
+        if rv != AF_OK:
+            raise RuntimeError(f"ufem_sys_update = {rv}: {last_error()}")
 
     def close(self):
         if getattr(self, "h", None):
@@ -593,3 +610,39 @@ def last_solve() -> dict:
     info = SolveInfo()
     lib().ufem_gpu_last_solve(C.byref(info))
     return info.as_dict()
+
+
+class Monte:
+    """The Monte plugin interface (include/ufem_monte.h; fem/fem_mont.c): one
+    model per process, read from a preparsed input file with a
+    random-variable section; each solution() is one sample on the GPU."""
+
+    def __init__(self, path: str):
+        self.param = os.fsencode(path)
+        rv = lib().monte_init_lib_stuff(self.param)
+        if rv != 0:
+            raise RuntimeError(f"monte_init_lib_stuff({path}) = {rv}: {last_error()}")
+        il, ol, ff = C.c_long(), C.c_long(), C.c_long()
+        lib().monte_nums_of_vars(self.param, C.byref(il), C.byref(ol), C.byref(ff))
+        self.ilen, self.olen, self.ffunc = il.value, ol.value, ff.value
+
+    @staticmethod
+    def interface_type() -> int:
+        return lib().monte_dlib_interface_type()
+
+    def inames(self):
+        return [lib().monte_ivar_name(self.param, i).decode() for i in range(self.ilen)]
+
+    def onames(self):
+        return [lib().monte_ovar_name(self.param, i).decode() for i in range(self.olen)]
+
+    def solution(self, ifld, if_type: int = 0, ofld_len: int | None = None):
+        """(return code, output field); the field has olen entries unless the
+        input file places outputs further out"""
+        x = np.ascontiguousarray(ifld, dtype=np.float64)
+        y = np.zeros(max(self.olen, ofld_len or 0, 1))
+        rv = lib().monte_solution(self.param, _pd(x), _pd(y), if_type)
+        return rv, y
+
+    def close(self):
+        lib().monte_clean_lib_stuff(self.param)

@@ -225,11 +225,14 @@ def batch_meshes(n: int = 256, seed: int = 7, sizes=BATCH_SIZES, cov: float = 0.
     return out
 
 
-def write_fem_input(mesh: Mesh, path: str, material_per_element: bool = True) -> None:
+def write_fem_input(mesh: Mesh, path: str, material_per_element: bool = True, monte=None) -> None:
     """Write `mesh` in the reference's preparsed input format (fem/feminput.5,
     read by femReadInput fem/fem_io.c:380-640 with -p): material type 1 with
     values {DENS, EX, NU, ALPHA, KXX, C, PRICE} (fem/fem_m001.c:39-41), one
-    material set per element when moduli differ; numbers printed %.17g."""
+    material set per element when moduli differ; numbers printed %.17g.
+    `monte` = (input rows, output rows), each row (pos, type, item, subitem,
+    subitemiter), appends the random-variable section fem_monte_read_data
+    reads (fem/fem_mont.c:866-909)."""
     ne, nn = mesh.n_elems, mesh.n_nodes
     lines = ["1 -1", "1 0 0", "0"]
     if material_per_element:
@@ -254,5 +257,9 @@ def write_fem_input(mesh: Mesh, path: str, material_per_element: bool = True) ->
     lines.append("0")        # element loads
     lines.append("0 0 0")    # time-dependent array data
     lines.append("0")        # dynamics
+    if monte is not None:
+        rows_in, rows_out = monte
+        lines.append(f"{len(rows_in)} {len(rows_out)}")
+        lines += [" ".join(str(int(v)) for v in row) for row in list(rows_in) + list(rows_out)]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
