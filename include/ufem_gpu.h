@@ -10,13 +10,13 @@
  *   ufem_sys_fill_K      fem_fill_K(AF_NO)       (fem/fem_sol.c:897-989)
  *   ufem_sys_add_loads   fem_add_loads(0)        (fem/fem_sol.c:1119-1189)
  *   ufem_sys_add_disps   fem_add_disps(AF_YES,0) (fem/fem_sol.c:1195-1247)
- *   ufem_sys_solve       femEqsCGwJ / femEqsBiCCSwJ (fem/fem_eqs.c:88, :262)
+ *   ufem_sys_solve       femEqsCGwJ / femEqsBiCCSwJ / femEqsCGwSSOR (fem/fem_eqs.c:88, :262, :999)
  *   ufem_sys_reactions   fem_fill_K(AF_YES) + fem_fill_K_React (fem/fem_sol.c:844)
  *
  * The mesh arrives in the reference's preparsed-input terms (fem/feminput.5):
  * node coordinates, 8 node positions per brick in the reference node order
  * (fem/fem_e009.c:36-69), per-element E and nu, and the nodal load/support
- * list in input order (type 1 = prescribed displacement, only 0 supported;
+ * list in input order (type 1 = prescribed displacement, any value;
  * type 3 = force; direction 1..3 = ux, uy, uz).  Every node must belong to an
  * element; DOF 3 p + (dir - 1) then equals the reference's node-major
  * numbering.  Return codes are the reference's AF_* values.
@@ -108,6 +108,9 @@ int  ufem_sys_fill_K(ufem_sys *s, int npts);   /* npts = 2 (exact for parallelep
 int  ufem_sys_add_loads(ufem_sys *s);
 int  ufem_sys_add_disps(ufem_sys *s);
 int  ufem_sys_solve(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info);
+/* reactions, one per support entry in input order (node position, direction,
+ * value); supports of non-zero prescribed value are included here, though the
+ * reference's own list leaves them out (fem/fem_sol.c:676-722) */
 int  ufem_sys_reactions(ufem_sys *s, double *R, long *rnode, long *rdof);
 /* strains and stresses at the npts^3 Gauss points (npts 2 or 3) of every
  * element for the current u, the results pass of fem_fill_K(AF_YES)
@@ -122,7 +125,8 @@ double ufem_sys_time_results(ufem_sys *s, int npts, int reps);
  * current u, with the 27-point element results in the reference's element
  * result layout; sol_id, dep_on and sim_num fill the header line (the input
  * file's solution ID and dependency, the simulation number); node and
- * element IDs are written as 1-based positions */
+ * element IDs are written as 1-based positions; the reactions of supports of
+ * zero value only, as the reference lists them */
 int  ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep_on, double sim_num);
 int  ufem_sys_get_u(ufem_sys *s, double *u);
 int  ufem_sys_set_u(ufem_sys *s, const double *u);

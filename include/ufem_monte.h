@@ -48,6 +48,18 @@ int monte_clean_lib_stuff(char *param);
  * (fem/fem_mont.c:694-855) */
 int monte_solution(char *param, double *ifld, double *ofld, long if_type);
 
+/* No counterpart in the reference: n samples as slots of one batched GPU
+ * launch, for a Monte driver that can hand over several samples at once.
+ * Sample k reads ifld[k ilen .. k ilen + ilen - 1] and writes
+ * ofld[k olen .. k olen + olen - 1] (every variable's slot must lie inside
+ * its field, else AF_ERR_SIZ); the outputs equal those of n monte_solution
+ * calls in sample order (reaction sums included), with each sample solved by
+ * Jacobi PCG to ||r|| <= 1e-15 ||f|| instead of SSOR-CG.  A model with a
+ * support of non-zero value, or inputs that set one, is solved one sample at
+ * a time.  status[k] (may be NULL): AF_OK or AF_ERR (not converged) per
+ * sample; returns the first failing code. */
+int monte_solution_batch(char *param, long n, const double *ifld, double *ofld, long if_type, int *status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,3 +124,46 @@ def test_sys_update_equals_fresh_system():
     assert np.abs(Ra - Rb).max() <= 1e-9 * np.abs(Rb).max()
     with pytest.raises(RuntimeError):
         sa.update(cantilever(9, 3, 3))      # other sizes
+
+
+@pytest.mark.parametrize("tag", ["mc_a", "mc_b"])
+def test_monte_batch_matches_reference_and_sequential(monte, tag):
+    """the golden samples as slots of one batched launch: the reference's
+    fields (reaction sums in sample order included), and the fields of the
+    same samples called one at a time on a fresh model"""
+    g = GOLD[tag]
+    names = g["onames"] if tag == "mc_a" else ["DISP", "DMIN", "DISP", "ERES_FAIL", "ERES_FAIL"]
+    m = monte(tag)
+    rv, fields, st = m.solution_batch(np.array(g["samples"]))
+    assert rv == 0 and np.all(st == 0), (rv, st, gpu.last_error())
+    for k in range(len(g["samples"])):
+        check_fields(fields[k], g["ofld"][k], names, (tag, "batch", k))
+    m.close()
+    m2 = monte(tag)
+    for k, ifld in enumerate(g["samples"]):
+        rv, seq = m2.solution(ifld)
+        assert rv == 0
+        check_fields(fields[k], seq[:g["olen"]], names, (tag, "batch vs one at a time", k))
+
+
+def test_monte_batch_with_support_values_goes_one_at_a_time(tmp_path):
+    """inputs that set a support value (NLD of type 1) take the one-sample
+    path inside monte_solution_batch; the fields equal separate calls"""
+    mesh, ri, ro = mc.case_rows("mc_a")
+    sup = int(np.nonzero(mesh.nl_type == 1)[0][5])
+    ri2 = ri[:3] + [(3, mc.NLD, int(mesh.nl_node[sup]), 1, int(mesh.nl_dir[sup]))]
+    ro2 = [(4 + k,) + r[1:] for k, r in enumerate(ro[:12])]
+    path = str(tmp_path / "sup.fem")
+    from ufem_amd.mesh import write_fem_input
+    write_fem_input(mesh, path, monte=(ri2, ro2))
+    smp = np.array([s[:3] + [1e-5 * (k + 1)] for k, s in enumerate(mc.samples("mc_a"))])
+    m = gpu.Monte(path)
+    rv, fields, st = m.solution_batch(smp)
+    m.close()
+    assert rv == 0 and np.all(st == 0), gpu.last_error()
+    m = gpu.Monte(path)
+    for k in range(smp.shape[0]):
+        rv, seq = m.solution(smp[k])
+        assert rv == 0
+        assert np.array_equal(fields[k], seq[:len(ro2)]), k   # the same calls, bitwise
+    m.close()

@@ -230,7 +230,7 @@ def test_result_file_matches_reference(golden, tmp_path):
         assert close(mine[k], ref[k]), (k, np.abs(mine[k] - ref[k]).max())
 
 
-def test_prescribed_displacements_vs_eliminated_system():
+def test_prescribed_displacements_vs_eliminated_system(golden, tmp_path):
     """Non-zero prescribed displacements (some listed before the supports): the
     loads of the free rows lose K(:, P) v in one SpMV and the supports zero
     full rows and columns, so u solves the eliminated system (a dense solve
@@ -265,6 +265,15 @@ def test_prescribed_displacements_vs_eliminated_system():
     assert rel(R, Rd) <= 1e-6
     load = mesh.nl_val[mesh.nl_type == 3].sum()
     assert abs(R[rd == 3].sum() + load) <= 1e-6 * np.abs(R[rd == 3]).sum()
+    # the result file lists the zero-valued supports only, as the reference's
+    # list (fem/fem_sol.c:676-722) that the pd golden case recorded
+    from tests.resfile import read_res
+    g.write_results(str(tmp_path / "pd.res"))
+    res = read_res(str(tmp_path / "pd.res"))
+    assert np.array_equal(res["r_node"], golden["pd_cg/Rnode"])
+    assert np.array_equal(res["r_dof"], golden["pd_cg/Rdof"])
+    zero = np.array([mesh.nl_val[i] == 0.0 for i in np.nonzero(mesh.nl_type == 1)[0]])
+    assert rel(res["R"], R[zero]) <= 1e-6
 
 
 def oracle_ssor(mesh, eps, npts=3):

@@ -232,33 +232,29 @@ static void model_mesh(ufem_mesh *m, long *enodes)
  * subtracts the load, fem/fem_loads.c:205; fem_fill_K_React adds K u,
  * fem/fem_sol.c:866), so sample k reports the sum of the reactions of
  * samples 1..k; the same sum is kept here (over the samples that solved). */
-static int linear_solution(int need_r, int need_e)
+/* the device system set to mesh m: created at the first sample, updated in
+ * place after; K filled, loads and supports applied */
+static int prepare_system(const ufem_mesh *m)
 {
-  ufem_mesh m;
-  ufem_solve_opts o;
-  ufem_solve_info info;
-  long *enodes = (long *)malloc(sizeof(long) * (size_t)M.in.e_len * 8);
-  long r;
   int rv;
-  if (!enodes) return AF_ERR_MEM;
-  model_mesh(&m, enodes);
   if (!M.sys) {
-    M.sys = ufem_sys_create(&m);
+    M.sys = ufem_sys_create(m);
     rv = M.sys ? AF_OK : AF_ERR;
   } else {
-    rv = ufem_sys_update(M.sys, &m);
+    rv = ufem_sys_update(M.sys, m);
   }
-  free(enodes);
   if (rv != AF_OK) return rv;
   if ((rv = ufem_sys_fill_K(M.sys, 3)) != AF_OK) return rv;
   if ((rv = ufem_sys_add_loads(M.sys)) != AF_OK) return rv;
-  if ((rv = ufem_sys_add_disps(M.sys)) != AF_OK) return rv;
-  memset(&o, 0, sizeof(o));
-  o.solver = UFEM_SOLVER_SSOR;
-  o.rule = UFEM_RULE_REFERENCE;
-  o.eps = FEM_ZERO / 10000.0;
-  o.maxit = 3 * ufem_sys_ndof(M.sys);
-  if ((rv = ufem_sys_solve(M.sys, &o, &info)) != AF_OK) return rv;
+  return ufem_sys_add_disps(M.sys);
+}
+
+/* after the solve (u on the device): host u, reactions added to their sums,
+ * element results */
+static int sample_results(int need_r, int need_e)
+{
+  long r;
+  int rv;
   M.solves++;
   if ((rv = ufem_sys_get_u(M.sys, M.u)) != AF_OK) return rv;
   if (need_r) {
@@ -272,6 +268,27 @@ static int linear_solution(int need_r, int need_e)
   }
   if (need_e && (rv = ufem_sys_element_results(M.sys, 3, M.res)) != AF_OK) return rv;
   return AF_OK;
+}
+
+static int linear_solution(int need_r, int need_e)
+{
+  ufem_mesh m;
+  ufem_solve_opts o;
+  ufem_solve_info info;
+  long *enodes = (long *)malloc(sizeof(long) * (size_t)M.in.e_len * 8);
+  int rv;
+  if (!enodes) return AF_ERR_MEM;
+  model_mesh(&m, enodes);
+  rv = prepare_system(&m);
+  free(enodes);
+  if (rv != AF_OK) return rv;
+  memset(&o, 0, sizeof(o));
+  o.solver = UFEM_SOLVER_SSOR;
+  o.rule = UFEM_RULE_REFERENCE;
+  o.eps = FEM_ZERO / 10000.0;
+  o.maxit = 3 * ufem_sys_ndof(M.sys);
+  if ((rv = ufem_sys_solve(M.sys, &o, &info)) != AF_OK) return rv;
+  return sample_results(need_r, need_e);
 }
 
 /* nDOFfld entry `pos` of the node-major DOF map: ux, uy, uz numbered from 1,
@@ -386,19 +403,13 @@ static void fill_outputs(double *ofld)
   }
 }
 
-/* monte_solution (fem/fem_mont.c:694-855): if_type 0 is the linear static
- * solution; 1 (price of the structure, printed before the reference exits)
- * and 2 (dynamics) return AF_ERR_TYP */
-int monte_solution(char *param, double *ifld, double *ofld, long if_type)
+/* the sampled values of ifld into the model, in row order
+ * (fem/fem_mont.c:709-785) */
+static void apply_inputs(const double *ifld)
 {
   ufem_input *in = &M.in;
-  const long il = in->mc_ilen;
   long i, j;
-  int rv, need_r = 0, need_e = 0;
-  (void)param;
-  if (!M.ready) { ufem_set_error("monte_init_lib_stuff first"); return -1; }
-  for (i = il; i < il + in->mc_olen; i++) ofld[in->mc_pos[i - il]] = 0.0;
-  for (i = 0; i < il; i++) {
+  for (i = 0; i < in->mc_ilen; i++) {
     const long it = in->mc_item[i], sb = in->mc_sub[i], ir = in->mc_iter[i];
     const double v = ifld[in->mc_pos[i]];
     switch (in->mc_type[i]) {
@@ -430,18 +441,138 @@ int monte_solution(char *param, double *ifld, double *ofld, long if_type)
         break;
     }
   }
+}
+
+static void outputs_needed(int *need_r, int *need_e)
+{
+  const ufem_input *in = &M.in;
+  long i;
+  *need_r = *need_e = 0;
+  for (i = in->mc_ilen; i < in->mc_ilen + in->mc_olen; i++) {
+    const long t = in->mc_type[i];
+    if (t == VT_RES_R) *need_r = 1;
+    if (t == VT_RES_E || t == VT_RES_SUM_E || t == VT_RES_MAX_E || t == VT_RES_MIN_E || t == VT_RES_FAIL_E)
+      *need_e = 1;
+  }
+}
+
+static void zero_outputs(double *ofld)
+{
+  const ufem_input *in = &M.in;
+  long i;
+  for (i = in->mc_ilen; i < in->mc_ilen + in->mc_olen; i++) ofld[in->mc_pos[i - in->mc_ilen]] = 0.0;
+}
+
+static int check_if_type(long if_type)
+{
   if (if_type == 1 || if_type == 2) {
     ufem_set_error("monte_solution: if_type %ld (%s) is not supported", if_type,
                    if_type == 1 ? "price" : "dynamics");
     return AF_ERR_TYP;
   }
-  for (i = il; i < il + in->mc_olen; i++) {
-    const long t = in->mc_type[i];
-    if (t == VT_RES_R) need_r = 1;
-    if (t == VT_RES_E || t == VT_RES_SUM_E || t == VT_RES_MAX_E || t == VT_RES_MIN_E || t == VT_RES_FAIL_E)
-      need_e = 1;
-  }
+  return AF_OK;
+}
+
+/* monte_solution (fem/fem_mont.c:694-855): if_type 0 is the linear static
+ * solution; 1 (price of the structure, printed before the reference exits)
+ * and 2 (dynamics) return AF_ERR_TYP */
+int monte_solution(char *param, double *ifld, double *ofld, long if_type)
+{
+  int rv, need_r, need_e;
+  (void)param;
+  if (!M.ready) { ufem_set_error("monte_init_lib_stuff first"); return -1; }
+  zero_outputs(ofld);
+  apply_inputs(ifld);
+  if ((rv = check_if_type(if_type)) != AF_OK) return rv;
+  outputs_needed(&need_r, &need_e);
   if ((rv = linear_solution(need_r, need_e)) != AF_OK) return rv;
   fill_outputs(ofld);
   return AF_OK;
+}
+
+/* n samples in one batched launch: sample k's inputs at ifld + k ilen, its
+ * output field at ofld + k olen.  The samples are applied to the model in
+ * order, as n calls of monte_solution would, and solved together by the
+ * batch's Jacobi PCG to ||r|| <= 1e-15 ||f|| (3 n iterations at most); the
+ * reactions and element results of each sample then follow on the device
+ * system in sample order.  Models with a support of non-zero value, or
+ * inputs that set one, are solved one sample at a time. */
+#define MC_BATCH_EPS 1e-15
+int monte_solution_batch(char *param, long n, const double *ifld, double *ofld, long if_type, int *status)
+{
+  ufem_input *in = &M.in;
+  const long il = in->mc_ilen, ol = in->mc_olen, ne = in->e_len, nn = in->n_len, nl = in->nl_len;
+  ufem_mesh *meshes = NULL;
+  ufem_batch *b = NULL;
+  double *snap = NULL;
+  long *enodes = NULL, *iters = NULL, k, i, sequential = 0;
+  int *st = NULL, rv = AF_OK, need_r, need_e;
+  const size_t per = (size_t)(2 * ne + 3 * nn + nl);
+  if (!M.ready) { ufem_set_error("monte_init_lib_stuff first"); return -1; }
+  if (n < 1) return AF_OK;
+  for (i = il; i < il + ol; i++)
+    if (in->mc_pos[i - il] < 0 || in->mc_pos[i - il] >= ol) {
+      ufem_set_error("batch: output slot %ld outside the field of %ld", in->mc_pos[i - il], ol);
+      return AF_ERR_SIZ;
+    }
+  for (i = 0; i < il; i++)
+    if (in->mc_pos[i] < 0 || in->mc_pos[i] >= il) {
+      ufem_set_error("batch: input slot %ld outside the field of %ld", in->mc_pos[i], il);
+      return AF_ERR_SIZ;
+    }
+  if ((rv = check_if_type(if_type)) != AF_OK) return rv;
+  for (i = 0; i < nl; i++)
+    if (in->nl_type[i] == 1 && in->nl_val[i] != 0.0) sequential = 1;
+  for (i = 0; i < il; i++)
+    if ((in->mc_type[i] == VT_NLD || in->mc_type[i] == VT_NLPOS) && in->mc_sub[i] == 1) sequential = 1;
+  if (sequential || n == 1) {
+    for (k = 0; k < n; k++) {
+      int r = monte_solution(param, (double *)(ifld + k * il), ofld + k * ol, if_type);
+      if (status) status[k] = r;
+      if (r != AF_OK && rv == AF_OK) rv = r;
+    }
+    return rv;
+  }
+  outputs_needed(&need_r, &need_e);
+  meshes = (ufem_mesh *)calloc((size_t)n, sizeof(ufem_mesh));
+  snap = (double *)malloc(sizeof(double) * per * (size_t)n);
+  enodes = (long *)malloc(sizeof(long) * (size_t)ne * 8);
+  iters = (long *)calloc((size_t)n, sizeof(long));
+  st = (int *)calloc((size_t)n, sizeof(int));
+  if (!meshes || !snap || !enodes || !iters || !st) { rv = AF_ERR_MEM; goto out; }
+  for (k = 0; k < n; k++) {   /* model of sample k, as monte_solution k would see it */
+    ufem_mesh m;
+    double *E = snap + per * (size_t)k, *nu = E + ne, *x = nu + ne, *y = x + nn, *z = y + nn, *v = z + nn;
+    apply_inputs(ifld + k * il);
+    model_mesh(&m, enodes);
+    memcpy(E, M.E, sizeof(double) * (size_t)ne);
+    memcpy(nu, M.nu, sizeof(double) * (size_t)ne);
+    memcpy(x, in->n_x, sizeof(double) * (size_t)nn);
+    memcpy(y, in->n_y, sizeof(double) * (size_t)nn);
+    memcpy(z, in->n_z, sizeof(double) * (size_t)nn);
+    if (nl > 0) memcpy(v, in->nl_val, sizeof(double) * (size_t)nl);
+    m.E = E; m.nu = nu; m.x = x; m.y = y; m.z = z; m.nl_val = v;
+    meshes[k] = m;
+  }
+  if (!(b = ufem_batch_create((int)n, meshes))) { rv = AF_ERR; goto out; }
+  if ((rv = ufem_batch_assemble(b, 3, NULL)) != AF_OK) goto out;
+  /* per-sample status 1 = converged; a launch that failed leaves all 0 */
+  (void)ufem_batch_solve(b, MC_BATCH_EPS, 3 * 3 * nn, NULL, iters, NULL, st, NULL);
+  rv = AF_OK;
+  for (k = 0; k < n; k++) {
+    double *o = ofld + k * ol;
+    int r = st[k] == 1 ? AF_OK : AF_ERR;
+    zero_outputs(o);
+    if (r == AF_OK) r = prepare_system(&meshes[k]);
+    if (r == AF_OK) r = ufem_batch_get_u(b, (int)k, M.u);
+    if (r == AF_OK) r = ufem_sys_set_u(M.sys, M.u);
+    if (r == AF_OK) r = sample_results(need_r, need_e);
+    if (r == AF_OK) fill_outputs(o);
+    if (status) status[k] = r;
+    if (r != AF_OK && rv == AF_OK) rv = r;
+  }
+out:
+  if (b) ufem_batch_destroy(b);
+  free(meshes); free(snap); free(enodes); free(iters); free(st);
+  return rv;
 }

@@ -679,8 +679,15 @@ int ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep
     for (d = 0; d < WR_KNOWN_DOFS; d++) fprintf(f, " %li\n", d < 3 ? 3 * i + d + 1 : 0L);
   fprintf(f, " %li\n", n);
   for (i = 0; i < n; i++) fprintf(f, " %e \n", u[i]);
-  fprintf(f, " %li \n", nr);
-  for (i = 0; i < nr; i++) fprintf(f, " %li %li %li %e\n", rn[i], rn[i] + 1, rd[i], R[i]);
+  /* the reference's reaction list leaves out supports of non-zero value
+     (fem_sol_res_alloc, fem/fem_sol.c:676-722) */
+  {
+    long nz = 0;
+    for (i = 0; i < nr; i++) nz += s->fix_val[s->react_fix[i]] == 0.0;
+    fprintf(f, " %li \n", nz);
+  }
+  for (i = 0; i < nr; i++)
+    if (s->fix_val[s->react_fix[i]] == 0.0) fprintf(f, " %li %li %li %e\n", rn[i], rn[i] + 1, rd[i], R[i]);
   fprintf(f, " %li %li\n", ne, (long)WR_ELEM_VALS * ne);
   for (e = 0; e < ne; e++) fprintf(f, " %li \n", e + 1);
   for (e = 0; e < ne; e++) fprintf(f, " %li \n", (long)WR_ELEM_VALS * e);

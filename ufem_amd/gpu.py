@@ -140,6 +140,7 @@ def lib():
             "monte_init_lib_stuff": (C.c_int, [C.c_char_p]),
             "monte_clean_lib_stuff": (C.c_int, [C.c_char_p]),
             "monte_solution": (C.c_int, [C.c_char_p, P_D, P_D, C.c_long]),
+            "monte_solution_batch": (C.c_int, [C.c_char_p, C.c_long, P_D, P_D, C.c_long, P_I]),
             "femVecVecMultBig": (C.c_double, [C.POINTER(tVector), C.POINTER(tVector)]),
             "femVecNormBig": (C.c_double, [C.POINTER(tVector)]),
             "femMatNormBig": (C.c_double, [C.POINTER(tMatrix)]),
@@ -643,6 +644,16 @@ class Monte:
         y = np.zeros(max(self.olen, ofld_len or 0, 1))
         rv = lib().monte_solution(self.param, _pd(x), _pd(y), if_type)
         return rv, y
+
+    def solution_batch(self, ifld, if_type: int = 0):
+        """n samples ([n, ilen]) in one batched launch: (return code,
+        [n, olen] output fields, per-sample status)"""
+        x = np.ascontiguousarray(ifld, dtype=np.float64).reshape(-1, self.ilen)
+        n = x.shape[0]
+        y = np.zeros((n, max(self.olen, 1)))
+        st = np.zeros(n, dtype=np.int32)
+        rv = lib().monte_solution_batch(self.param, n, _pd(x), _pd(y), if_type, _pi(st))
+        return rv, y, st
 
     def close(self):
         lib().monte_clean_lib_stuff(self.param)
