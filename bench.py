@@ -16,8 +16,10 @@ on the GPU from fresh lognormal fields; plain FP64 PCG vs flexible CG with
 the network, FP64 and with the FP32 matrix), the learned preconditioner on
 the 128 x 32 x 32 random-field cantilever (training on the GPU, then plain
 vs learned flexible CG on an unseen field), the batched
-Monte-Carlo solve of 256 uneven meshes in one launch, and the CPU baseline
-(the oracle's pthread restatement of uFEM's solver, bounded sample).
+Monte-Carlo solve of 256 uneven meshes in one launch, the Monte plugin
+interface (one sample per call, and samples as slots of one launch, beside
+the reference's own plugin functions when they are built), and the CPU
+baseline (the oracle's pthread restatement of uFEM's solver, bounded sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-mixed]
                     [--no-learned] [--no-batch] [--kernels-only] [--net-kernels-only]
@@ -253,6 +255,68 @@ def bench_batch(n_samples=256, seed=7, sequential_sample=16):
     return out
 
 
+def bench_monte(nx=40, ny=8, nz=8, n_samples=32, seed=11, sequential_sample=8, ref_sample=2):
+    """The Monte plugin (monte_* of fem/fem_mont.c) on a 40x8x8 cantilever
+    with one material set per element: 16 moduli and the tip load sampled,
+    outputs tip uz, a root reaction, the largest and summed sigma_x.
+    Samples/s one at a time (monte_solution: SSOR-CG, eps 1e-21, the
+    reference's call) and as slots of one batched launch
+    (monte_solution_batch); the reference's own plugin functions
+    (oracle/_ref/ufem_ref_monte, when built) timed on a small sample."""
+    import subprocess
+    import tempfile
+    from ufem_amd import gpu as G
+    from ufem_amd.mesh import cantilever, lognormal_E, write_fem_input
+    ne = nx * ny * nz
+    mesh = cantilever(nx, ny, nz, E=lognormal_E(seed, ne, cov=0.3))
+    rng = np.random.default_rng(seed)
+    sets = rng.choice(ne, 16, replace=False)
+    tip = nx + (nx + 1) * (ny // 2 + (ny + 1) * nz)
+    ins = [(1, int(e), 2, 0) for e in sets] + [(3, tip, 3, 3)]
+    outs = [(8, tip, 3, 0), (9, 0, 3, 0), (12, 0, 7, 0), (11, 0, 7, 0)]
+    ri = [(p,) + r for p, r in enumerate(ins)]
+    ro = [(len(ins) + p,) + r for p, r in enumerate(outs)]
+    smp = np.concatenate([mesh.E[sets][None, :] * rng.uniform(0.6, 1.4, (n_samples, 16)),
+                          rng.uniform(-1000.0, -200.0, (n_samples, 1))], axis=1)
+    out = {"workload": f"cantilever {nx}x{ny}x{nz} bricks, {mesh.n_dof} DOF, {ne} material sets; "
+                       f"16 moduli + tip load sampled; 4 outputs", "samples": n_samples}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "monte.fem")
+        write_fem_input(mesh, path, monte=(ri, ro))
+        m = G.Monte(path)
+        m.solution(smp[0])                         # device system built here
+        t0 = time.perf_counter()
+        for k in range(sequential_sample):
+            rv, _ = m.solution(smp[k])
+        seq_s = (time.perf_counter() - t0) / sequential_sample
+        m.close()
+        m = G.Monte(path)
+        t0 = time.perf_counter()
+        rv_b, fields, st = m.solution_batch(smp)
+        batch_s = time.perf_counter() - t0
+        m.close()
+        out.update({"one_at_a_time_samples_per_s": round(1.0 / seq_s, 2), "rv": int(rv),
+                    "batch_samples_per_s": round(n_samples / batch_s, 2), "batch_rv": int(rv_b),
+                    "batch_vs_one_at_a_time": round(seq_s * n_samples / batch_s, 3)})
+        ref = os.path.join(ROOT, "oracle", "_ref", "ufem_ref_monte")
+        if os.path.exists(ref):
+            fs = os.path.join(td, "samples.txt")
+            with open(fs, "w") as f:
+                f.write("\n".join(" ".join(repr(float(v)) for v in row) for row in smp[:ref_sample]) + "\n")
+            t0 = time.perf_counter()
+            res = subprocess.run([ref, path, fs], capture_output=True, text=True, timeout=600)
+            ref_s = (time.perf_counter() - t0) / ref_sample
+            if res.returncode == 0:
+                rows = [ln.split() for ln in res.stdout.splitlines()[1 + len(ri) + len(ro):]]
+                ref_tip = float(rows[0][1])
+                out["reference_cpu"] = {"samples_per_s": round(1.0 / ref_s, 3), "cores": 1,
+                                        "kind": "reference", "sample": f"{ref_sample} samples, serial",
+                                        "tip_uz_rel_diff": abs(fields[0][0] - ref_tip) / abs(ref_tip)}
+                out["speedup_one_at_a_time_vs_reference"] = round(ref_s / seq_s, 2)
+                out["speedup_batch_vs_reference"] = round(ref_s * n_samples / batch_s, 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,6 +486,12 @@ def main():
         batch = bench_batch()
         log("batch", batch)
 
+    monte = None
+    if not args.no_batch:
+        log("Monte plugin ...")
+        monte = bench_monte()
+        log("monte", monte)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline sample ...")
@@ -460,6 +530,7 @@ def main():
         "learned_mixed_1m": learned_mixed,
         "learned": learned,
         "batch": batch,
+        "monte_plugin": monte,
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),
         "device": dev["name"],
     }
