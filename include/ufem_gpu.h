@@ -50,8 +50,10 @@ enum {
   UFEM_SOLVER_LEARNED = 3,     /* flexible CG with the learned preconditioner */
   UFEM_SOLVER_LEARNED_MIXED = 4, /* the same with q = A p from the FP32 copy of A and
                                    FP64 residual refreshes (inner_eps, inner_maxit) */
-  UFEM_SOLVER_SSOR = 5          /* CG with the SSOR (symmetric Gauss-Seidel) preconditioner,
+  UFEM_SOLVER_SSOR = 5,         /* CG with the SSOR (symmetric Gauss-Seidel) preconditioner,
                                    rows in greedy colour order */
+  UFEM_SOLVER_TWOLEVEL = 6      /* PCG with z = dinv r + P A^-1 P^T r, A = P^T K P on a coarse
+                                   node grid (ufem_sys_set_coarse first) */
 };
 enum { UFEM_RULE_REFERENCE = 0, UFEM_RULE_RELATIVE = 1 };
 /* streamed: single-reduction PCG, two kernels per iteration (w = K u with
@@ -75,6 +77,9 @@ typedef struct {
   void *net;         /* learned: network handle from ufem_net_*           */
   int warm_start;    /* nonzero: start from the current u                 */
   int fcg_m;         /* learned: directions kept by flexible CG (1..8; 0: 1) */
+  int coarse;        /* learned, learned-mixed: add the two-level correction
+                        (ufem_sys_set_coarse first); net may then be NULL,
+                        which leaves dinv r + the correction */
 } ufem_solve_opts;
 
 typedef struct {
@@ -141,6 +146,25 @@ int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host 
 int  ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *bytes);
 /* phase device times of the last calls: fill_K, add_disps, solve (ms) */
 void ufem_sys_timings(const ufem_sys *s, double t[3]);
+
+/* ---- two-level correction ---------------------------------------------------
+ * No counterpart in the reference (its solvers precondition with the diagonal
+ * or SSOR).  For a box mesh of W x H x D nodes (node x + W (y + H z), element
+ * ex + (W-1) (ey + (H-1) ez) with the brick's node order, regular spacing),
+ * a coarse node grid every cx, cy, cz elements; P interpolates trilinearly
+ * from it, and z_c = P A^-1 P^T r with A = P^T K P (coarse DOFs whose
+ * coincident fine DOF is fixed are left out).  A is assembled on the device
+ * from the current moduli and inverted (dense) at the start of each
+ * UFEM_SOLVER_TWOLEVEL solve, and by the learned solvers when opts.coarse is
+ * set.  The factors must divide W-1, H-1, D-1; at most 1024 bricks per cell. */
+int  ufem_sys_set_coarse(ufem_sys *s, int W, int H, int D, int cx, int cy, int cz);
+long ufem_sys_coarse_size(const ufem_sys *s);     /* coarse DOFs, 0 without a coarse grid */
+/* the coarse matrix (inverse = 0) or its inverse (1) for the current moduli,
+ * dense row-major [size][size], identity rows at the left-out DOFs */
+int  ufem_sys_coarse_matrix(ufem_sys *s, int inverse, double *A);
+/* inverse of a dense SPD n x n matrix (row-major) on the device, by block
+ * Gauss-Jordan without pivoting -- the coarse solver's factorisation */
+int  ufem_gpu_dense_spd_inverse(long n, const double *A, double *Ainv);
 
 /* ---- many samples in one persistent launch ------------------------------
  * The reference solves Monte-Carlo samples one at a time through its plugin

@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libufem_gpu.so")
 
 HIP_SOURCES = ["ugk_kernels.hip", "ugk_unet.hip"]
-C_SOURCES = ["ufem_sys.c", "ufem_solve.c", "ufem_ssor.c", "fem_dropin.c", "ufem_batch.c", "ufem_net.c",
+C_SOURCES = ["ufem_sys.c", "ufem_solve.c", "ufem_ssor.c", "ufem_coarse.c", "fem_dropin.c", "ufem_batch.c", "ufem_net.c",
              "ufem_input.c", "ufem_monte.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

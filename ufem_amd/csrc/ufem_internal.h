@@ -27,6 +27,10 @@ typedef struct ufem_ssor {
 } ufem_ssor;
 int ufem_ssor_prepare(const ugk_mat *K, ufem_ssor **S, void *stream);
 void ufem_ssor_free(ufem_ssor *S);
+/* two-level correction (ufem_coarse.c): A = P^T K P assembled and inverted
+ * for the current element moduli */
+int ufem_coarse_build(ugk_coarse *C, void *stream);
+void ufem_coarse_free(ugk_coarse *C);
 
 /* device vectors of one Krylov workspace (FP64 plus the FP32 inner set) */
 typedef struct {
@@ -37,6 +41,7 @@ typedef struct {
   double *fP, *fQ;                                 /* flexible CG rings   */
   int fm;                                          /* ring slots          */
   ufem_ssor *ssor;                                 /* SSOR colour order   */
+  ugk_coarse *cc;                                  /* two-level correction */
   double *part;                                    /* reduction partials  */
   ugk_state *st;                                   /* device scalars      */
   ugk_state *st_host;                              /* pinned mirror       */

@@ -30,6 +30,7 @@ void ufem_work_free(ufem_work *w)
   ugk_free(w->p32[1]); ugk_free(w->q32); ugk_free(w->x32); ugk_free(w->dinv32);
   ugk_free(w->fP); ugk_free(w->fQ);
   ufem_ssor_free(w->ssor);
+  ufem_coarse_free(w->cc);
   ugk_free(w->part); ugk_free(w->st);
   ugk_host_free(w->st_host);
   memset(w, 0, sizeof(*w));
@@ -340,7 +341,8 @@ typedef struct {
   const double *b;
   double *x;
   ufem_work *w;
-  ufem_net *net;
+  ufem_net *net;             /* NULL: the coarse correction alone          */
+  const ugk_coarse *cc;      /* NULL: the network alone                    */
   int m;
   int mixed;
 } fcg_ctx;
@@ -351,9 +353,11 @@ static int fcg_step(void *c, long it, void *stream)
   ufem_work *w = f->w;
   (void)it;
   if (f->mixed && ugk_fcg_refresh(f->K, f->b, f->x, w->r, w->dinv, w->st, w->part, stream)) return -1;
-  if (ugk_fcg_head(f->K, f->n, w->r, w->dinv, ufem_net_input(f->net), w->st, w->part, stream) ||
-      ufem_net_infer(f->net, &w->st->halt, stream) ||
-      ugk_fcg_combine(f->n, w->r, w->dinv, ufem_net_mask(f->net), ufem_net_output(f->net), w->fP, w->fQ, f->m,
+  if (ugk_fcg_head(f->K, f->n, w->r, w->dinv, f->net ? ufem_net_input(f->net) : NULL, w->st, w->part, stream) ||
+      (f->net && ufem_net_infer(f->net, &w->st->halt, stream)) ||
+      (f->cc && ugk_cc_apply(f->cc, f->n, w->r, f->cc->d_zc, w->st, stream)) ||
+      ugk_fcg_combine(f->n, w->r, w->dinv, f->net ? ufem_net_mask(f->net) : NULL,
+                      f->net ? ufem_net_output(f->net) : NULL, w->fP, w->fQ, f->m, f->cc ? f->cc->d_zc : NULL,
                       w->z, w->st, w->part, stream) ||
       ugk_fcg_p(f->n, w->r, w->dinv, w->z, w->fP, f->m, w->st, w->part, stream) ||
       ugk_fcg_q(f->K, f->n, w->fP, w->fQ, f->m, f->mixed, w->st, w->part, stream))
@@ -367,10 +371,15 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   const ugk_mat *K = Km;
   ufem_net *N = (ufem_net *)o->net;
   const int m = o->fcg_m > 0 ? o->fcg_m : 1;   /* the network is a fixed SPD map */
-  fcg_ctx c = {K, n, b, x, w, N, m, mixed};
+  const ugk_coarse *cc = o->coarse ? w->cc : NULL;
+  fcg_ctx c = {K, n, b, x, w, N, cc, m, mixed};
   ugk_state *h = w->st_host;
   int rv;
-  if (!N || ufem_net_bound_n(N) != n) {
+  if (o->coarse && !w->cc) {
+    ufem_set_error("learned solver: coarse correction asked for without a coarse grid (ufem_sys_set_coarse)");
+    return AF_ERR_VAL;
+  }
+  if (N ? ufem_net_bound_n(N) != n : !cc) {
     ufem_set_error("learned solver: no network bound to this system");
     return AF_ERR_VAL;
   }
@@ -399,7 +408,8 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
     h->ru_at = 0;
     if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
   }
-  if (ufem_net_prepare(N, stream)) return AF_ERR;
+  if (N && ufem_net_prepare(N, stream)) return AF_ERR;
+  if (cc && ufem_coarse_build(w->cc, stream) != AF_OK) return AF_ERR;
   if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
   rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, fcg_step, &c, 1);
   if (rv != AF_OK) return rv;
@@ -476,6 +486,50 @@ static int solve_ssor(const ugk_mat *K, long n, const double *b, double *x, ufem
   return info->status;
 }
 
+/* PCG with the two-level preconditioner z = dinv r + z_c (ugk_cc_iter): the
+ * coarse matrix P^T K P is assembled and inverted first, on the device and
+ * inside the timed solve, since it follows the current moduli */
+typedef struct {
+  const ugk_mat *K;
+  long n;
+  double *x;
+  ufem_work *w;
+} cc_ctx;
+
+static int cc_step(void *c, long it, void *stream)
+{
+  cc_ctx *q = (cc_ctx *)c;
+  ufem_work *w = q->w;
+  (void)it;
+  return ugk_cc_iter(q->K, w->cc, q->n, q->x, w->r, w->z, w->p[0], w->q, w->dinv, w->st, w->part, stream);
+}
+
+static int solve_twolevel(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
+                          const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+{
+  cc_ctx c = {K, n, x, w};
+  ugk_state *h = w->st_host;
+  int rv;
+  if (!w->cc) { ufem_set_error("two-level solver: no coarse grid (ufem_sys_set_coarse)"); return AF_ERR_VAL; }
+  if ((rv = ufem_coarse_build(w->cc, stream)) != AF_OK) return rv;
+  rv = prepare(K, w, stream, o->eps, o->rule, maxit);
+  if (rv != AF_OK) return rv;
+  if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
+  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, cc_step, &c, 1);
+  if (rv != AF_OK) return rv;
+  if (get_state(w, stream)) return AF_ERR;
+  if (h->bb <= 0.0) {
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+    return AF_OK;
+  }
+  info->iters = h->iter;
+  info->outer = w->cc->nc;
+  info->relres = sqrt(h->rr / h->bb);
+  info->status = status_of(h->done);
+  return info->status;
+}
+
 int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
                       void *ev0, void *ev1, const ufem_solve_opts *o, ufem_solve_info *info)
 {
@@ -492,6 +546,7 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem
       rv = solve_learned((ugk_mat *)K, n, b, x, w, stream, o, maxit, 1, info);
       break;
     case UFEM_SOLVER_SSOR: rv = solve_ssor(K, n, b, x, w, stream, o, maxit, info); break;
+    case UFEM_SOLVER_TWOLEVEL: rv = solve_twolevel(K, n, b, x, w, stream, o, maxit, info); break;
     default: ufem_set_error("unknown solver %d", o->solver); return AF_ERR_VAL;
   }
   if (ugk_event_record(ev1, stream)) return gpu_fail("event");

@@ -266,8 +266,8 @@ int ugk_fcg_start(const ugk_mat *K, long n, const double *d_b, const double *d_x
 int ugk_fcg_head(const ugk_mat *K, long n, const double *d_r, const double *d_dinv, unsigned short *d_x0,
                  ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_combine(long n, const double *d_r, const double *d_dinv, const float *d_mask,
-                    const float *d_out, const double *d_P, const double *d_Q, int m, double *d_z,
-                    ugk_state *d_st, double *d_part, void *stream);
+                    const float *d_out, const double *d_P, const double *d_Q, int m, const double *d_zc,
+                    double *d_z, ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_p(long n, const double *d_r, const double *d_dinv, const double *d_z, double *d_P, int m,
               ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_q(const ugk_mat *K, long n, const double *d_P, double *d_Q, int m, int f32, ugk_state *d_st,
@@ -292,6 +292,52 @@ int ugk_ssor_permute_vec(long n_brows, int bs, const int *d_perm, const double *
 int ugk_ssor_iter(const ugk_mat *Kc, int ncol, const long *cb, long n, double *d_x, double *d_r, double *d_zz,
                   double *d_z, double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st, double *d_part,
                   void *stream);
+
+/* ---------------- two-level (coarse-grid) correction ---------------------- */
+/* On a box node grid W x H x D (node x + W (y + H z), element
+ * ex + (W-1) (ey + (H-1) ez)), coarsened by cx, cy, cz into a grid of
+ * Wc x Hc x Dc nodes (W - 1 = cx (Wc - 1), ...).  P interpolates trilinearly
+ * from the coarse nodes (cells of cx x cy x cz elements) to the fine nodes;
+ * fm masks the fixed fine DOFs, cm the coarse DOFs whose coincident fine DOF
+ * is fixed (their rows and columns of A are the identity).  The coarse
+ * matrix is the Galerkin product A = P^T K P, assembled per cell from the
+ * elements' Lame constants: K_cell = sum_p lambda_p G_p^l + mu_p G_p^m with
+ * G_p = T_p^T K^(l|m) T_p of the cell's element p (identical box elements),
+ * then inverted in place (dense, block Gauss-Jordan).  The correction is
+ * z_c = fm P A^-1 P^T fm r. */
+typedef struct {
+  int W, H, D, cx, cy, cz, Wc, Hc, Dc;
+  long ncell;               /* (Wc - 1) (Hc - 1) (Dc - 1)                    */
+  long nc;                  /* coarse DOFs, 3 Wc Hc Dc                       */
+  long ncp;                 /* nc padded to a multiple of 64                 */
+  double *d_G;              /* [cx cy cz][2][24 x 24]                        */
+  double *d_KC;             /* [ncell][24 x 24]                              */
+  double *d_A;              /* [ncp][ncp]: A, then A^-1                      */
+  double *d_piv, *d_R, *d_C;/* block Gauss-Jordan: [32][32], [32][ncp], [ncp][32] */
+  double *d_cpart;          /* [24][ncell] restriction partials              */
+  double *d_g, *d_y;        /* [ncp] coarse vectors                          */
+  double *d_fm;             /* [3 W H D] fine mask                           */
+  double *d_zc;             /* [3 W H D] the correction, for flexible CG     */
+  double *d_cm;             /* [nc] coarse mask                              */
+  const double *d_E, *d_nu; /* the system's element moduli                   */
+} ugk_coarse;
+/* K_cell from E, nu; dense A (identity at masked DOFs and in the padding) */
+int ugk_cc_galerkin(ugk_coarse *C, void *stream);
+int ugk_cc_assemble(ugk_coarse *C, void *stream);
+/* in-place inverse of an SPD n x n matrix (n a multiple of 64, leading
+ * dimension n) by block Gauss-Jordan without pivoting; work buffers piv
+ * [32 x 32], R [32 x n], Cb [n x 32] */
+int ugk_dense_spd_inverse(double *d_A, long n, double *d_piv, double *d_R, double *d_Cb, void *stream);
+/* PCG with z = dinv r + z_c: convergence test, restriction, coarse solve,
+ * z and (r, z), p = z + beta p, q = K p, x and r update (r and the
+ * partials from ugk_fcg_start) */
+int ugk_cc_iter(const ugk_mat *K, const ugk_coarse *C, long n, double *d_x, double *d_r, double *d_z,
+                double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
+/* z_c alone into d_zc (skipped once the solve is done); d_out NULL in
+ * ugk_fcg_combine drops the network term, d_zc NULL the coarse one, and
+ * ugk_fcg_head with d_x0 NULL writes no network input */
+int ugk_cc_apply(const ugk_coarse *C, long n, const double *d_r, double *d_zc, const ugk_state *d_st,
+                 void *stream);
 
 /* ---------------- 3-D convolution network kernels (ugk_unet.hip) ---------- */
 /* Activations are bf16 (stored as unsigned short), channels-last on a node

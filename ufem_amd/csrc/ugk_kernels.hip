@@ -1743,7 +1743,7 @@ __global__ __launch_bounds__(WG) void k_fcg_head(long n, const double *__restric
     st->halt = (done || fb) ? 1 : 0;
     st->net_s = sc;
   }
-  if (done || fb) return;
+  if (done || fb || !x0) return;
   const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
   // split format, one voxel (3 DOFs) per thread: hi[4], mid[4], lo[4] of its
   // exact truncation split, channel 3 zero
@@ -1775,6 +1775,7 @@ __global__ __launch_bounds__(WG) void k_fcg_combine(long n, const double *__rest
                                                     const float *__restrict__ out,
                                                     const double *__restrict__ P,
                                                     const double *__restrict__ Q, int m,
+                                                    const double *__restrict__ zc,
                                                     double *__restrict__ z, const ugk_state *st,
                                                     double *part)
 {
@@ -1790,10 +1791,11 @@ __global__ __launch_bounds__(WG) void k_fcg_combine(long n, const double *__rest
     const double ri = r[i];
     const double a = dinv[i] * ri;
     double zi = a;
-    if (!fb) {
+    if (!fb && out) {
       const long vx = i / 3;
       zi += sc * (double)mask[i] * sqrt(dinv[i]) * (double)out[vx * 4 + (i - 3 * vx)];
     }
+    if (zc) zi += zc[i];
     z[i] = zi;
     v[0] += zi * ri;
     v[1] += a * ri;
@@ -2158,6 +2160,326 @@ static SellView view(const ugk_mat *K)
   M.n_slices = K->n_slices;
   return M;
 }
+// --------------------------------------------------------------------------
+// Two-level correction on a box node grid (see ugk.h): z_c = fm P A^-1 P^T
+// fm r with P trilinear from a coarse node grid, A the Galerkin product
+// assembled per cell from the elements' Lame constants and inverted once
+// per solve by block Gauss-Jordan.  Fine node (i, j, k) belongs to the cell
+// min(i / cx, Wc - 2), ... (the last cell also owns the last node plane);
+// its weight for the cell's corner with offset 1 along x is t / cx with
+// t = i - cx * cell, 1 - t / cx for offset 0.  Corners are numbered in the
+// brick's node order: (0,0,0) (1,0,0) (1,1,0) (0,1,0), then z = 1.
+// --------------------------------------------------------------------------
+struct CCGrid {
+  int W, H, D, cx, cy, cz, Wc, Hc, Dc;
+  long ncell, nc, ncp;
+};
+
+__device__ __forceinline__ int cc_corner(int dx, int dy, int dz)
+{
+  return 4 * dz + (dy ? (dx ? 2 : 3) : (dx ? 1 : 0));
+}
+__device__ __forceinline__ void cc_offsets(int m, int &dx, int &dy, int &dz)
+{
+  dz = m >> 2;
+  const int q = m & 3;
+  dx = (q == 1 || q == 2);
+  dy = (q >= 2);
+}
+__device__ __forceinline__ int cc_cell(int i, int f, int Nc)
+{
+  const int c = i / f;
+  return c > Nc - 2 ? Nc - 2 : c;
+}
+
+// K_cell = sum over the cell's elements p of lambda_p G_p^l + mu_p G_p^m
+__global__ __launch_bounds__(WG) void k_cc_galerkin(CCGrid g, const double *__restrict__ G,
+                                                    const double *__restrict__ E, const double *__restrict__ nu,
+                                                    double *__restrict__ KC)
+{
+  __shared__ double lam[1024], mu[1024];
+  const long cell = blockIdx.x;
+  const int ci = (int)(cell % (g.Wc - 1)), cj = (int)((cell / (g.Wc - 1)) % (g.Hc - 1));
+  const int ck = (int)(cell / ((long)(g.Wc - 1) * (g.Hc - 1)));
+  const int np = g.cx * g.cy * g.cz;
+  const long nx = g.W - 1, ny = g.H - 1;
+  for (int p = threadIdx.x; p < np; p += WG) {
+    const int pi = p % g.cx, pj = (p / g.cx) % g.cy, pk = p / (g.cx * g.cy);
+    const long e = (long)(ci * g.cx + pi) + nx * ((long)(cj * g.cy + pj) + ny * (long)(ck * g.cz + pk));
+    const double Ee = E[e], v = nu[e];
+    lam[p] = Ee * v / ((1.0 + v) * (1.0 - 2.0 * v));
+    mu[p] = Ee / (2.0 * (1.0 + v));
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < 576; o += WG) {
+    double a = 0.0;
+    for (int p = 0; p < np; p++)
+      a += lam[p] * G[(size_t)(2 * p) * 576 + o] + mu[p] * G[(size_t)(2 * p + 1) * 576 + o];
+    KC[cell * 576 + o] = a;
+  }
+}
+
+// A = 0 with the padding's diagonal 1
+__global__ __launch_bounds__(WG) void k_cc_clear(double *A, long nc, long ncp)
+{
+  const long tot = ncp * ncp;
+  for (long q = (long)blockIdx.x * WG + threadIdx.x; q < tot; q += (long)gridDim.x * WG) {
+    const long i = q / ncp, j = q % ncp;
+    A[q] = (i == j && i >= nc) ? 1.0 : 0.0;
+  }
+}
+
+// A's blocks between coarse node I and its 27 neighbours: the sum over the
+// cells holding both, in a fixed order; identity rows/columns where masked
+__global__ __launch_bounds__(WG) void k_cc_assemble(CCGrid g, const double *__restrict__ KC,
+                                                    const double *__restrict__ cm, double *A)
+{
+  const long nn = (long)g.Wc * g.Hc * g.Dc;
+  const long t = (long)blockIdx.x * WG + threadIdx.x;
+  if (t >= nn * 243) return;
+  const long I = t / 243;
+  const int rem = (int)(t % 243), nb = rem / 9, a = (rem % 9) / 3, b = rem % 3;
+  const int Ii = (int)(I % g.Wc), Ij = (int)((I / g.Wc) % g.Hc), Ik = (int)(I / ((long)g.Wc * g.Hc));
+  const int Ji = Ii + nb % 3 - 1, Jj = Ij + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
+  if (Ji < 0 || Ji >= g.Wc || Jj < 0 || Jj >= g.Hc || Jk < 0 || Jk >= g.Dc) return;
+  const long J = Ji + (long)g.Wc * (Jj + (long)g.Hc * Jk);
+  const long row = 3 * I + a, col = 3 * J + b;
+  double v = 0.0;
+  if (cm[row] != 0.0 && cm[col] != 0.0) {
+    const int k0 = max(max(Ik, Jk) - 1, 0), k1 = min(min(Ik, Jk), g.Dc - 2);
+    const int j0 = max(max(Ij, Jj) - 1, 0), j1 = min(min(Ij, Jj), g.Hc - 2);
+    const int i0 = max(max(Ii, Ji) - 1, 0), i1 = min(min(Ii, Ji), g.Wc - 2);
+    for (int ck = k0; ck <= k1; ck++)
+      for (int cj = j0; cj <= j1; cj++)
+        for (int ci = i0; ci <= i1; ci++) {
+          const int mI = cc_corner(Ii - ci, Ij - cj, Ik - ck), mJ = cc_corner(Ji - ci, Jj - cj, Jk - ck);
+          const long cell = ci + (long)(g.Wc - 1) * (cj + (long)(g.Hc - 1) * ck);
+          v += KC[cell * 576 + (3 * mI + a) * 24 + 3 * mJ + b];
+        }
+  } else if (row == col) {
+    v = 1.0;
+  }
+  A[row * g.ncp + col] = v;
+}
+
+// ---- block Gauss-Jordan inversion of an SPD matrix, blocks of GJB --------
+#define GJB 32
+// the pivot block's inverse, scalar Gauss-Jordan in LDS
+__global__ __launch_bounds__(GJB * GJB) void k_gj_pivot(const double *__restrict__ A, long ld, long K0,
+                                                        double *__restrict__ piv)
+{
+  __shared__ double a[GJB][GJB + 1];
+  const int i = threadIdx.x / GJB, j = threadIdx.x % GJB;
+  a[i][j] = A[(K0 + i) * ld + K0 + j];
+  __syncthreads();
+  for (int k = 0; k < GJB; k++) {
+    const double p = 1.0 / a[k][k];
+    double v;
+    if (i == k && j == k) v = p;
+    else if (i == k) v = a[k][j] * p;
+    else if (j == k) v = -a[i][k] * p;
+    else v = a[i][j] - a[i][k] * a[k][j] * p;
+    __syncthreads();
+    a[i][j] = v;
+    __syncthreads();
+  }
+  piv[i * GJB + j] = a[i][j];
+}
+
+// R = piv * A[K, :] (GJB x n) and Cb = A[:, K] (n x GJB), copies taken
+// before the in-place update
+__global__ __launch_bounds__(WG) void k_gj_panel(const double *__restrict__ A, long n, long K0,
+                                                 const double *__restrict__ piv, double *__restrict__ R,
+                                                 double *__restrict__ Cb)
+{
+  __shared__ double P[GJB][GJB + 1];
+  for (int q = threadIdx.x; q < GJB * GJB; q += WG) P[q / GJB][q % GJB] = piv[q];
+  __syncthreads();
+  const long j = (long)blockIdx.x * WG + threadIdx.x;
+  if (j >= n) return;
+  double col[GJB];
+#pragma unroll
+  for (int s = 0; s < GJB; s++) col[s] = A[(K0 + s) * n + j];
+#pragma unroll 4
+  for (int t = 0; t < GJB; t++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int s = 0; s < GJB; s++) acc += P[t][s] * col[s];
+    R[(long)t * n + j] = acc;
+  }
+#pragma unroll
+  for (int t = 0; t < GJB; t++) Cb[j * GJB + t] = A[j * n + K0 + t];
+}
+
+// one 64 x 64 tile of the update: A_ij -= C_iK R_Kj off the pivot rows and
+// columns, A_iK = -C_iK piv, A_Kj = R_Kj, A_KK = piv.  The tile's R block is
+// staged with piv in place of R at the pivot columns, so that one outer
+// product serves both off-pivot cases and the pivot rows copy it; each
+// thread owns 4 x 4 outputs (rows tr + 16 a, columns tc + 16 b)
+__global__ __launch_bounds__(WG) void k_gj_update(double *__restrict__ A, long n, long K0,
+                                                  const double *__restrict__ piv, const double *__restrict__ R,
+                                                  const double *__restrict__ Cb)
+{
+  __shared__ double Cs[64][GJB + 1];
+  __shared__ double Rs[GJB][64 + 1];
+  const long I0 = (long)blockIdx.y * 64, J0 = (long)blockIdx.x * 64;
+  for (int q = threadIdx.x; q < 64 * GJB; q += WG) Cs[q / GJB][q % GJB] = Cb[(I0 + q / GJB) * GJB + q % GJB];
+  for (int q = threadIdx.x; q < GJB * 64; q += WG) {
+    const int t = q / 64, c = q % 64;
+    const long j = J0 + c;
+    Rs[t][c] = (j >= K0 && j < K0 + GJB) ? piv[t * GJB + (j - K0)] : R[(long)t * n + j];
+  }
+  __syncthreads();
+  const int tc = threadIdx.x % 16, tr = threadIdx.x / 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+#pragma unroll 4
+  for (int t = 0; t < GJB; t++) {
+    double ca[4], rb[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) ca[a] = Cs[tr + 16 * a][t];
+#pragma unroll
+    for (int b = 0; b < 4; b++) rb[b] = Rs[t][tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] += ca[a] * rb[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int li = tr + 16 * a;
+    const long i = I0 + li;
+    const bool inI = i >= K0 && i < K0 + GJB;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int lj = tc + 16 * b;
+      const long j = J0 + lj;
+      const bool inJ = j >= K0 && j < K0 + GJB;
+      double v;
+      if (inI) v = Rs[i - K0][lj];
+      else if (inJ) v = -acc[a][b];
+      else v = A[i * n + j] - acc[a][b];
+      A[i * n + j] = v;
+    }
+  }
+}
+
+// restriction, part 1: per cell, the weighted sums of its own fine nodes'
+// masked residuals for each of its 8 corners x 3 directions
+__global__ __launch_bounds__(WG) void k_cc_restrict(CCGrid g, const double *__restrict__ r,
+                                                    const double *__restrict__ fm, double *cpart,
+                                                    const int *stop)
+{
+  if (*stop) return;
+  const long cell = blockIdx.x;
+  const int ci = (int)(cell % (g.Wc - 1)), cj = (int)((cell / (g.Wc - 1)) % (g.Hc - 1));
+  const int ck = (int)(cell / ((long)(g.Wc - 1) * (g.Hc - 1)));
+  const int ni = g.cx + (ci == g.Wc - 2), nj = g.cy + (cj == g.Hc - 2), nk = g.cz + (ck == g.Dc - 2);
+  const int tot = ni * nj * nk;
+  double acc[24];
+#pragma unroll
+  for (int q = 0; q < 24; q++) acc[q] = 0.0;
+  for (int q = threadIdx.x; q < tot; q += WG) {
+    const int li = q % ni, lj = (q / ni) % nj, lk = q / (ni * nj);
+    const long node = (long)(ci * g.cx + li) + (long)g.W * ((long)(cj * g.cy + lj) + (long)g.H * (ck * g.cz + lk));
+    const double wx1 = (double)li / g.cx, wy1 = (double)lj / g.cy, wz1 = (double)lk / g.cz;
+    double rv[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) rv[a] = fm[3 * node + a] * r[3 * node + a];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      int dx, dy, dz;
+      cc_offsets(m, dx, dy, dz);
+      const double w = (dx ? wx1 : 1.0 - wx1) * (dy ? wy1 : 1.0 - wy1) * (dz ? wz1 : 1.0 - wz1);
+#pragma unroll
+      for (int a = 0; a < 3; a++) acc[3 * m + a] += w * rv[a];
+    }
+  }
+  block_partials<24>(acc, cpart);
+}
+
+// restriction, part 2: g = cm * (sum of the adjacent cells' corner parts)
+__global__ __launch_bounds__(WG) void k_cc_gather(CCGrid g, const double *__restrict__ cpart,
+                                                  const double *__restrict__ cm, double *__restrict__ gv,
+                                                  const int *stop)
+{
+  if (*stop) return;
+  const long t = (long)blockIdx.x * WG + threadIdx.x;
+  if (t >= g.ncp) return;
+  if (t >= g.nc) { gv[t] = 0.0; return; }
+  const long I = t / 3;
+  const int a = (int)(t % 3);
+  const int Ii = (int)(I % g.Wc), Ij = (int)((I / g.Wc) % g.Hc), Ik = (int)(I / ((long)g.Wc * g.Hc));
+  double s = 0.0;
+  for (int ck = max(Ik - 1, 0); ck <= min(Ik, g.Dc - 2); ck++)
+    for (int cj = max(Ij - 1, 0); cj <= min(Ij, g.Hc - 2); cj++)
+      for (int ci = max(Ii - 1, 0); ci <= min(Ii, g.Wc - 2); ci++) {
+        const long cell = ci + (long)(g.Wc - 1) * (cj + (long)(g.Hc - 1) * ck);
+        const int m = cc_corner(Ii - ci, Ij - cj, Ik - ck);
+        s += cpart[(long)(3 * m + a) * g.ncell + cell];
+      }
+  gv[t] = cm[t] * s;
+}
+
+// y = A^-1 g, one wavefront per row
+__global__ __launch_bounds__(WG) void k_cc_gemv(const double *__restrict__ A, long n, const double *__restrict__ gv,
+                                                double *__restrict__ y, const int *stop)
+{
+  if (stop && *stop) return;
+  const int lane = threadIdx.x & 63;
+  const long row = (long)blockIdx.x * NW + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double *a = A + row * n;
+  double s = 0.0;
+  for (long j = lane; j < n; j += 64) s += a[j] * gv[j];
+  s = wave_sum(s);
+  if (lane == 0) y[row] = s;
+}
+
+// prolongation: z_c = fm P y; MODE 1: z = dinv r + z_c with (r, z) partials
+// for the PCG direction update, MODE 0: z_c alone
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_cc_z(CCGrid g, const double *__restrict__ y, const double *__restrict__ fm,
+                                             const double *__restrict__ r, const double *__restrict__ dinv,
+                                             double *__restrict__ z, const int *stop, double *part)
+{
+  if (*stop) return;
+  const long nn = (long)g.W * g.H * g.D;
+  double v[1] = {0.0};
+  for (long node = (long)blockIdx.x * WG + threadIdx.x; node < nn; node += (long)gridDim.x * WG) {
+    const int i = (int)(node % g.W), j = (int)((node / g.W) % g.H), k = (int)(node / ((long)g.W * g.H));
+    const int ci = cc_cell(i, g.cx, g.Wc), cj = cc_cell(j, g.cy, g.Hc), ck = cc_cell(k, g.cz, g.Dc);
+    const double wx1 = (double)(i - ci * g.cx) / g.cx, wy1 = (double)(j - cj * g.cy) / g.cy;
+    const double wz1 = (double)(k - ck * g.cz) / g.cz;
+    double zc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      int dx, dy, dz;
+      cc_offsets(m, dx, dy, dz);
+      const double w = (dx ? wx1 : 1.0 - wx1) * (dy ? wy1 : 1.0 - wy1) * (dz ? wz1 : 1.0 - wz1);
+      const long I = (ci + dx) + (long)g.Wc * ((cj + dy) + (long)g.Hc * (ck + dz));
+#pragma unroll
+      for (int a = 0; a < 3; a++) zc[a] += w * y[3 * I + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const long d = 3 * node + a;
+      const double c = fm[d] * zc[a];
+      if (MODE) {
+        const double zi = dinv[d] * r[d] + c;
+        z[d] = zi;
+        v[0] += r[d] * zi;
+      } else {
+        z[d] = c;
+      }
+    }
+  }
+  if (MODE) block_partials<1>(v, PART_S(part));
+}
+
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 // blocks for one slice per wavefront, rounded up to a multiple of the 8 XCDs
 // so that remap_block() is a bijection on [0, gridDim.x)
@@ -2790,10 +3112,11 @@ int ugk_fcg_head(const ugk_mat *K, long n, const double *r, const double *dinv, 
 }
 
 int ugk_fcg_combine(long n, const double *r, const double *dinv, const float *mask, const float *out,
-                    const double *P, const double *Q, int m, double *z, ugk_state *st, double *part, void *s)
+                    const double *P, const double *Q, int m, const double *zc, double *z, ugk_state *st,
+                    double *part, void *s)
 {
   if (m < 1 || m > FCG_MAXM) { snprintf(g_err, sizeof(g_err), "FCG: 1 <= m <= %d", FCG_MAXM); return -1; }
-  k_fcg_combine<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, mask, out, P, Q, m, z, st, part);
+  k_fcg_combine<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, mask, out, P, Q, m, zc, z, st, part);
   CK_LAUNCH("k_fcg_combine");
   return 0;
 }
@@ -2835,6 +3158,86 @@ int ugk_fcg_update(const ugk_mat *K, long n, double *x, double *r, const double 
 {
   k_fcg_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, P, Q, m, dinv, st, part, spmv_blocks(K));
   CK_LAUNCH("k_fcg_update");
+  return 0;
+}
+
+static CCGrid ccgrid(const ugk_coarse *C)
+{
+  CCGrid g;
+  g.W = C->W; g.H = C->H; g.D = C->D;
+  g.cx = C->cx; g.cy = C->cy; g.cz = C->cz;
+  g.Wc = C->Wc; g.Hc = C->Hc; g.Dc = C->Dc;
+  g.ncell = C->ncell; g.nc = C->nc; g.ncp = C->ncp;
+  return g;
+}
+
+int ugk_cc_galerkin(ugk_coarse *C, void *s)
+{
+  if (C->cx * C->cy * C->cz > 1024) { snprintf(g_err, sizeof(g_err), "coarse cell of more than 1024 elements"); return -1; }
+  k_cc_galerkin<<<(unsigned)C->ncell, WG, 0, S(s)>>>(ccgrid(C), C->d_G, C->d_E, C->d_nu, C->d_KC);
+  CK_LAUNCH("k_cc_galerkin");
+  return 0;
+}
+
+int ugk_cc_assemble(ugk_coarse *C, void *s)
+{
+  const long nn = (long)C->Wc * C->Hc * C->Dc;
+  k_cc_clear<<<2048, WG, 0, S(s)>>>(C->d_A, C->nc, C->ncp);
+  CK_LAUNCH("k_cc_clear");
+  k_cc_assemble<<<(unsigned)((nn * 243 + WG - 1) / WG), WG, 0, S(s)>>>(ccgrid(C), C->d_KC, C->d_cm, C->d_A);
+  CK_LAUNCH("k_cc_assemble");
+  return 0;
+}
+
+int ugk_dense_spd_inverse(double *A, long n, double *piv, double *R, double *Cb, void *s)
+{
+  long K0;
+  if (n <= 0 || n % 64) { snprintf(g_err, sizeof(g_err), "dense inverse: n must be a positive multiple of 64"); return -1; }
+  for (K0 = 0; K0 < n; K0 += GJB) {
+    k_gj_pivot<<<1, GJB * GJB, 0, S(s)>>>(A, n, K0, piv);
+    k_gj_panel<<<(unsigned)((n + WG - 1) / WG), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
+    k_gj_update<<<dim3((unsigned)(n / 64), (unsigned)(n / 64)), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
+  }
+  CK_LAUNCH("k_gj_update");
+  return 0;
+}
+
+static int cc_restrict_solve(const ugk_coarse *C, const double *r, const int *stop, void *s)
+{
+  const CCGrid g = ccgrid(C);
+  k_cc_restrict<<<(unsigned)C->ncell, WG, 0, S(s)>>>(g, r, C->d_fm, C->d_cpart, stop);
+  k_cc_gather<<<(unsigned)((C->ncp + WG - 1) / WG), WG, 0, S(s)>>>(g, C->d_cpart, C->d_cm, C->d_g, stop);
+  k_cc_gemv<<<(unsigned)((C->ncp + NW - 1) / NW), WG, 0, S(s)>>>(C->d_A, C->ncp, C->d_g, C->d_y, stop);
+  CK_LAUNCH("k_cc_gemv");
+  return 0;
+}
+
+int ugk_cc_iter(const ugk_mat *K, const ugk_coarse *C, long n, double *x, double *r, double *z, double *p,
+                double *q, const double *dinv, ugk_state *st, double *part, void *s)
+{
+  const unsigned GV = vec_blocks(n), GA = spmv_blocks(K);
+  const SellView M = view(K);
+  if (K->bs != 3 || n != 3L * C->W * C->H * C->D) {
+    snprintf(g_err, sizeof(g_err), "two-level: a 3x3-block matrix on the coarse grid's nodes is needed");
+    return -1;
+  }
+  k_ssor_test<<<1, WG, 0, S(s)>>>(st, part, GA, GV);
+  CK_LAUNCH("k_ssor_test");
+  if (cc_restrict_solve(C, r, &st->done, s)) return -1;
+  k_cc_z<1><<<GV, WG, 0, S(s)>>>(ccgrid(C), C->d_y, C->d_fm, r, dinv, z, &st->done, part);
+  k_ssor_p<<<GV, WG, 0, S(s)>>>(n, z, p, st, part, GV);
+  k_pcg_q<3><<<GA, WG, 0, S(s)>>>(M, K->d_vals, p, q, st, part);
+  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA);
+  CK_LAUNCH("k_ssor_upd");
+  return 0;
+}
+
+int ugk_cc_apply(const ugk_coarse *C, long n, const double *r, double *zc, const ugk_state *st, void *s)
+{
+  if (n != 3L * C->W * C->H * C->D) { snprintf(g_err, sizeof(g_err), "two-level: size mismatch"); return -1; }
+  if (cc_restrict_solve(C, r, &st->done, s)) return -1;
+  k_cc_z<0><<<vec_blocks(n), WG, 0, S(s)>>>(ccgrid(C), C->d_y, C->d_fm, r, NULL, zc, &st->done, NULL);
+  CK_LAUNCH("k_cc_z");
   return 0;
 }
 

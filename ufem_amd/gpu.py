@@ -20,7 +20,7 @@ P_I = C.POINTER(C.c_int)
 P_F = C.POINTER(C.c_float)
 
 AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
-SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4, "ssor": 5}
+SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4, "ssor": 5, "twolevel": 6}
 EXEC = {"streamed": 0, "persistent": 1, "fused": 2, "classic": 3}
 
 
@@ -46,7 +46,8 @@ class _Mesh(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("solver", C.c_int), ("rule", C.c_int), ("eps", C.c_double), ("maxit", C.c_long),
                 ("exec", C.c_int), ("chunk", C.c_int), ("inner_eps", C.c_double),
-                ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int), ("fcg_m", C.c_int)]
+                ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int), ("fcg_m", C.c_int),
+                ("coarse", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -133,6 +134,10 @@ def lib():
                                         C.c_double, C.c_long]),
             "femMatVecMultBig": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector)]),
             "ufem_sys_update": (C.c_int, [C.c_void_p, C.POINTER(_Mesh)]),
+            "ufem_sys_set_coarse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ufem_sys_coarse_size": (C.c_long, [C.c_void_p]),
+            "ufem_sys_coarse_matrix": (C.c_int, [C.c_void_p, C.c_int, P_D]),
+            "ufem_gpu_dense_spd_inverse": (C.c_int, [C.c_long, P_D, P_D]),
             "monte_dlib_interface_type": (C.c_long, []),
             "monte_nums_of_vars": (None, [C.c_char_p, P_L, P_L, P_L]),
             "monte_ivar_name": (C.c_char_p, [C.c_char_p, C.c_long]),
@@ -205,6 +210,19 @@ class GpuSystem:
         self.nreact = lib().ufem_sys_nreact(self.h)
         self.ncolors = lib().ufem_sys_ncolors(self.h)
 
+    def set_coarse(self, W, H, D, cx, cy, cz):
+        """coarse node grid for the two-level correction (box meshes only)"""
+        rv = lib().ufem_sys_set_coarse(self.h, W, H, D, cx, cy, cz)
+        if rv != AF_OK:
+            raise RuntimeError(f"ufem_sys_set_coarse = {rv}: {last_error()}")
+        return lib().ufem_sys_coarse_size(self.h)
+
+    def coarse_matrix(self, inverse=False):
+        nc = lib().ufem_sys_coarse_size(self.h)
+        A = np.zeros((nc, nc))
+        self._chk(lib().ufem_sys_coarse_matrix(self.h, 1 if inverse else 0, _pd(A)), "coarse matrix")
+        return A
+
     def update(self, mesh):
         """new coordinates, moduli and load/support values on the same mesh;
         F, K and the reactions start over (fill_K, add_loads, add_disps next)"""
@@ -249,9 +267,9 @@ class GpuSystem:
         self.add_disps()
 
     def solve(self, solver="pcg", eps=1e-8, rule=1, maxit=0, exec="streamed", chunk=0,
-              inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, fcg_m=0, check=True):
+              inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, fcg_m=0, coarse=False, check=True):
         o = SolveOpts(SOLVER[solver], rule, eps, maxit, EXEC[exec], chunk, inner_eps, inner_maxit,
-                      net.h if net is not None else None, 1 if warm_start else 0, fcg_m)
+                      net.h if net is not None else None, 1 if warm_start else 0, fcg_m, 1 if coarse else 0)
         info = SolveInfo()
         rv = lib().ufem_sys_solve(self.h, C.byref(o), C.byref(info))
         if check and rv not in (AF_OK, AF_ERR):
@@ -657,3 +675,13 @@ class Monte:
 
     def close(self):
         lib().monte_clean_lib_stuff(self.param)
+
+
+def dense_spd_inverse(A):
+    """inverse of a dense SPD matrix on the device (block Gauss-Jordan)"""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    out = np.zeros_like(A)
+    rv = lib().ufem_gpu_dense_spd_inverse(A.shape[0], _pd(A), _pd(out))
+    if rv != AF_OK:
+        raise RuntimeError(f"dense inverse = {rv}: {last_error()}")
+    return out
