@@ -11,9 +11,11 @@ K, f already resident in HBM.
 Printed: one JSON line with the solve time (value, seconds), the roofline of
 the solve's dominant kernel (w = K u with (w, u) of the single-reduction PCG),
 SpMV bandwidth, the mixed-precision solve on the same system (and on a COV
-0.6 field), CG with the SSOR preconditioner on the same system, learned + mixed precision on the same system (a network trained
-on the GPU from fresh lognormal fields; plain FP64 PCG vs flexible CG with
-the network, FP64 and with the FP32 matrix), the learned preconditioner on
+0.6 field), CG with the SSOR preconditioner and PCG with the two-level
+(coarse-grid) correction on the same system, learned + mixed precision on the
+same system (a network trained on the GPU from fresh lognormal fields; plain
+FP64 PCG vs flexible CG with the network, FP64 and with the FP32 matrix, each
+without and with the coarse correction), the learned preconditioner on
 the 128 x 32 x 32 random-field cantilever (training on the GPU, then plain
 vs learned flexible CG on an unseen field), the batched
 Monte-Carlo solve of 256 uneven meshes in one launch, the Monte plugin
@@ -23,6 +25,7 @@ baseline (the oracle's pthread restatement of uFEM's solver, bounded sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-mixed]
                     [--no-learned] [--no-batch] [--kernels-only] [--net-kernels-only]
+                    [--twolevel-only]
 """
 
 from __future__ import annotations
@@ -139,6 +142,14 @@ def bench_learned(fields, steps, seed):
     rp, tp, up = timed(solver="pcg", eps=1e-8, rule=1)
     rl, tl, ul = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=1)
     stages = net.time_stages()
+    s.set_coarse(nx + 1, ny + 1, nz + 1, 8, 8, 8)
+    with_coarse = {}
+    for name, kw in (("twolevel", dict(solver="twolevel")),
+                     ("learned_coarse", dict(solver="learned", net=net, fcg_m=1, coarse=True))):
+        r, t, u = timed(eps=1e-8, rule=1, **kw)
+        with_coarse[name] = {"iters": r["iters"], "solve_s": round(t, 4), "rv": r["rv"],
+                             "true_relres": r["true_relres"], "speedup_vs_plain": round(tp / t, 3),
+                             "u_rel_err_vs_plain": float(np.linalg.norm(u - up) / np.linalg.norm(up))}
     out = {
         "workload": (f"cantilever 2.0x0.4x0.4 m, {nx}x{ny}x{nz} bricks, {mesh.n_dof} DOF, E = exp(GRF, "
                      f"corr 0.2 m, log-std 0.5, mean 30 GPa); test field seed {seed}, training fields "
@@ -153,19 +164,23 @@ def bench_learned(fields, steps, seed):
         "speedup_vs_plain": round(tp / tl, 3),
         "us_per_iteration": {"plain": round(tp / rp["iters"] * 1e6, 1), "learned": round(tl / rl["iters"] * 1e6, 1)},
         "network_stage_us": {k: round(v, 1) for k, v in stages.items()},
+        "with_coarse_8x8x8": with_coarse,
     }
     net.close()
     s.close()
     return out
 
 
-def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3):
+def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 8, 8)):
     """Learned + mixed precision on the metric's own system `g` (the 1M-DOF
     lognormal cantilever, E = mesh.E): pairs from plain-PCG solves on `fields`
     fresh i.i.d. lognormal fields of the same distribution (seeds seed_base..),
     `steps` Adam steps, then plain FP64 PCG, flexible CG with the network
     (FP64) and flexible CG with the network and q = K p from the FP32 copy of
-    K (FP64 residual refreshes), all to ||r||/||f|| <= 1e-8 from u = 0."""
+    K (FP64 residual refreshes), all to ||r||/||f|| <= 1e-8 from u = 0; then
+    the same with the two-level correction on a coarse grid every `coarse`
+    bricks (PCG without the network, FP32 matrix without the network, the
+    network in FP64 and with the FP32 matrix)."""
     from ufem_amd import gpu as G
     from ufem_amd import learned as Lr
     from ufem_amd.mesh import lognormal_E
@@ -194,6 +209,26 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3):
     def err(u):
         return float(np.linalg.norm(u - up) / np.linalg.norm(up))
 
+    # the same solves with the two-level (coarse-grid) correction added; the
+    # coarse matrix is assembled and inverted inside each timed solve
+    nc = g.set_coarse(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, *coarse)
+    variants = {
+        "twolevel": dict(solver="twolevel"),
+        "coarse_mixed": dict(solver="learned_mixed", coarse=True),
+        "learned_coarse": dict(solver="learned", net=net, fcg_m=1, coarse=True),
+        "learned_coarse_mixed": dict(solver="learned_mixed", net=net, fcg_m=1, coarse=True),
+    }
+    with_coarse = {"coarse_grid": {"factors": list(coarse), "coarse_dof": int(nc)}}
+    for name, kw in variants.items():
+        r, t, u = timed(eps=1e-8, rule=1, **kw)
+        with_coarse[name] = {"iters": r["iters"], "solve_s": round(t, 4), "rv": r["rv"],
+                             "true_relres": r["true_relres"], "u_rel_err_vs_fp64": err(u),
+                             "speedup_vs_plain": round(tp / t, 3),
+                             "us_per_iteration": round(t / max(r["iters"], 1) * 1e6, 1)}
+        if "mixed" in name:
+            with_coarse[name]["fp64_refreshes"] = r["outer"]
+        log(name, with_coarse[name])
+
     out = {
         "workload": (f"the metric's system: {mesh.nx}x{mesh.ny}x{mesh.nz} bricks, {mesh.n_dof} DOF, E lognormal "
                      f"COV {cov}; training fields: {fields} fresh lognormal fields, seeds "
@@ -210,6 +245,7 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3):
         "us_per_iteration": {"plain": round(tp / rp["iters"] * 1e6, 1), "learned": round(tl / rl["iters"] * 1e6, 1),
                              "learned_mixed": round(tm / rm["iters"] * 1e6, 1)},
         "network_stage_us": {k: round(v, 1) for k, v in stages.items()},
+        "with_coarse": with_coarse,
     }
     net.close()
     return out
@@ -334,6 +370,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the dominant solve kernel and a device copy of known size "
                          "(for hardware-counter runs)")
+    ap.add_argument("--twolevel-only", action="store_true",
+                    help="assemble the metric's system and run two two-level solves (for profiling)")
     ap.add_argument("--net-kernels-only", action="store_true",
                     help="launch only the network's inference stages on the 128x32x32 random-field "
                          "cantilever (initial weights; for kernel traces)")
@@ -381,6 +419,14 @@ def main():
     if args.kernels_only:
         g.time_kernel("copy", reps=5)
         g.time_kernel("cg1_w", reps=5)
+        g.close()
+        return
+    if args.twolevel_only:
+        # one warm and one profiled two-level solve of the metric's system
+        g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 8, 8)
+        for _ in range(2):
+            r = g.solve(solver="twolevel", eps=1e-8, rule=1)
+            log(f"two-level PCG: {r['iters']} iterations, {r['solve_ms']:.2f} ms device")
         g.close()
         return
     opts = dict(solver="pcg", eps=1e-8, rule=1, exec=args.exec)
@@ -463,6 +509,19 @@ def main():
             "speedup_vs_jacobi": round(solve_s / s_s, 3)}
     log("ssor", ssor)
 
+    # PCG with the two-level correction (coarse grid every 8 bricks; the
+    # coarse matrix is assembled and inverted inside the timed solve)
+    nc = g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 8, 8)
+    g.solve(solver="twolevel", eps=1e-8, rule=1)
+    t0 = time.perf_counter()
+    ti = g.solve(solver="twolevel", eps=1e-8, rule=1)
+    t_s = time.perf_counter() - t0
+    twolevel = {"iters": ti["iters"], "coarse_dof": int(nc), "solve_s": round(t_s, 4), "rv": ti["rv"],
+                "true_relres": ti["true_relres"],
+                "u_rel_err_vs_jacobi": float(np.linalg.norm(g.u() - u64) / np.linalg.norm(u64)),
+                "speedup_vs_jacobi": round(solve_s / t_s, 3)}
+    log("twolevel", twolevel)
+
     mixed_cov06 = None
     if not args.no_mixed:
         mixed_cov06 = bench_mixed_high_contrast(nx, ny, nz, args.seed + 1)
@@ -526,6 +585,7 @@ def main():
         "mixed": mixed,
         "mixed_cov06": mixed_cov06,
         "ssor": ssor,
+        "twolevel": twolevel,
         "results_pass": results,
         "learned_mixed_1m": learned_mixed,
         "learned": learned,
