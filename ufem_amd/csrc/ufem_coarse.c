@@ -180,9 +180,9 @@ int ufem_sys_set_coarse(ufem_sys *s, int W, int H, int D, int cx, int cy, int cz
   C->d_G = (double *)ugk_malloc(sizeof(double) * (size_t)cx * cy * cz * 2 * 576);
   C->d_KC = (double *)ugk_malloc(sizeof(double) * (size_t)C->ncell * 576);
   C->d_A = (double *)ugk_malloc(sizeof(double) * (size_t)C->ncp * (size_t)C->ncp);
-  C->d_piv = (double *)ugk_malloc(sizeof(double) * 32 * 32);
-  C->d_R = (double *)ugk_malloc(sizeof(double) * 32 * (size_t)C->ncp);
-  C->d_C = (double *)ugk_malloc(sizeof(double) * 32 * (size_t)C->ncp);
+  C->d_piv = (double *)ugk_malloc(sizeof(double) * UGK_GJ_BLOCK * UGK_GJ_BLOCK);
+  C->d_R = (double *)ugk_malloc(sizeof(double) * UGK_GJ_BLOCK * (size_t)C->ncp);
+  C->d_C = (double *)ugk_malloc(sizeof(double) * UGK_GJ_BLOCK * (size_t)C->ncp);
   C->d_cpart = (double *)ugk_malloc(sizeof(double) * 24 * (size_t)C->ncell);
   C->d_g = (double *)ugk_malloc(sizeof(double) * (size_t)C->ncp);
   C->d_y = (double *)ugk_malloc(sizeof(double) * (size_t)C->ncp);
@@ -253,9 +253,9 @@ int ufem_gpu_dense_spd_inverse(long n, const double *A, double *Ainv)
     for (j = 0; j < np; j++) h[i * np + j] = (i < n && j < n) ? A[i * n + j] : (i == j ? 1.0 : 0.0);
   strm = ugk_stream_create();
   dA = (double *)ugk_malloc(sizeof(double) * (size_t)np * (size_t)np);
-  piv = (double *)ugk_malloc(sizeof(double) * 32 * 32);
-  R = (double *)ugk_malloc(sizeof(double) * 32 * (size_t)np);
-  Cb = (double *)ugk_malloc(sizeof(double) * 32 * (size_t)np);
+  piv = (double *)ugk_malloc(sizeof(double) * UGK_GJ_BLOCK * UGK_GJ_BLOCK);
+  R = (double *)ugk_malloc(sizeof(double) * UGK_GJ_BLOCK * (size_t)np);
+  Cb = (double *)ugk_malloc(sizeof(double) * UGK_GJ_BLOCK * (size_t)np);
   if (!strm || !dA || !piv || !R || !Cb || ugk_h2d(dA, h, sizeof(double) * (size_t)np * (size_t)np, strm) ||
       ugk_dense_spd_inverse(dA, np, piv, R, Cb, strm) ||
       ugk_d2h(h, dA, sizeof(double) * (size_t)np * (size_t)np, strm)) {

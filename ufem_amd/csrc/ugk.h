@@ -313,7 +313,7 @@ typedef struct {
   double *d_G;              /* [cx cy cz][2][24 x 24]                        */
   double *d_KC;             /* [ncell][24 x 24]                              */
   double *d_A;              /* [ncp][ncp]: A, then A^-1                      */
-  double *d_piv, *d_R, *d_C;/* block Gauss-Jordan: [32][32], [32][ncp], [ncp][32] */
+  double *d_piv, *d_R, *d_C;/* block Gauss-Jordan: [B][B], [B][ncp], [ncp][B], B = UGK_GJ_BLOCK */
   double *d_cpart;          /* [24][ncell] restriction partials              */
   double *d_g, *d_y;        /* [ncp] coarse vectors                          */
   double *d_fm;             /* [3 W H D] fine mask                           */
@@ -325,8 +325,9 @@ typedef struct {
 int ugk_cc_galerkin(ugk_coarse *C, void *stream);
 int ugk_cc_assemble(ugk_coarse *C, void *stream);
 /* in-place inverse of an SPD n x n matrix (n a multiple of 64, leading
- * dimension n) by block Gauss-Jordan without pivoting; work buffers piv
- * [32 x 32], R [32 x n], Cb [n x 32] */
+ * dimension n) by block Gauss-Jordan without pivoting, blocks of
+ * UGK_GJ_BLOCK; work buffers piv [B x B], R [B x n], Cb [n x B] */
+#define UGK_GJ_BLOCK 64
 int ugk_dense_spd_inverse(double *d_A, long n, double *d_piv, double *d_R, double *d_Cb, void *stream);
 /* PCG with z = dinv r + z_c: convergence test, restriction, coarse solve,
  * z and (r, z), p = z + beta p, q = K p, x and r update (r and the

@@ -2262,53 +2262,80 @@ __global__ __launch_bounds__(WG) void k_cc_assemble(CCGrid g, const double *__re
   A[row * g.ncp + col] = v;
 }
 
-// ---- block Gauss-Jordan inversion of an SPD matrix, blocks of GJB --------
-#define GJB 32
+// ---- block Gauss-Jordan inversion of an SPD matrix, blocks of B -----------
 // the pivot block's inverse, scalar Gauss-Jordan in LDS
-__global__ __launch_bounds__(GJB * GJB) void k_gj_pivot(const double *__restrict__ A, long ld, long K0,
-                                                        double *__restrict__ piv)
+template <int B>
+__global__ __launch_bounds__(1024) void k_gj_pivot(const double *__restrict__ A, long ld, long K0,
+                                                   double *__restrict__ piv)
 {
-  __shared__ double a[GJB][GJB + 1];
-  const int i = threadIdx.x / GJB, j = threadIdx.x % GJB;
-  a[i][j] = A[(K0 + i) * ld + K0 + j];
+  constexpr int PE = (B * B + 1023) / 1024;
+  __shared__ double a[B][B + 1];
+  for (int q = threadIdx.x; q < B * B; q += 1024) a[q / B][q % B] = A[(K0 + q / B) * ld + K0 + q % B];
   __syncthreads();
-  for (int k = 0; k < GJB; k++) {
+  for (int k = 0; k < B; k++) {
     const double p = 1.0 / a[k][k];
-    double v;
-    if (i == k && j == k) v = p;
-    else if (i == k) v = a[k][j] * p;
-    else if (j == k) v = -a[i][k] * p;
-    else v = a[i][j] - a[i][k] * a[k][j] * p;
+    double v[PE];
+#pragma unroll
+    for (int e = 0; e < PE; e++) {
+      const int q = threadIdx.x + 1024 * e, i = q / B, j = q % B;
+      if (q >= B * B) v[e] = 0.0;
+      else if (i == k && j == k) v[e] = p;
+      else if (i == k) v[e] = a[k][j] * p;
+      else if (j == k) v[e] = -a[i][k] * p;
+      else v[e] = a[i][j] - a[i][k] * a[k][j] * p;
+    }
     __syncthreads();
-    a[i][j] = v;
+#pragma unroll
+    for (int e = 0; e < PE; e++) {
+      const int q = threadIdx.x + 1024 * e;
+      if (q < B * B) a[q / B][q % B] = v[e];
+    }
     __syncthreads();
   }
-  piv[i * GJB + j] = a[i][j];
+  for (int q = threadIdx.x; q < B * B; q += 1024) piv[q] = a[q / B][q % B];
 }
 
-// R = piv * A[K, :] (GJB x n) and Cb = A[:, K] (n x GJB), copies taken
-// before the in-place update
+// for the 64 columns J0.. of this block: R[:, J0..] = piv * A[K, J0..]
+// (B x n), and for the same 64 rows Cb[J0.., :] = A[J0.., K] (n x B),
+// copies taken before the in-place update; (B / 16) x 4 outputs per thread
+template <int B>
 __global__ __launch_bounds__(WG) void k_gj_panel(const double *__restrict__ A, long n, long K0,
                                                  const double *__restrict__ piv, double *__restrict__ R,
                                                  double *__restrict__ Cb)
 {
-  __shared__ double P[GJB][GJB + 1];
-  for (int q = threadIdx.x; q < GJB * GJB; q += WG) P[q / GJB][q % GJB] = piv[q];
+  constexpr int RA = B / 16;
+  __shared__ double P[B][B + 1];
+  __shared__ double Ak[B][64 + 1];
+  const long J0 = (long)blockIdx.x * 64;
+  for (int q = threadIdx.x; q < B * B; q += WG) P[q / B][q % B] = piv[q];
+  for (int q = threadIdx.x; q < B * 64; q += WG) Ak[q / 64][q % 64] = A[(K0 + q / 64) * n + J0 + q % 64];
+  for (int q = threadIdx.x; q < 64 * B; q += WG) {
+    const long j = J0 + q / B;
+    Cb[j * B + q % B] = A[j * n + K0 + q % B];
+  }
   __syncthreads();
-  const long j = (long)blockIdx.x * WG + threadIdx.x;
-  if (j >= n) return;
-  double col[GJB];
+  const int tc = threadIdx.x % 16, tr = threadIdx.x / 16;
+  double acc[RA][4];
 #pragma unroll
-  for (int s = 0; s < GJB; s++) col[s] = A[(K0 + s) * n + j];
+  for (int a = 0; a < RA; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
 #pragma unroll 4
-  for (int t = 0; t < GJB; t++) {
-    double acc = 0.0;
+  for (int s = 0; s < B; s++) {
+    double pa[RA], kb[4];
 #pragma unroll
-    for (int s = 0; s < GJB; s++) acc += P[t][s] * col[s];
-    R[(long)t * n + j] = acc;
+    for (int a = 0; a < RA; a++) pa[a] = P[tr + 16 * a][s];
+#pragma unroll
+    for (int b = 0; b < 4; b++) kb[b] = Ak[s][tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] += pa[a] * kb[b];
   }
 #pragma unroll
-  for (int t = 0; t < GJB; t++) Cb[j * GJB + t] = A[j * n + K0 + t];
+  for (int a = 0; a < RA; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) R[(long)(tr + 16 * a) * n + J0 + tc + 16 * b] = acc[a][b];
 }
 
 // one 64 x 64 tile of the update: A_ij -= C_iK R_Kj off the pivot rows and
@@ -2316,18 +2343,19 @@ __global__ __launch_bounds__(WG) void k_gj_panel(const double *__restrict__ A, l
 // staged with piv in place of R at the pivot columns, so that one outer
 // product serves both off-pivot cases and the pivot rows copy it; each
 // thread owns 4 x 4 outputs (rows tr + 16 a, columns tc + 16 b)
+template <int B>
 __global__ __launch_bounds__(WG) void k_gj_update(double *__restrict__ A, long n, long K0,
                                                   const double *__restrict__ piv, const double *__restrict__ R,
                                                   const double *__restrict__ Cb)
 {
-  __shared__ double Cs[64][GJB + 1];
-  __shared__ double Rs[GJB][64 + 1];
+  __shared__ double Cs[64][B + 1];
+  __shared__ double Rs[B][64 + 1];
   const long I0 = (long)blockIdx.y * 64, J0 = (long)blockIdx.x * 64;
-  for (int q = threadIdx.x; q < 64 * GJB; q += WG) Cs[q / GJB][q % GJB] = Cb[(I0 + q / GJB) * GJB + q % GJB];
-  for (int q = threadIdx.x; q < GJB * 64; q += WG) {
+  for (int q = threadIdx.x; q < 64 * B; q += WG) Cs[q / B][q % B] = Cb[(I0 + q / B) * B + q % B];
+  for (int q = threadIdx.x; q < B * 64; q += WG) {
     const int t = q / 64, c = q % 64;
     const long j = J0 + c;
-    Rs[t][c] = (j >= K0 && j < K0 + GJB) ? piv[t * GJB + (j - K0)] : R[(long)t * n + j];
+    Rs[t][c] = (j >= K0 && j < K0 + B) ? piv[t * B + (j - K0)] : R[(long)t * n + j];
   }
   __syncthreads();
   const int tc = threadIdx.x % 16, tr = threadIdx.x / 16;
@@ -2337,7 +2365,7 @@ __global__ __launch_bounds__(WG) void k_gj_update(double *__restrict__ A, long n
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
 #pragma unroll 4
-  for (int t = 0; t < GJB; t++) {
+  for (int t = 0; t < B; t++) {
     double ca[4], rb[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) ca[a] = Cs[tr + 16 * a][t];
@@ -2352,12 +2380,12 @@ __global__ __launch_bounds__(WG) void k_gj_update(double *__restrict__ A, long n
   for (int a = 0; a < 4; a++) {
     const int li = tr + 16 * a;
     const long i = I0 + li;
-    const bool inI = i >= K0 && i < K0 + GJB;
+    const bool inI = i >= K0 && i < K0 + B;
 #pragma unroll
     for (int b = 0; b < 4; b++) {
       const int lj = tc + 16 * b;
       const long j = J0 + lj;
-      const bool inJ = j >= K0 && j < K0 + GJB;
+      const bool inJ = j >= K0 && j < K0 + B;
       double v;
       if (inI) v = Rs[i - K0][lj];
       else if (inJ) v = -acc[a][b];
@@ -2496,6 +2524,7 @@ static inline unsigned spmv_blocks1(const ugk_mat *K)
 }
 static int g_ncu = 256;
 static int g_vec_cap = 512;   // blocks of the vector kernels (UFEM_VEC_BLOCKS)
+static int g_gj_block = 32;   // block size of the dense Gauss-Jordan inverse (UFEM_GJ_BLOCK: 32 or 64)
 // Vector kernels also sum the previous kernel's block partials redundantly in
 // every block, so their grid is kept small: 2 blocks per CU stream at ~95% of
 // the bandwidth of larger grids while reading 4x fewer partials.
@@ -2523,6 +2552,7 @@ int ugk_init(void)
     int v = atoi(e);
     if (v >= 8 && v <= 16384) g_vec_cap = v;
   }
+  if (const char *e = getenv("UFEM_GJ_BLOCK")) g_gj_block = atoi(e) == 64 ? 64 : 32;
   return 0;
 }
 
@@ -3193,10 +3223,18 @@ int ugk_dense_spd_inverse(double *A, long n, double *piv, double *R, double *Cb,
 {
   long K0;
   if (n <= 0 || n % 64) { snprintf(g_err, sizeof(g_err), "dense inverse: n must be a positive multiple of 64"); return -1; }
-  for (K0 = 0; K0 < n; K0 += GJB) {
-    k_gj_pivot<<<1, GJB * GJB, 0, S(s)>>>(A, n, K0, piv);
-    k_gj_panel<<<(unsigned)((n + WG - 1) / WG), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
-    k_gj_update<<<dim3((unsigned)(n / 64), (unsigned)(n / 64)), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
+  if (g_gj_block == 64) {
+    for (K0 = 0; K0 < n; K0 += 64) {
+      k_gj_pivot<64><<<1, 1024, 0, S(s)>>>(A, n, K0, piv);
+      k_gj_panel<64><<<(unsigned)(n / 64), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
+      k_gj_update<64><<<dim3((unsigned)(n / 64), (unsigned)(n / 64)), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
+    }
+  } else {
+    for (K0 = 0; K0 < n; K0 += 32) {
+      k_gj_pivot<32><<<1, 1024, 0, S(s)>>>(A, n, K0, piv);
+      k_gj_panel<32><<<(unsigned)(n / 64), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
+      k_gj_update<32><<<dim3((unsigned)(n / 64), (unsigned)(n / 64)), WG, 0, S(s)>>>(A, n, K0, piv, R, Cb);
+    }
   }
   CK_LAUNCH("k_gj_update");
   return 0;
