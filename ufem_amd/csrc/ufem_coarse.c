@@ -160,6 +160,11 @@ int ufem_sys_set_coarse(ufem_sys *s, int W, int H, int D, int cx, int cy, int cz
   nn = (long)C->Wc * C->Hc * C->Dc;
   C->nc = 3 * nn;
   C->ncp = (C->nc + 63) / 64 * 64;
+  if (C->ncp > 16384) {   /* the dense inverse (n^3) would outweigh the solve */
+    ufem_set_error("coarse: %ld coarse DOFs (at most 16384); use larger factors", C->nc);
+    free(C);
+    return AF_ERR_SIZ;
+  }
   C->d_E = s->d_E;
   C->d_nu = s->d_nu;
   rv = AF_ERR_MEM;
