@@ -116,6 +116,39 @@ def test_twolevel_solve_matches_plain_and_cuts_iterations():
     g.close()
 
 
+def test_coarse_in_flexible_cg():
+    """the correction inside flexible CG: without a network (FP64 and with the
+    FP32 matrix), and added to a network's term; all reach the plain solve's u"""
+    nx, ny, nz = 32, 8, 8
+    mesh = cantilever(nx, ny, nz, E=lognormal_E(13, nx * ny * nz, cov=0.3))
+    g = G.GpuSystem(mesh)
+    g.assemble(2)
+    ref = g.solve("pcg", eps=1e-12, rule=1)
+    u0 = g.u()
+    g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 4, 4)
+    tl = g.solve("twolevel", eps=1e-12, rule=1)
+    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=1)
+    net.init(3)
+    net.bind(g, mesh.E)
+    # the FP32-matrix form is held to 1e-10 (its refreshes stall short of 1e-12)
+    for kw, eps, tol in ((dict(solver="learned", coarse=True), 1e-12, 1e-8),
+                         (dict(solver="learned_mixed", coarse=True), 1e-10, 1e-7),
+                         (dict(solver="learned", coarse=True, net=net, fcg_m=1), 1e-12, 1e-8)):
+        r = g.solve(eps=eps, rule=1, **kw)
+        print(kw.get("solver"), "net" if "net" in kw else "", r["iters"], tl["iters"], ref["iters"])
+        assert r["rv"] == 0, (kw, G.last_error())
+        assert np.linalg.norm(g.u() - u0) <= tol * np.linalg.norm(u0)
+        assert r["iters"] < ref["iters"] / 2
+    # the coarse flag needs a coarse grid, and the network-free form needs it
+    h = G.GpuSystem(mesh)
+    h.assemble(2)
+    assert h.solve("learned", eps=1e-8, rule=1, coarse=True, check=False)["rv"] == G.AF_ERR_VAL
+    assert h.solve("learned", eps=1e-8, rule=1, check=False)["rv"] == G.AF_ERR_VAL
+    net.close()
+    h.close()
+    g.close()
+
+
 def test_twolevel_errors():
     mesh = cantilever(8, 4, 4, distort=0.2, seed=3)
     g = G.GpuSystem(mesh)
