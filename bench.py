@@ -171,7 +171,7 @@ def bench_learned(fields, steps, seed):
     return out
 
 
-def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 8, 8)):
+def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 10, 10)):
     """Learned + mixed precision on the metric's own system `g` (the 1M-DOF
     lognormal cantilever, E = mesh.E): pairs from plain-PCG solves on `fields`
     fresh i.i.d. lognormal fields of the same distribution (seeds seed_base..),
@@ -423,7 +423,7 @@ def main():
         return
     if args.twolevel_only:
         # one warm and one profiled two-level solve of the metric's system
-        g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 8, 8)
+        g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 10, 10)
         for _ in range(2):
             r = g.solve(solver="twolevel", eps=1e-8, rule=1)
             log(f"two-level PCG: {r['iters']} iterations, {r['solve_ms']:.2f} ms device")
@@ -509,9 +509,9 @@ def main():
             "speedup_vs_jacobi": round(solve_s / s_s, 3)}
     log("ssor", ssor)
 
-    # PCG with the two-level correction (coarse grid every 8 bricks; the
+    # PCG with the two-level correction (coarse grid every 8 x 10 x 10 bricks; the
     # coarse matrix is assembled and inverted inside the timed solve)
-    nc = g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 8, 8)
+    nc = g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 10, 10)
     g.solve(solver="twolevel", eps=1e-8, rule=1)
     t0 = time.perf_counter()
     ti = g.solve(solver="twolevel", eps=1e-8, rule=1)

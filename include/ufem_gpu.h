@@ -156,7 +156,7 @@ void ufem_sys_timings(const ufem_sys *s, double t[3]);
  * coincident fine DOF is fixed are left out).  A is assembled on the device
  * from the current moduli and inverted (dense) at the start of each
  * UFEM_SOLVER_TWOLEVEL solve, and by the learned solvers when opts.coarse is
- * set.  The factors must divide W-1, H-1, D-1; at most 1024 bricks per cell. */
+ * set.  The factors must divide W-1, H-1, D-1; at most 4096 bricks per cell. */
 int  ufem_sys_set_coarse(ufem_sys *s, int W, int H, int D, int cx, int cy, int cz);
 long ufem_sys_coarse_size(const ufem_sys *s);     /* coarse DOFs, 0 without a coarse grid */
 /* the coarse matrix (inverse = 0) or its inverse (1) for the current moduli,

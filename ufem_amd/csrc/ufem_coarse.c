@@ -147,8 +147,10 @@ int ufem_sys_set_coarse(ufem_sys *s, int W, int H, int D, int cx, int cy, int cz
     ufem_set_error("coarse: %d x %d x %d nodes do not match the mesh", W, H, D);
     return AF_ERR_SIZ;
   }
-  if (cx < 1 || cy < 1 || cz < 1 || (W - 1) % cx || (H - 1) % cy || (D - 1) % cz || cx * cy * cz > 1024) {
-    ufem_set_error("coarse: factors %d %d %d must divide the element counts (at most 1024 per cell)", cx, cy, cz);
+  if (cx < 1 || cy < 1 || cz < 1 || (W - 1) % cx || (H - 1) % cy || (D - 1) % cz ||
+      cx * cy * cz > UGK_CC_MAX_CELL) {
+    ufem_set_error("coarse: factors %d %d %d must divide the element counts (at most %d per cell)", cx, cy, cz,
+                   UGK_CC_MAX_CELL);
     return AF_ERR_VAL;
   }
   if ((rv = check_box(s, W, H, D, h)) != AF_OK) return rv;

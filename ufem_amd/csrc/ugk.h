@@ -305,6 +305,7 @@ int ugk_ssor_iter(const ugk_mat *Kc, int ncol, const long *cb, long n, double *d
  * G_p = T_p^T K^(l|m) T_p of the cell's element p (identical box elements),
  * then inverted in place (dense, block Gauss-Jordan).  The correction is
  * z_c = fm P A^-1 P^T fm r. */
+#define UGK_CC_MAX_CELL 4096   /* bricks per coarse cell */
 typedef struct {
   int W, H, D, cx, cy, cz, Wc, Hc, Dc;
   long ncell;               /* (Wc - 1) (Hc - 1) (Dc - 1)                    */

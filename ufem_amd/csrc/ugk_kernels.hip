@@ -2197,7 +2197,7 @@ __global__ __launch_bounds__(WG) void k_cc_galerkin(CCGrid g, const double *__re
                                                     const double *__restrict__ E, const double *__restrict__ nu,
                                                     double *__restrict__ KC)
 {
-  __shared__ double lam[1024], mu[1024];
+  __shared__ double lam[UGK_CC_MAX_CELL], mu[UGK_CC_MAX_CELL];
   const long cell = blockIdx.x;
   const int ci = (int)(cell % (g.Wc - 1)), cj = (int)((cell / (g.Wc - 1)) % (g.Hc - 1));
   const int ck = (int)(cell / ((long)(g.Wc - 1) * (g.Hc - 1)));
@@ -3203,7 +3203,10 @@ static CCGrid ccgrid(const ugk_coarse *C)
 
 int ugk_cc_galerkin(ugk_coarse *C, void *s)
 {
-  if (C->cx * C->cy * C->cz > 1024) { snprintf(g_err, sizeof(g_err), "coarse cell of more than 1024 elements"); return -1; }
+  if (C->cx * C->cy * C->cz > UGK_CC_MAX_CELL) {
+    snprintf(g_err, sizeof(g_err), "coarse cell of more than %d elements", UGK_CC_MAX_CELL);
+    return -1;
+  }
   k_cc_galerkin<<<(unsigned)C->ncell, WG, 0, S(s)>>>(ccgrid(C), C->d_G, C->d_E, C->d_nu, C->d_KC);
   CK_LAUNCH("k_cc_galerkin");
   return 0;
