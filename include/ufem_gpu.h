@@ -52,8 +52,11 @@ enum {
                                    FP64 residual refreshes (inner_eps, inner_maxit) */
   UFEM_SOLVER_SSOR = 5,         /* CG with the SSOR (symmetric Gauss-Seidel) preconditioner,
                                    rows in greedy colour order */
-  UFEM_SOLVER_TWOLEVEL = 6      /* PCG with z = dinv r + P A^-1 P^T r, A = P^T K P on a coarse
+  UFEM_SOLVER_TWOLEVEL = 6,     /* PCG with z = dinv r + P A^-1 P^T r, A = P^T K P on a coarse
                                    node grid (ufem_sys_set_coarse first) */
+  UFEM_SOLVER_TWOLEVEL_MIXED = 7 /* the same with q = K p from the FP32 copy of K and FP64
+                                   residual refreshes (inner_eps, inner_maxit), convergence
+                                   accepted on a refreshed residual only */
 };
 enum { UFEM_RULE_REFERENCE = 0, UFEM_RULE_RELATIVE = 1 };
 /* streamed: single-reduction PCG, two kernels per iteration (w = K u with

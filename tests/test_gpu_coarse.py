@@ -113,6 +113,14 @@ def test_twolevel_solve_matches_plain_and_cuts_iterations():
     # the same result again (the coarse matrix rebuilt, same sums)
     again = g.solve("twolevel", eps=1e-12, rule=1)
     assert again["iters"] == info["iters"] and np.array_equal(g.u(), u)
+    # q = K p from the FP32 copy with FP64 refreshes: the true residual meets
+    # the tolerance (accepted on a refreshed residual only)
+    mx = g.solve("twolevel_mixed", eps=1e-10, rule=1)
+    print("two-level mixed", mx["iters"], "refreshes", mx["outer"], mx["true_relres"])
+    assert mx["rv"] == 0, G.last_error()
+    assert mx["true_relres"] <= 1e-10 and mx["outer"] >= 1
+    assert np.linalg.norm(g.u() - u0) <= 1e-7 * np.linalg.norm(u0)
+    assert mx["iters"] <= 1.5 * info["iters"]
     g.close()
 
 

@@ -492,8 +492,10 @@ static int solve_ssor(const ugk_mat *K, long n, const double *b, double *x, ufem
 typedef struct {
   const ugk_mat *K;
   long n;
+  const double *b;
   double *x;
   ufem_work *w;
+  int mixed;
 } cc_ctx;
 
 static int cc_step(void *c, long it, void *stream)
@@ -501,19 +503,37 @@ static int cc_step(void *c, long it, void *stream)
   cc_ctx *q = (cc_ctx *)c;
   ufem_work *w = q->w;
   (void)it;
+  if (q->mixed)
+    return ugk_cc_iter_mixed(q->K, w->cc, q->n, q->b, q->x, w->r, w->z, w->p[0], w->q, w->dinv, w->st, w->part,
+                             stream);
   return ugk_cc_iter(q->K, w->cc, q->n, q->x, w->r, w->z, w->p[0], w->q, w->dinv, w->st, w->part, stream);
 }
 
-static int solve_twolevel(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
-                          const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+static int solve_twolevel(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
+                          const ufem_solve_opts *o, long maxit, int mixed, ufem_solve_info *info)
 {
-  cc_ctx c = {K, n, x, w};
+  const ugk_mat *K = Km;
+  cc_ctx c = {K, n, b, x, w, mixed};
   ugk_state *h = w->st_host;
   int rv;
   if (!w->cc) { ufem_set_error("two-level solver: no coarse grid (ufem_sys_set_coarse)"); return AF_ERR_VAL; }
+  if (mixed) {
+    if (!Km->d_vals32) {
+      Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
+      if (!Km->d_vals32) return gpu_fail("fp32 matrix");
+    }
+    if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+  }
   if ((rv = ufem_coarse_build(w->cc, stream)) != AF_OK) return rv;
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
+  if (mixed) {   /* the flexible-CG head's refresh rule */
+    h->delta = o->inner_eps > 0 ? o->inner_eps : 0.1;
+    h->max_gap = o->inner_maxit > 0 ? o->inner_maxit : 500;
+    h->ru_req = -1;
+    h->ru_at = 0;
+    if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
+  }
   if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
   rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, cc_step, &c, 1);
   if (rv != AF_OK) return rv;
@@ -524,7 +544,7 @@ static int solve_twolevel(const ugk_mat *K, long n, const double *b, double *x, 
     return AF_OK;
   }
   info->iters = h->iter;
-  info->outer = w->cc->nc;
+  info->outer = mixed ? h->n_ru : w->cc->nc;
   info->relres = sqrt(h->rr / h->bb);
   info->status = status_of(h->done);
   return info->status;
@@ -546,7 +566,10 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem
       rv = solve_learned((ugk_mat *)K, n, b, x, w, stream, o, maxit, 1, info);
       break;
     case UFEM_SOLVER_SSOR: rv = solve_ssor(K, n, b, x, w, stream, o, maxit, info); break;
-    case UFEM_SOLVER_TWOLEVEL: rv = solve_twolevel(K, n, b, x, w, stream, o, maxit, info); break;
+    case UFEM_SOLVER_TWOLEVEL: rv = solve_twolevel((ugk_mat *)K, n, b, x, w, stream, o, maxit, 0, info); break;
+    case UFEM_SOLVER_TWOLEVEL_MIXED:
+      rv = solve_twolevel((ugk_mat *)K, n, b, x, w, stream, o, maxit, 1, info);
+      break;
     default: ufem_set_error("unknown solver %d", o->solver); return AF_ERR_VAL;
   }
   if (ugk_event_record(ev1, stream)) return gpu_fail("event");

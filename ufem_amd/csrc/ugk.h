@@ -335,6 +335,13 @@ int ugk_dense_spd_inverse(double *d_A, long n, double *d_piv, double *d_R, doubl
  * partials from ugk_fcg_start) */
 int ugk_cc_iter(const ugk_mat *K, const ugk_coarse *C, long n, double *d_x, double *d_r, double *d_z,
                 double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
+/* the same with q = K p from the FP32 copy of K and the FP64 residual
+ * b - K x recomputed when the flexible-CG head asks for it (the state's
+ * mixed fields set as for flexible CG); convergence is accepted on a
+ * recomputed residual only */
+int ugk_cc_iter_mixed(const ugk_mat *K, const ugk_coarse *C, long n, const double *d_b, double *d_x, double *d_r,
+                      double *d_z, double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st,
+                      double *d_part, void *stream);
 /* z_c alone into d_zc (skipped once the solve is done); d_out NULL in
  * ugk_fcg_combine drops the network term, d_zc NULL the coarse one, and
  * ugk_fcg_head with d_x0 NULL writes no network input */

@@ -771,9 +771,10 @@ __global__ __launch_bounds__(WG) void k_pcg_p(long n, const double *__restrict__
     p[i] = z[i] + beta * p[i];
 }
 
-// variant 1, kernel 2: q = A p, partial (p, q)
-template <int BS>
-__global__ __launch_bounds__(WG) void k_pcg_q(SellView M, const double *__restrict__ vals,
+// variant 1, kernel 2: q = A p, partial (p, q); A's values in FP64 or, for
+// the two-level mixed form, from the FP32 copy (vectors and sums in FP64)
+template <int BS, typename T = double>
+__global__ __launch_bounds__(WG) void k_pcg_q(SellView M, const T *__restrict__ vals,
                                               const double *__restrict__ p, double *__restrict__ q,
                                               ugk_state *st, double *part)
 {
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(WG) void k_pcg_q(SellView M, const double *__restri
   double v[1] = {0.0};
   double acc[BS];
   long row;
-  if (sell_row_split<BS, double, double>(M, vals, p, acc, row)) {
+  if (sell_row_split<BS, T, double>(M, vals, p, acc, row)) {
 #pragma unroll
     for (int r = 0; r < BS; r++) {
       const long i = row * BS + r;
@@ -3268,6 +3269,27 @@ int ugk_cc_iter(const ugk_mat *K, const ugk_coarse *C, long n, double *x, double
   k_cc_z<1><<<GV, WG, 0, S(s)>>>(ccgrid(C), C->d_y, C->d_fm, r, dinv, z, &st->done, part);
   k_ssor_p<<<GV, WG, 0, S(s)>>>(n, z, p, st, part, GV);
   k_pcg_q<3><<<GA, WG, 0, S(s)>>>(M, K->d_vals, p, q, st, part);
+  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA);
+  CK_LAUNCH("k_ssor_upd");
+  return 0;
+}
+
+int ugk_cc_iter_mixed(const ugk_mat *K, const ugk_coarse *C, long n, const double *b, double *x, double *r,
+                      double *z, double *p, double *q, const double *dinv, ugk_state *st, double *part, void *s)
+{
+  const unsigned GV = vec_blocks(n), GA = spmv_blocks(K);
+  const SellView M = view(K);
+  if (K->bs != 3 || n != 3L * C->W * C->H * C->D || !K->d_vals32) {
+    snprintf(g_err, sizeof(g_err), "two-level mixed: a 3x3-block matrix with its FP32 copy on the grid's nodes");
+    return -1;
+  }
+  k_fcg_refresh<3><<<GA, WG, 0, S(s)>>>(M, K->d_vals, b, x, r, dinv, st, part);
+  k_fcg_head<<<GV, WG, 0, S(s)>>>(n, r, dinv, NULL, st, part, GA, GV);
+  CK_LAUNCH("k_fcg_head");
+  if (cc_restrict_solve(C, r, &st->done, s)) return -1;
+  k_cc_z<1><<<GV, WG, 0, S(s)>>>(ccgrid(C), C->d_y, C->d_fm, r, dinv, z, &st->done, part);
+  k_ssor_p<<<GV, WG, 0, S(s)>>>(n, z, p, st, part, GV);
+  k_pcg_q<3, float><<<GA, WG, 0, S(s)>>>(M, K->d_vals32, p, q, st, part);
   k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA);
   CK_LAUNCH("k_ssor_upd");
   return 0;

@@ -20,7 +20,8 @@ P_I = C.POINTER(C.c_int)
 P_F = C.POINTER(C.c_float)
 
 AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
-SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4, "ssor": 5, "twolevel": 6}
+SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4, "ssor": 5, "twolevel": 6,
+          "twolevel_mixed": 7}
 EXEC = {"streamed": 0, "persistent": 1, "fused": 2, "classic": 3}
 
 
