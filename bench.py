@@ -25,7 +25,7 @@ baseline (the oracle's pthread restatement of uFEM's solver, bounded sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-mixed]
                     [--no-learned] [--no-batch] [--kernels-only] [--net-kernels-only]
-                    [--twolevel-only]
+                    [--twolevel-only] [--plain-only]
 """
 
 from __future__ import annotations
@@ -373,6 +373,9 @@ def main():
                          "(for hardware-counter runs)")
     ap.add_argument("--twolevel-only", action="store_true",
                     help="assemble the metric's system and run two two-level solves (for profiling)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="stop after the plain FP64 solves, the dominant-kernel timing and the results "
+                         "pass (for kernel traces)")
     ap.add_argument("--net-kernels-only", action="store_true",
                     help="launch only the network's inference stages on the 128x32x32 random-field "
                          "cantilever (initial weights; for kernel traces)")
@@ -466,10 +469,21 @@ def main():
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
+    # the read rate a plain stream of the same matrix bytes reaches (the faster
+    # of ordinary and non-temporal loads)
+    reads = [(g.time_kernel(m, reps=50), m) for m in ("read", "read_nt")]
+    (r_ms, r_bytes), r_mode = min(reads, key=lambda t: t[0][0])
+    read_gbps = r_bytes / (r_ms * 1e-3) / 1e9
+    log("stream read GB/s", {m: round(b / (ms * 1e-3) / 1e9, 1) for (ms, b), m in reads})
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "kernel": dom[1],
-                "bytes_per_launch": a_bytes, "avg_launch_us": round(a_ms * 1e3, 2)}
+                "bytes_per_launch": a_bytes, "avg_launch_us": round(a_ms * 1e3, 2),
+                "stream_read": {"kernel": f"k_stream_read<{'true' if r_mode == 'read_nt' else 'false'}> "
+                                          "(matrix values + columns, no gathers)",
+                                "bytes": r_bytes, "avg_launch_us": round(r_ms * 1e3, 2),
+                                "GBps": round(read_gbps, 1)},
+                "frac_of_stream_read": round(achieved / read_gbps, 4)}
     spmv = {"kernel": "k_spmv<3,double,double>", "avg_launch_us": round(s_ms * 1e3, 2),
             "bytes_per_launch": s_bytes, "GBps": round(s_bytes / (s_ms * 1e-3) / 1e9, 1),
             "csr_equivalent_bytes": csr_bytes,
@@ -484,6 +498,9 @@ def main():
     results = {"ms": round(r_ms, 3), "elements_per_s": round(mesh.n_elems / r_ms * 1e3, 1),
                "write_GBps": round(r_bytes / (r_ms * 1e-3) / 1e9, 1)}
     log("results pass", results)
+    if args.plain_only:
+        g.close()
+        return
 
     mixed = None
     if not args.no_mixed:

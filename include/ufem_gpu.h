@@ -145,7 +145,11 @@ int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host 
 /* average device time (HIP events on the system's stream) of `reps`
  * launches of one kernel, and the bytes one launch moves by the algorithmic
  * count.  mode 0: y = K x (FP64); 1: y = K x (FP32 copy of K); 2: the PCG
- * phase-A kernel (p' = z + beta p, q = K p', (p', q)), which dominates a solve. */
+ * phase-A kernel (p' = z + beta p, q = K p', (p', q)); 3: q = K p with (p, q);
+ * 4: w = K u with (w, u), which dominates the default solve; 5: a device copy
+ * of one vector; 6: a plain streaming read of K's stored values and columns
+ * (the read rate the SpMV's matrix stream can reach; bytes as stored);
+ * 7: the same read with non-temporal loads. */
 int  ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *bytes);
 /* phase device times of the last calls: fill_K, add_disps, solve (ms) */
 void ufem_sys_timings(const ufem_sys *s, double t[3]);

@@ -747,7 +747,7 @@ int ufem_sys_get_K(ufem_sys *s, long *indptr, int *cols, double *vals)
         int J = hc[(so + sl) * 64 + ln];
         for (c = 0; c < 3; c++) {
           if (cols) cols[k] = 3 * J + c;
-          if (vals) vals[k] = hv[((so + sl) * 9 + 3 * r + c) * 64 + ln];
+          if (vals) vals[k] = hv[ugk_vpos3(so + sl, 3 * r + c, (int)ln)];
           k++;
         }
       }
@@ -781,6 +781,7 @@ static double kernel_bytes(const ugk_mat *K, int mode)
   if (mode == 1) return nnzb * (b2 * 4.0 + 4.0) + 8.0 * n;
   if (mode == 2) return nnzb * (b2 * 8.0 + 4.0) + 32.0 * n;
   if (mode == 5) return 16.0 * n;                      /* device copy of one vector */
+  if (mode >= 6) return (double)K->total_slots * 64.0 * (b2 * 8.0 + 4.0);   /* stored values + columns */
   return nnzb * (b2 * 8.0 + 4.0) + 16.0 * n;   /* modes 0, 3, 4: x read, y written */
 }
 
@@ -791,6 +792,7 @@ static int launch_mode(ufem_sys *s, int mode)
   if (mode == 3) return ugk_pcg_iter_q(&s->K, s->w.p[0], s->w.q, s->w.st, s->w.part, s->stream);
   if (mode == 4) return ugk_cg1_iter_w(&s->K, s->w.z, s->w.q, s->w.st, s->w.part, s->stream);
   if (mode == 5) return ugk_d2d(s->w.q, s->w.z, sizeof(double) * (size_t)s->n, s->stream);
+  if (mode == 6 || mode == 7) return ugk_stream_read(&s->K, mode == 7, s->w.part, s->stream);
   return ugk_spmv(&s->K, s->w.z, s->w.q, s->stream);
 }
 

@@ -36,6 +36,15 @@ typedef struct {
   unsigned long gen;  /* bumped whenever the pattern is (re)uploaded     */
 } ugk_mat;
 
+/* position of value q (= 3 r + c, row-major in the block) of lane l in slot S
+ * of a 3x3-block value array: a slot holds 576 values, each lane's first eight
+ * in adjacent pairs (pair k of lane l at 128 k + 2 l: one 16-byte load in
+ * FP64), the ninth in a 64-wide tail at 512 + l.  1x1 blocks: 64 S + l. */
+static inline long ugk_vpos3(long S, int q, int l)
+{
+  return S * 576 + (q < 8 ? (q >> 1) * 128 + 2 * l + (q & 1) : 512 + l);
+}
+
 /* largest grid of a reducing kernel, and the size (in doubles) of the
  * partial-sum buffer every reducing call takes (8 slots per block + counter) */
 #define UGK_RED_MAX_BLOCKS 65536
@@ -136,6 +145,9 @@ int ugk_vals_to_f32(const ugk_mat *K, void *stream);
 
 /* ---------------- sparse kernels ---------------------------------------- */
 int ugk_spmv(const ugk_mat *K, const double *d_x, double *d_y, void *stream);
+/* streaming read of K's values and columns (plain or non-temporal loads);
+ * one partial per wave into d_out */
+int ugk_stream_read(const ugk_mat *K, int nontemporal, double *d_out, void *stream);
 int ugk_spmv_f32(const ugk_mat *K, const float *d_x, float *d_y, void *stream);
 /* dinv[i] = 1 / K_ii; st->err = 2 if |K_ii| < 1e-17 */
 int ugk_diag_inv(const ugk_mat *K, double *d_dinv, ugk_state *d_st, void *stream);

@@ -247,6 +247,48 @@ __device__ __forceinline__ int find_slot(const SellView &M, long I, long J)
   return -1;
 }
 
+// Position of value q (= BS r + c, row-major in the block) of lane l in slot
+// S.  A 3x3 slot holds 576 values: each lane's first eight values in adjacent
+// pairs (pair k of lane l at 128 k + 2 l, one 16-byte load in FP64), the
+// ninth in a 64-wide tail at 512 + l.  A 1x1 slot holds one value per lane.
+// The host-side twin is ugk_vpos3 in ugk.h.
+template <int BS>
+__device__ __forceinline__ long vpos(long S, int q, int l)
+{
+  if (BS == 1) return S * 64 + l;
+  return S * 576 + (q < 8 ? (q >> 1) * 128 + 2 * l + (q & 1) : 512 + l);
+}
+
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p)
+{
+  return __builtin_nontemporal_load(p);
+}
+
+template <typename T> struct Pair;
+template <> struct Pair<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct Pair<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+// the BS*BS values of lane `lane` in slot S, row-major (NT: non-temporal loads)
+template <int BS, bool NT, typename T>
+__device__ __forceinline__ void ld_block(const T *__restrict__ vals, long S, int lane, T (&b)[BS * BS])
+{
+  if constexpr (BS == 1) {
+    b[0] = NT ? ldnt(vals + S * 64 + lane) : vals[S * 64 + lane];
+  } else {
+    typedef typename Pair<T>::type T2;
+    const T *p = vals + S * 576;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const T2 *a = (const T2 *)(p + k * 128 + 2 * lane);
+      const T2 t = NT ? ldnt(a) : *a;
+      b[2 * k] = t.x;
+      b[2 * k + 1] = t.y;
+    }
+    b[8] = NT ? ldnt(p + 512 + lane) : p[512 + lane];
+  }
+}
+
 // logical slice for a block: XCD-aware remap so that each XCD (blocks are
 // dealt round-robin over 8 XCDs) streams one contiguous range of rows and
 // its L2 keeps the x entries those rows gather
@@ -397,9 +439,9 @@ __global__ __launch_bounds__(WG) void k_elem_stiff(long n_list, const int *__res
     const long I = enodes[e * 8 + a], J = enodes[e * 8 + b];
     const int s = find_slot(M, I, J);
     if (s < 0) { st->err = 3; return; }
-    double *v = vals + ((M.soff[I >> 6] + s) * 9) * 64 + (I & 63);
+    const long S = M.soff[I >> 6] + s;
 #pragma unroll
-    for (int q = 0; q < 9; q++) v[q * 64] += k[q];
+    for (int q = 0; q < 9; q++) vals[vpos<3>(S, q, (int)(I & 63))] += k[q];
   }
 }
 
@@ -483,7 +525,7 @@ __global__ void k_save_rows(SellView M, const double *__restrict__ vals, long n_
   for (int s = 0; s < maxb; s++) {
     if (s < L) {
       rcols[f * maxb + s] = M.cols[(so + s) * 64 + ln];
-      for (int c = 0; c < 3; c++) rows[(f * maxb + s) * 3 + c] = vals[((so + s) * 9 + 3 * r + c) * 64 + ln];
+      for (int c = 0; c < 3; c++) rows[(f * maxb + s) * 3 + c] = vals[vpos<3>(so + s, (int)(3 * r + c), (int)ln)];
     } else {
       rcols[f * maxb + s] = -1;
       for (int c = 0; c < 3; c++) rows[(f * maxb + s) * 3 + c] = 0.0;
@@ -500,12 +542,12 @@ __global__ void k_zero_rowcol(SellView M, double *__restrict__ vals, long n_fixe
   const long so = M.soff[I >> 6], ln = I & 63;
   const int L = M.rowlen[I];
   for (int s = 0; s < L; s++) {
-    for (int c = 0; c < 3; c++) vals[((so + s) * 9 + 3 * r + c) * 64 + ln] = 0.0;  // row d
+    for (int c = 0; c < 3; c++) vals[vpos<3>(so + s, (int)(3 * r + c), (int)ln)] = 0.0;  // row d
     const long Jn = M.cols[(so + s) * 64 + ln];                                     // column d
     const int t = find_slot(M, Jn, I);
     if (t < 0) continue;
     const long soJ = M.soff[Jn >> 6], lnJ = Jn & 63;
-    for (int rr = 0; rr < 3; rr++) vals[((soJ + t) * 9 + 3 * rr + r) * 64 + lnJ] = 0.0;
+    for (int rr = 0; rr < 3; rr++) vals[vpos<3>(soJ + t, (int)(3 * rr + r), (int)lnJ)] = 0.0;
   }
 }
 
@@ -517,7 +559,7 @@ __global__ void k_set_diag_one(SellView M, double *__restrict__ vals, long n_fix
   const long d = fixed[f], I = d / 3, r = d % 3;
   const int s = find_slot(M, I, I);
   if (s < 0) return;
-  vals[((M.soff[I >> 6] + s) * 9 + 4 * r) * 64 + (I & 63)] = 1.0;
+  vals[vpos<3>(M.soff[I >> 6] + s, (int)(4 * r), (int)(I & 63))] = 1.0;
 }
 
 __global__ void k_reactions(long n_fixed, const int *__restrict__ fixed, int maxb,
@@ -546,7 +588,6 @@ __device__ __forceinline__ void sell_row(const SellView &M, const T *__restrict_
   const int L = M.slen[sl];
   const long so = M.soff[sl];
   const int *cp = M.cols + so * 64 + lane;
-  const T *vp = vals + so * 64 * BS * BS + lane;
 #pragma unroll
   for (int r = 0; r < BS; r++) acc[r] = 0;
 #pragma unroll 3
@@ -555,11 +596,12 @@ __device__ __forceinline__ void sell_row(const SellView &M, const T *__restrict_
     TX xv[BS];
 #pragma unroll
     for (int c = 0; c < BS; c++) xv[c] = x[J * BS + c];
-    const T *v = vp + (long)s * 64 * BS * BS;
+    T v[BS * BS];
+    ld_block<BS, false>(vals, so + s, lane, v);
 #pragma unroll
     for (int r = 0; r < BS; r++)
 #pragma unroll
-      for (int c = 0; c < BS; c++) acc[r] += (TX)v[(r * BS + c) * 64] * xv[c];
+      for (int c = 0; c < BS; c++) acc[r] += (TX)v[r * BS + c] * xv[c];
   }
 }
 
@@ -569,12 +611,6 @@ __device__ __forceinline__ void sell_row(const SellView &M, const T *__restrict_
 // values and columns are read once, so they are loaded non-temporally to
 // leave L2 to the gathered vector.  Returns true in the lanes that hold a
 // finished row sum (first-half wave, real row); `row` is the block row.
-template <typename T>
-__device__ __forceinline__ T ldnt(const T *p)
-{
-  return __builtin_nontemporal_load(p);
-}
-
 template <int BS, typename T, typename TX>
 __device__ __forceinline__ bool sell_row_split(const SellView &M, const T *__restrict__ vals,
                                                const TX *__restrict__ x, TX (&acc)[BS], long &row)
@@ -591,17 +627,17 @@ __device__ __forceinline__ bool sell_row_split(const SellView &M, const T *__res
     const int s0 = half ? h : 0, s1 = half ? L : h;
     const long so = M.soff[sl];
     const int *cp = M.cols + so * 64 + lane;
-    const T *vp = vals + so * 64 * BS * BS + lane;
     for (int s = s0; s < s1; s++) {
       const long J = ldnt(cp + (long)s * 64);
       TX xv[BS];
 #pragma unroll
       for (int c = 0; c < BS; c++) xv[c] = x[J * BS + c];
-      const T *v = vp + (long)s * 64 * BS * BS;
+      T v[BS * BS];
+      ld_block<BS, true>(vals, so + s, lane, v);
 #pragma unroll
       for (int r = 0; r < BS; r++)
 #pragma unroll
-        for (int c = 0; c < BS; c++) acc[r] += (TX)ldnt(v + (r * BS + c) * 64) * xv[c];
+        for (int c = 0; c < BS; c++) acc[r] += (TX)v[r * BS + c] * xv[c];
     }
   }
   if (half) {
@@ -636,7 +672,7 @@ __global__ void k_diag_inv(SellView M, const double *__restrict__ vals, double *
   const long I = i / BS, r = i % BS;
   const int s = find_slot(M, I, I);
   double v = 0.0;
-  if (s >= 0) v = vals[((M.soff[I >> 6] + s) * BS * BS + r * BS + r) * 64 + (I & 63)];
+  if (s >= 0) v = vals[vpos<BS>(M.soff[I >> 6] + s, (int)(r * BS + r), (int)(I & 63))];
   if (!(fabs(v) >= 1e-17)) { st->err = 2; dinv[i] = 0.0; return; }
   dinv[i] = 1.0 / v;
 }
@@ -652,7 +688,7 @@ __global__ __launch_bounds__(WG) void k_mat_norm(SellView M, const double *__res
     double s2 = 0.0;
     for (int s = 0; s < M.rowlen[I]; s++)
       for (int c = 0; c < BS; c++) {
-        const double a = vals[((so + s) * BS * BS + r * BS + c) * 64 + ln];
+        const double a = vals[vpos<BS>(so + s, (int)(r * BS + c), (int)ln)];
         s2 += a * a;
       }
     v[0] = fmax(v[0], sqrt(s2));
@@ -817,7 +853,6 @@ __global__ __launch_bounds__(WG) void k_pcg_a(SellView M, const double *__restri
     const int L = M.slen[sl];
     const long so = M.soff[sl];
     const int *cp = M.cols + so * 64 + lane;
-    const double *vp = vals + so * 64 * BS * BS + lane;
     double acc[BS];
 #pragma unroll
     for (int r = 0; r < BS; r++) acc[r] = 0.0;
@@ -827,11 +862,12 @@ __global__ __launch_bounds__(WG) void k_pcg_a(SellView M, const double *__restri
       double xv[BS];
 #pragma unroll
       for (int c = 0; c < BS; c++) xv[c] = z[J * BS + c] + beta * pold[J * BS + c];
-      const double *vv = vp + (long)s * 64 * BS * BS;
+      double vv[BS * BS];
+      ld_block<BS, false>(vals, so + s, lane, vv);
 #pragma unroll
       for (int r = 0; r < BS; r++)
 #pragma unroll
-        for (int c = 0; c < BS; c++) acc[r] += vv[(r * BS + c) * 64] * xv[c];
+        for (int c = 0; c < BS; c++) acc[r] += vv[r * BS + c] * xv[c];
     }
     const long row = sl * 64 + lane;
     if (row < M.n_brows) {
@@ -1052,7 +1088,6 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
       const int L = M.slen[sl];
       const long so = M.soff[sl];
       const int *cp = M.cols + so * 64 + lane;
-      const double *vp = vals + so * 64 * BS * BS + lane;
       double acc[BS];
 #pragma unroll
       for (int rr2 = 0; rr2 < BS; rr2++) acc[rr2] = 0.0;
@@ -1062,11 +1097,12 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
         double xv[BS];
 #pragma unroll
         for (int c = 0; c < BS; c++) xv[c] = z[J * BS + c] + beta * pold[J * BS + c];
-        const double *vv = vp + (long)s * 64 * BS * BS;
+        double vv[BS * BS];
+        ld_block<BS, false>(vals, so + s, lane, vv);
 #pragma unroll
         for (int r2 = 0; r2 < BS; r2++)
 #pragma unroll
-          for (int c = 0; c < BS; c++) acc[r2] += vv[(r2 * BS + c) * 64] * xv[c];
+          for (int c = 0; c < BS; c++) acc[r2] += vv[r2 * BS + c] * xv[c];
       }
       const long row = sl * 64 + lane;
       if (row < M.n_brows) {
@@ -1602,15 +1638,15 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
         const int L = S.slen[sl];
         const long so = S.soff[sl];
         const int *cp = S.cols + so * 64 + lane;
-        const double *vp = S.vals + so * 64 * 9 + lane;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
         for (int s = 0; s < L; s++) {
           const long J = ldnt(cp + (long)s * 64);
           const double x0 = S.u[3 * J], x1 = S.u[3 * J + 1], x2 = S.u[3 * J + 2];
-          const double *vv = vp + (long)s * 576;
-          a0 += ldnt(vv) * x0 + ldnt(vv + 64) * x1 + ldnt(vv + 128) * x2;
-          a1 += ldnt(vv + 192) * x0 + ldnt(vv + 256) * x1 + ldnt(vv + 320) * x2;
-          a2 += ldnt(vv + 384) * x0 + ldnt(vv + 448) * x1 + ldnt(vv + 512) * x2;
+          double vv[9];
+          ld_block<3, true>(S.vals, so + s, lane, vv);
+          a0 += vv[0] * x0 + vv[1] * x1 + vv[2] * x2;
+          a1 += vv[3] * x0 + vv[4] * x1 + vv[5] * x2;
+          a2 += vv[6] * x0 + vv[7] * x1 + vv[8] * x2;
         }
         const long row = sl * 64 + lane;
         if (row < S.n_brows) {
@@ -1989,7 +2025,6 @@ __global__ __launch_bounds__(WG) void k_sgs(SellView M, const double *__restrict
   const int L = M.rowlen[I];
   const long so = M.soff[sl];
   const int *cp = M.cols + so * 64 + lane;
-  const double *vp = vals + so * 64 * BS * BS + lane;
   double acc[BS], D[BS][BS];
 #pragma unroll
   for (int a = 0; a < BS; a++) {
@@ -1999,10 +2034,9 @@ __global__ __launch_bounds__(WG) void k_sgs(SellView M, const double *__restrict
   }
   for (int k = 0; k < L; k++) {
     const long J = cp[(long)k * 64];
-    const double *v = vp + (long)k * 64 * BS * BS;
     if (J == I) {
 #pragma unroll
-      for (int q = 0; q < BS * BS; q++) D[q / BS][q % BS] = v[q * 64];
+      for (int q = 0; q < BS * BS; q++) D[q / BS][q % BS] = vals[vpos<BS>(so + k, q, lane)];
     } else if (FWD ? J < r0 : J >= r1) {
       double yv[BS];
 #pragma unroll
@@ -2010,7 +2044,7 @@ __global__ __launch_bounds__(WG) void k_sgs(SellView M, const double *__restrict
 #pragma unroll
       for (int a = 0; a < BS; a++)
 #pragma unroll
-        for (int c = 0; c < BS; c++) acc[a] += v[(a * BS + c) * 64] * yv[c];
+        for (int c = 0; c < BS; c++) acc[a] += vals[vpos<BS>(so + k, a * BS + c, lane)] * yv[c];
     }
   }
   double yl[BS];
@@ -2132,9 +2166,9 @@ __global__ __launch_bounds__(WG) void k_ssor_permute_vals(SellView M, const doub
     const long at = (Mc.soff[sl] + k) * 64 + ln;
     const int Jo = perm[Mc.cols[at]];
     const int ko = find_slot(M, Io, Jo);
-    const double *v = vals + (M.soff[slo] + ko) * 64 * BS * BS + lno;
 #pragma unroll
-    for (int q = 0; q < BS * BS; q++) valc[(Mc.soff[sl] + k) * 64 * BS * BS + q * 64 + ln] = ko >= 0 ? v[q * 64] : 0.0;
+    for (int q = 0; q < BS * BS; q++)
+      valc[vpos<BS>(Mc.soff[sl] + k, q, (int)ln)] = ko >= 0 ? vals[vpos<BS>(M.soff[slo] + ko, q, (int)lno)] : 0.0;
   }
 }
 
@@ -2537,6 +2571,30 @@ static inline unsigned vec_blocks(long n)
 }
 static const unsigned RED_MAX_BLOCKS = UGK_RED_MAX_BLOCKS;
 
+// Plain streaming read of the matrix's values and columns: 16-byte loads,
+// grid-stride, one load in flight per lane per step (NT: non-temporal, as the
+// SpMV issues them), each wave folding what it read into one word of `out`.
+// It moves the SpMV's matrix bytes with none of its gathers or row sums: the
+// read rate that stream can reach on this device.
+template <bool NT>
+__global__ void __launch_bounds__(WG) k_stream_read(const double *__restrict__ a, long na,
+                                                     const double *__restrict__ b, long nb, double *out)
+{
+  typedef Pair<double>::type D2;
+  const long stride = (long)gridDim.x * WG;
+  double s = 0;
+  for (long i = blockIdx.x * (long)WG + threadIdx.x; i < na; i += stride) {
+    const D2 v = NT ? ldnt((const D2 *)a + i) : ((const D2 *)a)[i];
+    s += v.x + v.y;
+  }
+  for (long i = blockIdx.x * (long)WG + threadIdx.x; i < nb; i += stride) {
+    const D2 v = NT ? ldnt((const D2 *)b + i) : ((const D2 *)b)[i];
+    s += v.x + v.y;
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) out[blockIdx.x * NW + (threadIdx.x >> 6)] = s;
+}
+
 extern "C" {
 
 int ugk_init(void)
@@ -2790,6 +2848,19 @@ int ugk_vals_to_f32(const ugk_mat *K, void *s)
 }
 
 // ---- sparse ---------------------------------------------------------------
+int ugk_stream_read(const ugk_mat *K, int nontemporal, double *d_out, void *s)
+{
+  const long slots = K->total_slots * 64;                  // a multiple of 64
+  const long na = slots * K->bs * K->bs / 2, nb = slots / 4;   // 16-byte words (columns: 4 per word)
+  const unsigned blocks = (unsigned)(g_ncu * 8);
+  if ((long)blocks * NW > UGK_RED_MAX_BLOCKS) return -1;
+  if (nontemporal)
+    k_stream_read<true><<<blocks, WG, 0, S(s)>>>(K->d_vals, na, (const double *)K->d_cols, nb, d_out);
+  else
+    k_stream_read<false><<<blocks, WG, 0, S(s)>>>(K->d_vals, na, (const double *)K->d_cols, nb, d_out);
+  CK_LAUNCH("k_stream_read");
+  return 0;
+}
 int ugk_spmv(const ugk_mat *K, const double *d_x, double *d_y, void *s)
 {
   if (K->bs == 3) k_spmv<3, double, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, d_x, d_y);

@@ -335,8 +335,10 @@ class GpuSystem:
         return y
 
     def time_kernel(self, mode="spmv", reps=20):
-        """(avg ms, algorithmic bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a', 'pcg_q', 'cg1_w', 'copy'."""
-        m = {"spmv": 0, "spmv32": 1, "pcg_a": 2, "pcg_q": 3, "cg1_w": 4, "copy": 5}[mode]
+        """(avg ms, bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a', 'pcg_q', 'cg1_w', 'copy', or 'read' /
+        'read_nt' (a plain streaming read of the matrix's stored values and columns, with ordinary or
+        non-temporal loads)"""
+        m = {"spmv": 0, "spmv32": 1, "pcg_a": 2, "pcg_q": 3, "cg1_w": 4, "copy": 5, "read": 6, "read_nt": 7}[mode]
         ms = C.c_double(0)
         by = C.c_double(0)
         self._chk(lib().ufem_sys_time_spmv(self.h, reps, m, C.byref(ms), C.byref(by)), "time_kernel")
