@@ -801,6 +801,10 @@ int ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *byte
   int i;
   float t;
   if (reps < 1) reps = 1;
+  if (mode < 0 || mode > 7) {
+    ufem_set_error("ufem_sys_time_spmv: unknown kernel mode %d (0..7)", mode);
+    return AF_ERR_VAL;
+  }
   if (mode == 1 && !s->K.d_vals32) {
     s->K.d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)s->K.total_slots * 64 * 9);
     if (!s->K.d_vals32 || ugk_vals_to_f32(&s->K, s->stream)) return gpu_fail("fp32 copy");
