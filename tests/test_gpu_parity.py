@@ -5,6 +5,7 @@ Bars: element matrices and assembled K within 1e-12 of max|K| (bit-exact
 sparsity pattern), displacements within 1e-8 relative, reactions within 1e-6
 relative (FP64 plain path)."""
 
+import ctypes
 import os
 
 import numpy as np
@@ -428,3 +429,33 @@ def test_error_paths():
     K = G.RowSlotMatrix(3, np.array([1, 1, 1]), np.array([1, 2, 2]), np.array([1.0, 0.0, 1.0]))
     rv = G.femEqsCGwJ(K, G.Vec(np.ones(3)), G.Vec(np.zeros(3)), 1e-10, 10)
     assert rv == G.AF_ERR_ZER
+
+
+def test_timing_modes_leave_results_intact():
+    """the kernel timers (HIP events on the system's stream): every mode runs
+    and reports its byte count, the stream read moves the stored matrix, an
+    unknown mode is refused, and K and the solve are unchanged afterwards"""
+    mesh = MESHES["c1"]()
+    g = G.GpuSystem(mesh)
+    g.assemble(2)
+    ref = g.solve("pcg", eps=1e-12, rule=1)
+    u0 = g.u()
+    ip, cols, vals = g.K_csr()
+    nnzb = len(vals) // 9
+    alg = nnzb * 76.0 + 16.0 * g.n
+    for mode in ("spmv", "spmv32", "pcg_q", "cg1_w", "copy", "read", "read_nt"):
+        ms, by = g.time_kernel(mode, reps=3)
+        assert ms > 0.0 and by > 0.0, mode
+        if mode in ("spmv", "pcg_q", "cg1_w"):
+            assert by == alg
+        if mode.startswith("read"):
+            # stored values and columns: the real blocks plus the slice padding
+            assert nnzb * 76.0 <= by <= 1.5 * nnzb * 76.0
+    ms, by = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    assert G.lib().ufem_sys_time_spmv(g.h, 1, 8, ctypes.byref(ms), ctypes.byref(by)) == G.AF_ERR_VAL
+    assert "unknown kernel mode" in G.last_error()
+    ip2, cols2, vals2 = g.K_csr()
+    assert np.array_equal(ip, ip2) and np.array_equal(cols, cols2) and np.array_equal(vals, vals2)
+    again = g.solve("pcg", eps=1e-12, rule=1)
+    assert again["iters"] == ref["iters"] and np.array_equal(g.u(), u0)
+    g.close()
