@@ -2495,10 +2495,23 @@ __global__ __launch_bounds__(WG) void k_cc_gemv(const double *__restrict__ A, lo
   const int lane = threadIdx.x & 63;
   const long row = (long)blockIdx.x * NW + (threadIdx.x >> 6);
   if (row >= n) return;
-  const double *a = A + row * n;
-  double s = 0.0;
-  for (long j = lane; j < n; j += 64) s += a[j] * gv[j];
-  s = wave_sum(s);
+  // n is even (padded to a multiple of 64), so rows and g split into
+  // 16-byte pairs; two pairs per lane in flight, two accumulators
+  typedef Pair<double>::type D2;
+  const D2 *a = (const D2 *)(A + row * n), *g2 = (const D2 *)gv;
+  const long np = n >> 1;
+  double s0 = 0.0, s1 = 0.0;
+  long p = lane;
+  for (; p + 64 < np; p += 128) {
+    const D2 x0 = a[p], x1 = a[p + 64], h0 = g2[p], h1 = g2[p + 64];
+    s0 += x0.x * h0.x + x0.y * h0.y;
+    s1 += x1.x * h1.x + x1.y * h1.y;
+  }
+  if (p < np) {
+    const D2 x0 = a[p], h0 = g2[p];
+    s0 += x0.x * h0.x + x0.y * h0.y;
+  }
+  const double s = wave_sum(s0 + s1);
   if (lane == 0) y[row] = s;
 }
 
