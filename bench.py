@@ -382,7 +382,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by slabs, see DESIGN.md, but this build runs on one GPU)", file=sys.stderr)
+              "(the path would shard by slabs of the node grid, but it runs on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     from ufem_amd import gpu as G
