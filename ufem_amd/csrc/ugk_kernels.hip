@@ -299,6 +299,18 @@ __device__ __forceinline__ long remap_block(long b, long nb)
   return (b & 7) * per + (b >> 3);
 }
 
+// This block's contiguous range [i1, i2) of [0, n) for vector kernels: block
+// chunks in remapped order, so the chunks of one XCD form one range, as the
+// rows of the SpMV kernels do, and the entries an XCD's SpMV gathers or
+// wrote stay in its L2 between the kernels of an iteration
+__device__ __forceinline__ void xcd_chunk(long n, long &i1, long &i2)
+{
+  const long lb = (gridDim.x & 7) ? (long)blockIdx.x : remap_block(blockIdx.x, gridDim.x);
+  const long chunk = (n + gridDim.x - 1) / gridDim.x;
+  i1 = lb * chunk;
+  i2 = min(n, i1 + chunk);
+}
+
 // --------------------------------------------------------------------------
 // element stiffness
 // --------------------------------------------------------------------------
@@ -977,7 +989,9 @@ __global__ __launch_bounds__(WG) void k_cg1_upd(long n, double *__restrict__ x, 
   }
   if (done) return;
   double v[4] = {0, 0, 0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) {
     const double pi = u[i] + beta * p[i];
     const double si = w[i] + beta * s[i];
     const double xi = x[i] + alpha * pi;
@@ -2116,7 +2130,9 @@ __global__ __launch_bounds__(WG) void k_ssor_p(long n, const double *__restrict_
     st->rzB = ro;
     st->beta = beta;
   }
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG)
     p[i] = k == 0 ? z[i] : z[i] + beta * p[i];
 }
 
@@ -2135,7 +2151,9 @@ __global__ __launch_bounds__(WG) void k_ssor_upd(long n, double *__restrict__ x,
     return;
   }
   double v[4] = {0, 0, 0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) {
     const double xi = x[i] + alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     x[i] = xi;
@@ -2525,7 +2543,9 @@ __global__ __launch_bounds__(WG) void k_cc_z(CCGrid g, const double *__restrict_
   if (*stop) return;
   const long nn = (long)g.W * g.H * g.D;
   double v[1] = {0.0};
-  for (long node = (long)blockIdx.x * WG + threadIdx.x; node < nn; node += (long)gridDim.x * WG) {
+  long n1, n2;
+  xcd_chunk(nn, n1, n2);
+  for (long node = n1 + threadIdx.x; node < n2; node += WG) {
     const int i = (int)(node % g.W), j = (int)((node / g.W) % g.H), k = (int)(node / ((long)g.W * g.H));
     const int ci = cc_cell(i, g.cx, g.Wc), cj = cc_cell(j, g.cy, g.Hc), ck = cc_cell(k, g.cz, g.Dc);
     const double wx1 = (double)(i - ci * g.cx) / g.cx, wy1 = (double)(j - cj * g.cy) / g.cy;
