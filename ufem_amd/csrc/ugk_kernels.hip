@@ -815,7 +815,9 @@ __global__ __launch_bounds__(WG) void k_pcg_p(long n, const double *__restrict__
   long k;
   double beta;
   if (pcg_head(st, part, GS, GB, k, beta)) return;
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG)
     p[i] = z[i] + beta * p[i];
 }
 
@@ -1030,7 +1032,9 @@ __global__ __launch_bounds__(WG) void k_pcg_b(long n, double *__restrict__ x, do
   }
   const double *p = (pingpong && st->pidx) ? p1 : p0;
   double v[4] = {0, 0, 0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) {
     const double xi = x[i] + alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     const double zi = ri * dinv[i];
@@ -1255,7 +1259,9 @@ __global__ __launch_bounds__(WG) void k_mx_p(long n, const float *__restrict__ z
   }
   if (done) return;
   const float bf = (float)beta;
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) p[i] = z[i] + bf * p[i];
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) p[i] = z[i] + bf * p[i];
 }
 
 template <int BS>
@@ -1295,7 +1301,9 @@ __global__ __launch_bounds__(WG) void k_mx_b(long n, float *__restrict__ x, floa
   }
   const float af = (float)alpha;
   double v[2] = {0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) {
     const float xi = x[i] + af * p[i];
     const float ri = r[i] - af * q[i];
     const float zi = ri * dinv[i];
@@ -1477,7 +1485,9 @@ __global__ __launch_bounds__(WG) void k_bicgs_s(long n, const double *__restrict
   if (st->done) return;
   const double alpha = st->aux[0] / st->aux[1];
   double w[1] = {0.0}, tot[1];
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) {
     const double si = r[i] - alpha * v[i];
     s[i] = si;
     shat[i] = si * dinv[i];
@@ -1503,7 +1513,9 @@ __global__ __launch_bounds__(WG) void k_bicgs_x(long n, double *x, double *r,
   const double alpha = st->alpha;
   const double omega = sconv ? 0.0 : st->aux[3] / st->aux[4];
   double w[2] = {0, 0}, tot[2];
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) {
     const double xi = x[i] + alpha * phat[i] + omega * shat[i];
     const double ri = sconv ? s[i] : s[i] - omega * t[i];
     x[i] = xi;
@@ -1838,7 +1850,9 @@ __global__ __launch_bounds__(WG) void k_fcg_combine(long n, const double *__rest
   double v[FCG_NV];
 #pragma unroll
   for (int j = 0; j < FCG_NV; j++) v[j] = 0.0;
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) {
     const double ri = r[i];
     const double a = dinv[i] * ri;
     double zi = a;
@@ -1905,7 +1919,9 @@ __global__ __launch_bounds__(WG) void k_fcg_p(long n, const double *__restrict__
     st->beta = beta[0];
   }
   const long slot = k % m;
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) {
     double acc = restart ? dinv[i] * r[i] : z[i];
 #pragma unroll
     for (int j = 0; j < FCG_MAXM; j++)
@@ -1995,7 +2011,9 @@ __global__ __launch_bounds__(WG) void k_fcg_update(long n, double *__restrict__ 
     return;
   }
   double v[4] = {0, 0, 0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+  long i1_, i2_;
+  xcd_chunk(n, i1_, i2_);
+  for (long i = i1_ + threadIdx.x; i < i2_; i += WG) {
     const double xi = x[i] + alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     x[i] = xi;
