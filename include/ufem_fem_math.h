@@ -12,8 +12,16 @@
  * reference's row-slot sparse form (MAT_SPAR) with 1-based column ids, slots
  * filled in insertion order and 0 marking the first empty slot
  * (fem/fem_math.c:185-235).  The GPU serves MAT_SPAR matrices and VEC_FULL
- * vectors; any other type returns AF_ERR_TYP (a dense or sparse-vector call
- * belongs to the reference's own CPU code, which stays in place).
+ * vectors.  A call with any other type (a dense matrix, a sparse vector) is
+ * handed to the reference's own CPU function when the program links it under
+ * the name <function>_cpu (fem_math.o and fem_eqs.o compiled with
+ * -Dfunction=function_cpu, see the integration notes); without it the call
+ * returns AF_ERR_TYP (NAN for the functions returning double).
+ *
+ * The GPU keeps a device copy of every matrix it is handed (a side table
+ * keyed by the data pointer, len, rows and cols; see ufem_gpu.h for the
+ * policy calls ufem_gpu_set_mirror_policy, ufem_gpu_mark_dirty and
+ * ufem_gpu_release).
  */
 #ifndef UFEM_FEM_MATH_H
 #define UFEM_FEM_MATH_H
@@ -97,6 +105,15 @@ double femMatNormBig(tMatrix *a);
 
 /* c = amult a + bmult b.  Replaces femVecLinCombBig, fem/fem_math.c:1543. */
 int femVecLinCombBig(double amult, tVector *a, double bmult, tVector *b, tVector *c);
+
+/* every stored value (a->len of them) to 0, the slots' columns kept; a device
+ * copy of the matrix is cleared with it.  Replaces femMatSetZeroBig,
+ * fem/fem_math.c:483-523 (fem/fem_math.h:107). */
+void femMatSetZeroBig(tMatrix *a);
+
+/* every stored value (a->len) to 0.  Replaces femVecSetZeroBig,
+ * fem/fem_math.c:920-958 (fem/fem_math.h:125). */
+void femVecSetZeroBig(tVector *a);
 
 
 #ifdef __cplusplus

@@ -24,6 +24,8 @@
 #ifndef UFEM_GPU_H
 #define UFEM_GPU_H
 
+#include "ufem_fem_math.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -269,6 +271,39 @@ int  ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, co
  * row-major 24x24 each -- e009_stiff Mode=AF_NO, fem/fem_e009.c:261-635 */
 int  ufem_gpu_element_stiffness(long n, const double *xyz, const double *E, const double *nu,
                                 int npts, double *Ke);
+
+/* ---- the drop-in's device copies of row-slot matrices (fem_dropin.c) ------
+ * policy 0 (default): every drop-in call reads the host matrix once --
+ *   fingerprints its columns (a changed pattern rebuilds the copy) and
+ *   uploads its values -- so any host change between calls is seen;
+ * policy 1: a copy stays valid between calls until ufem_gpu_mark_dirty
+ *   (values changed on the host: femMatPutAdd, femMatSetZeroRow/Col, the
+ *   supports of fem_add_disps) or ufem_gpu_release (matrix freed, pattern
+ *   changed) is called for the matrix. */
+void ufem_gpu_set_mirror_policy(int policy);
+void ufem_gpu_mark_dirty(const tMatrix *a);
+void ufem_gpu_release(const tMatrix *a);
+/* shape of a matrix's device copy and its host costs: info[0] block size
+ * (3: node blocks, 1: scalar), [1] rows, [2] filled slots, [3] ms of the
+ * last build, [4] ms of the last refresh (0 when a valid copy was used) */
+int  ufem_gpu_mirror_info(const tMatrix *a, double *info);
+
+/* fem_fill_K(AF_NO) (fem/fem_sol.c:897-989) for a model of bricks (element
+ * type 9) with isotropic material (type 1): K += sum of the element
+ * matrices, computed on the GPU (npts 2 or 3 Gauss points per axis), added
+ * into the row-slot K that fem_sol_alloc made (frompos/defpos from femKhit)
+ * as femMatPutAdd(FEM_ADD) adds: existing columns take the sum, new ones
+ * fill the row's first empty slots (AF_ERR_BIG when none is left).  m gives
+ * node coordinates, element nodes (0-based positions, the brick's order), E
+ * and nu per element; its load list is not used.  dof_map is the
+ * reference's nDOFfld: known_dofs entries per node, entries 0..2 the 1-based
+ * K positions of ux, uy, uz, 0 for a DOF not in K (-fbc, unused node).
+ * The device system is kept for the next call on the same connectivity. */
+int  ufem_gpu_fill_K(tMatrix *K, const ufem_mesh *m, const long *dof_map, long known_dofs, int npts);
+/* of the last ufem_gpu_fill_K: host set-up, device assembly, write-back (ms) */
+void ufem_gpu_fill_K_timings(double t[3]);
+/* frees the device system ufem_gpu_fill_K keeps */
+void ufem_gpu_fill_K_release(void);
 
 /* stopping rule used by the drop-in femEqsCGwJ / femEqsBiCCSwJ (default 0) */
 void ufem_gpu_set_stop_rule(int rule);

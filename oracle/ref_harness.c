@@ -32,6 +32,12 @@
  * Usage:
  *   ufem_ref_harness -i in.fem -o outdir [-solver cg|bicgs] [-eps E]
  *                    [-maxit N] [-t THREADS] [-ke ePos ...] [-nodump-k]
+ *                    [-fbc] [-np]
+ *   -fbc  zero-valued supports left out of K (femFastBC, the fem binary's
+ *         -fbc, fem/fem_para.c:1053-1057)
+ *   -np   the input carries IDs, renumbered to positions by femRenumIndex
+ *         (fem/fem_io.c:50-80; the fem binary without -p); the renumbered
+ *         element nodes, load nodes and material/real-set indices are dumped
  * Output: raw little-endian arrays outdir/<name>.i64 / .f64 and a JSON
  * summary on stdout (phase wall times, iteration count, return codes).
  */
@@ -58,6 +64,7 @@ extern int fem_fill_K(long mode);
 extern int fem_add_loads(long step);
 extern int fem_add_disps(long disp_mode, long step);
 extern long femPreparsedData;
+extern long femFastBC;
 extern long femUseThreads;
 extern long femThreadNum;
 extern long femThreadMin;
@@ -135,7 +142,7 @@ int main(int argc, char **argv)
   double eps = FEM_ZERO / 10000.0;    /* value femSolve passes, fem/fem_sol.c:1722 */
   long maxit = -1, threads = 0, i;
   long ke_list[64]; int n_ke = 0;
-  int dumpk = 1, rv, rv_solve = 0;
+  int dumpk = 1, rv, rv_solve = 0, fbc = 0, preparsed = 1;
   double t0, t_read, t_dofs, t_fill, t_bc, t_solve, t_react;
   double normA = 0.0;
 
@@ -148,13 +155,16 @@ int main(int argc, char **argv)
     else if (!strcmp(argv[i], "-t") && i + 1 < argc) threads = atol(argv[++i]);
     else if (!strcmp(argv[i], "-ke") && i + 1 < argc && n_ke < 64) ke_list[n_ke++] = atol(argv[++i]);
     else if (!strcmp(argv[i], "-nodump-k")) dumpk = 0;
+    else if (!strcmp(argv[i], "-fbc")) fbc = 1;
+    else if (!strcmp(argv[i], "-np")) preparsed = 0;
     else { fprintf(stderr, "bad arg %s\n", argv[i]); return 2; }
   }
   if (fin == NULL) { fprintf(stderr, "need -i\n"); return 2; }
   mkdir(g_out, 0755);
 
   msgout = stderr;
-  femPreparsedData = AF_YES;          /* -p, fem/fem_para.c:275 */
+  femPreparsedData = preparsed ? AF_YES : AF_NO;   /* -p, fem/fem_para.c:275 */
+  femFastBC = fbc ? AF_YES : AF_NO;                 /* -fbc, fem/fem_para.c:1053 */
   if (threads > 0) {                  /* -t N, fem/fem_para.c:584-606 */
     femUseThreads = AF_YES;
     femThreadNum = threads > AF_MAX_THREADS ? AF_MAX_THREADS : threads;
@@ -167,6 +177,12 @@ int main(int argc, char **argv)
   if ((rv = femResNull()) != AF_OK) return 11;
   if ((rv = femReadInput((char *)fin)) != AF_OK) { fprintf(stderr, "read %d\n", rv); return 12; }
   t_read = now_s();
+  if (!preparsed) {
+    dump_raw("eNodesL", "i64", eNodesL, sizeof(long), (size_t)eLenL);
+    dump_raw("eMP", "i64", eMP, sizeof(long), (size_t)eLen);
+    dump_raw("eRS", "i64", eRS, sizeof(long), (size_t)eLen);
+    if (nlLen > 0) dump_raw("nlNode", "i64", nlNode, sizeof(long), (size_t)nlLen);
+  }
 
   if ((rv = femElemTypeInit()) != AF_OK) return 13;
   if ((rv = femMatTypeInit()) != AF_OK) return 14;
@@ -227,9 +243,9 @@ int main(int argc, char **argv)
   printf("{\"n\": %ld, \"solver\": \"%s\", \"rv\": %d, \"spmv_calls\": %ld, "
          "\"normA\": %.17g, \"eps\": %.17g, \"maxit\": %ld, \"threads\": %ld, "
          "\"t_read\": %.6f, \"t_dofs\": %.6f, \"t_fill\": %.6f, \"t_bc\": %.6f, "
-         "\"t_solve\": %.6f, \"t_react\": %.6f}\n",
+         "\"t_solve\": %.6f, \"t_react\": %.6f, \"fbc\": %d, \"preparsed\": %d}\n",
          nDOFAct, solver, rv_solve, g_spmv_calls, normA, eps, maxit, threads,
          t_read - t0, t_dofs - t_read, t_fill - t_dofs, t_bc - t_fill,
-         t_solve - t_bc, t_react - t_solve);
+         t_solve - t_bc, t_react - t_solve, fbc, preparsed);
   return 0;
 }

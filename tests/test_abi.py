@@ -26,7 +26,9 @@ def declared():
 def test_headers_declare_the_dropin_symbols():
     names = declared()
     for n in ("femEqsCGwJ", "femEqsBiCCSwJ", "femEqsCGwSSOR", "femMatVecMultBig", "femVecVecMultBig",
-              "femVecNormBig", "femMatNormBig", "femVecLinCombBig", "ufem_sys_create",
+              "femVecNormBig", "femMatNormBig", "femVecLinCombBig", "femMatSetZeroBig", "femVecSetZeroBig",
+              "ufem_gpu_fill_K", "ufem_gpu_set_mirror_policy", "ufem_gpu_mark_dirty", "ufem_gpu_release",
+              "ufem_sys_create",
               "ufem_sys_solve", "ufem_gpu_element_stiffness", "ufem_sys_update",
               "monte_dlib_interface_type", "monte_nums_of_vars", "monte_ivar_name", "monte_ovar_name",
               "monte_init_lib_stuff", "monte_clean_lib_stuff", "monte_solution"):

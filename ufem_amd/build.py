@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libufem_gpu.so")
 
 HIP_SOURCES = ["ugk_kernels.hip", "ugk_unet.hip"]
-C_SOURCES = ["ufem_sys.c", "ufem_solve.c", "ufem_ssor.c", "ufem_coarse.c", "fem_dropin.c", "ufem_batch.c", "ufem_net.c",
+C_SOURCES = ["ufem_sys.c", "ufem_solve.c", "ufem_ssor.c", "ufem_coarse.c", "fem_dropin.c", "fem_fillk.c", "ufem_batch.c", "ufem_net.c",
              "ufem_input.c", "ufem_monte.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -55,7 +55,7 @@ def build(verbose: bool = False) -> str:
             print(msg)
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lm"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lm", "-lpthread", "-ldl"])
     os.replace(tmp, LIB)
     return LIB
 

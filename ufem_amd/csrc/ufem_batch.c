@@ -114,6 +114,13 @@ ufem_batch *ufem_batch_create(int n, const ufem_mesh *meshes)
     if (c == b->n_cls) {
       b->cls[c].sys = ufem_sys_create(&meshes[i]);
       if (!b->cls[c].sys) goto fail;
+      /* the batch kernel solves K u = F with F zeroed at the supports: a
+         support of non-zero value would be solved as if it were 0 */
+      if (b->cls[c].sys->any_prescribed) {
+        ufem_set_error("batch: sample %d has a support of non-zero value (solve it with ufem_sys_*)", i);
+        b->n_cls++;
+        goto fail;
+      }
       /* loads and supports once per class (identical within a class) */
       if (ufem_sys_add_loads(b->cls[c].sys) != AF_OK) goto fail;
       rep[c] = &meshes[i];

@@ -67,7 +67,8 @@ struct ufem_sys {
   long n_nl;
   long *nl_node, *nl_type, *nl_dir;
   double *nl_val;
-  double *F_host;
+  double *F_host;           /* F as the solve takes it (after add_disps)     */
+  double *F_load;           /* nodal forces alone, as add_loads left them    */
   long n_fixed;             /* distinct fixed DOFs, first-occurrence order */
   int *fixed_host, *d_fixed;
   double *fix_val;          /* prescribed value of each fixed DOF          */

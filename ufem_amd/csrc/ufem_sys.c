@@ -258,7 +258,7 @@ void ufem_sys_destroy(ufem_sys *s)
   for (i = 0; i < 6; i++) ugk_event_destroy(s->ev[i]);
   ugk_stream_destroy(s->stream);
   free(s->color_off); free(s->nl_node); free(s->nl_type); free(s->nl_dir); free(s->nl_val);
-  free(s->F_host); free(s->fixed_host); free(s->react_node); free(s->react_dof);
+  free(s->F_host); free(s->F_load); free(s->fixed_host); free(s->react_node); free(s->react_dof);
   free(s->react_fix); free(s->react_init); free(s->fix_val);
   free(s);
 }
@@ -328,7 +328,8 @@ ufem_sys *ufem_sys_create(const ufem_mesh *m)
   s->nl_dir = (long *)malloc(sizeof(long) * (size_t)(m->n_nl + 1));
   s->nl_val = (double *)malloc(sizeof(double) * (size_t)(m->n_nl + 1));
   s->F_host = (double *)calloc((size_t)s->n, sizeof(double));
-  if (!s->nl_node || !s->nl_type || !s->nl_dir || !s->nl_val || !s->F_host) goto fail;
+  s->F_load = (double *)calloc((size_t)s->n, sizeof(double));
+  if (!s->nl_node || !s->nl_type || !s->nl_dir || !s->nl_val || !s->F_host || !s->F_load) goto fail;
   if (m->n_nl > 0) {
     memcpy(s->nl_node, m->nl_node, sizeof(long) * (size_t)m->n_nl);
     memcpy(s->nl_type, m->nl_type, sizeof(long) * (size_t)m->n_nl);
@@ -452,6 +453,7 @@ int ufem_sys_update(ufem_sys *s, const ufem_mesh *m)
       ugk_h2d(s->d_nu, m->nu, sizeof(double) * (size_t)s->n_elems, s->stream) || ugk_sync(s->stream))
     return gpu_fail("mesh upload");
   memset(s->F_host, 0, sizeof(double) * (size_t)s->n);
+  memset(s->F_load, 0, sizeof(double) * (size_t)s->n);
   memset(s->react_init, 0, sizeof(double) * (size_t)(s->n_react + 1));
   s->filled = 0;
   s->loaded = 0;
@@ -485,13 +487,19 @@ int ufem_sys_fill_K(ufem_sys *s, int npts)
 }
 
 /* fem_add_loads(0): nodal forces (type 3) into F in input order,
- * fem/fem_sol.c:1119-1189, fem/fem_loads.c:343-397 */
+ * fem/fem_sol.c:1119-1189, fem/fem_loads.c:343-397.  F starts from zero, as
+ * the reference's F does at every femSolve (fem_sol_alloc), so a second call
+ * gives the same F; the loads alone are kept for add_disps, which rebuilds
+ * the supported F from them after every fill_K. */
 int ufem_sys_add_loads(ufem_sys *s)
 {
   long i;
+  memset(s->F_load, 0, sizeof(double) * (size_t)s->n);
   for (i = 0; i < s->n_nl; i++)
-    if (s->nl_type[i] == 3) s->F_host[3 * s->nl_node[i] + s->nl_dir[i] - 1] += s->nl_val[i];
+    if (s->nl_type[i] == 3) s->F_load[3 * s->nl_node[i] + s->nl_dir[i] - 1] += s->nl_val[i];
+  memcpy(s->F_host, s->F_load, sizeof(double) * (size_t)s->n);
   s->loaded = 1;
+  s->bc_applied = 0;
   if (ugk_h2d(s->d_F, s->F_host, sizeof(double) * (size_t)s->n, s->stream) || ugk_sync(s->stream))
     return gpu_fail("F upload");
   return AF_OK;
@@ -509,6 +517,11 @@ int ufem_sys_add_disps(ufem_sys *s)
   long r;
   if (!s->filled) { ufem_set_error("fill_K first"); return AF_ERR; }
   if (s->bc_applied) return AF_OK;
+  /* F and the reaction slots from the loads alone: a refill of K (new
+     moduli, fill_K) is followed by a fresh application, never by a second
+     correction on top of the first */
+  memcpy(s->F_host, s->F_load, sizeof(double) * (size_t)s->n);
+  memset(s->react_init, 0, sizeof(double) * (size_t)(s->n_react + 1));
   for (r = 0; r < s->n_react; r++) {
     long d = 3 * s->react_node[r] + s->react_dof[r] - 1;
     long first = r, q;

@@ -142,6 +142,8 @@ int ugk_reactions(const ugk_mat *K, long n_fixed, const int *d_fixed, int maxb,
                   const double *d_Forig, double *d_R, void *stream);
 /* FP32 copy of the values */
 int ugk_vals_to_f32(const ugk_mat *K, void *stream);
+/* d_dst[d_map[i]] = d_src[i] for i < n (a matrix's values from host order) */
+int ugk_scatter_vals(long n, const int *d_map, const double *d_src, double *d_dst, void *stream);
 
 /* ---------------- sparse kernels ---------------------------------------- */
 int ugk_spmv(const ugk_mat *K, const double *d_x, double *d_y, void *stream);

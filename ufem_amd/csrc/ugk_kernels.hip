@@ -715,6 +715,14 @@ __global__ void k_vals_f32(long n, const double *__restrict__ a, T *__restrict__
     b[i] = (T)a[i];
 }
 
+// dst[map[i]] = src[i]: host-ordered values into their device slots
+__global__ void k_scatter_vals(long n, const int *__restrict__ map, const double *__restrict__ src,
+                               double *__restrict__ dst)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[map[i]] = src[i];
+}
+
 __global__ void k_fill_zero(long nwords, unsigned int *p)
 {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (long)gridDim.x * blockDim.x)
@@ -2895,6 +2903,14 @@ int ugk_vals_to_f32(const ugk_mat *K, void *s)
   if (!K->d_vals32) { snprintf(g_err, sizeof(g_err), "no fp32 buffer"); return -1; }
   k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_vals32);
   CK_LAUNCH("k_vals_f32");
+  return 0;
+}
+
+int ugk_scatter_vals(long n, const int *d_map, const double *d_src, double *d_dst, void *s)
+{
+  if (n <= 0) return 0;
+  k_scatter_vals<<<vec_blocks(n), WG, 0, S(s)>>>(n, d_map, d_src, d_dst);
+  CK_LAUNCH("k_scatter_vals");
   return 0;
 }
 

@@ -152,6 +152,15 @@ def lib():
             "femMatNormBig": (C.c_double, [C.POINTER(tMatrix)]),
             "femVecLinCombBig": (C.c_int, [C.c_double, C.POINTER(tVector), C.c_double,
                                            C.POINTER(tVector), C.POINTER(tVector)]),
+            "femMatSetZeroBig": (None, [C.POINTER(tMatrix)]),
+            "femVecSetZeroBig": (None, [C.POINTER(tVector)]),
+            "ufem_gpu_set_mirror_policy": (None, [C.c_int]),
+            "ufem_gpu_mark_dirty": (None, [C.POINTER(tMatrix)]),
+            "ufem_gpu_release": (None, [C.POINTER(tMatrix)]),
+            "ufem_gpu_mirror_info": (C.c_int, [C.POINTER(tMatrix), P_D]),
+            "ufem_gpu_fill_K": (C.c_int, [C.POINTER(tMatrix), C.POINTER(_Mesh), P_L, C.c_long, C.c_int]),
+            "ufem_gpu_fill_K_timings": (None, [P_D]),
+            "ufem_gpu_fill_K_release": (None, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -587,6 +596,50 @@ class RowSlotMatrix:
     """A uFEM row-slot matrix (fem/fem_math.h:47-57) owned by numpy arrays,
     for calling the drop-in entry points exactly as uFEM does."""
 
+    @classmethod
+    def empty(cls, n, capacity):
+        """all slots empty (pos 0), as femMatAlloc leaves them (fem/fem_math.c:57-133)"""
+        cap = np.asarray(capacity, dtype=np.int64)
+        return cls(n, np.zeros(n, dtype=np.int64), np.zeros(0, dtype=np.int64), np.zeros(0), capacity=cap)
+
+    def rows_csr(self):
+        """(indptr, 0-based cols, vals) of the filled slots, each row sorted by column"""
+        n = self.s.rows
+        rl = np.zeros(n, dtype=np.int64)
+        for i in range(n):
+            f, c = int(self.frompos[i]), int(self.defpos[i])
+            seg = self.pos[f:f + c]
+            z = np.nonzero(seg <= 0)[0]
+            rl[i] = z[0] if z.size else c
+        ip = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(rl, out=ip[1:])
+        cols = np.empty(ip[-1], dtype=np.int64)
+        vals = np.empty(ip[-1])
+        for i in range(n):
+            f, k = int(self.frompos[i]), int(rl[i])
+            o = np.argsort(self.pos[f:f + k], kind="stable")
+            cols[ip[i]:ip[i + 1]] = self.pos[f:f + k][o] - 1
+            vals[ip[i]:ip[i + 1]] = self.data[f:f + k][o]
+        return ip, cols, vals
+
+    def mirror_info(self):
+        """(block size, rows, filled slots, build ms, refresh ms) of its device copy"""
+        out = np.zeros(5)
+        rv = lib().ufem_gpu_mirror_info(C.byref(self.s), _pd(out))
+        if rv != AF_OK:
+            raise RuntimeError(last_error())
+        return {"bs": int(out[0]), "rows": int(out[1]), "nfill": int(out[2]), "build_ms": out[3],
+                "refresh_ms": out[4]}
+
+    def mark_dirty(self):
+        lib().ufem_gpu_mark_dirty(C.byref(self.s))
+
+    def release(self):
+        lib().ufem_gpu_release(C.byref(self.s))
+
+    def set_zero(self):
+        lib().femMatSetZeroBig(C.byref(self.s))
+
     def __init__(self, n, rowlen, cols1, vals, capacity=None):
         rowlen = np.asarray(rowlen, dtype=np.int64)
         cap = rowlen if capacity is None else np.asarray(capacity, dtype=np.int64)
@@ -626,6 +679,50 @@ def femEqsCGwSSOR(K: RowSlotMatrix, b: Vec, x: Vec, eps: float, maxit: int) -> i
 
 def set_stop_rule(rule: int) -> None:
     lib().ufem_gpu_set_stop_rule(rule)
+
+
+def set_mirror_policy(policy: int) -> None:
+    lib().ufem_gpu_set_mirror_policy(policy)
+
+
+def femMatVecMultBig(K: RowSlotMatrix, b: Vec, c: Vec) -> int:
+    return lib().femMatVecMultBig(C.byref(K.s), C.byref(b.s), C.byref(c.s))
+
+
+def fill_K(K: RowSlotMatrix, mesh, dof_map, known_dofs=7, npts=3) -> int:
+    """fem_fill_K(AF_NO) on the GPU into the row-slot K through the DOF map
+    (nDOFfld: known_dofs entries per node, 1-based positions, 0 = none)"""
+    keep = []
+    m = _mesh_struct(mesh, keep)
+    dm = np.ascontiguousarray(dof_map, dtype=np.int64)
+    return lib().ufem_gpu_fill_K(C.byref(K.s), C.byref(m), _pl(dm), known_dofs, npts)
+
+
+def fill_K_timings():
+    t = np.zeros(3)
+    lib().ufem_gpu_fill_K_timings(_pd(t))
+    return {"setup_ms": t[0], "device_ms": t[1], "writeback_ms": t[2]}
+
+
+def dof_map(mesh, fbc=False, known_dofs=7):
+    """the reference's nDOFfld (fem_dofs, fem/fem_sol.c:1263-1391): node-major
+    1-based positions of ux, uy, uz; with fbc the DOFs of zero-valued supports
+    are left out (femFastBC, :1280-1296)"""
+    nn = mesh.n_nodes
+    drop = np.zeros(nn * known_dofs, dtype=bool)
+    if fbc:
+        for nd, t, d, v in zip(mesh.nl_node, mesh.nl_type, mesh.nl_dir, mesh.nl_val):
+            if int(t) == 1 and abs(float(v)) <= 1e-17:
+                drop[int(nd) * known_dofs + int(d) - 1] = True
+    act = np.zeros(nn * known_dofs, dtype=bool)
+    used = np.zeros(nn, dtype=bool)
+    used[np.asarray(mesh.enodes).ravel()] = True
+    for d in range(3):
+        act[np.nonzero(used)[0] * known_dofs + d] = True
+    act &= ~drop
+    m = np.zeros(nn * known_dofs, dtype=np.int64)
+    m[act] = np.arange(1, int(act.sum()) + 1)
+    return m
 
 
 def last_solve() -> dict:

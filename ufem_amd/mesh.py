@@ -225,34 +225,55 @@ def batch_meshes(n: int = 256, seed: int = 7, sizes=BATCH_SIZES, cov: float = 0.
     return out
 
 
-def write_fem_input(mesh: Mesh, path: str, material_per_element: bool = True, monte=None) -> None:
+def input_ids(mesh: Mesh, seed: int = 0):
+    """Node, element and material-set IDs for an input file with IDs (the
+    form femReadInput renumbers without -p, fem/fem_io.c:50-80): shuffled,
+    gapped numbers, unique within each kind."""
+    rng = np.random.default_rng(seed)
+    nid = 7 + 10 * rng.permutation(mesh.n_nodes).astype(np.int64)
+    eid = 3 + 4 * rng.permutation(mesh.n_elems).astype(np.int64)
+    mid = 1000 + 3 * rng.permutation(mesh.n_elems).astype(np.int64)
+    return nid, eid, mid
+
+
+def write_fem_input(mesh: Mesh, path: str, material_per_element: bool = True, monte=None, ids=None) -> None:
     """Write `mesh` in the reference's preparsed input format (fem/feminput.5,
     read by femReadInput fem/fem_io.c:380-640 with -p): material type 1 with
     values {DENS, EX, NU, ALPHA, KXX, C, PRICE} (fem/fem_m001.c:39-41), one
     material set per element when moduli differ; numbers printed %.17g.
     `monte` = (input rows, output rows), each row (pos, type, item, subitem,
     subitemiter), appends the random-variable section fem_monte_read_data
-    reads (fem/fem_mont.c:866-909)."""
+    reads (fem/fem_mont.c:866-909).  `ids` = (node IDs, element IDs,
+    material-set IDs) writes the form with IDs instead (no -p): element node
+    lists, element material sets and load nodes then name IDs."""
     ne, nn = mesh.n_elems, mesh.n_nodes
+    if ids is None:
+        nid = np.arange(1, nn + 1)
+        eid = np.arange(1, ne + 1)
+        mid = np.arange(1, ne + 1)
+        nref = np.arange(nn)           # what element and load lines name: positions
+    else:
+        nid, eid, mid = (np.asarray(a, dtype=np.int64) for a in ids)
+        nref = nid
     lines = ["1 -1", "1 0 0", "0"]
     if material_per_element:
         lines.append(str(ne))
-        lines += [f"{e + 1} 1 7" for e in range(ne)]
+        lines += [f"{int(mid[e])} 1 7" for e in range(ne)]
         lines.append(str(7 * ne))
         lines += [f"2500 {mesh.E[e]:.17g} {mesh.nu[e]:.17g} 0 0 0 0" for e in range(ne)]
-        mat = np.arange(ne)
+        mat = np.arange(ne) if ids is None else mid
     else:
-        lines += ["1", "1 1 7", "7", f"2500 {mesh.E[0]:.17g} {mesh.nu[0]:.17g} 0 0 0 0"]
-        mat = np.zeros(ne, dtype=np.int64)
+        lines += ["1", f"{int(mid[0])} 1 7", "7", f"2500 {mesh.E[0]:.17g} {mesh.nu[0]:.17g} 0 0 0 0"]
+        mat = np.zeros(ne, dtype=np.int64) if ids is None else np.full(ne, mid[0])
     lines.append(str(nn))
-    lines += [f"{p + 1} {c[0]:.17g} {c[1]:.17g} {c[2]:.17g}" for p, c in enumerate(mesh.coords)]
+    lines += [f"{int(nid[p])} {c[0]:.17g} {c[1]:.17g} {c[2]:.17g}" for p, c in enumerate(mesh.coords)]
     lines.append(str(ne))
-    lines += [f"{e + 1} 9 0 {mat[e]} 8" for e in range(ne)]
+    lines += [f"{int(eid[e])} 9 0 {int(mat[e])} 8" for e in range(ne)]
     lines.append(str(8 * ne))
-    lines += [" ".join(str(int(v)) for v in row) for row in mesh.enodes]
+    lines += [" ".join(str(int(nref[v])) for v in row) for row in mesh.enodes]
     lines.append("0 0")
     lines.append(str(mesh.nl_node.size))
-    lines += [f"{int(n)} {int(t)} {int(d)} {v:.17g}" for n, t, d, v in
+    lines += [f"{int(nref[n])} {int(t)} {int(d)} {v:.17g}" for n, t, d, v in
               zip(mesh.nl_node, mesh.nl_type, mesh.nl_dir, mesh.nl_val)]
     lines.append("0")        # element loads
     lines.append("0 0 0")    # time-dependent array data
