@@ -56,9 +56,13 @@ enum {
                                    rows in greedy colour order */
   UFEM_SOLVER_TWOLEVEL = 6,     /* PCG with z = dinv r + P A^-1 P^T r, A = P^T K P on a coarse
                                    node grid (ufem_sys_set_coarse first) */
-  UFEM_SOLVER_TWOLEVEL_MIXED = 7 /* the same with q = K p from the FP32 copy of K and FP64
+  UFEM_SOLVER_TWOLEVEL_MIXED = 7, /* the same with q = K p from the FP32 copy of K and FP64
                                    residual refreshes (inner_eps, inner_maxit), convergence
                                    accepted on a refreshed residual only */
+  UFEM_SOLVER_MIXED_V64 = 8     /* Jacobi PCG with q = K p from the FP32 copy of K and every
+                                   vector, dot and recurrence in FP64; FP64 residual
+                                   refreshes (inner_eps, inner_maxit), convergence accepted
+                                   on a refreshed residual only */
 };
 enum { UFEM_RULE_REFERENCE = 0, UFEM_RULE_RELATIVE = 1 };
 /* streamed: single-reduction PCG, two kernels per iteration (w = K u with
@@ -85,6 +89,11 @@ typedef struct {
   int coarse;        /* learned, learned-mixed: add the two-level correction
                         (ufem_sys_set_coarse first); net may then be NULL,
                         which leaves dinv r + the correction */
+  int inner_restart; /* mixed: classical iterative refinement -- every FP64
+                        refresh starts a new FP32 CG on the correction
+                        equation, run until its residual has fallen by
+                        inner_eps (or inner_maxit iterations); 0: the FP32 CG
+                        keeps its direction across refreshes (reliable update) */
 } ufem_solve_opts;
 
 typedef struct {
@@ -92,7 +101,9 @@ typedef struct {
   long iters;        /* iterations (all inner iterations for mixed)       */
   long outer;        /* mixed, learned-mixed: FP64 residual refreshes;
                         learned: iteration of the Jacobi fallback;
-                        SSOR: colours of the sweep order                  */
+                        SSOR: colours of the sweep order;
+                        BiCGStab: restarts after a stop that the FP64
+                        true residual did not confirm                     */
   int fallback;      /* learned: 1 if Jacobi-PCG had to take over         */
   double relres;     /* last ||r||/||b|| the solver tested                */
   double true_relres;/* ||b - K u|| / ||b|| recomputed in FP64            */

@@ -253,41 +253,68 @@ static int solve_pcg(const ugk_mat *K, long n, const double *b, double *x, ufem_
   return info->status;
 }
 
+/* BiCGStab stops on its recursive residual, which drifts from b - A x as
+ * the iterations go on.  A stop is therefore checked on the FP64 true
+ * residual; if that misses the tolerance, BiCGStab restarts from the
+ * current x (r = b - A x, new shadow residual) with the iterations left, at
+ * most four times. */
 static int solve_bicgs(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
                        const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
 {
-  int rv;
+  int rv, round;
+  long total = 0;
+  double rr_true = 0.0;
   pcg_ctx c = {K, n, x, w, 0};
-  /* BiCGStab tests ||r|| <= eps ||b||; the reference rule is mapped on the
-     same relative test with eps (its own test is not usable, see header) */
-  rv = prepare(K, w, stream, o->eps, 1, maxit);
-  if (rv != AF_OK) return rv;
-  if (ugk_bicgs_start(K, n, b, x, w->r, w->rhat, w->p[0], w->v, w->st, w->part, stream)) return gpu_fail("start");
-  if (get_state(w, stream)) return AF_ERR;
-  if (!w->st_host->done) {
-    rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 16, bicgs_step, &c, 1);
+  ugk_state *h = w->st_host;
+  for (round = 0;; round++) {
+    /* BiCGStab tests ||r|| <= eps ||b||; the reference rule is mapped on the
+       same relative test with eps (its own test is not usable, see header) */
+    rv = prepare(K, w, stream, o->eps, 1, maxit - total);
     if (rv != AF_OK) return rv;
+    if (ugk_bicgs_start(K, n, b, x, w->r, w->rhat, w->p[0], w->v, w->st, w->part, stream))
+      return gpu_fail("start");
     if (get_state(w, stream)) return AF_ERR;
+    if (!h->done) {
+      rv = run_chunks(w, stream, maxit - total, o->chunk > 0 ? o->chunk : 16, bicgs_step, &c, 1);
+      if (rv != AF_OK) return rv;
+      if (get_state(w, stream)) return AF_ERR;
+    }
+    total += h->iter;
+    if (h->bb <= 0.0) {
+      if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+      info->status = AF_OK;
+      return AF_OK;
+    }
+    info->relres = sqrt(h->rr / h->bb);
+    info->status = status_of(h->done);
+    if (h->done != 1) break;
+    {
+      const double bb = h->bb;
+      if (ugk_residual(K, n, b, x, w->r, w->st, w->part, stream) || get_state(w, stream))
+        return gpu_fail("residual");
+      rr_true = h->rr;
+      if (sqrt(rr_true) <= o->eps * sqrt(bb)) break;
+      info->status = AF_ERR;            /* a stop the true residual does not confirm */
+      if (round >= 4 || total >= maxit) break;
+    }
   }
-  if (w->st_host->bb <= 0.0) {
-    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
-    info->status = AF_OK;
-    return AF_OK;
-  }
-  info->iters = w->st_host->iter;
-  info->relres = sqrt(w->st_host->rr / w->st_host->bb);
-  info->status = status_of(w->st_host->done);
+  info->iters = total;
+  info->outer = round;                  /* restarts */
   return info->status;
 }
 
 /* Mixed precision: K stored once more in FP32; CG runs on the FP32 matrix
- * and FP32 vectors (dot products summed in FP64) without restarts, and
- * whenever its recursive residual has dropped by inner_eps (default 0.1)
- * below the last true residual -- or inner_maxit (default 500) iterations
- * have passed -- the FP32 solution increment is added to the FP64 solution
- * and r = b - K u is recomputed with the FP64 matrix.  The solve stops only
- * on such an FP64 residual, ||b - K u|| <= eps ||b||, or after five refreshes
- * in a row without progress (AF_ERR). */
+ * and FP32 vectors (dot products summed in FP64), and whenever its recursive
+ * residual has dropped by inner_eps (default 0.1) below the last true
+ * residual -- or inner_maxit (default 500) iterations have passed -- the
+ * FP32 solution increment is added to the FP64 solution and r = b - K u is
+ * recomputed with the FP64 matrix.  By default the FP32 CG keeps its search
+ * direction across such a refresh (reliable update); with inner_restart it
+ * starts afresh on the correction equation K d = r, which is classical
+ * iterative refinement with an inner CG to inner_eps (inner_maxit then
+ * defaults to no cap).  The solve stops only on an FP64 residual,
+ * ||b - K u|| <= eps ||b||, or after five refreshes in a row without
+ * progress (AF_ERR). */
 static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
                        const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
 {
@@ -310,7 +337,9 @@ static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_wor
   h->delta = o->inner_eps > 0 ? o->inner_eps : 0.1;
   h->ru_req = -1;
   h->ru_at = 0;
-  h->max_gap = o->inner_maxit > 0 ? o->inner_maxit : 500;
+  /* classical refinement runs each inner CG to inner_eps, uncapped by default */
+  h->max_gap = o->inner_maxit > 0 ? o->inner_maxit : (o->inner_restart ? maxit : 500);
+  h->restart = o->inner_restart ? 1 : 0;
   if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
   if (ugk_mx_start(K, n, b, x, w->r, w->r32, w->z32, w->p32[0], w->x32, w->dinv32, w->part, stream))
     return gpu_fail("mixed start");
@@ -550,6 +579,64 @@ static int solve_twolevel(ugk_mat *Km, long n, const double *b, double *x, ufem_
   return info->status;
 }
 
+/* Mixed precision with FP64 vectors: Jacobi PCG whose only FP32 data is the
+ * stored copy of K, widened to FP64 as it is loaded for q = K p; every
+ * vector, dot product and recurrence stays FP64.  r drifts from b - K x as
+ * the FP32 products accumulate, so r is recomputed in FP64 whenever the
+ * recursive residual has fallen by inner_eps (default 0.1) since the last
+ * recomputation, or after inner_maxit (default 500) iterations, and a
+ * convergence is accepted on a recomputed residual only. */
+typedef struct {
+  const ugk_mat *K;
+  long n;
+  const double *b;
+  double *x;
+  ufem_work *w;
+} mx64_ctx;
+
+static int mx64_step(void *c, long it, void *stream)
+{
+  mx64_ctx *m = (mx64_ctx *)c;
+  ufem_work *w = m->w;
+  (void)it;
+  return ugk_mx64_iter(m->K, m->n, m->b, m->x, w->r, w->p[0], w->q, w->dinv, w->st, w->part, stream);
+}
+
+static int solve_mixed64(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
+                         const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
+{
+  const ugk_mat *K = Km;
+  mx64_ctx c = {K, n, b, x, w};
+  ugk_state *h = w->st_host;
+  int rv;
+  if (!Km->d_vals32) {
+    Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
+    if (!Km->d_vals32) return gpu_fail("fp32 matrix");
+  }
+  if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+  rv = prepare(K, w, stream, o->eps, o->rule, maxit);
+  if (rv != AF_OK) return rv;
+  h->delta = o->inner_eps > 0 ? o->inner_eps : 0.1;
+  h->max_gap = o->inner_maxit > 0 ? o->inner_maxit : 500;
+  h->ru_req = -1;
+  h->ru_at = 0;
+  if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
+  if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
+  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 16, mx64_step, &c, 1);
+  if (rv != AF_OK) return rv;
+  if (get_state(w, stream)) return AF_ERR;
+  if (h->bb <= 0.0) {
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+    return AF_OK;
+  }
+  info->iters = h->iter;
+  info->outer = h->n_ru;
+  info->relres = sqrt(h->rr / h->bb);
+  info->status = status_of(h->done);
+  return info->status;
+}
+
 int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
                       void *ev0, void *ev1, const ufem_solve_opts *o, ufem_solve_info *info)
 {
@@ -570,6 +657,7 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *b, double *x, ufem
     case UFEM_SOLVER_TWOLEVEL_MIXED:
       rv = solve_twolevel((ugk_mat *)K, n, b, x, w, stream, o, maxit, 1, info);
       break;
+    case UFEM_SOLVER_MIXED_V64: rv = solve_mixed64((ugk_mat *)K, n, b, x, w, stream, o, maxit, info); break;
     default: ufem_set_error("unknown solver %d", o->solver); return AF_ERR_VAL;
   }
   if (ugk_event_record(ev1, stream)) return gpu_fail("event");

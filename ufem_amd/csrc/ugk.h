@@ -75,7 +75,7 @@ typedef struct {
   int rule;           /* 0 reference rule, 1 relative residual          */
   int done;           /* 0 running, 1 converged, 2 breakdown, 3 maxit   */
   int err;            /* kernel-detected error (zero diagonal, detJ<=0)  */
-  int pad;
+  int restart;        /* mixed: restart the inner CG at every FP64 refresh */
   double net_s;       /* flexible CG: rms of sqrt(dinv) r (network input scale) */
   double pqs[8];      /* flexible CG: (p, A p) of the kept directions     */
   long fb_iter;       /* flexible CG: iteration of the Jacobi fallback    */
@@ -356,6 +356,14 @@ int ugk_cc_iter(const ugk_mat *K, const ugk_coarse *C, long n, double *d_x, doub
 int ugk_cc_iter_mixed(const ugk_mat *K, const ugk_coarse *C, long n, const double *d_b, double *d_x, double *d_r,
                       double *d_z, double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st,
                       double *d_part, void *stream);
+/* Jacobi PCG on the FP32 copy of K with FP64 vectors, recurrences and dots
+ * (the FP32 values are widened on load): refresh (r = b - K x in FP64 when
+ * the head asked for it), head (test; a convergence is only accepted on a
+ * refreshed residual), p = dinv r + beta p, q = K32 p with (p, q), x and r
+ * update with (r, dinv r).  r and the partials from ugk_fcg_start; the
+ * state's mixed fields set as for flexible CG */
+int ugk_mx64_iter(const ugk_mat *K, long n, const double *d_b, double *d_x, double *d_r, double *d_p, double *d_q,
+                  const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
 /* z_c alone into d_zc (skipped once the solve is done); d_out NULL in
  * ugk_fcg_combine drops the network term, d_zc NULL the coarse one, and
  * ugk_fcg_head with d_x0 NULL writes no network input */

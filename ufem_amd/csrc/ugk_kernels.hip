@@ -1252,7 +1252,9 @@ __global__ __launch_bounds__(WG) void k_mx_p(long n, const float *__restrict__ z
   else if (fresh && nr <= st->tol * nb) done = 1;
   else if (stall >= 5) done = 2;
   else if (k >= st->maxit) done = 3;
-  const double beta = (k == 0) ? 0.0 : t[0] / st->rz[(k + 1) & 1];
+  // classical refinement (st->restart): each refresh starts a new inner CG
+  // on the correction equation, p = z
+  const double beta = (k == 0 || (fresh && st->restart)) ? 0.0 : t[0] / st->rz[(k + 1) & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (k == 0) st->bb = bb;
     if (fresh) {
@@ -1809,6 +1811,7 @@ __global__ __launch_bounds__(WG) void k_fcg_head(long n, const double *__restric
     st->xx = t[2];
     if (k == 0) st->bb = bb;
     if (mixed && fresh) st->rr_true = t[1];
+    st->aux[6] = t[0];   // (r, dinv r) of this iteration, for the Jacobi direction update
     if (req >= 0) st->ru_req = req;
     if (done) st->done = done;
     st->halt = (done || fb) ? 1 : 0;
@@ -2162,9 +2165,36 @@ __global__ __launch_bounds__(WG) void k_ssor_p(long n, const double *__restrict_
     p[i] = k == 0 ? z[i] : z[i] + beta * p[i];
 }
 
+// x += alpha p, r -= alpha q with the partials (r dinv r, r r, x x) of the
+// next test; dinv may be NULL (SSOR, two-level: the first partial stays 0)
+// Jacobi direction from the head's (r, dinv r): beta = ro / ro_prev,
+// p = dinv r + beta p (p = dinv r at the first iteration); z is never stored
+__global__ __launch_bounds__(WG) void k_jac_p(long n, const double *__restrict__ r, const double *__restrict__ dinv,
+                                              double *p, ugk_state *st)
+{
+  if (st->done) return;
+  const long k = st->cur;
+  const double ro = st->aux[6];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!isfinite(ro)) {
+    if (lead) st->done = 2;
+    return;
+  }
+  const double beta = k == 0 ? 0.0 : ro / st->rz[(k + 1) & 1];
+  if (lead) {
+    st->rz[k & 1] = ro;
+    st->rzB = ro;
+    st->beta = beta;
+  }
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) p[i] = k == 0 ? dinv[i] * r[i] : dinv[i] * r[i] + beta * p[i];
+}
+
 __global__ __launch_bounds__(WG) void k_ssor_upd(long n, double *__restrict__ x, double *__restrict__ r,
                                                  const double *__restrict__ p, const double *__restrict__ q,
-                                                 ugk_state *st, double *part, unsigned GA)
+                                                 ugk_state *st, double *part, unsigned GA,
+                                                 const double *__restrict__ dinv)
 {
   if (st->done) return;
   const long k = st->cur;
@@ -2184,6 +2214,7 @@ __global__ __launch_bounds__(WG) void k_ssor_upd(long n, double *__restrict__ x,
     const double ri = r[i] - alpha * q[i];
     x[i] = xi;
     r[i] = ri;
+    if (dinv) v[0] += dinv[i] * ri * ri;
     v[1] += ri * ri;
     v[2] += xi * xi;
   }
@@ -3257,7 +3288,7 @@ int ugk_ssor_iter(const ugk_mat *Kc, int ncol, const long *cb, long n, double *x
   k_ssor_p<<<GV, WG, 0, S(s)>>>(n, z, p, st, part, GV);
   if (Kc->bs == 3) k_pcg_q<3><<<GA, WG, 0, S(s)>>>(M, Kc->d_vals, p, q, st, part);
   else k_pcg_q<1><<<GA, WG, 0, S(s)>>>(M, Kc->d_vals, p, q, st, part);
-  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA);
+  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA, nullptr);
   CK_LAUNCH("k_ssor_upd");
   return 0;
 }
@@ -3407,7 +3438,7 @@ int ugk_cc_iter(const ugk_mat *K, const ugk_coarse *C, long n, double *x, double
   k_cc_z<1><<<GV, WG, 0, S(s)>>>(ccgrid(C), C->d_y, C->d_fm, r, dinv, z, &st->done, part);
   k_ssor_p<<<GV, WG, 0, S(s)>>>(n, z, p, st, part, GV);
   k_pcg_q<3><<<GA, WG, 0, S(s)>>>(M, K->d_vals, p, q, st, part);
-  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA);
+  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA, nullptr);
   CK_LAUNCH("k_ssor_upd");
   return 0;
 }
@@ -3428,7 +3459,27 @@ int ugk_cc_iter_mixed(const ugk_mat *K, const ugk_coarse *C, long n, const doubl
   k_cc_z<1><<<GV, WG, 0, S(s)>>>(ccgrid(C), C->d_y, C->d_fm, r, dinv, z, &st->done, part);
   k_ssor_p<<<GV, WG, 0, S(s)>>>(n, z, p, st, part, GV);
   k_pcg_q<3, float><<<GA, WG, 0, S(s)>>>(M, K->d_vals32, p, q, st, part);
-  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA);
+  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA, nullptr);
+  CK_LAUNCH("k_ssor_upd");
+  return 0;
+}
+
+int ugk_mx64_iter(const ugk_mat *K, long n, const double *b, double *x, double *r, double *p, double *q,
+                  const double *dinv, ugk_state *st, double *part, void *s)
+{
+  const unsigned GV = vec_blocks(n), GA = spmv_blocks(K);
+  const SellView M = view(K);
+  if (!K->d_vals32) {
+    snprintf(g_err, sizeof(g_err), "mixed PCG: no FP32 copy of the matrix");
+    return -1;
+  }
+  if (K->bs == 3) k_fcg_refresh<3><<<GA, WG, 0, S(s)>>>(M, K->d_vals, b, x, r, dinv, st, part);
+  else k_fcg_refresh<1><<<GA, WG, 0, S(s)>>>(M, K->d_vals, b, x, r, dinv, st, part);
+  k_fcg_head<<<GV, WG, 0, S(s)>>>(n, r, dinv, NULL, st, part, GA, GV);
+  k_jac_p<<<GV, WG, 0, S(s)>>>(n, r, dinv, p, st);
+  if (K->bs == 3) k_pcg_q<3, float><<<GA, WG, 0, S(s)>>>(M, K->d_vals32, p, q, st, part);
+  else k_pcg_q<1, float><<<GA, WG, 0, S(s)>>>(M, K->d_vals32, p, q, st, part);
+  k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA, dinv);
   CK_LAUNCH("k_ssor_upd");
   return 0;
 }

@@ -21,7 +21,7 @@ P_F = C.POINTER(C.c_float)
 
 AF_OK, AF_ERR, AF_ERR_VAL, AF_ERR_TYP, AF_ERR_SIZ, AF_ERR_ZER = 0, -1, -3, -5, -9, -13
 SOLVER = {"pcg": 0, "bicgstab": 1, "mixed": 2, "learned": 3, "learned_mixed": 4, "ssor": 5, "twolevel": 6,
-          "twolevel_mixed": 7}
+          "twolevel_mixed": 7, "mixed64": 8}
 EXEC = {"streamed": 0, "persistent": 1, "fused": 2, "classic": 3}
 
 
@@ -48,7 +48,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("solver", C.c_int), ("rule", C.c_int), ("eps", C.c_double), ("maxit", C.c_long),
                 ("exec", C.c_int), ("chunk", C.c_int), ("inner_eps", C.c_double),
                 ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int), ("fcg_m", C.c_int),
-                ("coarse", C.c_int)]
+                ("coarse", C.c_int), ("inner_restart", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -277,9 +277,11 @@ class GpuSystem:
         self.add_disps()
 
     def solve(self, solver="pcg", eps=1e-8, rule=1, maxit=0, exec="streamed", chunk=0,
-              inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, fcg_m=0, coarse=False, check=True):
+              inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, fcg_m=0, coarse=False, check=True,
+              inner_restart=False):
         o = SolveOpts(SOLVER[solver], rule, eps, maxit, EXEC[exec], chunk, inner_eps, inner_maxit,
-                      net.h if net is not None else None, 1 if warm_start else 0, fcg_m, 1 if coarse else 0)
+                      net.h if net is not None else None, 1 if warm_start else 0, fcg_m, 1 if coarse else 0,
+                      1 if inner_restart else 0)
         info = SolveInfo()
         rv = lib().ufem_sys_solve(self.h, C.byref(o), C.byref(info))
         if check and rv not in (AF_OK, AF_ERR):
