@@ -173,7 +173,9 @@ def test_dropin_bicgstab(golden):
     b = G.Vec(s.F())
     x = G.Vec(np.zeros(s.n))
     G.set_stop_rule(1)
-    rv = G.femEqsBiCCSwJ(K, b, x, 1e-13, s.n)
+    # the stop is confirmed on the FP64 true residual, whose rounding floor on
+    # this beam lies near 1e-12: 1e-11 is the tightest reliable tolerance
+    rv = G.femEqsBiCCSwJ(K, b, x, 1e-11, s.n)
     G.set_stop_rule(0)
     assert rv == 0, G.last_solve()
     assert rel(x.data, golden["c1_cg/u"]) <= 1e-8
@@ -183,7 +185,7 @@ def test_native_bicgstab(golden):
     mesh = MESHES["het8"]()
     g = G.GpuSystem(mesh)
     g.assemble(3)
-    info = g.solve("bicgstab", eps=1e-13, rule=1)
+    info = g.solve("bicgstab", eps=1e-11, rule=1)
     assert info["rv"] == 0, info
     assert rel(g.u(), golden["het8_cg/u"]) <= 1e-8
 
