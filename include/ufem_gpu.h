@@ -129,6 +129,17 @@ int  ufem_sys_fill_K(ufem_sys *s, int npts);   /* npts = 2 (exact for parallelep
 int  ufem_sys_add_loads(ufem_sys *s);
 int  ufem_sys_add_disps(ufem_sys *s);
 int  ufem_sys_solve(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * the solve of ufem_sys_solve for o->solver = UFEM_SOLVER_PCG, exec
+ * UFEM_EXEC_STREAMED, calling no hipMalloc, hipFree, pinned-memory, event or
+ * graph routine.  Its vectors, partial sums and scalars live in one
+ * file-scope device array per GPU (about 52 MiB, held until the process
+ * ends) that bounds the system to 1,048,576 unknowns (AF_ERR_BIG beyond;
+ * AF_ERR_TYP for another solver or form).  All calls on one GPU share that
+ * array: calls that overlap on one GPU race.  The result equals
+ * ufem_sys_solve's; solve_ms is host wall time. */
+int  ufem_sys_solve_noalloc(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info);
 /* reactions, one per support entry in input order (node position, direction,
  * value); supports of non-zero prescribed value are included here, though the
  * reference's own list leaves them out (fem/fem_sol.c:676-722) */

@@ -18,7 +18,7 @@ INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libufem_gpu.so")
 
-HIP_SOURCES = ["ugk_kernels.hip", "ugk_unet.hip"]
+HIP_SOURCES = ["ugk_kernels.hip", "ugk_unet.hip", "ugk_noalloc.hip"]
 C_SOURCES = ["ufem_sys.c", "ufem_solve.c", "ufem_ssor.c", "ufem_coarse.c", "fem_dropin.c", "fem_fillk.c", "ufem_batch.c", "ufem_net.c",
              "ufem_input.c", "ufem_monte.c"]
 

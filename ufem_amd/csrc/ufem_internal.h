@@ -94,6 +94,10 @@ int ufem_solve_device(const ugk_mat *K, long n, const double *d_b, double *d_x, 
                       void *stream, void *ev0, void *ev1, const ufem_solve_opts *o,
                       ufem_solve_info *info);
 
+/* Jacobi PCG with its workspace in the fixed device array (allocates nothing) */
+int ufem_solve_device_noalloc(const ugk_mat *K, long n, const double *d_b, double *d_x, void *stream,
+                              const ufem_solve_opts *o, ufem_solve_info *info);
+
 /* learned preconditioner inside flexible CG (ufem_net.c) */
 long ufem_net_bound_n(const ufem_net *N);           /* DOFs of the bound system, 0 if none */
 unsigned short *ufem_net_input(ufem_net *N);       /* split format [V][3][4], channel 3 stays 0 */

@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "ufem_fem_math.h"
 #include "ufem_internal.h"
@@ -634,6 +635,81 @@ static int solve_mixed64(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   info->outer = h->n_ru;
   info->relres = sqrt(h->rr / h->bb);
   info->status = status_of(h->done);
+  return info->status;
+}
+
+/* Jacobi PCG (the streamed single-reduction form, kernels launched one by
+ * one, no graph) with every vector, partial sum and scalar it needs beyond
+ * b and x in the fixed device workspace of ugk_noalloc_workspace: the call
+ * allocates nothing -- no device or pinned memory, no events, no graph.
+ * The state is read back with a plain copy after every chunk of iterations
+ * and the time is taken on the host clock.  Calls on one GPU share its
+ * workspace: they must not overlap. */
+int ufem_solve_device_noalloc(const ugk_mat *K, long n, const double *b, double *x, void *stream,
+                              const ufem_solve_opts *o, ufem_solve_info *info)
+{
+  static ugk_state hbuf;   /* host copy of the workspace's state (same race rule) */
+  const long N = UGK_NOALLOC_MAX_N;
+  const int chunk = o->chunk > 0 ? o->chunk : 32;
+  long maxit = o->maxit > 0 ? o->maxit : (o->maxit == 0 ? n : 0), it = 0, k;
+  void *base = NULL;
+  size_t bytes = 0;
+  double *d, t0, bb;
+  struct timespec ts;
+  ufem_work w;
+  int rv;
+  if (o->solver != UFEM_SOLVER_PCG || o->exec != UFEM_EXEC_STREAMED) {
+    ufem_set_error("no-alloc solve: Jacobi PCG in the streamed form only");
+    return AF_ERR_TYP;
+  }
+  if (n > N) {
+    ufem_set_error("no-alloc solve: %ld unknowns, the fixed workspace holds at most %ld", n, N);
+    return AF_ERR_BIG;
+  }
+  if (sizeof(ugk_state) > sizeof(double) * UGK_NOALLOC_STATE_DOUBLES) {
+    ufem_set_error("no-alloc solve: solver state larger than its slot");
+    return AF_ERR;
+  }
+  if (ugk_noalloc_workspace(&base, &bytes)) return gpu_fail("no-alloc workspace");
+  d = (double *)base;
+  memset(&w, 0, sizeof(w));
+  w.n = n;
+  w.r = d;
+  w.z = d + N;
+  w.p[0] = d + 2 * N;
+  w.p[1] = d + 3 * N;
+  w.q = d + 4 * N;
+  w.dinv = d + 5 * N;
+  w.part = d + 6 * N;
+  w.st = (ugk_state *)(d + 6 * N + UGK_PART_DOUBLES);
+  w.st_host = &hbuf;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  t0 = 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+  memset(info, 0, sizeof(*info));
+  rv = prepare(K, &w, stream, o->eps, o->rule, maxit);
+  if (rv != AF_OK) return rv;
+  if (ugk_cg1_start(K, n, b, x, w.r, w.z, w.p[0], w.p[1], w.dinv, w.part, stream)) return gpu_fail("start");
+  for (;;) {
+    for (k = 0; k < chunk; k++)
+      if (ugk_cg1_iter(K, n, x, w.r, w.z, w.p[0], w.p[1], w.q, w.dinv, w.st, w.part, stream))
+        return gpu_fail("iteration");
+    it += chunk;
+    if (get_state(&w, stream)) return AF_ERR;
+    if (hbuf.done || it > maxit + 2L * chunk) break;
+  }
+  if (hbuf.bb <= 0.0) {   /* zero load: x = 0 */
+    if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
+    info->status = AF_OK;
+  } else {
+    info->iters = hbuf.iter;
+    info->relres = sqrt(hbuf.rr / hbuf.bb);
+    info->status = status_of(hbuf.done);
+  }
+  if (ugk_dot(n, b, b, &w.st->bb, w.part, stream) || get_state(&w, stream)) return gpu_fail("norm b");
+  bb = hbuf.bb;
+  if (true_residual(K, n, b, x, &w, stream, bb, &info->true_relres)) return AF_ERR;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  info->solve_ms = 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec - t0;
   return info->status;
 }
 

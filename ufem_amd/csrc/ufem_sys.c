@@ -581,6 +581,19 @@ int ufem_sys_solve(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info)
   return rv;
 }
 
+int ufem_sys_solve_noalloc(ufem_sys *s, const ufem_solve_opts *o, ufem_solve_info *info)
+{
+  ufem_solve_info tmp;
+  int rv;
+  if (!info) info = &tmp;
+  memset(info, 0, sizeof(*info));
+  if (!s->bc_applied) { ufem_set_error("supports not applied"); return AF_ERR; }
+  if (!o->warm_start && ugk_zero(s->d_u, sizeof(double) * (size_t)s->n, s->stream)) return gpu_fail("u reset");
+  rv = ufem_solve_device_noalloc(&s->K, s->n, s->d_F, s->d_u, s->stream, o, info);
+  s->t_solve_ms = info->solve_ms;
+  return rv;
+}
+
 /* fem_fill_K(AF_YES) + fem_fill_K_React (fem/fem_sol.c:844-871): the
  * reaction at a support is (K u) there minus the load that stood there,
  * with K the matrix before supports */

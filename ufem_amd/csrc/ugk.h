@@ -84,6 +84,15 @@ typedef struct {
   unsigned int counter[8];  /* last-block tickets, reset by the last block */
 } ugk_state;
 
+/* ---------------- fixed workspace for callers that cannot allocate -------- */
+/* the device address (for the current device) and size of a file-scope
+ * workspace of 6 vectors of UGK_NOALLOC_MAX_N doubles, UGK_PART_DOUBLES
+ * reduction partials and UGK_NOALLOC_STATE_DOUBLES for a ugk_state, in that
+ * order; taken with hipGetSymbolAddress once per device */
+#define UGK_NOALLOC_MAX_N (1L << 20)
+#define UGK_NOALLOC_STATE_DOUBLES 128
+int ugk_noalloc_workspace(void **ptr, size_t *bytes);
+
 /* ---------------- memory / stream / timing ------------------------------ */
 int   ugk_init(void);
 const char *ugk_error(void);

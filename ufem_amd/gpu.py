@@ -100,6 +100,7 @@ def lib():
             "ufem_sys_add_loads": (C.c_int, [C.c_void_p]),
             "ufem_sys_add_disps": (C.c_int, [C.c_void_p]),
             "ufem_sys_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+            "ufem_sys_solve_noalloc": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
             "ufem_sys_reactions": (C.c_int, [C.c_void_p, P_D, P_L, P_L]),
             "ufem_sys_element_results": (C.c_int, [C.c_void_p, C.c_int, P_D]),
             "ufem_sys_write_results": (C.c_int, [C.c_void_p, C.c_char_p, C.c_long, C.c_long, C.c_double]),
