@@ -22,6 +22,7 @@ typedef struct ufem_ssor {
   long src_slots;
   ugk_mat Kc;
   int *d_perm;              /* new block row -> old block row */
+  int *d_dpos;              /* slot of each row's diagonal block in Kc */
   int ncol;
   long *cb;                 /* host, ncol + 1: first row of each colour */
 } ufem_ssor;

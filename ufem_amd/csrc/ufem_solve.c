@@ -458,12 +458,17 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
 }
 
 /* CG with the SSOR preconditioner (femEqsCGwSSOR, fem/fem_eqs.c:999-1188)
- * in colour order: b, x and 1/diag moved into colour order, the iteration of
- * ugk_ssor_iter captured in the chunk graph, x moved back.  Uses the
- * BiCGStab buffers of the workspace: v = zz, s = b, t = x, shat = 1/diag. */
+ * in colour order and in Eisenstat's form: b, x moved into colour order,
+ * CG on the split-preconditioned system (one backward and one forward sweep
+ * per iteration, no product with K), the original residual formed and
+ * tested every SSOR_CHECK iterations, x recovered and moved back.  Uses the
+ * workspace's BiCGStab vectors: s = b, t = x, rhat = x^, v = s, z = t-sweep,
+ * shat = sqrt(diag), phat = the x-sweep. */
+#define SSOR_CHECK 4
 typedef struct {
   ufem_ssor *S;
   long n;
+  int rule;
   ufem_work *w;
 } ssor_ctx;
 
@@ -471,9 +476,8 @@ static int ssor_step(void *c, long it, void *stream)
 {
   ssor_ctx *q = (ssor_ctx *)c;
   ufem_work *w = q->w;
-  (void)it;
-  return ugk_ssor_iter(&q->S->Kc, q->S->ncol, q->S->cb, q->n, w->t, w->r, w->v, w->z, w->p[0], w->q, w->shat,
-                       w->st, w->part, stream);
+  return ugk_eis_iter(&q->S->Kc, q->S->d_dpos, q->S->ncol, q->S->cb, q->n, w->rhat, w->r, w->p[0], w->q, w->z,
+                      w->v, w->shat, w->t, w->phat, (it + 1) % SSOR_CHECK == 0, q->rule, w->st, w->part, stream);
 }
 
 static int solve_ssor(const ugk_mat *K, long n, const double *b, double *x, ufem_work *w, void *stream,
@@ -482,28 +486,36 @@ static int solve_ssor(const ugk_mat *K, long n, const double *b, double *x, ufem
   ssor_ctx c;
   ufem_ssor *S;
   ugk_state *h = w->st_host;
-  int rv;
-  if (!w->v || !w->s || !w->t || !w->shat) {
+  int rv, chunk = o->chunk > 0 ? o->chunk : 8;
+  if (!w->v || !w->s || !w->t || !w->shat || !w->rhat || !w->phat) {
     ufem_set_error("SSOR: workspace without the BiCGStab vectors");
     return AF_ERR_VAL;
   }
+  chunk = (chunk + SSOR_CHECK - 1) / SSOR_CHECK * SSOR_CHECK;   /* the check pattern repeats per chunk */
   rv = ufem_ssor_prepare(K, &w->ssor, stream);
   if (rv != AF_OK) return rv;
   S = w->ssor;
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
   if (ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, b, w->s, 0, stream) ||
-      ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, x, w->t, 0, stream) ||
-      ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, w->dinv, w->shat, 0, stream))
+      ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, x, w->t, 0, stream))
     return gpu_fail("SSOR: colour order");
-  if (ugk_fcg_start(&S->Kc, n, w->s, w->t, w->r, w->shat, w->part, stream)) return gpu_fail("SSOR: start");
+  if (ugk_fcg_start(&S->Kc, n, w->s, w->t, w->r, w->dinv, w->part, stream) ||
+      ugk_dot(n, w->s, w->s, &w->st->bb, w->part, stream) ||
+      ugk_eis_start(&S->Kc, S->d_dpos, S->ncol, S->cb, n, w->r, w->r, w->p[0], w->rhat, w->v, w->shat, w->st,
+                    w->part, stream))
+    return gpu_fail("SSOR: start");
   c.S = S;
   c.n = n;
+  c.rule = o->rule;
   c.w = w;
-  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, ssor_step, &c, 1);
+  rv = run_chunks(w, stream, maxit, chunk, ssor_step, &c, 1);
   if (rv != AF_OK) return rv;
-  if (ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, w->t, x, 1, stream)) return gpu_fail("SSOR: order back");
+  if (ugk_eis_finish(&S->Kc, S->d_dpos, S->ncol, S->cb, w->rhat, w->t, w->phat, w->st, w->part, stream) ||
+      ugk_ssor_permute_vec(K->n_brows, K->bs, S->d_perm, w->t, x, 1, stream))
+    return gpu_fail("SSOR: order back");
   if (get_state(w, stream)) return AF_ERR;
+  if (h->err == 4) { ufem_set_error("SSOR: a diagonal entry is not positive"); return AF_ERR_VAL; }
   if (h->bb <= 0.0) {
     if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
     info->status = AF_OK;

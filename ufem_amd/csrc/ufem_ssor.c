@@ -24,7 +24,7 @@ void ufem_ssor_free(ufem_ssor *S)
 {
   if (!S) return;
   ugk_free(S->Kc.d_slen); ugk_free(S->Kc.d_soff); ugk_free(S->Kc.d_rowlen); ugk_free(S->Kc.d_cols);
-  ugk_free(S->Kc.d_vals); ugk_free(S->d_perm);
+  ugk_free(S->Kc.d_vals); ugk_free(S->d_perm); ugk_free(S->d_dpos);
   free(S->cb);
   free(S);
 }
@@ -39,7 +39,7 @@ static int build(const ugk_mat *K, ufem_ssor *S, void *stream)
 {
   const long nb = K->n_brows, nsl = K->n_slices, tot = K->total_slots;
   int *slen = NULL, *rowlen = NULL, *cols = NULL, *color = NULL, *perm = NULL, *inv = NULL, *ncols = NULL;
-  int *nslen = NULL, *nrowlen = NULL, *row = NULL, maxlen = 1;
+  int *nslen = NULL, *nrowlen = NULL, *row = NULL, *dpos = NULL, maxlen = 1;
   long *soff = NULL, *nsoff = NULL, *cnt = NULL, i, ntot = 0;
   int rv = AF_ERR_MEM, ncol = 0;
   unsigned char used[SSOR_MAX_COLORS];
@@ -106,7 +106,8 @@ static int build(const ugk_mat *K, ufem_ssor *S, void *stream)
     if (nrowlen[i] > maxlen) maxlen = nrowlen[i];
   ncols = (int *)malloc(sizeof(int) * (size_t)ntot * 64);
   row = (int *)malloc(sizeof(int) * (size_t)maxlen);
-  if (!ncols || !row) goto out;
+  dpos = (int *)malloc(sizeof(int) * (size_t)nb);
+  if (!ncols || !row || !dpos) goto out;
   for (i = 0; i < nsl; i++) {
     int l;
     for (l = 0; l < 64; l++) {
@@ -116,6 +117,14 @@ static int build(const ugk_mat *K, ufem_ssor *S, void *stream)
         const long o = perm[I], so = soff[o >> 6], lo = o & 63;
         for (k = 0; k < nrowlen[I]; k++) row[k] = inv[cols[(so + k) * 64 + lo]];
         qsort(row, (size_t)nrowlen[I], sizeof(int), cmp_int);
+        dpos[I] = -1;            /* the diagonal block splits the row: earlier colours, later ones */
+        for (k = 0; k < nrowlen[I]; k++)
+          if (row[k] == (int)I) dpos[I] = k;
+        if (dpos[I] < 0) {
+          ufem_set_error("SSOR: row %ld has no diagonal block", I);
+          rv = AF_ERR_VAL;
+          goto out;
+        }
         for (k = 0; k < nslen[i]; k++) ncols[(nsoff[i] + k) * 64 + l] = k < nrowlen[I] ? row[k] : (int)I;
       } else {
         for (k = 0; k < nslen[i]; k++) ncols[(nsoff[i] + k) * 64 + l] = 0;
@@ -135,7 +144,9 @@ static int build(const ugk_mat *K, ufem_ssor *S, void *stream)
   S->Kc.d_cols = (int *)ugk_malloc(sizeof(int) * (size_t)ntot * 64);
   S->Kc.d_vals = (double *)ugk_malloc(sizeof(double) * (size_t)ntot * 64 * K->bs * K->bs);
   S->d_perm = (int *)ugk_malloc(sizeof(int) * (size_t)nb);
-  if (!S->Kc.d_slen || !S->Kc.d_soff || !S->Kc.d_rowlen || !S->Kc.d_cols || !S->Kc.d_vals || !S->d_perm) {
+  S->d_dpos = (int *)ugk_malloc(sizeof(int) * (size_t)nb);
+  if (!S->Kc.d_slen || !S->Kc.d_soff || !S->Kc.d_rowlen || !S->Kc.d_cols || !S->Kc.d_vals || !S->d_perm ||
+      !S->d_dpos) {
     ufem_set_error("SSOR: allocation: %s", ugk_error());
     goto out;
   }
@@ -143,7 +154,8 @@ static int build(const ugk_mat *K, ufem_ssor *S, void *stream)
       ugk_h2d(S->Kc.d_soff, nsoff, sizeof(long) * (size_t)nsl, stream) ||
       ugk_h2d(S->Kc.d_rowlen, nrowlen, sizeof(int) * (size_t)nb, stream) ||
       ugk_h2d(S->Kc.d_cols, ncols, sizeof(int) * (size_t)ntot * 64, stream) ||
-      ugk_h2d(S->d_perm, perm, sizeof(int) * (size_t)nb, stream) || ugk_sync(stream)) {
+      ugk_h2d(S->d_perm, perm, sizeof(int) * (size_t)nb, stream) ||
+      ugk_h2d(S->d_dpos, dpos, sizeof(int) * (size_t)nb, stream) || ugk_sync(stream)) {
     ufem_set_error("SSOR: upload: %s", ugk_error());
     rv = AF_ERR;
     goto out;
@@ -152,7 +164,7 @@ static int build(const ugk_mat *K, ufem_ssor *S, void *stream)
   rv = AF_OK;
 out:
   free(slen); free(soff); free(rowlen); free(cols); free(color); free(perm); free(inv); free(cnt);
-  free(nslen); free(nsoff); free(nrowlen); free(ncols); free(row);
+  free(nslen); free(nsoff); free(nrowlen); free(ncols); free(row); free(dpos);
   return rv;
 }
 

@@ -316,6 +316,26 @@ int ugk_ssor_iter(const ugk_mat *Kc, int ncol, const long *cb, long n, double *d
                   double *d_z, double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st, double *d_part,
                   void *stream);
 
+/* The same solve in Eisenstat's form (see ugk_kernels.hip): CG on
+ * D^1/2 W^-1 Kc W^-T D^1/2 (W = D + L in colour order) for the correction
+ * x^ of x = x0 + W^-T D^1/2 x^, one backward and one forward sweep per
+ * iteration over the slots above / below each row's diagonal (dpos).
+ * start: sqrt of the diagonal into dsq, r^ = p^ = D^1/2 W^-1 r0 from
+ * r0 = b - Kc x0 (r0 may be r^ itself), x^ = 0.  iter: one iteration; with
+ * check, the original residual r = W D^-1/2 r^ (and for rule 0 the current
+ * x into yx's sweep) is tested.  finish: x = x0 + W^-T D^1/2 x^ in place of
+ * x0.  Vectors in colour order; sv, t, qh and yx are work vectors. */
+int ugk_eis_check_size(const ugk_mat *Kc, int ncol);
+int ugk_eis_start(const ugk_mat *Kc, const int *d_dpos, int ncol, const long *cb, long n, const double *d_r0,
+                  double *d_rh, double *d_ph, double *d_xh, double *d_sv, double *d_dsq, ugk_state *d_st,
+                  double *d_part, void *stream);
+int ugk_eis_iter(const ugk_mat *Kc, const int *d_dpos, int ncol, const long *cb, long n, double *d_xh,
+                 double *d_rh, double *d_ph, double *d_qh, double *d_t, double *d_sv, const double *d_dsq,
+                 const double *d_x0, double *d_yx, int check, int rule, ugk_state *d_st, double *d_part,
+                 void *stream);
+int ugk_eis_finish(const ugk_mat *Kc, const int *d_dpos, int ncol, const long *cb, const double *d_xh,
+                   double *d_x, double *d_yx, ugk_state *d_st, double *d_part, void *stream);
+
 /* ---------------- two-level (coarse-grid) correction ---------------------- */
 /* On a box node grid W x H x D (node x + W (y + H z), element
  * ex + (W-1) (ey + (H-1) ez)), coarsened by cx, cy, cz into a grid of

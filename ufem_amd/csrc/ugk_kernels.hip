@@ -2225,6 +2225,273 @@ __global__ __launch_bounds__(WG) void k_ssor_upd(long n, double *__restrict__ x,
   }
 }
 
+// --------------------------------------------------------------------------
+// CG with the SSOR preconditioner by Eisenstat's trick (colour order, scalar
+// D, L, U of Kc = L + D + U, W = D + L, M = W D^-1 W^T).  CG runs on
+// A^ = D^1/2 W^-1 A W^-T D^1/2 for the correction x^ of x = x0 + W^-T D^1/2 x^:
+// since A = W + W^T - D, A^ p = D^1/2 (t + s) with W^T t = D^1/2 p and
+// W s = D^1/2 p - D t, one backward and one forward triangular sweep per
+// iteration and no separate product with A.  A sweep is one launch per
+// colour over its contiguous rows; a row reads only the slots below its
+// diagonal slot (forward, earlier colours) or above it (backward, later
+// colours) -- dpos[I] -- so the two sweeps together read Kc once.  One
+// workgroup per slice: its 4 waves take a quarter of each row's slots and
+// meet in LDS; the 3x3 (or 1x1) diagonal block is solved triangularly per
+// row.  The iterates are those of SSOR-PCG; the residual of the original
+// system, r = W D^-1/2 r^, is formed and tested every few iterations.
+//
+// Partials: sweeps -> PART_S (one per block, launches back to back, pbase
+// offsets); update(k) -> PART_B2((k+1)&1) (r^, r^); residual -> PART_R.
+// --------------------------------------------------------------------------
+enum { EIS_BWD_P = 0, EIS_FWD_Q = 1, EIS_FWD_R0 = 2, EIS_BWD_X = 3 };
+
+template <int BS, int MODE>
+__global__ __launch_bounds__(WG) void k_eis_sweep(SellView M, const double *__restrict__ vals,
+                                                  const int *__restrict__ dpos, long r0, long r1, long sl0,
+                                                  const double *__restrict__ in1, const double *__restrict__ in2,
+                                                  double *y, double *out1, double *out2, ugk_state *st,
+                                                  double *part, unsigned GB, int gate)
+{
+  constexpr bool FWD = (MODE == EIS_FWD_Q || MODE == EIS_FWD_R0);
+  if (gate && st->done) return;            // the same for every block
+  __shared__ double sh[NW - 1][BS][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long sl = sl0 + blockIdx.x;
+  const long I = sl * 64 + lane;
+  const bool act = I >= r0 && I < r1;
+  // backward sweep of an iteration after the first: the direction update
+  // p^ = r^ + beta p^ of this row first (beta from the update's partials)
+  double beta = 0.0;
+  if (MODE == EIS_BWD_P && st->iter > 0) {
+    double t[1];
+    reduce_prev<1>(PART_B2(part, st->iter & 1), GB, t);
+    beta = t[0] / st->rzB;
+  }
+  double acc[BS];
+#pragma unroll
+  for (int r = 0; r < BS; r++) acc[r] = 0.0;
+  const long so = M.soff[sl];
+  int dp = 0, L = 0;
+  if (act) {
+    dp = dpos[I];
+    L = M.rowlen[I];
+  }
+  const int lo = FWD ? 0 : dp + 1, hi = FWD ? dp : L, cnt = hi - lo;
+  const int q0 = lo + (cnt * w) / NW, q1 = lo + (cnt * (w + 1)) / NW;
+  const int *cp = M.cols + so * 64 + lane;
+  for (int k = q0; k < q1; k++) {
+    const long J = ldnt(cp + (long)k * 64);
+    double yv[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) yv[c] = y[J * BS + c];
+    double v[BS * BS];
+    ld_block<BS, true>(vals, so + k, lane, v);
+#pragma unroll
+    for (int r = 0; r < BS; r++)
+#pragma unroll
+      for (int c = 0; c < BS; c++) acc[r] += v[r * BS + c] * yv[c];
+  }
+  if (w > 0) {
+#pragma unroll
+    for (int r = 0; r < BS; r++) sh[w - 1][r][lane] = acc[r];
+  }
+  __syncthreads();
+  if (w > 0) return;                       // no barrier below
+  double pv = 0.0;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < NW - 1; k++)
+#pragma unroll
+      for (int r = 0; r < BS; r++) acc[r] += sh[k][r][lane];
+    double B[BS * BS];
+    ld_block<BS, false>(vals, so + dp, lane, B);
+    double dq[BS], rhs[BS], yl[BS];
+#pragma unroll
+    for (int d = 0; d < BS; d++) dq[d] = sqrt(B[d * BS + d]);
+#pragma unroll
+    for (int d = 0; d < BS; d++) {
+      const long i = I * BS + d;
+      if (MODE == EIS_BWD_P) {
+        double ph = out1[i];
+        if (st->iter > 0) {
+          ph = in1[i] + beta * ph;          // in1 = r^, out1 = p^
+          out1[i] = ph;
+        }
+        rhs[d] = dq[d] * ph;
+      } else if (MODE == EIS_FWD_Q) {
+        rhs[d] = dq[d] * in1[i] - B[d * BS + d] * in2[i];   // in1 = p^, in2 = t
+      } else if (MODE == EIS_FWD_R0) {
+        rhs[d] = in1[i];                    // r0 = b - A x0
+      } else {
+        rhs[d] = dq[d] * in1[i];            // in1 = x^
+      }
+    }
+    if (FWD) {
+#pragma unroll
+      for (int d = 0; d < BS; d++) {
+        double t = acc[d];
+#pragma unroll
+        for (int e = 0; e < d; e++) t += B[d * BS + e] * yl[e];
+        yl[d] = (rhs[d] - t) / B[d * BS + d];
+      }
+    } else {
+#pragma unroll
+      for (int d = BS - 1; d >= 0; d--) {
+        double t = acc[d];
+#pragma unroll
+        for (int e = d + 1; e < BS; e++) t += B[d * BS + e] * yl[e];
+        yl[d] = (rhs[d] - t) / B[d * BS + d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < BS; d++) {
+      const long i = I * BS + d;
+      y[i] = yl[d];
+      if (MODE == EIS_FWD_Q) {
+        const double qh = dq[d] * (in2[i] + yl[d]);
+        out1[i] = qh;
+        pv += in1[i] * qh;                  // (p^, q^)
+      } else if (MODE == EIS_FWD_R0) {
+        const double rh = dq[d] * yl[d];
+        out1[i] = rh;
+        out2[i] = rh;
+        pv += rh * rh;                      // (r^, r^)
+      } else if (MODE == EIS_BWD_X) {
+        const double xi = in2[i] + yl[d];   // in2 = x0
+        if (out1) out1[i] = xi;
+        pv += xi * xi;                      // (x, x)
+      }
+    }
+  }
+  if (MODE != EIS_BWD_P) {
+    pv = wave_sum(pv);
+    if (lane == 0) part[blockIdx.x] = pv;
+  }
+}
+
+// one block: the sum of G partials into dst[0] (a single partial for the next reader)
+__global__ __launch_bounds__(WG) void k_eis_sum(const double *__restrict__ src, unsigned G, double *dst)
+{
+  double t[1];
+  reduce_prev<1>(src, G, t);
+  if (threadIdx.x == 0) dst[0] = t[0];
+}
+
+// alpha = (r^, r^) / (p^, q^); x^ += alpha p^, r^ -= alpha q^; (r^, r^) partials
+__global__ __launch_bounds__(WG) void k_eis_upd(long n, double *__restrict__ xh, double *__restrict__ rh,
+                                                const double *__restrict__ ph, const double *__restrict__ qh,
+                                                ugk_state *st, double *part, unsigned GS, unsigned GB,
+                                                int check)
+{
+  if (st->done) return;
+  const long k = st->iter;
+  double t[1], u[1];
+  reduce_prev<1>(PART_S(part), GS, t);
+  reduce_prev<1>(PART_B2(part, k & 1), k == 0 ? 1u : GB, u);
+  const double rho = u[0], alpha = rho / t[0];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(t[0] > 0.0) || !isfinite(alpha)) {
+    if (lead) st->done = 2;
+    return;
+  }
+  double v[1] = {0.0};
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) {
+    xh[i] += alpha * ph[i];
+    const double ri = rh[i] - alpha * qh[i];
+    rh[i] = ri;
+    v[0] += ri * ri;
+  }
+  block_partials<1>(v, PART_B2(part, (k + 1) & 1));
+  if (lead) {
+    st->rzB = rho;
+    st->alpha = alpha;
+    st->iter = k + 1;
+    if (!check && k + 1 >= st->maxit) st->done = 3;
+  }
+}
+
+// r = W D^-1/2 r^ (the original residual, colour order) and its partials:
+// r_I = sum of the slots below the diagonal A_IJ v_J + the diagonal block's
+// lower triangle with its diagonal times v_I, v = r^ / sqrt(diag)
+template <int BS>
+__global__ __launch_bounds__(WG) void k_eis_resid(SellView M, const double *__restrict__ vals,
+                                                  const int *__restrict__ dpos, const double *__restrict__ rh,
+                                                  const double *__restrict__ dsq, ugk_state *st, double *part)
+{
+  if (st->done) return;
+  double v[1] = {0.0};
+  const int lane = threadIdx.x & 63;
+  for (long sl = (long)blockIdx.x * NW + (threadIdx.x >> 6); sl < M.n_slices; sl += (long)gridDim.x * NW) {
+    const long I = sl * 64 + lane;
+    if (I >= M.n_brows) continue;
+    const long so = M.soff[sl];
+    const int dp = dpos[I];
+    double acc[BS];
+#pragma unroll
+    for (int r = 0; r < BS; r++) acc[r] = 0.0;
+    for (int k = 0; k < dp; k++) {
+      const long J = M.cols[(so + k) * 64 + lane];
+      double yv[BS];
+#pragma unroll
+      for (int c = 0; c < BS; c++) yv[c] = rh[J * BS + c] / dsq[J * BS + c];
+      double a[BS * BS];
+      ld_block<BS, true>(vals, so + k, lane, a);
+#pragma unroll
+      for (int r = 0; r < BS; r++)
+#pragma unroll
+        for (int c = 0; c < BS; c++) acc[r] += a[r * BS + c] * yv[c];
+    }
+    double B[BS * BS], yo[BS];
+    ld_block<BS, false>(vals, so + dp, lane, B);
+#pragma unroll
+    for (int c = 0; c < BS; c++) yo[c] = rh[I * BS + c] / dsq[I * BS + c];
+#pragma unroll
+    for (int r = 0; r < BS; r++) {
+      double t = acc[r];
+#pragma unroll
+      for (int c = 0; c <= r; c++) t += B[r * BS + c] * yo[c];
+      v[0] += t * t;
+    }
+  }
+  block_partials<1>(v, PART_R(part));
+}
+
+// sqrt of the scalar diagonal, colour order
+template <int BS>
+__global__ void k_eis_dsq(SellView M, const double *__restrict__ vals, const int *__restrict__ dpos,
+                          double *dsq, ugk_state *st)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M.n_brows * BS) return;
+  const long I = i / BS, d = i % BS;
+  const double a = vals[vpos<BS>(M.soff[I >> 6] + dpos[I], (int)(d * BS + d), (int)(I & 63))];
+  if (!(a > 0.0)) {
+    st->err = 4;
+    dsq[i] = 1.0;
+    return;
+  }
+  dsq[i] = sqrt(a);
+}
+
+// convergence test on the residual's partials (and, for the reference rule,
+// the (x, x) partials of the sweep that formed x)
+__global__ __launch_bounds__(WG) void k_eis_test(ugk_state *st, const double *part, unsigned GR, unsigned GX)
+{
+  if (st->done) return;
+  double t[1], u[1] = {0.0};
+  reduce_prev<1>(PART_R(part), GR, t);
+  if (GX) reduce_prev<1>(PART_S(part), GX, u);
+  const long k = st->iter;
+  const int done = pcg_test(st, k, t[0], u[0], st->bb, t[0]);
+  if (threadIdx.x == 0) {
+    st->rr = t[0];
+    st->xx = u[0];
+    if (done) st->done = done;
+  }
+}
+
 // values of the colour-permuted matrix Kc from K: new block row I' is old
 // row perm[I'], its column J' old column perm[J'] (found by binary search)
 template <int BS>
@@ -3258,6 +3525,105 @@ int ugk_ssor_permute_vec(long n_brows, int bs, const int *perm, const double *sr
 {
   k_ssor_permute_vec<<<vec_blocks(n_brows * bs), WG, 0, S(s)>>>(n_brows, bs, perm, src, dst, dir);
   CK_LAUNCH("k_ssor_permute_vec");
+  return 0;
+}
+
+// ---- CG with SSOR, Eisenstat's form --------------------------------------
+}  // extern "C"
+
+static inline unsigned eis_blocks(long r0, long r1)
+{
+  return (unsigned)((r1 - 1) / 64 - r0 / 64 + 1);
+}
+
+template <int BS, int MODE>
+static unsigned eis_sweeps(const ugk_mat *Kc, const int *dpos, int ncol, const long *cb, const double *in1,
+                           const double *in2, double *y, double *out1, double *out2, ugk_state *st,
+                           double *part, unsigned GB, int gate, hipStream_t s)
+{
+  const SellView M = view(Kc);
+  unsigned pbase = 0;
+  for (int k = 0; k < ncol; k++) {
+    const int c = (MODE == EIS_BWD_P || MODE == EIS_BWD_X) ? ncol - 1 - k : k;
+    const long r0 = cb[c], r1 = cb[c + 1];
+    if (r1 <= r0) continue;
+    const unsigned g = eis_blocks(r0, r1);
+    k_eis_sweep<BS, MODE><<<g, WG, 0, s>>>(M, Kc->d_vals, dpos, r0, r1, r0 / 64, in1, in2, y, out1, out2, st,
+                                           MODE == EIS_BWD_P ? part : PART_S(part) + pbase, GB, gate);
+    pbase += g;
+  }
+  return pbase;
+}
+
+extern "C" {
+
+int ugk_eis_check_size(const ugk_mat *Kc, int ncol)
+{
+  if (Kc->n_slices + ncol > 4L * 65536) {
+    snprintf(g_err, sizeof(g_err), "SSOR: %ld slices exceed the partial-sum buffer", Kc->n_slices);
+    return -1;
+  }
+  return 0;
+}
+
+int ugk_eis_start(const ugk_mat *Kc, const int *dpos, int ncol, const long *cb, long n, const double *r0,
+                  double *rh, double *ph, double *xh, double *sv, double *dsq, ugk_state *st, double *part,
+                  void *s)
+{
+  const long nb = Kc->n_brows * Kc->bs;
+  if (ugk_eis_check_size(Kc, ncol)) return -1;
+  if (Kc->bs == 3) k_eis_dsq<3><<<(unsigned)((nb + WG - 1) / WG), WG, 0, S(s)>>>(view(Kc), Kc->d_vals, dpos, dsq, st);
+  else k_eis_dsq<1><<<(unsigned)((nb + WG - 1) / WG), WG, 0, S(s)>>>(view(Kc), Kc->d_vals, dpos, dsq, st);
+  k_fill_zero<<<vec_blocks(2 * n), WG, 0, S(s)>>>(2 * n, (unsigned int *)xh);
+  unsigned G;
+  if (Kc->bs == 3)
+    G = eis_sweeps<3, EIS_FWD_R0>(Kc, dpos, ncol, cb, r0, nullptr, sv, rh, ph, st, part, 0, 0, S(s));
+  else
+    G = eis_sweeps<1, EIS_FWD_R0>(Kc, dpos, ncol, cb, r0, nullptr, sv, rh, ph, st, part, 0, 0, S(s));
+  k_eis_sum<<<1, WG, 0, S(s)>>>(PART_S(part), G, PART_B2(part, 0));
+  CK_LAUNCH("k_eis_sum");
+  return 0;
+}
+
+int ugk_eis_iter(const ugk_mat *Kc, const int *dpos, int ncol, const long *cb, long n, double *xh, double *rh,
+                 double *ph, double *qh, double *t, double *sv, const double *dsq, const double *x0, double *yx,
+                 int check, int rule, ugk_state *st, double *part, void *s)
+{
+  const unsigned GV = vec_blocks(n);
+  unsigned G;
+  if (Kc->bs == 3) {
+    eis_sweeps<3, EIS_BWD_P>(Kc, dpos, ncol, cb, rh, nullptr, t, ph, nullptr, st, part, GV, 1, S(s));
+    G = eis_sweeps<3, EIS_FWD_Q>(Kc, dpos, ncol, cb, ph, t, sv, qh, nullptr, st, part, GV, 1, S(s));
+  } else {
+    eis_sweeps<1, EIS_BWD_P>(Kc, dpos, ncol, cb, rh, nullptr, t, ph, nullptr, st, part, GV, 1, S(s));
+    G = eis_sweeps<1, EIS_FWD_Q>(Kc, dpos, ncol, cb, ph, t, sv, qh, nullptr, st, part, GV, 1, S(s));
+  }
+  k_eis_upd<<<GV, WG, 0, S(s)>>>(n, xh, rh, ph, qh, st, part, G, GV, check);
+  CK_LAUNCH("k_eis_upd");
+  if (check) {
+    unsigned GX = 0;
+    if (rule == 0) {
+      if (Kc->bs == 3)
+        GX = eis_sweeps<3, EIS_BWD_X>(Kc, dpos, ncol, cb, xh, x0, yx, nullptr, nullptr, st, part, 0, 1, S(s));
+      else
+        GX = eis_sweeps<1, EIS_BWD_X>(Kc, dpos, ncol, cb, xh, x0, yx, nullptr, nullptr, st, part, 0, 1, S(s));
+    }
+    if (Kc->bs == 3) k_eis_resid<3><<<GV, WG, 0, S(s)>>>(view(Kc), Kc->d_vals, dpos, rh, dsq, st, part);
+    else k_eis_resid<1><<<GV, WG, 0, S(s)>>>(view(Kc), Kc->d_vals, dpos, rh, dsq, st, part);
+    k_eis_test<<<1, WG, 0, S(s)>>>(st, part, GV, GX);
+    CK_LAUNCH("k_eis_test");
+  }
+  return 0;
+}
+
+int ugk_eis_finish(const ugk_mat *Kc, const int *dpos, int ncol, const long *cb, const double *xh, double *x,
+                   double *yx, ugk_state *st, double *part, void *s)
+{
+  if (Kc->bs == 3)
+    eis_sweeps<3, EIS_BWD_X>(Kc, dpos, ncol, cb, xh, x, yx, x, nullptr, st, part, 0, 0, S(s));
+  else
+    eis_sweeps<1, EIS_BWD_X>(Kc, dpos, ncol, cb, xh, x, yx, x, nullptr, st, part, 0, 0, S(s));
+  CK_LAUNCH("k_eis_sweep");
   return 0;
 }
 
