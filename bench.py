@@ -49,43 +49,124 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh, gpu_iters, sample_iters=60):
-    """uFEM's threaded CPU solver (oracle restatement, pthreads <= 8 like
-    AF_MAX_THREADS) on the same system: K assembled with 27 Gauss points,
-    `sample_iters` CG iterations timed, the solve time extrapolated to the
-    iteration count the GPU needed."""
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(nx, ny, nz, seed, t1_iters=20):
+    """uFEM's threaded CPU path (the oracle's pthread restatement of fem_dofs,
+    fem_fill_K, fem_add_disps, femEqsCGwJ and the reactions pass, threads
+    capped at 8 like AF_MAX_THREADS) on the same 1M-DOF system, every phase
+    timed: the full solve to ||r||/||f|| <= 1e-8 (no extrapolation) at
+    T = min(8, cores), then `t1_iters` CG iterations at T = 1 (K from 2x2x2
+    points there: the per-iteration cost does not depend on K's values), and
+    the 20x4x4 cantilever under the reference's own criterion (eps 1e-21,
+    maxIt n), timed with the compiled reference when it is built.  Runs in a
+    child process beside the GPU legs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ufem_oracle as uo
-    threads = max(1, min(8, os.cpu_count() or 1))
+    from ufem_amd.mesh import cantilever, lognormal_E
+    ncpu = os.cpu_count() or 1
+    threads = max(1, min(8, ncpu))
+    mesh = cantilever(nx, ny, nz, E=lognormal_E(seed, nx * ny * nz, cov=0.3))
     t0 = time.perf_counter()
     s = uo.OracleSystem(mesh, threads=threads)
+    t_dofs = time.perf_counter() - t0
     s.fill_K(3)
     s.add_loads()
     s.add_disps()
-    t_asm = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    rv, it, rr = s.solve(0, 1e-8, sample_iters, 1)
-    t_cg = time.perf_counter() - t1
-    per_it = t_cg / max(it, 1)
+    rv, it, rr = s.solve(0, 1e-8, 20 * s.n, 1)
+    s.reactions(3)
+    tm = s.timings()
+    R, _, rdof = s.R()
     s.close()
-    return {
-        "value": per_it * gpu_iters,
-        "unit": "s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"oracle pthread restatement of femEqsCGwJ/fem_fill_K, {threads} threads, on the full "
-                   f"{mesh.n_dof}-DOF system: assembly (27 Gauss points) {t_asm:.2f} s, {it} CG iterations "
-                   f"{t_cg:.2f} s ({per_it * 1e3:.1f} ms/iteration), solve extrapolated to the GPU's "
-                   f"{gpu_iters} iterations"),
-        "assembly_s": t_asm,
-        "ms_per_iteration": per_it * 1e3,
-    }
+    out = {"threads": threads, "nproc": ncpu, "cpu_model": cpu_model(),
+           "phases_s": {"dofs_alloc": round(t_dofs, 3), "fill_K_27pt": round(tm["fill"], 3),
+                        "supports": round(tm["bc"], 4), "solve": round(tm["solve"], 3),
+                        "reactions_pass": round(tm["react"], 3)},
+           "iters": it, "rv": rv, "relres": rr, "sum_Rz": float(R[rdof == 3].sum()) if R.size else None}
+    # T = 1 per iteration
+    s1 = uo.OracleSystem(mesh, threads=1)
+    s1.fill_K(2)
+    s1.add_loads()
+    s1.add_disps()
+    t1 = time.perf_counter()
+    _, it1, _ = s1.solve(0, 1e-30, t1_iters, 1)
+    per1 = (time.perf_counter() - t1) / max(it1, 1)
+    s1.close()
+    out["t1_ms_per_iteration"] = round(per1 * 1e3, 2)
+    out["t8_ms_per_iteration"] = round(tm["solve"] / max(it, 1) * 1e3, 2)
+    # the reference's own criterion on the CPU-runnable configuration
+    m1 = cantilever(20, 4, 4)
+    harness = os.path.join(ROOT, "oracle", "_ref", "ufem_ref_harness")
+    if os.path.exists(harness):
+        import subprocess
+        import tempfile
+        from ufem_amd.mesh import write_fem_input
+        with tempfile.TemporaryDirectory() as td:
+            fin = os.path.join(td, "c1.fem")
+            write_fem_input(m1, fin)
+            r = subprocess.run([harness, "-i", fin, "-o", td, "-nodump-k"], capture_output=True, text=True,
+                               timeout=300)
+            j = json.loads(r.stdout)
+        out["config1_reference_rule"] = {"kind": "reference", "iters": j["spmv_calls"] - 1, "rv": j["rv"],
+                                         "solve_s": j["t_solve"], "eps": j["eps"], "maxit": j["maxit"]}
+    else:
+        o1 = uo.OracleSystem(m1)
+        o1.assemble(3)
+        rv1, it1c, _ = o1.solve(0, 1e-21, o1.n, 0)
+        out["config1_reference_rule"] = {"kind": "port", "iters": it1c, "rv": rv1,
+                                         "solve_s": o1.timings()["solve"], "eps": 1e-21, "maxit": o1.n}
+        o1.close()
+    return out
+
+
+def start_cpu_baseline(args):
+    """the CPU baseline as a child process (no GPU use), read by finish_cpu_baseline"""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--size", args.size,
+           "--seed", str(args.seed)]
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+
+
+def finish_cpu_baseline(proc, mesh, timeout=900):
+    try:
+        out, err = proc.communicate(timeout=timeout)
+    except Exception:
+        proc.kill()
+        proc.communicate()
+        log("CPU baseline did not finish in time")
+        return None
+    if proc.returncode != 0:
+        log("CPU baseline failed:", err[-2000:])
+        return None
+    c = json.loads(out.strip().splitlines()[-1])
+    ph = c["phases_s"]
+    c.update({
+        "value": ph["solve"], "unit": "s", "cores": c["threads"], "kind": "port",
+        "sample": (f"oracle pthread restatement of uFEM's path (fem_dofs, fem_fill_K 27-point, fem_add_disps, "
+                   f"femEqsCGwJ, reactions pass), {c['threads']} threads of {c['nproc']} ({c['cpu_model']}), "
+                   f"full {mesh.n_dof}-DOF system solved to ||r||/||f|| <= 1e-8 in {c['iters']} iterations, "
+                   f"timed whole (no extrapolation); phases {ph}; T=1 {c['t1_ms_per_iteration']} ms per "
+                   f"iteration, T={c['threads']} {c['t8_ms_per_iteration']} ms; run in a child process "
+                   f"beside the GPU legs"),
+    })
+    return c
 
 
 def bench_mixed_high_contrast(nx, ny, nz, seed):
-    """FP64 Jacobi PCG vs FP32-inner / FP64-refinement PCG on the high-contrast
-    field (lognormal E, COV 0.6), both to ||r||/||f|| <= 1e-8, and the
-    displacement error of the mixed solve against the FP64 one."""
+    """FP64 Jacobi PCG vs mixed precision on the high-contrast field (lognormal
+    E, COV 0.6), all to ||r||/||f|| <= 1e-8, with the displacement error of
+    each against the FP64 solve: the configuration's method (FP32 matrix and
+    inner CG to 1e-4 inside FP64 iterative refinement: the inner CG restarts
+    at every refinement), the reliable-update form (refresh every factor 0.1,
+    the direction kept), and the FP32 matrix with FP64 vectors."""
     from ufem_amd import gpu as G
     from ufem_amd.mesh import cantilever, lognormal_E
     mesh = cantilever(nx, ny, nz, E=lognormal_E(seed, nx * ny * nz, cov=0.6))
@@ -96,18 +177,93 @@ def bench_mixed_high_contrast(nx, ny, nz, seed):
     r64 = g.solve(solver="pcg", eps=1e-8, rule=1)
     t64 = time.perf_counter() - t0
     u64 = g.u()
-    g.solve(solver="mixed", eps=1e-8, rule=1, maxit=40000)
-    t0 = time.perf_counter()
-    rm = g.solve(solver="mixed", eps=1e-8, rule=1, maxit=40000)
-    tm = time.perf_counter() - t0
-    um = g.u()
+    out = {"workload": f"cantilever {nx}x{ny}x{nz}, E lognormal COV 0.6 seed {seed}",
+           "fp64": {"iters": r64["iters"], "solve_s": round(t64, 4), "true_relres": r64["true_relres"]}}
+    forms = {"refinement_inner_1e-4": dict(solver="mixed", inner_eps=1e-4, inner_restart=True),
+             "reliable_update_0.1": dict(solver="mixed", inner_eps=0.1),
+             "fp64_vectors_0.1": dict(solver="mixed64", inner_eps=0.1)}
+    for name, kw in forms.items():
+        g.solve(eps=1e-8, rule=1, maxit=60000, **kw)
+        t0 = time.perf_counter()
+        rm = g.solve(eps=1e-8, rule=1, maxit=60000, **kw)
+        tm = time.perf_counter() - t0
+        um = g.u()
+        out[name] = {"iters": rm["iters"], "fp64_refreshes": rm["outer"], "solve_s": round(tm, 4), "rv": rm["rv"],
+                     "true_relres": rm["true_relres"],
+                     "u_rel_err_vs_fp64": float(np.linalg.norm(um - u64) / np.linalg.norm(u64)),
+                     "speedup_vs_fp64": round(t64 / tm, 3)}
     g.close()
-    return {"workload": f"cantilever {nx}x{ny}x{nz}, E lognormal COV 0.6 seed {seed}",
-            "fp64": {"iters": r64["iters"], "solve_s": round(t64, 4), "true_relres": r64["true_relres"]},
-            "mixed": {"iters": rm["iters"], "outer": rm["outer"], "solve_s": round(tm, 4), "rv": rm["rv"],
-                      "true_relres": rm["true_relres"]},
-            "u_rel_err_vs_fp64": float(np.linalg.norm(um - u64) / np.linalg.norm(u64)),
-            "speedup_vs_fp64": round(t64 / tm, 3)}
+    return out
+
+
+def khit_capacity(mesh, dmap, known=7):
+    """row capacities as femKhit sizes them (fem/fem_mloc.c:83-116): 24 per
+    element touching the row's node"""
+    inc = np.bincount(np.asarray(mesh.enodes).ravel(), minlength=mesh.n_nodes)
+    n = int(dmap.max())
+    cap = np.zeros(n, dtype=np.int64)
+    for d in range(3):
+        r = dmap[np.arange(mesh.n_nodes) * known + d]
+        sel = r > 0
+        cap[r[sel] - 1] = 24 * inc[sel]
+    return cap
+
+
+def bench_dropin(mesh, native_s, native_ms, u64):
+    """The drop-in boundary at the metric's size: fem_fill_K(AF_NO) on the GPU
+    into the reference's row-slot K (DOF map with the zero supports left
+    out, as -fbc), then femEqsCGwJ through the C-ABI on that row-slot matrix
+    (relative rule, 1e-8): the first call builds the device copy (3x3 node
+    blocks), a second re-reads the host matrix (default policy), a third
+    uses the valid copy (policy 1).  Times: host wall of each call and the
+    device time of the solve inside it, beside the native solve."""
+    from ufem_amd import gpu as G
+    dm = G.dof_map(mesh, fbc=True)
+    n = int(dm.max())
+    cap = khit_capacity(mesh, dm)
+    t0 = time.perf_counter()
+    K = G.RowSlotMatrix.empty(n, cap)
+    t_alloc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rv = G.fill_K(K, mesh, dm, 7, 2)
+    t_fill = time.perf_counter() - t0
+    if rv != 0:
+        raise SystemExit(f"fill_K failed ({rv}): {G.last_error()}")
+    fk = G.fill_K_timings()
+    F = np.zeros(n)
+    for nd, t, d, v in zip(mesh.nl_node, mesh.nl_type, mesh.nl_dir, mesh.nl_val):
+        r = dm[int(nd) * 7 + int(d) - 1]
+        if int(t) == 3 and r > 0:
+            F[r - 1] += v
+    b = G.Vec(F)
+    G.set_stop_rule(1)
+    calls = []
+    for policy in (0, 0, 1):
+        G.set_mirror_policy(policy)
+        x = G.Vec(np.zeros(n))
+        t0 = time.perf_counter()
+        rv = G.femEqsCGwJ(K, b, x, 1e-8, n)
+        wall = time.perf_counter() - t0
+        info = G.last_solve()
+        mi = K.mirror_info()
+        calls.append({"policy": policy, "rv": rv, "wall_s": round(wall, 4), "solve_ms": round(info["solve_ms"], 2),
+                      "iters": info["iters"], "build_ms": round(mi["build_ms"], 1),
+                      "refresh_ms": round(mi["refresh_ms"], 1)})
+    G.set_mirror_policy(0)
+    G.set_stop_rule(0)
+    full = np.zeros(mesh.n_nodes * 3)
+    act = dm.reshape(mesh.n_nodes, 7)[:, :3].reshape(-1)
+    full[act > 0] = x.data[act[act > 0] - 1]
+    err = float(np.linalg.norm(full - u64) / np.linalg.norm(u64))
+    K.release()
+    return {"workload": "the metric's system as uFEM's row-slot K (zero supports left out, as -fbc)",
+            "rows": n, "filled_slots": mi["nfill"], "block_size": mi["bs"],
+            "rowslot_alloc_s": round(t_alloc, 3),
+            "fill_K": {"wall_s": round(t_fill, 3), "setup_ms": round(fk["setup_ms"], 1),
+                       "device_ms": round(fk["device_ms"], 2), "writeback_ms": round(fk["writeback_ms"], 1)},
+            "calls": calls, "u_rel_err_vs_native": err,
+            "native_solve_ms": round(native_ms, 2), "native_wall_s": round(native_s, 4),
+            "solve_ms_ratio_to_native": round(calls[-1]["solve_ms"] / native_ms, 3)}
 
 
 def bench_learned(fields, steps, seed):
@@ -278,6 +434,14 @@ def bench_batch(n_samples=256, seed=7, sequential_sample=16):
         seq_ms.append(g.solve(solver="pcg", eps=1e-8, rule=1)["solve_ms"])
         g.close()
     seq_total = float(np.mean(seq_ms)) * n_samples
+    # algorithmic bytes: per sample and iteration its SpMV (76 B per 3x3 block,
+    # u read and w written) and the single-reduction update (7 vectors read,
+    # 5 written) = 76 nnzb + 112 n
+    nbytes = 0.0
+    for m, it in zip(meshes, r["iters"]):
+        nnzb = (3 * (m.nx + 1) - 2) * (3 * (m.ny + 1) - 2) * (3 * (m.nz + 1) - 2)
+        nbytes += float(it) * (76.0 * nnzb + 112.0 * m.n_dof)
+    gbps = nbytes / (r["total_ms"] * 1e-3) / 1e9
     out = {
         "samples": n_samples, "classes": b.nclasses, "assembly_ms": round(asm_ms, 2),
         "autotune": {"wg": r["wg"], "grid": r["grid"], "team_big": r["team_big"], "seconds": round(tune_s, 2)},
@@ -287,6 +451,9 @@ def bench_batch(n_samples=256, seed=7, sequential_sample=16):
         "all_converged": bool(np.all(r["status"] == 1)), "iters_min_max": [int(r["iters"].min()), int(r["iters"].max())],
         "one_at_a_time_ms_estimate": round(seq_total, 1),
         "speedup_vs_one_at_a_time": round(seq_total / r["total_ms"], 3),
+        "roofline": {"bound": "hbm", "algorithmic_bytes": nbytes, "achieved_GBps": round(gbps, 1),
+                     "peak": HBM_PEAK_GBPS, "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                     "bytes_per_sample_iteration": "76 nnzb + 112 n"},
     }
     b.close()
     return out
@@ -379,7 +546,13 @@ def main():
     ap.add_argument("--net-kernels-only", action="store_true",
                     help="launch only the network's inference stages on the 128x32x32 random-field "
                          "cantilever (initial weights; for kernel traces)")
+    ap.add_argument("--cpu-baseline-only", action="store_true",
+                    help="(internal) run the CPU baseline leg alone and print its JSON")
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        nx, ny, nz = (int(v) for v in args.size.split(","))
+        print(json.dumps(cpu_baseline(nx, ny, nz, args.seed)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path would shard by slabs of the node grid, but it runs on one GPU)", file=sys.stderr)
@@ -403,6 +576,10 @@ def main():
         return
 
     nx, ny, nz = (int(v) for v in args.size.split(","))
+    cpu_proc = None
+    if not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.plain_only):
+        cpu_proc = start_cpu_baseline(args)
+        log("CPU baseline started in a child process")
     t0 = time.perf_counter()
     mesh = cantilever(nx, ny, nz, E=lognormal_E(args.seed, nx * ny * nz, cov=0.3))
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_dof} DOF in {time.perf_counter() - t0:.1f} s")
@@ -502,6 +679,10 @@ def main():
         g.close()
         return
 
+    log("drop-in boundary ...")
+    dropin = bench_dropin(mesh, solve_s, float(np.mean(devs)) * 1e3, u64)
+    log("dropin", dropin)
+
     mixed = None
     if not args.no_mixed:
         g.solve(solver="mixed", eps=1e-8, rule=1, maxit=20000)      # warm-up
@@ -576,10 +757,11 @@ def main():
         log("monte", monte)
 
     cpu = None
-    if not args.no_cpu:
-        log("CPU baseline sample ...")
-        cpu = cpu_baseline(mesh, info["iters"])
-        log("cpu", cpu["sample"])
+    if cpu_proc is not None:
+        log("waiting for the CPU baseline ...")
+        cpu = finish_cpu_baseline(cpu_proc, mesh)
+        if cpu:
+            log("cpu", cpu["sample"])
 
     out = {
         "metric": METRIC,
@@ -615,7 +797,12 @@ def main():
         "learned": learned,
         "batch": batch,
         "monte_plugin": monte,
+        "dropin": dropin,
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if cpu else None),
+        "cpu_detail": ({k: cpu[k] for k in ("phases_s", "iters", "rv", "relres", "sum_Rz", "t1_ms_per_iteration",
+                                              "t8_ms_per_iteration", "nproc", "cpu_model",
+                                              "config1_reference_rule")} if cpu else None),
+        "gpu_vs_cpu_solve": (round(cpu["value"] / solve_s, 1) if cpu else None),
         "device": dev["name"],
     }
     print(json.dumps(out), flush=True)

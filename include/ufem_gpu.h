@@ -160,6 +160,18 @@ double ufem_sys_time_results(ufem_sys *s, int npts, int reps);
  * element IDs are written as 1-based positions; the reactions of supports of
  * zero value only, as the reference lists them */
 int  ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep_on, double sim_num);
+/* node and element IDs the result file names (the input's IDs, see
+ * ufem_model_*); NULL: 1-based positions */
+int  ufem_sys_set_ids(ufem_sys *s, const long *node_id, const long *elem_id);
+/* the result file of the fem binary's -fbc (fem/fem_sol.c:1280-1296): DOFs
+ * of zero-valued supports left out of the DOF map and of u, their reactions
+ * 0 as the reference leaves them.  The solve is the same: on the device a
+ * support is already a masked row and column of the node-block matrix. */
+int  ufem_sys_set_fbc(ufem_sys *s, int fbc);
+/* the reference's nDOFfld (fem_dofs, fem/fem_sol.c:1263-1391): 7 entries per
+ * node, the 1-based K positions of ux, uy, uz (node-major), 0 elsewhere and,
+ * with -fbc, at zero-valued supports; returns the active DOFs (nDOFAct) */
+long ufem_sys_dof_map(const ufem_sys *s, long *map);
 int  ufem_sys_get_u(ufem_sys *s, double *u);
 int  ufem_sys_set_u(ufem_sys *s, const double *u);
 int  ufem_sys_get_F(ufem_sys *s, double *F);
@@ -177,6 +189,31 @@ int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host 
 int  ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *bytes);
 /* phase device times of the last calls: fill_K, add_disps, solve (ms) */
 void ufem_sys_timings(const ufem_sys *s, double t[3]);
+
+/* ---- uFEM's input file ------------------------------------------------------
+ * femReadInput (fem/fem_io.c:380-640) for the linear-static brick path:
+ * solution control, real sets, material sets (type 1), nodes, elements
+ * (type 9), gravity (none), nodal loads (types 1, 3), element loads (none).
+ * preparsed = 1: element node lists, material / real sets and load nodes are
+ * 0-based positions (the fem binary's -p); preparsed = 0: they are IDs,
+ * renumbered to positions through one hash table per ID list, where
+ * femRenumIndex (fem/fem_io.c:50-80) searches each list linearly per entry;
+ * the first item carrying an ID wins, as there, and an ID no item carries is
+ * AF_ERR_VAL.  *rv receives the AF_* code; NULL on error. */
+typedef struct ufem_model ufem_model;
+ufem_model *ufem_model_read(const char *fname, int preparsed, int *rv);
+void ufem_model_free(ufem_model *m);
+/* the model as a mesh for ufem_sys_create (pointers into the model, valid
+ * until ufem_model_free), E and nu from each element's material set */
+int  ufem_model_mesh(const ufem_model *m, ufem_mesh *mesh);
+/* IDs as read (node, element) and the renumbered per-element material and
+ * real-set positions */
+const long *ufem_model_node_ids(const ufem_model *m);
+const long *ufem_model_elem_ids(const ufem_model *m);
+const long *ufem_model_elem_mat(const ufem_model *m);
+const long *ufem_model_elem_rset(const ufem_model *m);
+long ufem_model_sol_id(const ufem_model *m);
+long ufem_model_sol_dep(const ufem_model *m);
 
 /* ---- two-level correction ---------------------------------------------------
  * No counterpart in the reference (its solvers precondition with the diagonal

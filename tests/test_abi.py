@@ -11,7 +11,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "ufem_amd", "libufem_gpu.so")
 HEADERS = [os.path.join(ROOT, "include", h) for h in ("ufem_fem_math.h", "ufem_gpu.h", "ufem_monte.h")]
 
-DECL = re.compile(r"^\s*(?:const\s+)?(?:int|long|double|void|char|ufem_sys)\s*\*?\s*(\w+)\s*\(", re.M)
+DECL = re.compile(r"^\s*(?:const\s+)?(?:int|long|double|void|char|ufem_sys|ufem_model|ufem_net|ufem_batch)"
+                  r"\s*\*?\s*(\w+)\s*\(", re.M)
 
 
 def declared():

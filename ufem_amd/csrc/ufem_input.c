@@ -11,7 +11,8 @@
  * Preparsed files carry 0-based positions where the reference would carry
  * IDs (element material and real-set fields, element node lists, load
  * nodes), so nothing is renumbered (femRenumIndex returns at once,
- * fem/fem_io.c:55).  Sizes that disagree with their item counts return
+ * fem/fem_io.c:55); the form with IDs (the fem binary without -p) is
+ * renumbered by hash lookups (renumber_ids below).  Sizes that disagree with their item counts return
  * AF_ERR_SIZ, fewer than 2 nodes or no element AF_ERR_SML, as the reference
  * does; a short or unreadable file returns AF_ERR_IO.
  */
@@ -165,6 +166,83 @@ static int read_body(FILE *f, ufem_input *in)
   return AF_OK;
 }
 
+/* ---- IDs -> positions (the input form without -p) ------------------------
+ * femRenumIndex (fem/fem_io.c:50-80) replaces every ID of a list by the
+ * position of the first item carrying it, by a linear search of the item
+ * IDs per entry (O(n m)).  Here the item IDs go into an open-addressing hash
+ * table once (the first position of an ID kept), and every entry is one
+ * lookup.  An ID no item carries is AF_ERR_VAL, as there. */
+typedef struct {
+  long *key, *val;
+  unsigned long mask;
+} idmap;
+
+static unsigned long id_hash(long id)
+{
+  unsigned long z = (unsigned long)id + 0x9E3779B97F4A7C15UL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9UL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBUL;
+  return z ^ (z >> 31);
+}
+
+static int idmap_build(idmap *m, const long *ids, long n)
+{
+  unsigned long cap = 16, h;
+  long i;
+  while (cap < 2UL * (unsigned long)n) cap <<= 1;
+  m->key = (long *)malloc(sizeof(long) * cap);
+  m->val = (long *)malloc(sizeof(long) * cap);
+  m->mask = cap - 1;
+  if (!m->key || !m->val) return AF_ERR_MEM;
+  for (h = 0; h < cap; h++) m->val[h] = -1;
+  for (i = 0; i < n; i++) {
+    for (h = id_hash(ids[i]) & m->mask; m->val[h] >= 0 && m->key[h] != ids[i]; h = (h + 1) & m->mask) {}
+    if (m->val[h] < 0) {      /* a repeated ID keeps its first position */
+      m->key[h] = ids[i];
+      m->val[h] = i;
+    }
+  }
+  return AF_OK;
+}
+
+static void idmap_free(idmap *m)
+{
+  free(m->key);
+  free(m->val);
+}
+
+static int renumber(long *ind, long n, const long *ids, long nids, const char *what)
+{
+  idmap m;
+  long i;
+  int rv = idmap_build(&m, ids, nids);
+  if (rv != AF_OK) { idmap_free(&m); return rv; }
+  for (i = 0; i < n; i++) {
+    unsigned long h;
+    for (h = id_hash(ind[i]) & m.mask; m.val[h] >= 0 && m.key[h] != ind[i]; h = (h + 1) & m.mask) {}
+    if (m.val[h] < 0) {
+      ufem_set_error("%s %ld: ID %ld names no item", what, i, ind[i]);
+      idmap_free(&m);
+      return AF_ERR_VAL;
+    }
+    ind[i] = m.val[h];
+  }
+  idmap_free(&m);
+  return AF_OK;
+}
+
+/* the reference's order (fem/fem_io.c:568-605, :113): element material sets,
+ * real sets (when there are any), element nodes, load nodes */
+static int renumber_ids(ufem_input *in)
+{
+  int rv;
+  if ((rv = renumber(in->e_mp, in->e_len, in->mp_id, in->mp_len, "element material set")) != AF_OK) return rv;
+  if (in->rs_len > 0 && (rv = renumber(in->e_rs, in->e_len, in->rs_id, in->rs_len, "element real set")) != AF_OK)
+    return rv;
+  if ((rv = renumber(in->e_nodes, in->e_len_l, in->n_id, in->n_len, "element node")) != AF_OK) return rv;
+  return renumber(in->nl_node, in->nl_len, in->n_id, in->n_len, "load node");
+}
+
 /* what the linear-static brick path supports: element type 9 with 8 nodes,
  * material type 1, no gravity, nodal loads of type 1 (displacement) and 3
  * (force) in directions 1-3, positions in range */
@@ -193,7 +271,7 @@ static int check_scope(ufem_input *in)
   return AF_OK;
 }
 
-int ufem_input_read(const char *fname, ufem_input *in)
+int ufem_input_read_form(const char *fname, int preparsed, ufem_input *in)
 {
   FILE *f;
   int rv;
@@ -203,10 +281,80 @@ int ufem_input_read(const char *fname, ufem_input *in)
   rv = read_body(f, in);
   fclose(f);
   if (rv == AF_ERR_IO) ufem_set_error("%s: short or unreadable input", fname);
+  if (rv == AF_OK && !preparsed) rv = renumber_ids(in);
   if (rv == AF_OK) rv = check_scope(in);
   if (rv != AF_OK) ufem_input_free(in);
   return rv;
 }
+
+int ufem_input_read(const char *fname, ufem_input *in)
+{
+  return ufem_input_read_form(fname, 1, in);
+}
+
+/* ---- the public model handle -------------------------------------------- */
+struct ufem_model {
+  ufem_input in;
+  double *E, *nu;
+};
+
+ufem_model *ufem_model_read(const char *fname, int preparsed, int *rv)
+{
+  ufem_model *m = (ufem_model *)calloc(1, sizeof(ufem_model));
+  int r;
+  if (!m) { if (rv) *rv = AF_ERR_MEM; return NULL; }
+  if ((r = ufem_input_read_form(fname, preparsed, &m->in)) != AF_OK) {
+    free(m);
+    if (rv) *rv = r;
+    return NULL;
+  }
+  m->E = dalloc(m->in.e_len);
+  m->nu = dalloc(m->in.e_len);
+  if (!m->E || !m->nu) {
+    ufem_model_free(m);
+    if (rv) *rv = AF_ERR_MEM;
+    return NULL;
+  }
+  ufem_input_elastic(&m->in, m->E, m->nu);
+  if (rv) *rv = AF_OK;
+  return m;
+}
+
+void ufem_model_free(ufem_model *m)
+{
+  if (!m) return;
+  ufem_input_free(&m->in);
+  free(m->E);
+  free(m->nu);
+  free(m);
+}
+
+int ufem_model_mesh(const ufem_model *m, ufem_mesh *mesh)
+{
+  if (!m || !mesh) return AF_ERR_VAL;
+  memset(mesh, 0, sizeof(*mesh));
+  mesh->n_nodes = m->in.n_len;
+  mesh->x = m->in.n_x;
+  mesh->y = m->in.n_y;
+  mesh->z = m->in.n_z;
+  mesh->n_elems = m->in.e_len;
+  mesh->enodes = m->in.e_nodes;      /* 8 per element, in order (checked on reading) */
+  mesh->E = m->E;
+  mesh->nu = m->nu;
+  mesh->n_nl = m->in.nl_len;
+  mesh->nl_node = m->in.nl_node;
+  mesh->nl_type = m->in.nl_type;
+  mesh->nl_dir = m->in.nl_dir;
+  mesh->nl_val = m->in.nl_val;
+  return AF_OK;
+}
+
+const long *ufem_model_node_ids(const ufem_model *m) { return m ? m->in.n_id : NULL; }
+const long *ufem_model_elem_ids(const ufem_model *m) { return m ? m->in.e_id : NULL; }
+const long *ufem_model_elem_mat(const ufem_model *m) { return m ? m->in.e_mp : NULL; }
+const long *ufem_model_elem_rset(const ufem_model *m) { return m ? m->in.e_rs : NULL; }
+long ufem_model_sol_id(const ufem_model *m) { return m ? m->in.sol_id : 0; }
+long ufem_model_sol_dep(const ufem_model *m) { return m ? m->in.sol_dep_on : 0; }
 
 /* femGetRepValIndex (fem/fem_accs.c:297-381) over material type 1: a
  * non-repeating value missing from the list or past the set gives -1; a

@@ -81,6 +81,8 @@ struct ufem_sys {
   int *d_saved_cols;
   double *d_Rfix;
   int filled, loaded, bc_applied;
+  int fbc;                  /* result file as with -fbc: zero supports left out   */
+  long *node_id, *elem_id;  /* IDs for the result file (NULL: 1-based positions) */
   /* vectors */
   double *d_F, *d_u;
   ufem_work w;
@@ -132,7 +134,8 @@ typedef struct {
   long mc_ilen, mc_olen, mc_fail_id;
   long *mc_pos, *mc_type, *mc_item, *mc_sub, *mc_iter;
 } ufem_input;
-int  ufem_input_read(const char *fname, ufem_input *in);
+int  ufem_input_read(const char *fname, ufem_input *in);          /* preparsed (-p) */
+int  ufem_input_read_form(const char *fname, int preparsed, ufem_input *in);
 void ufem_input_free(ufem_input *in);
 /* value `type` (repetition `rep`, 0: the non-repeating part) of material set
  * `mp` of type 1: its index in mp_val, -1 when absent (femGetRepValIndex) */

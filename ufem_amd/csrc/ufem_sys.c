@@ -7,6 +7,7 @@
  * each function cites the reference step it replaces.
  */
 #define _POSIX_C_SOURCE 200809L
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -260,6 +261,7 @@ void ufem_sys_destroy(ufem_sys *s)
   free(s->color_off); free(s->nl_node); free(s->nl_type); free(s->nl_dir); free(s->nl_val);
   free(s->F_host); free(s->F_load); free(s->fixed_host); free(s->react_node); free(s->react_dof);
   free(s->react_fix); free(s->react_init); free(s->fix_val);
+  free(s->node_id); free(s->elem_id);
   free(s);
 }
 
@@ -677,13 +679,75 @@ double ufem_sys_time_results(ufem_sys *s, int npts, int reps)
 #define WR_KNOWN_DOFS 7
 #define WR_RES_RP 20
 #define WR_ELEM_VALS (14 * 27)
+/* IDs for the result file (the input's node and element IDs); NULL resets
+ * to 1-based positions */
+int ufem_sys_set_ids(ufem_sys *s, const long *node_id, const long *elem_id)
+{
+  free(s->node_id);
+  free(s->elem_id);
+  s->node_id = s->elem_id = NULL;
+  if (node_id) {
+    if (!(s->node_id = (long *)malloc(sizeof(long) * (size_t)s->n_nodes))) return AF_ERR_MEM;
+    memcpy(s->node_id, node_id, sizeof(long) * (size_t)s->n_nodes);
+  }
+  if (elem_id) {
+    if (!(s->elem_id = (long *)malloc(sizeof(long) * (size_t)s->n_elems))) return AF_ERR_MEM;
+    memcpy(s->elem_id, elem_id, sizeof(long) * (size_t)s->n_elems);
+  }
+  return AF_OK;
+}
+
+int ufem_sys_set_fbc(ufem_sys *s, int fbc)
+{
+  s->fbc = fbc ? 1 : 0;
+  return AF_OK;
+}
+
+/* a zero-valued support of the -fbc elimination (|value| <= FEM_ZERO, the
+ * test of fem_dofs, fem/fem_sol.c:1286) */
+#define FBC_ZERO 1e-17
+static void fbc_mask(const ufem_sys *s, unsigned char *drop)
+{
+  long f;
+  memset(drop, 0, (size_t)s->n);
+  for (f = 0; f < s->n_fixed; f++)
+    if (fabs(s->fix_val[f]) <= FBC_ZERO) drop[s->fixed_host[f]] = 1;
+}
+
+/* the reference's DOF map nDOFfld (fem_dofs, fem/fem_sol.c:1263-1391) with
+ * WR_KNOWN_DOFS entries per node: the 1-based K position of ux, uy, uz,
+ * node-major, the rest 0; with -fbc the zero-valued supports get 0 and the
+ * positions close up.  Returns the number of active DOFs (nDOFAct). */
+long ufem_sys_dof_map(const ufem_sys *s, long *map)
+{
+  unsigned char *drop = NULL;
+  long i, act = 0;
+  int d;
+  if (s->fbc) {
+    if (!(drop = (unsigned char *)malloc((size_t)s->n))) return AF_ERR_MEM;
+    fbc_mask(s, drop);
+  }
+  for (i = 0; i < s->n_nodes; i++)
+    for (d = 0; d < WR_KNOWN_DOFS; d++) {
+      long v = 0;
+      if (d < 3 && !(drop && drop[3 * i + d])) v = ++act;
+      if (map) map[i * WR_KNOWN_DOFS + d] = v;
+    }
+  free(drop);
+  return act;
+}
+
 int ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep_on, double sim_num)
 {
   const long nn = s->n_nodes, ne = s->n_elems, n = s->n, nr = s->n_react;
   double *u = NULL, *R = NULL, *res = NULL;
-  long *rn = NULL, *rd = NULL, i, e;
+  long *rn = NULL, *rd = NULL, *map = NULL, i, e, act;
   FILE *f = NULL;
   int rv = AF_ERR_MEM, d;
+  map = (long *)malloc(sizeof(long) * (size_t)nn * WR_KNOWN_DOFS);
+  if (!map) goto out;
+  act = ufem_sys_dof_map(s, map);
+  if (act < 0) goto out;
   u = (double *)malloc(sizeof(double) * (size_t)n);
   R = (double *)malloc(sizeof(double) * (size_t)(nr > 0 ? nr : 1));
   rn = (long *)malloc(sizeof(long) * (size_t)(nr > 0 ? nr : 1));
@@ -699,12 +763,15 @@ int ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep
     goto out;
   }
   fprintf(f, "S\n %li %li %e \n", sol_id, dep_on, sim_num);
-  fprintf(f, " %li \n %li \n %li\n", n, (long)WR_KNOWN_DOFS * nn, nn);
-  for (i = 0; i < nn; i++) fprintf(f, " %li\n", i + 1);
+  fprintf(f, " %li \n %li \n %li\n", act, (long)WR_KNOWN_DOFS * nn, nn);
+  for (i = 0; i < nn; i++) fprintf(f, " %li\n", s->node_id ? s->node_id[i] : i + 1);
+  for (i = 0; i < nn * WR_KNOWN_DOFS; i++) fprintf(f, " %li\n", map[i]);
+  /* u at the active DOFs in K order (the DOFs -fbc leaves out are 0) */
+  fprintf(f, " %li\n", act);
   for (i = 0; i < nn; i++)
-    for (d = 0; d < WR_KNOWN_DOFS; d++) fprintf(f, " %li\n", d < 3 ? 3 * i + d + 1 : 0L);
-  fprintf(f, " %li\n", n);
-  for (i = 0; i < n; i++) fprintf(f, " %e \n", u[i]);
+    for (d = 0; d < 3; d++)
+      if (map[i * WR_KNOWN_DOFS + d] > 0) fprintf(f, " %e \n", u[3 * i + d]);
+  (void)n;
   /* the reference's reaction list leaves out supports of non-zero value
      (fem_sol_res_alloc, fem/fem_sol.c:676-722) */
   {
@@ -712,10 +779,15 @@ int ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep
     for (i = 0; i < nr; i++) nz += s->fix_val[s->react_fix[i]] == 0.0;
     fprintf(f, " %li \n", nz);
   }
+  /* under -fbc the eliminated supports keep the reaction slot's initial 0:
+     femApplyDisp and fem_fill_K_React skip DOFs outside K (fem/fem_loads.c:132,
+     fem/fem_sol.c:862) */
   for (i = 0; i < nr; i++)
-    if (s->fix_val[s->react_fix[i]] == 0.0) fprintf(f, " %li %li %li %e\n", rn[i], rn[i] + 1, rd[i], R[i]);
+    if (s->fix_val[s->react_fix[i]] == 0.0)
+      fprintf(f, " %li %li %li %e\n", rn[i], s->node_id ? s->node_id[rn[i]] : rn[i] + 1, rd[i],
+              s->fbc ? 0.0 : R[i]);
   fprintf(f, " %li %li\n", ne, (long)WR_ELEM_VALS * ne);
-  for (e = 0; e < ne; e++) fprintf(f, " %li \n", e + 1);
+  for (e = 0; e < ne; e++) fprintf(f, " %li \n", s->elem_id ? s->elem_id[e] : e + 1);
   for (e = 0; e < ne; e++) fprintf(f, " %li \n", (long)WR_ELEM_VALS * e);
   for (e = 0; e < ne; e++)
     for (i = 0; i < WR_ELEM_VALS; i++) {
@@ -727,7 +799,7 @@ int ufem_sys_write_results(ufem_sys *s, const char *fname, long sol_id, long dep
   if (rv != AF_OK) ufem_set_error("cannot close %s", fname);
 out:
   if (f) fclose(f);
-  free(u); free(R); free(rn); free(rd); free(res);
+  free(u); free(R); free(rn); free(rd); free(res); free(map);
   return rv;
 }
 

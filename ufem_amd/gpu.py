@@ -162,6 +162,18 @@ def lib():
             "ufem_gpu_fill_K": (C.c_int, [C.POINTER(tMatrix), C.POINTER(_Mesh), P_L, C.c_long, C.c_int]),
             "ufem_gpu_fill_K_timings": (None, [P_D]),
             "ufem_gpu_fill_K_release": (None, []),
+            "ufem_model_read": (C.c_void_p, [C.c_char_p, C.c_int, P_I]),
+            "ufem_model_free": (None, [C.c_void_p]),
+            "ufem_model_mesh": (C.c_int, [C.c_void_p, C.POINTER(_Mesh)]),
+            "ufem_model_node_ids": (P_L, [C.c_void_p]),
+            "ufem_model_elem_ids": (P_L, [C.c_void_p]),
+            "ufem_model_elem_mat": (P_L, [C.c_void_p]),
+            "ufem_model_elem_rset": (P_L, [C.c_void_p]),
+            "ufem_model_sol_id": (C.c_long, [C.c_void_p]),
+            "ufem_model_sol_dep": (C.c_long, [C.c_void_p]),
+            "ufem_sys_set_ids": (C.c_int, [C.c_void_p, P_L, P_L]),
+            "ufem_sys_set_fbc": (C.c_int, [C.c_void_p, C.c_int]),
+            "ufem_sys_dof_map": (C.c_long, [C.c_void_p, P_L]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -315,6 +327,21 @@ class GpuSystem:
         if ms < 0:
             raise RuntimeError("time_results: " + last_error())
         return ms
+
+    def set_ids(self, node_id, elem_id):
+        """the input's node and element IDs for the result file"""
+        self._ids = (np.ascontiguousarray(node_id, dtype=np.int64), np.ascontiguousarray(elem_id, dtype=np.int64))
+        self._chk(lib().ufem_sys_set_ids(self.h, _pl(self._ids[0]), _pl(self._ids[1])), "set_ids")
+
+    def set_fbc(self, fbc=True):
+        """result file as the fem binary's -fbc writes it (zero supports left out)"""
+        self._chk(lib().ufem_sys_set_fbc(self.h, 1 if fbc else 0), "set_fbc")
+
+    def dof_map(self):
+        """(nDOFfld with 7 entries per node, nDOFAct)"""
+        m = np.zeros(7 * (self.n // 3), dtype=np.int64)
+        act = lib().ufem_sys_dof_map(self.h, _pl(m))
+        return m, int(act)
 
     def write_results(self, path, sol_id=1, dep_on=-1, sim_num=0.0):
         """the reference's result file (femWriteRes format) for the current u"""
@@ -652,13 +679,15 @@ class RowSlotMatrix:
         ln = int(cap.sum())
         self.pos = np.zeros(ln + 1, dtype=np.int64)
         self.data = np.zeros(ln + 1)
-        src = 0
-        for i in range(n):
-            k = int(rowlen[i])
-            f = int(self.frompos[i])
-            self.pos[f:f + k] = cols1[src:src + k]
-            self.data[f:f + k] = vals[src:src + k]
-            src += k
+        tot = int(rowlen.sum())
+        if tot:
+            # slot of entry j of row i: frompos[i] + j
+            start = np.zeros(n, dtype=np.int64)
+            np.cumsum(rowlen[:-1], out=start[1:])
+            row = np.repeat(np.arange(n, dtype=np.int64), rowlen)
+            at = self.frompos[row] + (np.arange(tot, dtype=np.int64) - start[row])
+            self.pos[at] = np.asarray(cols1, dtype=np.int64)[:tot]
+            self.data[at] = np.asarray(vals, dtype=np.float64)[:tot]
         self.s = tMatrix(1, n, n, ln, _pl(self.pos), _pd(self.data), _pl(self.frompos), _pl(self.defpos))
 
 
@@ -778,6 +807,60 @@ class Monte:
 
     def close(self):
         lib().monte_clean_lib_stuff(self.param)
+
+
+class Model:
+    """uFEM's input file read by the library (femReadInput for the brick
+    path): preparsed (-p, positions) or with IDs renumbered to positions."""
+
+    def __init__(self, path: str, preparsed: bool = True):
+        rv = C.c_int(0)
+        self.h = lib().ufem_model_read(os.fsencode(path), 1 if preparsed else 0, C.byref(rv))
+        self.rv = rv.value
+        if not self.h:
+            raise RuntimeError(f"ufem_model_read({path}) = {rv.value}: {last_error()}")
+        m = _Mesh()
+        lib().ufem_model_mesh(self.h, C.byref(m))
+        nn, ne, nl = m.n_nodes, m.n_elems, m.n_nl
+
+        def arr(p, k, dt):
+            return np.ctypeslib.as_array(p, shape=(k,)).astype(dt, copy=True) if k > 0 else np.zeros(0, dt)
+        from .mesh import Mesh
+        coords = np.stack([arr(m.x, nn, np.float64), arr(m.y, nn, np.float64), arr(m.z, nn, np.float64)], axis=1)
+        self.mesh = Mesh(0, 0, 0, coords, arr(m.enodes, 8 * ne, np.int64).reshape(ne, 8), arr(m.E, ne, np.float64),
+                         arr(m.nu, ne, np.float64), arr(m.nl_node, nl, np.int64), arr(m.nl_type, nl, np.int64),
+                         arr(m.nl_dir, nl, np.int64), arr(m.nl_val, nl, np.float64))
+        self.node_ids = arr(lib().ufem_model_node_ids(self.h), nn, np.int64)
+        self.elem_ids = arr(lib().ufem_model_elem_ids(self.h), ne, np.int64)
+        self.elem_mat = arr(lib().ufem_model_elem_mat(self.h), ne, np.int64)
+        self.elem_rset = arr(lib().ufem_model_elem_rset(self.h), ne, np.int64)
+        self.sol_id = lib().ufem_model_sol_id(self.h)
+        self.sol_dep = lib().ufem_model_sol_dep(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ufem_model_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_model(path: str, preparsed: bool = True) -> Model:
+    return Model(path, preparsed)
+
+
+def model_error(path: str, preparsed: bool = True):
+    """the AF_* code and message of reading a file that fails"""
+    rv = C.c_int(0)
+    h = lib().ufem_model_read(os.fsencode(path), 1 if preparsed else 0, C.byref(rv))
+    if h:
+        lib().ufem_model_free(h)
+        return 0, ""
+    return rv.value, last_error()
 
 
 def dense_spd_inverse(A):
