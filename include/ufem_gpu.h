@@ -257,8 +257,17 @@ void ufem_batch_shape(const ufem_batch *b, int *wg, int *grid, int *team_big);
 void ufem_batch_set_shape(ufem_batch *b, int wg, int grid, int team_big);
 /* workgroups that solved each sample in the last solve */
 int  ufem_batch_teams(const ufem_batch *b, int *team);
+/* every team wait of the launch gives up after wait_ms (<= 0: 10 s), and
+ * the host stops a solve that has run timeout_ms (<= 0: never) -- the
+ * kernel then ends normally, with status 4 for the samples it stopped */
+void ufem_batch_set_limits(ufem_batch *b, double wait_ms, double timeout_ms);
+/* stop a running ufem_batch_solve (call from another host thread) */
+void ufem_batch_abort(ufem_batch *b);
 /* solve all samples; per-sample outputs may be NULL; maxit <= 0: largest n.
- * sample_ms: device wall-clock time from a sample's start to its end */
+ * sample_ms: device wall-clock time from a sample's start to its end.
+ * status per sample: 1 converged, 2 breakdown, 3 maxit, 4 stopped (a team
+ * wait gave up or the host stopped the solve), 0 never started; AF_ERR
+ * unless every sample converged */
 int  ufem_batch_solve(ufem_batch *b, double eps, long maxit, double *total_ms, long *iters,
                       double *relres, int *status, double *sample_ms);
 int  ufem_batch_get_u(ufem_batch *b, int i, double *u);

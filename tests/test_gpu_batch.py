@@ -78,3 +78,36 @@ def test_batch_teams_agree(team):
     for i in range(len(meshes)):
         assert np.linalg.norm(b.u(i) - u1[i]) <= 1e-9 * np.linalg.norm(u1[i])
         assert abs(int(r1["iters"][i]) - int(r2["iters"][i])) <= 3
+
+
+def test_stopped_batch_ends_normally():
+    """The team waits of the batched kernel are bounded: a host that gives up
+    waiting (timeout) and a wait limit too short for any team both end the
+    launch normally, with the stopped samples reported (status 4 or not
+    started) and AF_ERR; the same batch then solves in full again."""
+    from ufem_amd.mesh import batch_meshes
+    meshes = batch_meshes(64, seed=3)
+    b = G.GpuBatch(meshes)
+    b.assemble(2)
+    b.set_shape(512, 64, 8)                 # teams of up to 8 workgroups on the largest meshes
+    full = b.solve(eps=1e-8)
+    assert full["rv"] == 0 and np.all(full["status"] == 1)
+    b.set_limits(wait_ms=0, timeout_ms=1.0)   # the host stops the solve after 1 ms
+    r = b.solve(eps=1e-8)
+    print("host stop:", np.bincount(r["status"], minlength=5), G.last_error())
+    assert r["rv"] == -1 and "host gave up" in G.last_error()
+    assert np.any(r["status"] != 1)
+    b.set_limits(wait_ms=1e-6, timeout_ms=0)  # every team wait gives up at once
+    r = b.solve(eps=1e-8)
+    print("wait limit:", np.bincount(r["status"], minlength=5), G.last_error())
+    assert r["rv"] == -1 and np.any(r["status"] == 4)
+    b.set_limits(0, 0)
+    again = b.solve(eps=1e-8)
+    assert again["rv"] == 0 and np.all(again["status"] == 1)
+    for i in (0, 13, 63):
+        g = G.GpuSystem(meshes[i])
+        g.assemble(2)
+        g.solve("pcg", eps=1e-12)
+        assert np.linalg.norm(b.u(i) - g.u()) <= 1e-7 * np.linalg.norm(g.u())
+        g.close()
+    b.close()

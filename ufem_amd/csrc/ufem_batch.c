@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "ufem_fem_math.h"
 #include "ufem_internal.h"
@@ -38,6 +39,11 @@ struct ufem_batch {
   int assembled;
   int wg, grid;           /* launch shape (0: not tuned yet) */
   int team_big;           /* team size of the largest class */
+  int *h_abort;           /* mapped host flag: the kernel stops when it is set */
+  int *d_abort;
+  double wait_ms;         /* a team wait gives up after this long */
+  double timeout_ms;      /* the host sets h_abort when a solve takes longer (0: never) */
+  int aborted;            /* the last solve was stopped */
 };
 
 static int gpu_fail(const char *what)
@@ -83,7 +89,22 @@ void ufem_batch_destroy(ufem_batch *b)
   ugk_event_destroy(b->ev0);
   ugk_event_destroy(b->ev1);
   ugk_stream_destroy(b->stream);
+  ugk_host_free(b->h_abort);
   free(b);
+}
+
+/* a team wait gives up after wait_ms (<= 0: 10 s); the host stops a solve
+ * that has run timeout_ms (<= 0: never) by setting the kernel's stop flag */
+void ufem_batch_set_limits(ufem_batch *b, double wait_ms, double timeout_ms)
+{
+  b->wait_ms = wait_ms > 0 ? wait_ms : 10000.0;
+  b->timeout_ms = timeout_ms > 0 ? timeout_ms : 0.0;
+}
+
+/* set the stop flag of a running solve (from another host thread) */
+void ufem_batch_abort(ufem_batch *b)
+{
+  if (b->h_abort) __atomic_store_n(b->h_abort, 1, __ATOMIC_SEQ_CST);
 }
 
 ufem_batch *ufem_batch_create(int n, const ufem_mesh *meshes)
@@ -142,9 +163,11 @@ ufem_batch *ufem_batch_create(int n, const ufem_mesh *meshes)
   b->d_tctr = (unsigned *)ugk_malloc(sizeof(unsigned) * (size_t)n);
   b->d_tpart = (double *)ugk_malloc(sizeof(double) * 64 * (size_t)n);
   b->d_res = (ugk_bresult *)ugk_malloc(sizeof(ugk_bresult) * (size_t)n);
+  b->h_abort = (int *)ugk_host_alloc_mapped(64, (void **)&b->d_abort);
+  b->wait_ms = 10000.0;
   b->team_big = 1;
   if (!b->stream || !b->ev0 || !b->ev1 || !b->d_samples || !b->d_slots || !b->d_ticket || !b->d_tctr ||
-      !b->d_tpart || !b->d_res) {
+      !b->d_tpart || !b->d_res || !b->h_abort) {
     gpu_fail("batch allocation");
     goto fail;
   }
@@ -284,19 +307,54 @@ static int build_slots(ufem_batch *b, int team_big, int grid, int budget, int *n
   return AF_OK;
 }
 
+static double wall_ms(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
+
 static int run_batch(ufem_batch *b, double eps, long maxit, int wg, int grid, int team_big, int budget,
                      double *ms)
 {
-  int ns = 0;
+  int ns = 0, i;
+  const long khz = ugk_wall_clock_khz();
+  const long wait_ticks = khz > 0 ? (long)ceil(b->wait_ms * (double)khz) : 0;   /* at least 1 tick */
   if (build_slots(b, team_big, grid, budget, &ns) != AF_OK) return AF_ERR;
+  __atomic_store_n(b->h_abort, 0, __ATOMIC_SEQ_CST);
+  b->aborted = 0;
+  memset(b->h_res, 0, sizeof(ugk_bresult) * (size_t)b->n);
   if (ugk_zero(b->d_ticket, 64, b->stream) || ugk_zero(b->d_tctr, sizeof(unsigned) * (size_t)b->n, b->stream) ||
+      ugk_h2d(b->d_res, b->h_res, sizeof(ugk_bresult) * (size_t)b->n, b->stream) ||
       ugk_event_record(b->ev0, b->stream) ||
       ugk_batch_pcg(b->d_samples, b->d_slots, ns, b->d_ticket, b->d_tctr, b->d_tpart, b->d_res, eps, maxit, wg,
-                    grid, b->stream) ||
-      ugk_event_record(b->ev1, b->stream) || ugk_sync(b->stream))
+                    grid, b->d_abort, wait_ticks, b->stream) ||
+      ugk_event_record(b->ev1, b->stream))
     return gpu_fail("batched solve");
+  if (b->timeout_ms > 0) {            /* the host gives up a bounded wait */
+    const double t0 = wall_ms();
+    int d;
+    while ((d = ugk_event_done(b->ev1)) == 0) {
+      if (wall_ms() - t0 > b->timeout_ms) {
+        __atomic_store_n(b->h_abort, 1, __ATOMIC_SEQ_CST);
+        b->aborted = 1;
+        break;
+      }
+      struct timespec nap = {0, 200000};
+      nanosleep(&nap, NULL);
+    }
+    if (d < 0) return gpu_fail("batched solve");
+  }
+  if (ugk_sync(b->stream)) return gpu_fail("batched solve");
   *ms = ugk_event_ms(b->ev0, b->ev1);
   if (ugk_d2h(b->h_res, b->d_res, sizeof(ugk_bresult) * (size_t)b->n, b->stream)) return gpu_fail("results");
+  for (i = 0; i < b->n; i++)
+    if (b->h_res[i].status < 1 || b->h_res[i].status > 3) b->aborted = 1;
+  if (b->aborted) {
+    ufem_set_error("batch: stopped before every sample was solved (%s)",
+                   __atomic_load_n(b->h_abort, __ATOMIC_SEQ_CST) ? "the host gave up waiting" : "a team wait gave up");
+    return AF_ERR;
+  }
   return AF_OK;
 }
 
@@ -384,7 +442,10 @@ int ufem_batch_solve(ufem_batch *b, double eps, long maxit, double *total_ms, lo
     if (ugk_device_info(&ncu, &hbm, dev, sizeof(dev))) return gpu_fail("device");
     grid = ncu * (ugk_batch_occupancy(wg) >= 2 ? 2 : 1);
   }
-  if (run_batch(b, eps, mx, wg, grid, b->team_big, 0, &ms) != AF_OK) return AF_ERR;
+  {
+    int rv = run_batch(b, eps, mx, wg, grid, b->team_big, 0, &ms);
+    if (rv != AF_OK && !b->aborted) return rv;   /* a stopped solve still reports its samples */
+  }
   if (total_ms) *total_ms = ms;
   khz = (double)ugk_wall_clock_khz();
   for (i = 0; i < b->n; i++) {

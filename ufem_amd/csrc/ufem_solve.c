@@ -243,6 +243,10 @@ static int solve_pcg(const ugk_mat *K, long n, const double *b, double *x, ufem_
     if (rv != AF_OK) return rv;
   }
   if (get_state(w, stream)) return AF_ERR;
+  if (w->st_host->err == 5) {
+    ufem_set_error("persistent PCG: a block did not reach the grid barrier (gave up)");
+    return AF_ERR;
+  }
   if (w->st_host->bb <= 0.0) {   /* zero load: the reference returns x = 0 */
     if (ugk_zero(x, sizeof(double) * (size_t)n, stream)) return gpu_fail("zero x");
     info->status = AF_OK;

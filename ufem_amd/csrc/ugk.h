@@ -81,7 +81,8 @@ typedef struct {
   long fb_iter;       /* flexible CG: iteration of the Jacobi fallback    */
   int fb;             /* flexible CG: 1 once the network is switched off  */
   int halt;           /* flexible CG: done or fallback (network skipped)  */
-  unsigned int counter[8];  /* last-block tickets, reset by the last block */
+  unsigned int counter[10]; /* last-block tickets, reset by the last block;
+                               6..8: the persistent kernel's grid barrier */
 } ugk_state;
 
 /* ---------------- fixed workspace for callers that cannot allocate -------- */
@@ -111,6 +112,11 @@ int   ugk_event_record(void *ev, void *stream);
 float ugk_event_ms(void *ev0, void *ev1);        /* after both completed */
 int   ugk_device_info(int *n_cu, long *hbm_bytes, char *name, int name_len);
 void *ugk_host_alloc(size_t bytes);              /* pinned host memory */
+/* pinned, coherent host memory the device reads while a kernel runs; *dptr
+ * gets its device address (free with ugk_host_free) */
+void *ugk_host_alloc_mapped(size_t bytes, void **dptr);
+/* 1 once the event's work is done, 0 while pending, -1 on error */
+int   ugk_event_done(void *ev);
 void  ugk_host_free(void *p);
 int   ugk_d2h_async(void *h, const void *d, size_t bytes, void *stream);
 int   ugk_event_sync(void *ev);
@@ -258,10 +264,14 @@ typedef struct {
  * d_tctr[sample] when the team has more than one workgroup (d_tpart:
  * 64 doubles per sample).  d_tctr must be zero at launch.  The grid must not
  * exceed the residency (ugk_batch_occupancy), so that every team forms.
- * wg: 256, 512 or 1024 threads. */
+ * wg: 256, 512 or 1024 threads.  A team wait gives up after wait_ticks of
+ * the device wall clock (<= 0: no practical limit) or when the int at
+ * d_abort (mapped host memory, may be NULL) becomes nonzero; the sample then
+ * reports status 4, and no workgroup takes a new sample after a stop.
+ * d_ticket holds 2 zeroed words (ticket, device stop flag). */
 int ugk_batch_pcg(const ugk_bsample *d_samples, const int *d_slots, int n_slots, unsigned *d_ticket,
                   unsigned *d_tctr, double *d_tpart, ugk_bresult *d_res, double tol, long maxit,
-                  int wg, int grid, void *stream);
+                  int wg, int grid, const int *d_abort, long wait_ticks, void *stream);
 /* blocks of `wg` threads the batch kernel can keep resident per CU */
 int ugk_batch_occupancy(int wg);
 /* dinv = 1 / diag(K) for one block-3 matrix (no state) */

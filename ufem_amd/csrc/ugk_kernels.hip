@@ -1064,21 +1064,37 @@ __global__ __launch_bounds__(WG) void k_pcg_b(long n, double *__restrict__ x, do
 // ticket per block on `bar[0]`; the last arrival resets it and bumps the
 // generation `bar[1]` that the others wait on.  Every wait ends because the
 // launch is cooperative (all blocks resident) and every block arrives.
-__device__ __forceinline__ void grid_barrier(unsigned int *bar, unsigned G)
+// The wait is bounded by `limit` device wall-clock ticks: a block that gives
+// up raises bar[2], which every later wait sees at once, and returns false.
+__device__ __forceinline__ bool grid_barrier(unsigned int *bar, unsigned G, long limit)
 {
+  __shared__ int s_ok;
   __syncthreads();
   if (threadIdx.x == 0) {
+    int ok = 1;
     const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned t = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (t == G - 1) {
       __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen)
+      const long t0 = (long)wall_clock64();
+      unsigned spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
         __builtin_amdgcn_s_sleep(1);
+        if ((++spins & 31u) == 0u &&
+            (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+             (long)wall_clock64() - t0 > limit)) {
+          __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+      }
     }
+    s_ok = ok;
   }
   __syncthreads();
+  return s_ok != 0;
 }
 
 // whole PCG in one cooperative launch; same arithmetic as k_pcg_a/k_pcg_b
@@ -1089,7 +1105,7 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
                                                        long n, double *x, double *r, double *z,
                                                        double *p0, double *p1, double *q,
                                                        const double *__restrict__ dinv,
-                                                       ugk_state *st, double *part, unsigned GS)
+                                                       ugk_state *st, double *part, unsigned GS, long limit)
 {
   cg::grid_group grid = cg::this_grid();
   const unsigned G = gridDim.x;
@@ -1143,7 +1159,14 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
       }
     }
     block_publish<1>(v, partA);
-    if (CGSYNC) grid.sync(); else grid_barrier(&st->counter[6], G);
+    if (CGSYNC) grid.sync();
+    else if (!grid_barrier(&st->counter[6], G, limit)) {
+      if (threadIdx.x == 0) {          // a block missing from the barrier: end with an error
+        st->err = 5;
+        st->done = 2;
+      }
+      return;
+    }
     double tA[1];
     grid_partials_sum<1>(partA, G, tA);
     const double pq = tA[0];
@@ -1161,7 +1184,14 @@ __global__ __launch_bounds__(WG) void k_pcg_persistent(SellView M, const double 
       w[2] += xi * xi;
     }
     block_publish<3>(w, partB);
-    if (CGSYNC) grid.sync(); else grid_barrier(&st->counter[6], G);
+    if (CGSYNC) grid.sync();
+    else if (!grid_barrier(&st->counter[6], G, limit)) {
+      if (threadIdx.x == 0) {          // a block missing from the barrier: end with an error
+        st->err = 5;
+        st->done = 2;
+      }
+      return;
+    }
     double tB[3];
     grid_partials_sum<3>(partB, G, tB);
     it++;
@@ -1578,35 +1608,65 @@ __device__ __forceinline__ void wg_sum(double (&v)[NV], double (*red)[NV])
   __syncthreads();
 }
 
+// Stop request of a batched launch: the host's flag (mapped host memory,
+// written while the kernel runs) or the device flag a workgroup raised when
+// one of its waits gave up.
+struct BatchStop {
+  const int *host;          // may be NULL
+  unsigned int *dev;
+  long limit;               // ticks of the device wall clock a wait may take
+};
+__device__ __forceinline__ bool stop_asked(const BatchStop &b)
+{
+  return (b.host && __hip_atomic_load(b.host, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) ||
+         __hip_atomic_load(b.dev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+
 // Team of T workgroups on one sample: the slices and vector entries are
 // dealt round robin over the team; sums go through per-rank partial slots
 // (two buffers, by reduction parity) and a monotonic team counter: barrier k
 // waits until the counter reaches T k.  A team forms only from resident
-// workgroups (the grid never exceeds residency), so every wait ends.
-__device__ __forceinline__ void team_barrier(unsigned int *ctr, unsigned target)
+// workgroups (the grid never exceeds residency), so every wait ends; the
+// wait is still bounded -- by the wall-clock limit and by a stop request --
+// so that a miscount ends the launch with an error instead of hanging it.
+// Returns false (in every thread) when the wait gave up.
+__device__ __forceinline__ bool team_barrier(unsigned int *ctr, unsigned target, const BatchStop &bs)
 {
+  __shared__ int s_ok;
   __syncthreads();
   if (threadIdx.x == 0) {
+    int ok = 1;
     __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+    const long t0 = (long)wall_clock64();
+    unsigned spins = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(2);
+      if ((++spins & 31u) == 0u && (stop_asked(bs) || (long)wall_clock64() - t0 > bs.limit)) {
+        ok = 0;
+        __hip_atomic_store(bs.dev, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    s_ok = ok;
   }
   __syncthreads();
+  return s_ok != 0;
 }
 
 template <int WGS, int NV>
-__device__ __forceinline__ void team_sum(double (&v)[NV], double (*red)[NV], int T, int rank,
-                                         double *tpart, unsigned int *ctr, unsigned &nbar, int &par)
+__device__ __forceinline__ bool team_sum(double (&v)[NV], double (*red)[NV], int T, int rank,
+                                         double *tpart, unsigned int *ctr, unsigned &nbar, int &par,
+                                         const BatchStop &bs)
 {
   wg_sum<WGS, NV>(v, red);
-  if (T == 1) return;
+  if (T == 1) return true;
   double *slot = tpart + (size_t)par * 8 * 4;   // [2][8 ranks][4]
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; k++) slot[rank * 4 + k] = v[k];
   }
   nbar++;
-  team_barrier(ctr, nbar * (unsigned)T);
+  if (!team_barrier(ctr, nbar * (unsigned)T, bs)) return false;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double a = 0.0;
@@ -1614,6 +1674,7 @@ __device__ __forceinline__ void team_sum(double (&v)[NV], double (*red)[NV], int
     v[k] = a;
   }
   par ^= 1;
+  return true;
 }
 
 template <int WGS>
@@ -1621,7 +1682,7 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
                                                    const int2 *__restrict__ slots, int nslots,
                                                    unsigned *ticket, unsigned int *tctr,
                                                    double *tpart, ugk_bresult *res, double tol,
-                                                   long maxit)
+                                                   long maxit, BatchStop bs)
 {
   constexpr int W = WGS / 64;
   __shared__ int s_k;
@@ -1629,7 +1690,8 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
   __shared__ double red1[W][1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (;;) {
-    if (threadIdx.x == 0) s_k = (int)atomicAdd(ticket, 1u);
+    // no new sample once a stop was asked for
+    if (threadIdx.x == 0) s_k = stop_asked(bs) ? nslots : (int)atomicAdd(ticket, 1u);
     __syncthreads();
     const int k = s_k;
     __syncthreads();
@@ -1660,12 +1722,21 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
       v3[0] += ri * ui;
       v3[1] += ri * ri;
     }
-    team_sum<WGS, 3>(v3, red3, T, rank, tp, ctr, nbar, par);   // also publishes u
+    int status = 0;
+    const bool formed = team_sum<WGS, 3>(v3, red3, T, rank, tp, ctr, nbar, par, bs);   // also publishes u
     double gam = v3[0], rr = v3[1], gam_old = 1.0, alf_old = 1.0;
     const double nb = sqrt(rr);
     long it = 0;
-    int status = 0;
-    for (;;) {
+    if (!formed) status = 4;
+    while (status == 0) {
+      if ((it & 15) == 15) {           // a stop asked for ends a sample too
+        __shared__ int s_stop;
+        if (threadIdx.x == 0) s_stop = stop_asked(bs) ? 1 : 0;
+        __syncthreads();
+        const int stop = s_stop;
+        __syncthreads();
+        if (stop) { status = 4; break; }
+      }
       if (!(nb > 0.0) || sqrt(rr) <= tol * nb) { status = 1; break; }
       if (!(gam == gam) || !(rr == rr)) { status = 2; break; }
       if (it >= mxit) { status = 3; break; }
@@ -1692,7 +1763,7 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
           v1[0] += a0 * S.u[3 * row] + a1 * S.u[3 * row + 1] + a2 * S.u[3 * row + 2];
         }
       }
-      team_sum<WGS, 1>(v1, red1, T, rank, tp, ctr, nbar, par);   // publishes w
+      if (!team_sum<WGS, 1>(v1, red1, T, rank, tp, ctr, nbar, par, bs)) { status = 4; break; }   // publishes w
       const double dlt = v1[0];
       const double beta = (it == 0) ? 0.0 : gam / gam_old;
       const double den = (it == 0) ? dlt : dlt - beta * gam / alf_old;
@@ -1712,7 +1783,7 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
         w3[0] += ri * ui;
         w3[1] += ri * ri;
       }
-      team_sum<WGS, 3>(w3, red3, T, rank, tp, ctr, nbar, par);   // publishes u
+      if (!team_sum<WGS, 3>(w3, red3, T, rank, tp, ctr, nbar, par, bs)) { status = 4; break; }   // publishes u
       gam_old = gam;
       alf_old = alpha;
       gam = w3[0];
@@ -1730,6 +1801,7 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
       res[id] = R;
     }
     __syncthreads();
+    if (status == 4) return;           // this workgroup takes no further sample
   }
 }
 
@@ -3076,6 +3148,24 @@ void ugk_host_free(void *p)
 {
   if (p) (void)hipHostFree(p);
 }
+void *ugk_host_alloc_mapped(size_t bytes, void **dptr)
+{
+  void *p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+  if (e != hipSuccess) { fail("hipHostMalloc(mapped)", e); return nullptr; }
+  memset(p, 0, bytes);
+  e = hipHostGetDevicePointer(dptr, p, 0);
+  if (e != hipSuccess) { fail("hipHostGetDevicePointer", e); (void)hipHostFree(p); return nullptr; }
+  return p;
+}
+int ugk_event_done(void *e)
+{
+  hipError_t r = hipEventQuery((hipEvent_t)e);
+  if (r == hipSuccess) return 1;
+  if (r == hipErrorNotReady) return 0;
+  fail("hipEventQuery", r);
+  return -1;
+}
 int ugk_d2h_async(void *h, const void *d, size_t bytes, void *s)
 {
   CK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, S(s)));
@@ -3383,9 +3473,12 @@ int ugk_pcg_persistent(const ugk_mat *K, long n, double *x, double *r, double *z
   const double *vals = K->d_vals;
   unsigned GS = spmv_blocks(K);
   if (G > 16384) G = 16384;
+  /* the grid barrier gives up after about 10 s of the device wall clock */
+  long limit = ugk_wall_clock_khz() * 10000L;
+  if (limit <= 0) limit = 1L << 40;
   void *args[] = {(void *)&M, (void *)&vals, (void *)&n, (void *)&x, (void *)&r, (void *)&z,
                   (void *)&p0, (void *)&p1, (void *)&q, (void *)&dinv, (void *)&st, (void *)&part,
-                  (void *)&GS};
+                  (void *)&GS, (void *)&limit};
   CK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(WG), args, 0, S(s)));
   return 0;
 }
@@ -3487,13 +3580,18 @@ int ugk_batch_occupancy(int wg)
 }
 
 int ugk_batch_pcg(const ugk_bsample *samples, const int *slots, int nslots, unsigned *ticket, unsigned *tctr,
-                  double *tpart, ugk_bresult *res, double tol, long maxit, int wg, int grid, void *s)
+                  double *tpart, ugk_bresult *res, double tol, long maxit, int wg, int grid, const int *d_abort,
+                  long wait_ticks, void *s)
 {
   if (grid < 1) grid = g_ncu;
   const int2 *sl = (const int2 *)slots;
-  if (wg == 256) k_batch_pcg<256><<<grid, 256, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit);
-  else if (wg == 512) k_batch_pcg<512><<<grid, 512, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit);
-  else if (wg == 1024) k_batch_pcg<1024><<<grid, 1024, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit);
+  BatchStop bs;
+  bs.host = d_abort;
+  bs.dev = ticket + 1;                 // zeroed with the ticket
+  bs.limit = wait_ticks > 0 ? wait_ticks : (1L << 40);
+  if (wg == 256) k_batch_pcg<256><<<grid, 256, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit, bs);
+  else if (wg == 512) k_batch_pcg<512><<<grid, 512, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit, bs);
+  else if (wg == 1024) k_batch_pcg<1024><<<grid, 1024, 0, S(s)>>>(samples, sl, nslots, ticket, tctr, tpart, res, tol, maxit, bs);
   else { snprintf(g_err, sizeof(g_err), "workgroup size %d not built", wg); return -1; }
   CK_LAUNCH("k_batch_pcg");
   return 0;

@@ -128,6 +128,8 @@ def lib():
             "ufem_batch_teams": (C.c_int, [C.c_void_p, P_I]),
             "ufem_batch_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_long, P_D, P_L, P_D, P_I, P_D]),
             "ufem_batch_get_u": (C.c_int, [C.c_void_p, C.c_int, P_D]),
+            "ufem_batch_set_limits": (None, [C.c_void_p, C.c_double, C.c_double]),
+            "ufem_batch_abort": (None, [C.c_void_p]),
             "femEqsCGwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
                                      C.c_double, C.c_long]),
             "femEqsBiCCSwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
@@ -597,6 +599,10 @@ class GpuBatch:
         lib().ufem_batch_teams(self.h, _pi(team))
         return {"rv": rv, "total_ms": ms.value, "iters": it, "relres": rr, "status": st, "sample_ms": sm,
                 "wg": wg.value, "grid": grid.value, "team_big": tb.value, "team": team}
+
+    def set_limits(self, wait_ms=0.0, timeout_ms=0.0):
+        """team waits give up after wait_ms; the host stops a solve after timeout_ms"""
+        lib().ufem_batch_set_limits(self.h, wait_ms, timeout_ms)
 
     def set_shape(self, wg, grid, team_big):
         lib().ufem_batch_set_shape(self.h, wg, grid, team_big)
