@@ -546,6 +546,11 @@ def main():
     ap.add_argument("--net-kernels-only", action="store_true",
                     help="launch only the network's inference stages on the 128x32x32 random-field "
                          "cantilever (initial weights; for kernel traces)")
+    ap.add_argument("--ssor-only", action="store_true",
+                    help="two SSOR-CG solves of the metric's system (for kernel traces)")
+    ap.add_argument("--learned-solve-only", action="store_true",
+                    help="flexible-CG solves with the (untrained) network and with the two-level "
+                         "correction on the metric's system (for kernel traces)")
     ap.add_argument("--cpu-baseline-only", action="store_true",
                     help="(internal) run the CPU baseline leg alone and print its JSON")
     args = ap.parse_args()
@@ -577,7 +582,8 @@ def main():
 
     nx, ny, nz = (int(v) for v in args.size.split(","))
     cpu_proc = None
-    if not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.plain_only):
+    if not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.plain_only or args.ssor_only
+                                or args.learned_solve_only):
         cpu_proc = start_cpu_baseline(args)
         log("CPU baseline started in a child process")
     t0 = time.perf_counter()
@@ -600,6 +606,23 @@ def main():
     if args.kernels_only:
         g.time_kernel("copy", reps=5)
         g.time_kernel("cg1_w", reps=5)
+        g.close()
+        return
+    if args.ssor_only:
+        for _ in range(2):
+            r = g.solve(solver="ssor", eps=1e-8, rule=1)
+            log(f"SSOR-CG: {r['iters']} iterations, {r['solve_ms']:.2f} ms device")
+        g.close()
+        return
+    if args.learned_solve_only:
+        net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=1)
+        net.init(1)
+        net.bind(g, mesh.E)
+        g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 10, 10)
+        for name, kw in (("learned", {}), ("learned_mixed", {}), ("learned_mixed", {"coarse": True})):
+            r = g.solve(solver=name, eps=1e-8, rule=1, net=net, maxit=4000, **kw)
+            log(f"{name} {kw}: {r['iters']} iterations, {r['solve_ms']:.2f} ms device")
+        net.close()
         g.close()
         return
     if args.twolevel_only:

@@ -110,6 +110,10 @@ static int run_chunks(ufem_work *w, void *stream, long maxit, int chunk, iter_fn
   int rv = AF_OK, first = 1;
   if (chunk < 2) chunk = 2;
   chunk += chunk & 1;                    /* even: ping-pong parity repeats */
+  {                                      /* UFEM_GRAPHS=0: direct launches (kernel traces) */
+    const char *e = getenv("UFEM_GRAPHS");
+    if (e && e[0] == '0') use_graph = 0;
+  }
   ev = ugk_event_create();
   if (!ev) return gpu_fail("event");
   if (w->hook && w->hook(w->hook_ctx, 0, stream)) { rv = gpu_fail("chunk hook"); goto out; }
