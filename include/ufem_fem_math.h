@@ -30,6 +30,10 @@
 extern "C" {
 #endif
 
+/* A source of the reference that has included its own fem_math.h (and with
+ * it fem.h) already holds these definitions, with the same values and
+ * members; they are skipped there so that it can include this header. */
+#ifndef __FEM_MATH_H__
 #define MAT_FULL 0
 #define MAT_SPAR 1
 #define VEC_FULL 0
@@ -65,6 +69,7 @@ typedef struct
   long   *pos;
   double *data;
 } tVector;
+#endif
 
 /* Jacobi-preconditioned CG.  a, b read-only; x is the start vector in and the
  * solution out.  Stops when ||r|| <= eps (||A|| ||x|| + ||b||) with ||A|| the
@@ -78,8 +83,12 @@ int femEqsCGwJ(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt);
 /* Jacobi (right-) preconditioned BiCGStab, same arguments.  Convergence is
  * tested on the true residual b - A x, which is what the reference documents;
  * its own test passes (b, x) swapped to eqsCompResid and effectively stops
- * only when ||s|| <= 1e-17 (fem/fem_eqs.c:425).  Unlike the reference, which
- * always returns AF_OK (fem/fem_eqs.c:467), non-convergence returns AF_ERR.
+ * only when ||s|| <= 1e-17 (fem/fem_eqs.c:425).  As the reference does
+ * (fem/fem_eqs.c:441-467), a run that ends unconverged (maxIt reached, or a
+ * stop the true residual did not confirm after the restarts) keeps its last
+ * x and returns AF_OK, with a line on stderr; ufem_gpu_last_solve() then
+ * reports status AF_ERR.  A diagonal smaller than 1e-17 returns AF_ERR_ZER
+ * (the reference jumps to its exit there and returns AF_OK, x untouched).
  * Replaces femEqsBiCCSwJ, fem/fem_eqs.c:262-468 (fem/fem_math.h:155). */
 int femEqsBiCCSwJ(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt);
 

@@ -84,9 +84,24 @@ void ufem_gpu_last_solve(ufem_solve_info *info)
   if (info) *info = g_last;
 }
 
+/* UFEM_TRACE set to anything but "" or "0": one line on stderr per solver
+ * call (route, size, iterations, status, device time) */
+static int trace_on(void)
+{
+  static int t = -1;
+  if (t < 0) {
+    const char *e = getenv("UFEM_TRACE");
+    t = e && *e && strcmp(e, "0") != 0;
+  }
+  return t;
+}
+
+/* a failure of the device itself is printed as well as returned: the
+ * program that called the drop-in may print nothing but its own status */
 static int gpu_fail(const char *what)
 {
   ufem_set_error("%s: %s", what, ugk_error());
+  fprintf(stderr, "[ufem] GPU error: %s\n", ufem_gpu_last_error());
   return AF_ERR;
 }
 
@@ -642,7 +657,25 @@ static int served(const tMatrix *a, const tVector *b, const tVector *x)
   return a->type == MAT_SPAR && b->type == VEC_FULL && x->type == VEC_FULL;
 }
 
-static int dropin_solve(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt, int solver)
+static int dropin_solve_(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt, int solver);
+
+static int dropin_solve(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt, int solver,
+                        const char *name)
+{
+  int rv = dropin_solve_(a, b, x, eps, maxIt, solver);
+  if (trace_on())
+    fprintf(stderr, "[ufem] %s on the GPU: n %ld, %ld iterations, rv %d, %.2f ms\n", name, a->rows,
+            g_last.iters, rv, g_last.solve_ms);
+  return rv;
+}
+
+static solver_fn cpu_solver(solver_fn f, const char *name)
+{
+  if (f && trace_on()) fprintf(stderr, "[ufem] %s on the CPU (%s_cpu)\n", name, name);
+  return f;
+}
+
+static int dropin_solve_(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt, int solver)
 {
   ufem_solve_opts o;
   mirror *e;
@@ -686,30 +719,40 @@ static int dropin_solve(tMatrix *a, tVector *b, tVector *x, double eps, long max
 int femEqsCGwJ(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt)
 {
   if (!served(a, b, x)) {
-    solver_fn f = (solver_fn)CPU_FN(femEqsCGwJ);
+    solver_fn f = cpu_solver((solver_fn)CPU_FN(femEqsCGwJ), "femEqsCGwJ");
     return f ? f(a, b, x, eps, maxIt) : no_cpu("femEqsCGwJ");
   }
-  return dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_PCG);
+  return dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_PCG, "femEqsCGwJ");
 }
 
 /* femEqsCGwSSOR, fem/fem_eqs.c:999-1188 (colour-ordered sweeps, see header) */
 int femEqsCGwSSOR(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt)
 {
   if (!served(a, b, x)) {
-    solver_fn f = (solver_fn)CPU_FN(femEqsCGwSSOR);
+    solver_fn f = cpu_solver((solver_fn)CPU_FN(femEqsCGwSSOR), "femEqsCGwSSOR");
     return f ? f(a, b, x, eps, maxIt) : no_cpu("femEqsCGwSSOR");
   }
-  return dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_SSOR);
+  return dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_SSOR, "femEqsCGwSSOR");
 }
 
 /* femEqsBiCCSwJ, fem/fem_eqs.c:262-468 (true-residual test, see header) */
 int femEqsBiCCSwJ(tMatrix *a, tVector *b, tVector *x, double eps, long maxIt)
 {
+  int rv;
   if (!served(a, b, x)) {
-    solver_fn f = (solver_fn)CPU_FN(femEqsBiCCSwJ);
+    solver_fn f = cpu_solver((solver_fn)CPU_FN(femEqsBiCCSwJ), "femEqsBiCCSwJ");
     return f ? f(a, b, x, eps, maxIt) : no_cpu("femEqsBiCCSwJ");
   }
-  return dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_BICGSTAB);
+  rv = dropin_solve(a, b, x, eps, maxIt, UFEM_SOLVER_BICGSTAB, "femEqsBiCCSwJ");
+  /* the reference reports an unconverged run on msgout and returns AF_OK
+     with the last x (fem/fem_eqs.c:441-467); the status stays in
+     ufem_gpu_last_solve */
+  if (rv == AF_ERR && g_last.status == AF_ERR) {
+    fprintf(stderr, "[ufem] femEqsBiCCSwJ: unconverged solution (true relative residual %.3e after %ld "
+            "iterations)\n", g_last.true_relres, g_last.iters);
+    rv = AF_OK;
+  }
+  return rv;
 }
 
 /* ---- the "Big" kernels ------------------------------------------------- */
