@@ -266,7 +266,7 @@ def bench_dropin(mesh, native_s, native_ms, u64):
             "solve_ms_ratio_to_native": round(calls[-1]["solve_ms"] / native_ms, 3)}
 
 
-def bench_learned(fields, steps, seed):
+def bench_learned(fields, steps, seed, levels=3):
     """Learned preconditioner on the 128 x 32 x 32 GRF cantilever (421,443 DOF):
     pairs from plain-PCG solves on `fields` fresh fields (32 each), `steps`
     Adam steps, then plain PCG and flexible CG with the network on a field
@@ -280,7 +280,7 @@ def bench_learned(fields, steps, seed):
     mesh.E[:] = E_test
     s = G.GpuSystem(mesh)
     s.assemble(2)
-    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=4)
+    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=4, levels=levels)
     net.init(1)
     net.reserve(32 * fields)
     pairs, collect_s = Lr.collect(s, net, mesh, range(1000, 1000 + fields), 32, every=8, log=log)
@@ -310,6 +310,7 @@ def bench_learned(fields, steps, seed):
         "workload": (f"cantilever 2.0x0.4x0.4 m, {nx}x{ny}x{nz} bricks, {mesh.n_dof} DOF, E = exp(GRF, "
                      f"corr 0.2 m, log-std 0.5, mean 30 GPa); test field seed {seed}, training fields "
                      f"1000..{1000 + fields - 1}"),
+        "levels": levels, "parameters": int(net.P),
         "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
         "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
         "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
@@ -327,7 +328,7 @@ def bench_learned(fields, steps, seed):
     return out
 
 
-def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 10, 10)):
+def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 10, 10), levels=3):
     """Learned + mixed precision on the metric's own system `g` (the 1M-DOF
     lognormal cantilever, E = mesh.E): pairs from plain-PCG solves on `fields`
     fresh i.i.d. lognormal fields of the same distribution (seeds seed_base..),
@@ -341,7 +342,7 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
     from ufem_amd import learned as Lr
     from ufem_amd.mesh import lognormal_E
     E_test = mesh.E.copy()
-    net = G.GpuNet(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, batch=4)
+    net = G.GpuNet(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, batch=4, levels=levels)
     net.init(1)
     net.reserve(32 * fields)
     pairs, collect_s = Lr.collect(g, net, mesh, range(seed_base, seed_base + fields), 32, every=8, log=log,
@@ -390,6 +391,7 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
         "workload": (f"the metric's system: {mesh.nx}x{mesh.ny}x{mesh.nz} bricks, {mesh.n_dof} DOF, E lognormal "
                      f"COV {cov}; training fields: {fields} fresh lognormal fields, seeds "
                      f"{seed_base}..{seed_base + fields - 1}"),
+        "levels": levels, "parameters": int(net.P),
         "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
         "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
         "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
@@ -535,6 +537,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--learn-fields", type=int, default=63)
     ap.add_argument("--learn-steps", type=int, default=3000)
+    ap.add_argument("--net-levels", type=int, default=6, help="levels of the learned preconditioner's U-Net (2..6)")
+    ap.add_argument("--learned-1m-only", action="store_true",
+                    help="assemble the metric's system and run only the learned + mixed comparison")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the dominant solve kernel and a device copy of known size "
                          "(for hardware-counter runs)")
@@ -583,7 +588,7 @@ def main():
     nx, ny, nz = (int(v) for v in args.size.split(","))
     cpu_proc = None
     if not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.plain_only or args.ssor_only
-                                or args.learned_solve_only):
+                                or args.learned_solve_only or args.learned_1m_only):
         cpu_proc = start_cpu_baseline(args)
         log("CPU baseline started in a child process")
     t0 = time.perf_counter()
@@ -623,6 +628,11 @@ def main():
             r = g.solve(solver=name, eps=1e-8, rule=1, net=net, maxit=4000, **kw)
             log(f"{name} {kw}: {r['iters']} iterations, {r['solve_ms']:.2f} ms device")
         net.close()
+        g.close()
+        return
+    if args.learned_1m_only:
+        out = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels)
+        print(json.dumps({"learned_mixed_1m": out}), flush=True)
         g.close()
         return
     if args.twolevel_only:
@@ -758,13 +768,13 @@ def main():
     learned_mixed = None
     if not args.no_learned:
         log("learned + mixed precision at the metric's size ...")
-        learned_mixed = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps)
+        learned_mixed = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels)
         log("learned_mixed", learned_mixed)
 
     learned = None
     if not args.no_learned:
         log("learned preconditioner ...")
-        learned = bench_learned(args.learn_fields, args.learn_steps, 9999)
+        learned = bench_learned(args.learn_fields, args.learn_steps, 9999, levels=args.net_levels)
         log("learned", learned)
 
     batch = None

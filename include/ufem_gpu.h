@@ -277,21 +277,26 @@ int  ufem_batch_get_u(ufem_batch *b, int i, double *u);
  * of a box mesh of W x H x D nodes numbered x fastest (node = x + W (y + H z),
  * DOF 3 node + dir - 1, as ufem_sys numbers them).  Flexible CG with
  *   z = dinv r + mask dsq N(mask dsq r),  dsq = sqrt(dinv),
- * where N is a symmetric 3-level 3-D convolution U-Net (3x3x3 taps, no
- * bias): C0 3->16 channels, C1 16->32 and C2 32->64 with stride 2 (levels of
- * (n + 1) / 2 nodes per axis), N(v) = C0^T (g0 h0 + C1^T (g1 h1 +
- * C2^T (g2 h2))), h0 = C0 v, h1 = C1 h0, h2 = C2 h1, and per-node, per-channel
- * gates g_l = exp(b_l - (1 + a_l) ebar_l) of the log-modulus log(E / 30 GPa)
- * (node average of the bricks, then 3x3x3 means per coarser level).  N is
- * linear, symmetric and positive semi-definite, so M is SPD.  Jacobi PCG
- * takes over if a direction fails to descend.  Parameters (f32, packed):
- * W0 [27][16][3], W1 [27][32][16], W2 [27][64][32], then a0, b0 (16), a1, b1
- * (32), a2, b2 (64); taps t = 9 (dz+1) + 3 (dy+1) + (dx+1).  Weights enter
- * the matrix cores as bf16, activations as f32 (exact bf16 splits). */
+ * where N is a symmetric L-level 3-D convolution U-Net (3x3x3 taps, no
+ * bias; L = 2..6, 3 by default): C0 3->16 channels, C1 16->32, C2 32->64 and
+ * C3.. 64->64 with stride 2 (levels of (n + 1) / 2 nodes per axis),
+ * N(v) = C0^T (g0 h0 + C1^T (g1 h1 + ... + C_{L-1}^T (g_{L-1} h_{L-1}))),
+ * h0 = C0 v, h_l = C_l h_{l-1}, and per-node, per-channel gates
+ * g_l = exp(b_l - (1 + a_l) ebar_l) of the log-modulus log(E / 30 GPa) (node
+ * average of the bricks, then 3x3x3 means per coarser level).  N is linear,
+ * symmetric and positive semi-definite, so M is SPD.  Jacobi PCG takes over
+ * if a direction fails to descend.  Parameters (f32, packed): W0 [27][16][3],
+ * W1 [27][32][16], W2 [27][64][32], W3.. [27][64][64], then a_l, b_l per
+ * level (its channel count each); taps t = 9 (dz+1) + 3 (dy+1) + (dx+1).
+ * Weights enter the matrix cores as bf16, activations as f32 (exact bf16
+ * splits). */
 typedef struct ufem_net ufem_net;
-ufem_net *ufem_net_create(int W, int H, int D, int batch);  /* batch: training samples per step */
+ufem_net *ufem_net_create(int W, int H, int D, int batch);  /* 3 levels; batch: training samples per step */
+ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels);
 void ufem_net_destroy(ufem_net *N);
-long ufem_net_nparams(void);
+long ufem_net_nparams(void);                 /* of the 3-level network: 70,640 */
+long ufem_net_nparams_levels(int levels);    /* -1 outside 2..6 */
+int  ufem_net_levels(const ufem_net *N);
 int  ufem_net_init(ufem_net *N, unsigned long seed);   /* W ~ N(0, 1/(27 cin)), a = 0, b = -3 */
 int  ufem_net_set_weights(ufem_net *N, const float *w);
 int  ufem_net_get_weights(ufem_net *N, float *w);
@@ -328,8 +333,9 @@ int  ufem_net_train(ufem_net *N, long steps, int batch, double lr, unsigned long
  * x_f32 0 or 1) */
 int  ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin,
                          int cout, const float *x, const float *w, float *y, const float *dy, float *dw);
-/* average device time (us) of each of the 6 inference stages on the bound
- * system's grid: C0, C1, C2 (with the gate product), C2^T, C1^T, C0^T */
+/* average device time (us) of each of the 2 L inference stages on the bound
+ * system's grid: C0, C1, .., C_{L-1} (with the gate product), C_{L-1}^T, ..,
+ * C1^T, C0^T */
 int  ufem_net_time_stages(ufem_net *N, int reps, double *us);
 /* average device time (us) of `reps` launches of one such convolution */
 int  ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin, int cout,

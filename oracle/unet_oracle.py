@@ -11,16 +11,17 @@ solve (femEqsCGwJ, fem/fem_eqs.c:88-237).  Definition:
 * conv(X, W, s): 3x3x3 taps t = 9 (dz+1) + 3 (dy+1) + (dx+1), zero padding,
   no bias, W[t][co][ci]; stride 1: Y[v] = sum_t W[t] X[v + off_t]; stride 2:
   Y[c] = sum_t W[t] X[2c + off_t]; convT is its adjoint;
-* symmetric U-Net N with weights W0 (3->16, stride 1), W1 (16->32, stride 2),
-  W2 (32->64, stride 2) and per-channel gate parameters (a_l, b_l):
-      h0 = conv(v, W0, 1), h1 = conv(h0, W1, 2), h2 = conv(h1, W2, 2)
+* symmetric U-Net N of L levels (2..6, 3 by default) with weights W0 (3->16,
+  stride 1), W1 (16->32, stride 2), W2 (32->64, stride 2), W3.. (64->64,
+  stride 2) and per-channel gate parameters (a_l, b_l):
+      h0 = conv(v, W0, 1), h_l = conv(h_{l-1}, W_l, 2)
       g_l = exp(b_l - (1 + a_l) ebar_l)                (per node, per channel)
-      t2 = g2 h2
-      t1 = g1 h1 + convT(t2, W2, 2)
-      t0 = g0 h0 + convT(t1, W1, 2)
+      t_{L-1} = g_{L-1} h_{L-1}
+      t_l = g_l h_l + convT(t_{l+1}, W_{l+1}, 2)
       N(v) = convT(t0, W0, 1)
   so N = sum_l H_l^T diag(g_l) H_l with H_l = conv chains: linear in v,
-  symmetric and positive semi-definite;
+  symmetric and positive semi-definite; the number of levels is the number
+  of W entries of the parameter dict;
 * ebar_0 = log(E / 30 GPa) averaged over a node's bricks; ebar_{l+1} = mean
   of ebar_l over the 3x3x3 fine nodes around 2c (those inside the grid);
 * preconditioner: z = dinv r + mask dsq N(mask dsq r), dsq = sqrt(dinv);
@@ -32,8 +33,16 @@ from __future__ import annotations
 
 import numpy as np
 
-CH = (16, 32, 64)
-LAYERS = [("W0", 3, 16, 1), ("W1", 16, 32, 2), ("W2", 32, 64, 2)]
+CH_ALL = (16, 32, 64, 64, 64, 64)
+CI_ALL = (3, 16, 32, 64, 64, 64)
+
+
+def layers(levels=3):
+    return [(f"W{l}", CI_ALL[l], CH_ALL[l], 1 if l == 0 else 2) for l in range(levels)]
+
+
+CH = CH_ALL[:3]
+LAYERS = layers(3)
 OFFS = np.array([(dz, dy, dx) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)])
 E_REF = 30e9
 
@@ -113,60 +122,67 @@ def node_log_modulus(E_elem, nx, ny, nz):
     return acc / cnt
 
 
-def ebar_levels(e0):
-    e1 = pool(e0)
-    return [e0, e1, pool(e1)]
+def ebar_levels(e0, levels=3):
+    out = [e0]
+    for _ in range(1, levels):
+        out.append(pool(out[-1]))
+    return out
 
 
-def init_params(seed=0):
+def n_levels(P):
+    return sum(1 for k in P if k.startswith("W"))
+
+
+def init_params(seed=0, levels=3):
     rng = np.random.default_rng(seed)
     P = {}
-    for name, ci, co, _ in LAYERS:
+    for name, ci, co, _ in layers(levels):
         P[name] = rng.standard_normal((27, co, ci)) / np.sqrt(27 * ci)
-    for l, c in enumerate(CH):
-        P[f"a{l}"] = np.zeros(c)
-        P[f"b{l}"] = np.zeros(c)
+    for l in range(levels):
+        P[f"a{l}"] = np.zeros(CH_ALL[l])
+        P[f"b{l}"] = np.zeros(CH_ALL[l])
     return P
 
 
 def gates(P, eb):
-    return [np.exp(P[f"b{l}"] - (1.0 + P[f"a{l}"]) * eb[l][..., None]) for l in range(3)]
+    return [np.exp(P[f"b{l}"] - (1.0 + P[f"a{l}"]) * eb[l][..., None]) for l in range(n_levels(P))]
 
 
 def forward(P, v, eb):
     """v [D][H][W][3] -> N(v) [D][H][W][3], with the intermediate fields."""
-    s0 = v.shape[:3]
-    s1 = coarse_shape(s0)
+    L = n_levels(P)
     g = gates(P, eb)
     c = {"v": v, "g": g, "eb": eb}
-    c["h0"] = conv(v, P["W0"], 1)
-    c["h1"] = conv(c["h0"], P["W1"], 2)
-    c["h2"] = conv(c["h1"], P["W2"], 2)
-    c["t2"] = g[2] * c["h2"]
-    c["t1"] = g[1] * c["h1"] + conv_adjoint(c["t2"], P["W2"], 2, s1)
-    c["t0"] = g[0] * c["h0"] + conv_adjoint(c["t1"], P["W1"], 2, s0)
-    y = conv_adjoint(c["t0"], P["W0"], 1, s0)
+    h = [conv(v, P["W0"], 1)]
+    for l in range(1, L):
+        h.append(conv(h[-1], P[f"W{l}"], 2))
+    t = [None] * L
+    t[L - 1] = g[L - 1] * h[L - 1]
+    for l in range(L - 2, -1, -1):
+        t[l] = g[l] * h[l] + conv_adjoint(t[l + 1], P[f"W{l + 1}"], 2, h[l].shape[:3])
+    for l in range(L):
+        c[f"h{l}"], c[f"t{l}"] = h[l], t[l]
+    y = conv_adjoint(t[0], P["W0"], 1, v.shape[:3])
     return y, c
 
 
 def backward(P, c, dy):
     """gradient of <dy, N(v)> with respect to every parameter"""
-    g, s0 = c["g"], c["v"].shape[:3]
+    L = n_levels(P)
+    g = c["g"]
     G = {}
-    dt0 = conv(dy, P["W0"], 1)                   # adjoint of y = conv_adjoint(t0)
+    dt = [conv(dy, P["W0"], 1)]                  # adjoint of y = conv_adjoint(t0)
     G["W0"] = conv_wgrad(dy, c["t0"], 1)
-    dt1 = conv(dt0, P["W1"], 2)
-    G["W1"] = conv_wgrad(dt0, c["t1"], 2)
-    dt2 = conv(dt1, P["W2"], 2)
-    G["W2"] = conv_wgrad(dt1, c["t2"], 2)
-    dh2 = g[2] * dt2
-    dh1 = g[1] * dt1 + conv_adjoint(dh2, P["W2"], 2, c["h1"].shape[:3])
-    G["W2"] += conv_wgrad(c["h1"], dh2, 2)
-    dh0 = g[0] * dt0 + conv_adjoint(dh1, P["W1"], 2, s0)
-    G["W1"] += conv_wgrad(c["h0"], dh1, 2)
-    G["W0"] += conv_wgrad(c["v"], dh0, 1)
-    for l, (h, dt) in enumerate(((c["h0"], dt0), (c["h1"], dt1), (c["h2"], dt2))):
-        q = g[l] * h * dt
+    for l in range(1, L):
+        dt.append(conv(dt[-1], P[f"W{l}"], 2))
+        G[f"W{l}"] = conv_wgrad(dt[l - 1], c[f"t{l}"], 2)
+    dh = g[L - 1] * dt[L - 1]
+    for l in range(L - 1, 0, -1):
+        G[f"W{l}"] += conv_wgrad(c[f"h{l - 1}"], dh, 2)
+        dh = g[l - 1] * dt[l - 1] + conv_adjoint(dh, P[f"W{l}"], 2, c[f"h{l - 1}"].shape[:3])
+    G["W0"] += conv_wgrad(c["v"], dh, 1)
+    for l in range(L):
+        q = g[l] * c[f"h{l}"] * dt[l]
         G[f"b{l}"] = q.reshape(-1, q.shape[-1]).sum(0)
         G[f"a{l}"] = -(q * c["eb"][l][..., None]).reshape(-1, q.shape[-1]).sum(0)
     return G

@@ -118,29 +118,33 @@ def test_weight_gradient_exact(stride, shape, nb, cin, cout):
 
 
 def pack(P):
-    return np.concatenate([P["W0"].ravel(), P["W1"].ravel(), P["W2"].ravel()] +
-                          [P[f"{k}{l}"] for l in range(3) for k in ("a", "b")]).astype(np.float32)
+    L = U.n_levels(P)
+    return np.concatenate([P[f"W{l}"].ravel() for l in range(L)] +
+                          [P[f"{k}{l}"] for l in range(L) for k in ("a", "b")]).astype(np.float32)
 
 
-def unpack(v):
+def unpack(v, levels=3):
     P, o = {}, 0
-    for name, ci, co, _ in U.LAYERS:
+    for name, ci, co, _ in U.layers(levels):
         P[name] = v[o:o + 27 * co * ci].reshape(27, co, ci).astype(np.float64)
         o += 27 * co * ci
-    for l, c in enumerate(U.CH):
+    for l in range(levels):
+        c = U.CH_ALL[l]
         for k in ("a", "b"):
             P[f"{k}{l}"] = v[o:o + c].astype(np.float64)
             o += c
+    assert o == v.size
     return P
 
 
-def random_params(seed):
+def random_params(seed, levels=3):
     rng = np.random.default_rng(seed)
-    P = U.init_params(seed)
-    for l, c in enumerate(U.CH):
+    P = U.init_params(seed, levels)
+    for l in range(levels):
+        c = U.CH_ALL[l]
         P[f"a{l}"] = 0.3 * rng.standard_normal(c)
         P[f"b{l}"] = 0.3 * rng.standard_normal(c)
-    return unpack(bf16(pack(P)))      # weights exactly as the kernels see them
+    return unpack(bf16(pack(P)), levels)      # weights exactly as the kernels see them
 
 
 @pytest.fixture(scope="module")
@@ -182,6 +186,45 @@ def test_network_gradient_matches_oracle(small_net):
         cos = float(a @ r / (np.linalg.norm(a) * np.linalg.norm(r)))
         tol = 3e-2 if k.startswith("W") else 1e-4
         assert rel(a, r) <= tol and cos >= 0.999, (k, rel(a, r), cos)
+
+
+@pytest.mark.parametrize("levels", [2, 5])
+def test_deeper_network_forward_and_gradient_match_oracle(levels):
+    """the L-level network (down to grids of one node along an axis) against
+    the oracle: forward within 1e-5, parameter gradient as above"""
+    D, H, W = 5, 4, 11
+    shape = (D, H, W)
+    net = G.GpuNet(W, H, D, batch=2, levels=levels)
+    assert net.P == G.lib().ufem_net_nparams_levels(levels) and net.weights().size == net.P
+    rng = np.random.default_rng(20 + levels)
+    P = random_params(30 + levels, levels)
+    net.set_weights(pack(P))
+    x = rng.standard_normal((2,) + shape + (3,))
+    e0 = 0.5 * rng.standard_normal((2,) + shape)
+    dy = rng.standard_normal((2,) + shape + (3,))
+    y = net.forward(x.reshape(2, -1, 3), e0.reshape(2, -1))
+    g = unpack(net.backward(x.reshape(2, -1, 3), e0.reshape(2, -1), dy.reshape(2, -1, 3)), levels)
+    ref = {k: 0.0 for k in P}
+    for b in range(2):
+        eb = U.ebar_levels(e0[b].astype(np.float32).astype(np.float64), levels)
+        yr, c = U.forward(P, x[b], eb)
+        assert rel(y[b], yr) <= 1e-5
+        gb = U.backward(P, c, dy[b])
+        for k in gb:
+            ref[k] = ref[k] + gb[k]
+    for k in P:
+        a, r = g[k].ravel(), ref[k].ravel()
+        cos = float(a @ r / (np.linalg.norm(a) * np.linalg.norm(r)))
+        tol = 3e-2 if k.startswith("W") else 1e-4
+        assert rel(a, r) <= tol and cos >= 0.999, (k, rel(a, r), cos)
+    net.close()
+
+
+def test_levels_out_of_range():
+    with pytest.raises(RuntimeError, match="levels"):
+        G.GpuNet(9, 5, 5, levels=7)
+    assert G.lib().ufem_net_nparams_levels(1) == -1
+    assert G.lib().ufem_net_nparams_levels(3) == G.lib().ufem_net_nparams() == 70640
 
 
 def beam_mesh(seed, nx=24, ny=6, nz=6):

@@ -148,3 +148,71 @@ def test_energy_loss_value_and_planted_mistake():
     assert np.isclose(L, (e - zj) @ (r - K @ zj) / (e @ r), rtol=1e-13)
     assert np.allclose(z, zj)
     assert all(np.all(G[k] == 0) for k in ("W0", "W1", "W2"))
+
+
+def random_params_levels(seed, levels, gate_scale=0.3):
+    rng = np.random.default_rng(seed)
+    P = U.init_params(seed, levels)
+    for l in range(levels):
+        c = U.CH_ALL[l]
+        P[f"a{l}"] = gate_scale * rng.standard_normal(c)
+        P[f"b{l}"] = gate_scale * rng.standard_normal(c)
+    return P
+
+
+@pytest.mark.parametrize("levels", [2, 5])
+def test_deeper_network_is_symmetric_positive_semidefinite(levels):
+    """more levels (down to grids of one node along an axis) keep N linear,
+    symmetric and PSD; an untied decoder at the deepest level breaks it"""
+    shape = (5, 4, 11)
+    P = random_params_levels(6, levels)
+    eb = U.ebar_levels(0.5 * np.random.default_rng(7).standard_normal(shape), levels)
+    assert len(eb) == levels and eb[-1].shape == tuple(U.coarse_shape(eb[-2].shape))
+    v1, v2 = RNG.standard_normal(shape + (3,)), RNG.standard_normal(shape + (3,))
+    n1, n2 = U.forward(P, v1, eb)[0], U.forward(P, v2, eb)[0]
+    a, b = np.sum(n1 * v2), np.sum(v1 * n2)
+    assert abs(a - b) <= 1e-12 * max(abs(a), 1.0)
+    assert np.sum(n1 * v1) > 0 and np.sum(n2 * v2) > 0
+    # linearity
+    n12 = U.forward(P, v1 + 2.0 * v2, eb)[0]
+    assert np.allclose(n12, n1 + 2.0 * n2, rtol=1e-11, atol=1e-11 * np.abs(n12).max())
+    # planted: the deepest transpose with a perturbed weight
+    L = levels
+    Pb = {k: v.copy() for k, v in P.items()}
+    Pb[f"W{L - 1}"] = P[f"W{L - 1}"] + 0.05 * np.random.default_rng(8).standard_normal(P[f"W{L - 1}"].shape)
+
+    def untied(v):
+        _, c = U.forward(P, v, eb)
+        t = c[f"t{L - 1}"]
+        for l in range(L - 2, -1, -1):
+            W = Pb[f"W{l + 1}"] if l == L - 2 else P[f"W{l + 1}"]
+            t = c["g"][l] * c[f"h{l}"] + U.conv_adjoint(t, W, 2, c[f"h{l}"].shape[:3])
+        return U.conv_adjoint(t, P["W0"], 1, shape)
+    u1, u2 = untied(v1), untied(v2)
+    assert abs(np.sum(u1 * v2) - np.sum(v1 * u2)) > 1e-6 * abs(np.sum(u1 * v2))
+
+
+def test_deeper_network_gradients_match_finite_differences():
+    shape = (5, 4, 9)
+    levels = 4
+    n = 3 * int(np.prod(shape))
+    K = laplace3(shape)
+    dinv = 1.0 / K.diagonal()
+    mask = np.ones(n)
+    mask[:6] = 0.0
+    e = RNG.standard_normal(n) * mask
+    r = K @ e
+    P = random_params_levels(9, levels)
+    eb = U.ebar_levels(0.5 * np.random.default_rng(10).standard_normal(shape), levels)
+    L, G, _ = U.energy_loss(P, K, r, e, dinv, mask, eb, shape)
+    rng = np.random.default_rng(11)
+    h = 1e-4      # the deep gates' gradients are small: a smaller step is lost in rounding
+    for name in [f"W{l}" for l in range(levels)] + [f"{p}{l}" for l in range(levels) for p in "ab"]:
+        idx = tuple(rng.integers(s) for s in P[name].shape)
+        Pp = {k: v.copy() for k, v in P.items()}
+        Pm = {k: v.copy() for k, v in P.items()}
+        Pp[name][idx] += h
+        Pm[name][idx] -= h
+        fd = (U.energy_loss(Pp, K, r, e, dinv, mask, eb, shape)[0] -
+              U.energy_loss(Pm, K, r, e, dinv, mask, eb, shape)[0]) / (2 * h)
+        assert abs(fd - G[name][idx]) <= 1e-5 * abs(fd) + 1e-9, (name, idx, fd, G[name][idx])

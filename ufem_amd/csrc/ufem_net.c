@@ -1,21 +1,23 @@
 /*
- * ufem_net.c -- the learned preconditioner: a symmetric 3-level 3-D
- * convolution U-Net on the structured node grid of a box mesh, trained and
- * applied on the GPU by the kernels of ugk_unet.hip.
+ * ufem_net.c -- the learned preconditioner: a symmetric L-level 3-D
+ * convolution U-Net (L = 2..6, 3 by default) on the structured node grid of
+ * a box mesh, trained and applied on the GPU by the kernels of ugk_unet.hip.
  *
  * The network has no counterpart in the reference; it accelerates the
  * Jacobi-PCG solve (femEqsCGwJ, fem/fem_eqs.c:88-237) as the preconditioner
  *
  *   z = dinv r + mask dsq N(mask dsq r),   dsq = sqrt(dinv),
- *   N(v) = C0^T (g0 h0 + C1^T (g1 h1 + C2^T (g2 h2))),
- *   h0 = C0 v, h1 = C1 h0, h2 = C2 h1,
+ *   N(v) = C0^T (g0 h0 + C1^T (g1 h1 + ... + C_{L-1}^T (g_{L-1} h_{L-1}))),
+ *   h0 = C0 v, h_l = C_l h_{l-1},
  *   g_l = exp(b_l - (1 + a_l) ebar_l)   (per node and channel),
  *
- * with C0 a 3x3x3 convolution 3 -> 16 channels, C1 16 -> 32 and C2 32 -> 64
- * stride-2 convolutions (levels of (n + 1) / 2 nodes per axis), C^T their
- * adjoints (the same weights, transposed: the up path is tied to the down
- * path), and ebar_l the log-modulus log(E / 30 GPa) at the nodes, averaged
- * over 3x3x3 windows per coarser level.  N = sum_l H_l^T diag(g_l) H_l is
+ * with C0 a 3x3x3 convolution 3 -> 16 channels, C1 16 -> 32, C2 32 -> 64
+ * and C3.. 64 -> 64 stride-2 convolutions (levels of (n + 1) / 2 nodes per
+ * axis), C^T their adjoints (the same weights, transposed: the up path is
+ * tied to the down path), and ebar_l the log-modulus log(E / 30 GPa) at the
+ * nodes, averaged over 3x3x3 windows per coarser level.  More levels widen
+ * the receptive field towards the beam's bending modes.  N = sum_l H_l^T
+ * diag(g_l) H_l is
  * linear in r, symmetric and positive semi-definite, so M is SPD and the
  * short recurrences of CG stay valid.  The input is scaled by 1 / rms(dsq r)
  * and the output by rms(dsq r) only to keep the numbers near 1.
@@ -43,23 +45,25 @@
 
 #define E_REF 30e9
 
-static const int CH[3] = {16, 32, 64};    /* output channels of C0, C1, C2 */
-static const int CI[3] = {3, 16, 32};     /* input channels                */
+#define NL_MAX 6
+static const int CH[NL_MAX] = {16, 32, 64, 64, 64, 64};   /* output channels of C0, C1, .. */
+static const int CI[NL_MAX] = {3, 16, 32, 64, 64, 64};    /* input channels                */
 
 struct ufem_net {
-  ugk_grid3 g[3];
-  long V[3];
+  int L;                          /* levels */
+  ugk_grid3 g[NL_MAX];
+  long V[NL_MAX], Vt;             /* nodes per level, and their sum */
   int nbmax;
   void *stream;
   void *ev[2];
-  long P, offW[3], offA[3], offB[3];
+  long P, offW[NL_MAX], offA[NL_MAX], offB[NL_MAX];
   float *w, *gr, *m, *v;          /* packed f32 parameters, gradient, Adam moments */
-  unsigned short *wf[3], *wt[3];  /* bf16 [27][cop][cip] and [27][cip][cop]        */
+  unsigned short *wf[NL_MAX], *wt[NL_MAX];  /* bf16 [27][cop][cip] and [27][cip][cop] */
   float *dwpart;
   double *gpart;
   /* activations (f32, nbmax samples) */
-  float *x0, *h[3], *t[3], *gt[3], *eb[3], *y;
-  float *dy, *dt[3], *dh[3];
+  float *x0, *h[NL_MAX], *t[NL_MAX], *gt[NL_MAX], *eb[NL_MAX], *y;
+  float *dy, *dt[NL_MAX], *dh[NL_MAX];
   float *zz, *kz;
   double *dpart, *hpart;
   long step;
@@ -74,7 +78,7 @@ struct ufem_net {
   /* binding to a system */
   ufem_sys *sys;
   float *mask;
-  float *yi, *gi[3], *ebi;
+  float *yi, *gi[NL_MAX], *ebi;
   unsigned short *x0s;            /* inference input, split format [V][3][4] */
   float *ebh;                     /* host ebar levels of the bound field */
   double *snap;
@@ -96,13 +100,17 @@ static ugk_grid3 coarse(ugk_grid3 g)
   return c;
 }
 
-long ufem_net_nparams(void)
+long ufem_net_nparams_levels(int levels)
 {
   long P = 0;
   int l;
-  for (l = 0; l < 3; l++) P += 27L * CI[l] * CH[l] + 2L * CH[l];
+  if (levels < 2 || levels > NL_MAX) return -1;
+  for (l = 0; l < levels; l++) P += 27L * CI[l] * CH[l] + 2L * CH[l];
   return P;
 }
+
+long ufem_net_nparams(void) { return ufem_net_nparams_levels(3); }
+int ufem_net_levels(const ufem_net *N) { return N->L; }
 
 static void *dalloc(size_t bytes, int *bad)
 {
@@ -117,7 +125,7 @@ void ufem_net_destroy(ufem_net *N)
   if (!N) return;
   if (N->stream) (void)ugk_sync(N->stream);
   ugk_free(N->w); ugk_free(N->gr); ugk_free(N->m); ugk_free(N->v);
-  for (l = 0; l < 3; l++) {
+  for (l = 0; l < NL_MAX; l++) {
     ugk_free(N->wf[l]); ugk_free(N->wt[l]);
     ugk_free(N->h[l]); ugk_free(N->t[l]); ugk_free(N->gt[l]); ugk_free(N->eb[l]);
     ugk_free(N->dt[l]); ugk_free(N->dh[l]); ugk_free(N->gi[l]);
@@ -154,17 +162,27 @@ static ugk_dw_desc dw_desc(const ufem_net *N, int l, const float *x, int ldx, in
 
 ufem_net *ufem_net_create(int W, int H, int D, int batch)
 {
+  return ufem_net_create_levels(W, H, D, batch, 3);
+}
+
+ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels)
+{
   ufem_net *N;
   int l, bad = 0;
   long nb, maxpart = 0, n;
   if (W < 2 || H < 2 || D < 2 || batch < 1) { ufem_set_error("net grid or batch out of range"); return NULL; }
+  if (levels < 2 || levels > NL_MAX) { ufem_set_error("levels %d out of range (2..%d)", levels, NL_MAX); return NULL; }
   if (ugk_init()) { net_fail("device"); return NULL; }
   N = (ufem_net *)calloc(1, sizeof(*N));
   if (!N) return NULL;
+  N->L = levels;
   N->g[0].D = D; N->g[0].H = H; N->g[0].W = W;
-  N->g[1] = coarse(N->g[0]);
-  N->g[2] = coarse(N->g[1]);
-  for (l = 0; l < 3; l++) N->V[l] = (long)N->g[l].D * N->g[l].H * N->g[l].W;
+  for (l = 1; l < levels; l++) N->g[l] = coarse(N->g[l - 1]);
+  N->Vt = 0;
+  for (l = 0; l < levels; l++) {
+    N->V[l] = (long)N->g[l].D * N->g[l].H * N->g[l].W;
+    N->Vt += N->V[l];
+  }
   N->nbmax = batch;
   nb = batch;
   n = 3 * N->V[0];
@@ -173,8 +191,8 @@ ufem_net *ufem_net_create(int W, int H, int D, int batch)
   N->ev[1] = ugk_event_create();
   if (!N->stream || !N->ev[0] || !N->ev[1]) { net_fail("stream"); ufem_net_destroy(N); return NULL; }
   N->P = 0;
-  for (l = 0; l < 3; l++) { N->offW[l] = N->P; N->P += 27L * CI[l] * CH[l]; }
-  for (l = 0; l < 3; l++) {
+  for (l = 0; l < levels; l++) { N->offW[l] = N->P; N->P += 27L * CI[l] * CH[l]; }
+  for (l = 0; l < levels; l++) {
     N->offA[l] = N->P; N->P += CH[l];
     N->offB[l] = N->P; N->P += CH[l];
   }
@@ -182,7 +200,7 @@ ufem_net *ufem_net_create(int W, int H, int D, int batch)
   N->gr = (float *)dalloc(sizeof(float) * N->P, &bad);
   N->m = (float *)dalloc(sizeof(float) * N->P, &bad);
   N->v = (float *)dalloc(sizeof(float) * N->P, &bad);
-  for (l = 0; l < 3; l++) {
+  for (l = 0; l < levels; l++) {
     const size_t b = sizeof(unsigned short) * 27 * r16(CI[l]) * r16(CH[l]);
     const long Vl = N->V[l];
     N->wf[l] = (unsigned short *)dalloc(b, &bad);
@@ -212,7 +230,7 @@ ufem_net *ufem_net_create(int W, int H, int D, int batch)
   N->hpart = (double *)malloc(sizeof(double) * 1024 * (size_t)(nb > 4 ? nb : 4));
   N->yi = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] * 4), &bad);
   N->x0s = (unsigned short *)dalloc(3 * sizeof(unsigned short) * (size_t)(N->V[0] * 4), &bad);
-  N->ebi = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] + N->V[1] + N->V[2]), &bad);
+  N->ebi = (float *)dalloc(sizeof(float) * (size_t)N->Vt, &bad);
   if (bad || !N->hpart) { net_fail("network buffers"); ufem_net_destroy(N); return NULL; }
   return N;
 }
@@ -222,7 +240,7 @@ static int adam_all(ufem_net *N, float lr)
 {
   int l;
   if (lr > 0) N->step++;
-  for (l = 0; l < 3; l++) {
+  for (l = 0; l < N->L; l++) {
     if (ugk_adam(27L * CI[l] * CH[l], CH[l], CI[l], r16(CH[l]), r16(CI[l]), N->w + N->offW[l], N->gr + N->offW[l],
                  N->m + N->offW[l], N->v + N->offW[l], N->wf[l], N->wt[l], lr, 0.9f, 0.999f, 1e-8f, N->step,
                  N->stream))
@@ -269,11 +287,11 @@ int ufem_net_init(ufem_net *N, unsigned long seed)
   long i = 0;
   int l, rv;
   if (!w) return AF_ERR_MEM;
-  for (l = 0; l < 3; l++) {
+  for (l = 0; l < N->L; l++) {
     long k;
     for (k = 0; k < CH[l]; k++) w[N->offB[l] + k] = -3.f;
   }
-  for (l = 0; l < 3; l++) {
+  for (l = 0; l < N->L; l++) {
     const long Pl = 27L * CI[l] * CH[l];
     const double sd = 1.0 / sqrt(27.0 * CI[l]);
     long k;
@@ -324,57 +342,68 @@ static int conv(ufem_net *N, const float *x, int ldx, int cin, int l, int adjoin
                 stream);
 }
 
-/* stage k of y = N(v) for nb samples with gates g[l] (nb x V_l x C_l):
- * 0 h0 = C0 v, 1 h1 = C1 h0, 2 h2 = C2 h1, 3 t2 = g2 h2, 4 t1 = g1 h1 +
- * C2^T t2, 5 t0 = g0 h0 + C1^T t1, 6 y = C0^T t0 */
-#define NET_STAGES 7
+/* stage k of y = N(v) for nb samples with gates g[l] (nb x V_l x C_l), L
+ * levels, 2 L + 1 stages: k = 0: h0 = C0 v; k = 1 .. L-1: h_k = C_k h_{k-1};
+ * k = L: t_{L-1} = g_{L-1} h_{L-1}; k = L+1 .. 2L-1: t_l = g_l h_l +
+ * C_{l+1}^T t_{l+1} with l = 2L-1-k; k = 2L: y = C0^T t0 */
+static int net_stages(const ufem_net *N) { return 2 * N->L + 1; }
 static int net_stage(ufem_net *N, int k, const float *v, float *const *g, float *y, int nb, const int *skip,
                      void *st)
 {
-  switch (k) {
-    case 0: return conv(N, v, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, 0, nb, skip, st);
-    case 1: return conv(N, N->h[0], 16, 16, 1, 0, N->h[1], 32, 32, 0, 1, NULL, NULL, 0, nb, skip, st);
-    case 2: return conv(N, N->h[1], 32, 32, 2, 0, N->h[2], 64, 64, 1, 2, NULL, NULL, 0, nb, skip, st);
-    case 3: return ugk_mul((long)nb * N->V[2] * 64, g[2], N->h[2], N->t[2], skip, st);
-    case 4: return conv(N, N->t[2], 64, 64, 2, 1, N->t[1], 32, 32, 2, 1, g[1], N->h[1], 32, nb, skip, st);
-    case 5: return conv(N, N->t[1], 32, 32, 1, 1, N->t[0], 16, 16, 1, 0, g[0], N->h[0], 16, nb, skip, st);
-    default: return conv(N, N->t[0], 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, 0, nb, skip, st);
+  const int L = N->L;
+  if (k == 0) return conv(N, v, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, 0, nb, skip, st);
+  if (k < L)
+    return conv(N, N->h[k - 1], CH[k - 1], CH[k - 1], k, 0, N->h[k], CH[k], CH[k], k - 1, k, NULL, NULL, 0, nb,
+                skip, st);
+  if (k == L) return ugk_mul((long)nb * N->V[L - 1] * CH[L - 1], g[L - 1], N->h[L - 1], N->t[L - 1], skip, st);
+  if (k < 2 * L) {
+    const int l = 2 * L - 1 - k;
+    return conv(N, N->t[l + 1], CH[l + 1], CH[l + 1], l + 1, 1, N->t[l], CH[l], CH[l], l + 1, l, g[l], N->h[l],
+                CH[l], nb, skip, st);
   }
+  return conv(N, N->t[0], 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, 0, nb, skip, st);
 }
 
 static int net_forward(ufem_net *N, const float *v, float *const *g, float *y, int nb, const int *skip,
                        void *st)
 {
   int k;
-  for (k = 0; k < NET_STAGES; k++)
+  for (k = 0; k < net_stages(N); k++)
     if (net_stage(N, k, v, g, y, nb, skip, st)) return net_fail("network forward");
   return AF_OK;
 }
 
 /* inference, one sample, input x0s in the split format (made once by the
  * producer, so the first layer does no per-load split work), f32
- * activations in the first sample of the training buffers:
- * 0 h0 = C0 v, 1 h1 = C1 h0, 2 t2 = g2 C2 h1 (the gate product in C2's
- * epilogue), 3 t1 = g1 h1 + C2^T t2, 4 t0 = g0 h0 + C1^T t1, 5 y = C0^T t0
- * (f32 [V][4]); the same products as net_forward, bit for bit */
-#define INFER_STAGES 6
+ * activations in the first sample of the training buffers, 2 L stages:
+ * k = 0: h0 = C0 v; k = 1 .. L-2: h_k = C_k h_{k-1}; k = L-1: t_{L-1} =
+ * g_{L-1} C_{L-1} h_{L-2} (the gate product in the convolution's epilogue);
+ * k = L .. 2L-2: t_l = g_l h_l + C_{l+1}^T t_{l+1} with l = 2L-2-k;
+ * k = 2L-1: y = C0^T t0 (f32 [V][4]); the same products as net_forward,
+ * bit for bit */
+static int infer_stages(const ufem_net *N) { return 2 * N->L; }
 static int infer_stage(ufem_net *N, int k, float *const *g, float *y, const int *skip, void *st)
 {
-  const int F = UGK_ACT_F32;
-  switch (k) {
-    case 0: return conv_f(N, N->x0s, UGK_ACT_SPLIT, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
-    case 1: return conv_f(N, N->h[0], F, 16, 16, 1, 0, N->h[1], 32, 32, 0, 1, NULL, NULL, NULL, 0, 1, skip, st);
-    case 2: return conv_f(N, N->h[1], F, 32, 32, 2, 0, N->t[2], 64, 64, 1, 2, g[2], NULL, NULL, 64, 1, skip, st);
-    case 3: return conv_f(N, N->t[2], F, 64, 64, 2, 1, N->t[1], 32, 32, 2, 1, NULL, g[1], N->h[1], 32, 1, skip, st);
-    case 4: return conv_f(N, N->t[1], F, 32, 32, 1, 1, N->t[0], 16, 16, 1, 0, NULL, g[0], N->h[0], 16, 1, skip, st);
-    default: return conv_f(N, N->t[0], F, 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
+  const int F = UGK_ACT_F32, L = N->L;
+  if (k == 0) return conv_f(N, N->x0s, UGK_ACT_SPLIT, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
+  if (k < L - 1)
+    return conv_f(N, N->h[k - 1], F, CH[k - 1], CH[k - 1], k, 0, N->h[k], CH[k], CH[k], k - 1, k, NULL, NULL, NULL,
+                  0, 1, skip, st);
+  if (k == L - 1)
+    return conv_f(N, N->h[k - 1], F, CH[k - 1], CH[k - 1], k, 0, N->t[k], CH[k], CH[k], k - 1, k, g[k], NULL, NULL,
+                  CH[k], 1, skip, st);
+  if (k < 2 * L - 1) {
+    const int l = 2 * L - 2 - k;
+    return conv_f(N, N->t[l + 1], F, CH[l + 1], CH[l + 1], l + 1, 1, N->t[l], CH[l], CH[l], l + 1, l, NULL, g[l],
+                  N->h[l], CH[l], 1, skip, st);
   }
+  return conv_f(N, N->t[0], F, 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
 }
 
 static int infer(ufem_net *N, float *const *g, float *y, const int *skip, void *st)
 {
   int k;
-  for (k = 0; k < INFER_STAGES; k++)
+  for (k = 0; k < infer_stages(N); k++)
     if (infer_stage(N, k, g, y, skip, st)) return net_fail("network inference");
   return AF_OK;
 }
@@ -386,26 +415,33 @@ static int wgrad(ufem_net *N, int l, const float *x, int ldx, int cin, const flo
   return ugk_conv_dw(&d, N->dwpart, ugk_conv_dw_slices(&d), N->gr + N->offW[l], acc, N->stream);
 }
 
-/* parameter gradient of <dy, N(v)> into N->gr (forward fields in place) */
+/* parameter gradient of <dy, N(v)> into N->gr (forward fields in place):
+ * dt0 = C0 dy, dt_l = C_l dt_{l-1} (the adjoints of the up path), with the
+ * weight gradients of the up path's C_l^T; dh_{L-1} = g_{L-1} dt_{L-1},
+ * dh_{l-1} = C_l^T dh_l + g_{l-1} dt_{l-1} (the down path), with the weight
+ * gradients of its C_l; the gate gradients from h_l and dt_l */
 static int net_backward(ufem_net *N, const float *v, const float *dy, int nb)
 {
   float *const *g = N->gt;
-  const long n2 = (long)nb * N->V[2] * 64;
+  const int L = N->L;
   int l;
   if (conv(N, dy, 4, 4, 0, 0, N->dt[0], 16, 16, 0, 0, NULL, NULL, 0, nb, NULL, N->stream) ||
-      wgrad(N, 0, dy, 4, 3, N->t[0], 16, 0, nb, 0) ||
-      conv(N, N->dt[0], 16, 16, 1, 0, N->dt[1], 32, 32, 0, 1, NULL, NULL, 0, nb, NULL, N->stream) ||
-      wgrad(N, 1, N->dt[0], 16, 16, N->t[1], 32, 0, nb, 0) ||
-      conv(N, N->dt[1], 32, 32, 2, 0, N->dt[2], 64, 64, 1, 2, NULL, NULL, 0, nb, NULL, N->stream) ||
-      wgrad(N, 2, N->dt[1], 32, 32, N->t[2], 64, 1, nb, 0) ||
-      ugk_mul(n2, g[2], N->dt[2], N->dh[2], NULL, N->stream) ||
-      conv(N, N->dh[2], 64, 64, 2, 1, N->dh[1], 32, 32, 2, 1, g[1], N->dt[1], 32, nb, NULL, N->stream) ||
-      wgrad(N, 2, N->h[1], 32, 32, N->dh[2], 64, 1, nb, 1) ||
-      conv(N, N->dh[1], 32, 32, 1, 1, N->dh[0], 16, 16, 1, 0, g[0], N->dt[0], 16, nb, NULL, N->stream) ||
-      wgrad(N, 1, N->h[0], 16, 16, N->dh[1], 32, 0, nb, 1) ||
-      wgrad(N, 0, v, 4, 3, N->dh[0], 16, 0, nb, 1))
+      wgrad(N, 0, dy, 4, 3, N->t[0], 16, 0, nb, 0))
     return net_fail("network backward");
-  for (l = 0; l < 3; l++)
+  for (l = 1; l < L; l++)
+    if (conv(N, N->dt[l - 1], CH[l - 1], CH[l - 1], l, 0, N->dt[l], CH[l], CH[l], l - 1, l, NULL, NULL, 0, nb, NULL,
+             N->stream) ||
+        wgrad(N, l, N->dt[l - 1], CH[l - 1], CH[l - 1], N->t[l], CH[l], l - 1, nb, 0))
+      return net_fail("network backward");
+  if (ugk_mul((long)nb * N->V[L - 1] * CH[L - 1], g[L - 1], N->dt[L - 1], N->dh[L - 1], NULL, N->stream))
+    return net_fail("network backward");
+  for (l = L - 1; l >= 1; l--)
+    if (conv(N, N->dh[l], CH[l], CH[l], l, 1, N->dh[l - 1], CH[l - 1], CH[l - 1], l, l - 1, g[l - 1], N->dt[l - 1],
+             CH[l - 1], nb, NULL, N->stream) ||
+        wgrad(N, l, N->h[l - 1], CH[l - 1], CH[l - 1], N->dh[l], CH[l], l - 1, nb, 1))
+      return net_fail("network backward");
+  if (wgrad(N, 0, v, 4, 3, N->dh[0], 16, 0, nb, 1)) return net_fail("network backward");
+  for (l = 0; l < L; l++)
     if (ugk_gate_grad((long)nb * N->V[l], CH[l], g[l], N->h[l], N->dt[l], N->eb[l], N->gpart,
                       N->gr + N->offA[l], N->gr + N->offB[l], N->stream))
       return net_fail("gate gradient");
@@ -415,7 +451,7 @@ static int net_backward(ufem_net *N, const float *v, const float *dy, int nb)
 static int make_gates(ufem_net *N, float *const *eb, float *const *g, int nb, void *st)
 {
   int l;
-  for (l = 0; l < 3; l++)
+  for (l = 0; l < N->L; l++)
     if (ugk_gates((long)nb * N->V[l], CH[l], eb[l], N->w + N->offA[l], N->w + N->offB[l], g[l], st))
       return net_fail("gates");
   return AF_OK;
@@ -444,22 +480,25 @@ static void pool_host(const float *f, ugk_grid3 gf, float *c, ugk_grid3 gc)
 
 static void levels_host(const ufem_net *N, const float *e0, float *out)
 {
+  long off = 0;
+  int l;
   memcpy(out, e0, sizeof(float) * (size_t)N->V[0]);
-  pool_host(out, N->g[0], out + N->V[0], N->g[1]);
-  pool_host(out + N->V[0], N->g[1], out + N->V[0] + N->V[1], N->g[2]);
+  for (l = 1; l < N->L; l++) {
+    pool_host(out + off, N->g[l - 1], out + off + N->V[l - 1], N->g[l]);
+    off += N->V[l - 1];
+  }
 }
 
 /* upload nb samples' ebar0 (host [nb][V0]) into the batch level buffers */
 static int upload_levels(ufem_net *N, const float *eb0, int nb)
 {
-  const long Vt = N->V[0] + N->V[1] + N->V[2];
-  float *lv = (float *)malloc(sizeof(float) * (size_t)Vt);
+  float *lv = (float *)malloc(sizeof(float) * (size_t)N->Vt);
   int b, l, rv = AF_OK;
   if (!lv) return AF_ERR_MEM;
   for (b = 0; b < nb && rv == AF_OK; b++) {
     long off = 0;
     levels_host(N, eb0 + b * N->V[0], lv);
-    for (l = 0; l < 3; l++) {
+    for (l = 0; l < N->L; l++) {
       if (ugk_h2d(N->eb[l] + b * N->V[l], lv + off, sizeof(float) * (size_t)N->V[l], N->stream) ||
           ugk_sync(N->stream))
         rv = net_fail("levels upload");
@@ -496,9 +535,9 @@ int ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, floa
     return AF_ERR;
   /* the inference path (split activations), one sample at a time */
   for (b = 0; b < nb; b++) {
-    float *g[3];
+    float *g[NL_MAX];
     int l;
-    for (l = 0; l < 3; l++) g[l] = N->gt[l] + (long)b * N->V[l] * CH[l];
+    for (l = 0; l < N->L; l++) g[l] = N->gt[l] + (long)b * N->V[l] * CH[l];
     if (ugk_to_split(V, N->x0 + (long)b * V * 4, N->x0s, N->stream)) return net_fail("split input");
     if (infer(N, g, N->y + (long)b * V * 4, NULL, N->stream)) return AF_ERR;
   }
@@ -542,7 +581,7 @@ int ufem_net_bind(ufem_net *N, ufem_sys *s, const double *E)
 {
   const ugk_grid3 g = N->g[0];
   const long V = N->V[0], nx = g.W - 1, ny = g.H - 1, nz = g.D - 1;
-  const long Vt = N->V[0] + N->V[1] + N->V[2];
+  const long Vt = N->Vt;
   double *acc;
   int *cnt;
   float *e0, *mk;
@@ -643,7 +682,7 @@ static int snap_hook(void *ctx, long it, void *stream)
 static int add_field(ufem_net *N)
 {
   ufem_sys *s = N->sys;
-  const long Vt = N->V[0] + N->V[1] + N->V[2];
+  const long Vt = N->Vt;
   const size_t nv = (size_t)s->K.total_slots * 64 * 9;
   ugk_mat tmp;
   int f = N->nfield;
@@ -767,7 +806,7 @@ static int load_batch(ufem_net *N, const long *idx, int nb)
     if (ugk_d2d(N->x0 + b * N->V[0] * 4, N->pv + q * N->V[0] * 4, sizeof(float) * (size_t)(N->V[0] * 4),
                 N->stream))
       return net_fail("batch copy");
-    for (l = 0; l < 3; l++) {
+    for (l = 0; l < N->L; l++) {
       if (ugk_d2d(N->eb[l] + b * N->V[l], N->feb[N->pfield[q]] + off, sizeof(float) * (size_t)N->V[l], N->stream))
         return net_fail("batch copy");
       off += N->V[l];
@@ -864,7 +903,13 @@ const float *ufem_net_output(const ufem_net *N) { return N->yi; }
 /* gates of the bound field, before a solve (the weights may have changed) */
 int ufem_net_prepare(ufem_net *N, void *stream)
 {
-  float *eb[3] = {N->ebi, N->ebi + N->V[0], N->ebi + N->V[0] + N->V[1]};
+  float *eb[NL_MAX];
+  long off = 0;
+  int l;
+  for (l = 0; l < N->L; l++) {
+    eb[l] = N->ebi + off;
+    off += N->V[l];
+  }
   return make_gates(N, eb, N->gi, 1, stream);
 }
 
@@ -1039,7 +1084,7 @@ int ufem_net_time_stages(ufem_net *N, int reps, double *us)
   int k, r;
   if (!N->sys) { ufem_set_error("bind the network to a system first"); return AF_ERR; }
   if (ufem_net_prepare(N, N->stream)) return AF_ERR;
-  for (k = 0; k < INFER_STAGES; k++) {
+  for (k = 0; k < infer_stages(N); k++) {
     if (infer_stage(N, k, N->gi, N->yi, NULL, N->stream)) return net_fail("stage");
     if (ugk_event_record(N->ev[0], N->stream)) return net_fail("event");
     for (r = 0; r < reps; r++)

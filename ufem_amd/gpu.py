@@ -80,6 +80,9 @@ def lib():
             "ufem_net_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
             "ufem_net_destroy": (None, [C.c_void_p]),
             "ufem_net_nparams": (C.c_long, []),
+            "ufem_net_create_levels": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ufem_net_nparams_levels": (C.c_long, [C.c_int]),
+            "ufem_net_levels": (C.c_int, [C.c_void_p]),
             "ufem_net_init": (C.c_int, [C.c_void_p, C.c_ulong]),
             "ufem_net_set_weights": (C.c_int, [C.c_void_p, P_F]),
             "ufem_net_get_weights": (C.c_int, [C.c_void_p, P_F]),
@@ -396,15 +399,17 @@ def _pf(a):
 
 
 class GpuNet:
-    """The learned preconditioner's U-Net (ufem_net_*) on a W x H x D node grid."""
+    """The learned preconditioner's U-Net (ufem_net_*) on a W x H x D node
+    grid, with `levels` levels (2..6)."""
 
-    def __init__(self, W, H, D, batch=1):
-        self.h = lib().ufem_net_create(W, H, D, batch)
+    def __init__(self, W, H, D, batch=1, levels=3):
+        self.h = lib().ufem_net_create_levels(W, H, D, batch, levels)
         if not self.h:
             raise RuntimeError("ufem_net_create: " + last_error())
         self.shape = (D, H, W)
         self.V = W * H * D
-        self.P = lib().ufem_net_nparams()
+        self.levels = levels
+        self.P = lib().ufem_net_nparams_levels(levels)
 
     def close(self):
         if getattr(self, "h", None):
@@ -461,12 +466,16 @@ class GpuNet:
         self._chk(lib().ufem_net_get_pair(self.h, i, _pf(v), _pf(rr), _pf(ee), C.byref(er), C.byref(f)), "pair")
         return v, rr, ee, er.value, f.value
 
-    STAGES = ("C0", "C1", "C2g", "C2T", "C1T", "C0T")
+    def stage_names(self):
+        L = self.levels
+        return ([f"C{l}" for l in range(L - 1)] + [f"C{L - 1}g"] +
+                [f"C{l}T" for l in range(L - 1, -1, -1)])
 
     def time_stages(self, reps=50):
-        us = np.zeros(len(self.STAGES))
+        names = self.stage_names()
+        us = np.zeros(len(names))
         self._chk(lib().ufem_net_time_stages(self.h, reps, _pd(us)), "time_stages")
-        return dict(zip(self.STAGES, us))
+        return dict(zip(names, us))
 
     def eval(self, i0, cnt):
         out = np.zeros(cnt)
