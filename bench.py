@@ -266,7 +266,7 @@ def bench_dropin(mesh, native_s, native_ms, u64):
             "solve_ms_ratio_to_native": round(calls[-1]["solve_ms"] / native_ms, 3)}
 
 
-def bench_learned(fields, steps, seed, levels=3):
+def bench_learned(fields, steps, seed, levels=3, fine=True, lr=3e-3):
     """Learned preconditioner on the 128 x 32 x 32 GRF cantilever (421,443 DOF):
     pairs from plain-PCG solves on `fields` fresh fields (32 each), `steps`
     Adam steps, then plain PCG and flexible CG with the network on a field
@@ -280,11 +280,11 @@ def bench_learned(fields, steps, seed, levels=3):
     mesh.E[:] = E_test
     s = G.GpuSystem(mesh)
     s.assemble(2)
-    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=4, levels=levels)
+    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=4, levels=levels, fine=fine)
     net.init(1)
     net.reserve(32 * fields)
     pairs, collect_s = Lr.collect(s, net, mesh, range(1000, 1000 + fields), 32, every=8, log=log)
-    loss, train_ms = Lr.train(net, steps, batch=4, lr=3e-3)
+    loss, train_ms = Lr.train(net, steps, batch=4, lr=lr)
     log(f"learned: {pairs} pairs in {collect_s:.1f} s, {steps} steps in {train_ms / 1e3:.1f} s, "
         f"loss {loss[:20].mean():.3f} -> {loss[-50:].mean():.3f}")
     Lr.rebind(s, net, mesh, E_test)
@@ -310,7 +310,7 @@ def bench_learned(fields, steps, seed, levels=3):
         "workload": (f"cantilever 2.0x0.4x0.4 m, {nx}x{ny}x{nz} bricks, {mesh.n_dof} DOF, E = exp(GRF, "
                      f"corr 0.2 m, log-std 0.5, mean 30 GPa); test field seed {seed}, training fields "
                      f"1000..{1000 + fields - 1}"),
-        "levels": levels, "parameters": int(net.P),
+        "levels": levels, "node_grid_level": fine, "parameters": int(net.P),
         "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
         "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
         "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
@@ -328,7 +328,8 @@ def bench_learned(fields, steps, seed, levels=3):
     return out
 
 
-def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 10, 10), levels=3):
+def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 10, 10), levels=3, lr=3e-3,
+                        fine=True):
     """Learned + mixed precision on the metric's own system `g` (the 1M-DOF
     lognormal cantilever, E = mesh.E): pairs from plain-PCG solves on `fields`
     fresh i.i.d. lognormal fields of the same distribution (seeds seed_base..),
@@ -342,12 +343,12 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
     from ufem_amd import learned as Lr
     from ufem_amd.mesh import lognormal_E
     E_test = mesh.E.copy()
-    net = G.GpuNet(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, batch=4, levels=levels)
+    net = G.GpuNet(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, batch=4, levels=levels, fine=fine)
     net.init(1)
     net.reserve(32 * fields)
     pairs, collect_s = Lr.collect(g, net, mesh, range(seed_base, seed_base + fields), 32, every=8, log=log,
                                   field=lambda m, sd: lognormal_E(sd, m.n_elems, cov=cov))
-    loss, train_ms = Lr.train(net, steps, batch=4, lr=3e-3)
+    loss, train_ms = Lr.train(net, steps, batch=4, lr=lr)
     log(f"learned (1M DOF): {pairs} pairs in {collect_s:.1f} s, {steps} steps in {train_ms / 1e3:.1f} s, "
         f"loss {loss[:20].mean():.3f} -> {loss[-50:].mean():.3f}")
     Lr.rebind(g, net, mesh, E_test)
@@ -391,7 +392,7 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
         "workload": (f"the metric's system: {mesh.nx}x{mesh.ny}x{mesh.nz} bricks, {mesh.n_dof} DOF, E lognormal "
                      f"COV {cov}; training fields: {fields} fresh lognormal fields, seeds "
                      f"{seed_base}..{seed_base + fields - 1}"),
-        "levels": levels, "parameters": int(net.P),
+        "levels": levels, "node_grid_level": fine, "parameters": int(net.P),
         "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
         "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
         "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
@@ -537,7 +538,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--learn-fields", type=int, default=63)
     ap.add_argument("--learn-steps", type=int, default=3000)
-    ap.add_argument("--net-levels", type=int, default=6, help="levels of the learned preconditioner's U-Net (2..6)")
+    ap.add_argument("--learn-lr", type=float, default=1e-3)
+    ap.add_argument("--net-node-grid", action="store_true",
+                    help="the network with a node-grid level (default: the coarse-first form)")
+    ap.add_argument("--net-levels", type=int, default=6, help="levels of the learned preconditioner's U-Net (2..8)")
     ap.add_argument("--learned-1m-only", action="store_true",
                     help="assemble the metric's system and run only the learned + mixed comparison")
     ap.add_argument("--kernels-only", action="store_true",
@@ -631,7 +635,8 @@ def main():
         g.close()
         return
     if args.learned_1m_only:
-        out = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels)
+        out = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels,
+                                  lr=args.learn_lr, fine=args.net_node_grid)
         print(json.dumps({"learned_mixed_1m": out}), flush=True)
         g.close()
         return
@@ -768,13 +773,15 @@ def main():
     learned_mixed = None
     if not args.no_learned:
         log("learned + mixed precision at the metric's size ...")
-        learned_mixed = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels)
+        learned_mixed = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels,
+                                            lr=args.learn_lr, fine=args.net_node_grid)
         log("learned_mixed", learned_mixed)
 
     learned = None
     if not args.no_learned:
         log("learned preconditioner ...")
-        learned = bench_learned(args.learn_fields, args.learn_steps, 9999, levels=args.net_levels)
+        learned = bench_learned(args.learn_fields, args.learn_steps, 9999, levels=args.net_levels,
+                                fine=args.net_node_grid, lr=args.learn_lr)
         log("learned", learned)
 
     batch = None

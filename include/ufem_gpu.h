@@ -278,7 +278,7 @@ int  ufem_batch_get_u(ufem_batch *b, int i, double *u);
  * DOF 3 node + dir - 1, as ufem_sys numbers them).  Flexible CG with
  *   z = dinv r + mask dsq N(mask dsq r),  dsq = sqrt(dinv),
  * where N is a symmetric L-level 3-D convolution U-Net (3x3x3 taps, no
- * bias; L = 2..6, 3 by default): C0 3->16 channels, C1 16->32, C2 32->64 and
+ * bias; L = 2..8, 3 by default): C0 3->16 channels, C1 16->32, C2 32->64 and
  * C3.. 64->64 with stride 2 (levels of (n + 1) / 2 nodes per axis),
  * N(v) = C0^T (g0 h0 + C1^T (g1 h1 + ... + C_{L-1}^T (g_{L-1} h_{L-1}))),
  * h0 = C0 v, h_l = C_l h_{l-1}, and per-node, per-channel gates
@@ -288,14 +288,21 @@ int  ufem_batch_get_u(ufem_batch *b, int i, double *u);
  * if a direction fails to descend.  Parameters (f32, packed): W0 [27][16][3],
  * W1 [27][32][16], W2 [27][64][32], W3.. [27][64][64], then a_l, b_l per
  * level (its channel count each); taps t = 9 (dz+1) + 3 (dy+1) + (dx+1).
- * Weights enter the matrix cores as bf16, activations as f32 (exact bf16
- * splits). */
+ * The coarse-first form (fine = 0) has no node-grid level: C0 is a stride-2
+ * 3 -> 32 convolution from the node grid, C1 32 -> 64, C2.. 64 -> 64, level
+ * l on the node grid coarsened l + 1 times (ebar pooled once more), and
+ * N(v) = C0^T (g0 h0 + ...) back on the node grid; it skips the node-grid
+ * activations, the largest of the network.  Weights enter the matrix cores
+ * as bf16, activations as f32 (exact bf16 splits). */
 typedef struct ufem_net ufem_net;
 ufem_net *ufem_net_create(int W, int H, int D, int batch);  /* 3 levels; batch: training samples per step */
-ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels);
+ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels);   /* fine = 1 */
+ufem_net *ufem_net_create_form(int W, int H, int D, int batch, int levels, int fine);
 void ufem_net_destroy(ufem_net *N);
 long ufem_net_nparams(void);                 /* of the 3-level network: 70,640 */
-long ufem_net_nparams_levels(int levels);    /* -1 outside 2..6 */
+long ufem_net_nparams_levels(int levels);    /* -1 outside 2..8 */
+long ufem_net_nparams_form(int levels, int fine);
+int  ufem_net_fine(const ufem_net *N);
 int  ufem_net_levels(const ufem_net *N);
 int  ufem_net_init(ufem_net *N, unsigned long seed);   /* W ~ N(0, 1/(27 cin)), a = 0, b = -3 */
 int  ufem_net_set_weights(ufem_net *N, const float *w);

@@ -11,7 +11,7 @@ solve (femEqsCGwJ, fem/fem_eqs.c:88-237).  Definition:
 * conv(X, W, s): 3x3x3 taps t = 9 (dz+1) + 3 (dy+1) + (dx+1), zero padding,
   no bias, W[t][co][ci]; stride 1: Y[v] = sum_t W[t] X[v + off_t]; stride 2:
   Y[c] = sum_t W[t] X[2c + off_t]; convT is its adjoint;
-* symmetric U-Net N of L levels (2..6, 3 by default) with weights W0 (3->16,
+* symmetric U-Net N of L levels (2..8, 3 by default) with weights W0 (3->16,
   stride 1), W1 (16->32, stride 2), W2 (32->64, stride 2), W3.. (64->64,
   stride 2) and per-channel gate parameters (a_l, b_l):
       h0 = conv(v, W0, 1), h_l = conv(h_{l-1}, W_l, 2)
@@ -22,6 +22,11 @@ solve (femEqsCGwJ, fem/fem_eqs.c:88-237).  Definition:
   so N = sum_l H_l^T diag(g_l) H_l with H_l = conv chains: linear in v,
   symmetric and positive semi-definite; the number of levels is the number
   of W entries of the parameter dict;
+* the coarse-first form has no node-grid level: W0 is a stride-2
+  convolution 3->32 and the levels are W1 32->64, W2.. 64->64, so level l
+  lives on the grid of the node-grid form's level l + 1 and ebar starts
+  pooled once; the form is told by W0's output channels (16: node grid,
+  32: coarse first);
 * ebar_0 = log(E / 30 GPa) averaged over a node's bricks; ebar_{l+1} = mean
   of ebar_l over the 3x3x3 fine nodes around 2c (those inside the grid);
 * preconditioner: z = dinv r + mask dsq N(mask dsq r), dsq = sqrt(dinv);
@@ -33,12 +38,20 @@ from __future__ import annotations
 
 import numpy as np
 
-CH_ALL = (16, 32, 64, 64, 64, 64)
-CI_ALL = (3, 16, 32, 64, 64, 64)
+CH_ALL = (16, 32, 64, 64, 64, 64, 64, 64)
+CI_ALL = (3, 16, 32, 64, 64, 64, 64, 64)
 
 
-def layers(levels=3):
-    return [(f"W{l}", CI_ALL[l], CH_ALL[l], 1 if l == 0 else 2) for l in range(levels)]
+def layers(levels=3, fine=True):
+    if fine:
+        return [(f"W{l}", CI_ALL[l], CH_ALL[l], 1 if l == 0 else 2) for l in range(levels)]
+    ch = CH_ALL[1:levels + 1]
+    ci = (3,) + tuple(ch[:-1])
+    return [(f"W{l}", ci[l], ch[l], 2) for l in range(levels)]
+
+
+def is_fine(P):
+    return P["W0"].shape[1] == CH_ALL[0]
 
 
 CH = CH_ALL[:3]
@@ -122,8 +135,8 @@ def node_log_modulus(E_elem, nx, ny, nz):
     return acc / cnt
 
 
-def ebar_levels(e0, levels=3):
-    out = [e0]
+def ebar_levels(e0, levels=3, fine=True):
+    out = [e0 if fine else pool(e0)]
     for _ in range(1, levels):
         out.append(pool(out[-1]))
     return out
@@ -133,14 +146,14 @@ def n_levels(P):
     return sum(1 for k in P if k.startswith("W"))
 
 
-def init_params(seed=0, levels=3):
+def init_params(seed=0, levels=3, fine=True):
     rng = np.random.default_rng(seed)
     P = {}
-    for name, ci, co, _ in layers(levels):
+    for name, ci, co, _ in layers(levels, fine):
         P[name] = rng.standard_normal((27, co, ci)) / np.sqrt(27 * ci)
-    for l in range(levels):
-        P[f"a{l}"] = np.zeros(CH_ALL[l])
-        P[f"b{l}"] = np.zeros(CH_ALL[l])
+    for l, (_, _, co, _) in enumerate(layers(levels, fine)):
+        P[f"a{l}"] = np.zeros(co)
+        P[f"b{l}"] = np.zeros(co)
     return P
 
 
@@ -151,9 +164,10 @@ def gates(P, eb):
 def forward(P, v, eb):
     """v [D][H][W][3] -> N(v) [D][H][W][3], with the intermediate fields."""
     L = n_levels(P)
+    s0 = 1 if is_fine(P) else 2
     g = gates(P, eb)
     c = {"v": v, "g": g, "eb": eb}
-    h = [conv(v, P["W0"], 1)]
+    h = [conv(v, P["W0"], s0)]
     for l in range(1, L):
         h.append(conv(h[-1], P[f"W{l}"], 2))
     t = [None] * L
@@ -162,17 +176,18 @@ def forward(P, v, eb):
         t[l] = g[l] * h[l] + conv_adjoint(t[l + 1], P[f"W{l + 1}"], 2, h[l].shape[:3])
     for l in range(L):
         c[f"h{l}"], c[f"t{l}"] = h[l], t[l]
-    y = conv_adjoint(t[0], P["W0"], 1, v.shape[:3])
+    y = conv_adjoint(t[0], P["W0"], s0, v.shape[:3])
     return y, c
 
 
 def backward(P, c, dy):
     """gradient of <dy, N(v)> with respect to every parameter"""
     L = n_levels(P)
+    s0 = 1 if is_fine(P) else 2
     g = c["g"]
     G = {}
-    dt = [conv(dy, P["W0"], 1)]                  # adjoint of y = conv_adjoint(t0)
-    G["W0"] = conv_wgrad(dy, c["t0"], 1)
+    dt = [conv(dy, P["W0"], s0)]                 # adjoint of y = conv_adjoint(t0)
+    G["W0"] = conv_wgrad(dy, c["t0"], s0)
     for l in range(1, L):
         dt.append(conv(dt[-1], P[f"W{l}"], 2))
         G[f"W{l}"] = conv_wgrad(dt[l - 1], c[f"t{l}"], 2)
@@ -180,7 +195,7 @@ def backward(P, c, dy):
     for l in range(L - 1, 0, -1):
         G[f"W{l}"] += conv_wgrad(c[f"h{l - 1}"], dh, 2)
         dh = g[l - 1] * dt[l - 1] + conv_adjoint(dh, P[f"W{l}"], 2, c[f"h{l - 1}"].shape[:3])
-    G["W0"] += conv_wgrad(c["v"], dh, 1)
+    G["W0"] += conv_wgrad(c["v"], dh, s0)
     for l in range(L):
         q = g[l] * c[f"h{l}"] * dt[l]
         G[f"b{l}"] = q.reshape(-1, q.shape[-1]).sum(0)

@@ -123,13 +123,14 @@ def pack(P):
                           [P[f"{k}{l}"] for l in range(L) for k in ("a", "b")]).astype(np.float32)
 
 
-def unpack(v, levels=3):
+def unpack(v, levels=3, fine=True):
     P, o = {}, 0
-    for name, ci, co, _ in U.layers(levels):
+    lay = U.layers(levels, fine)
+    for name, ci, co, _ in lay:
         P[name] = v[o:o + 27 * co * ci].reshape(27, co, ci).astype(np.float64)
         o += 27 * co * ci
     for l in range(levels):
-        c = U.CH_ALL[l]
+        c = lay[l][2]
         for k in ("a", "b"):
             P[f"{k}{l}"] = v[o:o + c].astype(np.float64)
             o += c
@@ -137,14 +138,14 @@ def unpack(v, levels=3):
     return P
 
 
-def random_params(seed, levels=3):
+def random_params(seed, levels=3, fine=True):
     rng = np.random.default_rng(seed)
-    P = U.init_params(seed, levels)
+    P = U.init_params(seed, levels, fine)
     for l in range(levels):
-        c = U.CH_ALL[l]
+        c = P[f"a{l}"].size
         P[f"a{l}"] = 0.3 * rng.standard_normal(c)
         P[f"b{l}"] = 0.3 * rng.standard_normal(c)
-    return unpack(bf16(pack(P)), levels)      # weights exactly as the kernels see them
+    return unpack(bf16(pack(P)), levels, fine)      # weights exactly as the kernels see them
 
 
 @pytest.fixture(scope="module")
@@ -188,25 +189,27 @@ def test_network_gradient_matches_oracle(small_net):
         assert rel(a, r) <= tol and cos >= 0.999, (k, rel(a, r), cos)
 
 
-@pytest.mark.parametrize("levels", [2, 5])
-def test_deeper_network_forward_and_gradient_match_oracle(levels):
-    """the L-level network (down to grids of one node along an axis) against
-    the oracle: forward within 1e-5, parameter gradient as above"""
+@pytest.mark.parametrize("levels,fine", [(2, True), (5, True), (2, False), (4, False)])
+def test_deeper_network_forward_and_gradient_match_oracle(levels, fine):
+    """the L-level network (down to grids of one node along an axis) and the
+    coarse-first form against the oracle: forward within 1e-5, parameter
+    gradient as above"""
     D, H, W = 5, 4, 11
     shape = (D, H, W)
-    net = G.GpuNet(W, H, D, batch=2, levels=levels)
-    assert net.P == G.lib().ufem_net_nparams_levels(levels) and net.weights().size == net.P
+    net = G.GpuNet(W, H, D, batch=2, levels=levels, fine=fine)
+    assert net.P == G.lib().ufem_net_nparams_form(levels, int(fine)) and net.weights().size == net.P
     rng = np.random.default_rng(20 + levels)
-    P = random_params(30 + levels, levels)
+    P = random_params(30 + levels, levels, fine)
+    assert U.is_fine(P) == fine
     net.set_weights(pack(P))
     x = rng.standard_normal((2,) + shape + (3,))
     e0 = 0.5 * rng.standard_normal((2,) + shape)
     dy = rng.standard_normal((2,) + shape + (3,))
     y = net.forward(x.reshape(2, -1, 3), e0.reshape(2, -1))
-    g = unpack(net.backward(x.reshape(2, -1, 3), e0.reshape(2, -1), dy.reshape(2, -1, 3)), levels)
+    g = unpack(net.backward(x.reshape(2, -1, 3), e0.reshape(2, -1), dy.reshape(2, -1, 3)), levels, fine)
     ref = {k: 0.0 for k in P}
     for b in range(2):
-        eb = U.ebar_levels(e0[b].astype(np.float32).astype(np.float64), levels)
+        eb = U.ebar_levels(e0[b].astype(np.float32).astype(np.float64), levels, fine)
         yr, c = U.forward(P, x[b], eb)
         assert rel(y[b], yr) <= 1e-5
         gb = U.backward(P, c, dy[b])
@@ -222,7 +225,7 @@ def test_deeper_network_forward_and_gradient_match_oracle(levels):
 
 def test_levels_out_of_range():
     with pytest.raises(RuntimeError, match="levels"):
-        G.GpuNet(9, 5, 5, levels=7)
+        G.GpuNet(9, 5, 5, levels=9)
     assert G.lib().ufem_net_nparams_levels(1) == -1
     assert G.lib().ufem_net_nparams_levels(3) == G.lib().ufem_net_nparams() == 70640
 

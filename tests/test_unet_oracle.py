@@ -150,24 +150,27 @@ def test_energy_loss_value_and_planted_mistake():
     assert all(np.all(G[k] == 0) for k in ("W0", "W1", "W2"))
 
 
-def random_params_levels(seed, levels, gate_scale=0.3):
+def random_params_levels(seed, levels, gate_scale=0.3, fine=True):
     rng = np.random.default_rng(seed)
-    P = U.init_params(seed, levels)
+    P = U.init_params(seed, levels, fine)
     for l in range(levels):
-        c = U.CH_ALL[l]
+        c = P[f"a{l}"].size
         P[f"a{l}"] = gate_scale * rng.standard_normal(c)
         P[f"b{l}"] = gate_scale * rng.standard_normal(c)
     return P
 
 
-@pytest.mark.parametrize("levels", [2, 5])
-def test_deeper_network_is_symmetric_positive_semidefinite(levels):
-    """more levels (down to grids of one node along an axis) keep N linear,
-    symmetric and PSD; an untied decoder at the deepest level breaks it"""
+@pytest.mark.parametrize("levels,fine", [(2, True), (5, True), (2, False), (4, False)])
+def test_deeper_network_is_symmetric_positive_semidefinite(levels, fine):
+    """more levels (down to grids of one node along an axis), and the
+    coarse-first form, keep N linear, symmetric and PSD; an untied decoder at
+    the deepest level breaks it"""
     shape = (5, 4, 11)
-    P = random_params_levels(6, levels)
-    eb = U.ebar_levels(0.5 * np.random.default_rng(7).standard_normal(shape), levels)
+    P = random_params_levels(6, levels, fine=fine)
+    assert U.is_fine(P) == fine and U.n_levels(P) == levels
+    eb = U.ebar_levels(0.5 * np.random.default_rng(7).standard_normal(shape), levels, fine)
     assert len(eb) == levels and eb[-1].shape == tuple(U.coarse_shape(eb[-2].shape))
+    assert eb[0].shape == (shape if fine else tuple(U.coarse_shape(shape)))
     v1, v2 = RNG.standard_normal(shape + (3,)), RNG.standard_normal(shape + (3,))
     n1, n2 = U.forward(P, v1, eb)[0], U.forward(P, v2, eb)[0]
     a, b = np.sum(n1 * v2), np.sum(v1 * n2)
@@ -187,12 +190,13 @@ def test_deeper_network_is_symmetric_positive_semidefinite(levels):
         for l in range(L - 2, -1, -1):
             W = Pb[f"W{l + 1}"] if l == L - 2 else P[f"W{l + 1}"]
             t = c["g"][l] * c[f"h{l}"] + U.conv_adjoint(t, W, 2, c[f"h{l}"].shape[:3])
-        return U.conv_adjoint(t, P["W0"], 1, shape)
+        return U.conv_adjoint(t, P["W0"], 1 if fine else 2, shape)
     u1, u2 = untied(v1), untied(v2)
     assert abs(np.sum(u1 * v2) - np.sum(v1 * u2)) > 1e-6 * abs(np.sum(u1 * v2))
 
 
-def test_deeper_network_gradients_match_finite_differences():
+@pytest.mark.parametrize("fine", [True, False])
+def test_deeper_network_gradients_match_finite_differences(fine):
     shape = (5, 4, 9)
     levels = 4
     n = 3 * int(np.prod(shape))
@@ -202,8 +206,8 @@ def test_deeper_network_gradients_match_finite_differences():
     mask[:6] = 0.0
     e = RNG.standard_normal(n) * mask
     r = K @ e
-    P = random_params_levels(9, levels)
-    eb = U.ebar_levels(0.5 * np.random.default_rng(10).standard_normal(shape), levels)
+    P = random_params_levels(9, levels, fine=fine)
+    eb = U.ebar_levels(0.5 * np.random.default_rng(10).standard_normal(shape), levels, fine)
     L, G, _ = U.energy_loss(P, K, r, e, dinv, mask, eb, shape)
     rng = np.random.default_rng(11)
     h = 1e-4      # the deep gates' gradients are small: a smaller step is lost in rounding

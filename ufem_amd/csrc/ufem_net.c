@@ -1,6 +1,6 @@
 /*
  * ufem_net.c -- the learned preconditioner: a symmetric L-level 3-D
- * convolution U-Net (L = 2..6, 3 by default) on the structured node grid of
+ * convolution U-Net (L = 2..8, 3 by default) on the structured node grid of
  * a box mesh, trained and applied on the GPU by the kernels of ugk_unet.hip.
  *
  * The network has no counterpart in the reference; it accelerates the
@@ -45,12 +45,18 @@
 
 #define E_REF 30e9
 
-#define NL_MAX 6
-static const int CH[NL_MAX] = {16, 32, 64, 64, 64, 64};   /* output channels of C0, C1, .. */
-static const int CI[NL_MAX] = {3, 16, 32, 64, 64, 64};    /* input channels                */
+#define NL_MAX 8
+/* channels of the node-grid form: output and input of C0, C1, ..; the
+ * coarse-first form drops C0 and starts with a stride-2 3 -> 32 layer */
+static const int CH_F[NL_MAX + 1] = {16, 32, 64, 64, 64, 64, 64, 64, 64};
+static const int CI_F[NL_MAX + 1] = {3, 16, 32, 64, 64, 64, 64, 64, 64};
 
 struct ufem_net {
   int L;                          /* levels */
+  int fine;                       /* 1: C0 is stride 1 on the node grid; 0: coarse first */
+  int ch[NL_MAX], ci[NL_MAX];     /* output and input channels of C_l */
+  ugk_grid3 gf;                   /* the node grid (network input and output) */
+  long Vf;
   ugk_grid3 g[NL_MAX];
   long V[NL_MAX], Vt;             /* nodes per level, and their sum */
   int nbmax;
@@ -100,16 +106,35 @@ static ugk_grid3 coarse(ugk_grid3 g)
   return c;
 }
 
-long ufem_net_nparams_levels(int levels)
+static void channels(int levels, int fine, int *ch, int *ci)
 {
-  long P = 0;
   int l;
+  for (l = 0; l < levels; l++) {
+    ch[l] = fine ? CH_F[l] : CH_F[l + 1];
+    ci[l] = fine ? CI_F[l] : (l ? CH_F[l] : 3);
+  }
+}
+
+long ufem_net_nparams_form(int levels, int fine)
+{
+  int ch[NL_MAX], ci[NL_MAX], l;
+  long P = 0;
   if (levels < 2 || levels > NL_MAX) return -1;
-  for (l = 0; l < levels; l++) P += 27L * CI[l] * CH[l] + 2L * CH[l];
+  channels(levels, fine, ch, ci);
+  for (l = 0; l < levels; l++) P += 27L * ci[l] * ch[l] + 2L * ch[l];
   return P;
 }
 
+long ufem_net_nparams_levels(int levels) { return ufem_net_nparams_form(levels, 1); }
 long ufem_net_nparams(void) { return ufem_net_nparams_levels(3); }
+int ufem_net_fine(const ufem_net *N) { return N->fine; }
+
+/* grid of level lev (-1: the node grid); stride of C_l and the level it reads */
+static ugk_grid3 grid_of(const ufem_net *N, int lev) { return lev < 0 ? N->gf : N->g[lev]; }
+static int stride_of(const ufem_net *N, int l) { return (l == 0 && N->fine) ? 1 : 2; }
+static int lev_in_of(const ufem_net *N, int l) { return (l == 0 && N->fine) ? 0 : l - 1; }
+/* the level of the network's input and output: the node grid */
+static int io_lev(const ufem_net *N) { return N->fine ? 0 : -1; }
 int ufem_net_levels(const ufem_net *N) { return N->L; }
 
 static void *dalloc(size_t bytes, int *bad)
@@ -154,8 +179,8 @@ static ugk_dw_desc dw_desc(const ufem_net *N, int l, const float *x, int ldx, in
   d.cip = r16(cin); d.cop = r16(cout);
   d.mode = stride == 2 ? UGK_CONV_S2 : UGK_CONV_S1;
   d.nb = nb;
-  d.gi = N->g[lev_in];
-  d.go = N->g[stride == 2 ? lev_in + 1 : lev_in];
+  d.gi = grid_of(N, lev_in);
+  d.go = grid_of(N, stride == 2 ? lev_in + 1 : lev_in);
   (void)l;
   return d;
 }
@@ -167,6 +192,11 @@ ufem_net *ufem_net_create(int W, int H, int D, int batch)
 
 ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels)
 {
+  return ufem_net_create_form(W, H, D, batch, levels, 1);
+}
+
+ufem_net *ufem_net_create_form(int W, int H, int D, int batch, int levels, int fine)
+{
   ufem_net *N;
   int l, bad = 0;
   long nb, maxpart = 0, n;
@@ -176,7 +206,11 @@ ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels)
   N = (ufem_net *)calloc(1, sizeof(*N));
   if (!N) return NULL;
   N->L = levels;
-  N->g[0].D = D; N->g[0].H = H; N->g[0].W = W;
+  N->fine = fine ? 1 : 0;
+  channels(levels, N->fine, N->ch, N->ci);
+  N->gf.D = D; N->gf.H = H; N->gf.W = W;
+  N->Vf = (long)D * H * W;
+  N->g[0] = N->fine ? N->gf : coarse(N->gf);
   for (l = 1; l < levels; l++) N->g[l] = coarse(N->g[l - 1]);
   N->Vt = 0;
   for (l = 0; l < levels; l++) {
@@ -185,51 +219,51 @@ ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels)
   }
   N->nbmax = batch;
   nb = batch;
-  n = 3 * N->V[0];
+  n = 3 * N->Vf;
   N->stream = ugk_stream_create();
   N->ev[0] = ugk_event_create();
   N->ev[1] = ugk_event_create();
   if (!N->stream || !N->ev[0] || !N->ev[1]) { net_fail("stream"); ufem_net_destroy(N); return NULL; }
   N->P = 0;
-  for (l = 0; l < levels; l++) { N->offW[l] = N->P; N->P += 27L * CI[l] * CH[l]; }
+  for (l = 0; l < levels; l++) { N->offW[l] = N->P; N->P += 27L * N->ci[l] * N->ch[l]; }
   for (l = 0; l < levels; l++) {
-    N->offA[l] = N->P; N->P += CH[l];
-    N->offB[l] = N->P; N->P += CH[l];
+    N->offA[l] = N->P; N->P += N->ch[l];
+    N->offB[l] = N->P; N->P += N->ch[l];
   }
   N->w = (float *)dalloc(sizeof(float) * N->P, &bad);
   N->gr = (float *)dalloc(sizeof(float) * N->P, &bad);
   N->m = (float *)dalloc(sizeof(float) * N->P, &bad);
   N->v = (float *)dalloc(sizeof(float) * N->P, &bad);
   for (l = 0; l < levels; l++) {
-    const size_t b = sizeof(unsigned short) * 27 * r16(CI[l]) * r16(CH[l]);
+    const size_t b = sizeof(unsigned short) * 27 * r16(N->ci[l]) * r16(N->ch[l]);
     const long Vl = N->V[l];
     N->wf[l] = (unsigned short *)dalloc(b, &bad);
     N->wt[l] = (unsigned short *)dalloc(b, &bad);
-    N->h[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
-    N->t[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
-    N->gt[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
-    N->dt[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
-    N->dh[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * CH[l]), &bad);
+    N->h[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * N->ch[l]), &bad);
+    N->t[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * N->ch[l]), &bad);
+    N->gt[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * N->ch[l]), &bad);
+    N->dt[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * N->ch[l]), &bad);
+    N->dh[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl * N->ch[l]), &bad);
     N->eb[l] = (float *)dalloc(sizeof(float) * (size_t)(nb * Vl), &bad);
-    N->gi[l] = (float *)dalloc(sizeof(float) * (size_t)(Vl * CH[l]), &bad);
+    N->gi[l] = (float *)dalloc(sizeof(float) * (size_t)(Vl * N->ch[l]), &bad);
     {
       /* weight-gradient partials: the largest of the calls the backward makes */
-      ugk_dw_desc d = dw_desc(N, l, NULL, 4, CI[l], NULL, 4, CH[l], l ? 2 : 1, l ? l - 1 : 0, batch);
+      ugk_dw_desc d = dw_desc(N, l, NULL, 4, N->ci[l], NULL, 4, N->ch[l], stride_of(N, l), lev_in_of(N, l), batch);
       long need = (long)ugk_conv_dw_slices(&d) * 27L * d.cop * d.cip;
       if (need > maxpart) maxpart = need;
     }
   }
   N->dwpart = (float *)dalloc(sizeof(float) * maxpart, &bad);
   N->gpart = (double *)dalloc(sizeof(double) * 512 * 2 * 64, &bad);
-  N->x0 = (float *)dalloc(sizeof(float) * (size_t)(nb * N->V[0] * 4), &bad);
-  N->y = (float *)dalloc(sizeof(float) * (size_t)(nb * N->V[0] * 4), &bad);
-  N->dy = (float *)dalloc(sizeof(float) * (size_t)(nb * N->V[0] * 4), &bad);
+  N->x0 = (float *)dalloc(sizeof(float) * (size_t)(nb * N->Vf * 4), &bad);
+  N->y = (float *)dalloc(sizeof(float) * (size_t)(nb * N->Vf * 4), &bad);
+  N->dy = (float *)dalloc(sizeof(float) * (size_t)(nb * N->Vf * 4), &bad);
   N->zz = (float *)dalloc(sizeof(float) * (size_t)n, &bad);
   N->kz = (float *)dalloc(sizeof(float) * (size_t)n, &bad);
   N->dpart = (double *)dalloc(sizeof(double) * 1024 * (size_t)(nb > 4 ? nb : 4), &bad);
   N->hpart = (double *)malloc(sizeof(double) * 1024 * (size_t)(nb > 4 ? nb : 4));
-  N->yi = (float *)dalloc(sizeof(float) * (size_t)(N->V[0] * 4), &bad);
-  N->x0s = (unsigned short *)dalloc(3 * sizeof(unsigned short) * (size_t)(N->V[0] * 4), &bad);
+  N->yi = (float *)dalloc(sizeof(float) * (size_t)(N->Vf * 4), &bad);
+  N->x0s = (unsigned short *)dalloc(3 * sizeof(unsigned short) * (size_t)(N->Vf * 4), &bad);
   N->ebi = (float *)dalloc(sizeof(float) * (size_t)N->Vt, &bad);
   if (bad || !N->hpart) { net_fail("network buffers"); ufem_net_destroy(N); return NULL; }
   return N;
@@ -241,14 +275,14 @@ static int adam_all(ufem_net *N, float lr)
   int l;
   if (lr > 0) N->step++;
   for (l = 0; l < N->L; l++) {
-    if (ugk_adam(27L * CI[l] * CH[l], CH[l], CI[l], r16(CH[l]), r16(CI[l]), N->w + N->offW[l], N->gr + N->offW[l],
+    if (ugk_adam(27L * N->ci[l] * N->ch[l], N->ch[l], N->ci[l], r16(N->ch[l]), r16(N->ci[l]), N->w + N->offW[l], N->gr + N->offW[l],
                  N->m + N->offW[l], N->v + N->offW[l], N->wf[l], N->wt[l], lr, 0.9f, 0.999f, 1e-8f, N->step,
                  N->stream))
       return net_fail("weights");
     if (lr > 0 &&
-        (ugk_adam(CH[l], 1, CH[l], 0, 0, N->w + N->offA[l], N->gr + N->offA[l], N->m + N->offA[l],
+        (ugk_adam(N->ch[l], 1, N->ch[l], 0, 0, N->w + N->offA[l], N->gr + N->offA[l], N->m + N->offA[l],
                   N->v + N->offA[l], NULL, NULL, lr, 0.9f, 0.999f, 1e-8f, N->step, N->stream) ||
-         ugk_adam(CH[l], 1, CH[l], 0, 0, N->w + N->offB[l], N->gr + N->offB[l], N->m + N->offB[l],
+         ugk_adam(N->ch[l], 1, N->ch[l], 0, 0, N->w + N->offB[l], N->gr + N->offB[l], N->m + N->offB[l],
                   N->v + N->offB[l], NULL, NULL, lr, 0.9f, 0.999f, 1e-8f, N->step, N->stream)))
       return net_fail("gate parameters");
   }
@@ -289,11 +323,11 @@ int ufem_net_init(ufem_net *N, unsigned long seed)
   if (!w) return AF_ERR_MEM;
   for (l = 0; l < N->L; l++) {
     long k;
-    for (k = 0; k < CH[l]; k++) w[N->offB[l] + k] = -3.f;
+    for (k = 0; k < N->ch[l]; k++) w[N->offB[l] + k] = -3.f;
   }
   for (l = 0; l < N->L; l++) {
-    const long Pl = 27L * CI[l] * CH[l];
-    const double sd = 1.0 / sqrt(27.0 * CI[l]);
+    const long Pl = 27L * N->ci[l] * N->ch[l];
+    const double sd = 1.0 / sqrt(27.0 * N->ci[l]);
     long k;
     for (k = 0; k < Pl; k++, i++) {
       unsigned long long a = mix64((unsigned long long)seed * 0x632BE59BD9B4E019ULL + 2ULL * (unsigned long long)i);
@@ -319,16 +353,16 @@ static int conv_f(ufem_net *N, const void *x, int xf, int ldx, int cin, int l, i
   memset(&d, 0, sizeof(d));
   d.x = x; d.ldx = ldx; d.cin = cin; d.x_f32 = xf;
   d.w = adjoint ? N->wt[l] : N->wf[l];
-  d.cip = adjoint ? r16(CH[l]) : r16(CI[l]);
-  d.cop = adjoint ? r16(CI[l]) : r16(CH[l]);
+  d.cip = adjoint ? r16(N->ch[l]) : r16(N->ci[l]);
+  d.cop = adjoint ? r16(N->ci[l]) : r16(N->ch[l]);
   d.y = y; d.ldy = ldy; d.cout = cout; d.y_f32 = 1;
   d.emul = emul; d.ldm = ldg;
   d.ga = ga; d.ldga = ldg; d.gb = gb; d.ldgb = ldg;
-  d.mode = l == 0 ? UGK_CONV_S1 : (adjoint ? UGK_CONV_T2 : UGK_CONV_S2);
-  d.flip = (l == 0 && adjoint) ? 1 : 0;
+  d.mode = stride_of(N, l) == 1 ? UGK_CONV_S1 : (adjoint ? UGK_CONV_T2 : UGK_CONV_S2);
+  d.flip = (stride_of(N, l) == 1 && adjoint) ? 1 : 0;
   d.nb = nb;
-  d.gi = N->g[lev_in];
-  d.go = N->g[lev_out];
+  d.gi = grid_of(N, lev_in);
+  d.go = grid_of(N, lev_out);
   d.skip = skip;
   return ugk_conv(&d, stream);
 }
@@ -351,17 +385,17 @@ static int net_stage(ufem_net *N, int k, const float *v, float *const *g, float 
                      void *st)
 {
   const int L = N->L;
-  if (k == 0) return conv(N, v, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, 0, nb, skip, st);
+  if (k == 0) return conv(N, v, 4, 4, 0, 0, N->h[0], N->ch[0], N->ch[0], io_lev(N), 0, NULL, NULL, 0, nb, skip, st);
   if (k < L)
-    return conv(N, N->h[k - 1], CH[k - 1], CH[k - 1], k, 0, N->h[k], CH[k], CH[k], k - 1, k, NULL, NULL, 0, nb,
+    return conv(N, N->h[k - 1], N->ch[k - 1], N->ch[k - 1], k, 0, N->h[k], N->ch[k], N->ch[k], k - 1, k, NULL, NULL, 0, nb,
                 skip, st);
-  if (k == L) return ugk_mul((long)nb * N->V[L - 1] * CH[L - 1], g[L - 1], N->h[L - 1], N->t[L - 1], skip, st);
+  if (k == L) return ugk_mul((long)nb * N->V[L - 1] * N->ch[L - 1], g[L - 1], N->h[L - 1], N->t[L - 1], skip, st);
   if (k < 2 * L) {
     const int l = 2 * L - 1 - k;
-    return conv(N, N->t[l + 1], CH[l + 1], CH[l + 1], l + 1, 1, N->t[l], CH[l], CH[l], l + 1, l, g[l], N->h[l],
-                CH[l], nb, skip, st);
+    return conv(N, N->t[l + 1], N->ch[l + 1], N->ch[l + 1], l + 1, 1, N->t[l], N->ch[l], N->ch[l], l + 1, l, g[l], N->h[l],
+                N->ch[l], nb, skip, st);
   }
-  return conv(N, N->t[0], 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, 0, nb, skip, st);
+  return conv(N, N->t[0], N->ch[0], N->ch[0], 0, 1, y, 4, 3, 0, io_lev(N), NULL, NULL, 0, nb, skip, st);
 }
 
 static int net_forward(ufem_net *N, const float *v, float *const *g, float *y, int nb, const int *skip,
@@ -385,19 +419,21 @@ static int infer_stages(const ufem_net *N) { return 2 * N->L; }
 static int infer_stage(ufem_net *N, int k, float *const *g, float *y, const int *skip, void *st)
 {
   const int F = UGK_ACT_F32, L = N->L;
-  if (k == 0) return conv_f(N, N->x0s, UGK_ACT_SPLIT, 4, 4, 0, 0, N->h[0], 16, 16, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
+  if (k == 0)
+    return conv_f(N, N->x0s, UGK_ACT_SPLIT, 4, 4, 0, 0, N->h[0], N->ch[0], N->ch[0], io_lev(N), 0, NULL, NULL, NULL, 0, 1,
+                  skip, st);
   if (k < L - 1)
-    return conv_f(N, N->h[k - 1], F, CH[k - 1], CH[k - 1], k, 0, N->h[k], CH[k], CH[k], k - 1, k, NULL, NULL, NULL,
+    return conv_f(N, N->h[k - 1], F, N->ch[k - 1], N->ch[k - 1], k, 0, N->h[k], N->ch[k], N->ch[k], k - 1, k, NULL, NULL, NULL,
                   0, 1, skip, st);
   if (k == L - 1)
-    return conv_f(N, N->h[k - 1], F, CH[k - 1], CH[k - 1], k, 0, N->t[k], CH[k], CH[k], k - 1, k, g[k], NULL, NULL,
-                  CH[k], 1, skip, st);
+    return conv_f(N, N->h[k - 1], F, N->ch[k - 1], N->ch[k - 1], k, 0, N->t[k], N->ch[k], N->ch[k], k - 1, k, g[k], NULL, NULL,
+                  N->ch[k], 1, skip, st);
   if (k < 2 * L - 1) {
     const int l = 2 * L - 2 - k;
-    return conv_f(N, N->t[l + 1], F, CH[l + 1], CH[l + 1], l + 1, 1, N->t[l], CH[l], CH[l], l + 1, l, NULL, g[l],
-                  N->h[l], CH[l], 1, skip, st);
+    return conv_f(N, N->t[l + 1], F, N->ch[l + 1], N->ch[l + 1], l + 1, 1, N->t[l], N->ch[l], N->ch[l], l + 1, l, NULL, g[l],
+                  N->h[l], N->ch[l], 1, skip, st);
   }
-  return conv_f(N, N->t[0], F, 16, 16, 0, 1, y, 4, 3, 0, 0, NULL, NULL, NULL, 0, 1, skip, st);
+  return conv_f(N, N->t[0], F, N->ch[0], N->ch[0], 0, 1, y, 4, 3, 0, io_lev(N), NULL, NULL, NULL, 0, 1, skip, st);
 }
 
 static int infer(ufem_net *N, float *const *g, float *y, const int *skip, void *st)
@@ -411,7 +447,7 @@ static int infer(ufem_net *N, float *const *g, float *y, const int *skip, void *
 static int wgrad(ufem_net *N, int l, const float *x, int ldx, int cin, const float *dy, int ldd, int lev_in,
                  int nb, int acc)
 {
-  ugk_dw_desc d = dw_desc(N, l, x, ldx, cin, dy, ldd, CH[l], l ? 2 : 1, lev_in, nb);
+  ugk_dw_desc d = dw_desc(N, l, x, ldx, cin, dy, ldd, N->ch[l], stride_of(N, l), lev_in, nb);
   return ugk_conv_dw(&d, N->dwpart, ugk_conv_dw_slices(&d), N->gr + N->offW[l], acc, N->stream);
 }
 
@@ -425,24 +461,24 @@ static int net_backward(ufem_net *N, const float *v, const float *dy, int nb)
   float *const *g = N->gt;
   const int L = N->L;
   int l;
-  if (conv(N, dy, 4, 4, 0, 0, N->dt[0], 16, 16, 0, 0, NULL, NULL, 0, nb, NULL, N->stream) ||
-      wgrad(N, 0, dy, 4, 3, N->t[0], 16, 0, nb, 0))
+  if (conv(N, dy, 4, 4, 0, 0, N->dt[0], N->ch[0], N->ch[0], io_lev(N), 0, NULL, NULL, 0, nb, NULL, N->stream) ||
+      wgrad(N, 0, dy, 4, 3, N->t[0], N->ch[0], io_lev(N), nb, 0))
     return net_fail("network backward");
   for (l = 1; l < L; l++)
-    if (conv(N, N->dt[l - 1], CH[l - 1], CH[l - 1], l, 0, N->dt[l], CH[l], CH[l], l - 1, l, NULL, NULL, 0, nb, NULL,
+    if (conv(N, N->dt[l - 1], N->ch[l - 1], N->ch[l - 1], l, 0, N->dt[l], N->ch[l], N->ch[l], l - 1, l, NULL, NULL, 0, nb, NULL,
              N->stream) ||
-        wgrad(N, l, N->dt[l - 1], CH[l - 1], CH[l - 1], N->t[l], CH[l], l - 1, nb, 0))
+        wgrad(N, l, N->dt[l - 1], N->ch[l - 1], N->ch[l - 1], N->t[l], N->ch[l], l - 1, nb, 0))
       return net_fail("network backward");
-  if (ugk_mul((long)nb * N->V[L - 1] * CH[L - 1], g[L - 1], N->dt[L - 1], N->dh[L - 1], NULL, N->stream))
+  if (ugk_mul((long)nb * N->V[L - 1] * N->ch[L - 1], g[L - 1], N->dt[L - 1], N->dh[L - 1], NULL, N->stream))
     return net_fail("network backward");
   for (l = L - 1; l >= 1; l--)
-    if (conv(N, N->dh[l], CH[l], CH[l], l, 1, N->dh[l - 1], CH[l - 1], CH[l - 1], l, l - 1, g[l - 1], N->dt[l - 1],
-             CH[l - 1], nb, NULL, N->stream) ||
-        wgrad(N, l, N->h[l - 1], CH[l - 1], CH[l - 1], N->dh[l], CH[l], l - 1, nb, 1))
+    if (conv(N, N->dh[l], N->ch[l], N->ch[l], l, 1, N->dh[l - 1], N->ch[l - 1], N->ch[l - 1], l, l - 1, g[l - 1], N->dt[l - 1],
+             N->ch[l - 1], nb, NULL, N->stream) ||
+        wgrad(N, l, N->h[l - 1], N->ch[l - 1], N->ch[l - 1], N->dh[l], N->ch[l], l - 1, nb, 1))
       return net_fail("network backward");
-  if (wgrad(N, 0, v, 4, 3, N->dh[0], 16, 0, nb, 1)) return net_fail("network backward");
+  if (wgrad(N, 0, v, 4, 3, N->dh[0], N->ch[0], io_lev(N), nb, 1)) return net_fail("network backward");
   for (l = 0; l < L; l++)
-    if (ugk_gate_grad((long)nb * N->V[l], CH[l], g[l], N->h[l], N->dt[l], N->eb[l], N->gpart,
+    if (ugk_gate_grad((long)nb * N->V[l], N->ch[l], g[l], N->h[l], N->dt[l], N->eb[l], N->gpart,
                       N->gr + N->offA[l], N->gr + N->offB[l], N->stream))
       return net_fail("gate gradient");
   return AF_OK;
@@ -452,7 +488,7 @@ static int make_gates(ufem_net *N, float *const *eb, float *const *g, int nb, vo
 {
   int l;
   for (l = 0; l < N->L; l++)
-    if (ugk_gates((long)nb * N->V[l], CH[l], eb[l], N->w + N->offA[l], N->w + N->offB[l], g[l], st))
+    if (ugk_gates((long)nb * N->V[l], N->ch[l], eb[l], N->w + N->offA[l], N->w + N->offB[l], g[l], st))
       return net_fail("gates");
   return AF_OK;
 }
@@ -482,7 +518,8 @@ static void levels_host(const ufem_net *N, const float *e0, float *out)
 {
   long off = 0;
   int l;
-  memcpy(out, e0, sizeof(float) * (size_t)N->V[0]);
+  if (N->fine) memcpy(out, e0, sizeof(float) * (size_t)N->V[0]);
+  else pool_host(e0, N->gf, out, N->g[0]);
   for (l = 1; l < N->L; l++) {
     pool_host(out + off, N->g[l - 1], out + off + N->V[l - 1], N->g[l]);
     off += N->V[l - 1];
@@ -497,7 +534,7 @@ static int upload_levels(ufem_net *N, const float *eb0, int nb)
   if (!lv) return AF_ERR_MEM;
   for (b = 0; b < nb && rv == AF_OK; b++) {
     long off = 0;
-    levels_host(N, eb0 + b * N->V[0], lv);
+    levels_host(N, eb0 + b * N->Vf, lv);
     for (l = 0; l < N->L; l++) {
       if (ugk_h2d(N->eb[l] + b * N->V[l], lv + off, sizeof(float) * (size_t)N->V[l], N->stream) ||
           ugk_sync(N->stream))
@@ -511,7 +548,7 @@ static int upload_levels(ufem_net *N, const float *eb0, int nb)
 
 static int upload_input(ufem_net *N, const float *x, int nb)
 {
-  const long V = N->V[0];
+  const long V = N->Vf;
   float *h = (float *)calloc((size_t)(nb * V * 4), sizeof(float));
   long i;
   int c, rv;
@@ -525,7 +562,7 @@ static int upload_input(ufem_net *N, const float *x, int nb)
 
 int ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, float *y)
 {
-  const long V = N->V[0];
+  const long V = N->Vf;
   float *h;
   long i;
   int c;
@@ -537,7 +574,7 @@ int ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, floa
   for (b = 0; b < nb; b++) {
     float *g[NL_MAX];
     int l;
-    for (l = 0; l < N->L; l++) g[l] = N->gt[l] + (long)b * N->V[l] * CH[l];
+    for (l = 0; l < N->L; l++) g[l] = N->gt[l] + (long)b * N->V[l] * N->ch[l];
     if (ugk_to_split(V, N->x0 + (long)b * V * 4, N->x0s, N->stream)) return net_fail("split input");
     if (infer(N, g, N->y + (long)b * V * 4, NULL, N->stream)) return AF_ERR;
   }
@@ -554,7 +591,7 @@ int ufem_net_forward(ufem_net *N, int nb, const float *x, const float *eb0, floa
  * weights), for nb samples */
 int ufem_net_backward(ufem_net *N, int nb, const float *x, const float *eb0, const float *dy, float *grad)
 {
-  const long V = N->V[0];
+  const long V = N->Vf;
   float *h;
   long i;
   int c, rv;
@@ -579,8 +616,8 @@ int ufem_net_backward(ufem_net *N, int nb, const float *x, const float *eb0, con
 /* ---------------- binding, training pairs, training ---------------------- */
 int ufem_net_bind(ufem_net *N, ufem_sys *s, const double *E)
 {
-  const ugk_grid3 g = N->g[0];
-  const long V = N->V[0], nx = g.W - 1, ny = g.H - 1, nz = g.D - 1;
+  const ugk_grid3 g = N->gf;
+  const long V = N->Vf, nx = g.W - 1, ny = g.H - 1, nz = g.D - 1;
   const long Vt = N->Vt;
   double *acc;
   int *cnt;
@@ -631,7 +668,7 @@ out:
 
 int ufem_net_reserve(ufem_net *N, long cap)
 {
-  const long V = N->V[0], n = 3 * V;
+  const long V = N->Vf, n = 3 * V;
   float *pv, *prr, *pee;
   double *per;
   int *pf;
@@ -766,7 +803,7 @@ long ufem_net_collect(ufem_net *N, int pairs, double eps, int every, ufem_solve_
     /* skip converged snapshots, whose error is at the solve's own accuracy */
     if (!(a2 > 0.0) || !(er > 1e-14 * er0)) continue;
     sc = sqrt(a2 / (double)s->n);
-    if (ugk_pair_write(s->n, x, r, s->d_u, s->w.dinv, N->mask, sc, N->pv + N->count * N->V[0] * 4,
+    if (ugk_pair_write(s->n, x, r, s->d_u, s->w.dinv, N->mask, sc, N->pv + N->count * N->Vf * 4,
                        N->prr + N->count * s->n, N->pee + N->count * s->n, N->stream))
       return net_fail("pair write");
     N->per[N->count] = er / (sc * sc);
@@ -787,11 +824,11 @@ static int pair_loss(ufem_net *N, int b, long q, int nb, double *part_out)
   ugk_mat Kf = s->K;
   int G;
   Kf.d_vals32 = N->fvals[f];
-  if (ugk_el_z(n, N->prr + q * n, N->fdinv[f], N->mask, N->y + b * N->V[0] * 4, N->zz, N->stream) ||
+  if (ugk_el_z(n, N->prr + q * n, N->fdinv[f], N->mask, N->y + b * N->Vf * 4, N->zz, N->stream) ||
       ugk_spmv_f32(&Kf, N->zz, N->kz, N->stream))
     return net_fail("loss");
   G = ugk_el_grad(n, N->prr + q * n, N->pee + q * n, N->zz, N->kz, N->fdinv[f], N->mask,
-                  (float)(-2.0 / (N->per[q] * nb)), N->dy + b * N->V[0] * 4, part_out, N->stream);
+                  (float)(-2.0 / (N->per[q] * nb)), N->dy + b * N->Vf * 4, part_out, N->stream);
   if (G < 0) return net_fail("loss");
   return G;
 }
@@ -803,7 +840,7 @@ static int load_batch(ufem_net *N, const long *idx, int nb)
   for (b = 0; b < nb; b++) {
     const long q = idx[b];
     long off = 0;
-    if (ugk_d2d(N->x0 + b * N->V[0] * 4, N->pv + q * N->V[0] * 4, sizeof(float) * (size_t)(N->V[0] * 4),
+    if (ugk_d2d(N->x0 + b * N->Vf * 4, N->pv + q * N->Vf * 4, sizeof(float) * (size_t)(N->Vf * 4),
                 N->stream))
       return net_fail("batch copy");
     for (l = 0; l < N->L; l++) {
@@ -921,9 +958,9 @@ int ufem_net_infer(ufem_net *N, const int *skip, void *stream)
 /* pair i as stored: v [V][4], rr [n], ee [n], e.r and its field index */
 int ufem_net_get_pair(ufem_net *N, long i, float *v, float *rr, float *ee, double *er, int *field)
 {
-  const long n = 3 * N->V[0];
+  const long n = 3 * N->Vf;
   if (i < 0 || i >= N->count) { ufem_set_error("pair %ld out of range", i); return AF_ERR_VAL; }
-  if (ugk_d2h(v, N->pv + i * N->V[0] * 4, sizeof(float) * (size_t)(N->V[0] * 4), N->stream) ||
+  if (ugk_d2h(v, N->pv + i * N->Vf * 4, sizeof(float) * (size_t)(N->Vf * 4), N->stream) ||
       ugk_d2h(rr, N->prr + i * n, sizeof(float) * (size_t)n, N->stream) ||
       ugk_d2h(ee, N->pee + i * n, sizeof(float) * (size_t)n, N->stream))
     return net_fail("pair download");

@@ -81,7 +81,10 @@ def lib():
             "ufem_net_destroy": (None, [C.c_void_p]),
             "ufem_net_nparams": (C.c_long, []),
             "ufem_net_create_levels": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "ufem_net_create_form": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "ufem_net_nparams_levels": (C.c_long, [C.c_int]),
+            "ufem_net_nparams_form": (C.c_long, [C.c_int, C.c_int]),
+            "ufem_net_fine": (C.c_int, [C.c_void_p]),
             "ufem_net_levels": (C.c_int, [C.c_void_p]),
             "ufem_net_init": (C.c_int, [C.c_void_p, C.c_ulong]),
             "ufem_net_set_weights": (C.c_int, [C.c_void_p, P_F]),
@@ -400,16 +403,18 @@ def _pf(a):
 
 class GpuNet:
     """The learned preconditioner's U-Net (ufem_net_*) on a W x H x D node
-    grid, with `levels` levels (2..6)."""
+    grid, with `levels` levels (2..8); fine=False: the coarse-first form
+    (no node-grid level)."""
 
-    def __init__(self, W, H, D, batch=1, levels=3):
-        self.h = lib().ufem_net_create_levels(W, H, D, batch, levels)
+    def __init__(self, W, H, D, batch=1, levels=3, fine=True):
+        self.h = lib().ufem_net_create_form(W, H, D, batch, levels, 1 if fine else 0)
         if not self.h:
             raise RuntimeError("ufem_net_create: " + last_error())
         self.shape = (D, H, W)
         self.V = W * H * D
         self.levels = levels
-        self.P = lib().ufem_net_nparams_levels(levels)
+        self.fine = fine
+        self.P = lib().ufem_net_nparams_form(levels, 1 if fine else 0)
 
     def close(self):
         if getattr(self, "h", None):
