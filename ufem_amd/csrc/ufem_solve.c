@@ -403,6 +403,27 @@ static int fcg_step(void *c, long it, void *stream)
   return ugk_fcg_update(f->K, f->n, f->x, w->r, w->fP, w->fQ, f->m, w->dinv, w->st, w->part, stream);
 }
 
+/* one kept direction: [refresh], network, [coarse term], combine, w = K z,
+ * update (which also writes the next network input); p and q in ring slot
+ * 0, w in the workspace's q */
+static int fcg1_step(void *c, long it, void *stream)
+{
+  fcg_ctx *f = (fcg_ctx *)c;
+  ufem_work *w = f->w;
+  unsigned short *x0 = f->net ? ufem_net_input(f->net) : NULL;
+  (void)it;
+  if (f->mixed && ugk_f1_refresh(f->K, f->n, f->b, f->x, w->r, w->dinv, w->fP, w->fQ, x0, w->st, w->part, stream))
+    return -1;
+  if ((f->net && ufem_net_infer(f->net, &w->st->halt, stream)) ||
+      (f->cc && ugk_cc_apply(f->cc, f->n, w->r, f->cc->d_zc, w->st, stream)) ||
+      ugk_f1_combine(f->n, w->r, w->dinv, f->net ? ufem_net_mask(f->net) : NULL,
+                     f->net ? ufem_net_output(f->net) : NULL, f->cc ? f->cc->d_zc : NULL, w->fQ, w->z, w->st,
+                     w->part, stream) ||
+      ugk_f1_w(f->K, w->z, w->q, f->mixed, w->st, w->part, stream))
+    return -1;
+  return ugk_f1_update(f->K, f->n, f->x, w->r, w->fP, w->fQ, w->z, w->q, w->dinv, x0, w->st, w->part, stream);
+}
+
 static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
                          const ufem_solve_opts *o, long maxit, int mixed, ufem_solve_info *info)
 {
@@ -444,12 +465,20 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
     h->max_gap = o->inner_maxit > 0 ? o->inner_maxit : 500;
     h->ru_req = -1;
     h->ru_at = 0;
+    h->restart = o->inner_restart ? 1 : 0;   /* 0: a refresh keeps the direction */
     if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
   }
   if (N && ufem_net_prepare(N, stream)) return AF_ERR;
   if (cc && ufem_coarse_build(w->cc, stream) != AF_OK) return AF_ERR;
-  if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
-  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, fcg_step, &c, 1);
+  if (m == 1) {
+    if (ugk_f1_start(K, n, b, x, w->r, w->dinv, w->st, w->part, stream) ||
+        (N && ugk_f1_input0(n, w->r, w->dinv, ufem_net_input(N), w->st, w->part, stream)))
+      return gpu_fail("start");
+    rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, fcg1_step, &c, 1);
+  } else {
+    if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
+    rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, fcg_step, &c, 1);
+  }
   if (rv != AF_OK) return rv;
   if (get_state(w, stream)) return AF_ERR;
   if (h->bb <= 0.0) {

@@ -81,6 +81,12 @@ typedef struct {
   long fb_iter;       /* flexible CG: iteration of the Jacobi fallback    */
   int fb;             /* flexible CG: 1 once the network is switched off  */
   int halt;           /* flexible CG: done or fallback (network skipped)  */
+  /* flexible CG, one kept direction (ugk_f1_*) */
+  long hgrid;         /* grid of the kernel that wrote the head partials   */
+  long restart_at;    /* iteration whose direction starts afresh (beta 0)  */
+  double rdr;         /* (r, dinv r) of the running iteration              */
+  double pr_old;      /* (p_{k-1}, r_k)                                    */
+  double pq_old;      /* (p_{k-1}, q_{k-1}), q the stored product           */
   unsigned int counter[10]; /* last-block tickets, reset by the last block;
                                6..8: the persistent kernel's grid barrier */
 } ugk_state;
@@ -309,6 +315,41 @@ int ugk_fcg_refresh(const ugk_mat *K, const double *d_b, const double *d_x, doub
                     ugk_state *d_st, double *d_part, void *stream);
 int ugk_fcg_update(const ugk_mat *K, long n, double *d_x, double *d_r, const double *d_P, const double *d_Q,
                    int m, const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
+
+/* FCG(1) in three kernels per iteration (one kept direction; the network or
+ * the coarse term, or both, in z = M r).  Per iteration k:
+ *   [ugk_f1_refresh]  mixed form only: r = b - A x in FP64 when asked for;
+ *   the network on the input the previous update (or refresh) wrote;
+ *   [ugk_cc_apply]    the coarse term into d_zc;
+ *   ugk_f1_combine    convergence test of r_k (and, mixed, the refresh
+ *                     request), z = dinv r + s mask sqrt(dinv) out (+ zc),
+ *                     partials (z, r), (z, q_{k-1});
+ *   ugk_f1_w          w = A z (FP64, or the FP32 copy with f32 = 1), (z, w);
+ *   ugk_f1_update     beta = -(z, q_{k-1}) / (p_{k-1}, q_{k-1}),
+ *                     p = z + beta p, q = w + beta q (= A p), alpha =
+ *                     (p, r) / (p, q) with (p, q) from the symmetric
+ *                     identity, x and r update, the next network input
+ *                     d_x0 (split format, scaled by the rms of the last r) and the
+ *                     head partials of r_{k+1}.
+ * A direction that does not descend (or is not finite) is not taken: that
+ * iteration moves nothing and the next one starts afresh with z = dinv r,
+ * the network skipped from there on (st->fb, st->halt).  ugk_f1_start forms r = b - A x and its
+ * partials; ugk_f1_input0 writes the first network input.  A refresh keeps
+ * the direction (st->restart = 0) or starts it afresh (1). */
+int ugk_f1_start(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r, const double *d_dinv,
+                 ugk_state *d_st, double *d_part, void *stream);
+int ugk_f1_input0(long n, const double *d_r, const double *d_dinv, unsigned short *d_x0, ugk_state *d_st,
+                  double *d_part, void *stream);
+int ugk_f1_refresh(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r, const double *d_dinv,
+                   const double *d_p, const double *d_q, unsigned short *d_x0, ugk_state *d_st, double *d_part,
+                   void *stream);
+int ugk_f1_combine(long n, const double *d_r, const double *d_dinv, const float *d_mask, const float *d_out,
+                   const double *d_zc, const double *d_q, double *d_z, ugk_state *d_st, double *d_part, void *stream);
+int ugk_f1_w(const ugk_mat *K, const double *d_z, double *d_w, int f32, const ugk_state *d_st, double *d_part,
+             void *stream);
+int ugk_f1_update(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_p, double *d_q, const double *d_z,
+                  const double *d_w, const double *d_dinv, unsigned short *d_x0, ugk_state *d_st, double *d_part,
+                  void *stream);
 
 /* ---------------- CG with the SSOR preconditioner ------------------------ */
 /* Kc: A with block rows (and columns) in colour order, new row I' = old row

@@ -2113,6 +2113,284 @@ __global__ __launch_bounds__(WG) void k_fcg_update(long n, double *__restrict__ 
   }
 }
 
+// --------------------------------------------------------------------------
+// FCG(1) in three kernels per iteration (one kept direction).  With
+// q_{k-1} = A p_{k-1} stored and w = A z_k formed from z_k directly,
+//   beta  = -(z, q_{k-1}) / (p_{k-1}, q_{k-1}),
+//   p_k   = z + beta p_{k-1},   q_k = w + beta q_{k-1}  (= A p_k),
+//   (p_k, r_k) = (z, r) + beta (p_{k-1}, r_k),
+//   (p_k, q_k) = (z, w) + 2 beta (z, q_{k-1}) + beta^2 (p_{k-1}, q_{k-1})
+// (A symmetric: (p_{k-1}, A z) = (A p_{k-1}, z)), so alpha is known when the
+// update starts, and one vector pass does p, q, x, r, the next network input
+// and the partials the next iteration needs.  The update's (p, r_{k+1}) and
+// (p, q) are summed directly, not carried.  Partial regions ([value][grid]
+// with the writer's grid): head H at 0, 6 values -- (r, dinv r), (r, r),
+// (x, x), (b, b), (p_{k-1}, r_k), (p_{k-1}, q_{k-1}) -- from the start, the
+// update or a refresh (grid in st->hgrid); Z in region 6, (z, r) and
+// (z, q_{k-1}) from the combine; W in region 7, (z, w) from the SpMV.
+// --------------------------------------------------------------------------
+#define F1_H(p) (p)
+#define F1_Z(p) ((p) + 6L * 65536)
+#define F1_W(p) ((p) + 7L * 65536)
+
+// network input scale: rms(sqrt(dinv) r) of the iteration before (the
+// network is linear; the scale only keeps its f32 activations near 1)
+__device__ __forceinline__ double f1_scale(double rdr, long n)
+{
+  const double rms = sqrt(rdr / (double)n);
+  return (rms > 0.0 && isfinite(rms)) ? rms : 1.0;
+}
+
+// channel c of node v of the network input, split format [V][3 parts][4]:
+// the exact truncation split of f (hi + mid + lo = f)
+__device__ __forceinline__ void f1_put_input(unsigned short *x0, long i, float f)
+{
+  const long v = i / 3, c = i - 3 * v;
+  const unsigned u = __float_as_uint(f), h = u & 0xffff0000u;
+  const float r1 = f - __uint_as_float(h);
+  const unsigned m = __float_as_uint(r1) & 0xffff0000u;
+  const float r2 = r1 - __uint_as_float(m);
+  unsigned short *o = x0 + v * 12 + c;
+  o[0] = (unsigned short)(h >> 16);
+  o[4] = (unsigned short)(m >> 16);
+  o[8] = (unsigned short)(__float_as_uint(r2) >> 16);
+}
+
+template <int BS>
+__global__ __launch_bounds__(WG) void k_f1_start(SellView M, const double *__restrict__ vals,
+                                                 const double *__restrict__ b, const double *__restrict__ x,
+                                                 double *r, const double *__restrict__ dinv, ugk_state *st,
+                                                 double *part)
+{
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      const long i = row * BS + k;
+      const double ri = b[i] - acc[k];
+      r[i] = ri;
+      v[0] += dinv[i] * ri * ri;
+      v[1] += ri * ri;
+      v[2] += x[i] * x[i];
+      v[3] += b[i] * b[i];
+    }
+  }
+  block_partials<6>(v, F1_H(part));
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->hgrid = gridDim.x;
+}
+
+// the first network input, from r_0 and the start's partials
+__global__ __launch_bounds__(WG) void k_f1_input0(long n, const double *__restrict__ r,
+                                                  const double *__restrict__ dinv, unsigned short *x0,
+                                                  ugk_state *st, const double *part)
+{
+  double t[1];
+  reduce_prev<1>(F1_H(part), (unsigned)st->hgrid, t);
+  const double s = f1_scale(t[0], n), inv = 1.0 / s;
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) f1_put_input(x0, i, (float)(sqrt(dinv[i]) * r[i] * inv));
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->net_s = s;
+}
+
+// mixed form: r = b - A x in FP64 for the iteration the combine asked for,
+// with the head partials (and the network input) taken from it; the kept
+// direction's (p, r) and (p, q) are summed anew
+template <int BS>
+__global__ __launch_bounds__(WG) void k_f1_refresh(SellView M, const double *__restrict__ vals,
+                                                   const double *__restrict__ b, const double *__restrict__ x,
+                                                   double *r, const double *__restrict__ dinv,
+                                                   const double *__restrict__ p, const double *__restrict__ q,
+                                                   unsigned short *x0, ugk_state *st, double *part)
+{
+  const long k = st->iter;
+  if (st->done || st->ru_req != k) return;   // the same for every block (the lead writes neither)
+  const bool in = x0 && !st->fb;
+  const double inv = 1.0 / st->net_s;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, double, double>(M, vals, x, acc, row)) {
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      const long i = row * BS + c;
+      const double ri = b[i] - acc[c];
+      r[i] = ri;
+      v[0] += dinv[i] * ri * ri;
+      v[1] += ri * ri;
+      v[2] += x[i] * x[i];
+      v[4] += p[i] * ri;
+      v[5] += p[i] * q[i];
+      if (in) f1_put_input(x0, i, (float)(sqrt(dinv[i]) * ri * inv));
+    }
+  }
+  block_partials<6>(v, F1_H(part));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->ru_at = k;
+    st->n_ru += 1;
+    st->hgrid = gridDim.x;
+    if (st->restart) st->restart_at = k;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_f1_combine(long n, const double *__restrict__ r,
+                                                   const double *__restrict__ dinv,
+                                                   const float *__restrict__ mask,
+                                                   const float *__restrict__ out,
+                                                   const double *__restrict__ zc,
+                                                   const double *__restrict__ q, double *__restrict__ z,
+                                                   ugk_state *st, double *part)
+{
+  if (st->done) return;
+  const long k = st->iter;
+  double t[6];
+  reduce_prev<6>(F1_H(part), (unsigned)st->hgrid, t);
+  // mixed form (max_gap > 0): convergence is only accepted on a residual
+  // recomputed in FP64 (the refresh of this iteration, or iteration 0)
+  const bool mixed = st->max_gap > 0;
+  const bool fresh = k == 0 || (mixed && st->ru_at == k);
+  const double bb = (k == 0) ? t[3] : st->bb;
+  int done = pcg_test(st, k, t[1], t[2], bb, t[0]);
+  long req = -1;
+  if (mixed && !fresh) {
+    if (done == 1) {
+      done = 0;
+      req = k + 1;
+    } else if (done == 0 && (t[1] < st->delta * st->delta * st->rr_true || k - st->ru_at >= st->max_gap))
+      req = k + 1;
+  }
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) {
+    st->cur = k;
+    st->rr = t[1];
+    st->xx = t[2];
+    if (k == 0) st->bb = bb;
+    if (mixed && fresh) st->rr_true = t[1];
+    st->rdr = t[0];
+    st->pr_old = t[4];
+    st->pq_old = t[5];
+    if (req >= 0) st->ru_req = req;
+    if (done) {
+      st->done = done;
+      st->halt = 1;
+    }
+  }
+  if (done) return;
+  const bool net = out && !st->fb;            // the network ran for this iteration
+  const bool useq = !(k == 0 || k == st->restart_at);
+  const double sc = st->net_s;
+  double v[2] = {0, 0};
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) {
+    const double ri = r[i], di = dinv[i];
+    double zi = di * ri;
+    if (net) {
+      const long vx = i / 3;
+      zi += sc * (double)mask[i] * sqrt(di) * (double)out[vx * 4 + (i - 3 * vx)];
+    }
+    if (zc) zi += zc[i];
+    z[i] = zi;
+    v[0] += zi * ri;
+    if (useq) v[1] += zi * q[i];
+  }
+  block_partials<2>(v, F1_Z(part));
+}
+
+// w = A z with (z, w); T = float: the FP32 copy of A, FP64 vectors and sums
+template <int BS, typename T>
+__global__ __launch_bounds__(WG) void k_f1_w(SellView M, const T *__restrict__ vals,
+                                             const double *__restrict__ z, double *__restrict__ w,
+                                             const ugk_state *st, double *part)
+{
+  if (st->done) return;
+  double v[1] = {0.0};
+  double acc[BS];
+  long row;
+  if (sell_row_split<BS, T, double>(M, vals, z, acc, row)) {
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      const long i = row * BS + c;
+      w[i] = acc[c];
+      v[0] += z[i] * acc[c];
+    }
+  }
+  block_partials<1>(v, F1_W(part));
+}
+
+__global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x, double *__restrict__ r,
+                                                  double *__restrict__ p, double *__restrict__ q,
+                                                  const double *__restrict__ z, const double *__restrict__ w,
+                                                  const double *__restrict__ dinv, unsigned short *x0,
+                                                  ugk_state *st, double *part, unsigned GZ, unsigned GW)
+{
+  if (st->done) return;
+  const long k = st->cur;
+  double tz[2], tw[1];
+  reduce_prev<2>(F1_Z(part), GZ, tz);
+  reduce_prev<1>(F1_W(part), GW, tw);
+  const bool restart = k == 0 || k == st->restart_at;
+  const double zr = tz[0], zq = tz[1], zw = tw[0];
+  const double pq_old = st->pq_old;
+  const double beta = restart ? 0.0 : -zq / pq_old;
+  const double pr = restart ? zr : zr + beta * st->pr_old;
+  const double pq = restart ? zw : zw + 2.0 * beta * zq + beta * beta * pq_old;
+  const bool ok = isfinite(beta) && isfinite(pr) && isfinite(pq) && pr > 0.0 && pq > 0.0;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!ok) {
+    // not a descent direction: iteration k moves nothing (x, r and the head
+    // partials stay), and iteration k + 1 starts afresh with z = dinv r, the
+    // network off from here on; with z = dinv r already, a breakdown.  The
+    // lead writes only fields this kernel's blocks do not read (restart_at
+    // becomes k + 1, never k), so every block takes this same branch.
+    if (lead) {
+      if (!st->fb) {
+        st->fb = 1;
+        st->fb_iter = k;
+        st->restart_at = k + 1;
+        st->iter = k + 1;
+      } else {
+        st->done = 2;
+      }
+      st->halt = 1;
+    }
+    return;
+  }
+  const double alpha = pr / pq;
+  const bool in = x0 && !st->fb;
+  const double s = f1_scale(st->rdr, n), inv = 1.0 / s;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) {
+    const double pi = restart ? z[i] : z[i] + beta * p[i];
+    const double qi = restart ? w[i] : w[i] + beta * q[i];
+    const double xi = x[i] + alpha * pi;
+    const double ri = r[i] - alpha * qi;
+    const double di = dinv[i];
+    p[i] = pi;
+    q[i] = qi;
+    x[i] = xi;
+    r[i] = ri;
+    v[0] += di * ri * ri;
+    v[1] += ri * ri;
+    v[2] += xi * xi;
+    v[4] += pi * ri;
+    v[5] += pi * qi;
+    if (in) f1_put_input(x0, i, (float)(sqrt(di) * ri * inv));
+  }
+  block_partials<6>(v, F1_H(part));
+  if (lead) {
+    st->iter = k + 1;
+    st->alpha = alpha;
+    st->beta = beta;
+    st->net_s = s;
+    st->hgrid = gridDim.x;
+  }
+}
+
 
 // --------------------------------------------------------------------------
 // CG with the SSOR (omega = 1, symmetric Gauss-Seidel) preconditioner,
@@ -3822,6 +4100,81 @@ int ugk_fcg_update(const ugk_mat *K, long n, double *x, double *r, const double 
 {
   k_fcg_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, P, Q, m, dinv, st, part, spmv_blocks(K));
   CK_LAUNCH("k_fcg_update");
+  return 0;
+}
+
+// FCG(1): the head region holds 6 values per block of the grid that wrote it
+static int f1_grid_ok(const ugk_mat *K, long n)
+{
+  if (spmv_blocks(K) > 65536 || vec_blocks(n) > 65536 || n != K->n_brows * K->bs) {
+    snprintf(g_err, sizeof(g_err), "flexible CG: matrix of %ld rows does not fit its partials", n);
+    return 0;
+  }
+  return 1;
+}
+
+int ugk_f1_start(const ugk_mat *K, long n, const double *b, const double *x, double *r, const double *dinv,
+                 ugk_state *st, double *part, void *s)
+{
+  if (!f1_grid_ok(K, n)) return -1;
+  if (K->bs == 3) k_f1_start<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, st, part);
+  else k_f1_start<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, st, part);
+  CK_LAUNCH("k_f1_start");
+  return 0;
+}
+
+int ugk_f1_input0(long n, const double *r, const double *dinv, unsigned short *x0, ugk_state *st, double *part,
+                  void *s)
+{
+  k_f1_input0<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, x0, st, part);
+  CK_LAUNCH("k_f1_input0");
+  return 0;
+}
+
+int ugk_f1_refresh(const ugk_mat *K, long n, const double *b, const double *x, double *r, const double *dinv,
+                   const double *p, const double *q, unsigned short *x0, ugk_state *st, double *part, void *s)
+{
+  if (!f1_grid_ok(K, n)) return -1;
+  if (K->bs == 3)
+    k_f1_refresh<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, p, q, x0, st, part);
+  else
+    k_f1_refresh<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, p, q, x0, st, part);
+  CK_LAUNCH("k_f1_refresh");
+  return 0;
+}
+
+int ugk_f1_combine(long n, const double *r, const double *dinv, const float *mask, const float *out,
+                   const double *zc, const double *q, double *z, ugk_state *st, double *part, void *s)
+{
+  k_f1_combine<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, mask, out, zc, q, z, st, part);
+  CK_LAUNCH("k_f1_combine");
+  return 0;
+}
+
+int ugk_f1_w(const ugk_mat *K, const double *z, double *w, int f32, const ugk_state *st, double *part, void *s)
+{
+  const SellView M = view(K);
+  if (f32) {
+    if (!K->d_vals32) {
+      snprintf(g_err, sizeof(g_err), "flexible CG: no FP32 copy of the matrix");
+      return -1;
+    }
+    if (K->bs == 3) k_f1_w<3, float><<<spmv_blocks(K), WG, 0, S(s)>>>(M, K->d_vals32, z, w, st, part);
+    else k_f1_w<1, float><<<spmv_blocks(K), WG, 0, S(s)>>>(M, K->d_vals32, z, w, st, part);
+  } else {
+    if (K->bs == 3) k_f1_w<3, double><<<spmv_blocks(K), WG, 0, S(s)>>>(M, K->d_vals, z, w, st, part);
+    else k_f1_w<1, double><<<spmv_blocks(K), WG, 0, S(s)>>>(M, K->d_vals, z, w, st, part);
+  }
+  CK_LAUNCH("k_f1_w");
+  return 0;
+}
+
+int ugk_f1_update(const ugk_mat *K, long n, double *x, double *r, double *p, double *q, const double *z,
+                  const double *w, const double *dinv, unsigned short *x0, ugk_state *st, double *part, void *s)
+{
+  k_f1_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, p, q, z, w, dinv, x0, st, part, vec_blocks(n),
+                                              spmv_blocks(K));
+  CK_LAUNCH("k_f1_update");
   return 0;
 }
 
