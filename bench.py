@@ -553,8 +553,8 @@ def main():
                     help="stop after the plain FP64 solves, the dominant-kernel timing and the results "
                          "pass (for kernel traces)")
     ap.add_argument("--net-kernels-only", action="store_true",
-                    help="launch only the network's inference stages on the 128x32x32 random-field "
-                         "cantilever (initial weights; for kernel traces)")
+                    help="launch only the network's inference stages on the metric's node grid "
+                         "(initial weights; for kernel traces)")
     ap.add_argument("--ssor-only", action="store_true",
                     help="two SSOR-CG solves of the metric's system (for kernel traces)")
     ap.add_argument("--learned-solve-only", action="store_true",
@@ -576,17 +576,18 @@ def main():
     from ufem_amd.mesh import cantilever, lognormal_E
 
     if args.net_kernels_only:
-        from ufem_amd import learned as Lr
-        m5 = cantilever(128, 32, 32)
-        m5.E[:] = Lr.field_E(m5, 9999)
-        s5 = G.GpuSystem(m5)
-        s5.assemble(2)
-        net = G.GpuNet(129, 33, 33, batch=4)
+        # the network's inference stages alone on the metric's node grid, in
+        # the bench's form (initial weights: the kernels do not depend on them)
+        nx, ny, nz = (int(v) for v in args.size.split(","))
+        mk = cantilever(nx, ny, nz, E=lognormal_E(args.seed, nx * ny * nz, cov=0.3))
+        sk = G.GpuSystem(mk)
+        sk.assemble(2)
+        net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=1, levels=args.net_levels, fine=args.net_node_grid)
         net.init(1)
-        net.bind(s5, m5.E)
+        net.bind(sk, mk.E)
         log("network stages (us)", {k: round(v, 1) for k, v in net.time_stages(reps=20).items()})
         net.close()
-        s5.close()
+        sk.close()
         return
 
     nx, ny, nz = (int(v) for v in args.size.split(","))

@@ -353,3 +353,28 @@ def test_trained_network_beats_jacobi(beam):
     assert rm["rv"] == 0 and not rm["fallback"] and rm["true_relres"] <= 1e-8 * (1 + 1e-9)
     assert rel(s.u(), u_plain) <= 1e-6
     assert rm["iters"] <= 2.2 * r["iters"]
+
+
+@pytest.mark.parametrize("solver", ["learned", "learned_mixed"])
+def test_fused_last_layer_matches_the_network_stage(beam, solver, monkeypatch):
+    """The coarse-first network's last layer y = C0^T t0 folded into the
+    flexible-CG combine (f32 on the vector ALUs) against the network's own
+    stage followed by the plain combine: x after a few iterations agrees to
+    f32 rounding (a wrong tap, weight or parity would move it at O(1)), and
+    both reach the same solution."""
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D, levels=4, fine=False)
+    net.init(5)
+    net.bind(s, m.E)
+    for maxit in (3, 11):
+        monkeypatch.setenv("UFEM_NO_FUSED_C0T", "1")
+        a = s.solve(solver, eps=1e-30, maxit=maxit, net=net, check=False)
+        ua = s.u()
+        monkeypatch.delenv("UFEM_NO_FUSED_C0T")
+        b = s.solve(solver, eps=1e-30, maxit=maxit, net=net, check=False)
+        ub = s.u()
+        assert a["iters"] == b["iters"] == maxit and not a["fallback"] and not b["fallback"]
+        assert rel(ub, ua) <= 1e-5, (maxit, rel(ub, ua))
+    r = s.solve(solver, eps=1e-8, net=net)
+    assert r["rv"] == 0 and not r["fallback"] and r["true_relres"] <= 1e-8 * (1 + 1e-9)
+    assert rel(s.u(), u_ref) <= 1e-6

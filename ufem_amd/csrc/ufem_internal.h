@@ -108,6 +108,8 @@ const float *ufem_net_mask(const ufem_net *N);     /* 0 at fixed DOFs, else 1   
 const float *ufem_net_output(const ufem_net *N);   /* f32 [V][4]                          */
 int ufem_net_prepare(ufem_net *N, void *stream);   /* gates of the bound field            */
 int ufem_net_infer(ufem_net *N, const int *d_skip, void *stream);
+int ufem_net_infer_head(ufem_net *N, const int *d_skip, void *stream);  /* all stages but the last */
+int ufem_net_last_stage(const ufem_net *N, ugk_conv_desc *d);          /* 1: *d is y = C0^T t0 (T2) */
 
 /* error text (ufem_sys.c) */
 void ufem_set_error(const char *fmt, ...);

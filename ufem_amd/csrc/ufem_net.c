@@ -955,6 +955,32 @@ int ufem_net_infer(ufem_net *N, const int *skip, void *stream)
   return infer(N, N->gi, N->yi, skip, stream);
 }
 
+/* the last stage, y = C0^T t0, as a convolution descriptor for a fused
+ * consumer (the coarse-first form only: there C0^T is a stride-2
+ * transpose); 0 when the form has no such stage */
+int ufem_net_last_stage(const ufem_net *N, ugk_conv_desc *d)
+{
+  if (N->fine) return 0;
+  memset(d, 0, sizeof(*d));
+  d->x = N->t[0]; d->ldx = N->ch[0]; d->cin = N->ch[0]; d->x_f32 = UGK_ACT_F32;
+  d->w = N->wt[0]; d->cip = r16(N->ch[0]); d->cop = r16(N->ci[0]);
+  d->y = N->yi; d->ldy = 4; d->cout = 3; d->y_f32 = 1;
+  d->mode = UGK_CONV_T2;
+  d->nb = 1;
+  d->gi = N->g[0];
+  d->go = N->gf;
+  return 1;
+}
+
+/* all inference stages but the last */
+int ufem_net_infer_head(ufem_net *N, const int *skip, void *stream)
+{
+  int k;
+  for (k = 0; k < infer_stages(N) - 1; k++)
+    if (infer_stage(N, k, N->gi, N->yi, skip, stream)) return net_fail("network inference");
+  return AF_OK;
+}
+
 /* pair i as stored: v [V][4], rr [n], ee [n], e.r and its field index */
 int ufem_net_get_pair(ufem_net *N, long i, float *v, float *rr, float *ee, double *er, int *field)
 {

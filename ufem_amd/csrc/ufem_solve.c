@@ -383,6 +383,8 @@ typedef struct {
   const ugk_coarse *cc;      /* NULL: the network alone                    */
   int m;
   int mixed;
+  int fused;                 /* m = 1, coarse-first network: last layer in the combine */
+  ugk_conv_desc last;
 } fcg_ctx;
 
 static int fcg_step(void *c, long it, void *stream)
@@ -414,13 +416,19 @@ static int fcg1_step(void *c, long it, void *stream)
   (void)it;
   if (f->mixed && ugk_f1_refresh(f->K, f->n, f->b, f->x, w->r, w->dinv, w->fP, w->fQ, x0, w->st, w->part, stream))
     return -1;
-  if ((f->net && ufem_net_infer(f->net, &w->st->halt, stream)) ||
-      (f->cc && ugk_cc_apply(f->cc, f->n, w->r, f->cc->d_zc, w->st, stream)) ||
-      ugk_f1_combine(f->n, w->r, w->dinv, f->net ? ufem_net_mask(f->net) : NULL,
-                     f->net ? ufem_net_output(f->net) : NULL, f->cc ? f->cc->d_zc : NULL, w->fQ, w->z, w->st,
-                     w->part, stream) ||
-      ugk_f1_w(f->K, w->z, w->q, f->mixed, w->st, w->part, stream))
+  if (f->fused) {                       /* the network's last layer inside the combine */
+    if (ufem_net_infer_head(f->net, &w->st->halt, stream) ||
+        (f->cc && ugk_cc_apply(f->cc, f->n, w->r, f->cc->d_zc, w->st, stream)) ||
+        ugk_f1_combine_c0t(f->n, w->r, w->dinv, ufem_net_mask(f->net), &f->last, f->cc ? f->cc->d_zc : NULL, w->fQ,
+                           w->z, w->st, w->part, stream))
+      return -1;
+  } else if ((f->net && ufem_net_infer(f->net, &w->st->halt, stream)) ||
+             (f->cc && ugk_cc_apply(f->cc, f->n, w->r, f->cc->d_zc, w->st, stream)) ||
+             ugk_f1_combine(f->n, w->r, w->dinv, f->net ? ufem_net_mask(f->net) : NULL,
+                            f->net ? ufem_net_output(f->net) : NULL, f->cc ? f->cc->d_zc : NULL, w->fQ, w->z, w->st,
+                            w->part, stream))
     return -1;
+  if (ugk_f1_w(f->K, w->z, w->q, f->mixed, w->st, w->part, stream)) return -1;
   return ugk_f1_update(f->K, f->n, f->x, w->r, w->fP, w->fQ, w->z, w->q, w->dinv, x0, w->st, w->part, stream);
 }
 
@@ -431,7 +439,7 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   ufem_net *N = (ufem_net *)o->net;
   const int m = o->fcg_m > 0 ? o->fcg_m : 1;   /* the network is a fixed SPD map */
   const ugk_coarse *cc = o->coarse ? w->cc : NULL;
-  fcg_ctx c = {K, n, b, x, w, N, cc, m, mixed};
+  fcg_ctx c = {K, n, b, x, w, N, cc, m, mixed, 0, {0}};
   ugk_state *h = w->st_host;
   int rv;
   if (o->coarse && !w->cc) {
@@ -471,6 +479,7 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   if (N && ufem_net_prepare(N, stream)) return AF_ERR;
   if (cc && ufem_coarse_build(w->cc, stream) != AF_OK) return AF_ERR;
   if (m == 1) {
+    c.fused = N && !getenv("UFEM_NO_FUSED_C0T") && ufem_net_last_stage(N, &c.last);
     if (ugk_f1_start(K, n, b, x, w->r, w->dinv, w->st, w->part, stream) ||
         (N && ugk_f1_input0(n, w->r, w->dinv, ufem_net_input(N), w->st, w->part, stream)))
       return gpu_fail("start");

@@ -87,6 +87,7 @@ typedef struct {
   double rdr;         /* (r, dinv r) of the running iteration              */
   double pr_old;      /* (p_{k-1}, r_k)                                    */
   double pq_old;      /* (p_{k-1}, q_{k-1}), q the stored product           */
+  long zgrid;         /* grid of the combine that wrote the z partials     */
   unsigned int counter[10]; /* last-block tickets, reset by the last block;
                                6..8: the persistent kernel's grid barrier */
 } ugk_state;
@@ -316,6 +317,7 @@ int ugk_fcg_refresh(const ugk_mat *K, const double *d_b, const double *d_x, doub
 int ugk_fcg_update(const ugk_mat *K, long n, double *d_x, double *d_r, const double *d_P, const double *d_Q,
                    int m, const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
 
+struct ugk_conv_desc_s;
 /* FCG(1) in three kernels per iteration (one kept direction; the network or
  * the coarse term, or both, in z = M r).  Per iteration k:
  *   [ugk_f1_refresh]  mixed form only: r = b - A x in FP64 when asked for;
@@ -347,6 +349,13 @@ int ugk_f1_combine(long n, const double *d_r, const double *d_dinv, const float 
                    const double *d_zc, const double *d_q, double *d_z, ugk_state *d_st, double *d_part, void *stream);
 int ugk_f1_w(const ugk_mat *K, const double *d_z, double *d_w, int f32, const ugk_state *d_st, double *d_part,
              void *stream);
+/* ugk_f1_combine with the network's last layer folded in: y = C0^T t0, the
+ * stride-2 transpose (UGK_CONV_T2) described by *cd (f32 input of at most
+ * 32 channels, 3 outputs, one sample), on the vector ALUs in f32; the
+ * network then runs all stages but that one */
+int ugk_f1_combine_c0t(long n, const double *d_r, const double *d_dinv, const float *d_mask,
+                       const struct ugk_conv_desc_s *cd, const double *d_zc, const double *d_q, double *d_z,
+                       ugk_state *d_st, double *d_part, void *stream);
 int ugk_f1_update(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_p, double *d_q, const double *d_z,
                   const double *d_w, const double *d_dinv, unsigned short *d_x0, ugk_state *d_st, double *d_part,
                   void *stream);
@@ -474,7 +483,7 @@ enum { UGK_ACT_BF16 = 0, UGK_ACT_F32 = 1, UGK_ACT_SPLIT = 2 };
  * the split made once by the producer).  Epilogue per output element:
  * v = acc (* emul) (+ ga * gb) (f32 fields, optional), stored as bf16, or
  * f32 if y_f32. */
-typedef struct {
+typedef struct ugk_conv_desc_s {
   const void *x; int ldx, cin, x_f32;      /* cin % 4 == 0, cin <= cip        */
   const unsigned short *w; int cip, cop;   /* [27][cop][cip], zero padded, %16 */
   void *y; int ldy, cout, y_f32;           /* cout <= cop                      */

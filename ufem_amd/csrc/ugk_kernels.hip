@@ -186,18 +186,55 @@ __device__ void block_partials(double (&v)[NV], double *part)
   }
 }
 
+// A thread's share of the partials of NV values (grid G, stride TB), with
+// RP_U loads of every value in flight per pass (a loop that waits on each
+// load costs one memory round trip per entry: the partials were written by
+// other XCDs); fixed order, so every block gets the same sums.
+template <int NV, int TB>
+__device__ __forceinline__ void thread_partials(const double *__restrict__ part, unsigned G, double (&a)[NV])
+{
+  constexpr int RP_U = NV >= 4 ? 4 : 8;
+#pragma unroll
+  for (int k = 0; k < NV; k++) a[k] = 0.0;
+  unsigned i = threadIdx.x;
+  for (; i + (RP_U - 1) * TB < G; i += RP_U * TB) {
+    double t[NV][RP_U];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+      for (int u = 0; u < RP_U; u++) t[k][u] = part[(size_t)k * G + i + u * TB];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+      for (int u = 0; u < RP_U; u++) a[k] += t[k][u];
+  }
+  double t[NV][RP_U];
+#pragma unroll
+  for (int u = 0; u < RP_U; u++)
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      const bool in = i + u * TB < G;        // the address stays in the region either way
+      const double pv = part[(size_t)k * G + (in ? i + u * TB : 0)];
+      t[k][u] = in ? pv : 0.0;
+    }
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int u = 0; u < RP_U; u++) a[k] += t[k][u];
+}
+
 // Every block sums the partials a previous kernel (grid G) published, in
-// index order, so all blocks hold bit-identical totals.
+// a fixed order, so all blocks hold bit-identical totals.
 template <int NV>
 __device__ void reduce_prev(const double *__restrict__ part, unsigned G, double (&tot)[NV])
 {
   __shared__ double sh[NV][NW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double a[NV];
+  thread_partials<NV, WG>(part, G, a);
 #pragma unroll
   for (int k = 0; k < NV; k++) {
-    double a = 0.0;
-    for (unsigned i = threadIdx.x; i < G; i += WG) a += part[(size_t)k * G + i];
-    double s = wave_sum(a);
+    double s = wave_sum(a[k]);
     if (lane == 0) sh[k][wave] = s;
   }
   __syncthreads();
@@ -206,6 +243,45 @@ __device__ void reduce_prev(const double *__restrict__ part, unsigned G, double 
     double r = sh[k][0];
     for (int w = 1; w < NW; w++) r += sh[k][w];
     tot[k] = r;
+  }
+}
+
+// reduce_prev and block_partials for blocks of TB threads
+template <int NV, int TB>
+__device__ void reduce_prev_tb(const double *__restrict__ part, unsigned G, double (&tot)[NV])
+{
+  __shared__ double sh[NV][TB / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double a[NV];
+  thread_partials<NV, TB>(part, G, a);
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(a[k]);
+    if (lane == 0) sh[k][wave] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double r = sh[k][0];
+    for (int w = 1; w < TB / 64; w++) r += sh[k][w];
+    tot[k] = r;
+  }
+}
+template <int NV, int TB>
+__device__ void block_partials_tb(double (&v)[NV], double *part)
+{
+  __shared__ double sh[NV][TB / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) sh[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double a = sh[threadIdx.x][0];
+    for (int w = 1; w < TB / 64; w++) a += sh[threadIdx.x][w];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
   }
 }
 
@@ -306,7 +382,9 @@ __device__ __forceinline__ long remap_block(long b, long nb)
 __device__ __forceinline__ void xcd_chunk(long n, long &i1, long &i2)
 {
   const long lb = (gridDim.x & 7) ? (long)blockIdx.x : remap_block(blockIdx.x, gridDim.x);
-  const long chunk = (n + gridDim.x - 1) / gridDim.x;
+  // a 32-bit division where n allows (a 64-bit one is a long software sequence)
+  const long chunk = n < (1L << 32) - gridDim.x ? (long)(((unsigned)n + gridDim.x - 1) / gridDim.x)
+                                                : (n + gridDim.x - 1) / gridDim.x;
   i1 = lb * chunk;
   i2 = min(n, i1 + chunk);
 }
@@ -2130,6 +2208,7 @@ __global__ __launch_bounds__(WG) void k_fcg_update(long n, double *__restrict__ 
 // (z, q_{k-1}) from the combine; W in region 7, (z, w) from the SpMV.
 // --------------------------------------------------------------------------
 #define F1_H(p) (p)
+#define F1_UB 4   /* entries per thread and pass of the update */
 #define F1_Z(p) ((p) + 6L * 65536)
 #define F1_W(p) ((p) + 7L * 65536)
 
@@ -2271,6 +2350,7 @@ __global__ __launch_bounds__(WG) void k_f1_combine(long n, const double *__restr
     st->rdr = t[0];
     st->pr_old = t[4];
     st->pq_old = t[5];
+    st->zgrid = gridDim.x;
     if (req >= 0) st->ru_req = req;
     if (done) {
       st->done = done;
@@ -2299,6 +2379,162 @@ __global__ __launch_bounds__(WG) void k_f1_combine(long n, const double *__restr
   block_partials<2>(v, F1_Z(part));
 }
 
+// The same combine with the network's last layer in it: y = C0^T t0, the
+// stride-2 transpose from coarse level 0 (32 f32 channels) to the node grid,
+// 3 outputs, on the vector ALUs in f32.  A node f = 2 c + p of parity class
+// p takes, per axis, the coarse node c (offset 0) when p = 0, and c (offset
+// +1) and c + 1 (offset -1, when it exists) when p = 1: 1 to 8 taps.  A
+// block of 8 waves takes 64 coarse nodes, wave w the nodes of class w, in
+// rounds of 16 (so a wave's taps are compile-time and its lanes never
+// diverge); four lanes share a node, each with 8 of the 32 channels, all
+// its gathers and its epilogue loads in flight together; the four partial
+// sums meet by lane shuffles in a fixed order, and lane q < 3 forms z for
+// DOF 3 f + q.  The weights ([27][cop][cip] bf16, the adjoint's layout) are
+// widened to f32 in LDS as [27][3][32].  Replaces the last network stage
+// and k_f1_combine.
+#define C0T_CIN 32
+#define C0T_TB 512
+#define C0T_NODES 64
+
+template <int PZ, int PY, int PX>
+__device__ __forceinline__ void c0t_class(const float *__restrict__ x, int ldx, const float *sw, ugk_grid3 gc, int cz,
+                                          int cy, int cx, bool fok, int qd, float (&y)[3])
+{
+  constexpr int NZ = PZ + 1, NY = PY + 1, NX = PX + 1, NT = NZ * NY * NX;
+  float4 xa[NT], xb[NT];
+  bool ok[NT];
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    const int iz = j / (NY * NX), iy = (j / NX) % NY, ix = j % NX;
+    const int sz = cz + iz, sy = cy + iy, sx = cx + ix;   // offset -1 reads c + 1
+    ok[j] = fok && sz < gc.D && sy < gc.H && sx < gc.W;
+    xa[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    xb[j] = xa[j];
+    if (ok[j]) {
+      const float *xs = x + (((long)sz * gc.H + sy) * gc.W + sx) * ldx + 8 * qd;
+      xa[j] = *(const float4 *)xs;
+      xb[j] = *(const float4 *)(xs + 4);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    const int iz = j / (NY * NX), iy = (j / NX) % NY, ix = j % NX;
+    const int tz = PZ ? 2 - 2 * iz : 1, ty = PY ? 2 - 2 * iy : 1, tx = PX ? 2 - 2 * ix : 1;
+    const float *w = sw + (9 * tz + 3 * ty + tx) * 3 * C0T_CIN + 8 * qd;
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      const float4 w0 = *(const float4 *)(w + o * C0T_CIN), w1 = *(const float4 *)(w + o * C0T_CIN + 4);
+      y[o] += w0.x * xa[j].x + w0.y * xa[j].y + w0.z * xa[j].z + w0.w * xa[j].w + w1.x * xb[j].x +
+              w1.y * xb[j].y + w1.z * xb[j].z + w1.w * xb[j].w;
+    }
+  }
+}
+
+__global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double *__restrict__ r,
+                                                           const double *__restrict__ dinv,
+                                                           const float *__restrict__ mask, ugk_conv_desc cd,
+                                                           const double *__restrict__ zc,
+                                                           const double *__restrict__ q, double *__restrict__ z,
+                                                           ugk_state *st, double *part)
+{
+  __shared__ float sw[27 * 3 * C0T_CIN];
+  if (st->done) return;
+  const long k = st->iter;
+  double t[6];
+  reduce_prev_tb<6, C0T_TB>(F1_H(part), (unsigned)st->hgrid, t);
+  const bool mixed = st->max_gap > 0;
+  const bool fresh = k == 0 || (mixed && st->ru_at == k);
+  const double bb = (k == 0) ? t[3] : st->bb;
+  int done = pcg_test(st, k, t[1], t[2], bb, t[0]);
+  long req = -1;
+  if (mixed && !fresh) {
+    if (done == 1) {
+      done = 0;
+      req = k + 1;
+    } else if (done == 0 && (t[1] < st->delta * st->delta * st->rr_true || k - st->ru_at >= st->max_gap))
+      req = k + 1;
+  }
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) {
+    st->cur = k;
+    st->rr = t[1];
+    st->xx = t[2];
+    if (k == 0) st->bb = bb;
+    if (mixed && fresh) st->rr_true = t[1];
+    st->rdr = t[0];
+    st->pr_old = t[4];
+    st->pq_old = t[5];
+    st->zgrid = gridDim.x;
+    if (req >= 0) st->ru_req = req;
+    if (done) {
+      st->done = done;
+      st->halt = 1;
+    }
+  }
+  if (done) return;
+  const bool net = !st->fb;                   // the network ran for this iteration
+  if (net) {
+    const unsigned short *w = cd.w;
+    for (int i = threadIdx.x; i < 27 * 3 * C0T_CIN; i += C0T_TB) {
+      const int tp = i / (3 * C0T_CIN), o = (i / C0T_CIN) % 3, c = i % C0T_CIN;
+      sw[i] = __uint_as_float((unsigned)w[((long)tp * cd.cop + o) * cd.cip + c] << 16);
+    }
+  }
+  __syncthreads();
+  const bool useq = !(k == 0 || k == st->restart_at);
+  const double sc = st->net_s;
+  const float *x = (const float *)cd.x;
+  const ugk_grid3 gc = cd.gi, gf = cd.go;
+  const int lane = threadIdx.x & 63, cls = threadIdx.x >> 6, qd = lane & 3;
+  const int pz = cls >> 2, py = (cls >> 1) & 1, px = cls & 1;
+  const unsigned Vc = (unsigned)gc.D * gc.H * gc.W, HWc = (unsigned)gc.H * gc.W, Wc = (unsigned)gc.W;
+  double v[2] = {0, 0};
+  for (int rd = 0; rd < C0T_NODES / 16; rd++) {
+    const unsigned c = (unsigned)blockIdx.x * C0T_NODES + rd * 16 + (lane >> 2);
+    const unsigned cz = c / HWc, cr = c - cz * HWc, cy = cr / Wc, cx = cr - cy * Wc;
+    const int fz = 2 * cz + pz, fy = 2 * cy + py, fx = 2 * cx + px;
+    const bool fok = c < Vc && fz < gf.D && fy < gf.H && fx < gf.W;
+    const long i = fok ? 3 * (((long)fz * gf.H + fy) * gf.W + fx) + (qd < 3 ? qd : 0) : 0;
+    const bool mine = fok && qd < 3;
+    double ri = 0.0, di = 0.0, qi = 0.0, zci = 0.0;
+    float mi = 0.f;
+    if (mine) {          // the epilogue's operands, in flight with the gathers
+      ri = r[i];
+      di = dinv[i];
+      mi = mask[i];
+      if (useq) qi = q[i];
+      if (zc) zci = zc[i];
+    }
+    float y[3] = {0.f, 0.f, 0.f};
+    if (net) {
+      switch (cls) {                        // wave-uniform
+        case 0: c0t_class<0, 0, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 1: c0t_class<0, 0, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 2: c0t_class<0, 1, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 3: c0t_class<0, 1, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 4: c0t_class<1, 0, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 5: c0t_class<1, 0, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 6: c0t_class<1, 1, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        default: c0t_class<1, 1, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+      }
+#pragma unroll
+      for (int o = 0; o < 3; o++) {
+        y[o] += __shfl_xor(y[o], 1);
+        y[o] += __shfl_xor(y[o], 2);
+      }
+    }
+    if (mine) {
+      double zi = di * ri;
+      if (net) zi += sc * (double)mi * sqrt(di) * (double)(qd == 0 ? y[0] : (qd == 1 ? y[1] : y[2]));
+      zi += zci;
+      z[i] = zi;
+      v[0] += zi * ri;
+      v[1] += zi * qi;
+    }
+  }
+  block_partials_tb<2, C0T_TB>(v, F1_Z(part));
+}
+
 // w = A z with (z, w); T = float: the FP32 copy of A, FP64 vectors and sums
 template <int BS, typename T>
 __global__ __launch_bounds__(WG) void k_f1_w(SellView M, const T *__restrict__ vals,
@@ -2324,12 +2560,12 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
                                                   double *__restrict__ p, double *__restrict__ q,
                                                   const double *__restrict__ z, const double *__restrict__ w,
                                                   const double *__restrict__ dinv, unsigned short *x0,
-                                                  ugk_state *st, double *part, unsigned GZ, unsigned GW)
+                                                  ugk_state *st, double *part, unsigned GW)
 {
   if (st->done) return;
   const long k = st->cur;
   double tz[2], tw[1];
-  reduce_prev<2>(F1_Z(part), GZ, tz);
+  reduce_prev<2>(F1_Z(part), (unsigned)st->zgrid, tz);
   reduce_prev<1>(F1_W(part), GW, tw);
   const bool restart = k == 0 || k == st->restart_at;
   const double zr = tz[0], zq = tz[1], zw = tw[0];
@@ -2364,7 +2600,40 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
   double v[6] = {0, 0, 0, 0, 0, 0};
   long i1, i2;
   xcd_chunk(n, i1, i2);
-  for (long i = i1 + threadIdx.x; i < i2; i += WG) {
+  // F1_UB entries per thread and pass, all loads issued before the first
+  // use: one wave then keeps 7 F1_UB loads in flight, not 7
+  long i = i1 + threadIdx.x;
+  for (; i + (F1_UB - 1) * WG < i2; i += F1_UB * WG) {
+    double zz[F1_UB], pp[F1_UB], ww[F1_UB], qq[F1_UB], xx[F1_UB], rr[F1_UB], dd[F1_UB];
+#pragma unroll
+    for (int u = 0; u < F1_UB; u++) {
+      const long j = i + u * WG;
+      zz[u] = z[j];
+      ww[u] = w[j];
+      pp[u] = restart ? 0.0 : p[j];
+      qq[u] = restart ? 0.0 : q[j];
+      xx[u] = x[j];
+      rr[u] = r[j];
+      dd[u] = dinv[j];
+    }
+#pragma unroll
+    for (int u = 0; u < F1_UB; u++) {
+      const long j = i + u * WG;
+      const double pi = zz[u] + beta * pp[u], qi = ww[u] + beta * qq[u];
+      const double xi = xx[u] + alpha * pi, ri = rr[u] - alpha * qi, di = dd[u];
+      p[j] = pi;
+      q[j] = qi;
+      x[j] = xi;
+      r[j] = ri;
+      v[0] += di * ri * ri;
+      v[1] += ri * ri;
+      v[2] += xi * xi;
+      v[4] += pi * ri;
+      v[5] += pi * qi;
+      if (in) f1_put_input(x0, j, (float)(sqrt(di) * ri * inv));
+    }
+  }
+  for (; i < i2; i += WG) {
     const double pi = restart ? z[i] : z[i] + beta * p[i];
     const double qi = restart ? w[i] : w[i] + beta * q[i];
     const double xi = x[i] + alpha * pi;
@@ -4151,6 +4420,27 @@ int ugk_f1_combine(long n, const double *r, const double *dinv, const float *mas
   return 0;
 }
 
+int ugk_f1_combine_c0t(long n, const double *r, const double *dinv, const float *mask, const ugk_conv_desc *cd,
+                       const double *zc, const double *q, double *z, ugk_state *st, double *part, void *s)
+{
+  if (cd->mode != UGK_CONV_T2 || cd->x_f32 != UGK_ACT_F32 || cd->cin != C0T_CIN || cd->ldx % 4 ||
+      cd->ldx < C0T_CIN || cd->cout != 3 || cd->cop < 3 || cd->cip < C0T_CIN || cd->nb != 1 ||
+      n != 3L * cd->go.D * cd->go.H * cd->go.W || n / 3 >= (1L << 31) || cd->gi.D != (cd->go.D + 1) / 2 ||
+      cd->gi.H != (cd->go.H + 1) / 2 || cd->gi.W != (cd->go.W + 1) / 2) {
+    snprintf(g_err, sizeof(g_err), "flexible CG: the network's last layer does not fit the fused combine");
+    return -1;
+  }
+  const long vc = (long)cd->gi.D * cd->gi.H * cd->gi.W;
+  if ((vc + C0T_NODES - 1) / C0T_NODES > 65536 / 2) {
+    snprintf(g_err, sizeof(g_err), "flexible CG: coarse level too large for the fused combine's partials");
+    return -1;
+  }
+  k_f1_combine_c0t<<<(unsigned)((vc + C0T_NODES - 1) / C0T_NODES), C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z,
+                                                                                      st, part);
+  CK_LAUNCH("k_f1_combine_c0t");
+  return 0;
+}
+
 int ugk_f1_w(const ugk_mat *K, const double *z, double *w, int f32, const ugk_state *st, double *part, void *s)
 {
   const SellView M = view(K);
@@ -4172,8 +4462,7 @@ int ugk_f1_w(const ugk_mat *K, const double *z, double *w, int f32, const ugk_st
 int ugk_f1_update(const ugk_mat *K, long n, double *x, double *r, double *p, double *q, const double *z,
                   const double *w, const double *dinv, unsigned short *x0, ugk_state *st, double *part, void *s)
 {
-  k_f1_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, p, q, z, w, dinv, x0, st, part, vec_blocks(n),
-                                              spmv_blocks(K));
+  k_f1_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, p, q, z, w, dinv, x0, st, part, spmv_blocks(K));
   CK_LAUNCH("k_f1_update");
   return 0;
 }

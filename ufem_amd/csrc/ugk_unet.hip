@@ -208,12 +208,14 @@ __device__ __forceinline__ void mfma_frag(const bf16x4 &a, const typename Frag<X
 // many taps are in flight together.  PK: the input has 4 channels, and the
 // four lane groups of the MFMA's K dimension take four taps at once.
 // KS = 1: the (16-channel chunk, dz plane) units of the sum in a rolled
-// loop (fewer registers, more waves resident); KS = 4 (small grids): the
-// four waves of a block share one tile and take every fourth unit, which
-// the block adds in wave order in LDS -- four times the waves, a quarter of
-// the chain.  KS = 0: all three planes of a chunk unrolled.
+// loop (fewer registers, more waves resident); KS >= 4: the KS waves of a
+// block (64 KS threads) share one tile and take every KS-th unit, which the
+// block adds in wave order in LDS -- KS times the waves and 1 / KS of the
+// chain (on the small coarse grids KS is the number of units, so that every
+// gather of the tile is in flight at once).  KS = 0: all three planes of a
+// chunk unrolled.
 template <int NV, int MODE, int XF, int PK, int KS>
-__global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
+__global__ __launch_bounds__(KS >= 4 ? 64 * KS : 256) void k_conv(ugk_conv_desc d)
 {
   if (d.skip && *d.skip) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -221,24 +223,25 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
   const int mg = blockIdx.y;
   const long Vo = vox(d.go), Vi = vox(d.gi);
   const long total = Vo * d.nb;
-  const long base = (KS == 4 ? (long)blockIdx.x : ((long)blockIdx.x * 4 + wave)) * (16 * NV);
-  if (base >= total) return;               // uniform over the block when KS = 4
-  const int HWo = d.go.H * d.go.W;
+  const long base = (KS >= 4 ? (long)blockIdx.x : ((long)blockIdx.x * 4 + wave)) * (16 * NV);
+  if (base >= total) return;               // uniform over the block when KS >= 4
+  const unsigned HWo = (unsigned)(d.go.H * d.go.W), Wo = (unsigned)d.go.W;
   int vz[NV], vy[NV], vx[NV];
   long vb[NV];
   bool vok[NV];
 #pragma unroll
   for (int n = 0; n < NV; n++) {
+    // 32-bit index arithmetic (ugk_conv checks that the sizes fit)
     const long g = base + n * 16 + col;
     vok[n] = g < total;
-    const long gg = vok[n] ? g : 0;
-    const long b = gg / Vo;
-    const int v = (int)(gg - b * Vo);
-    vz[n] = v / HWo;
-    const int r = v - vz[n] * HWo;
-    vy[n] = r / d.go.W;
-    vx[n] = r - vy[n] * d.go.W;
-    vb[n] = b * Vi;
+    const unsigned gg = vok[n] ? (unsigned)g : 0u;
+    const unsigned b = gg / (unsigned)Vo;
+    const unsigned v = gg - b * (unsigned)Vo;
+    const unsigned z = v / HWo, r = v - z * HWo, y = r / Wo;
+    vz[n] = (int)z;
+    vy[n] = (int)y;
+    vx[n] = (int)(r - y * Wo);
+    vb[n] = (long)b * Vi;
   }
   f32x4 acc[NV];
 #pragma unroll
@@ -247,14 +250,14 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
   const long wstep = (long)d.cop * d.cip;
 
   Epi ep[NV];
-  if constexpr (KS == 4) {
+  if constexpr (KS >= 4) {
     const int co = mg * 16 + 4 * kq;
 #pragma unroll
     for (int n = 0; n < NV; n++) ep[n] = epi_load(d, base + n * 16 + col, co, wave == 0 && vok[n] && co < d.cout);
   }
   if constexpr (KS >= 1) {
     const int U = (d.cin + 15) / 16 * 3;
-    for (int u = KS == 4 ? wave : 0; u < U; u += KS) {
+    for (int u = KS >= 4 ? wave : 0; u < U; u += KS) {
       const int c = (u / 3) * 16 + 4 * kq, pz = u % 3;
       const bool cok = c < d.cin;
       const int cs = cok ? c : 0;
@@ -279,15 +282,19 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
 #pragma unroll
         for (int n = 0; n < NV; n++) mfma_frag<XF>(a[k], raw[k][n], ok[k][n], acc[n]);
     }
-    if constexpr (KS == 4) {
-      __shared__ f32x4 red[4][NV][64];
+    if constexpr (KS >= 4) {
+      __shared__ f32x4 red[KS][NV][64];
 #pragma unroll
       for (int n = 0; n < NV; n++) red[wave][n][lane] = acc[n];
       __syncthreads();
       if (wave) return;
 #pragma unroll
-      for (int n = 0; n < NV; n++)
-        acc[n] = ((red[0][n][lane] + red[1][n][lane]) + red[2][n][lane]) + red[3][n][lane];
+      for (int n = 0; n < NV; n++) {
+        f32x4 a = red[0][n][lane];
+#pragma unroll
+        for (int w = 1; w < KS; w++) a += red[w][n][lane];
+        acc[n] = a;
+      }
     }
   } else if constexpr (PK) {
     // tap groups of 4: lane group kq takes tap 4 q + kq, channels 0..3; all
@@ -352,7 +359,7 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
 #pragma unroll
   for (int n = 0; n < NV; n++) {
     if (!vok[n]) continue;
-    if constexpr (KS == 4) store_pre(d, base + n * 16 + col, co, acc[n], ep[n]);
+    if constexpr (KS >= 4) store_pre(d, base + n * 16 + col, co, acc[n], ep[n]);
     else store_out(d, base + n * 16 + col, co, acc[n]);
   }
 }
@@ -367,10 +374,10 @@ __global__ __launch_bounds__(256) void k_conv(ugk_conv_desc d)
 // of all live taps of a 16-channel chunk are in flight before its products.
 template <int PZ, int PY, int PX, int XF>
 __device__ __forceinline__ void t2_class(const ugk_conv_desc &d, const bf16 *wrow, long wstep, long xb, int cz,
-                                         int cy, int cx, bool fok, int kq, f32x4 &acc)
+                                         int cy, int cx, bool fok, int kq, f32x4 &acc, int cfirst, int cstep)
 {
   constexpr int NY = PY + 1, NX = PX + 1, NT = (PZ + 1) * NY * NX;
-  for (int c0 = 0; c0 < d.cin; c0 += 16) {
+  for (int c0 = cfirst; c0 < d.cin; c0 += cstep) {
     const int c = c0 + 4 * kq;
     const bool cok = c < d.cin;
     const int cs = cok ? c : 0;
@@ -395,7 +402,11 @@ __device__ __forceinline__ void t2_class(const ugk_conv_desc &d, const bf16 *wro
   }
 }
 
-template <int XF>
+// KS = 4: the four waves of a block share one group of 16 items and take
+// every fourth 16-channel chunk of the input, added in wave order in LDS
+// (four times the waves on the small coarse grids); KS = 0: one group of
+// items per wave, all chunks in its own chain.
+template <int XF, int KS>
 __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
 {
   if (d.skip && *d.skip) return;
@@ -405,16 +416,17 @@ __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
   const long Vc = vox(d.gi), Vf = vox(d.go);
   const long Vp = (Vc + 15) / 16 * 16;
   const long total = Vp * 8 * d.nb;
-  const long base = ((long)blockIdx.x * 4 + wave) * 16;
-  if (base >= total) return;
-  const long cls = base / Vp;               // (b, p), uniform over the wave
+  const long base = (KS ? (long)blockIdx.x : ((long)blockIdx.x * 4 + wave)) * 16;
+  if (base >= total) return;               // uniform over the block when KS = 4
+  // 32-bit index arithmetic (ugk_conv checks that the sizes fit)
+  const unsigned cls = (unsigned)base / (unsigned)Vp;   // (b, p), uniform over the wave
   const int p = (int)(cls & 7);
   const long b = cls >> 3;
-  const long ci = base - cls * Vp + col;     // coarse voxel of this lane
-  const bool cin_ok = ci < Vc;
-  const int HWc = d.gi.H * d.gi.W;
+  const unsigned ci = (unsigned)base - cls * (unsigned)Vp + col;   // coarse voxel of this lane
+  const bool cin_ok = ci < (unsigned)Vc;
+  const unsigned HWc = (unsigned)(d.gi.H * d.gi.W);
   const int cz = cin_ok ? (int)(ci / HWc) : 0;
-  const int rr = cin_ok ? (int)(ci - (long)cz * HWc) : 0;
+  const int rr = cin_ok ? (int)(ci - (unsigned)cz * HWc) : 0;
   const int cy = rr / d.gi.W, cx = rr - cy * d.gi.W;
   const int pz = (p >> 2) & 1, py = (p >> 1) & 1, px = p & 1;
   const int fz = 2 * cz + pz, fy = 2 * cy + py, fx = 2 * cx + px;
@@ -425,16 +437,27 @@ __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
   const long xb = b * Vc;
   const int co = mg * 16 + 4 * kq;
   const long gout = b * Vf + ((long)fz * d.go.H + fy) * d.go.W + fx;
-  const Epi ep = epi_load(d, gout, co, fok && co < d.cout);
+  const Epi ep = epi_load(d, gout, co, (!KS || wave == 0) && fok && co < d.cout);
+  const int c1 = KS ? 16 * wave : 0, cs = KS ? 16 * KS : 16;
   switch (p) {                              // wave-uniform
-    case 0: t2_class<0, 0, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
-    case 1: t2_class<0, 0, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
-    case 2: t2_class<0, 1, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
-    case 3: t2_class<0, 1, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
-    case 4: t2_class<1, 0, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
-    case 5: t2_class<1, 0, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
-    case 6: t2_class<1, 1, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
-    default: t2_class<1, 1, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc); break;
+    case 0: t2_class<0, 0, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+    case 1: t2_class<0, 0, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+    case 2: t2_class<0, 1, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+    case 3: t2_class<0, 1, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+    case 4: t2_class<1, 0, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+    case 5: t2_class<1, 0, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+    case 6: t2_class<1, 1, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+    default: t2_class<1, 1, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
+  }
+  if constexpr (KS) {
+    __shared__ f32x4 red[KS][64];
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave) return;
+    f32x4 a = red[0][lane];
+#pragma unroll
+    for (int w = 1; w < KS; w++) a += red[w][lane];
+    acc = a;
   }
   if (co >= d.cout || !fok) return;
   store_pre(d, gout, co, acc, ep);
@@ -503,17 +526,26 @@ __global__ __launch_bounds__(256) void k_conv_small_out(ugk_conv_desc d)
 template <int NV, int XF, int PK, int KS = 0>
 static void launch_conv_nv(const ugk_conv_desc &d, dim3 grid, hipStream_t s)
 {
-  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF, PK, KS><<<grid, 256, 0, s>>>(d);
-  else k_conv<NV, UGK_CONV_S2, XF, PK, KS><<<grid, 256, 0, s>>>(d);
+  const int threads = KS >= 4 ? 64 * KS : 256;
+  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF, PK, KS><<<grid, threads, 0, s>>>(d);
+  else k_conv<NV, UGK_CONV_S2, XF, PK, KS><<<grid, threads, 0, s>>>(d);
 }
 
+// below these many blocks the coarse-grid forms (more waves per tile) run
+#define CONV_WIDE_MAX 1024
+#define T2_WIDE_MAX 8192
 template <int XF>
 static void launch_conv(const ugk_conv_desc &d, long total, hipStream_t s)
 {
   const unsigned groups = (unsigned)(d.cop / 16);
   if (d.mode == UGK_CONV_T2) {
+    // few groups of 16 items (the coarse grids): the chunks of the sum over
+    // the waves of a block
     const long items = (vox(d.gi) + 15) / 16 * 16 * 8 * d.nb;
-    k_conv_t2<XF><<<dim3((unsigned)((items + 63) / 64), groups), 256, 0, s>>>(d);
+    if (d.cin >= 64 && items / 16 * groups < T2_WIDE_MAX)
+      k_conv_t2<XF, 4><<<dim3((unsigned)(items / 16), groups), 256, 0, s>>>(d);
+    else
+      k_conv_t2<XF, 0><<<dim3((unsigned)((items + 63) / 64), groups), 256, 0, s>>>(d);
     return;
   }
   if (d.mode == UGK_CONV_S1 && d.cout <= 4 && XF == 1 && d.cin <= SO_MAX_CIN && d.cin % 16 == 0) {
@@ -538,6 +570,14 @@ static void launch_conv(const ugk_conv_desc &d, long total, hipStream_t s)
   // fastest at every layer size of the network)
   int ks = 0;
   if (d.cin != 4) ks = (ks_env == 1 || ks_env == 9) ? (ks_env == 1 ? 1 : 0) : 4;
+  const long tiles = (total + 15) / 16 * groups;
+  if (ks == 4 && tiles < CONV_WIDE_MAX && (d.cin == 64 || d.cin == 32)) {
+    // few tiles (the coarse levels): one wave per (chunk, plane) unit
+    const dim3 g((unsigned)((total + 15) / 16), groups);
+    if (d.cin == 64) launch_conv_nv<1, XF, 0, 12>(d, g, s);
+    else launch_conv_nv<1, XF, 0, 6>(d, g, s);
+    return;
+  }
   if (ks == 4) {
     launch_conv_nv<1, XF, 0, 4>(d, dim3((unsigned)((total + 15) / 16), groups), s);
     return;
@@ -883,6 +923,8 @@ int ugk_conv(const ugk_conv_desc *d, void *s)
     return ugk_hip_fail("ugk_conv: channel layout", hipErrorInvalidValue);
   if (d->x_f32 < 0 || d->x_f32 > 2 || d->y_f32 < 0 || d->y_f32 > 1)
     return ugk_hip_fail("ugk_conv: activation format", hipErrorInvalidValue);
+  if (total >= (1L << 31) || (vox(d->gi) + 15) / 16 * 16 * 8 * d->nb >= (1L << 31))   // 32-bit voxel indices
+    return ugk_hip_fail("ugk_conv: grid too large", hipErrorInvalidValue);
   if (d->x_f32 == UGK_ACT_SPLIT) launch_conv<2>(*d, total, S(s));
   else if (d->x_f32 == UGK_ACT_F32) launch_conv<1>(*d, total, S(s));
   else launch_conv<0>(*d, total, S(s));
