@@ -537,7 +537,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--learn-fields", type=int, default=63)
-    ap.add_argument("--learn-steps", type=int, default=3000)
+    ap.add_argument("--learn-steps", type=int, default=10000)
     ap.add_argument("--learn-lr", type=float, default=1e-3)
     ap.add_argument("--net-node-grid", action="store_true",
                     help="the network with a node-grid level (default: the coarse-first form)")

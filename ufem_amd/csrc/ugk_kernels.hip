@@ -2489,6 +2489,7 @@ __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double 
   const int pz = cls >> 2, py = (cls >> 1) & 1, px = cls & 1;
   const unsigned Vc = (unsigned)gc.D * gc.H * gc.W, HWc = (unsigned)gc.H * gc.W, Wc = (unsigned)gc.W;
   double v[2] = {0, 0};
+#pragma unroll 1   // one copy of the eight class bodies: the code of all classes is live on every CU
   for (int rd = 0; rd < C0T_NODES / 16; rd++) {
     const unsigned c = (unsigned)blockIdx.x * C0T_NODES + rd * 16 + (lane >> 2);
     const unsigned cz = c / HWc, cr = c - cz * HWc, cy = cr / Wc, cx = cr - cy * Wc;
