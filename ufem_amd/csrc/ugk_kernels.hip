@@ -389,6 +389,19 @@ __device__ __forceinline__ void xcd_chunk(long n, long &i1, long &i2)
   i2 = min(n, i1 + chunk);
 }
 
+// The same with an even chunk, for kernels that move two entries per lane
+// with 16-byte accesses (chunk starts stay 16-byte aligned)
+__device__ __forceinline__ void xcd_chunk_even(long n, long &i1, long &i2)
+{
+  const long lb = (gridDim.x & 7) ? (long)blockIdx.x : remap_block(blockIdx.x, gridDim.x);
+  const long h = (n + 1) / 2;
+  const long c2 = h < (1L << 32) - gridDim.x ? (long)(((unsigned)h + gridDim.x - 1) / gridDim.x)
+                                             : (h + gridDim.x - 1) / gridDim.x;
+  i1 = min(n, lb * 2 * c2);
+  i2 = min(n, i1 + 2 * c2);
+}
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 // --------------------------------------------------------------------------
 // element stiffness
 // --------------------------------------------------------------------------
@@ -1078,21 +1091,39 @@ __global__ __launch_bounds__(WG) void k_cg1_upd(long n, double *__restrict__ x, 
   if (done) return;
   double v[4] = {0, 0, 0, 0};
   long i1, i2;
-  xcd_chunk(n, i1, i2);
-  for (long i = i1 + threadIdx.x; i < i2; i += WG) {
-    const double pi = u[i] + beta * p[i];
-    const double si = w[i] + beta * s[i];
-    const double xi = x[i] + alpha * pi;
-    const double ri = r[i] - alpha * si;
-    const double ui = ri * dinv[i];
-    p[i] = pi;
-    s[i] = si;
-    x[i] = xi;
-    r[i] = ri;
-    u[i] = ui;
-    v[0] += ri * ui;
-    v[1] += ri * ri;
-    v[2] += xi * xi;
+  xcd_chunk_even(n, i1, i2);
+  // two entries per lane: 16-byte loads and stores
+  for (long i = i1 + 2 * threadIdx.x; i < i2; i += 2 * WG) {
+    if (i + 1 < i2) {
+      const dbl2 u2 = *(const dbl2 *)(u + i), p2 = *(const dbl2 *)(p + i), w2 = *(const dbl2 *)(w + i),
+                 s2 = *(const dbl2 *)(s + i), x2 = *(const dbl2 *)(x + i), r2 = *(const dbl2 *)(r + i),
+                 d2 = *(const dbl2 *)(dinv + i);
+      const dbl2 pi = u2 + beta * p2, si = w2 + beta * s2;
+      const dbl2 xi = x2 + alpha * pi, ri = r2 - alpha * si;
+      const dbl2 ui = ri * d2;
+      *(dbl2 *)(p + i) = pi;
+      *(dbl2 *)(s + i) = si;
+      *(dbl2 *)(x + i) = xi;
+      *(dbl2 *)(r + i) = ri;
+      *(dbl2 *)(u + i) = ui;
+      v[0] += ri.x * ui.x + ri.y * ui.y;
+      v[1] += ri.x * ri.x + ri.y * ri.y;
+      v[2] += xi.x * xi.x + xi.y * xi.y;
+    } else {
+      const double pi = u[i] + beta * p[i];
+      const double si = w[i] + beta * s[i];
+      const double xi = x[i] + alpha * pi;
+      const double ri = r[i] - alpha * si;
+      const double ui = ri * dinv[i];
+      p[i] = pi;
+      s[i] = si;
+      x[i] = xi;
+      r[i] = ri;
+      u[i] = ui;
+      v[0] += ri * ui;
+      v[1] += ri * ri;
+      v[2] += xi * xi;
+    }
   }
   block_partials<4>(v, PART_B2(part, (i0 + 1) & 1));
   if (lead) st->iter = i0 + 1;
