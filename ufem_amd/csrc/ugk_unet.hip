@@ -202,6 +202,36 @@ __device__ __forceinline__ void mfma_frag(const bf16x4 &a, const typename Frag<X
   }
 }
 
+// the bf16 parts of a B fragment (zeros when !ok), and the products of one
+// weight fragment with them in the order of mfma_frag (smallest part
+// first), so a fragment split once serves several weight fragments with the
+// same results
+template <int XF>
+__device__ __forceinline__ void to_parts(const typename Frag<XF>::T &raw, bool ok, bf16x4 (&b)[3])
+{
+  const bf16x4 z = (bf16x4){0, 0, 0, 0};
+  if constexpr (XF == 2) {
+    b[0] = ok ? raw.p[0] : z;
+    b[1] = ok ? raw.p[1] : z;
+    b[2] = ok ? raw.p[2] : z;
+  } else if constexpr (XF == 1) {
+    split3(raw, ok, b);
+  } else {
+    b[0] = ok ? raw : z;
+  }
+}
+template <int XF>
+__device__ __forceinline__ void mfma_parts(const bf16x4 &a, const bf16x4 (&b)[3], f32x4 &acc)
+{
+  if constexpr (XF == 0) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[0], acc, 0, 0, 0);
+  } else {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[0], acc, 0, 0, 0);
+  }
+}
+
 // Stride-1 and stride-2 convolution.  Each wave computes one 16-channel
 // output group (blockIdx.y) for NV x 16 voxels; the 27 taps are unrolled
 // without branches (clamped addresses, zeroed fragments), so the gathers of
@@ -213,14 +243,17 @@ __device__ __forceinline__ void mfma_frag(const bf16x4 &a, const typename Frag<X
 // block adds in wave order in LDS -- KS times the waves and 1 / KS of the
 // chain (on the small coarse grids KS is the number of units, so that every
 // gather of the tile is in flight at once).  KS = 0: all three planes of a
-// chunk unrolled.
-template <int NV, int MODE, int XF, int PK, int KS>
+// chunk unrolled.  NG (KS >= 4, NV = 1): the wave computes NG 16-channel
+// output groups (blockIdx.y NG .. NG + NG - 1) from one set of gathered and
+// split input fragments.
+template <int NV, int MODE, int XF, int PK, int KS, int NG = 1>
 __global__ __launch_bounds__(KS >= 4 ? 64 * KS : 256) void k_conv(ugk_conv_desc d)
 {
+  static_assert(NG == 1 || (KS >= 4 && NV == 1), "output groups per wave need the KS >= 4 form");
   if (d.skip && *d.skip) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
-  const int mg = blockIdx.y;
+  const int mg = blockIdx.y * NG;
   const long Vo = vox(d.go), Vi = vox(d.gi);
   const long total = Vo * d.nb;
   const long base = (KS >= 4 ? (long)blockIdx.x : ((long)blockIdx.x * 4 + wave)) * (16 * NV);
@@ -250,6 +283,58 @@ __global__ __launch_bounds__(KS >= 4 ? 64 * KS : 256) void k_conv(ugk_conv_desc 
   const long wstep = (long)d.cop * d.cip;
 
   Epi ep[NV];
+  if constexpr (KS >= 4 && NG > 1) {
+    // one tile, NG output groups: every fragment split once for all groups
+    f32x4 ag[NG];
+    Epi eg[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+      ag[g] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      const int co = (mg + g) * 16 + 4 * kq;
+      eg[g] = epi_load(d, base + col, co, wave == 0 && vok[0] && co < d.cout);
+    }
+    const int U = (d.cin + 15) / 16 * 3;
+    for (int u = wave; u < U; u += KS) {
+      const int c = (u / 3) * 16 + 4 * kq, pz = u % 3;
+      const bool cok = c < d.cin;
+      const int cs = cok ? c : 0;
+      typename Frag<XF>::T raw[9];
+      bool ok[9];
+      bf16x4 a[NG][9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int t = 9 * pz + k;
+        const int dz = pz - 1, dy = k / 3 - 1, dx = k % 3 - 1;
+        const long sidx = tap_src<MODE>(vz[0], vy[0], vx[0], dz, dy, dx, d.gi);
+        ok[k] = vok[0] && cok && sidx >= 0;
+        raw[k] = load_raw<XF>(d.x, (ok[k] ? vb[0] + sidx : 0) * d.ldx + cs);
+#pragma unroll
+        for (int g = 0; g < NG; g++) a[g][k] = *(const bf16x4 *)(wrow + (long)g * 16 * d.cip + (d.flip ? 26 - t : t) * wstep + c);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        bf16x4 bx[3];
+        to_parts<XF>(raw[k], ok[k], bx);
+#pragma unroll
+        for (int g = 0; g < NG; g++) mfma_parts<XF>(a[g][k], bx, ag[g]);
+      }
+    }
+    __shared__ f32x4 redg[KS][NG][64];
+#pragma unroll
+    for (int g = 0; g < NG; g++) redg[wave][g][lane] = ag[g];
+    __syncthreads();
+    if (wave || !vok[0]) return;
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+      f32x4 a = redg[0][g][lane];
+#pragma unroll
+      for (int w = 1; w < KS; w++) a += redg[w][g][lane];
+      const int co = (mg + g) * 16 + 4 * kq;
+      if (co < d.cout) store_pre(d, base + col, co, a, eg[g]);
+    }
+    return;
+  }
   if constexpr (KS >= 4) {
     const int co = mg * 16 + 4 * kq;
 #pragma unroll
@@ -437,8 +522,8 @@ __global__ __launch_bounds__(256) void k_conv_t2(ugk_conv_desc d)
   const long xb = b * Vc;
   const int co = mg * 16 + 4 * kq;
   const long gout = b * Vf + ((long)fz * d.go.H + fy) * d.go.W + fx;
-  const Epi ep = epi_load(d, gout, co, (!KS || wave == 0) && fok && co < d.cout);
   const int c1 = KS ? 16 * wave : 0, cs = KS ? 16 * KS : 16;
+  const Epi ep = epi_load(d, gout, co, (!KS || wave == 0) && fok && co < d.cout);
   switch (p) {                              // wave-uniform
     case 0: t2_class<0, 0, 0, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
     case 1: t2_class<0, 0, 1, XF>(d, wrow, wstep, xb, cz, cy, cx, fok, kq, acc, c1, cs); break;
@@ -523,12 +608,20 @@ __global__ __launch_bounds__(256) void k_conv_small_out(ugk_conv_desc d)
   }
 }
 
-template <int NV, int XF, int PK, int KS = 0>
+template <int NV, int XF, int PK, int KS = 0, int NG = 1>
 static void launch_conv_nv(const ugk_conv_desc &d, dim3 grid, hipStream_t s)
 {
   const int threads = KS >= 4 ? 64 * KS : 256;
-  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF, PK, KS><<<grid, threads, 0, s>>>(d);
-  else k_conv<NV, UGK_CONV_S2, XF, PK, KS><<<grid, threads, 0, s>>>(d);
+  if (d.mode == UGK_CONV_S1) k_conv<NV, UGK_CONV_S1, XF, PK, KS, NG><<<grid, threads, 0, s>>>(d);
+  else k_conv<NV, UGK_CONV_S2, XF, PK, KS, NG><<<grid, threads, 0, s>>>(d);
+}
+// KS >= 4 forms with as many output groups per wave as divide the groups (<= 4)
+template <int XF, int KS>
+static void launch_conv_wide(const ugk_conv_desc &d, unsigned tiles_x, unsigned groups, hipStream_t s)
+{
+  if (groups % 4 == 0) launch_conv_nv<1, XF, 0, KS, 4>(d, dim3(tiles_x, groups / 4), s);
+  else if (groups % 2 == 0) launch_conv_nv<1, XF, 0, KS, 2>(d, dim3(tiles_x, groups / 2), s);
+  else launch_conv_nv<1, XF, 0, KS, 1>(d, dim3(tiles_x, groups), s);
 }
 
 // below these many blocks the coarse-grid forms (more waves per tile) run
@@ -573,13 +666,13 @@ static void launch_conv(const ugk_conv_desc &d, long total, hipStream_t s)
   const long tiles = (total + 15) / 16 * groups;
   if (ks == 4 && tiles < CONV_WIDE_MAX && (d.cin == 64 || d.cin == 32)) {
     // few tiles (the coarse levels): one wave per (chunk, plane) unit
-    const dim3 g((unsigned)((total + 15) / 16), groups);
-    if (d.cin == 64) launch_conv_nv<1, XF, 0, 12>(d, g, s);
-    else launch_conv_nv<1, XF, 0, 6>(d, g, s);
+    // (one output group per wave: on these grids more waves beat fewer, longer ones)
+    if (d.cin == 64) launch_conv_nv<1, XF, 0, 12>(d, dim3((unsigned)((total + 15) / 16), groups), s);
+    else launch_conv_nv<1, XF, 0, 6>(d, dim3((unsigned)((total + 15) / 16), groups), s);
     return;
   }
   if (ks == 4) {
-    launch_conv_nv<1, XF, 0, 4>(d, dim3((unsigned)((total + 15) / 16), groups), s);
+    launch_conv_wide<XF, 4>(d, (unsigned)((total + 15) / 16), groups, s);
     return;
   }
   if (ks == 1) {
