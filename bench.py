@@ -266,7 +266,7 @@ def bench_dropin(mesh, native_s, native_ms, u64):
             "solve_ms_ratio_to_native": round(calls[-1]["solve_ms"] / native_ms, 3)}
 
 
-def bench_learned(fields, steps, seed, levels=3, fine=True, lr=3e-3):
+def bench_learned(fields, steps, seed, levels=3, fine=True, lr=3e-3, width=64):
     """Learned preconditioner on the 128 x 32 x 32 GRF cantilever (421,443 DOF):
     pairs from plain-PCG solves on `fields` fresh fields (32 each), `steps`
     Adam steps, then plain PCG and flexible CG with the network on a field
@@ -280,7 +280,7 @@ def bench_learned(fields, steps, seed, levels=3, fine=True, lr=3e-3):
     mesh.E[:] = E_test
     s = G.GpuSystem(mesh)
     s.assemble(2)
-    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=4, levels=levels, fine=fine)
+    net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=4, levels=levels, fine=fine, width=width)
     net.init(1)
     net.reserve(32 * fields)
     pairs, collect_s = Lr.collect(s, net, mesh, range(1000, 1000 + fields), 32, every=8, log=log)
@@ -310,7 +310,7 @@ def bench_learned(fields, steps, seed, levels=3, fine=True, lr=3e-3):
         "workload": (f"cantilever 2.0x0.4x0.4 m, {nx}x{ny}x{nz} bricks, {mesh.n_dof} DOF, E = exp(GRF, "
                      f"corr 0.2 m, log-std 0.5, mean 30 GPa); test field seed {seed}, training fields "
                      f"1000..{1000 + fields - 1}"),
-        "levels": levels, "node_grid_level": fine, "parameters": int(net.P),
+        "levels": levels, "node_grid_level": fine, "width": width, "parameters": int(net.P),
         "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
         "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
         "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
@@ -329,7 +329,7 @@ def bench_learned(fields, steps, seed, levels=3, fine=True, lr=3e-3):
 
 
 def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=(8, 10, 10), levels=3, lr=3e-3,
-                        fine=True):
+                        fine=True, width=64):
     """Learned + mixed precision on the metric's own system `g` (the 1M-DOF
     lognormal cantilever, E = mesh.E): pairs from plain-PCG solves on `fields`
     fresh i.i.d. lognormal fields of the same distribution (seeds seed_base..),
@@ -343,7 +343,7 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
     from ufem_amd import learned as Lr
     from ufem_amd.mesh import lognormal_E
     E_test = mesh.E.copy()
-    net = G.GpuNet(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, batch=4, levels=levels, fine=fine)
+    net = G.GpuNet(mesh.nx + 1, mesh.ny + 1, mesh.nz + 1, batch=4, levels=levels, fine=fine, width=width)
     net.init(1)
     net.reserve(32 * fields)
     pairs, collect_s = Lr.collect(g, net, mesh, range(seed_base, seed_base + fields), 32, every=8, log=log,
@@ -392,7 +392,7 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
         "workload": (f"the metric's system: {mesh.nx}x{mesh.ny}x{mesh.nz} bricks, {mesh.n_dof} DOF, E lognormal "
                      f"COV {cov}; training fields: {fields} fresh lognormal fields, seeds "
                      f"{seed_base}..{seed_base + fields - 1}"),
-        "levels": levels, "node_grid_level": fine, "parameters": int(net.P),
+        "levels": levels, "node_grid_level": fine, "width": width, "parameters": int(net.P),
         "pairs": pairs, "collect_s": round(collect_s, 2), "train_steps": steps, "train_s": round(train_ms / 1e3, 2),
         "loss_first20": round(float(loss[:20].mean()), 4), "loss_last50": round(float(loss[-50:].mean()), 4),
         "plain": {"iters": rp["iters"], "solve_s": round(tp, 4), "true_relres": rp["true_relres"]},
@@ -542,6 +542,8 @@ def main():
     ap.add_argument("--net-node-grid", action="store_true",
                     help="the network with a node-grid level (default: the coarse-first form)")
     ap.add_argument("--net-levels", type=int, default=6, help="levels of the learned preconditioner's U-Net (2..8)")
+    ap.add_argument("--net-width", type=int, default=16,
+                    help="at most this many channels per layer of the U-Net (a multiple of 4, 4..64)")
     ap.add_argument("--learned-1m-only", action="store_true",
                     help="assemble the metric's system and run only the learned + mixed comparison")
     ap.add_argument("--kernels-only", action="store_true",
@@ -582,7 +584,8 @@ def main():
         mk = cantilever(nx, ny, nz, E=lognormal_E(args.seed, nx * ny * nz, cov=0.3))
         sk = G.GpuSystem(mk)
         sk.assemble(2)
-        net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=1, levels=args.net_levels, fine=args.net_node_grid)
+        net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=1, levels=args.net_levels, fine=args.net_node_grid,
+                       width=args.net_width)
         net.init(1)
         net.bind(sk, mk.E)
         log("network stages (us)", {k: round(v, 1) for k, v in net.time_stages(reps=20).items()})
@@ -625,7 +628,8 @@ def main():
         g.close()
         return
     if args.learned_solve_only:
-        net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=1)
+        net = G.GpuNet(nx + 1, ny + 1, nz + 1, batch=1, levels=args.net_levels, fine=args.net_node_grid,
+                       width=args.net_width)
         net.init(1)
         net.bind(g, mesh.E)
         g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 10, 10)
@@ -637,7 +641,7 @@ def main():
         return
     if args.learned_1m_only:
         out = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels,
-                                  lr=args.learn_lr, fine=args.net_node_grid)
+                                  lr=args.learn_lr, fine=args.net_node_grid, width=args.net_width)
         print(json.dumps({"learned_mixed_1m": out}), flush=True)
         g.close()
         return
@@ -775,14 +779,14 @@ def main():
     if not args.no_learned:
         log("learned + mixed precision at the metric's size ...")
         learned_mixed = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels,
-                                            lr=args.learn_lr, fine=args.net_node_grid)
+                                            lr=args.learn_lr, fine=args.net_node_grid, width=args.net_width)
         log("learned_mixed", learned_mixed)
 
     learned = None
     if not args.no_learned:
         log("learned preconditioner ...")
         learned = bench_learned(args.learn_fields, args.learn_steps, 9999, levels=args.net_levels,
-                                fine=args.net_node_grid, lr=args.learn_lr)
+                                fine=args.net_node_grid, lr=args.learn_lr, width=args.net_width)
         log("learned", learned)
 
     batch = None

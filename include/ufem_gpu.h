@@ -298,10 +298,14 @@ typedef struct ufem_net ufem_net;
 ufem_net *ufem_net_create(int W, int H, int D, int batch);  /* 3 levels; batch: training samples per step */
 ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels);   /* fine = 1 */
 ufem_net *ufem_net_create_form(int W, int H, int D, int batch, int levels, int fine);
+/* width: at most this many channels per layer (a multiple of 4, 4..64; the
+ * forms above use 64) */
+ufem_net *ufem_net_create_width(int W, int H, int D, int batch, int levels, int fine, int width);
 void ufem_net_destroy(ufem_net *N);
 long ufem_net_nparams(void);                 /* of the 3-level network: 70,640 */
 long ufem_net_nparams_levels(int levels);    /* -1 outside 2..8 */
 long ufem_net_nparams_form(int levels, int fine);
+long ufem_net_nparams_width(int levels, int fine, int width);   /* -1 out of range */
 int  ufem_net_fine(const ufem_net *N);
 int  ufem_net_levels(const ufem_net *N);
 int  ufem_net_init(ufem_net *N, unsigned long seed);   /* W ~ N(0, 1/(27 cin)), a = 0, b = -3 */

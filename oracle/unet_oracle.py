@@ -42,16 +42,26 @@ CH_ALL = (16, 32, 64, 64, 64, 64, 64, 64)
 CI_ALL = (3, 16, 32, 64, 64, 64, 64, 64)
 
 
-def layers(levels=3, fine=True):
+def layers(levels=3, fine=True, width=64):
+    """(name, in channels, out channels, stride) of C0 .. C_{L-1}; every
+    channel count capped at `width`"""
     if fine:
-        return [(f"W{l}", CI_ALL[l], CH_ALL[l], 1 if l == 0 else 2) for l in range(levels)]
-    ch = CH_ALL[1:levels + 1]
+        return [(f"W{l}", min(CI_ALL[l], width), min(CH_ALL[l], width), 1 if l == 0 else 2) for l in range(levels)]
+    ch = tuple(min(c, width) for c in CH_ALL[1:levels + 1])
     ci = (3,) + tuple(ch[:-1])
     return [(f"W{l}", ci[l], ch[l], 2) for l in range(levels)]
 
 
 def is_fine(P):
+    """the form of a full-width parameter set (W0 with 16 outputs: node grid);
+    a narrower set says its form through the grids of ebar (forward)"""
     return P["W0"].shape[1] == CH_ALL[0]
+
+
+def stride0(v, eb):
+    """C0's stride: 1 when ebar_0 lies on the node grid of v (node-grid
+    form), 2 when it is pooled once (coarse first)"""
+    return 1 if tuple(eb[0].shape) == tuple(v.shape[:3]) else 2
 
 
 CH = CH_ALL[:3]
@@ -146,12 +156,12 @@ def n_levels(P):
     return sum(1 for k in P if k.startswith("W"))
 
 
-def init_params(seed=0, levels=3, fine=True):
+def init_params(seed=0, levels=3, fine=True, width=64):
     rng = np.random.default_rng(seed)
     P = {}
-    for name, ci, co, _ in layers(levels, fine):
+    for name, ci, co, _ in layers(levels, fine, width):
         P[name] = rng.standard_normal((27, co, ci)) / np.sqrt(27 * ci)
-    for l, (_, _, co, _) in enumerate(layers(levels, fine)):
+    for l, (_, _, co, _) in enumerate(layers(levels, fine, width)):
         P[f"a{l}"] = np.zeros(co)
         P[f"b{l}"] = np.zeros(co)
     return P
@@ -164,9 +174,9 @@ def gates(P, eb):
 def forward(P, v, eb):
     """v [D][H][W][3] -> N(v) [D][H][W][3], with the intermediate fields."""
     L = n_levels(P)
-    s0 = 1 if is_fine(P) else 2
+    s0 = stride0(v, eb)
     g = gates(P, eb)
-    c = {"v": v, "g": g, "eb": eb}
+    c = {"v": v, "g": g, "eb": eb, "s0": s0}
     h = [conv(v, P["W0"], s0)]
     for l in range(1, L):
         h.append(conv(h[-1], P[f"W{l}"], 2))
@@ -183,7 +193,7 @@ def forward(P, v, eb):
 def backward(P, c, dy):
     """gradient of <dy, N(v)> with respect to every parameter"""
     L = n_levels(P)
-    s0 = 1 if is_fine(P) else 2
+    s0 = c["s0"]
     g = c["g"]
     G = {}
     dt = [conv(dy, P["W0"], s0)]                 # adjoint of y = conv_adjoint(t0)

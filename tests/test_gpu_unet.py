@@ -123,9 +123,9 @@ def pack(P):
                           [P[f"{k}{l}"] for l in range(L) for k in ("a", "b")]).astype(np.float32)
 
 
-def unpack(v, levels=3, fine=True):
+def unpack(v, levels=3, fine=True, width=64):
     P, o = {}, 0
-    lay = U.layers(levels, fine)
+    lay = U.layers(levels, fine, width)
     for name, ci, co, _ in lay:
         P[name] = v[o:o + 27 * co * ci].reshape(27, co, ci).astype(np.float64)
         o += 27 * co * ci
@@ -138,14 +138,14 @@ def unpack(v, levels=3, fine=True):
     return P
 
 
-def random_params(seed, levels=3, fine=True):
+def random_params(seed, levels=3, fine=True, width=64):
     rng = np.random.default_rng(seed)
-    P = U.init_params(seed, levels, fine)
+    P = U.init_params(seed, levels, fine, width)
     for l in range(levels):
         c = P[f"a{l}"].size
         P[f"a{l}"] = 0.3 * rng.standard_normal(c)
         P[f"b{l}"] = 0.3 * rng.standard_normal(c)
-    return unpack(bf16(pack(P)), levels, fine)      # weights exactly as the kernels see them
+    return unpack(bf16(pack(P)), levels, fine, width)      # weights exactly as the kernels see them
 
 
 @pytest.fixture(scope="module")
@@ -189,24 +189,28 @@ def test_network_gradient_matches_oracle(small_net):
         assert rel(a, r) <= tol and cos >= 0.999, (k, rel(a, r), cos)
 
 
-@pytest.mark.parametrize("levels,fine", [(2, True), (5, True), (2, False), (4, False)])
-def test_deeper_network_forward_and_gradient_match_oracle(levels, fine):
-    """the L-level network (down to grids of one node along an axis) and the
-    coarse-first form against the oracle: forward within 1e-5, parameter
-    gradient as above"""
+@pytest.mark.parametrize("levels,fine,width", [(2, True, 64), (5, True, 64), (2, False, 64), (4, False, 64),
+                                                (4, False, 16), (3, True, 16), (6, False, 32)])
+def test_deeper_network_forward_and_gradient_match_oracle(levels, fine, width):
+    """the L-level network (down to grids of one node along an axis), the
+    coarse-first form and narrower layers against the oracle: forward within
+    1e-5, parameter gradient as above"""
     D, H, W = 5, 4, 11
     shape = (D, H, W)
-    net = G.GpuNet(W, H, D, batch=2, levels=levels, fine=fine)
-    assert net.P == G.lib().ufem_net_nparams_form(levels, int(fine)) and net.weights().size == net.P
+    net = G.GpuNet(W, H, D, batch=2, levels=levels, fine=fine, width=width)
+    assert net.P == G.lib().ufem_net_nparams_width(levels, int(fine), width) and net.weights().size == net.P
+    if width == 64:
+        assert net.P == G.lib().ufem_net_nparams_form(levels, int(fine))
     rng = np.random.default_rng(20 + levels)
-    P = random_params(30 + levels, levels, fine)
-    assert U.is_fine(P) == fine
+    P = random_params(30 + levels, levels, fine, width)
+    if width == 64:
+        assert U.is_fine(P) == fine
     net.set_weights(pack(P))
     x = rng.standard_normal((2,) + shape + (3,))
     e0 = 0.5 * rng.standard_normal((2,) + shape)
     dy = rng.standard_normal((2,) + shape + (3,))
     y = net.forward(x.reshape(2, -1, 3), e0.reshape(2, -1))
-    g = unpack(net.backward(x.reshape(2, -1, 3), e0.reshape(2, -1), dy.reshape(2, -1, 3)), levels, fine)
+    g = unpack(net.backward(x.reshape(2, -1, 3), e0.reshape(2, -1), dy.reshape(2, -1, 3)), levels, fine, width)
     ref = {k: 0.0 for k in P}
     for b in range(2):
         eb = U.ebar_levels(e0[b].astype(np.float32).astype(np.float64), levels, fine)
@@ -228,6 +232,11 @@ def test_levels_out_of_range():
         G.GpuNet(9, 5, 5, levels=9)
     assert G.lib().ufem_net_nparams_levels(1) == -1
     assert G.lib().ufem_net_nparams_levels(3) == G.lib().ufem_net_nparams() == 70640
+    with pytest.raises(RuntimeError, match="width"):
+        G.GpuNet(9, 5, 5, levels=3, width=6)
+    assert G.lib().ufem_net_nparams_width(3, 1, 0) == -1 and G.lib().ufem_net_nparams_width(3, 1, 68) == -1
+    # six coarse-first levels of at most 16 channels (the bench's form)
+    assert G.lib().ufem_net_nparams_width(6, 0, 16) == 6 * 2 * 16 + 27 * 16 * (3 + 5 * 16)
 
 
 def beam_mesh(seed, nx=24, ny=6, nz=6):
@@ -355,15 +364,16 @@ def test_trained_network_beats_jacobi(beam):
     assert rm["iters"] <= 2.2 * r["iters"]
 
 
+@pytest.mark.parametrize("width", [64, 16])
 @pytest.mark.parametrize("solver", ["learned", "learned_mixed"])
-def test_fused_last_layer_matches_the_network_stage(beam, solver, monkeypatch):
+def test_fused_last_layer_matches_the_network_stage(beam, solver, width, monkeypatch):
     """The coarse-first network's last layer y = C0^T t0 folded into the
     flexible-CG combine (f32 on the vector ALUs) against the network's own
     stage followed by the plain combine: x after a few iterations agrees to
     f32 rounding (a wrong tap, weight or parity would move it at O(1)), and
     both reach the same solution."""
     m, s, ref, u_ref, (W, H, D) = beam
-    net = G.GpuNet(W, H, D, levels=4, fine=False)
+    net = G.GpuNet(W, H, D, levels=4, fine=False, width=width)   # C0 with 32 or 16 channels
     net.init(5)
     net.bind(s, m.E)
     for maxit in (3, 11):

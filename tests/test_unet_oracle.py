@@ -150,9 +150,9 @@ def test_energy_loss_value_and_planted_mistake():
     assert all(np.all(G[k] == 0) for k in ("W0", "W1", "W2"))
 
 
-def random_params_levels(seed, levels, gate_scale=0.3, fine=True):
+def random_params_levels(seed, levels, gate_scale=0.3, fine=True, width=64):
     rng = np.random.default_rng(seed)
-    P = U.init_params(seed, levels, fine)
+    P = U.init_params(seed, levels, fine, width)
     for l in range(levels):
         c = P[f"a{l}"].size
         P[f"a{l}"] = gate_scale * rng.standard_normal(c)
@@ -160,14 +160,17 @@ def random_params_levels(seed, levels, gate_scale=0.3, fine=True):
     return P
 
 
-@pytest.mark.parametrize("levels,fine", [(2, True), (5, True), (2, False), (4, False)])
-def test_deeper_network_is_symmetric_positive_semidefinite(levels, fine):
-    """more levels (down to grids of one node along an axis), and the
-    coarse-first form, keep N linear, symmetric and PSD; an untied decoder at
-    the deepest level breaks it"""
+@pytest.mark.parametrize("levels,fine,width", [(2, True, 64), (5, True, 64), (2, False, 64), (4, False, 64),
+                                                (4, False, 16), (3, True, 16)])
+def test_deeper_network_is_symmetric_positive_semidefinite(levels, fine, width):
+    """more levels (down to grids of one node along an axis), the
+    coarse-first form and narrower layers keep N linear, symmetric and PSD;
+    an untied decoder at the deepest level breaks it"""
     shape = (5, 4, 11)
-    P = random_params_levels(6, levels, fine=fine)
-    assert U.is_fine(P) == fine and U.n_levels(P) == levels
+    P = random_params_levels(6, levels, fine=fine, width=width)
+    assert U.n_levels(P) == levels and all(P[f"W{l}"].shape[1] <= width for l in range(levels))
+    if width == 64:
+        assert U.is_fine(P) == fine
     eb = U.ebar_levels(0.5 * np.random.default_rng(7).standard_normal(shape), levels, fine)
     assert len(eb) == levels and eb[-1].shape == tuple(U.coarse_shape(eb[-2].shape))
     assert eb[0].shape == (shape if fine else tuple(U.coarse_shape(shape)))
@@ -195,8 +198,8 @@ def test_deeper_network_is_symmetric_positive_semidefinite(levels, fine):
     assert abs(np.sum(u1 * v2) - np.sum(v1 * u2)) > 1e-6 * abs(np.sum(u1 * v2))
 
 
-@pytest.mark.parametrize("fine", [True, False])
-def test_deeper_network_gradients_match_finite_differences(fine):
+@pytest.mark.parametrize("fine,width", [(True, 64), (False, 64), (False, 16)])
+def test_deeper_network_gradients_match_finite_differences(fine, width):
     shape = (5, 4, 9)
     levels = 4
     n = 3 * int(np.prod(shape))
@@ -206,7 +209,7 @@ def test_deeper_network_gradients_match_finite_differences(fine):
     mask[:6] = 0.0
     e = RNG.standard_normal(n) * mask
     r = K @ e
-    P = random_params_levels(9, levels, fine=fine)
+    P = random_params_levels(9, levels, fine=fine, width=width)
     eb = U.ebar_levels(0.5 * np.random.default_rng(10).standard_normal(shape), levels, fine)
     L, G, _ = U.energy_loss(P, K, r, e, dinv, mask, eb, shape)
     rng = np.random.default_rng(11)

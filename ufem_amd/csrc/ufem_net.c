@@ -106,24 +106,31 @@ static ugk_grid3 coarse(ugk_grid3 g)
   return c;
 }
 
-static void channels(int levels, int fine, int *ch, int *ci)
+/* channels of C_l, capped at `width` (a multiple of 4, 4 .. 64) */
+static void channels(int levels, int fine, int width, int *ch, int *ci)
 {
   int l;
   for (l = 0; l < levels; l++) {
     ch[l] = fine ? CH_F[l] : CH_F[l + 1];
     ci[l] = fine ? CI_F[l] : (l ? CH_F[l] : 3);
+    if (ch[l] > width) ch[l] = width;
+    if (ci[l] > width) ci[l] = width;
   }
 }
 
-long ufem_net_nparams_form(int levels, int fine)
+static int width_ok(int width) { return width >= 4 && width <= 64 && width % 4 == 0; }
+
+long ufem_net_nparams_width(int levels, int fine, int width)
 {
   int ch[NL_MAX], ci[NL_MAX], l;
   long P = 0;
-  if (levels < 2 || levels > NL_MAX) return -1;
-  channels(levels, fine, ch, ci);
+  if (levels < 2 || levels > NL_MAX || !width_ok(width)) return -1;
+  channels(levels, fine, width, ch, ci);
   for (l = 0; l < levels; l++) P += 27L * ci[l] * ch[l] + 2L * ch[l];
   return P;
 }
+
+long ufem_net_nparams_form(int levels, int fine) { return ufem_net_nparams_width(levels, fine, 64); }
 
 long ufem_net_nparams_levels(int levels) { return ufem_net_nparams_form(levels, 1); }
 long ufem_net_nparams(void) { return ufem_net_nparams_levels(3); }
@@ -197,17 +204,23 @@ ufem_net *ufem_net_create_levels(int W, int H, int D, int batch, int levels)
 
 ufem_net *ufem_net_create_form(int W, int H, int D, int batch, int levels, int fine)
 {
+  return ufem_net_create_width(W, H, D, batch, levels, fine, 64);
+}
+
+ufem_net *ufem_net_create_width(int W, int H, int D, int batch, int levels, int fine, int width)
+{
   ufem_net *N;
   int l, bad = 0;
   long nb, maxpart = 0, n;
   if (W < 2 || H < 2 || D < 2 || batch < 1) { ufem_set_error("net grid or batch out of range"); return NULL; }
   if (levels < 2 || levels > NL_MAX) { ufem_set_error("levels %d out of range (2..%d)", levels, NL_MAX); return NULL; }
+  if (!width_ok(width)) { ufem_set_error("width %d out of range (a multiple of 4, 4..64)", width); return NULL; }
   if (ugk_init()) { net_fail("device"); return NULL; }
   N = (ufem_net *)calloc(1, sizeof(*N));
   if (!N) return NULL;
   N->L = levels;
   N->fine = fine ? 1 : 0;
-  channels(levels, N->fine, N->ch, N->ci);
+  channels(levels, N->fine, width, N->ch, N->ci);
   N->gf.D = D; N->gf.H = H; N->gf.W = W;
   N->Vf = (long)D * H * W;
   N->g[0] = N->fine ? N->gf : coarse(N->gf);
@@ -960,7 +973,7 @@ int ufem_net_infer(ufem_net *N, const int *skip, void *stream)
  * transpose); 0 when the form has no such stage */
 int ufem_net_last_stage(const ufem_net *N, ugk_conv_desc *d)
 {
-  if (N->fine) return 0;
+  if (N->fine || (N->ch[0] != 16 && N->ch[0] != 32)) return 0;
   memset(d, 0, sizeof(*d));
   d->x = N->t[0]; d->ldx = N->ch[0]; d->cin = N->ch[0]; d->x_f32 = UGK_ACT_F32;
   d->w = N->wt[0]; d->cip = r16(N->ch[0]); d->cop = r16(N->ci[0]);

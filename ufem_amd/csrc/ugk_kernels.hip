@@ -2411,27 +2411,27 @@ __global__ __launch_bounds__(WG) void k_f1_combine(long n, const double *__restr
 }
 
 // The same combine with the network's last layer in it: y = C0^T t0, the
-// stride-2 transpose from coarse level 0 (32 f32 channels) to the node grid,
+// stride-2 transpose from coarse level 0 (CIN = 16 or 32 f32 channels) to the node grid,
 // 3 outputs, on the vector ALUs in f32.  A node f = 2 c + p of parity class
 // p takes, per axis, the coarse node c (offset 0) when p = 0, and c (offset
 // +1) and c + 1 (offset -1, when it exists) when p = 1: 1 to 8 taps.  A
 // block of 8 waves takes 64 coarse nodes, wave w the nodes of class w, in
 // rounds of 16 (so a wave's taps are compile-time and its lanes never
-// diverge); four lanes share a node, each with 8 of the 32 channels, all
+// diverge); four lanes share a node, each with a quarter of the channels, all
 // its gathers and its epilogue loads in flight together; the four partial
 // sums meet by lane shuffles in a fixed order, and lane q < 3 forms z for
 // DOF 3 f + q.  The weights ([27][cop][cip] bf16, the adjoint's layout) are
-// widened to f32 in LDS as [27][3][32].  Replaces the last network stage
+// widened to f32 in LDS as [27][3][CIN].  Replaces the last network stage
 // and k_f1_combine.
-#define C0T_CIN 32
+#define C0T_CIN 32          /* the largest CIN */
 #define C0T_TB 512
 #define C0T_NODES 64
 
-template <int PZ, int PY, int PX>
+template <int PZ, int PY, int PX, int CIN>
 __device__ __forceinline__ void c0t_class(const float *__restrict__ x, int ldx, const float *sw, ugk_grid3 gc, int cz,
                                           int cy, int cx, bool fok, int qd, float (&y)[3])
 {
-  constexpr int NZ = PZ + 1, NY = PY + 1, NX = PX + 1, NT = NZ * NY * NX;
+  constexpr int NZ = PZ + 1, NY = PY + 1, NX = PX + 1, NT = NZ * NY * NX, CQ = CIN / 4;
   float4 xa[NT], xb[NT];
   bool ok[NT];
 #pragma unroll
@@ -2442,25 +2442,30 @@ __device__ __forceinline__ void c0t_class(const float *__restrict__ x, int ldx, 
     xa[j] = make_float4(0.f, 0.f, 0.f, 0.f);
     xb[j] = xa[j];
     if (ok[j]) {
-      const float *xs = x + (((long)sz * gc.H + sy) * gc.W + sx) * ldx + 8 * qd;
+      const float *xs = x + (((long)sz * gc.H + sy) * gc.W + sx) * ldx + CQ * qd;
       xa[j] = *(const float4 *)xs;
-      xb[j] = *(const float4 *)(xs + 4);
+      if constexpr (CQ == 8) xb[j] = *(const float4 *)(xs + 4);
     }
   }
 #pragma unroll
   for (int j = 0; j < NT; j++) {
     const int iz = j / (NY * NX), iy = (j / NX) % NY, ix = j % NX;
     const int tz = PZ ? 2 - 2 * iz : 1, ty = PY ? 2 - 2 * iy : 1, tx = PX ? 2 - 2 * ix : 1;
-    const float *w = sw + (9 * tz + 3 * ty + tx) * 3 * C0T_CIN + 8 * qd;
+    const float *w = sw + (9 * tz + 3 * ty + tx) * 3 * CIN + CQ * qd;
 #pragma unroll
     for (int o = 0; o < 3; o++) {
-      const float4 w0 = *(const float4 *)(w + o * C0T_CIN), w1 = *(const float4 *)(w + o * C0T_CIN + 4);
-      y[o] += w0.x * xa[j].x + w0.y * xa[j].y + w0.z * xa[j].z + w0.w * xa[j].w + w1.x * xb[j].x +
-              w1.y * xb[j].y + w1.z * xb[j].z + w1.w * xb[j].w;
+      const float4 w0 = *(const float4 *)(w + o * CIN);
+      float acc = w0.x * xa[j].x + w0.y * xa[j].y + w0.z * xa[j].z + w0.w * xa[j].w;
+      if constexpr (CQ == 8) {
+        const float4 w1 = *(const float4 *)(w + o * CIN + 4);
+        acc += w1.x * xb[j].x + w1.y * xb[j].y + w1.z * xb[j].z + w1.w * xb[j].w;
+      }
+      y[o] += acc;
     }
   }
 }
 
+template <int CIN>
 __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double *__restrict__ r,
                                                            const double *__restrict__ dinv,
                                                            const float *__restrict__ mask, ugk_conv_desc cd,
@@ -2506,8 +2511,8 @@ __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double 
   const bool net = !st->fb;                   // the network ran for this iteration
   if (net) {
     const unsigned short *w = cd.w;
-    for (int i = threadIdx.x; i < 27 * 3 * C0T_CIN; i += C0T_TB) {
-      const int tp = i / (3 * C0T_CIN), o = (i / C0T_CIN) % 3, c = i % C0T_CIN;
+    for (int i = threadIdx.x; i < 27 * 3 * CIN; i += C0T_TB) {
+      const int tp = i / (3 * CIN), o = (i / CIN) % 3, c = i % CIN;
       sw[i] = __uint_as_float((unsigned)w[((long)tp * cd.cop + o) * cd.cip + c] << 16);
     }
   }
@@ -2540,14 +2545,14 @@ __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double 
     float y[3] = {0.f, 0.f, 0.f};
     if (net) {
       switch (cls) {                        // wave-uniform
-        case 0: c0t_class<0, 0, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 1: c0t_class<0, 0, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 2: c0t_class<0, 1, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 3: c0t_class<0, 1, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 4: c0t_class<1, 0, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 5: c0t_class<1, 0, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 6: c0t_class<1, 1, 0>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        default: c0t_class<1, 1, 1>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 0: c0t_class<0, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 1: c0t_class<0, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 2: c0t_class<0, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 3: c0t_class<0, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 4: c0t_class<1, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 5: c0t_class<1, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        case 6: c0t_class<1, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        default: c0t_class<1, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
       }
 #pragma unroll
       for (int o = 0; o < 3; o++) {
@@ -4455,8 +4460,8 @@ int ugk_f1_combine(long n, const double *r, const double *dinv, const float *mas
 int ugk_f1_combine_c0t(long n, const double *r, const double *dinv, const float *mask, const ugk_conv_desc *cd,
                        const double *zc, const double *q, double *z, ugk_state *st, double *part, void *s)
 {
-  if (cd->mode != UGK_CONV_T2 || cd->x_f32 != UGK_ACT_F32 || cd->cin != C0T_CIN || cd->ldx % 4 ||
-      cd->ldx < C0T_CIN || cd->cout != 3 || cd->cop < 3 || cd->cip < C0T_CIN || cd->nb != 1 ||
+  if (cd->mode != UGK_CONV_T2 || cd->x_f32 != UGK_ACT_F32 || (cd->cin != 32 && cd->cin != 16) || cd->ldx % 4 ||
+      cd->ldx < cd->cin || cd->cout != 3 || cd->cop < 3 || cd->cip < cd->cin || cd->nb != 1 ||
       n != 3L * cd->go.D * cd->go.H * cd->go.W || n / 3 >= (1L << 31) || cd->gi.D != (cd->go.D + 1) / 2 ||
       cd->gi.H != (cd->go.H + 1) / 2 || cd->gi.W != (cd->go.W + 1) / 2) {
     snprintf(g_err, sizeof(g_err), "flexible CG: the network's last layer does not fit the fused combine");
@@ -4467,8 +4472,9 @@ int ugk_f1_combine_c0t(long n, const double *r, const double *dinv, const float 
     snprintf(g_err, sizeof(g_err), "flexible CG: coarse level too large for the fused combine's partials");
     return -1;
   }
-  k_f1_combine_c0t<<<(unsigned)((vc + C0T_NODES - 1) / C0T_NODES), C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z,
-                                                                                      st, part);
+  const unsigned G = (unsigned)((vc + C0T_NODES - 1) / C0T_NODES);
+  if (cd->cin == 32) k_f1_combine_c0t<32><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part);
+  else k_f1_combine_c0t<16><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part);
   CK_LAUNCH("k_f1_combine_c0t");
   return 0;
 }

@@ -84,6 +84,8 @@ def lib():
             "ufem_net_create_form": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "ufem_net_nparams_levels": (C.c_long, [C.c_int]),
             "ufem_net_nparams_form": (C.c_long, [C.c_int, C.c_int]),
+            "ufem_net_create_width": (C.c_void_p, [C.c_int] * 7),
+            "ufem_net_nparams_width": (C.c_long, [C.c_int, C.c_int, C.c_int]),
             "ufem_net_fine": (C.c_int, [C.c_void_p]),
             "ufem_net_levels": (C.c_int, [C.c_void_p]),
             "ufem_net_init": (C.c_int, [C.c_void_p, C.c_ulong]),
@@ -404,17 +406,18 @@ def _pf(a):
 class GpuNet:
     """The learned preconditioner's U-Net (ufem_net_*) on a W x H x D node
     grid, with `levels` levels (2..8); fine=False: the coarse-first form
-    (no node-grid level)."""
+    (no node-grid level); width: at most this many channels per layer."""
 
-    def __init__(self, W, H, D, batch=1, levels=3, fine=True):
-        self.h = lib().ufem_net_create_form(W, H, D, batch, levels, 1 if fine else 0)
+    def __init__(self, W, H, D, batch=1, levels=3, fine=True, width=64):
+        self.h = lib().ufem_net_create_width(W, H, D, batch, levels, 1 if fine else 0, width)
         if not self.h:
             raise RuntimeError("ufem_net_create: " + last_error())
         self.shape = (D, H, W)
         self.V = W * H * D
         self.levels = levels
         self.fine = fine
-        self.P = lib().ufem_net_nparams_form(levels, 1 if fine else 0)
+        self.width = width
+        self.P = lib().ufem_net_nparams_width(levels, 1 if fine else 0, width)
 
     def close(self):
         if getattr(self, "h", None):
