@@ -68,7 +68,7 @@ def cpu_baseline(nx, ny, nz, seed, t1_iters=20):
     points there: the per-iteration cost does not depend on K's values), and
     the 20x4x4 cantilever under the reference's own criterion (eps 1e-21,
     maxIt n), timed with the compiled reference when it is built.  Runs in a
-    child process beside the GPU legs."""
+    child process after the GPU legs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ufem_oracle as uo
     from ufem_amd.mesh import cantilever, lognormal_E
@@ -155,7 +155,7 @@ def finish_cpu_baseline(proc, mesh, timeout=900):
                    f"full {mesh.n_dof}-DOF system solved to ||r||/||f|| <= 1e-8 in {c['iters']} iterations, "
                    f"timed whole (no extrapolation); phases {ph}; T=1 {c['t1_ms_per_iteration']} ms per "
                    f"iteration, T={c['threads']} {c['t8_ms_per_iteration']} ms; run in a child process "
-                   f"beside the GPU legs"),
+                   f"after the GPU legs"),
     })
     return c
 
@@ -594,11 +594,10 @@ def main():
         return
 
     nx, ny, nz = (int(v) for v in args.size.split(","))
-    cpu_proc = None
-    if not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.plain_only or args.ssor_only
-                                or args.learned_solve_only or args.learned_1m_only):
-        cpu_proc = start_cpu_baseline(args)
-        log("CPU baseline started in a child process")
+    # the CPU baseline runs after the GPU legs: its threads beside them would
+    # delay the host side of the short solves (two-level: about 20 ms)
+    want_cpu = not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.plain_only or args.ssor_only
+                                        or args.learned_solve_only or args.learned_1m_only)
     t0 = time.perf_counter()
     mesh = cantilever(nx, ny, nz, E=lognormal_E(args.seed, nx * ny * nz, cov=0.3))
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_dof} DOF in {time.perf_counter() - t0:.1f} s")
@@ -802,9 +801,9 @@ def main():
         log("monte", monte)
 
     cpu = None
-    if cpu_proc is not None:
-        log("waiting for the CPU baseline ...")
-        cpu = finish_cpu_baseline(cpu_proc, mesh)
+    if want_cpu:
+        log("CPU baseline (child process) ...")
+        cpu = finish_cpu_baseline(start_cpu_baseline(args), mesh)
         if cpu:
             log("cpu", cpu["sample"])
 
