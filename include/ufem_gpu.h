@@ -178,6 +178,13 @@ int  ufem_sys_get_F(ufem_sys *s, double *F);
 /* K as sorted scalar CSR (0-based), explicit zeros included */
 int  ufem_sys_get_K(ufem_sys *s, long *indptr, int *cols, double *vals);
 int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host out */
+/* y = K x from the FP32 copy of K that the mixed-precision solvers use
+ * (no counterpart in the reference): the copy rounds every value to FP32 and,
+ * for 3x3 node blocks, adds to each diagonal block the symmetric part of its
+ * block row's summed rounding errors, so rigid translations, which a
+ * stiffness matrix's rows cancel, stay cancelled; FP64 vectors and sums.
+ * UFEM_NO_CORR32 set in the environment drops the correction. */
+int  ufem_sys_spmv_f32(ufem_sys *s, const double *x, double *y);
 /* average device time (HIP events on the system's stream) of `reps`
  * launches of one kernel, and the bytes one launch moves by the algorithmic
  * count.  mode 0: y = K x (FP64); 1: y = K x (FP32 copy of K); 2: the PCG
@@ -344,6 +351,12 @@ int  ufem_net_train(ufem_net *N, long steps, int batch, double lr, unsigned long
  * x_f32 0 or 1) */
 int  ufem_net_probe_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin,
                          int cout, const float *x, const float *w, float *y, const float *dy, float *dw);
+/* y = N(v) for the bound field with the solver's FP64 inference (FP64
+ * activations and sums, bf16 weights widened exactly, f32 gates): v, y
+ * [D][H][W][3] on the node grid, host FP64; the coarse-first form with 16 or
+ * 32 channels per level (no counterpart in the reference: the network
+ * preconditions its femEqsCGwJ, fem/fem_eqs.c:88-237) */
+int  ufem_net_apply64(ufem_net *N, const double *v, double *y);
 /* average device time (us) of each of the 2 L inference stages on the bound
  * system's grid: C0, C1, .., C_{L-1} (with the gate product), C_{L-1}^T, ..,
  * C1^T, C0^T */

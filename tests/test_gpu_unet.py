@@ -371,11 +371,13 @@ def test_fused_last_layer_matches_the_network_stage(beam, solver, width, monkeyp
     flexible-CG combine (f32 on the vector ALUs) against the network's own
     stage followed by the plain combine: x after a few iterations agrees to
     f32 rounding (a wrong tap, weight or parity would move it at O(1)), and
-    both reach the same solution."""
+    both reach the same solution.  Both on the matrix-core path
+    (UFEM_NET_F32)."""
     m, s, ref, u_ref, (W, H, D) = beam
     net = G.GpuNet(W, H, D, levels=4, fine=False, width=width)   # C0 with 32 or 16 channels
     net.init(5)
     net.bind(s, m.E)
+    monkeypatch.setenv("UFEM_NET_F32", "1")
     for maxit in (3, 11):
         monkeypatch.setenv("UFEM_NO_FUSED_C0T", "1")
         a = s.solve(solver, eps=1e-30, maxit=maxit, net=net, check=False)
@@ -388,3 +390,130 @@ def test_fused_last_layer_matches_the_network_stage(beam, solver, width, monkeyp
     r = s.solve(solver, eps=1e-8, net=net)
     assert r["rv"] == 0 and not r["fallback"] and r["true_relres"] <= 1e-8 * (1 + 1e-9)
     assert rel(s.u(), u_ref) <= 1e-6
+
+
+# ---- FP64 inference (the solver's path for coarse-first networks of 16 or 32
+# channels): the same operator with FP64 activations and sums
+
+
+def exact_gate_params(seed, levels, width):
+    """random bf16 weights scaled by 1/4 and every gate exactly 1 (a = -1,
+    b = 0: g = exp(0)), so the device's f32 gates equal the oracle's and the
+    FP64 path can be held to FP64 rounding"""
+    P = random_params(seed, levels, False, width)
+    for l in range(levels):
+        P[f"W{l}"] = P[f"W{l}"] * 0.25
+        P[f"a{l}"][:] = -1.0
+        P[f"b{l}"][:] = 0.0
+    return P
+
+
+def beam_ebar(m, levels):
+    e0 = U.node_log_modulus(m.E, m.nx, m.ny, m.nz).astype(np.float32).astype(np.float64)
+    return U.ebar_levels(e0, levels, fine=False)
+
+
+@pytest.mark.parametrize("levels,width", [(4, 16), (6, 16), (3, 32), (5, 32)])
+def test_fp64_inference_matches_oracle(beam, levels, width):
+    """N(v) by the FP64 kernels against the oracle's FP64 forward: 1e-13 with
+    exact gates; with general gates only the f32 gate values differ (1e-6)"""
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D, levels=levels, fine=False, width=width)
+    net.bind(s, m.E)
+    eb = beam_ebar(m, levels)
+    v = np.random.default_rng(levels + width).standard_normal((D, H, W, 3))
+    P = exact_gate_params(11, levels, width)
+    net.set_weights(pack(P))
+    y = net.apply64(v)
+    y_ref, _ = U.forward(P, v, eb)
+    print("exact gates", rel(y, y_ref))
+    assert rel(y, y_ref) <= 1e-13
+    P2 = random_params(12, levels, False, width)
+    net.set_weights(pack(P2))
+    y2 = net.apply64(v)
+    print("general gates", rel(y2, U.forward(P2, v, eb)[0]))
+    assert rel(y2, U.forward(P2, v, eb)[0]) <= 1e-6
+
+
+def cpu_fcg1(K, b, prec, eps, maxit):
+    """flexible CG with one kept direction (beta = -(z, q_old) / (p_old, q_old)),
+    the recurrences of the device solver, in FP64 on the CPU"""
+    x = np.zeros_like(b)
+    r = b.copy()
+    bn = np.linalg.norm(b)
+    p = q = None
+    it = 0
+    while it < maxit and np.linalg.norm(r) > eps * bn:
+        z = prec(r)
+        w = K @ z
+        if p is None:
+            p, q = z, w
+        else:
+            beta = -(z @ q) / (p @ q)
+            p, q = z + beta * p, w + beta * q
+        a = (p @ r) / (p @ q)
+        x, r = x + a * p, r - a * q
+        it += 1
+    return x, it
+
+
+def test_fp64_learned_solve_follows_cpu_fcg(beam):
+    """The learned solve with FP64 inference (fused last layer) and the same
+    flexible CG on the CPU with the oracle's preconditioner z = dinv r +
+    mask sqrt(dinv) N(mask sqrt(dinv) r): the same iteration count (+-1) and
+    solution; after 8 iterations the iterates agree to 1e-10, which a wrong
+    tap, channel or scale in any stage would break"""
+    m, s, ref, u_ref, (W, H, D) = beam
+    levels, width = 6, 16
+    net = G.GpuNet(W, H, D, levels=levels, fine=False, width=width)
+    net.bind(s, m.E)
+    P = exact_gate_params(13, levels, width)
+    net.set_weights(pack(P))
+    ip, cols, vals = s.K_csr()
+    K = sp.csr_matrix((vals, cols, ip), shape=(s.n, s.n))
+    b = s.F()
+    dinv = 1.0 / K.diagonal()
+    fixed = np.unique(3 * m.nl_node[m.nl_type == 1] + m.nl_dir[m.nl_type == 1] - 1)
+    mask = np.ones(s.n)
+    mask[fixed] = 0.0
+    eb = beam_ebar(m, levels)
+
+    def prec(r):
+        return U.precondition(P, r, dinv, mask, eb, (D, H, W))[0]
+
+    g8 = s.solve("learned", eps=1e-30, maxit=8, net=net, check=False)
+    x8, _ = cpu_fcg1(K, b, prec, 1e-30, 8)
+    assert g8["iters"] == 8 and not g8["fallback"]
+    print("8 iterations: GPU vs CPU", rel(s.u(), x8))
+    assert rel(s.u(), x8) <= 1e-10
+    r = s.solve("learned", eps=1e-10, rule=1, net=net)
+    xc, itc = cpu_fcg1(K, b, prec, 1e-10, 2000)
+    print("learned FP64", r["iters"], "CPU", itc)
+    assert r["rv"] == 0 and not r["fallback"]
+    assert abs(r["iters"] - itc) <= 1
+    assert rel(s.u(), xc) <= 1e-8
+
+
+def test_fp64_learned_solve_is_scale_invariant(beam):
+    """The network is linear: loads scaled by 1.1 change only rounding, and
+    with FP64 inference the iteration count stays (+-1; the matrix-core path
+    moves by up to a third on the 1M-DOF cantilever).  Not asked of the
+    mixed form: when its FP64 refreshes fall depends on rounding."""
+    solver = "learned"
+    m, s, ref, u_ref, (W, H, D) = beam
+    levels, width = 6, 16
+    net = G.GpuNet(W, H, D, levels=levels, fine=False, width=width)
+    net.init(7)
+    its = []
+    for scale in (1.0, 1.1, 0.9):
+        mk = cantilever(m.nx, m.ny, m.nz, L=2.0, B=0.5, H=0.5, E=m.E, total_load=-10000.0 * scale)
+        sk = G.GpuSystem(mk)
+        sk.assemble(2)
+        net.bind(sk, mk.E)
+        r = sk.solve(solver, eps=1e-8, net=net)
+        assert r["rv"] == 0 and not r["fallback"]
+        its.append(r["iters"])
+        net.bind(s, m.E)
+        sk.close()
+    print(solver, "iterations at load scales 1, 1.1, 0.9:", its)
+    assert max(its) - min(its) <= 1

@@ -110,6 +110,15 @@ int ufem_net_prepare(ufem_net *N, void *stream);   /* gates of the bound field  
 int ufem_net_infer(ufem_net *N, const int *d_skip, void *stream);
 int ufem_net_infer_head(ufem_net *N, const int *d_skip, void *stream);  /* all stages but the last */
 int ufem_net_last_stage(const ufem_net *N, ugk_conv_desc *d);          /* 1: *d is y = C0^T t0 (T2) */
+int ufem_net_f64_ok(const ufem_net *N);             /* the form has an FP64 inference path */
+double *ufem_net_input64(ufem_net *N);              /* FP64 input mask sqrt(dinv) r, [V][3] */
+int ufem_net_prepare64(ufem_net *N, void *stream);  /* buffers and FP64 weight copies */
+int ufem_net_infer64_head(ufem_net *N, const int *d_skip, void *stream);
+int ufem_net_last64(const ufem_net *N, ugk_conv_desc *d);  /* 1: *d is y = C0^T t0 from FP64 t0 */
+
+/* FP32 copy of K (ufem_sys.c): allocates d_vals32 (and, for 3x3 blocks,
+ * d_corr32) on first use and converts the current FP64 values */
+int ufem_mat_f32(ugk_mat *K, void *stream);
 
 /* error text (ufem_sys.c) */
 void ufem_set_error(const char *fmt, ...);

@@ -90,6 +90,9 @@ struct ufem_net {
   double *snap;
   long snap_cap, snap_n;
   int snap_every;
+  /* FP64 inference (the solver's path): activations [V][C], weight copies */
+  double *h64[NL_MAX], *t64[NL_MAX], *fd64[NL_MAX], *fu64[NL_MAX], *wu64;
+  double *v64, *y64;
 };
 
 static int r16(int c) { return (c + 15) / 16 * 16; }
@@ -169,6 +172,8 @@ void ufem_net_destroy(ufem_net *N)
   free(N->fvals); free(N->fdinv); free(N->feb);
   ugk_free(N->mask); ugk_free(N->yi); ugk_free(N->ebi); ugk_free(N->snap);
   ugk_free(N->x0s);
+  for (l = 0; l < NL_MAX; l++) { ugk_free(N->h64[l]); ugk_free(N->t64[l]); ugk_free(N->fd64[l]); ugk_free(N->fu64[l]); }
+  ugk_free(N->wu64); ugk_free(N->v64); ugk_free(N->y64);
   free(N->hpart); free(N->per); free(N->pfield); free(N->ebh);
   if (N->ev[0]) ugk_event_destroy(N->ev[0]);
   if (N->ev[1]) ugk_event_destroy(N->ev[1]);
@@ -756,6 +761,7 @@ static int add_field(ufem_net *N)
   }
   tmp = s->K;
   tmp.d_vals32 = N->fvals[f];
+  tmp.d_corr32 = NULL;    /* the training loss uses the plain FP32 copy */
   if (ugk_vals_to_f32(&tmp, N->stream) || ugk_to_f32(s->n, s->w.dinv, N->fdinv[f], N->stream) ||
       ugk_h2d(N->feb[f], N->ebh, sizeof(float) * (size_t)Vt, N->stream) || ugk_sync(N->stream))
     return net_fail("field store");
@@ -837,6 +843,7 @@ static int pair_loss(ufem_net *N, int b, long q, int nb, double *part_out)
   ugk_mat Kf = s->K;
   int G;
   Kf.d_vals32 = N->fvals[f];
+  Kf.d_corr32 = NULL;
   if (ugk_el_z(n, N->prr + q * n, N->fdinv[f], N->mask, N->y + b * N->Vf * 4, N->zz, N->stream) ||
       ugk_spmv_f32(&Kf, N->zz, N->kz, N->stream))
     return net_fail("loss");
@@ -991,6 +998,102 @@ int ufem_net_infer_head(ufem_net *N, const int *skip, void *stream)
   int k;
   for (k = 0; k < infer_stages(N) - 1; k++)
     if (infer_stage(N, k, N->gi, N->yi, skip, stream)) return net_fail("network inference");
+  return AF_OK;
+}
+
+/* ---- FP64 inference --------------------------------------------------------
+ * The solver applies the network with FP64 activations and sums (the same
+ * operator: bf16 weights widened exactly, f32 gates): flexible CG with one
+ * kept direction needs one linear map from iteration to iteration, and the
+ * matrix cores' f32 rounding noise costs it iterations.  The coarse-first
+ * form with 16 or 32 channels per level. */
+int ufem_net_f64_ok(const ufem_net *N)
+{
+  int l;
+  if (N->fine || N->ci[0] != 3) return 0;
+  for (l = 0; l < N->L; l++)
+    if (N->ch[l] != 16 && N->ch[l] != 32) return 0;
+  return 1;
+}
+
+double *ufem_net_input64(ufem_net *N)
+{
+  if (!N->v64) N->v64 = (double *)ugk_malloc(sizeof(double) * 3 * (size_t)N->Vf);
+  return N->v64;
+}
+
+/* buffers and the FP64 copies of the current weights */
+int ufem_net_prepare64(ufem_net *N, void *stream)
+{
+  int l, bad = 0;
+  if (!ufem_net_f64_ok(N)) {
+    ufem_set_error("FP64 inference: the coarse-first form with 16 or 32 channels per level only");
+    return AF_ERR_VAL;
+  }
+  if (!ufem_net_input64(N)) return net_fail("FP64 input");
+  for (l = 0; l < N->L; l++) {
+    if (!N->h64[l]) N->h64[l] = (double *)dalloc(sizeof(double) * (size_t)N->V[l] * N->ch[l], &bad);
+    if (!N->t64[l]) N->t64[l] = (double *)dalloc(sizeof(double) * (size_t)N->V[l] * N->ch[l], &bad);
+    if (!N->fd64[l]) N->fd64[l] = (double *)dalloc(sizeof(double) * (size_t)ugk_n64_frag_len(N->ci[l], N->ch[l]), &bad);
+    if (l && !N->fu64[l]) N->fu64[l] = (double *)dalloc(sizeof(double) * (size_t)ugk_n64_frag_len(N->ch[l], N->ci[l]), &bad);
+  }
+  if (!N->wu64) N->wu64 = (double *)dalloc(sizeof(double) * 27 * (size_t)N->ch[0] * N->ci[0], &bad);
+  if (bad) return net_fail("FP64 activations");
+  for (l = 0; l < N->L; l++) {
+    const int cop = r16(N->ch[l]), cip = r16(N->ci[l]);
+    if (ugk_w64_frag(N->wf[l], cop, cip, N->ch[l], N->ci[l], 0, N->fd64[l], stream) ||
+        (l && ugk_w64_frag(N->wf[l], cop, cip, N->ch[l], N->ci[l], 1, N->fu64[l], stream)))
+      return net_fail("FP64 weights");
+  }
+  if (ugk_w64(N->wf[0], r16(N->ch[0]), r16(N->ci[0]), N->ch[0], N->ci[0], N->wu64, stream)) return net_fail("FP64 weights");
+  return AF_OK;
+}
+
+/* all stages but the last from the input v64 (mask sqrt(dinv) r, [V][3]):
+ * h0 = C0 v, h_l = C_l h_{l-1}, t_{L-1} = g_{L-1} h_{L-1},
+ * t_l = g_l h_l + C_{l+1}^T t_{l+1} */
+int ufem_net_infer64_head(ufem_net *N, const int *skip, void *stream)
+{
+  const int L = N->L;
+  int l;
+  for (l = 0; l < L; l++) {
+    const double *x = l ? N->h64[l - 1] : N->v64;
+    const int cin = l ? N->ch[l - 1] : 3;
+    const ugk_grid3 gi = l ? N->g[l - 1] : N->gf;
+    if (ugk_n64_down(x, cin, cin, gi, N->g[l], N->fd64[l], N->ch[l], l == L - 1 ? N->gi[l] : NULL,
+                     l == L - 1 ? N->t64[l] : N->h64[l], skip, stream))
+      return net_fail("FP64 inference");
+  }
+  for (l = L - 2; l >= 0; l--)
+    if (ugk_n64_up(N->t64[l + 1], N->ch[l + 1], N->g[l + 1], N->g[l], N->fu64[l + 1], N->ch[l], N->gi[l], N->h64[l],
+                   N->t64[l], skip, stream))
+      return net_fail("FP64 inference");
+  return AF_OK;
+}
+
+/* the last stage for the fused combine: y = C0^T t0 from the FP64 t0 */
+int ufem_net_last64(const ufem_net *N, ugk_conv_desc *d)
+{
+  if (!ufem_net_last_stage(N, d)) return 0;
+  d->x = N->t64[0];
+  d->x_f32 = UGK_ACT_F64;
+  return 1;
+}
+
+/* y = N(v) with FP64 inference for the bound field: v, y [D][H][W][3] (host) */
+int ufem_net_apply64(ufem_net *N, const double *v, double *y)
+{
+  const long n = 3 * N->Vf;
+  int rv;
+  if (!N->sys) { ufem_set_error("ufem_net_apply64: no bound system"); return AF_ERR_VAL; }
+  if ((rv = ufem_net_prepare64(N, N->stream)) != AF_OK) return rv;
+  if (!N->y64) N->y64 = (double *)ugk_malloc(sizeof(double) * (size_t)n);
+  if (!N->y64) return net_fail("FP64 output");
+  if (ugk_h2d(N->v64, v, sizeof(double) * (size_t)n, N->stream) || ufem_net_prepare(N, N->stream) ||
+      ufem_net_infer64_head(N, NULL, N->stream) ||
+      ugk_n64_out(N->t64[0], N->ch[0], N->g[0], N->gf, N->wu64, N->y64, N->stream) ||
+      ugk_d2h(y, N->y64, sizeof(double) * (size_t)n, N->stream) || ugk_sync(N->stream))
+    return net_fail("FP64 inference");
   return AF_OK;
 }
 

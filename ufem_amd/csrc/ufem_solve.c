@@ -331,11 +331,7 @@ static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_wor
   ugk_state *h = w->st_host;
   mx_ctx c = {K, n, b, x, w};
   int rv;
-  if (!Km->d_vals32) {
-    Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
-    if (!Km->d_vals32) return gpu_fail("fp32 matrix");
-  }
-  if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+  if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
   rv = prepare(K, w, stream, o->eps, 1, maxit);
   if (rv != AF_OK) return rv;
   if (ugk_to_f32(n, w->dinv, w->dinv32, stream)) return gpu_fail("fp32 diagonal");
@@ -384,6 +380,7 @@ typedef struct {
   int m;
   int mixed;
   int fused;                 /* m = 1, coarse-first network: last layer in the combine */
+  int f64;                   /* fused, with FP64 inference */
   ugk_conv_desc last;
 } fcg_ctx;
 
@@ -412,12 +409,16 @@ static int fcg1_step(void *c, long it, void *stream)
 {
   fcg_ctx *f = (fcg_ctx *)c;
   ufem_work *w = f->w;
-  unsigned short *x0 = f->net ? ufem_net_input(f->net) : NULL;
+  unsigned short *x0 = f->net && !f->f64 ? ufem_net_input(f->net) : NULL;
+  double *v64 = f->f64 ? ufem_net_input64(f->net) : NULL;
+  const float *mk = f->net ? ufem_net_mask(f->net) : NULL;
   (void)it;
-  if (f->mixed && ugk_f1_refresh(f->K, f->n, f->b, f->x, w->r, w->dinv, w->fP, w->fQ, x0, w->st, w->part, stream))
+  if (f->mixed &&
+      ugk_f1_refresh(f->K, f->n, f->b, f->x, w->r, w->dinv, w->fP, w->fQ, x0, v64, mk, w->st, w->part, stream))
     return -1;
   if (f->fused) {                       /* the network's last layer inside the combine */
-    if (ufem_net_infer_head(f->net, &w->st->halt, stream) ||
+    if ((f->f64 ? ufem_net_infer64_head(f->net, &w->st->halt, stream)
+                : ufem_net_infer_head(f->net, &w->st->halt, stream)) ||
         (f->cc && ugk_cc_apply(f->cc, f->n, w->r, f->cc->d_zc, w->st, stream)) ||
         ugk_f1_combine_c0t(f->n, w->r, w->dinv, ufem_net_mask(f->net), &f->last, f->cc ? f->cc->d_zc : NULL, w->fQ,
                            w->z, w->st, w->part, stream))
@@ -429,7 +430,8 @@ static int fcg1_step(void *c, long it, void *stream)
                             w->part, stream))
     return -1;
   if (ugk_f1_w(f->K, w->z, w->q, f->mixed, w->st, w->part, stream)) return -1;
-  return ugk_f1_update(f->K, f->n, f->x, w->r, w->fP, w->fQ, w->z, w->q, w->dinv, x0, w->st, w->part, stream);
+  return ugk_f1_update(f->K, f->n, f->x, w->r, w->fP, w->fQ, w->z, w->q, w->dinv, x0, v64, mk, w->st, w->part,
+                       stream);
 }
 
 static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
@@ -439,7 +441,7 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   ufem_net *N = (ufem_net *)o->net;
   const int m = o->fcg_m > 0 ? o->fcg_m : 1;   /* the network is a fixed SPD map */
   const ugk_coarse *cc = o->coarse ? w->cc : NULL;
-  fcg_ctx c = {K, n, b, x, w, N, cc, m, mixed, 0, {0}};
+  fcg_ctx c = {K, n, b, x, w, N, cc, m, mixed, 0, 0, {0}};
   ugk_state *h = w->st_host;
   int rv;
   if (o->coarse && !w->cc) {
@@ -460,11 +462,7 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
     if (!w->fm) return gpu_fail("direction rings");
   }
   if (mixed) {
-    if (!Km->d_vals32) {
-      Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
-      if (!Km->d_vals32) return gpu_fail("fp32 matrix");
-    }
-    if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+    if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
   }
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
@@ -480,8 +478,13 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   if (cc && ufem_coarse_build(w->cc, stream) != AF_OK) return AF_ERR;
   if (m == 1) {
     c.fused = N && !getenv("UFEM_NO_FUSED_C0T") && ufem_net_last_stage(N, &c.last);
+    /* FP64 inference unless asked for the matrix-core path (UFEM_NET_F32) */
+    c.f64 = c.fused && !getenv("UFEM_NET_F32") && ufem_net_f64_ok(N);
+    if (c.f64 && ((rv = ufem_net_prepare64(N, stream)) != AF_OK || !ufem_net_last64(N, &c.last)))
+      return rv != AF_OK ? rv : AF_ERR;
     if (ugk_f1_start(K, n, b, x, w->r, w->dinv, w->st, w->part, stream) ||
-        (N && ugk_f1_input0(n, w->r, w->dinv, ufem_net_input(N), w->st, w->part, stream)))
+        (N && ugk_f1_input0(n, w->r, w->dinv, c.f64 ? NULL : ufem_net_input(N), c.f64 ? ufem_net_input64(N) : NULL,
+                            ufem_net_mask(N), w->st, w->part, stream)))
       return gpu_fail("start");
     rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 8, fcg1_step, &c, 1);
   } else {
@@ -606,11 +609,7 @@ static int solve_twolevel(ugk_mat *Km, long n, const double *b, double *x, ufem_
   int rv;
   if (!w->cc) { ufem_set_error("two-level solver: no coarse grid (ufem_sys_set_coarse)"); return AF_ERR_VAL; }
   if (mixed) {
-    if (!Km->d_vals32) {
-      Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
-      if (!Km->d_vals32) return gpu_fail("fp32 matrix");
-    }
-    if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+    if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
   }
   if ((rv = ufem_coarse_build(w->cc, stream)) != AF_OK) return rv;
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
@@ -668,11 +667,7 @@ static int solve_mixed64(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   mx64_ctx c = {K, n, b, x, w};
   ugk_state *h = w->st_host;
   int rv;
-  if (!Km->d_vals32) {
-    Km->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)Km->total_slots * 64 * Km->bs * Km->bs);
-    if (!Km->d_vals32) return gpu_fail("fp32 matrix");
-  }
-  if (ugk_vals_to_f32(Km, stream)) return gpu_fail("fp32 matrix");
+  if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
   h->delta = o->inner_eps > 0 ? o->inner_eps : 0.1;

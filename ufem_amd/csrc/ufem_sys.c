@@ -242,6 +242,24 @@ out:
 long ufem_sys_ndof(const ufem_sys *s) { return s->n; }
 long ufem_sys_nnz(const ufem_sys *s) { return s->K.nnzb * 9; }
 long ufem_sys_ncolors(const ufem_sys *s) { return s->n_colors; }
+int ufem_mat_f32(ugk_mat *K, void *stream)
+{
+  if (!K->d_vals32) {
+    K->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)K->total_slots * 64 * K->bs * K->bs);
+    if (!K->d_vals32) return gpu_fail("fp32 matrix");
+  }
+  if (K->d_corr32 && getenv("UFEM_NO_CORR32")) {   /* plain FP32 copy, for comparisons */
+    ugk_free(K->d_corr32);
+    K->d_corr32 = NULL;
+  }
+  if (K->bs == 3 && !K->d_corr32 && !getenv("UFEM_NO_CORR32")) {
+    K->d_corr32 = (float *)ugk_malloc(sizeof(float) * 9 * (size_t)K->n_brows);
+    if (!K->d_corr32) return gpu_fail("fp32 matrix correction");
+  }
+  if (ugk_vals_to_f32(K, stream)) return gpu_fail("fp32 matrix");
+  return AF_OK;
+}
+
 long ufem_sys_nreact(const ufem_sys *s) { return s->n_react; }
 
 void ufem_sys_destroy(ufem_sys *s)
@@ -250,7 +268,7 @@ void ufem_sys_destroy(ufem_sys *s)
   if (!s) return;
   if (s->stream) ugk_sync(s->stream);
   ugk_free(s->K.d_slen); ugk_free(s->K.d_soff); ugk_free(s->K.d_rowlen);
-  ugk_free(s->K.d_cols); ugk_free(s->K.d_vals); ugk_free(s->K.d_vals32);
+  ugk_free(s->K.d_cols); ugk_free(s->K.d_vals); ugk_free(s->K.d_vals32); ugk_free(s->K.d_corr32);
   ugk_free(s->d_x); ugk_free(s->d_y); ugk_free(s->d_z); ugk_free(s->d_E); ugk_free(s->d_nu);
   ugk_free(s->d_enodes); ugk_free(s->d_elist); ugk_free(s->d_fixed);
   ugk_free(s->d_saved_rows); ugk_free(s->d_saved_cols); ugk_free(s->d_Rfix);
@@ -865,6 +883,17 @@ int ufem_sys_spmv(ufem_sys *s, const double *x, double *y)
   return AF_OK;
 }
 
+int ufem_sys_spmv_f32(ufem_sys *s, const double *x, double *y)
+{
+  int rv = ufem_mat_f32(&s->K, s->stream);
+  if (rv != AF_OK) return rv;
+  if (ugk_h2d(s->w.z, x, sizeof(double) * (size_t)s->n, s->stream) ||
+      ugk_spmv_f32x64(&s->K, s->w.z, s->w.q, s->stream) ||
+      ugk_d2h(y, s->w.q, sizeof(double) * (size_t)s->n, s->stream))
+    return gpu_fail("spmv (fp32 copy)");
+  return AF_OK;
+}
+
 /* Algorithmic bytes of one launch, counted on the real (unpadded) blocks:
  *   mode 0  y = K x, FP64:      (72 + 4) nnzb + 8 n (x) + 8 n (y)
  *   mode 1  y = K x, FP32:      (36 + 4) nnzb + 4 n + 4 n
@@ -904,8 +933,8 @@ int ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *byte
     return AF_ERR_VAL;
   }
   if (mode == 1 && !s->K.d_vals32) {
-    s->K.d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)s->K.total_slots * 64 * 9);
-    if (!s->K.d_vals32 || ugk_vals_to_f32(&s->K, s->stream)) return gpu_fail("fp32 copy");
+    int rv = ufem_mat_f32(&s->K, s->stream);
+    if (rv != AF_OK) return rv;
   }
   if (mode >= 2) {   /* a running state (after a solve) so the kernel does its work */
     ugk_state h;

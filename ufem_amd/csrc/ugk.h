@@ -33,6 +33,9 @@ typedef struct {
   int *d_cols;        /* [total_slots * 64]                             */
   double *d_vals;     /* [total_slots * 64 * b*b]                       */
   float *d_vals32;    /* FP32 copy for mixed precision (may be NULL)    */
+  float *d_corr32;    /* b = 3: [9][n_brows] symmetric part of the block row
+                         sums of (vals - vals32), added to the FP32 copy's
+                         diagonal blocks (may be NULL)                  */
   unsigned long gen;  /* bumped whenever the pattern is (re)uploaded     */
 } ugk_mat;
 
@@ -173,6 +176,8 @@ int ugk_spmv(const ugk_mat *K, const double *d_x, double *d_y, void *stream);
  * one partial per wave into d_out */
 int ugk_stream_read(const ugk_mat *K, int nontemporal, double *d_out, void *stream);
 int ugk_spmv_f32(const ugk_mat *K, const float *d_x, float *d_y, void *stream);
+/* y = K x with the FP32 copy of K (and its diagonal correction), FP64 vectors */
+int ugk_spmv_f32x64(const ugk_mat *K, const double *d_x, double *d_y, void *stream);
 /* dinv[i] = 1 / K_ii; st->err = 2 if |K_ii| < 1e-17 */
 int ugk_diag_inv(const ugk_mat *K, double *d_dinv, ugk_state *d_st, void *stream);
 /* max row 2-norm into st->normA */
@@ -336,29 +341,33 @@ struct ugk_conv_desc_s;
  * A direction that does not descend (or is not finite) is not taken: that
  * iteration moves nothing and the next one starts afresh with z = dinv r,
  * the network skipped from there on (st->fb, st->halt).  ugk_f1_start forms r = b - A x and its
- * partials; ugk_f1_input0 writes the first network input.  A refresh keeps
+ * partials; ugk_f1_input0 writes the first network input.  The network input
+ * (written by input0, refresh and update) is d_x0, split f32 of
+ * sqrt(dinv) r / s, or, when d_v64 is given, FP64 mask sqrt(dinv) r
+ * (unscaled: the FP64 inference path).  A refresh keeps
  * the direction (st->restart = 0) or starts it afresh (1). */
 int ugk_f1_start(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r, const double *d_dinv,
                  ugk_state *d_st, double *d_part, void *stream);
-int ugk_f1_input0(long n, const double *d_r, const double *d_dinv, unsigned short *d_x0, ugk_state *d_st,
-                  double *d_part, void *stream);
+int ugk_f1_input0(long n, const double *d_r, const double *d_dinv, unsigned short *d_x0, double *d_v64,
+                  const float *d_mask, ugk_state *d_st, double *d_part, void *stream);
 int ugk_f1_refresh(const ugk_mat *K, long n, const double *d_b, const double *d_x, double *d_r, const double *d_dinv,
-                   const double *d_p, const double *d_q, unsigned short *d_x0, ugk_state *d_st, double *d_part,
-                   void *stream);
+                   const double *d_p, const double *d_q, unsigned short *d_x0, double *d_v64, const float *d_mask,
+                   ugk_state *d_st, double *d_part, void *stream);
 int ugk_f1_combine(long n, const double *d_r, const double *d_dinv, const float *d_mask, const float *d_out,
                    const double *d_zc, const double *d_q, double *d_z, ugk_state *d_st, double *d_part, void *stream);
 int ugk_f1_w(const ugk_mat *K, const double *d_z, double *d_w, int f32, const ugk_state *d_st, double *d_part,
              void *stream);
 /* ugk_f1_combine with the network's last layer folded in: y = C0^T t0, the
- * stride-2 transpose (UGK_CONV_T2) described by *cd (f32 input of at most
- * 32 channels, 3 outputs, one sample), on the vector ALUs in f32; the
- * network then runs all stages but that one */
+ * stride-2 transpose (UGK_CONV_T2) described by *cd (16 or 32 input
+ * channels, 3 outputs, one sample) on the vector ALUs: f32 for an f32 input
+ * (scaled by st->net_s), FP64 for UGK_ACT_F64 (unscaled); the network then
+ * runs all stages but that one */
 int ugk_f1_combine_c0t(long n, const double *d_r, const double *d_dinv, const float *d_mask,
                        const struct ugk_conv_desc_s *cd, const double *d_zc, const double *d_q, double *d_z,
                        ugk_state *d_st, double *d_part, void *stream);
 int ugk_f1_update(const ugk_mat *K, long n, double *d_x, double *d_r, double *d_p, double *d_q, const double *d_z,
-                  const double *d_w, const double *d_dinv, unsigned short *d_x0, ugk_state *d_st, double *d_part,
-                  void *stream);
+                  const double *d_w, const double *d_dinv, unsigned short *d_x0, double *d_v64, const float *d_mask,
+                  ugk_state *d_st, double *d_part, void *stream);
 
 /* ---------------- CG with the SSOR preconditioner ------------------------ */
 /* Kc: A with block rows (and columns) in colour order, new row I' = old row
@@ -475,7 +484,8 @@ enum { UGK_CONV_S1 = 0,   /* y[v] = sum_t W[t] x[v + off_t]                 */
  * stored as hi[4], mid[4], lo[4] (24 bytes; f32 = (hi + mid) + lo exactly),
  * so the element (v, c) of a tensor of stride ld starts at bf16 offset
  * 3 (v ld + c) for c % 4 == 0. */
-enum { UGK_ACT_BF16 = 0, UGK_ACT_F32 = 1, UGK_ACT_SPLIT = 2 };
+enum { UGK_ACT_BF16 = 0, UGK_ACT_F32 = 1, UGK_ACT_SPLIT = 2,
+       UGK_ACT_F64 = 3 };  /* FP64 (the FP64 inference path only) */
 
 /* 3x3x3 taps t = 9 (dz+1) + 3 (dy+1) + (dx+1), zero padding, no bias.
  * Activations x are bf16, f32 (fed to the bf16 matrix cores as an exact
@@ -495,6 +505,29 @@ typedef struct ugk_conv_desc_s {
   const int *skip;                         /* optional: no-op while *skip != 0 */
 } ugk_conv_desc;
 int ugk_conv(const ugk_conv_desc *d, void *stream);
+
+/* FP64 inference of the network: the same operator (bf16 weights widened
+ * exactly, f32 gates) with FP64 activations [V][C] and FP64 sums; the 16-
+ * and 32-channel stages on the FP64 matrix cores.
+ *   ugk_w64_frag  the matrix-core fragments of one layer (bf16 [27][cop][cip],
+ *                 co outputs, ci inputs): up = 0 for the convolution, 1 for its
+ *                 adjoint; ugk_n64_frag_len(nin, nout) doubles
+ *   ugk_w64       wu [27][co][ci] FP64 (for ugk_n64_out)
+ *   ugk_n64_down  y = (gate *) conv_s2(x): x [Vi][ldx] on gi (ci = 3, 16, 32),
+ *                 y [Vo][co] on go = coarse(gi) (co = 16, 32)
+ *   ugk_n64_up    y = gate h + conv_s2^T(x): x [Vc][ci] on gc (ci = 16, 32),
+ *                 y, h [Vf][co] on gf with gc = coarse(gf) (co = 16, 32),
+ *                 the adjoint fragments of the convolution gf -> gc
+ *   ugk_n64_out   y = conv_s2^T(x) to 3 channels (ci = 16, 32), wu of that layer
+ * (d_skip optional: no-op while *d_skip != 0) */
+long ugk_n64_frag_len(int nin, int nout);
+int ugk_w64_frag(const unsigned short *d_wf, int cop, int cip, int co, int ci, int up, double *d_frag, void *stream);
+int ugk_w64(const unsigned short *d_wf, int cop, int cip, int co, int ci, double *d_wu, void *stream);
+int ugk_n64_down(const double *d_x, int ldx, int ci, ugk_grid3 gi, ugk_grid3 go, const double *d_frag, int co,
+                 const float *d_gate, double *d_y, const int *d_skip, void *stream);
+int ugk_n64_up(const double *d_x, int ci, ugk_grid3 gc, ugk_grid3 gf, const double *d_frag, int co,
+               const float *d_gate, const double *d_h, double *d_y, const int *d_skip, void *stream);
+int ugk_n64_out(const double *d_x, int ci, ugk_grid3 gc, ugk_grid3 gf, const double *d_wu, double *d_y, void *stream);
 
 /* weight gradient of y = conv(x) (mode S1 or S2), x and dy bf16 or f32
  * (rounded to bf16 for the matrix cores): block partials over ks voxel

@@ -306,6 +306,7 @@ struct SellView {
   const int *rowlen;
   const int *cols;
   long n_brows, n_slices;
+  const float *corr;  // FP32 copy's diagonal correction [9][n_brows], or null
 };
 
 __device__ __forceinline__ int find_slot(const SellView &M, long I, long J)
@@ -743,12 +744,27 @@ __device__ __forceinline__ bool sell_row_split(const SellView &M, const T *__res
         for (int c = 0; c < BS; c++) acc[r] += (TX)v[r * BS + c] * xv[c];
     }
   }
+  row = sl * 64 + lane;
   if (half) {
+    if constexpr (BS == 3 && sizeof(T) == 4) {
+      // FP32 copy: add C_I x_I (C_I: the block row's summed rounding errors,
+      // symmetrised), which keeps rigid translations, where a stiffness
+      // matrix's rows cancel, cancelled.  The second-half wave has no more
+      // slots than the first, so it takes this term.
+      if (M.corr && sl < M.n_slices && row < M.n_brows) {
+        TX xi[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) xi[c] = x[row * 3 + c];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[r] += (TX)M.corr[(r * 3 + c) * M.n_brows + row] * xi[c];
+      }
+    }
 #pragma unroll
     for (int r = 0; r < BS; r++) sh[pair][r][lane] = acc[r];
   }
   __syncthreads();
-  row = sl * 64 + lane;
   if (half || sl >= M.n_slices) return false;
 #pragma unroll
   for (int r = 0; r < BS; r++) acc[r] += sh[pair][r][lane];
@@ -804,6 +820,34 @@ __global__ void k_vals_f32(long n, const double *__restrict__ a, T *__restrict__
 {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     b[i] = (T)a[i];
+}
+
+// Diagonal correction of the FP32 copy of a 3x3-block matrix: for block row I,
+// C = sum over its blocks J of (K_IJ - fl32(K_IJ)), kept as (C + C^T) / 2 so
+// the corrected matrix stays symmetric.  A stiffness matrix's block rows sum
+// to (nearly) zero: fl32 alone breaks that cancellation by about 1e-7 |K| |x|
+// on smooth fields, which stalls the true residual near 1e-3; with C the error
+// on a smooth field falls with its differences x_J - x_I instead.
+__global__ void k_corr32(SellView M, const double *__restrict__ vals, float *__restrict__ corr)
+{
+  const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= M.n_brows) return;
+  const long so = M.soff[I >> 6];
+  const int ln = (int)(I & 63), L = M.rowlen[I];
+  double c[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) c[q] = 0.0;
+  for (int s = 0; s < L; s++) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const double v = vals[vpos<3>(so + s, q, ln)];
+      c[q] += v - (double)(float)v;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) corr[(r * 3 + q) * M.n_brows + I] = (float)(0.5 * (c[r * 3 + q] + c[q * 3 + r]));
 }
 
 // dst[map[i]] = src[i]: host-ordered values into their device slots
@@ -2266,6 +2310,15 @@ __device__ __forceinline__ void f1_put_input(unsigned short *x0, long i, float f
   o[8] = (unsigned short)(__float_as_uint(r2) >> 16);
 }
 
+// the network input of DOF i from sdr = sqrt(dinv) r: split f32 scaled by
+// 1/s (the matrix-core path), or FP64, unscaled and masked (the FP64 path)
+__device__ __forceinline__ void f1_input(unsigned short *x0, double *v64, const float *mask, long i, double sdr,
+                                         double inv)
+{
+  if (v64) v64[i] = (double)mask[i] * sdr;
+  else f1_put_input(x0, i, (float)(sdr * inv));
+}
+
 template <int BS>
 __global__ __launch_bounds__(WG) void k_f1_start(SellView M, const double *__restrict__ vals,
                                                  const double *__restrict__ b, const double *__restrict__ x,
@@ -2294,14 +2347,15 @@ __global__ __launch_bounds__(WG) void k_f1_start(SellView M, const double *__res
 // the first network input, from r_0 and the start's partials
 __global__ __launch_bounds__(WG) void k_f1_input0(long n, const double *__restrict__ r,
                                                   const double *__restrict__ dinv, unsigned short *x0,
-                                                  ugk_state *st, const double *part)
+                                                  double *v64, const float *__restrict__ mask, ugk_state *st,
+                                                  const double *part)
 {
   double t[1];
   reduce_prev<1>(F1_H(part), (unsigned)st->hgrid, t);
   const double s = f1_scale(t[0], n), inv = 1.0 / s;
   long i1, i2;
   xcd_chunk(n, i1, i2);
-  for (long i = i1 + threadIdx.x; i < i2; i += WG) f1_put_input(x0, i, (float)(sqrt(dinv[i]) * r[i] * inv));
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) f1_input(x0, v64, mask, i, sqrt(dinv[i]) * r[i], inv);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->net_s = s;
 }
 
@@ -2313,11 +2367,12 @@ __global__ __launch_bounds__(WG) void k_f1_refresh(SellView M, const double *__r
                                                    const double *__restrict__ b, const double *__restrict__ x,
                                                    double *r, const double *__restrict__ dinv,
                                                    const double *__restrict__ p, const double *__restrict__ q,
-                                                   unsigned short *x0, ugk_state *st, double *part)
+                                                   unsigned short *x0, double *v64, const float *__restrict__ mask,
+                                                   ugk_state *st, double *part)
 {
   const long k = st->iter;
   if (st->done || st->ru_req != k) return;   // the same for every block (the lead writes neither)
-  const bool in = x0 && !st->fb;
+  const bool in = (x0 || v64) && !st->fb;
   const double inv = 1.0 / st->net_s;
   double v[6] = {0, 0, 0, 0, 0, 0};
   double acc[BS];
@@ -2333,7 +2388,7 @@ __global__ __launch_bounds__(WG) void k_f1_refresh(SellView M, const double *__r
       v[2] += x[i] * x[i];
       v[4] += p[i] * ri;
       v[5] += p[i] * q[i];
-      if (in) f1_put_input(x0, i, (float)(sqrt(dinv[i]) * ri * inv));
+      if (in) f1_input(x0, v64, mask, i, sqrt(dinv[i]) * ri, inv);
     }
   }
   block_partials<6>(v, F1_H(part));
@@ -2467,7 +2522,42 @@ __device__ __forceinline__ void c0t_class(const float *__restrict__ x, int ldx, 
   }
 }
 
-template <int CIN>
+// the same for FP64 activations (the FP64 inference path): CQ doubles per lane
+template <int PZ, int PY, int PX, int CIN>
+__device__ __forceinline__ void c0t_class64(const double *__restrict__ x, int ldx, const double *sw, ugk_grid3 gc,
+                                            int cz, int cy, int cx, bool fok, int qd, double (&y)[3])
+{
+  constexpr int NZ = PZ + 1, NY = PY + 1, NX = PX + 1, NT = NZ * NY * NX, CQ = CIN / 4, CH = CQ / 2;
+  dbl2 xa[NT][CH];
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    const int iz = j / (NY * NX), iy = (j / NX) % NY, ix = j % NX;
+    const int sz = cz + iz, sy = cy + iy, sx = cx + ix;   // offset -1 reads c + 1
+    const bool ok = fok && sz < gc.D && sy < gc.H && sx < gc.W;
+#pragma unroll
+    for (int k = 0; k < CH; k++) xa[j][k] = dbl2{0.0, 0.0};
+    if (ok) {
+      const double *xs = x + (((long)sz * gc.H + sy) * gc.W + sx) * ldx + CQ * qd;
+#pragma unroll
+      for (int k = 0; k < CH; k++) xa[j][k] = *(const dbl2 *)(xs + 2 * k);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    const int iz = j / (NY * NX), iy = (j / NX) % NY, ix = j % NX;
+    const int tz = PZ ? 2 - 2 * iz : 1, ty = PY ? 2 - 2 * iy : 1, tx = PX ? 2 - 2 * ix : 1;
+    const double *w = sw + (9 * tz + 3 * ty + tx) * 3 * CIN + CQ * qd;
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < CH; k++) acc += w[o * CIN + 2 * k] * xa[j][k].x + w[o * CIN + 2 * k + 1] * xa[j][k].y;
+      y[o] += acc;
+    }
+  }
+}
+
+template <int CIN, typename TA>
 __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double *__restrict__ r,
                                                            const double *__restrict__ dinv,
                                                            const float *__restrict__ mask, ugk_conv_desc cd,
@@ -2475,7 +2565,7 @@ __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double 
                                                            const double *__restrict__ q, double *__restrict__ z,
                                                            ugk_state *st, double *part, int npb)
 {
-  __shared__ float sw[27 * 3 * C0T_CIN];
+  __shared__ TA sw[27 * 3 * C0T_CIN];
   if (st->done) return;
   const long k = st->iter;
   double t[6];
@@ -2515,13 +2605,13 @@ __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double 
     const unsigned short *w = cd.w;
     for (int i = threadIdx.x; i < 27 * 3 * CIN; i += C0T_TB) {
       const int tp = i / (3 * CIN), o = (i / CIN) % 3, c = i % CIN;
-      sw[i] = __uint_as_float((unsigned)w[((long)tp * cd.cop + o) * cd.cip + c] << 16);
+      sw[i] = (TA)__uint_as_float((unsigned)w[((long)tp * cd.cop + o) * cd.cip + c] << 16);
     }
   }
   __syncthreads();
   const bool useq = !(k == 0 || k == st->restart_at);
-  const double sc = st->net_s;
-  const float *x = (const float *)cd.x;
+  const double sc = sizeof(TA) == 8 ? 1.0 : st->net_s;   // FP64 input: unscaled
+  const TA *x = (const TA *)cd.x;
   const ugk_grid3 gc = cd.gi, gf = cd.go;
   const int lane = threadIdx.x & 63, cls = threadIdx.x >> 6, qd = lane & 3;
   const int pz = cls >> 2, py = (cls >> 1) & 1, px = cls & 1;
@@ -2544,17 +2634,30 @@ __global__ __launch_bounds__(C0T_TB) void k_f1_combine_c0t(long n, const double 
       if (useq) qi = q[i];
       if (zc) zci = zc[i];
     }
-    float y[3] = {0.f, 0.f, 0.f};
+    TA y[3] = {0, 0, 0};
     if (net) {
-      switch (cls) {                        // wave-uniform
-        case 0: c0t_class<0, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 1: c0t_class<0, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 2: c0t_class<0, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 3: c0t_class<0, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 4: c0t_class<1, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 5: c0t_class<1, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        case 6: c0t_class<1, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
-        default: c0t_class<1, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+      if constexpr (sizeof(TA) == 8) {
+        switch (cls) {                        // wave-uniform
+          case 0: c0t_class64<0, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 1: c0t_class64<0, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 2: c0t_class64<0, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 3: c0t_class64<0, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 4: c0t_class64<1, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 5: c0t_class64<1, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 6: c0t_class64<1, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          default: c0t_class64<1, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        }
+      } else {
+        switch (cls) {                        // wave-uniform
+          case 0: c0t_class<0, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 1: c0t_class<0, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 2: c0t_class<0, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 3: c0t_class<0, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 4: c0t_class<1, 0, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 5: c0t_class<1, 0, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          case 6: c0t_class<1, 1, 0, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+          default: c0t_class<1, 1, 1, CIN>(x, cd.ldx, sw, gc, cz, cy, cx, fok, qd, y); break;
+        }
       }
 #pragma unroll
       for (int o = 0; o < 3; o++) {
@@ -2599,6 +2702,7 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
                                                   double *__restrict__ p, double *__restrict__ q,
                                                   const double *__restrict__ z, const double *__restrict__ w,
                                                   const double *__restrict__ dinv, unsigned short *x0,
+                                                  double *v64, const float *__restrict__ mask,
                                                   ugk_state *st, double *part, unsigned GW)
 {
   if (st->done) return;
@@ -2634,7 +2738,7 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
     return;
   }
   const double alpha = pr / pq;
-  const bool in = x0 && !st->fb;
+  const bool in = (x0 || v64) && !st->fb;
   const double s = f1_scale(st->rdr, n), inv = 1.0 / s;
   double v[6] = {0, 0, 0, 0, 0, 0};
   long i1, i2;
@@ -2669,7 +2773,7 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
       v[2] += xi * xi;
       v[4] += pi * ri;
       v[5] += pi * qi;
-      if (in) f1_put_input(x0, j, (float)(sqrt(di) * ri * inv));
+      if (in) f1_input(x0, v64, mask, j, sqrt(di) * ri, inv);
     }
   }
   for (; i < i2; i += WG) {
@@ -2687,7 +2791,7 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
     v[2] += xi * xi;
     v[4] += pi * ri;
     v[5] += pi * qi;
-    if (in) f1_put_input(x0, i, (float)(sqrt(di) * ri * inv));
+    if (in) f1_input(x0, v64, mask, i, sqrt(di) * ri, inv);
   }
   block_partials<6>(v, F1_H(part));
   if (lead) {
@@ -3193,6 +3297,7 @@ static SellView view(const ugk_mat *K)
   M.cols = K->d_cols;
   M.n_brows = K->n_brows;
   M.n_slices = K->n_slices;
+  M.corr = K->bs == 3 ? K->d_corr32 : nullptr;
   return M;
 }
 // --------------------------------------------------------------------------
@@ -3877,6 +3982,10 @@ int ugk_vals_to_f32(const ugk_mat *K, void *s)
   if (!K->d_vals32) { snprintf(g_err, sizeof(g_err), "no fp32 buffer"); return -1; }
   k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_vals32);
   CK_LAUNCH("k_vals_f32");
+  if (K->bs == 3 && K->d_corr32) {
+    k_corr32<<<(unsigned)((K->n_brows + WG - 1) / WG), WG, 0, S(s)>>>(view(K), K->d_vals, K->d_corr32);
+    CK_LAUNCH("k_corr32");
+  }
   return 0;
 }
 
@@ -3907,6 +4016,14 @@ int ugk_spmv(const ugk_mat *K, const double *d_x, double *d_y, void *s)
   if (K->bs == 3) k_spmv<3, double, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, d_x, d_y);
   else k_spmv<1, double, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, d_x, d_y);
   CK_LAUNCH("k_spmv");
+  return 0;
+}
+int ugk_spmv_f32x64(const ugk_mat *K, const double *d_x, double *d_y, void *s)
+{
+  if (!K->d_vals32) { snprintf(g_err, sizeof(g_err), "no fp32 copy of the matrix"); return -1; }
+  if (K->bs == 3) k_spmv<3, float, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, d_x, d_y);
+  else k_spmv<1, float, double><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals32, d_x, d_y);
+  CK_LAUNCH("k_spmv_f32x64");
   return 0;
 }
 int ugk_spmv_f32(const ugk_mat *K, const float *d_x, float *d_y, void *s)
@@ -4431,22 +4548,23 @@ int ugk_f1_start(const ugk_mat *K, long n, const double *b, const double *x, dou
   return 0;
 }
 
-int ugk_f1_input0(long n, const double *r, const double *dinv, unsigned short *x0, ugk_state *st, double *part,
-                  void *s)
+int ugk_f1_input0(long n, const double *r, const double *dinv, unsigned short *x0, double *v64, const float *mask,
+                  ugk_state *st, double *part, void *s)
 {
-  k_f1_input0<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, x0, st, part);
+  k_f1_input0<<<vec_blocks(n), WG, 0, S(s)>>>(n, r, dinv, x0, v64, mask, st, part);
   CK_LAUNCH("k_f1_input0");
   return 0;
 }
 
 int ugk_f1_refresh(const ugk_mat *K, long n, const double *b, const double *x, double *r, const double *dinv,
-                   const double *p, const double *q, unsigned short *x0, ugk_state *st, double *part, void *s)
+                   const double *p, const double *q, unsigned short *x0, double *v64, const float *mask,
+                   ugk_state *st, double *part, void *s)
 {
   if (!f1_grid_ok(K, n)) return -1;
   if (K->bs == 3)
-    k_f1_refresh<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, p, q, x0, st, part);
+    k_f1_refresh<3><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, p, q, x0, v64, mask, st, part);
   else
-    k_f1_refresh<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, p, q, x0, st, part);
+    k_f1_refresh<1><<<spmv_blocks(K), WG, 0, S(s)>>>(view(K), K->d_vals, b, x, r, dinv, p, q, x0, v64, mask, st, part);
   CK_LAUNCH("k_f1_refresh");
   return 0;
 }
@@ -4462,7 +4580,8 @@ int ugk_f1_combine(long n, const double *r, const double *dinv, const float *mas
 int ugk_f1_combine_c0t(long n, const double *r, const double *dinv, const float *mask, const ugk_conv_desc *cd,
                        const double *zc, const double *q, double *z, ugk_state *st, double *part, void *s)
 {
-  if (cd->mode != UGK_CONV_T2 || cd->x_f32 != UGK_ACT_F32 || (cd->cin != 32 && cd->cin != 16) || cd->ldx % 4 ||
+  const bool f64 = cd->x_f32 == UGK_ACT_F64;
+  if (cd->mode != UGK_CONV_T2 || (cd->x_f32 != UGK_ACT_F32 && !f64) || (cd->cin != 32 && cd->cin != 16) || cd->ldx % 4 ||
       cd->ldx < cd->cin || cd->cout != 3 || cd->cop < 3 || cd->cip < cd->cin || cd->nb != 1 ||
       n != 3L * cd->go.D * cd->go.H * cd->go.W || n / 3 >= (1L << 31) || cd->gi.D != (cd->go.D + 1) / 2 ||
       cd->gi.H != (cd->go.H + 1) / 2 || cd->gi.W != (cd->go.W + 1) / 2) {
@@ -4476,8 +4595,13 @@ int ugk_f1_combine_c0t(long n, const double *r, const double *dinv, const float 
     return -1;
   }
   const unsigned G = (unsigned)((vc + npb - 1) / npb);
-  if (cd->cin == 32) k_f1_combine_c0t<32><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part, npb);
-  else k_f1_combine_c0t<16><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part, npb);
+  if (f64) {
+    if (cd->cin == 32) k_f1_combine_c0t<32, double><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part, npb);
+    else k_f1_combine_c0t<16, double><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part, npb);
+  } else {
+    if (cd->cin == 32) k_f1_combine_c0t<32, float><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part, npb);
+    else k_f1_combine_c0t<16, float><<<G, C0T_TB, 0, S(s)>>>(n, r, dinv, mask, *cd, zc, q, z, st, part, npb);
+  }
   CK_LAUNCH("k_f1_combine_c0t");
   return 0;
 }
@@ -4501,9 +4625,10 @@ int ugk_f1_w(const ugk_mat *K, const double *z, double *w, int f32, const ugk_st
 }
 
 int ugk_f1_update(const ugk_mat *K, long n, double *x, double *r, double *p, double *q, const double *z,
-                  const double *w, const double *dinv, unsigned short *x0, ugk_state *st, double *part, void *s)
+                  const double *w, const double *dinv, unsigned short *x0, double *v64, const float *mask,
+                  ugk_state *st, double *part, void *s)
 {
-  k_f1_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, p, q, z, w, dinv, x0, st, part, spmv_blocks(K));
+  k_f1_update<<<vec_blocks(n), WG, 0, S(s)>>>(n, x, r, p, q, z, w, dinv, x0, v64, mask, st, part, spmv_blocks(K));
   CK_LAUNCH("k_f1_update");
   return 0;
 }

@@ -1005,6 +1005,245 @@ static unsigned grid_for(long work, int per = 256, long cap = 4096)
   return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
+
+// ---------------------------------------------------------------------------
+// FP64 inference: the solver's application of the network
+// ---------------------------------------------------------------------------
+// The operator is the matrix-core path's (bf16 weights, widened exactly; f32
+// gates, a fixed diagonal scaling and so part of the operator), applied with
+// FP64 activations and sums.  Flexible CG with one kept direction needs the
+// preconditioner to be one linear map from iteration to iteration: rounding
+// noise as small as f32's (1e-7 relative) costs it its global conjugacy and
+// iterations (a 24,336-DOF beam: 189 with FP64 activations, 231 to 243 with
+// f32; the 1M-DOF cantilever: 284 against 331 to 427).  At 16 channels the
+// network is 0.4 GFLOP, so FP64 costs little beside the launches' latency.
+
+// FP64 copy of one layer's bf16 weights [27][cop][cip] as wu [27][co][ci]
+// (the last adjoint stage, y = C0^T t0, on the vector ALUs)
+__global__ void k_w64(const bf16 *__restrict__ wf, int cop, int cip, int co, int ci, double *__restrict__ wu)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 27 * co * ci) return;
+  const int t = i / (co * ci), o = (i / ci) % co, c = i % ci;
+  wu[i] = (double)bf2f(wf[((long)t * cop + o) * cip + c]);
+}
+
+// The 16- and 32-channel stages run on the FP64 matrix cores:
+// v_mfma_f64_16x16x4f64, D (16 x 16) += A (16 x 4) B (4 x 16); lane l holds
+// A[l % 16][l / 16], B[l / 16][l % 16] and D[4 v + l / 16][l % 16], v < 4.
+// A wave takes 16 nodes (rows) and 16 output channels (columns); per tap the
+// K dimension runs over the input channels in S = ceil(cin / 4) steps, and
+// lane group g = l / 16 holds the channels g S .. g S + S - 1 (one
+// contiguous gather per tap; any fixed channel-to-step map will do, the
+// weights follow it).  Fragments: for output tile nt, tap t, step s, lane l
+// holds W(k, j), j = 16 nt + l % 16, k = (l / 16) S + s (0 past the channels);
+// the convolution's W(k, j) = W[t][j][k], its adjoint's W(k, j) = W[t][k][j].
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+
+__global__ void k_w64_frag(const bf16 *__restrict__ wf, int cop, int cip, int co, int ci, int up,
+                           double *__restrict__ frag)
+{
+  const int nin = up ? co : ci, nout = up ? ci : co, S = (nin + 3) / 4, NT = (nout + 15) / 16;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NT * 27 * S * 64) return;
+  const int l = i % 64, st = (i / 64) % S, t = (i / (64 * S)) % 27, nt = i / (64 * S * 27);
+  const int j = 16 * nt + l % 16, k = (l / 16) * S + st;
+  double v = 0.0;
+  if (j < nout && k < nin) {
+    const int o = up ? k : j, c = up ? j : k;   // the convolution's W[t][o][c]
+    v = (double)bf2f(wf[((long)t * cop + o) * cip + c]);
+  }
+  frag[i] = v;
+}
+
+// y[c][j] = (gate[c][j] *) sum_t sum_k W[t][j][k] x[2 c + off_t][k], stride 2,
+// zero padding.  A block of 4 waves takes 16 coarse nodes and output tile
+// blockIdx.y; wave w the taps 7 w .. 7 w + 6, all its gathers and fragment
+// loads issued before its first matrix op (one latency, not 27), and the
+// four partial tiles meet in LDS in a fixed order.
+#define N64_TPW 7
+template <int S>
+__global__ __launch_bounds__(256) void k_n64_down(const double *__restrict__ x, int ldx, int ci, ugk_grid3 gi,
+                                                  ugk_grid3 go, const double *__restrict__ frag, int co,
+                                                  const float *__restrict__ gate, double *__restrict__ y,
+                                                  const int *skip)
+{
+  __shared__ double part[3][4][64];
+  if (skip && *skip) return;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c0 = g * S;
+  const unsigned HWo = (unsigned)go.H * go.W, Vo = (unsigned)go.D * HWo;
+  const unsigned node = blockIdx.x * 16 + (l & 15);
+  const bool nok = node < Vo;
+  const unsigned nn = nok ? node : 0;
+  const int cz = (int)(nn / HWo), cr = (int)(nn - (unsigned)cz * HWo), cy = cr / go.W, cx = cr - cy * go.W;
+  const double *fb = frag + (long)blockIdx.y * 27 * S * 64 + l;
+  double a[N64_TPW][S], b[N64_TPW][S];
+#pragma unroll
+  for (int u = 0; u < N64_TPW; u++) {
+    const int t = N64_TPW * w + u;
+    const int fz = 2 * cz + t / 9 - 1, fy = 2 * cy + (t / 3) % 3 - 1, fx = 2 * cx + t % 3 - 1;
+    const bool ok = nok && t < 27 && fz >= 0 && fz < gi.D && fy >= 0 && fy < gi.H && fx >= 0 && fx < gi.W;
+    const double *xs = x + ((long)(fz * gi.H + fy) * gi.W + fx) * ldx + c0;
+#pragma unroll
+    for (int q = 0; q < S; q++) {
+      a[u][q] = (ok && c0 + q < ci) ? xs[q] : 0.0;
+      b[u][q] = t < 27 ? fb[(t * S + q) * 64] : 0.0;
+    }
+  }
+  mf64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < N64_TPW; u++)
+#pragma unroll
+    for (int q = 0; q < S; q++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][q], b[u][q], acc, 0, 0, 0);
+  if (w) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) part[w - 1][v][l] = acc[v];
+  }
+  __syncthreads();
+  if (w) return;
+#pragma unroll
+  for (int v = 0; v < 4; v++) acc[v] = ((acc[v] + part[0][v][l]) + part[1][v][l]) + part[2][v][l];
+  const int j = blockIdx.y * 16 + (l & 15);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const unsigned nd = blockIdx.x * 16 + 4 * v + g;
+    if (nd < Vo && j < co) {
+      const long e = (long)nd * co + j;
+      y[e] = gate ? (double)gate[e] * acc[v] : acc[v];
+    }
+  }
+}
+
+// the taps of class (PZ, PY, PX) of the adjoint: per axis the coarse node c
+// with offset 0 (P = 0), or c with offset +1 and c + 1 with offset -1 (P = 1);
+// the gathers and fragment loads of up to 32 / S taps issued before their
+// matrix ops (all 8 taps of a class at 16 channels)
+template <int PZ, int PY, int PX, int S>
+__device__ __forceinline__ mf64x4 n64_up_class(const double *__restrict__ x, int ci, ugk_grid3 gc, int cz, int cy,
+                                               int cx, bool fok, int c0, const double *__restrict__ fb)
+{
+  constexpr int NY = PY + 1, NX = PX + 1, NT = (PZ + 1) * NY * NX, BT = 32 / S < NT ? 32 / S : NT;
+  mf64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u0 = 0; u0 < NT; u0 += BT) {
+    double a[BT][S], b[BT][S];
+#pragma unroll
+    for (int uu = 0; uu < BT; uu++) {
+      const int u = u0 + uu;
+      const int iz = u / (NY * NX), iy = (u / NX) % NY, ix = u % NX;
+      const int sz = cz + iz, sy = cy + iy, sx = cx + ix;   // offset -1 reads c + 1
+      const bool ok = fok && sz < gc.D && sy < gc.H && sx < gc.W;
+      const int t = 9 * (PZ ? 2 - 2 * iz : 1) + 3 * (PY ? 2 - 2 * iy : 1) + (PX ? 2 - 2 * ix : 1);
+      const double *xs = x + ((long)(sz * gc.H + sy) * gc.W + sx) * ci + c0;
+#pragma unroll
+      for (int q = 0; q < S; q++) {
+        a[uu][q] = (ok && c0 + q < ci) ? xs[q] : 0.0;
+        b[uu][q] = fb[(t * S + q) * 64];
+      }
+    }
+#pragma unroll
+    for (int uu = 0; uu < BT; uu++)
+#pragma unroll
+      for (int q = 0; q < S; q++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[uu][q], b[uu][q], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// y[f][j] = gate[f][j] h[f][j] + sum over the taps of f of W[t][k][j] x[c][k]:
+// the stride-2 adjoint from gc (ci channels) to gf (co channels) with the
+// gated skip of the level.  Fine node f = 2 c + p of parity class p; a block
+// of 8 waves takes 16 coarse nodes, wave w the fine nodes of class w (rows:
+// the 16 coarse nodes), so its taps are compile-time; output tile blockIdx.y.
+template <int S>
+__global__ __launch_bounds__(512) void k_n64_up(const double *__restrict__ x, int ci, ugk_grid3 gc, ugk_grid3 gf,
+                                                const double *__restrict__ frag, int co,
+                                                const float *__restrict__ gate, const double *__restrict__ h,
+                                                double *__restrict__ y, const int *skip)
+{
+  if (skip && *skip) return;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c0 = g * S;
+  const int pz = w >> 2, py = (w >> 1) & 1, px = w & 1;
+  const unsigned HWc = (unsigned)gc.H * gc.W, Vc = (unsigned)gc.D * HWc;
+  const unsigned c = blockIdx.x * 16 + (l & 15);
+  const bool cok = c < Vc;
+  const unsigned cc = cok ? c : 0;
+  const int cz = (int)(cc / HWc), cr = (int)(cc - (unsigned)cz * HWc), cy = cr / gc.W, cx = cr - cy * gc.W;
+  const bool fok = cok && 2 * cz + pz < gf.D && 2 * cy + py < gf.H && 2 * cx + px < gf.W;
+  const double *fb = frag + (long)blockIdx.y * 27 * S * 64 + l;
+  mf64x4 acc;
+  switch (w) {   // wave-uniform
+    case 0: acc = n64_up_class<0, 0, 0, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+    case 1: acc = n64_up_class<0, 0, 1, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+    case 2: acc = n64_up_class<0, 1, 0, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+    case 3: acc = n64_up_class<0, 1, 1, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+    case 4: acc = n64_up_class<1, 0, 0, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+    case 5: acc = n64_up_class<1, 0, 1, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+    case 6: acc = n64_up_class<1, 1, 0, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+    default: acc = n64_up_class<1, 1, 1, S>(x, ci, gc, cz, cy, cx, fok, c0, fb); break;
+  }
+  const int j = blockIdx.y * 16 + (l & 15);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const unsigned cn = blockIdx.x * 16 + 4 * v + g;
+    if (cn >= Vc || j >= co) continue;
+    const int nz = (int)(cn / HWc), nr = (int)(cn - (unsigned)nz * HWc), ny = nr / gc.W, nx = nr - ny * gc.W;
+    const int fz = 2 * nz + pz, fy = 2 * ny + py, fx = 2 * nx + px;
+    if (fz >= gf.D || fy >= gf.H || fx >= gf.W) continue;
+    const long e = (((long)fz * gf.H + fy) * gf.W + fx) * co + j;
+    y[e] = gate ? acc[v] + (double)gate[e] * h[e] : acc[v];
+  }
+}
+
+// y[f][o] = sum over the taps of f of W[t][k][o] x[c][k], o < 3: the last
+// adjoint stage (C0^T, 16 or 32 channels to the 3 DOFs of the node grid) on
+// the vector ALUs, for the network on its own (the solver folds it into its
+// combine).  A block of 8 waves takes 8 coarse nodes, wave w the fine nodes
+// of class w, 8 lanes per node, lane l its tap l; the sums meet by lane
+// shuffles.
+template <int CI>
+__global__ __launch_bounds__(512) void k_n64_out(const double *__restrict__ x, ugk_grid3 gc, ugk_grid3 gf,
+                                                 const double *__restrict__ wu, double *__restrict__ y)
+{
+  const int cls = threadIdx.x >> 6, lane = threadIdx.x & 63, ls = lane & 7;
+  const int pz = cls >> 2, py = (cls >> 1) & 1, px = cls & 1;
+  const int ny = py + 1, nx = px + 1, nt = (pz + 1) * ny * nx;
+  const unsigned c = blockIdx.x * 8 + (lane >> 3);
+  const unsigned HWc = (unsigned)gc.H * gc.W;
+  const bool cok = c < (unsigned)gc.D * HWc;
+  const unsigned cc = cok ? c : 0;
+  const int cz = (int)(cc / HWc), cr = (int)(cc - (unsigned)cz * HWc), cy = cr / gc.W, cx = cr - cy * gc.W;
+  const int fz = 2 * cz + pz, fy = 2 * cy + py, fx = 2 * cx + px;
+  const bool fok = cok && fz < gf.D && fy < gf.H && fx < gf.W;
+  const int iz = ls / (ny * nx), iy = (ls / nx) % ny, ix = ls % nx;
+  const int sz = cz + iz, sy = cy + iy, sx = cx + ix;
+  const bool ok = fok && ls < nt && sz < gc.D && sy < gc.H && sx < gc.W;
+  const int t = 9 * (pz ? 2 - 2 * iz : 1) + 3 * (py ? 2 - 2 * iy : 1) + (px ? 2 - 2 * ix : 1);
+  const double *xs = x + ((long)(sz * gc.H + sy) * gc.W + sx) * CI;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (ok) {
+#pragma unroll 4
+    for (int k = 0; k < CI; k++) {
+      const double xv = xs[k];
+#pragma unroll
+      for (int o = 0; o < 3; o++) acc[o] = fma(wu[((long)t * CI + k) * 3 + o], xv, acc[o]);
+    }
+  }
+#pragma unroll
+  for (int m = 4; m >= 1; m >>= 1)
+#pragma unroll
+    for (int o = 0; o < 3; o++) acc[o] += __shfl_xor(acc[o], m);
+  if (ls == 0 && fok) {
+    double *yo = y + (((long)fz * gf.H + fy) * gf.W + fx) * 3;
+#pragma unroll
+    for (int o = 0; o < 3; o++) yo[o] = acc[o];
+  }
+}
+
+static bool n64_grid_ok(ugk_grid3 gi, ugk_grid3 go)
+{
+  return go.D == (gi.D + 1) / 2 && go.H == (gi.H + 1) / 2 && go.W == (gi.W + 1) / 2 && vox(gi) < (1L << 31);
+}
+
 extern "C" {
 
 int ugk_conv(const ugk_conv_desc *d, void *s)
@@ -1131,6 +1370,60 @@ int ugk_pair_write(long n, const double *x, const double *r, const double *u, co
 {
   k_pair_write<<<grid_for(n), 256, 0, S(s)>>>(n, x, r, u, dinv, mask, sc > 0 ? 1.0 / sc : 0.0, V, rr, ee);
   CK_LAUNCH("k_pair_write");
+  return 0;
+}
+
+int ugk_w64(const unsigned short *wf, int cop, int cip, int co, int ci, double *wu, void *s)
+{
+  const int n = 27 * co * ci;
+  k_w64<<<(n + 255) / 256, 256, 0, S(s)>>>(wf, cop, cip, co, ci, wu);
+  CK_LAUNCH("k_w64");
+  return 0;
+}
+
+long ugk_n64_frag_len(int nin, int nout) { return (long)((nout + 15) / 16) * 27 * ((nin + 3) / 4) * 64; }
+
+int ugk_w64_frag(const unsigned short *wf, int cop, int cip, int co, int ci, int up, double *frag, void *s)
+{
+  const long n = up ? ugk_n64_frag_len(co, ci) : ugk_n64_frag_len(ci, co);
+  k_w64_frag<<<(unsigned)((n + 255) / 256), 256, 0, S(s)>>>(wf, cop, cip, co, ci, up, frag);
+  CK_LAUNCH("k_w64_frag");
+  return 0;
+}
+
+int ugk_n64_down(const double *x, int ldx, int ci, ugk_grid3 gi, ugk_grid3 go, const double *frag, int co,
+                 const float *gate, double *y, const int *skip, void *s)
+{
+  if (!n64_grid_ok(gi, go) || ldx < ci || (co != 16 && co != 32) || (ci != 3 && ci != 16 && ci != 32))
+    return ugk_hip_fail("ugk_n64_down: channels or grids", hipErrorInvalidValue);
+  const dim3 G((unsigned)((vox(go) + 15) / 16), (unsigned)(co / 16));
+  if (ci == 3) k_n64_down<1><<<G, 256, 0, S(s)>>>(x, ldx, ci, gi, go, frag, co, gate, y, skip);
+  else if (ci == 16) k_n64_down<4><<<G, 256, 0, S(s)>>>(x, ldx, ci, gi, go, frag, co, gate, y, skip);
+  else k_n64_down<8><<<G, 256, 0, S(s)>>>(x, ldx, ci, gi, go, frag, co, gate, y, skip);
+  CK_LAUNCH("k_n64_down");
+  return 0;
+}
+
+int ugk_n64_up(const double *x, int ci, ugk_grid3 gc, ugk_grid3 gf, const double *frag, int co, const float *gate,
+               const double *h, double *y, const int *skip, void *s)
+{
+  if (!n64_grid_ok(gf, gc) || (co != 16 && co != 32) || (ci != 16 && ci != 32))
+    return ugk_hip_fail("ugk_n64_up: channels or grids", hipErrorInvalidValue);
+  const dim3 G((unsigned)((vox(gc) + 15) / 16), (unsigned)(co / 16));
+  if (ci == 16) k_n64_up<4><<<G, 512, 0, S(s)>>>(x, ci, gc, gf, frag, co, gate, h, y, skip);
+  else k_n64_up<8><<<G, 512, 0, S(s)>>>(x, ci, gc, gf, frag, co, gate, h, y, skip);
+  CK_LAUNCH("k_n64_up");
+  return 0;
+}
+
+int ugk_n64_out(const double *x, int ci, ugk_grid3 gc, ugk_grid3 gf, const double *wu, double *y, void *s)
+{
+  if (!n64_grid_ok(gf, gc) || (ci != 16 && ci != 32))
+    return ugk_hip_fail("ugk_n64_out: channels or grids", hipErrorInvalidValue);
+  const unsigned G = (unsigned)((vox(gc) + 7) / 8);
+  if (ci == 16) k_n64_out<16><<<G, 512, 0, S(s)>>>(x, gc, gf, wu, y);
+  else k_n64_out<32><<<G, 512, 0, S(s)>>>(x, gc, gf, wu, y);
+  CK_LAUNCH("k_n64_out");
   return 0;
 }
 

@@ -101,6 +101,7 @@ def lib():
             "ufem_net_collect": (C.c_long, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p]),
             "ufem_net_train": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_double, C.c_ulong, P_D, P_D]),
             "ufem_net_time_stages": (C.c_int, [C.c_void_p, C.c_int, P_D]),
+            "ufem_net_apply64": (C.c_int, [C.c_void_p, P_D, P_D]),
             "ufem_net_time_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P_I, P_I, C.c_int, C.c_int,
                                              C.c_int, P_D]),
             "ufem_net_probe_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P_I, P_I, C.c_int, C.c_int,
@@ -118,6 +119,7 @@ def lib():
             "ufem_sys_get_F": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_get_K": (C.c_int, [C.c_void_p, P_L, P_I, P_D]),
             "ufem_sys_spmv": (C.c_int, [C.c_void_p, P_D, P_D]),
+            "ufem_sys_spmv_f32": (C.c_int, [C.c_void_p, P_D, P_D]),
             "ufem_sys_time_spmv": (C.c_int, [C.c_void_p, C.c_int, C.c_int, P_D, P_D]),
             "ufem_sys_timings": (None, [C.c_void_p, P_D]),
             "ufem_gpu_element_stiffness": (C.c_int, [C.c_long, P_D, P_D, P_D, C.c_int, P_D]),
@@ -383,6 +385,13 @@ class GpuSystem:
         self._chk(lib().ufem_sys_spmv(self.h, _pd(x), _pd(y)), "spmv")
         return y
 
+    def spmv_f32(self, x):
+        """y = K x with the mixed solvers' FP32 copy of K (FP64 vectors)"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        self._chk(lib().ufem_sys_spmv_f32(self.h, _pd(x), _pd(y)), "spmv_f32")
+        return y
+
     def time_kernel(self, mode="spmv", reps=20):
         """(avg ms, bytes) of one kernel: 'spmv', 'spmv32', 'pcg_a', 'pcg_q', 'cg1_w', 'copy', or 'read' /
         'read_nt' (a plain streaming read of the matrix's stored values and columns, with ordinary or
@@ -478,6 +487,15 @@ class GpuNet:
         L = self.levels
         return ([f"C{l}" for l in range(L - 1)] + [f"C{L - 1}g"] +
                 [f"C{l}T" for l in range(L - 1, -1, -1)])
+
+    def apply64(self, v):
+        """N(v) with the solver's FP64 inference, v [D][H][W][3] on the bound system's node grid"""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        y = np.zeros_like(v)
+        rv = lib().ufem_net_apply64(self.h, _pd(v), _pd(y))
+        if rv != AF_OK:
+            raise RuntimeError(f"apply64 failed ({rv}): {last_error()}")
+        return y
 
     def time_stages(self, reps=50):
         names = self.stage_names()
