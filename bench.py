@@ -335,10 +335,13 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
     fresh i.i.d. lognormal fields of the same distribution (seeds seed_base..),
     `steps` Adam steps, then plain FP64 PCG, flexible CG with the network
     (FP64) and flexible CG with the network and q = K p from the FP32 copy of
-    K (FP64 residual refreshes), all to ||r||/||f|| <= 1e-8 from u = 0; then
-    the same with the two-level correction on a coarse grid every `coarse`
-    bricks (PCG without the network, FP32 matrix without the network, the
-    network in FP64 and with the FP32 matrix)."""
+    K (FP64 residual refreshes), the same with the copy's bf16 remainders
+    (48-bit values), all to ||r||/||f|| <= 1e-8 from u = 0, the network
+    applied in FP64 (and, for comparison, the FP64 learned and the mixed solve
+    with it on the bf16 matrix cores with f32 activations); then the same with
+    the two-level correction on a coarse grid every `coarse` bricks (PCG
+    without the network, FP32 matrix (and 48-bit) without the network, the
+    network in FP64 and with the FP32 (and 48-bit) matrix)."""
     from ufem_amd import gpu as G
     from ufem_amd import learned as Lr
     from ufem_amd.mesh import lognormal_E
@@ -362,7 +365,17 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
     rp, tp, up = timed(solver="pcg", eps=1e-8, rule=1)
     rl, tl, ul = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=1)
     rm, tm, um = timed(solver="learned_mixed", eps=1e-8, rule=1, net=net, fcg_m=1)
-    stages = net.time_stages()
+    rr, tr, ur = timed(solver="learned_mixed", eps=1e-8, rule=1, net=net, fcg_m=1, f32_rem=True)
+    # the same network applied on the bf16 matrix cores with f32 activations
+    # (its rounding noise makes the iteration count depend on the load's scale)
+    os.environ["UFEM_NET_F32"] = "1"
+    try:
+        rf, tf, _ = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=1)
+        rfm, tfm, _ = timed(solver="learned_mixed", eps=1e-8, rule=1, net=net, fcg_m=1)
+    finally:
+        del os.environ["UFEM_NET_F32"]
+    stages = net.time_stages64()
+    stages32 = net.time_stages()
 
     def err(u):
         return float(np.linalg.norm(u - up) / np.linalg.norm(up))
@@ -373,9 +386,11 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
     variants = {
         "twolevel": dict(solver="twolevel"),
         "twolevel_mixed": dict(solver="twolevel_mixed"),
+        "twolevel_mixed_rem": dict(solver="twolevel_mixed", f32_rem=True),
         "coarse_mixed": dict(solver="learned_mixed", coarse=True),
         "learned_coarse": dict(solver="learned", net=net, fcg_m=1, coarse=True),
         "learned_coarse_mixed": dict(solver="learned_mixed", net=net, fcg_m=1, coarse=True),
+        "learned_coarse_mixed_rem": dict(solver="learned_mixed", net=net, fcg_m=1, coarse=True, f32_rem=True),
     }
     with_coarse = {"coarse_grid": {"factors": list(coarse), "coarse_dof": int(nc)}}
     for name, kw in variants.items():
@@ -400,11 +415,24 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
                     "true_relres": rl["true_relres"], "u_rel_err_vs_fp64": err(ul)},
         "learned_mixed": {"iters": rm["iters"], "solve_s": round(tm, 4), "fp64_refreshes": rm["outer"],
                           "fallback": rm["fallback"], "true_relres": rm["true_relres"], "u_rel_err_vs_fp64": err(um)},
+        "learned_mixed_rem": {"matrix": "FP32 + bf16 remainder per value (48-bit), FP64 vectors",
+                              "iters": rr["iters"], "solve_s": round(tr, 4), "fp64_refreshes": rr["outer"],
+                              "fallback": rr["fallback"], "true_relres": rr["true_relres"],
+                              "u_rel_err_vs_fp64": err(ur)},
         "iteration_ratio": round(rp["iters"] / max(rm["iters"], 1), 3),
-        "speedup_vs_plain": {"learned": round(tp / tl, 3), "learned_mixed": round(tp / tm, 3)},
+        "speedup_vs_plain": {"learned": round(tp / tl, 3), "learned_mixed": round(tp / tm, 3),
+                             "learned_mixed_rem": round(tp / tr, 3)},
         "us_per_iteration": {"plain": round(tp / rp["iters"] * 1e6, 1), "learned": round(tl / rl["iters"] * 1e6, 1),
-                             "learned_mixed": round(tm / rm["iters"] * 1e6, 1)},
+                             "learned_mixed": round(tm / rm["iters"] * 1e6, 1),
+                             "learned_mixed_rem": round(tr / rr["iters"] * 1e6, 1)},
+        "inference": "FP64 activations and sums on the FP64 matrix cores (the bf16 weights widened exactly)",
+        "inference_f32": {"what": "the same network on the bf16 matrix cores with f32 activations",
+                          "learned": {"iters": rf["iters"], "solve_s": round(tf, 4),
+                                      "speedup_vs_plain": round(tp / tf, 3)},
+                          "learned_mixed": {"iters": rfm["iters"], "solve_s": round(tfm, 4),
+                                            "speedup_vs_plain": round(tp / tfm, 3)}},
         "network_stage_us": {k: round(v, 1) for k, v in stages.items()},
+        "network_stage_us_f32": {k: round(v, 1) for k, v in stages32.items()},
         "with_coarse": with_coarse,
     }
     net.close()
@@ -588,7 +616,11 @@ def main():
                        width=args.net_width)
         net.init(1)
         net.bind(sk, mk.E)
-        log("network stages (us)", {k: round(v, 1) for k, v in net.time_stages(reps=20).items()})
+        log("network stages, bf16 matrix cores, f32 activations (us)",
+            {k: round(v, 1) for k, v in net.time_stages(reps=20).items()})
+        if not args.net_node_grid and args.net_width <= 32:
+            log("network stages, FP64 (the solver's inference; us)",
+                {k: round(v, 1) for k, v in net.time_stages64(reps=20).items()})
         net.close()
         sk.close()
         return
@@ -632,7 +664,8 @@ def main():
         net.init(1)
         net.bind(g, mesh.E)
         g.set_coarse(nx + 1, ny + 1, nz + 1, 8, 10, 10)
-        for name, kw in (("learned", {}), ("learned_mixed", {}), ("learned_mixed", {"coarse": True})):
+        for name, kw in (("learned", {}), ("learned_mixed", {}), ("learned_mixed", {"f32_rem": True}),
+                         ("learned_mixed", {"coarse": True})):
             r = g.solve(solver=name, eps=1e-8, rule=1, net=net, maxit=4000, **kw)
             log(f"{name} {kw}: {r['iters']} iterations, {r['solve_ms']:.2f} ms device")
         net.close()

@@ -94,6 +94,11 @@ typedef struct {
                         equation, run until its residual has fallen by
                         inner_eps (or inner_maxit iterations); 0: the FP32 CG
                         keeps its direction across refreshes (reliable update) */
+  int f32_rem;       /* mixed forms with FP64 vectors (learned-mixed,
+                        two-level-mixed, mixed-v64): the FP32 copy of K also
+                        keeps each value's remainder K - fl32(K) rounded to
+                        bf16 (48-bit values, 6 bytes instead of 8); FP32
+                        vectors (UFEM_SOLVER_MIXED) ignore it */
 } ufem_solve_opts;
 
 typedef struct {
@@ -179,12 +184,13 @@ int  ufem_sys_get_F(ufem_sys *s, double *F);
 int  ufem_sys_get_K(ufem_sys *s, long *indptr, int *cols, double *vals);
 int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host out */
 /* y = K x from the FP32 copy of K that the mixed-precision solvers use
- * (no counterpart in the reference): the copy rounds every value to FP32 and,
- * for 3x3 node blocks, adds to each diagonal block the symmetric part of its
- * block row's summed rounding errors, so rigid translations, which a
- * stiffness matrix's rows cancel, stay cancelled; FP64 vectors and sums.
+ * (no counterpart in the reference): the copy rounds every value to FP32
+ * (with rem, plus its remainder rounded to bf16: 48-bit values) and, for 3x3
+ * node blocks, adds to each diagonal block the symmetric part of its block
+ * row's summed rounding errors, so rigid translations, which a stiffness
+ * matrix's rows cancel, stay cancelled; FP64 vectors and sums.
  * UFEM_NO_CORR32 set in the environment drops the correction. */
-int  ufem_sys_spmv_f32(ufem_sys *s, const double *x, double *y);
+int  ufem_sys_spmv_f32(ufem_sys *s, const double *x, double *y, int rem);
 /* average device time (HIP events on the system's stream) of `reps`
  * launches of one kernel, and the bytes one launch moves by the algorithmic
  * count.  mode 0: y = K x (FP64); 1: y = K x (FP32 copy of K); 2: the PCG
@@ -361,6 +367,10 @@ int  ufem_net_apply64(ufem_net *N, const double *v, double *y);
  * system's grid: C0, C1, .., C_{L-1} (with the gate product), C_{L-1}^T, ..,
  * C1^T, C0^T */
 int  ufem_net_time_stages(ufem_net *N, int reps, double *us);
+/* the same for the solver's FP64 inference (coarse-first form, 16 or 32
+ * channels): C0 .. C_{L-1}, C_{L-1}^T .. C1^T, and C0^T on its own (in a
+ * solve it runs inside the flexible-CG combine) */
+int  ufem_net_time_stages64(ufem_net *N, int reps, double *us);
 /* average device time (us) of `reps` launches of one such convolution */
 int  ufem_net_time_conv(int mode, int flip, int x_f32, int nb, const int *gi, const int *go, int cin, int cout,
                         int reps, double *us);

@@ -1,8 +1,9 @@
 """The mixed solvers' FP32 copy of K (`ufem_sys_spmv_f32`).
 
-The copy rounds every value of K to FP32 and adds to each diagonal 3x3 block
-C_I = sym(sum_J (K_IJ - fl32(K_IJ))), the symmetric part of its block row's
-summed rounding errors.  Checked here:
+The copy rounds every value of K to FP32 (optionally keeping the remainder
+K - fl32(K) rounded to bf16 beside it: 48-bit values) and adds to each
+diagonal 3x3 block C_I = sym(sum_J (K_IJ - stored(K_IJ))), the symmetric
+part of its block row's summed rounding errors.  Checked here:
 - y = K32' x against a numpy model of that copy built from K's FP64 values;
 - the property the correction is for: on a rigid translation, where the rows
   of a stiffness matrix cancel, the corrected product stays within a few
@@ -21,9 +22,17 @@ from ufem_amd.mesh import cantilever, lognormal_E
 pytestmark = pytest.mark.gpu
 
 
-def fp32_copy_model(ip, cols, vals, n):
+def bf16_rne(x):
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def fp32_copy_model(ip, cols, vals, n, rem=False):
     rows = np.repeat(np.arange(n), np.diff(ip))
     v32 = vals.astype(np.float32).astype(np.float64)
+    if rem:
+        v32 = v32 + bf16_rne(vals - v32)
     err = vals - v32
     C = np.zeros((n // 3, 3, 3))
     np.add.at(C, (rows // 3, rows % 3, cols % 3), err)
@@ -46,16 +55,17 @@ def system():
     return g, ip, cols, vals
 
 
-def test_fp32_copy_matches_its_model(system):
+@pytest.mark.parametrize("rem", [False, True])
+def test_fp32_copy_matches_its_model(system, rem):
     g, ip, cols, vals = system
     n = g.n
-    model = fp32_copy_model(ip, cols, vals, n)
+    model = fp32_copy_model(ip, cols, vals, n, rem)
     absK = np.abs(vals)
     rows = np.repeat(np.arange(n), np.diff(ip))
     rng = np.random.default_rng(7)
     for _ in range(3):
         x = rng.standard_normal(n)
-        y = g.spmv_f32(x)
+        y = g.spmv_f32(x, rem)
         scale = np.zeros(n)
         np.add.at(scale, rows, absK * np.abs(x[cols]))
         err = np.abs(y - model(x)) / scale.max()
@@ -64,6 +74,8 @@ def test_fp32_copy_matches_its_model(system):
 
 
 def test_fp32_copy_keeps_translations(system, monkeypatch):
+    """(the remainder form: tested through its model above and the solves
+    below; its translation error is below FP64 rounding of the sums)"""
     g, ip, cols, vals = system
     n = g.n
     Kmax = np.abs(vals).max()
@@ -90,9 +102,9 @@ def test_fp32_copy_is_symmetric(system):
     assert abs(a - b) <= 1e-13 * (abs(a) + abs(b))
 
 
-@pytest.mark.parametrize("solver", ["mixed", "mixed64"])
+@pytest.mark.parametrize("solver,rem", [("mixed", False), ("mixed64", False), ("mixed64", True)])
 @pytest.mark.parametrize("corr", [True, False])
-def test_mixed_solves_with_and_without_correction(solver, corr, monkeypatch):
+def test_mixed_solves_with_and_without_correction(solver, rem, corr, monkeypatch):
     mesh = cantilever(24, 6, 6, E=lognormal_E(22, 864, cov=0.3))
     if not corr:
         monkeypatch.setenv("UFEM_NO_CORR32", "1")
@@ -101,8 +113,9 @@ def test_mixed_solves_with_and_without_correction(solver, corr, monkeypatch):
     ref = G.GpuSystem(mesh)
     ref.assemble(2)
     r = ref.solve("pcg", eps=1e-11, rule=1)
-    info = g.solve(solver, eps=1e-10, rule=1, maxit=20000)
+    info = g.solve(solver, eps=1e-10, rule=1, maxit=20000, f32_rem=rem)
     err = float(np.linalg.norm(g.u() - ref.u()) / np.linalg.norm(ref.u()))
-    print(solver, "corr" if corr else "plain", "iters", info["iters"], "fp64", r["iters"], "u err", err)
+    print(solver, "rem" if rem else "", "corr" if corr else "plain", "iters", info["iters"], "fp64", r["iters"],
+          "u err", err)
     assert info["rv"] == 0 and info["true_relres"] <= 1e-10
     assert err <= 1e-8

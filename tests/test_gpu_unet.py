@@ -517,3 +517,22 @@ def test_fp64_learned_solve_is_scale_invariant(beam):
         sk.close()
     print(solver, "iterations at load scales 1, 1.1, 0.9:", its)
     assert max(its) - min(its) <= 1
+
+
+def test_learned_mixed_with_remainders_tracks_fp64(beam):
+    """learned + mixed with the FP32 copy's bf16 remainders (48-bit values):
+    close to the iterations of the FP64 learned solve (here 285 against 265;
+    a CPU restatement on a 24k-DOF beam with a trained network: 191 against
+    190) and well below the FP32 copy's (435), and the FP64 solution"""
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D, levels=6, fine=False, width=16)
+    net.init(7)
+    net.bind(s, m.E)
+    r64 = s.solve("learned", eps=1e-8, net=net)
+    rm = s.solve("learned_mixed", eps=1e-8, net=net, f32_rem=True)
+    assert rm["rv"] == 0 and not rm["fallback"] and rm["true_relres"] <= 1e-8 * (1 + 1e-9)
+    assert rel(s.u(), u_ref) <= 1e-6
+    r32 = s.solve("learned_mixed", eps=1e-8, net=net)
+    print("learned", r64["iters"], "learned + mixed, 48-bit K", rm["iters"], "FP32 K", r32["iters"])
+    assert rm["iters"] <= 1.1 * r64["iters"] + 5
+    assert rm["iters"] <= 0.8 * r32["iters"]

@@ -209,7 +209,7 @@ static void mirror_free(mirror *e)
 {
   if (g_ctx.ready) ugk_sync(g_ctx.stream);
   ugk_free(e->K.d_slen); ugk_free(e->K.d_soff); ugk_free(e->K.d_rowlen);
-  ugk_free(e->K.d_cols); ugk_free(e->K.d_vals); ugk_free(e->K.d_vals32); ugk_free(e->K.d_corr32);
+  ugk_free(e->K.d_cols); ugk_free(e->K.d_vals); ugk_free(e->K.d_vals32); ugk_free(e->K.d_corr32); ugk_free(e->K.d_rem16);
   ugk_free(e->d_map); ugk_free(e->d_stage);
   ugk_host_free(e->h_stage);
   /* the SSOR colour order cached for this copy is keyed by its address */

@@ -204,6 +204,7 @@ int ufem_batch_assemble(ufem_batch *b, int npts, double *ms)
     K.d_vals = b->d_vals[i];
     K.d_vals32 = NULL;
     K.d_corr32 = NULL;
+    K.d_rem16 = NULL;
     if (ugk_h2d(S->d_E, b->E_host[i], sizeof(double) * (size_t)S->n_elems, b->stream) ||
         ugk_h2d(S->d_nu, b->nu_host[i], sizeof(double) * (size_t)S->n_elems, b->stream) ||
         ugk_zero(K.d_vals, sizeof(double) * (size_t)K.total_slots * 64 * 9, b->stream))

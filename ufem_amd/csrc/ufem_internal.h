@@ -117,8 +117,9 @@ int ufem_net_infer64_head(ufem_net *N, const int *d_skip, void *stream);
 int ufem_net_last64(const ufem_net *N, ugk_conv_desc *d);  /* 1: *d is y = C0^T t0 from FP64 t0 */
 
 /* FP32 copy of K (ufem_sys.c): allocates d_vals32 (and, for 3x3 blocks,
- * d_corr32) on first use and converts the current FP64 values */
-int ufem_mat_f32(ugk_mat *K, void *stream);
+ * d_corr32; with rem, d_rem16) on first use and converts the current FP64
+ * values */
+int ufem_mat_f32(ugk_mat *K, int rem, void *stream);
 
 /* error text (ufem_sys.c) */
 void ufem_set_error(const char *fmt, ...);

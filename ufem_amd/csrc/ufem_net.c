@@ -762,6 +762,7 @@ static int add_field(ufem_net *N)
   tmp = s->K;
   tmp.d_vals32 = N->fvals[f];
   tmp.d_corr32 = NULL;    /* the training loss uses the plain FP32 copy */
+  tmp.d_rem16 = NULL;
   if (ugk_vals_to_f32(&tmp, N->stream) || ugk_to_f32(s->n, s->w.dinv, N->fdinv[f], N->stream) ||
       ugk_h2d(N->feb[f], N->ebh, sizeof(float) * (size_t)Vt, N->stream) || ugk_sync(N->stream))
     return net_fail("field store");
@@ -844,6 +845,7 @@ static int pair_loss(ufem_net *N, int b, long q, int nb, double *part_out)
   int G;
   Kf.d_vals32 = N->fvals[f];
   Kf.d_corr32 = NULL;
+  Kf.d_rem16 = NULL;
   if (ugk_el_z(n, N->prr + q * n, N->fdinv[f], N->mask, N->y + b * N->Vf * 4, N->zz, N->stream) ||
       ugk_spmv_f32(&Kf, N->zz, N->kz, N->stream))
     return net_fail("loss");
@@ -1258,6 +1260,45 @@ out:
 
 /* average device time (us) of each inference stage on the bound system's
  * grid (us[0..5], stages as in infer_stage), `reps` launches each */
+/* stage k of FP64 inference: C0 .. C_{L-1} (k < L), C_{L-1}^T .. C1^T, and
+ * C0^T on its own (k = 2 L - 1; the solver folds it into its combine) */
+static int infer64_stage(ufem_net *N, int k)
+{
+  const int L = N->L;
+  if (k < L) {
+    const double *x = k ? N->h64[k - 1] : N->v64;
+    const int cin = k ? N->ch[k - 1] : 3;
+    return ugk_n64_down(x, cin, cin, k ? N->g[k - 1] : N->gf, N->g[k], N->fd64[k], N->ch[k],
+                        k == L - 1 ? N->gi[k] : NULL, k == L - 1 ? N->t64[k] : N->h64[k], NULL, N->stream);
+  }
+  if (k < 2 * L - 1) {
+    const int l = 2 * L - 2 - k;
+    return ugk_n64_up(N->t64[l + 1], N->ch[l + 1], N->g[l + 1], N->g[l], N->fu64[l + 1], N->ch[l], N->gi[l],
+                      N->h64[l], N->t64[l], NULL, N->stream);
+  }
+  return ugk_n64_out(N->t64[0], N->ch[0], N->g[0], N->gf, N->wu64, N->y64, N->stream);
+}
+
+int ufem_net_time_stages64(ufem_net *N, int reps, double *us)
+{
+  int k, r, rv;
+  if (!N->sys) { ufem_set_error("bind the network to a system first"); return AF_ERR; }
+  if ((rv = ufem_net_prepare64(N, N->stream)) != AF_OK) return rv;
+  if (!N->y64) N->y64 = (double *)ugk_malloc(sizeof(double) * 3 * (size_t)N->Vf);
+  if (!N->y64) return net_fail("FP64 output");
+  if (ufem_net_prepare(N, N->stream) || ugk_zero(N->v64, sizeof(double) * 3 * (size_t)N->Vf, N->stream))
+    return net_fail("FP64 stages");
+  for (k = 0; k < 2 * N->L; k++) {
+    if (infer64_stage(N, k)) return net_fail("stage");
+    if (ugk_event_record(N->ev[0], N->stream)) return net_fail("event");
+    for (r = 0; r < reps; r++)
+      if (infer64_stage(N, k)) return net_fail("stage");
+    if (ugk_event_record(N->ev[1], N->stream)) return net_fail("event");
+    us[k] = 1000.0 * ugk_event_ms(N->ev[0], N->ev[1]) / reps;
+  }
+  return AF_OK;
+}
+
 int ufem_net_time_stages(ufem_net *N, int reps, double *us)
 {
   int k, r;

@@ -331,7 +331,7 @@ static int solve_mixed(ugk_mat *Km, long n, const double *b, double *x, ufem_wor
   ugk_state *h = w->st_host;
   mx_ctx c = {K, n, b, x, w};
   int rv;
-  if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
+  if ((rv = ufem_mat_f32(Km, 0, stream)) != AF_OK) return rv;
   rv = prepare(K, w, stream, o->eps, 1, maxit);
   if (rv != AF_OK) return rv;
   if (ugk_to_f32(n, w->dinv, w->dinv32, stream)) return gpu_fail("fp32 diagonal");
@@ -462,7 +462,7 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
     if (!w->fm) return gpu_fail("direction rings");
   }
   if (mixed) {
-    if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
+    if ((rv = ufem_mat_f32(Km, o->f32_rem, stream)) != AF_OK) return rv;
   }
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
@@ -609,7 +609,7 @@ static int solve_twolevel(ugk_mat *Km, long n, const double *b, double *x, ufem_
   int rv;
   if (!w->cc) { ufem_set_error("two-level solver: no coarse grid (ufem_sys_set_coarse)"); return AF_ERR_VAL; }
   if (mixed) {
-    if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
+    if ((rv = ufem_mat_f32(Km, o->f32_rem, stream)) != AF_OK) return rv;
   }
   if ((rv = ufem_coarse_build(w->cc, stream)) != AF_OK) return rv;
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
@@ -667,7 +667,7 @@ static int solve_mixed64(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   mx64_ctx c = {K, n, b, x, w};
   ugk_state *h = w->st_host;
   int rv;
-  if ((rv = ufem_mat_f32(Km, stream)) != AF_OK) return rv;
+  if ((rv = ufem_mat_f32(Km, o->f32_rem, stream)) != AF_OK) return rv;
   rv = prepare(K, w, stream, o->eps, o->rule, maxit);
   if (rv != AF_OK) return rv;
   h->delta = o->inner_eps > 0 ? o->inner_eps : 0.1;

@@ -242,11 +242,19 @@ out:
 long ufem_sys_ndof(const ufem_sys *s) { return s->n; }
 long ufem_sys_nnz(const ufem_sys *s) { return s->K.nnzb * 9; }
 long ufem_sys_ncolors(const ufem_sys *s) { return s->n_colors; }
-int ufem_mat_f32(ugk_mat *K, void *stream)
+int ufem_mat_f32(ugk_mat *K, int rem, void *stream)
 {
   if (!K->d_vals32) {
     K->d_vals32 = (float *)ugk_malloc(sizeof(float) * (size_t)K->total_slots * 64 * K->bs * K->bs);
     if (!K->d_vals32) return gpu_fail("fp32 matrix");
+  }
+  if (!rem && K->d_rem16) {
+    ugk_free(K->d_rem16);
+    K->d_rem16 = NULL;
+  }
+  if (rem && !K->d_rem16) {
+    K->d_rem16 = (unsigned short *)ugk_malloc(sizeof(unsigned short) * (size_t)K->total_slots * 64 * K->bs * K->bs);
+    if (!K->d_rem16) return gpu_fail("fp32 matrix remainders");
   }
   if (K->d_corr32 && getenv("UFEM_NO_CORR32")) {   /* plain FP32 copy, for comparisons */
     ugk_free(K->d_corr32);
@@ -268,7 +276,7 @@ void ufem_sys_destroy(ufem_sys *s)
   if (!s) return;
   if (s->stream) ugk_sync(s->stream);
   ugk_free(s->K.d_slen); ugk_free(s->K.d_soff); ugk_free(s->K.d_rowlen);
-  ugk_free(s->K.d_cols); ugk_free(s->K.d_vals); ugk_free(s->K.d_vals32); ugk_free(s->K.d_corr32);
+  ugk_free(s->K.d_cols); ugk_free(s->K.d_vals); ugk_free(s->K.d_vals32); ugk_free(s->K.d_corr32); ugk_free(s->K.d_rem16);
   ugk_free(s->d_x); ugk_free(s->d_y); ugk_free(s->d_z); ugk_free(s->d_E); ugk_free(s->d_nu);
   ugk_free(s->d_enodes); ugk_free(s->d_elist); ugk_free(s->d_fixed);
   ugk_free(s->d_saved_rows); ugk_free(s->d_saved_cols); ugk_free(s->d_Rfix);
@@ -883,9 +891,9 @@ int ufem_sys_spmv(ufem_sys *s, const double *x, double *y)
   return AF_OK;
 }
 
-int ufem_sys_spmv_f32(ufem_sys *s, const double *x, double *y)
+int ufem_sys_spmv_f32(ufem_sys *s, const double *x, double *y, int rem)
 {
-  int rv = ufem_mat_f32(&s->K, s->stream);
+  int rv = ufem_mat_f32(&s->K, rem, s->stream);
   if (rv != AF_OK) return rv;
   if (ugk_h2d(s->w.z, x, sizeof(double) * (size_t)s->n, s->stream) ||
       ugk_spmv_f32x64(&s->K, s->w.z, s->w.q, s->stream) ||
@@ -933,7 +941,7 @@ int ufem_sys_time_spmv(ufem_sys *s, int reps, int mode, double *ms, double *byte
     return AF_ERR_VAL;
   }
   if (mode == 1 && !s->K.d_vals32) {
-    int rv = ufem_mat_f32(&s->K, s->stream);
+    int rv = ufem_mat_f32(&s->K, 0, s->stream);
     if (rv != AF_OK) return rv;
   }
   if (mode >= 2) {   /* a running state (after a solve) so the kernel does its work */

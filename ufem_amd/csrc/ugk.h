@@ -33,9 +33,12 @@ typedef struct {
   int *d_cols;        /* [total_slots * 64]                             */
   double *d_vals;     /* [total_slots * 64 * b*b]                       */
   float *d_vals32;    /* FP32 copy for mixed precision (may be NULL)    */
+  unsigned short *d_rem16;  /* bf16 of vals - vals32 at the same positions:
+                         with it the copy holds 48-bit values (products
+                         with FP64 vectors only; may be NULL)           */
   float *d_corr32;    /* b = 3: [9][n_brows] symmetric part of the block row
-                         sums of (vals - vals32), added to the FP32 copy's
-                         diagonal blocks (may be NULL)                  */
+                         sums of (vals - (vals32 + rem16)), added to the
+                         FP32 copy's diagonal blocks (may be NULL)      */
   unsigned long gen;  /* bumped whenever the pattern is (re)uploaded     */
 } ugk_mat;
 

@@ -307,6 +307,7 @@ struct SellView {
   const int *cols;
   long n_brows, n_slices;
   const float *corr;  // FP32 copy's diagonal correction [9][n_brows], or null
+  const unsigned short *rem;  // FP32 copy's bf16 remainders (same positions), or null
 };
 
 __device__ __forceinline__ int find_slot(const SellView &M, long I, long J)
@@ -345,6 +346,7 @@ __device__ __forceinline__ T ldnt(const T *p)
 template <typename T> struct Pair;
 template <> struct Pair<double> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct Pair<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct Pair<unsigned short> { typedef unsigned short type __attribute__((ext_vector_type(2))); };
 
 // the BS*BS values of lane `lane` in slot S, row-major (NT: non-temporal loads)
 template <int BS, bool NT, typename T>
@@ -738,6 +740,18 @@ __device__ __forceinline__ bool sell_row_split(const SellView &M, const T *__res
       for (int c = 0; c < BS; c++) xv[c] = x[J * BS + c];
       T v[BS * BS];
       ld_block<BS, true>(vals, so + s, lane, v);
+      if constexpr (sizeof(T) == 4 && sizeof(TX) == 8) {
+        if (M.rem) {   // 48-bit values: FP32 + bf16 remainder, summed exactly in FP64
+          unsigned short u[BS * BS];
+          ld_block<BS, true>(M.rem, so + s, lane, u);
+#pragma unroll
+          for (int r = 0; r < BS; r++)
+#pragma unroll
+            for (int c = 0; c < BS; c++)
+              acc[r] += ((TX)v[r * BS + c] + (TX)__uint_as_float((unsigned)u[r * BS + c] << 16)) * xv[c];
+          continue;
+        }
+      }
 #pragma unroll
       for (int r = 0; r < BS; r++)
 #pragma unroll
@@ -822,13 +836,27 @@ __global__ void k_vals_f32(long n, const double *__restrict__ a, T *__restrict__
     b[i] = (T)a[i];
 }
 
+// bf16 (round to nearest even) of v - fl32(v): with it the FP32 copy holds
+// 48-bit values (about 2e-10 relative)
+__global__ void k_vals_rem16(long n, const double *__restrict__ a, unsigned short *__restrict__ b)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float r = (float)(a[i] - (double)(float)a[i]);
+    unsigned u = __float_as_uint(r);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    b[i] = (unsigned short)(u >> 16);
+  }
+}
+
 // Diagonal correction of the FP32 copy of a 3x3-block matrix: for block row I,
-// C = sum over its blocks J of (K_IJ - fl32(K_IJ)), kept as (C + C^T) / 2 so
+// C = sum over its blocks J of (K_IJ - fl32(K_IJ)) (minus the bf16 remainder
+// rem, when the copy keeps one), kept as (C + C^T) / 2 so
 // the corrected matrix stays symmetric.  A stiffness matrix's block rows sum
 // to (nearly) zero: fl32 alone breaks that cancellation by about 1e-7 |K| |x|
 // on smooth fields, which stalls the true residual near 1e-3; with C the error
 // on a smooth field falls with its differences x_J - x_I instead.
-__global__ void k_corr32(SellView M, const double *__restrict__ vals, float *__restrict__ corr)
+__global__ void k_corr32(SellView M, const double *__restrict__ vals, const unsigned short *__restrict__ rem,
+                         float *__restrict__ corr)
 {
   const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= M.n_brows) return;
@@ -840,8 +868,9 @@ __global__ void k_corr32(SellView M, const double *__restrict__ vals, float *__r
   for (int s = 0; s < L; s++) {
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-      const double v = vals[vpos<3>(so + s, q, ln)];
-      c[q] += v - (double)(float)v;
+      const long at = vpos<3>(so + s, q, ln);
+      const double v = vals[at], hi = (double)(float)v;
+      c[q] += v - (rem ? hi + (double)__uint_as_float((unsigned)rem[at] << 16) : hi);
     }
   }
 #pragma unroll
@@ -2283,7 +2312,7 @@ __global__ __launch_bounds__(WG) void k_fcg_update(long n, double *__restrict__ 
 // (z, q_{k-1}) from the combine; W in region 7, (z, w) from the SpMV.
 // --------------------------------------------------------------------------
 #define F1_H(p) (p)
-#define F1_UB 4   /* entries per thread and pass of the update */
+#define F1_UP 2   /* entry pairs per thread and pass of the update */
 #define F1_Z(p) ((p) + 6L * 65536)
 #define F1_W(p) ((p) + 7L * 65536)
 
@@ -2742,28 +2771,57 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
   const double s = f1_scale(st->rdr, n), inv = 1.0 / s;
   double v[6] = {0, 0, 0, 0, 0, 0};
   long i1, i2;
-  xcd_chunk(n, i1, i2);
-  // F1_UB entries per thread and pass, all loads issued before the first
-  // use: one wave then keeps 7 F1_UB loads in flight, not 7
-  long i = i1 + threadIdx.x;
-  for (; i + (F1_UB - 1) * WG < i2; i += F1_UB * WG) {
-    double zz[F1_UB], pp[F1_UB], ww[F1_UB], qq[F1_UB], xx[F1_UB], rr[F1_UB], dd[F1_UB];
+  xcd_chunk_even(n, i1, i2);
+  // two entries per lane (16-byte accesses), F1_UP pairs per thread and
+  // pass with all loads issued before the first use
+  long i = i1 + 2 * threadIdx.x;
+  for (; i + 2L * WG * (F1_UP - 1) + 1 < i2; i += 2L * WG * F1_UP) {
+    dbl2 zz[F1_UP], pp[F1_UP], ww[F1_UP], qq[F1_UP], xx[F1_UP], rr[F1_UP], dd[F1_UP];
 #pragma unroll
-    for (int u = 0; u < F1_UB; u++) {
-      const long j = i + u * WG;
-      zz[u] = z[j];
-      ww[u] = w[j];
-      pp[u] = restart ? 0.0 : p[j];
-      qq[u] = restart ? 0.0 : q[j];
-      xx[u] = x[j];
-      rr[u] = r[j];
-      dd[u] = dinv[j];
+    for (int u = 0; u < F1_UP; u++) {
+      const long j = i + 2L * WG * u;
+      zz[u] = *(const dbl2 *)(z + j);
+      ww[u] = *(const dbl2 *)(w + j);
+      pp[u] = restart ? dbl2{0.0, 0.0} : *(const dbl2 *)(p + j);
+      qq[u] = restart ? dbl2{0.0, 0.0} : *(const dbl2 *)(q + j);
+      xx[u] = *(const dbl2 *)(x + j);
+      rr[u] = *(const dbl2 *)(r + j);
+      dd[u] = *(const dbl2 *)(dinv + j);
     }
 #pragma unroll
-    for (int u = 0; u < F1_UB; u++) {
-      const long j = i + u * WG;
-      const double pi = zz[u] + beta * pp[u], qi = ww[u] + beta * qq[u];
-      const double xi = xx[u] + alpha * pi, ri = rr[u] - alpha * qi, di = dd[u];
+    for (int u = 0; u < F1_UP; u++) {
+      const long j = i + 2L * WG * u;
+      const dbl2 pi = zz[u] + beta * pp[u], qi = ww[u] + beta * qq[u];
+      const dbl2 xi = xx[u] + alpha * pi, ri = rr[u] - alpha * qi, di = dd[u];
+      *(dbl2 *)(p + j) = pi;
+      *(dbl2 *)(q + j) = qi;
+      *(dbl2 *)(x + j) = xi;
+      *(dbl2 *)(r + j) = ri;
+      v[0] += di.x * ri.x * ri.x + di.y * ri.y * ri.y;
+      v[1] += ri.x * ri.x + ri.y * ri.y;
+      v[2] += xi.x * xi.x + xi.y * xi.y;
+      v[4] += pi.x * ri.x + pi.y * ri.y;
+      v[5] += pi.x * qi.x + pi.y * qi.y;
+      if (in) {
+        if (v64) {
+          const float2 m2 = *(const float2 *)(mask + j);
+          *(dbl2 *)(v64 + j) = dbl2{(double)m2.x * sqrt(di.x) * ri.x, (double)m2.y * sqrt(di.y) * ri.y};
+        } else {
+          f1_put_input(x0, j, (float)(sqrt(di.x) * ri.x * inv));
+          f1_put_input(x0, j + 1, (float)(sqrt(di.y) * ri.y * inv));
+        }
+      }
+    }
+  }
+  for (; i < i2; i += 2L * WG) {
+    const int m = i + 1 < i2 ? 2 : 1;
+    for (int c = 0; c < m; c++) {
+      const long j = i + c;
+      const double pi = restart ? z[j] : z[j] + beta * p[j];
+      const double qi = restart ? w[j] : w[j] + beta * q[j];
+      const double xi = x[j] + alpha * pi;
+      const double ri = r[j] - alpha * qi;
+      const double di = dinv[j];
       p[j] = pi;
       q[j] = qi;
       x[j] = xi;
@@ -2775,23 +2833,6 @@ __global__ __launch_bounds__(WG) void k_f1_update(long n, double *__restrict__ x
       v[5] += pi * qi;
       if (in) f1_input(x0, v64, mask, j, sqrt(di) * ri, inv);
     }
-  }
-  for (; i < i2; i += WG) {
-    const double pi = restart ? z[i] : z[i] + beta * p[i];
-    const double qi = restart ? w[i] : w[i] + beta * q[i];
-    const double xi = x[i] + alpha * pi;
-    const double ri = r[i] - alpha * qi;
-    const double di = dinv[i];
-    p[i] = pi;
-    q[i] = qi;
-    x[i] = xi;
-    r[i] = ri;
-    v[0] += di * ri * ri;
-    v[1] += ri * ri;
-    v[2] += xi * xi;
-    v[4] += pi * ri;
-    v[5] += pi * qi;
-    if (in) f1_input(x0, v64, mask, i, sqrt(di) * ri, inv);
   }
   block_partials<6>(v, F1_H(part));
   if (lead) {
@@ -3298,6 +3339,7 @@ static SellView view(const ugk_mat *K)
   M.n_brows = K->n_brows;
   M.n_slices = K->n_slices;
   M.corr = K->bs == 3 ? K->d_corr32 : nullptr;
+  M.rem = K->d_rem16;
   return M;
 }
 // --------------------------------------------------------------------------
@@ -3982,8 +4024,12 @@ int ugk_vals_to_f32(const ugk_mat *K, void *s)
   if (!K->d_vals32) { snprintf(g_err, sizeof(g_err), "no fp32 buffer"); return -1; }
   k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_vals32);
   CK_LAUNCH("k_vals_f32");
+  if (K->d_rem16) {
+    k_vals_rem16<<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_rem16);
+    CK_LAUNCH("k_vals_rem16");
+  }
   if (K->bs == 3 && K->d_corr32) {
-    k_corr32<<<(unsigned)((K->n_brows + WG - 1) / WG), WG, 0, S(s)>>>(view(K), K->d_vals, K->d_corr32);
+    k_corr32<<<(unsigned)((K->n_brows + WG - 1) / WG), WG, 0, S(s)>>>(view(K), K->d_vals, K->d_rem16, K->d_corr32);
     CK_LAUNCH("k_corr32");
   }
   return 0;
@@ -4589,7 +4635,8 @@ int ugk_f1_combine_c0t(long n, const double *r, const double *dinv, const float 
     return -1;
   }
   const long vc = (long)cd->gi.D * cd->gi.H * cd->gi.W;
-  const int npb = C0T_NODES;
+  int npb = C0T_NODES;
+  if (const char *e = getenv("UFEM_C0T_NODES")) npb = atoi(e) >= 16 ? atoi(e) / 16 * 16 : C0T_NODES;
   if ((vc + npb - 1) / npb > 65536 / 2) {
     snprintf(g_err, sizeof(g_err), "flexible CG: coarse level too large for the fused combine's partials");
     return -1;

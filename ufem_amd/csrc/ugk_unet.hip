@@ -1057,10 +1057,11 @@ __global__ void k_w64_frag(const bf16 *__restrict__ wf, int cop, int cip, int co
 }
 
 // y[c][j] = (gate[c][j] *) sum_t sum_k W[t][j][k] x[2 c + off_t][k], stride 2,
-// zero padding.  A block of 4 waves takes 16 coarse nodes and output tile
-// blockIdx.y; wave w the taps 7 w .. 7 w + 6, all its gathers and fragment
-// loads issued before its first matrix op (one latency, not 27), and the
-// four partial tiles meet in LDS in a fixed order.
+// zero padding.  A block of 4 waves and output tile blockIdx.y; with 16 or 32
+// input channels the block takes 16 coarse nodes and wave w the taps
+// 7 w .. 7 w + 6, all its gathers and fragment loads issued before its first
+// matrix op (one latency, not 27), the four partial tiles meeting in LDS in a
+// fixed order; with 3 (S = 1) each wave takes all 27 taps of its own 16 nodes.
 #define N64_TPW 7
 template <int S>
 __global__ __launch_bounds__(256) void k_n64_down(const double *__restrict__ x, int ldx, int ci, ugk_grid3 gi,
@@ -1068,19 +1069,22 @@ __global__ __launch_bounds__(256) void k_n64_down(const double *__restrict__ x, 
                                                   const float *__restrict__ gate, double *__restrict__ y,
                                                   const int *skip)
 {
-  __shared__ double part[3][4][64];
+  constexpr bool SPLIT = S > 1;              // the taps split over the 4 waves
+  constexpr int TPW = SPLIT ? N64_TPW : 27;
+  __shared__ double part[SPLIT ? 3 : 1][4][64];
   if (skip && *skip) return;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c0 = g * S;
   const unsigned HWo = (unsigned)go.H * go.W, Vo = (unsigned)go.D * HWo;
-  const unsigned node = blockIdx.x * 16 + (l & 15);
+  const unsigned base = SPLIT ? blockIdx.x * 16 : (blockIdx.x * 4 + w) * 16;
+  const unsigned node = base + (l & 15);
   const bool nok = node < Vo;
   const unsigned nn = nok ? node : 0;
   const int cz = (int)(nn / HWo), cr = (int)(nn - (unsigned)cz * HWo), cy = cr / go.W, cx = cr - cy * go.W;
   const double *fb = frag + (long)blockIdx.y * 27 * S * 64 + l;
-  double a[N64_TPW][S], b[N64_TPW][S];
+  double a[TPW][S], b[TPW][S];
 #pragma unroll
-  for (int u = 0; u < N64_TPW; u++) {
-    const int t = N64_TPW * w + u;
+  for (int u = 0; u < TPW; u++) {
+    const int t = SPLIT ? N64_TPW * w + u : u;
     const int fz = 2 * cz + t / 9 - 1, fy = 2 * cy + (t / 3) % 3 - 1, fx = 2 * cx + t % 3 - 1;
     const bool ok = nok && t < 27 && fz >= 0 && fz < gi.D && fy >= 0 && fy < gi.H && fx >= 0 && fx < gi.W;
     const double *xs = x + ((long)(fz * gi.H + fy) * gi.W + fx) * ldx + c0;
@@ -1092,21 +1096,23 @@ __global__ __launch_bounds__(256) void k_n64_down(const double *__restrict__ x, 
   }
   mf64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int u = 0; u < N64_TPW; u++)
+  for (int u = 0; u < TPW; u++)
 #pragma unroll
     for (int q = 0; q < S; q++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][q], b[u][q], acc, 0, 0, 0);
-  if (w) {
+  if constexpr (SPLIT) {
+    if (w) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) part[w - 1][v][l] = acc[v];
+      for (int v = 0; v < 4; v++) part[w - 1][v][l] = acc[v];
+    }
+    __syncthreads();
+    if (w) return;
+#pragma unroll
+    for (int v = 0; v < 4; v++) acc[v] = ((acc[v] + part[0][v][l]) + part[1][v][l]) + part[2][v][l];
   }
-  __syncthreads();
-  if (w) return;
-#pragma unroll
-  for (int v = 0; v < 4; v++) acc[v] = ((acc[v] + part[0][v][l]) + part[1][v][l]) + part[2][v][l];
   const int j = blockIdx.y * 16 + (l & 15);
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const unsigned nd = blockIdx.x * 16 + 4 * v + g;
+    const unsigned nd = base + 4 * v + g;
     if (nd < Vo && j < co) {
       const long e = (long)nd * co + j;
       y[e] = gate ? (double)gate[e] * acc[v] : acc[v];
@@ -1396,7 +1402,7 @@ int ugk_n64_down(const double *x, int ldx, int ci, ugk_grid3 gi, ugk_grid3 go, c
 {
   if (!n64_grid_ok(gi, go) || ldx < ci || (co != 16 && co != 32) || (ci != 3 && ci != 16 && ci != 32))
     return ugk_hip_fail("ugk_n64_down: channels or grids", hipErrorInvalidValue);
-  const dim3 G((unsigned)((vox(go) + 15) / 16), (unsigned)(co / 16));
+  const dim3 G((unsigned)((vox(go) + (ci == 3 ? 63 : 15)) / (ci == 3 ? 64 : 16)), (unsigned)(co / 16));
   if (ci == 3) k_n64_down<1><<<G, 256, 0, S(s)>>>(x, ldx, ci, gi, go, frag, co, gate, y, skip);
   else if (ci == 16) k_n64_down<4><<<G, 256, 0, S(s)>>>(x, ldx, ci, gi, go, frag, co, gate, y, skip);
   else k_n64_down<8><<<G, 256, 0, S(s)>>>(x, ldx, ci, gi, go, frag, co, gate, y, skip);
@@ -1426,5 +1432,6 @@ int ugk_n64_out(const double *x, int ci, ugk_grid3 gc, ugk_grid3 gf, const doubl
   CK_LAUNCH("k_n64_out");
   return 0;
 }
+
 
 }  // extern "C"

@@ -48,7 +48,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("solver", C.c_int), ("rule", C.c_int), ("eps", C.c_double), ("maxit", C.c_long),
                 ("exec", C.c_int), ("chunk", C.c_int), ("inner_eps", C.c_double),
                 ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int), ("fcg_m", C.c_int),
-                ("coarse", C.c_int), ("inner_restart", C.c_int)]
+                ("coarse", C.c_int), ("inner_restart", C.c_int), ("f32_rem", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -101,6 +101,7 @@ def lib():
             "ufem_net_collect": (C.c_long, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p]),
             "ufem_net_train": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_double, C.c_ulong, P_D, P_D]),
             "ufem_net_time_stages": (C.c_int, [C.c_void_p, C.c_int, P_D]),
+            "ufem_net_time_stages64": (C.c_int, [C.c_void_p, C.c_int, P_D]),
             "ufem_net_apply64": (C.c_int, [C.c_void_p, P_D, P_D]),
             "ufem_net_time_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P_I, P_I, C.c_int, C.c_int,
                                              C.c_int, P_D]),
@@ -119,7 +120,7 @@ def lib():
             "ufem_sys_get_F": (C.c_int, [C.c_void_p, P_D]),
             "ufem_sys_get_K": (C.c_int, [C.c_void_p, P_L, P_I, P_D]),
             "ufem_sys_spmv": (C.c_int, [C.c_void_p, P_D, P_D]),
-            "ufem_sys_spmv_f32": (C.c_int, [C.c_void_p, P_D, P_D]),
+            "ufem_sys_spmv_f32": (C.c_int, [C.c_void_p, P_D, P_D, C.c_int]),
             "ufem_sys_time_spmv": (C.c_int, [C.c_void_p, C.c_int, C.c_int, P_D, P_D]),
             "ufem_sys_timings": (None, [C.c_void_p, P_D]),
             "ufem_gpu_element_stiffness": (C.c_int, [C.c_long, P_D, P_D, P_D, C.c_int, P_D]),
@@ -303,10 +304,10 @@ class GpuSystem:
 
     def solve(self, solver="pcg", eps=1e-8, rule=1, maxit=0, exec="streamed", chunk=0,
               inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, fcg_m=0, coarse=False, check=True,
-              inner_restart=False):
+              inner_restart=False, f32_rem=False):
         o = SolveOpts(SOLVER[solver], rule, eps, maxit, EXEC[exec], chunk, inner_eps, inner_maxit,
                       net.h if net is not None else None, 1 if warm_start else 0, fcg_m, 1 if coarse else 0,
-                      1 if inner_restart else 0)
+                      1 if inner_restart else 0, 1 if f32_rem else 0)
         info = SolveInfo()
         rv = lib().ufem_sys_solve(self.h, C.byref(o), C.byref(info))
         if check and rv not in (AF_OK, AF_ERR):
@@ -385,11 +386,12 @@ class GpuSystem:
         self._chk(lib().ufem_sys_spmv(self.h, _pd(x), _pd(y)), "spmv")
         return y
 
-    def spmv_f32(self, x):
-        """y = K x with the mixed solvers' FP32 copy of K (FP64 vectors)"""
+    def spmv_f32(self, x, rem=False):
+        """y = K x with the mixed solvers' FP32 copy of K (FP64 vectors; rem: with
+        the bf16 remainders, 48-bit values)"""
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.n)
-        self._chk(lib().ufem_sys_spmv_f32(self.h, _pd(x), _pd(y)), "spmv_f32")
+        self._chk(lib().ufem_sys_spmv_f32(self.h, _pd(x), _pd(y), 1 if rem else 0), "spmv_f32")
         return y
 
     def time_kernel(self, mode="spmv", reps=20):
@@ -501,6 +503,13 @@ class GpuNet:
         names = self.stage_names()
         us = np.zeros(len(names))
         self._chk(lib().ufem_net_time_stages(self.h, reps, _pd(us)), "time_stages")
+        return dict(zip(names, us))
+
+    def time_stages64(self, reps=50):
+        """device us of each FP64 inference stage (the solver's path)"""
+        names = self.stage_names()
+        us = np.zeros(len(names))
+        self._chk(lib().ufem_net_time_stages64(self.h, reps, _pd(us)), "time_stages64")
         return dict(zip(names, us))
 
     def eval(self, i0, cnt):
