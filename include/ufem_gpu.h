@@ -188,7 +188,8 @@ int  ufem_sys_spmv(ufem_sys *s, const double *x, double *y);   /* host in, host 
  * (with rem, plus its remainder rounded to bf16: 48-bit values) and, for 3x3
  * node blocks, adds to each diagonal block the symmetric part of its block
  * row's summed rounding errors, so rigid translations, which a stiffness
- * matrix's rows cancel, stay cancelled; FP64 vectors and sums.
+ * matrix's rows cancel, stay cancelled; FP64 vectors and sums (the
+ * FP32-vector solver, UFEM_SOLVER_MIXED, uses the plain rounded values).
  * UFEM_NO_CORR32 set in the environment drops the correction. */
 int  ufem_sys_spmv_f32(ufem_sys *s, const double *x, double *y, int rem);
 /* average device time (HIP events on the system's stream) of `reps`

@@ -760,11 +760,12 @@ __device__ __forceinline__ bool sell_row_split(const SellView &M, const T *__res
   }
   row = sl * 64 + lane;
   if (half) {
-    if constexpr (BS == 3 && sizeof(T) == 4) {
-      // FP32 copy: add C_I x_I (C_I: the block row's summed rounding errors,
-      // symmetrised), which keeps rigid translations, where a stiffness
-      // matrix's rows cancel, cancelled.  The second-half wave has no more
-      // slots than the first, so it takes this term.
+    if constexpr (BS == 3 && sizeof(T) == 4 && sizeof(TX) == 8) {
+      // FP32 copy with FP64 vectors: add C_I x_I (C_I: the block row's summed
+      // rounding errors, symmetrised), which keeps rigid translations, where
+      // a stiffness matrix's rows cancel, cancelled (with FP32 vectors their
+      // own rounding dominates, and the term is left out).  The second-half
+      // wave has no more slots than the first, so it takes this term.
       if (M.corr && sl < M.n_slices && row < M.n_brows) {
         TX xi[3];
 #pragma unroll
