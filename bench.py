@@ -368,12 +368,8 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
     rr, tr, ur = timed(solver="learned_mixed", eps=1e-8, rule=1, net=net, fcg_m=1, f32_rem=True)
     # the same network applied on the bf16 matrix cores with f32 activations
     # (its rounding noise makes the iteration count depend on the load's scale)
-    os.environ["UFEM_NET_F32"] = "1"
-    try:
-        rf, tf, _ = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=1)
-        rfm, tfm, _ = timed(solver="learned_mixed", eps=1e-8, rule=1, net=net, fcg_m=1)
-    finally:
-        del os.environ["UFEM_NET_F32"]
+    rf, tf, _ = timed(solver="learned", eps=1e-8, rule=1, net=net, fcg_m=1, net_prec="f32")
+    rfm, tfm, _ = timed(solver="learned_mixed", eps=1e-8, rule=1, net=net, fcg_m=1, net_prec="f32")
     stages = net.time_stages64()
     stages32 = net.time_stages()
 
@@ -390,7 +386,8 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
         "coarse_mixed": dict(solver="learned_mixed", coarse=True),
         "learned_coarse": dict(solver="learned", net=net, fcg_m=1, coarse=True),
         "learned_coarse_mixed": dict(solver="learned_mixed", net=net, fcg_m=1, coarse=True),
-        "learned_coarse_mixed_rem": dict(solver="learned_mixed", net=net, fcg_m=1, coarse=True, f32_rem=True),
+        "learned_coarse_f64": dict(solver="learned", net=net, fcg_m=1, coarse=True, net_prec="f64"),
+        "learned_coarse_mixed_f64": dict(solver="learned_mixed", net=net, fcg_m=1, coarse=True, net_prec="f64"),
     }
     with_coarse = {"coarse_grid": {"factors": list(coarse), "coarse_dof": int(nc)}}
     for name, kw in variants.items():

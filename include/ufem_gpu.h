@@ -99,6 +99,14 @@ typedef struct {
                         keeps each value's remainder K - fl32(K) rounded to
                         bf16 (48-bit values, 6 bytes instead of 8); FP32
                         vectors (UFEM_SOLVER_MIXED) ignore it */
+  int net_prec;      /* learned: how the network is applied (coarse-first
+                        form, 16 or 32 channels): 0 FP64 activations, or,
+                        with the two-level correction, f32; 1 FP64; 2 f32
+                        activations on the bf16 matrix cores (the training
+                        path).  Flexible CG loses iterations to f32 rounding
+                        noise in the preconditioner, which FP64 removes at a
+                        higher cost per iteration; with the coarse term the
+                        solve is short and the noise costs little */
 } ufem_solve_opts;
 
 typedef struct {

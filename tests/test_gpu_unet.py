@@ -372,22 +372,21 @@ def test_fused_last_layer_matches_the_network_stage(beam, solver, width, monkeyp
     stage followed by the plain combine: x after a few iterations agrees to
     f32 rounding (a wrong tap, weight or parity would move it at O(1)), and
     both reach the same solution.  Both on the matrix-core path
-    (UFEM_NET_F32)."""
+    (net_prec f32)."""
     m, s, ref, u_ref, (W, H, D) = beam
     net = G.GpuNet(W, H, D, levels=4, fine=False, width=width)   # C0 with 32 or 16 channels
     net.init(5)
     net.bind(s, m.E)
-    monkeypatch.setenv("UFEM_NET_F32", "1")
     for maxit in (3, 11):
         monkeypatch.setenv("UFEM_NO_FUSED_C0T", "1")
-        a = s.solve(solver, eps=1e-30, maxit=maxit, net=net, check=False)
+        a = s.solve(solver, eps=1e-30, maxit=maxit, net=net, check=False, net_prec="f32")
         ua = s.u()
         monkeypatch.delenv("UFEM_NO_FUSED_C0T")
-        b = s.solve(solver, eps=1e-30, maxit=maxit, net=net, check=False)
+        b = s.solve(solver, eps=1e-30, maxit=maxit, net=net, check=False, net_prec="f32")
         ub = s.u()
         assert a["iters"] == b["iters"] == maxit and not a["fallback"] and not b["fallback"]
         assert rel(ub, ua) <= 1e-5, (maxit, rel(ub, ua))
-    r = s.solve(solver, eps=1e-8, net=net)
+    r = s.solve(solver, eps=1e-8, net=net, net_prec="f32")
     assert r["rv"] == 0 and not r["fallback"] and r["true_relres"] <= 1e-8 * (1 + 1e-9)
     assert rel(s.u(), u_ref) <= 1e-6
 

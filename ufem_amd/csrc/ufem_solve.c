@@ -478,8 +478,8 @@ static int solve_learned(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   if (cc && ufem_coarse_build(w->cc, stream) != AF_OK) return AF_ERR;
   if (m == 1) {
     c.fused = N && !getenv("UFEM_NO_FUSED_C0T") && ufem_net_last_stage(N, &c.last);
-    /* FP64 inference unless asked for the matrix-core path (UFEM_NET_F32) */
-    c.f64 = c.fused && !getenv("UFEM_NET_F32") && ufem_net_f64_ok(N);
+    /* FP64 inference: asked for, or by default without the coarse term */
+    c.f64 = c.fused && ufem_net_f64_ok(N) && (o->net_prec == 1 || (o->net_prec == 0 && !cc));
     if (c.f64 && ((rv = ufem_net_prepare64(N, stream)) != AF_OK || !ufem_net_last64(N, &c.last)))
       return rv != AF_OK ? rv : AF_ERR;
     if (ugk_f1_start(K, n, b, x, w->r, w->dinv, w->st, w->part, stream) ||

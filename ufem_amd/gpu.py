@@ -48,7 +48,8 @@ class SolveOpts(C.Structure):
     _fields_ = [("solver", C.c_int), ("rule", C.c_int), ("eps", C.c_double), ("maxit", C.c_long),
                 ("exec", C.c_int), ("chunk", C.c_int), ("inner_eps", C.c_double),
                 ("inner_maxit", C.c_long), ("net", C.c_void_p), ("warm_start", C.c_int), ("fcg_m", C.c_int),
-                ("coarse", C.c_int), ("inner_restart", C.c_int), ("f32_rem", C.c_int)]
+                ("coarse", C.c_int), ("inner_restart", C.c_int), ("f32_rem", C.c_int),
+                ("net_prec", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -304,10 +305,10 @@ class GpuSystem:
 
     def solve(self, solver="pcg", eps=1e-8, rule=1, maxit=0, exec="streamed", chunk=0,
               inner_eps=0.0, inner_maxit=0, warm_start=False, net=None, fcg_m=0, coarse=False, check=True,
-              inner_restart=False, f32_rem=False):
+              inner_restart=False, f32_rem=False, net_prec=0):
         o = SolveOpts(SOLVER[solver], rule, eps, maxit, EXEC[exec], chunk, inner_eps, inner_maxit,
                       net.h if net is not None else None, 1 if warm_start else 0, fcg_m, 1 if coarse else 0,
-                      1 if inner_restart else 0, 1 if f32_rem else 0)
+                      1 if inner_restart else 0, 1 if f32_rem else 0, {0: 0, "auto": 0, "f64": 1, "f32": 2}[net_prec])
         info = SolveInfo()
         rv = lib().ufem_sys_solve(self.h, C.byref(o), C.byref(info))
         if check and rv not in (AF_OK, AF_ERR):
