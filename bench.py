@@ -417,6 +417,8 @@ def bench_learned_mixed(g, mesh, fields, steps, seed_base=5000, cov=0.3, coarse=
                               "fallback": rr["fallback"], "true_relres": rr["true_relres"],
                               "u_rel_err_vs_fp64": err(ur)},
         "iteration_ratio": round(rp["iters"] / max(rm["iters"], 1), 3),
+        "fastest_learned_mixed": ({"form": "learned_mixed_rem", "speedup_vs_plain": round(tp / tr, 3)} if tr <= tm
+                                  else {"form": "learned_mixed", "speedup_vs_plain": round(tp / tm, 3)}),
         "speedup_vs_plain": {"learned": round(tp / tl, 3), "learned_mixed": round(tp / tm, 3),
                              "learned_mixed_rem": round(tp / tr, 3)},
         "us_per_iteration": {"plain": round(tp / rp["iters"] * 1e6, 1), "learned": round(tl / rl["iters"] * 1e6, 1),
