@@ -2496,8 +2496,9 @@ __global__ __launch_bounds__(WG) void k_f1_combine(long n, const double *__restr
 }
 
 // The same combine with the network's last layer in it: y = C0^T t0, the
-// stride-2 transpose from coarse level 0 (CIN = 16 or 32 f32 channels) to the node grid,
-// 3 outputs, on the vector ALUs in f32.  A node f = 2 c + p of parity class
+// stride-2 transpose from coarse level 0 (CIN = 16 or 32 channels) to the
+// node grid, 3 outputs, on the vector ALUs in the activations' type TA (f32,
+// the input scaled by st->net_s; or FP64, unscaled).  A node f = 2 c + p of parity class
 // p takes, per axis, the coarse node c (offset 0) when p = 0, and c (offset
 // +1) and c + 1 (offset -1, when it exists) when p = 1: 1 to 8 taps.  A
 // block of 8 waves takes C0T_NODES coarse nodes, wave w the nodes of class w, in
@@ -2506,13 +2507,14 @@ __global__ __launch_bounds__(WG) void k_f1_combine(long n, const double *__restr
 // its gathers and its epilogue loads in flight together; the four partial
 // sums meet by lane shuffles in a fixed order, and lane q < 3 forms z for
 // DOF 3 f + q.  The weights ([27][cop][cip] bf16, the adjoint's layout) are
-// widened to f32 in LDS as [27][3][CIN].  Replaces the last network stage
+// widened to TA in LDS as [27][3][CIN].  Replaces the last network stage
 // and k_f1_combine.
 #define C0T_CIN 32          /* the largest CIN */
 #define C0T_TB 512
 #define C0T_NODES 128     /* coarse nodes per block (a multiple of 16): fewer blocks
                              pay the start (partials, weights) fewer times; 64:
-                             +1.5 us per iteration at 1M DOF, 16: +24 us */
+                             +1.5 us per iteration at 1M DOF, 16: +24 us; 256
+                             (FP64 t0): +11 us */
 
 template <int PZ, int PY, int PX, int CIN>
 __device__ __forceinline__ void c0t_class(const float *__restrict__ x, int ldx, const float *sw, ugk_grid3 gc, int cz,
@@ -4636,8 +4638,7 @@ int ugk_f1_combine_c0t(long n, const double *r, const double *dinv, const float 
     return -1;
   }
   const long vc = (long)cd->gi.D * cd->gi.H * cd->gi.W;
-  int npb = C0T_NODES;
-  if (const char *e = getenv("UFEM_C0T_NODES")) npb = atoi(e) >= 16 ? atoi(e) / 16 * 16 : C0T_NODES;
+  const int npb = C0T_NODES;
   if ((vc + npb - 1) / npb > 65536 / 2) {
     snprintf(g_err, sizeof(g_err), "flexible CG: coarse level too large for the fused combine's partials");
     return -1;
