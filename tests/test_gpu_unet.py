@@ -535,3 +535,21 @@ def test_learned_mixed_with_remainders_tracks_fp64(beam):
     print("learned", r64["iters"], "learned + mixed, 48-bit K", rm["iters"], "FP32 K", r32["iters"])
     assert rm["iters"] <= 1.1 * r64["iters"] + 5
     assert rm["iters"] <= 0.8 * r32["iters"]
+
+
+def test_net_prec_option_and_stage_timer(beam):
+    """net_prec: FP64 inference by default without the coarse term (the same
+    iterates as asking for it), f32 on request (the same solution); the FP64
+    stage timer reports every stage"""
+    m, s, ref, u_ref, (W, H, D) = beam
+    net = G.GpuNet(W, H, D, levels=6, fine=False, width=16)
+    net.init(3)
+    net.bind(s, m.E)
+    a = s.solve("learned", eps=1e-8, net=net)
+    ua = s.u()
+    b = s.solve("learned", eps=1e-8, net=net, net_prec="f64")
+    assert a["iters"] == b["iters"] and np.array_equal(ua, s.u())
+    c = s.solve("learned", eps=1e-8, net=net, net_prec="f32")
+    assert c["rv"] == 0 and not c["fallback"] and rel(s.u(), u_ref) <= 1e-6
+    us = net.time_stages64(reps=3)
+    assert len(us) == 12 and all(v > 0.0 for v in us.values())
