@@ -1258,8 +1258,6 @@ out:
   return rv;
 }
 
-/* average device time (us) of each inference stage on the bound system's
- * grid (us[0..5], stages as in infer_stage), `reps` launches each */
 /* stage k of FP64 inference: C0 .. C_{L-1} (k < L), C_{L-1}^T .. C1^T, and
  * C0^T on its own (k = 2 L - 1; the solver folds it into its combine) */
 static int infer64_stage(ufem_net *N, int k)
@@ -1299,6 +1297,8 @@ int ufem_net_time_stages64(ufem_net *N, int reps, double *us)
   return AF_OK;
 }
 
+/* average device time (us) of each inference stage on the bound system's
+ * grid (us[0 .. 2 L - 1], stages as in infer_stage), `reps` launches each */
 int ufem_net_time_stages(ufem_net *N, int reps, double *us)
 {
   int k, r;
