@@ -485,6 +485,25 @@ def bench_batch(n_samples=256, seed=7, sequential_sample=16):
                      "peak": HBM_PEAK_GBPS, "frac": round(gbps / HBM_PEAK_GBPS, 4),
                      "bytes_per_sample_iteration": "76 nnzb + 112 n"},
     }
+    # mixed: FP32 copies of every sample's K (with the row-sum correction;
+    # mixed48: plus bf16 remainders), FP64 vectors and residual refreshes,
+    # each with its own autotuned shape
+    check = idx[:4]
+    u64 = [b.u(int(i)) for i in check]
+    for name, mode in (("mixed", 1), ("mixed48", 2)):
+        b.set_mixed(mode)
+        b.autotune(None, 100)
+        b.solve(eps=1e-8)
+        rm = b.solve(eps=1e-8)
+        uerr = max(float(np.linalg.norm(b.u(int(i)) - u) / np.linalg.norm(u)) for i, u in zip(check, u64))
+        out[name] = {
+            "total_ms": round(rm["total_ms"], 2), "samples_per_s": round(n_samples / (rm["total_ms"] / 1e3), 2),
+            "vs_fp64_batch": round(r["total_ms"] / rm["total_ms"], 3),
+            "all_converged": bool(np.all(rm["status"] == 1)),
+            "iters_vs_fp64": round(float(rm["iters"].sum()) / float(r["iters"].sum()), 3),
+            "u_rel_err_vs_fp64_max": uerr,
+            "autotune": {"wg": rm["wg"], "grid": rm["grid"], "team_big": rm["team_big"]},
+        }
     b.close()
     return out
 

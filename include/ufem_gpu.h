@@ -285,6 +285,15 @@ int  ufem_batch_teams(const ufem_batch *b, int *team);
 void ufem_batch_set_limits(ufem_batch *b, double wait_ms, double timeout_ms);
 /* stop a running ufem_batch_solve (call from another host thread) */
 void ufem_batch_abort(ufem_batch *b);
+/* no counterpart in the reference: mode 1 solves every sample with FP64
+ * vectors over an FP32 copy of its K (each diagonal 3x3 block carries its
+ * block row's summed rounding errors, so rigid translations stay exact),
+ * mode 2 with the bf16 remainder K - fl32(K) beside each value (48-bit
+ * values), mode 0 (the default) with K in FP64;
+ * the residual is recomputed in FP64 (r = f - K x, the direction kept) when it
+ * has fallen 100-fold, every 500 iterations and before a sample is declared
+ * converged.  The copies are built at the next autotune or solve. */
+int  ufem_batch_set_mixed(ufem_batch *b, int mode);
 /* solve all samples; per-sample outputs may be NULL; maxit <= 0: largest n.
  * sample_ms: device wall-clock time from a sample's start to its end.
  * status per sample: 1 converged, 2 breakdown, 3 maxit, 4 stopped (a team

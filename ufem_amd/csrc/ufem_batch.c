@@ -30,6 +30,10 @@ struct ufem_batch {
   int *cls_of;            /* sample -> class */
   double **E_host, **nu_host;
   double **d_vals, **d_dinv, **d_vec;   /* per sample: K values, 1/diag, 6 vectors */
+  float **d_vals32, **d_corr32;         /* per sample, mixed: FP32 copy and its correction */
+  unsigned short **d_rem16;             /* per sample, mixed 2: bf16 remainders of the copy */
+  int mixed;              /* 0 FP64 K; 1 the FP32 copies; 2 FP32 + bf16 remainders (48-bit) */
+  int mixed_ready;        /* the mode the sample table holds copies for */
   ugk_bsample *h_samples, *d_samples;
   int *d_slots;           /* 2 ints per slot: sample, (team << 8) | rank */
   unsigned *d_ticket, *d_tctr;
@@ -78,12 +82,16 @@ void ufem_batch_destroy(ufem_batch *b)
     if (b->d_vals) ugk_free(b->d_vals[i]);
     if (b->d_dinv) ugk_free(b->d_dinv[i]);
     if (b->d_vec) ugk_free(b->d_vec[i]);
+    if (b->d_vals32) ugk_free(b->d_vals32[i]);
+    if (b->d_corr32) ugk_free(b->d_corr32[i]);
+    if (b->d_rem16) ugk_free(b->d_rem16[i]);
     if (b->E_host) free(b->E_host[i]);
     if (b->nu_host) free(b->nu_host[i]);
   }
   for (i = 0; i < b->n_cls; i++) ufem_sys_destroy(b->cls[i].sys);
   free(b->cls); free(b->cls_of); free(b->E_host); free(b->nu_host);
   free(b->d_vals); free(b->d_dinv); free(b->d_vec); free(b->h_samples); free(b->h_res);
+  free(b->d_vals32); free(b->d_corr32); free(b->d_rem16);
   ugk_free(b->d_samples); ugk_free(b->d_slots); ugk_free(b->d_ticket); ugk_free(b->d_tctr);
   ugk_free(b->d_tpart); ugk_free(b->d_res);
   ugk_event_destroy(b->ev0);
@@ -124,10 +132,13 @@ ufem_batch *ufem_batch_create(int n, const ufem_mesh *meshes)
   b->d_vals = (double **)calloc((size_t)n, sizeof(double *));
   b->d_dinv = (double **)calloc((size_t)n, sizeof(double *));
   b->d_vec = (double **)calloc((size_t)n, sizeof(double *));
+  b->d_vals32 = (float **)calloc((size_t)n, sizeof(float *));
+  b->d_corr32 = (float **)calloc((size_t)n, sizeof(float *));
+  b->d_rem16 = (unsigned short **)calloc((size_t)n, sizeof(unsigned short *));
   b->h_samples = (ugk_bsample *)calloc((size_t)n, sizeof(ugk_bsample));
   b->h_res = (ugk_bresult *)calloc((size_t)n, sizeof(ugk_bresult));
   if (!b->cls || !b->cls_of || !b->E_host || !b->nu_host || !b->d_vals || !b->d_dinv || !b->d_vec ||
-      !b->h_samples || !b->h_res)
+      !b->d_vals32 || !b->d_corr32 || !b->d_rem16 || !b->h_samples || !b->h_res)
     goto fail;
   for (i = 0; i < n; i++) {
     for (c = 0; c < b->n_cls; c++)
@@ -246,6 +257,9 @@ int ufem_batch_assemble(ufem_batch *b, int npts, double *ms)
     q->n_brows = S->K.n_brows;
     q->n_slices = S->K.n_slices;
     q->vals = b->d_vals[i];
+    q->vals32 = NULL;   /* FP32 copies follow at the next solve in mixed mode */
+    q->corr32 = NULL;
+    q->rem16 = NULL;
     q->dinv = b->d_dinv[i];
     q->f = S->d_F;
     q->x = v;
@@ -258,6 +272,53 @@ int ufem_batch_assemble(ufem_batch *b, int npts, double *ms)
   if (ugk_h2d(b->d_samples, b->h_samples, sizeof(ugk_bsample) * (size_t)b->n, b->stream) || ugk_sync(b->stream))
     return gpu_fail("sample table");
   b->assembled = 1;
+  b->mixed_ready = 0;
+  return AF_OK;
+}
+
+/* the sample table's FP32 copies: made from the assembled values when the
+ * batch is in mixed mode, left out otherwise */
+static int mixed_table(ufem_batch *b)
+{
+  int i;
+  const int want = b->mixed;
+  if (want == b->mixed_ready) return AF_OK;
+  for (i = 0; i < b->n; i++) {
+    ugk_bsample *q = &b->h_samples[i];
+    q->vals32 = NULL;
+    q->corr32 = NULL;
+    q->rem16 = NULL;
+    if (want) {
+      const ufem_sys *S = b->cls[b->cls_of[i]].sys;
+      ugk_mat K = S->K;
+      if (!b->d_vals32[i]) b->d_vals32[i] = (float *)ugk_malloc(sizeof(float) * (size_t)K.total_slots * 64 * 9);
+      if (!b->d_corr32[i]) b->d_corr32[i] = (float *)ugk_malloc(sizeof(float) * 9 * (size_t)K.n_brows);
+      if (want == 2 && !b->d_rem16[i])
+        b->d_rem16[i] = (unsigned short *)ugk_malloc(sizeof(unsigned short) * (size_t)K.total_slots * 64 * 9);
+      if (!b->d_vals32[i] || !b->d_corr32[i] || (want == 2 && !b->d_rem16[i])) return gpu_fail("fp32 copies");
+      K.d_vals = b->d_vals[i];
+      K.d_vals32 = b->d_vals32[i];
+      K.d_corr32 = b->d_corr32[i];
+      K.d_rem16 = want == 2 ? b->d_rem16[i] : NULL;
+      if (ugk_vals_to_f32(&K, b->stream)) return gpu_fail("fp32 copies");
+      q->vals32 = b->d_vals32[i];
+      q->corr32 = b->d_corr32[i];
+      q->rem16 = K.d_rem16;
+    }
+  }
+  if (ugk_h2d(b->d_samples, b->h_samples, sizeof(ugk_bsample) * (size_t)b->n, b->stream) || ugk_sync(b->stream))
+    return gpu_fail("sample table");
+  b->mixed_ready = want;
+  return AF_OK;
+}
+
+int ufem_batch_set_mixed(ufem_batch *b, int mode)
+{
+  if (mode < 0 || mode > 2) {
+    ufem_set_error("batch mixed mode %d: 0 (FP64), 1 (FP32 copy) or 2 (FP32 + bf16 remainder)", mode);
+    return AF_ERR_VAL;
+  }
+  b->mixed = mode;
   return AF_OK;
 }
 
@@ -376,9 +437,11 @@ int ufem_batch_autotune(ufem_batch *b, const char *cache, int budget_iters, int 
   long hbm = 0, rows = 0;
   double best = 1e30, ms;
   if (!b->assembled) { ufem_set_error("assemble first"); return AF_ERR; }
+  if (mixed_table(b) != AF_OK) return AF_ERR;
   if (ugk_device_info(&ncu, &hbm, dev, sizeof(dev))) return gpu_fail("device");
   for (i = 0; i < b->n; i++) rows += b->h_samples[i].n_brows;
-  snprintf(key, sizeof(key), "%s|n=%d|rows=%ld|cls=%d|budget=%d", dev, b->n, rows, b->n_cls, budget_iters);
+  snprintf(key, sizeof(key), "%s|n=%d|rows=%ld|cls=%d|budget=%d%s", dev, b->n, rows, b->n_cls, budget_iters,
+           b->mixed == 2 ? "|mixed48" : (b->mixed ? "|mixed" : ""));
   if (cache) {
     FILE *f = fopen(cache, "r");
     char line[512];
@@ -434,6 +497,7 @@ int ufem_batch_solve(ufem_batch *b, double eps, long maxit, double *total_ms, lo
   int i, wg = b->wg ? b->wg : 512, grid = b->grid;
   long mx = maxit;
   if (!b->assembled) { ufem_set_error("assemble first"); return AF_ERR; }
+  if (mixed_table(b) != AF_OK) return AF_ERR;
   if (mx <= 0) {
     for (i = 0; i < b->n; i++) if (3 * b->h_samples[i].n_brows > mx) mx = 3 * b->h_samples[i].n_brows;
   }

@@ -262,6 +262,9 @@ typedef struct {
   const double *f;
   double *x, *r, *u, *p, *s, *w;
   long maxit;         /* per-sample iteration cap (0: the launch's) */
+  const float *vals32;  /* mixed: the FP32 copy of vals (NULL: FP64 products) */
+  const float *corr32;  /* mixed: its row-sum correction [9][n_brows] */
+  const unsigned short *rem16;  /* mixed, may be NULL: bf16 of vals - vals32 */
 } ugk_bsample;
 
 typedef struct {
@@ -283,7 +286,14 @@ typedef struct {
  * the device wall clock (<= 0: no practical limit) or when the int at
  * d_abort (mapped host memory, may be NULL) becomes nonzero; the sample then
  * reports status 4, and no workgroup takes a new sample after a stop.
- * d_ticket holds 2 zeroed words (ticket, device stop flag). */
+ * d_ticket holds 2 zeroed words (ticket, device stop flag).  A sample with
+ * vals32 takes w = K u from the FP32 copy (plus rem16 when set, and its
+ * correction), every
+ * vector in FP64, and replaces r by f - K x in FP64 (keeping the direction:
+ * reliable update) each time ||r|| has fallen by 10 since the last such
+ * refresh or after 500 iterations; it converges only on a refreshed
+ * residual, and a breakdown of the recurrence (p, K p) <= 0 restarts it
+ * along dinv r from a refreshed residual before it counts as status 2. */
 int ugk_batch_pcg(const ugk_bsample *d_samples, const int *d_slots, int n_slots, unsigned *d_ticket,
                   unsigned *d_tctr, double *d_tpart, ugk_bresult *d_res, double tol, long maxit,
                   int wg, int grid, const int *d_abort, long wait_ticks, void *stream);

@@ -1860,6 +1860,34 @@ __device__ __forceinline__ bool team_sum(double (&v)[NV], double (*red)[NV], int
   return true;
 }
 
+// the three sums of a block row over its slots for vector v (FP64 or FP32
+// values; REM: FP32 plus the bf16 remainder at the same position, summed
+// exactly in FP64)
+template <typename T, bool REM>
+__device__ __forceinline__ void batch_row(const T *__restrict__ vals, const unsigned short *__restrict__ rem,
+                                          const double *__restrict__ v, long so, int L,
+                                          const int *__restrict__ cp, int lane, double &a0, double &a1, double &a2)
+{
+  for (int s = 0; s < L; s++) {
+    const long J = ldnt(cp + (long)s * 64);
+    const double x0 = v[3 * J], x1 = v[3 * J + 1], x2 = v[3 * J + 2];
+    T vv[9];
+    ld_block<3, true>(vals, so + s, lane, vv);
+    double d[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) d[q] = (double)vv[q];
+    if constexpr (REM) {
+      unsigned short u[9];
+      ld_block<3, true>(rem, so + s, lane, u);
+#pragma unroll
+      for (int q = 0; q < 9; q++) d[q] += (double)__uint_as_float((unsigned)u[q] << 16);
+    }
+    a0 += d[0] * x0 + d[1] * x1 + d[2] * x2;
+    a1 += d[3] * x0 + d[4] * x1 + d[5] * x2;
+    a2 += d[6] * x0 + d[7] * x1 + d[8] * x2;
+  }
+}
+
 template <int WGS>
 __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict__ samples,
                                                    const int2 *__restrict__ slots, int nslots,
@@ -1909,7 +1937,13 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
     const bool formed = team_sum<WGS, 3>(v3, red3, T, rank, tp, ctr, nbar, par, bs);   // also publishes u
     double gam = v3[0], rr = v3[1], gam_old = 1.0, alf_old = 1.0;
     const double nb = sqrt(rr);
-    long it = 0;
+    const bool mixed = S.vals32 != nullptr;
+    double rr_ref = rr;        // mixed: ||r||^2 of the last FP64 residual
+    double den_old = 0.0;      // (p, s) of the last direction
+    long it = 0, it_ref = 0;
+    bool exact = true;         // r is an FP64 residual f - K x (the start's is)
+    bool refreshed = false;    // r was replaced since the last update
+    bool restart = false;      // mixed: the next direction starts afresh (beta = 0)
     if (!formed) status = 4;
     while (status == 0) {
       if ((it & 15) == 15) {           // a stop asked for ends a sample too
@@ -1920,38 +1954,93 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
         __syncthreads();
         if (stop) { status = 4; break; }
       }
-      if (!(nb > 0.0) || sqrt(rr) <= tol * nb) { status = 1; break; }
+      const bool conv = !(nb > 0.0) || sqrt(rr) <= tol * nb;
+      if (mixed && !exact && (conv || restart || rr < 0.01 * rr_ref || it - it_ref >= 500)) {
+        // FP64 refresh: r = f - K x, u = dinv r, (r, u), (r, r); the
+        // direction is kept (reliable update)
+        double v3r[3] = {0, 0, 0};
+        for (long sl = w0; sl < S.n_slices; sl += wstep) {
+          const long so = S.soff[sl];
+          double a[3] = {0.0, 0.0, 0.0};
+          batch_row<double, false>(S.vals, nullptr, S.x, so, S.slen[sl], S.cols + so * 64 + lane, lane, a[0], a[1],
+                                   a[2]);
+          const long row = sl * 64 + lane;
+          if (row < S.n_brows) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              const long i = 3 * row + c;
+              const double ri = S.f[i] - a[c], ui = ri * S.dinv[i];
+              S.r[i] = ri;
+              S.u[i] = ui;
+              v3r[0] += ri * ui;
+              v3r[1] += ri * ri;
+            }
+          }
+        }
+        if (!team_sum<WGS, 3>(v3r, red3, T, rank, tp, ctr, nbar, par, bs)) { status = 4; break; }   // publishes r, u
+        gam = v3r[0];
+        rr = v3r[1];
+        rr_ref = rr;
+        it_ref = it;
+        exact = true;
+        refreshed = true;
+        continue;
+      }
+      if (conv) { status = 1; break; }
       if (!(gam == gam) || !(rr == rr)) { status = 2; break; }
       if (it >= mxit) { status = 3; break; }
-      double v1[1] = {0.0};
+      double v3s[3] = {0.0, 0.0, 0.0};   // (w, u) [, (u, s) after a refresh]
       for (long sl = w0; sl < S.n_slices; sl += wstep) {
         const int L = S.slen[sl];
         const long so = S.soff[sl];
         const int *cp = S.cols + so * 64 + lane;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int s = 0; s < L; s++) {
-          const long J = ldnt(cp + (long)s * 64);
-          const double x0 = S.u[3 * J], x1 = S.u[3 * J + 1], x2 = S.u[3 * J + 2];
-          double vv[9];
-          ld_block<3, true>(S.vals, so + s, lane, vv);
-          a0 += vv[0] * x0 + vv[1] * x1 + vv[2] * x2;
-          a1 += vv[3] * x0 + vv[4] * x1 + vv[5] * x2;
-          a2 += vv[6] * x0 + vv[7] * x1 + vv[8] * x2;
-        }
+        if (!mixed) batch_row<double, false>(S.vals, nullptr, S.u, so, L, cp, lane, a0, a1, a2);
+        else if (S.rem16) batch_row<float, true>(S.vals32, S.rem16, S.u, so, L, cp, lane, a0, a1, a2);
+        else batch_row<float, false>(S.vals32, nullptr, S.u, so, L, cp, lane, a0, a1, a2);
         const long row = sl * 64 + lane;
         if (row < S.n_brows) {
+          const double u0 = S.u[3 * row], u1 = S.u[3 * row + 1], u2 = S.u[3 * row + 2];
+          if (mixed) {   // the FP32 copy's row-sum correction: rigid translations stay exact
+            const float *C = S.corr32;
+            const long nbr = S.n_brows;
+            a0 += (double)C[row] * u0 + (double)C[nbr + row] * u1 + (double)C[2 * nbr + row] * u2;
+            a1 += (double)C[3 * nbr + row] * u0 + (double)C[4 * nbr + row] * u1 + (double)C[5 * nbr + row] * u2;
+            a2 += (double)C[6 * nbr + row] * u0 + (double)C[7 * nbr + row] * u1 + (double)C[8 * nbr + row] * u2;
+          }
           S.w[3 * row] = a0;
           S.w[3 * row + 1] = a1;
           S.w[3 * row + 2] = a2;
-          v1[0] += a0 * S.u[3 * row] + a1 * S.u[3 * row + 1] + a2 * S.u[3 * row + 2];
+          v3s[0] += a0 * u0 + a1 * u1 + a2 * u2;
+          if (refreshed) v3s[1] += u0 * S.s[3 * row] + u1 * S.s[3 * row + 1] + u2 * S.s[3 * row + 2];
         }
       }
-      if (!team_sum<WGS, 1>(v1, red1, T, rank, tp, ctr, nbar, par, bs)) { status = 4; break; }   // publishes w
-      const double dlt = v1[0];
-      const double beta = (it == 0) ? 0.0 : gam / gam_old;
-      const double den = (it == 0) ? dlt : dlt - beta * gam / alf_old;
+      if (refreshed) {
+        if (!team_sum<WGS, 3>(v3s, red3, T, rank, tp, ctr, nbar, par, bs)) { status = 4; break; }   // publishes w
+      } else {
+        double v1[1] = {v3s[0]};
+        if (!team_sum<WGS, 1>(v1, red1, T, rank, tp, ctr, nbar, par, bs)) { status = 4; break; }
+        v3s[0] = v1[0];
+      }
+      const double dlt = v3s[0];
+      const bool fresh = it == 0 || restart;
+      const double beta = fresh ? 0.0 : gam / gam_old;
+      // (p, K p) of p = u + beta p_old: after a refresh from its three terms
+      // ((u, s_old) is no longer -gam / alpha_old), else the single-reduction form
+      const double den = fresh ? dlt
+                         : (refreshed ? dlt + 2.0 * beta * v3s[1] + beta * beta * den_old : dlt - beta * gam / alf_old);
       const double alpha = gam / den;
-      if (!(den > 0.0) || !(alpha == alpha)) { status = 2; break; }
+      if (!(den > 0.0) || !(alpha == alpha)) {
+        // mixed: FP32 rounding can break the short recurrence near
+        // stagnation; restart along u from an FP64 residual, once
+        if (mixed && !fresh) {
+          restart = true;
+          exact = false;
+          continue;
+        }
+        status = 2;
+        break;
+      }
       double w3[3] = {0, 0, 0};
       for (long i = e0; i < n; i += estep) {
         const double pi = S.u[i] + beta * S.p[i];
@@ -1969,9 +2058,13 @@ __global__ __launch_bounds__(WGS) void k_batch_pcg(const ugk_bsample *__restrict
       if (!team_sum<WGS, 3>(w3, red3, T, rank, tp, ctr, nbar, par, bs)) { status = 4; break; }   // publishes u
       gam_old = gam;
       alf_old = alpha;
+      den_old = den;
       gam = w3[0];
       rr = w3[1];
       it++;
+      exact = false;
+      refreshed = false;
+      restart = false;
     }
     if (threadIdx.x == 0 && rank == 0) {
       ugk_bresult R;

@@ -141,6 +141,7 @@ def lib():
             "ufem_batch_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_long, P_D, P_L, P_D, P_I, P_D]),
             "ufem_batch_get_u": (C.c_int, [C.c_void_p, C.c_int, P_D]),
             "ufem_batch_set_limits": (None, [C.c_void_p, C.c_double, C.c_double]),
+            "ufem_batch_set_mixed": (C.c_int, [C.c_void_p, C.c_int]),
             "ufem_batch_abort": (None, [C.c_void_p]),
             "femEqsCGwJ": (C.c_int, [C.POINTER(tMatrix), C.POINTER(tVector), C.POINTER(tVector),
                                      C.c_double, C.c_long]),
@@ -651,6 +652,14 @@ class GpuBatch:
 
     def set_shape(self, wg, grid, team_big):
         lib().ufem_batch_set_shape(self.h, wg, grid, team_big)
+
+    def set_mixed(self, mode=1):
+        """FP64 vectors over FP32 copies of the samples' K (mode 1, True), with
+        bf16 remainders beside the FP32 values (mode 2, 48-bit values) and FP64
+        residual refreshes; 0 or False: K in FP64"""
+        rv = lib().ufem_batch_set_mixed(self.h, int(mode))
+        if rv != 0:
+            raise ValueError(last_error())
 
     def u(self, i):
         out = np.zeros(lib().ufem_batch_ndof(self.h, i))
