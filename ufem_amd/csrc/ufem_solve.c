@@ -643,28 +643,32 @@ static int solve_twolevel(ugk_mat *Km, long n, const double *b, double *x, ufem_
  * the FP32 products accumulate, so r is recomputed in FP64 whenever the
  * recursive residual has fallen by inner_eps (default 0.1) since the last
  * recomputation, or after inner_maxit (default 500) iterations, and a
- * convergence is accepted on a recomputed residual only. */
+ * convergence is accepted on a recomputed residual only.  The refresh is
+ * launched once per chunk of iterations (a request waits at most a chunk):
+ * a launch that finds nothing to do still costs about 5 us. */
 typedef struct {
   const ugk_mat *K;
   long n;
   const double *b;
   double *x;
   ufem_work *w;
+  int chunk;    /* iterations per chunk; the refresh opens each */
 } mx64_ctx;
 
 static int mx64_step(void *c, long it, void *stream)
 {
   mx64_ctx *m = (mx64_ctx *)c;
   ufem_work *w = m->w;
-  (void)it;
-  return ugk_mx64_iter(m->K, m->n, m->b, m->x, w->r, w->p[0], w->q, w->dinv, w->st, w->part, stream);
+  return ugk_mx64_iter(m->K, m->n, m->b, m->x, w->r, w->p[0], w->q, w->dinv, w->st, w->part, it % m->chunk == 0,
+                       stream);
 }
 
 static int solve_mixed64(ugk_mat *Km, long n, const double *b, double *x, ufem_work *w, void *stream,
                          const ufem_solve_opts *o, long maxit, ufem_solve_info *info)
 {
   const ugk_mat *K = Km;
-  mx64_ctx c = {K, n, b, x, w};
+  const int chunk = o->chunk > 0 ? o->chunk : 16;
+  mx64_ctx c = {K, n, b, x, w, chunk < 2 ? 2 : chunk + (chunk & 1)};   /* as run_chunks rounds it */
   ugk_state *h = w->st_host;
   int rv;
   if ((rv = ufem_mat_f32(Km, o->f32_rem, stream)) != AF_OK) return rv;
@@ -676,7 +680,7 @@ static int solve_mixed64(ugk_mat *Km, long n, const double *b, double *x, ufem_w
   h->ru_at = 0;
   if (ugk_h2d(w->st, h, sizeof(*h), stream)) return gpu_fail("state");
   if (ugk_fcg_start(K, n, b, x, w->r, w->dinv, w->part, stream)) return gpu_fail("start");
-  rv = run_chunks(w, stream, maxit, o->chunk > 0 ? o->chunk : 16, mx64_step, &c, 1);
+  rv = run_chunks(w, stream, maxit, chunk, mx64_step, &c, 1);
   if (rv != AF_OK) return rv;
   if (get_state(w, stream)) return AF_ERR;
   if (h->bb <= 0.0) {

@@ -32,8 +32,10 @@ typedef struct {
   int *d_rowlen;      /* [n_brows] real blocks per row                  */
   int *d_cols;        /* [total_slots * 64]                             */
   double *d_vals;     /* [total_slots * 64 * b*b]                       */
-  float *d_vals32;    /* FP32 copy for mixed precision (may be NULL)    */
-  unsigned short *d_rem16;  /* bf16 of vals - vals32 at the same positions:
+  float *d_vals32;    /* FP32 copy for mixed precision (may be NULL); b = 3:
+                         each lane's first eight values of a slot in two
+                         runs of four (256 k + 4 l), the ninth at 512 + l */
+  unsigned short *d_rem16;  /* bf16 of vals - vals32 at vals32's positions:
                          with it the copy holds 48-bit values (products
                          with FP64 vectors only; may be NULL)           */
   float *d_corr32;    /* b = 3: [9][n_brows] symmetric part of the block row
@@ -468,13 +470,14 @@ int ugk_cc_iter_mixed(const ugk_mat *K, const ugk_coarse *C, long n, const doubl
                       double *d_z, double *d_p, double *d_q, const double *d_dinv, ugk_state *d_st,
                       double *d_part, void *stream);
 /* Jacobi PCG on the FP32 copy of K with FP64 vectors, recurrences and dots
- * (the FP32 values are widened on load): refresh (r = b - K x in FP64 when
- * the head asked for it), head (test; a convergence is only accepted on a
- * refreshed residual), p = dinv r + beta p, q = K32 p with (p, q), x and r
+ * (the FP32 values are widened on load): [refresh != 0: r = b - K x in FP64
+ * if a head asked for it since the last refresh -- launch it once per chunk
+ * of iterations], head and p = dinv r + beta p in one kernel (a convergence
+ * is only accepted on a refreshed residual), q = K32 p with (p, q), x and r
  * update with (r, dinv r).  r and the partials from ugk_fcg_start; the
  * state's mixed fields set as for flexible CG */
 int ugk_mx64_iter(const ugk_mat *K, long n, const double *d_b, double *d_x, double *d_r, double *d_p, double *d_q,
-                  const double *d_dinv, ugk_state *d_st, double *d_part, void *stream);
+                  const double *d_dinv, ugk_state *d_st, double *d_part, int refresh, void *stream);
 /* z_c alone into d_zc (skipped once the solve is done); d_out NULL in
  * ugk_fcg_combine drops the network term, d_zc NULL the coarse one, and
  * ugk_fcg_head with d_x0 NULL writes no network input */

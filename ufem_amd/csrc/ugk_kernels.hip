@@ -348,12 +348,38 @@ template <> struct Pair<double> { typedef double type __attribute__((ext_vector_
 template <> struct Pair<float> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct Pair<unsigned short> { typedef unsigned short type __attribute__((ext_vector_type(2))); };
 
+template <typename T> struct Quad;
+template <> struct Quad<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Quad<unsigned short> { typedef unsigned short type __attribute__((ext_vector_type(4))); };
+
+// Position of value q (row-major in the 3x3 block) of lane l in slot S of the
+// FP32 copy and of its bf16 remainders: the first eight in two runs of four
+// (run k of lane l at 256 k + 4 l: one 16-byte load in FP32), the ninth in
+// the 64-wide tail at 512 + l.  (The FP64 values keep pairs, ugk_vpos3.)
+__device__ __forceinline__ long vpos32(long S, int q, int l)
+{
+  return S * 576 + (q < 8 ? (q >> 2) * 256 + 4 * l + (q & 3) : 512 + l);
+}
+
 // the BS*BS values of lane `lane` in slot S, row-major (NT: non-temporal loads)
 template <int BS, bool NT, typename T>
 __device__ __forceinline__ void ld_block(const T *__restrict__ vals, long S, int lane, T (&b)[BS * BS])
 {
   if constexpr (BS == 1) {
     b[0] = NT ? ldnt(vals + S * 64 + lane) : vals[S * 64 + lane];
+  } else if constexpr (sizeof(T) <= 4) {   // the FP32 copy's layout (vpos32)
+    typedef typename Quad<T>::type T4;
+    const T *p = vals + S * 576;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const T4 *a = (const T4 *)(p + k * 256 + 4 * lane);
+      const T4 t = NT ? ldnt(a) : *a;
+      b[4 * k] = t.x;
+      b[4 * k + 1] = t.y;
+      b[4 * k + 2] = t.z;
+      b[4 * k + 3] = t.w;
+    }
+    b[8] = NT ? ldnt(p + 512 + lane) : p[512 + lane];
   } else {
     typedef typename Pair<T>::type T2;
     const T *p = vals + S * 576;
@@ -837,15 +863,33 @@ __global__ void k_vals_f32(long n, const double *__restrict__ a, T *__restrict__
     b[i] = (T)a[i];
 }
 
+// FP64 value index (pairs layout of a 3x3-block slot) -> its FP32 copy index (vpos32)
+__device__ __forceinline__ long vidx64_to32(long i)
+{
+  const long S = i / 576;
+  const int off = (int)(i - S * 576);
+  if (off >= 512) return i;
+  const int k = off >> 7, l = (off & 127) >> 1, q = 2 * k + (off & 1);
+  return vpos32(S, q, l);
+}
+
+// FP32 copy of 3x3-block values, in the copy's own layout
+__global__ void k_vals_f32_b3(long n, const double *__restrict__ a, float *__restrict__ b)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[vidx64_to32(i)] = (float)a[i];
+}
+
 // bf16 (round to nearest even) of v - fl32(v): with it the FP32 copy holds
-// 48-bit values (about 2e-10 relative)
+// 48-bit values (about 2e-10 relative); 3x3 blocks (B3) in the copy's layout
+template <bool B3>
 __global__ void k_vals_rem16(long n, const double *__restrict__ a, unsigned short *__restrict__ b)
 {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const float r = (float)(a[i] - (double)(float)a[i]);
     unsigned u = __float_as_uint(r);
     u += 0x7fffu + ((u >> 16) & 1u);
-    b[i] = (unsigned short)(u >> 16);
+    b[B3 ? vidx64_to32(i) : i] = (unsigned short)(u >> 16);
   }
 }
 
@@ -869,9 +913,8 @@ __global__ void k_corr32(SellView M, const double *__restrict__ vals, const unsi
   for (int s = 0; s < L; s++) {
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-      const long at = vpos<3>(so + s, q, ln);
-      const double v = vals[at], hi = (double)(float)v;
-      c[q] += v - (rem ? hi + (double)__uint_as_float((unsigned)rem[at] << 16) : hi);
+      const double v = vals[vpos<3>(so + s, q, ln)], hi = (double)(float)v;
+      c[q] += v - (rem ? hi + (double)__uint_as_float((unsigned)rem[vpos32(so + s, q, ln)] << 16) : hi);
     }
   }
 #pragma unroll
@@ -2316,14 +2359,16 @@ __global__ __launch_bounds__(WG) void k_fcg_q(SellView M, const T *__restrict__ 
 
 // mixed form: when the head kernel of the previous iteration asked for it,
 // r = b - A x in FP64 and the head's partials of this iteration taken from it
-template <int BS>
+template <int BS, bool LAZY = false>
 __global__ __launch_bounds__(WG) void k_fcg_refresh(SellView M, const double *__restrict__ vals,
                                                     const double *__restrict__ b,
                                                     const double *__restrict__ x, double *r,
                                                     const double *__restrict__ dinv, ugk_state *st,
                                                     double *part)
 {
-  if (st->done || st->ru_req != st->iter) return;   // the same for every block
+  // the same for every block; LAZY: launched once per chunk, it serves any
+  // request made since the last refresh (the head clears it once served)
+  if (st->done || (LAZY ? (st->ru_req < 0 || st->ru_req > st->iter) : st->ru_req != st->iter)) return;
   const long k = st->iter;
   double v[4] = {0, 0, 0, 0};
   double acc[BS];
@@ -3085,6 +3130,54 @@ __global__ __launch_bounds__(WG) void k_jac_p(long n, const double *__restrict__
     st->rzB = ro;
     st->beta = beta;
   }
+  long i1, i2;
+  xcd_chunk(n, i1, i2);
+  for (long i = i1 + threadIdx.x; i < i2; i += WG) p[i] = k == 0 ? dinv[i] * r[i] : dinv[i] * r[i] + beta * p[i];
+}
+
+// Head and Jacobi direction of the FP64-vector mixed PCG in one launch:
+// every block sums the partials of (r, dinv r), (r, r), (x, x), (b, b) (the
+// refresh's, G0 blocks, when it ran for this iteration), tests convergence
+// (accepted on a refreshed residual only), and block 0 records the refresh
+// request: the recursive ||r|| has fallen by delta since the last FP64
+// residual, max_gap iterations, or a convergence to confirm.  Then
+// p = dinv r + beta p.  Fields other blocks read are not written here
+// (rz by parity), so every block takes the same decisions.
+__global__ __launch_bounds__(WG) void k_mx64_p(long n, const double *__restrict__ r,
+                                               const double *__restrict__ dinv, double *p, ugk_state *st,
+                                               const double *part, unsigned G0, unsigned GB)
+{
+  if (st->done) return;
+  const long k = st->iter;
+  const bool fresh = k == 0 || st->ru_at == k;
+  double t[4];
+  reduce_prev<4>(PART_B2(part, k & 1), fresh ? G0 : GB, t);
+  const double bb = (k == 0) ? t[3] : st->bb;
+  const double ro = t[0];
+  int done = pcg_test(st, k, t[1], t[2], bb, ro);
+  const bool confirm = done == 1 && !fresh;
+  if (confirm) done = 0;
+  if (!done && !isfinite(ro)) done = 2;
+  const double beta = k == 0 ? 0.0 : ro / st->rz[(k + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->cur = k;
+    st->rr = t[1];
+    st->xx = t[2];
+    if (k == 0) st->bb = bb;
+    if (fresh) {
+      st->rr_true = t[1];
+      st->ru_req = -1;   // served
+    } else if (confirm || (done == 0 && (t[1] < st->delta * st->delta * st->rr_true ||
+                                         k - st->ru_at >= st->max_gap)))
+      st->ru_req = k + 1;
+    if (done) st->done = done;
+    else {
+      st->rz[k & 1] = ro;
+      st->rzB = ro;
+      st->beta = beta;
+    }
+  }
+  if (done) return;
   long i1, i2;
   xcd_chunk(n, i1, i2);
   for (long i = i1 + threadIdx.x; i < i2; i += WG) p[i] = k == 0 ? dinv[i] * r[i] : dinv[i] * r[i] + beta * p[i];
@@ -4118,10 +4211,12 @@ int ugk_vals_to_f32(const ugk_mat *K, void *s)
 {
   long n = K->total_slots * 64L * K->bs * K->bs;
   if (!K->d_vals32) { snprintf(g_err, sizeof(g_err), "no fp32 buffer"); return -1; }
-  k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_vals32);
+  if (K->bs == 3) k_vals_f32_b3<<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_vals32);
+  else k_vals_f32<float><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_vals32);
   CK_LAUNCH("k_vals_f32");
   if (K->d_rem16) {
-    k_vals_rem16<<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_rem16);
+    if (K->bs == 3) k_vals_rem16<true><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_rem16);
+    else k_vals_rem16<false><<<vec_blocks(n), WG, 0, S(s)>>>(n, K->d_vals, K->d_rem16);
     CK_LAUNCH("k_vals_rem16");
   }
   if (K->bs == 3 && K->d_corr32) {
@@ -4879,7 +4974,7 @@ int ugk_cc_iter_mixed(const ugk_mat *K, const ugk_coarse *C, long n, const doubl
 }
 
 int ugk_mx64_iter(const ugk_mat *K, long n, const double *b, double *x, double *r, double *p, double *q,
-                  const double *dinv, ugk_state *st, double *part, void *s)
+                  const double *dinv, ugk_state *st, double *part, int refresh, void *s)
 {
   const unsigned GV = vec_blocks(n), GA = spmv_blocks(K);
   const SellView M = view(K);
@@ -4887,10 +4982,12 @@ int ugk_mx64_iter(const ugk_mat *K, long n, const double *b, double *x, double *
     snprintf(g_err, sizeof(g_err), "mixed PCG: no FP32 copy of the matrix");
     return -1;
   }
-  if (K->bs == 3) k_fcg_refresh<3><<<GA, WG, 0, S(s)>>>(M, K->d_vals, b, x, r, dinv, st, part);
-  else k_fcg_refresh<1><<<GA, WG, 0, S(s)>>>(M, K->d_vals, b, x, r, dinv, st, part);
-  k_fcg_head<<<GV, WG, 0, S(s)>>>(n, r, dinv, NULL, st, part, GA, GV);
-  k_jac_p<<<GV, WG, 0, S(s)>>>(n, r, dinv, p, st);
+  if (refresh) {
+    if (K->bs == 3) k_fcg_refresh<3, true><<<GA, WG, 0, S(s)>>>(M, K->d_vals, b, x, r, dinv, st, part);
+    else k_fcg_refresh<1, true><<<GA, WG, 0, S(s)>>>(M, K->d_vals, b, x, r, dinv, st, part);
+    CK_LAUNCH("k_fcg_refresh");
+  }
+  k_mx64_p<<<GV, WG, 0, S(s)>>>(n, r, dinv, p, st, part, GA, GV);
   if (K->bs == 3) k_pcg_q<3, float><<<GA, WG, 0, S(s)>>>(M, K->d_vals32, p, q, st, part);
   else k_pcg_q<1, float><<<GA, WG, 0, S(s)>>>(M, K->d_vals32, p, q, st, part);
   k_ssor_upd<<<GV, WG, 0, S(s)>>>(n, x, r, p, q, st, part, GA, dinv);
