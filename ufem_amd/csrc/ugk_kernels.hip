@@ -3179,8 +3179,15 @@ __global__ __launch_bounds__(WG) void k_mx64_p(long n, const double *__restrict_
   }
   if (done) return;
   long i1, i2;
-  xcd_chunk(n, i1, i2);
-  for (long i = i1 + threadIdx.x; i < i2; i += WG) p[i] = k == 0 ? dinv[i] * r[i] : dinv[i] * r[i] + beta * p[i];
+  xcd_chunk_even(n, i1, i2);   // two entries per lane, 16-byte accesses
+  for (long i = i1 + 2 * threadIdx.x; i < i2; i += 2 * WG) {
+    if (i + 1 < i2) {
+      dbl2 pn = *(const dbl2 *)(dinv + i) * *(const dbl2 *)(r + i);
+      if (k > 0) pn += beta * *(const dbl2 *)(p + i);
+      *(dbl2 *)(p + i) = pn;
+    } else
+      p[i] = k == 0 ? dinv[i] * r[i] : dinv[i] * r[i] + beta * p[i];
+  }
 }
 
 __global__ __launch_bounds__(WG) void k_ssor_upd(long n, double *__restrict__ x, double *__restrict__ r,
