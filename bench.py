@@ -787,6 +787,17 @@ def main():
                  "true_relres": mi["true_relres"],
                  "u_rel_err_vs_fp64": float(np.linalg.norm(um - u64) / np.linalg.norm(u64)),
                  "speedup_vs_fp64": round(solve_s / m_s, 3)}
+        # FP64 vectors over the corrected FP32 copy of K (and the 48-bit copy)
+        for name, rem in (("fp64_vectors", False), ("fp64_vectors_48bit", True)):
+            g.solve(solver="mixed64", eps=1e-8, rule=1, maxit=20000, f32_rem=rem)
+            t0 = time.perf_counter()
+            mi = g.solve(solver="mixed64", eps=1e-8, rule=1, maxit=20000, f32_rem=rem)
+            m_s = time.perf_counter() - t0
+            um = g.u()
+            mixed[name] = {"solve_s": round(m_s, 4), "iters": mi["iters"], "outer": mi["outer"], "rv": mi["rv"],
+                           "true_relres": mi["true_relres"],
+                           "u_rel_err_vs_fp64": float(np.linalg.norm(um - u64) / np.linalg.norm(u64)),
+                           "speedup_vs_fp64": round(solve_s / m_s, 3)}
         log("mixed", mixed)
 
     # CG with the SSOR preconditioner (femEqsCGwSSOR) on the same system; the
