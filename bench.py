@@ -595,6 +595,9 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the dominant solve kernel and a device copy of known size "
                          "(for hardware-counter runs)")
+    ap.add_argument("--mixed64-only", action="store_true",
+                    help="assemble the metric's system and run two solves with FP64 vectors over the FP32 copy "
+                         "of K (for profiling)")
     ap.add_argument("--twolevel-only", action="store_true",
                     help="assemble the metric's system and run two two-level solves (for profiling)")
     ap.add_argument("--plain-only", action="store_true",
@@ -646,7 +649,8 @@ def main():
     nx, ny, nz = (int(v) for v in args.size.split(","))
     # the CPU baseline runs after the GPU legs: its threads beside them would
     # delay the host side of the short solves (two-level: about 20 ms)
-    want_cpu = not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.plain_only or args.ssor_only
+    want_cpu = not args.no_cpu and not (args.kernels_only or args.twolevel_only or args.mixed64_only or args.plain_only
+                                        or args.ssor_only
                                         or args.learned_solve_only or args.learned_1m_only)
     t0 = time.perf_counter()
     mesh = cantilever(nx, ny, nz, E=lognormal_E(args.seed, nx * ny * nz, cov=0.3))
@@ -693,6 +697,14 @@ def main():
         out = bench_learned_mixed(g, mesh, args.learn_fields, args.learn_steps, levels=args.net_levels,
                                   lr=args.learn_lr, fine=args.net_node_grid, width=args.net_width)
         print(json.dumps({"learned_mixed_1m": out}), flush=True)
+        g.close()
+        return
+    if args.mixed64_only:
+        # one warm and one profiled FP64-vector mixed solve of the metric's system
+        for _ in range(2):
+            r = g.solve(solver="mixed64", eps=1e-8, rule=1, maxit=20000)
+            log(f"FP64-vector mixed PCG: {r['iters']} iterations, {r['outer']} FP64 refreshes, "
+                f"{r['solve_ms']:.2f} ms device")
         g.close()
         return
     if args.twolevel_only:
