@@ -314,6 +314,10 @@ static int mixed_table(ufem_batch *b)
 
 int ufem_batch_set_mixed(ufem_batch *b, int mode)
 {
+  if (!b) {
+    ufem_set_error("batch mixed mode: no batch");
+    return AF_ERR_VAL;
+  }
   if (mode < 0 || mode > 2) {
     ufem_set_error("batch mixed mode %d: 0 (FP64), 1 (FP32 copy) or 2 (FP32 + bf16 remainder)", mode);
     return AF_ERR_VAL;
